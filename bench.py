@@ -1,0 +1,278 @@
+#!/usr/bin/env python
+"""Particle-updates/s of pmwd's KDK N-body step on one MI355X.
+
+Workload (default): 512^3 particles on a 512^3 mesh (mesh_shape=1), box
+1000 Mpc/h, FP32 state, 2LPT initial conditions at a=1/32 from seeded white
+noise (numpy generator, seed 0), sigma8=0.8, Omega_m=0.3, Omega_b=0.05,
+h=0.7, n_s=0.96, then KDK steps with a_nbody_maxstep=1/32 (31 steps to a=1).
+A "step" is one KDK step (kick, drift, CIC scatter, FFT Poisson solve,
+3-component CIC gather, kick) over all particles; ``value`` = particles x
+steps / wall time of the timed steps, inputs resident in HBM.  After the
+forward run the reverse-time adjoint runs the same number of steps from the
+final state and ``fwd_adj_value`` = particles x steps / (forward + adjoint).
+
+Prints one JSON line.  ``--gpus`` above 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__.split('\n')[0])
+    p.add_argument('--gpus', type=int, default=1)
+    p.add_argument('--steps', type=int, default=29)
+    p.add_argument('--warmup', type=int, default=2)
+    p.add_argument('--grid', type=int, default=512, help='particles per dimension')
+    p.add_argument('--mesh-ratio', type=int, default=1)
+    p.add_argument('--box', type=float, default=1000.0)
+    p.add_argument('--no-adjoint', action='store_true')
+    p.add_argument('--no-cpu-baseline', action='store_true')
+    p.add_argument('--cpu-slab', type=int, default=64,
+                   help='fraction 1/slab of the particles in the CPU baseline sample')
+    return p.parse_args()
+
+
+def group_bytes(N, M, Mh):
+    """Algorithmic bytes per step of each kernel group."""
+    return {
+        # fused kick+drift (read vel, acc, disp; write vel, disp) + CIC scatter
+        # (read pmid, write the density mesh)
+        'kick_drift_scatter': 60 * N + 6 * N + 4 * M,
+        # r2c (4M + 8Mh) + Poisson: Green, -ik, 3 c2r (8Mh + 12M)
+        'fft_poisson': 4 * M + 8 * Mh + 8 * Mh + 12 * M,
+        # 3-component gather (18N + 12M + 12N) fused with the closing kick
+        # (read and write vel, 24N)
+        'gather_kick': 18 * N + 12 * M + 12 * N + 24 * N,
+    }
+
+
+GROUPS = {
+    'kick_drift_scatter': ('bin', 'scan', 'fill', 'deposit', 'fold'),
+    'fft_poisson': ('z_r2c', 'y_fwd', 'poisson_x', 'y_inv', 'z_c2r'),
+    'gather_kick': ('gather',),
+}
+
+
+def cpu_baseline(conf, cosmo_om, slab):
+    """Time the numpy oracle on a bounded sample of one KDK step.
+
+    Sample: the full-mesh FFT Poisson solve plus kick/drift/CIC scatter/
+    3-component gather for 1/slab of the particles; the particle part is
+    scaled by slab.  Returns (upd/s, details).
+    """
+    import numpy as np
+    import scipy.fft as sfft
+    from oracle import pm as opm
+    cores = min(16, os.cpu_count() or 1)
+    shape = conf.mesh_shape
+    n_all = conf.ptcl_num
+    ns = n_all // slab
+    sub = (conf.ptcl_grid_shape[0] // slab,) + tuple(conf.ptcl_grid_shape[1:])
+    pmid, disp = opm.gen_grid(conf.ptcl_grid_shape, shape, conf.cell_size)
+    pmid, disp = pmid[:ns], disp[:ns]
+    rng = np.random.default_rng(0)
+    disp = (disp + 2.0 * rng.standard_normal(disp.shape)).astype(np.float32)
+    vel = rng.standard_normal(disp.shape).astype(np.float32)
+    acc = rng.standard_normal(disp.shape).astype(np.float32)
+    t0 = time.perf_counter()
+    vel = vel + acc * np.float32(0.01)
+    disp = disp + vel * np.float32(0.01)
+    dens = opm.scatter(pmid, disp, conf.cell_size, shape, ptcl_num=n_all)
+    t1 = time.perf_counter()
+    dens_full = (1 + 0.1 * rng.standard_normal(shape)).astype(np.float32)
+    t2 = time.perf_counter()
+    with sfft.set_workers(cores):
+        g = opm.gravity_mesh(dens_full, cosmo_om, conf.cell_size, shape)
+    t3 = time.perf_counter()
+    acc = np.stack([opm.gather(pmid, disp, conf.cell_size, shape, gi) for gi in g], axis=-1)
+    vel = vel + acc * np.float32(0.01)
+    t4 = time.perf_counter()
+    t_ptcl = (t1 - t0) + (t4 - t3)
+    t_mesh = t3 - t2
+    t_step = t_ptcl * slab + t_mesh
+    return n_all / t_step, dict(cores=cores, t_step_s=t_step, t_sample_s=t_ptcl + t_mesh,
+                                sample=f'one KDK step: full {shape[0]}^3-mesh FFT Poisson solve '
+                                       f'(scipy.fft, {cores} workers) + kick/drift/scatter/gather '
+                                       f'of 1/{slab} of the particles ({ns}, numpy, 1 thread), '
+                                       f'particle part scaled x{slab}')
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get('WORLD_SIZE', '1'))
+    if args.gpus != 1 or world != 1:
+        print(f'bench.py: this build runs on one GPU; --gpus {args.gpus} (WORLD_SIZE={world}) '
+              'is not supported. The multi-GPU decomposition is a design note only.',
+              file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes
+    import numpy as np
+    import torch
+    import pmwd_amd as pa
+    from pmwd_amd import _lib
+    nb = sys.modules['pmwd_amd.nbody']
+
+    dev = torch.device('cuda:0')
+    torch.cuda.set_device(dev)
+    G = args.grid
+    spacing = args.box / G
+    conf = pa.Configuration(spacing, (G, G, G), mesh_shape=args.mesh_ratio, a_start=1 / 32,
+                            a_nbody_maxstep=1 / 32)
+    cosmo = pa.Cosmology.from_sigma8(conf, 0.8, n_s=0.96, Omega_m=0.3, Omega_b=0.05, h=0.7)
+    cosmo = pa.boltzmann(cosmo, conf)
+    N, M = conf.ptcl_num, conf.mesh_size
+    Mh = conf.mesh_shape[0] * conf.mesh_shape[1] * (conf.mesh_shape[2] // 2 + 1)
+
+    t = time.perf_counter()
+    modes = pa.white_noise(0, conf, real=True, device=dev)
+    modes = pa.linear_modes(modes, cosmo, conf)
+    ptcl, _ = pa.lpt(modes, cosmo, conf)
+    del modes
+    torch.cuda.synchronize()
+    t_ic = time.perf_counter() - t
+
+    pmid = ptcl.pmid.contiguous()
+    disp = ptcl.disp.clone().contiguous()
+    vel = ptcl.vel.clone().contiguous()
+    acc = torch.empty_like(disp)
+    L = _lib.lib()
+    ctx = _lib.context(conf.mesh_shape, conf.cell_size, dev)
+    h = ctx.bind()
+    val, coef = nb._mass_val(conf), nb.omega_coef(cosmo, conf)
+
+    a = conf.a_nbody
+    nsteps_sched = len(a) - 1
+    sched = []
+    for i in range(nsteps_sched):
+        a0, a1 = a[i], a[i + 1]
+        ah = a0 * (1 - 0.5) + a1 * 0.5
+        sched.append((nb._f32(nb.kick_factor(a0, a0, ah, cosmo, conf)),
+                      nb._f32(nb.drift_factor(ah, a0, a1, cosmo, conf)),
+                      nb._f32(nb.kick_factor(a1, ah, a1, cosmo, conf))))
+
+    def step(i):
+        k1, d, k2 = sched[i % nsteps_sched]
+        _lib.check(L.pmx_nbody_step(h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel),
+                                    _lib.ptr(acc), N, val, coef, k1, d, k2))
+
+    _lib.check(L.pmx_gravity(h, _lib.ptr(pmid), _lib.ptr(disp), N, val, coef, _lib.ptr(acc)))
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+
+    _lib.check(L.pmx_timing_enable(h, 1))
+    _lib.timing_read(ctx)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.warmup, args.warmup + args.steps):
+        step(i)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    timing = _lib.timing_read(ctx)
+    _lib.check(L.pmx_timing_enable(h, 0))
+
+    K = args.steps
+    ms_per_step = 1e3 * t_fwd / K
+    value = N * K / t_fwd
+
+    # kernel groups: time per step and algorithmic bytes per step
+    gb = group_bytes(N, M, Mh)
+    groups = {}
+    for gname, classes in GROUPS.items():
+        ms = sum(timing[c][0] for c in classes) / K
+        groups[gname] = dict(ms=ms, bytes=gb[gname],
+                             gbs=gb[gname] / (ms * 1e-3) / 1e9 if ms > 0 else None)
+        groups[gname]['frac'] = groups[gname]['gbs'] / HBM_PEAK_GBS if ms > 0 else None
+    kernels = {k: dict(ms_per_step=v[0] / K, launches_per_step=v[1] / K) for k, v in timing.items()
+               if v[1]}
+    dom = max(groups, key=lambda g: groups[g]['ms'])
+    B_step = 78 * N + 8 * M + 16 * Mh + 24 * M
+
+    traffic = None
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                         f'traffic_{dom}_{G}.json')
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get('bytes_per_step')
+
+    out = {
+        'metric': 'particle-updates/s fwd & fwd+adj at 512^3 (1 GPU); p50/p99 mixed-size latency',
+        'value': value,
+        'unit': 'particle-updates/s',
+        'n_gpus': 1,
+        'steps': K,
+        'warmup': args.warmup,
+        'ms_per_step': ms_per_step,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic: 2LPT ICs from seeded white noise (numpy seed 0)',
+        'config': {'workload': f'pmwd KDK N-body step, {G}^3 particles on a '
+                               f'{conf.mesh_shape[0]}^3 mesh, box {args.box:g} Mpc/h, '
+                               f'a_start=1/32, 31-step schedule, sigma8=0.8, Omega_m=0.3',
+                   'ptcl_grid': G, 'mesh': conf.mesh_shape[0], 'box_mpc_h': args.box,
+                   'parallelism': 'single GPU'},
+        'roofline': {'bound': 'hbm', 'kernel': dom,
+                     'achieved': groups[dom]['gbs'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': groups[dom]['frac'], 'traffic': traffic,
+                     'algorithmic_bytes_per_step': groups[dom]['bytes'],
+                     'ms_per_step': groups[dom]['ms']},
+        'whole_step_roofline': {'bytes_per_step': B_step,
+                                'frac': B_step / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        'kernel_groups': groups,
+        'kernels': kernels,
+        't_ic_s': t_ic,
+    }
+
+    if not args.no_adjoint:
+        dc = torch.randn_like(disp) * 1e-3
+        vc = torch.randn_like(vel) * 1e-3
+        acc_cot = torch.empty_like(disp)
+        dot = ctypes.c_double()
+        adj_sched = []
+        a_rev = list(zip(a[:0:-1], a[-2::-1]))
+        for a_prev, a_next in a_rev:
+            ah = a_prev * 0.5 + a_next * 0.5
+            adj_sched.append((nb._f32(nb.kick_factor(a_prev, a_prev, ah, cosmo, conf)),
+                              nb._f32(nb.drift_factor(ah, a_prev, a_next, cosmo, conf)),
+                              nb._f32(nb.kick_factor(a_next, ah, a_next, cosmo, conf))))
+        _lib.check(L.pmx_gravity_vjp(h, _lib.ptr(pmid), _lib.ptr(disp), N, val, coef, _lib.ptr(vc),
+                                     _lib.ptr(acc), _lib.ptr(acc_cot), ctypes.byref(dot)))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(K):
+            ka, d, kb = adj_sched[i % nsteps_sched]
+            _lib.check(L.pmx_kick_adj(h, N, _lib.ptr(vel), _lib.ptr(acc), _lib.ptr(dc),
+                                      _lib.ptr(acc_cot), _lib.ptr(vc), ka, ctypes.byref(dot)))
+            _lib.check(L.pmx_drift_adj(h, N, _lib.ptr(disp), _lib.ptr(vel), _lib.ptr(dc),
+                                       _lib.ptr(vc), d, ctypes.byref(dot)))
+            _lib.check(L.pmx_gravity_vjp(h, _lib.ptr(pmid), _lib.ptr(disp), N, val, coef,
+                                         _lib.ptr(vc), _lib.ptr(acc), _lib.ptr(acc_cot),
+                                         ctypes.byref(dot)))
+            _lib.check(L.pmx_kick_adj(h, N, _lib.ptr(vel), _lib.ptr(acc), _lib.ptr(dc),
+                                      _lib.ptr(acc_cot), _lib.ptr(vc), kb, ctypes.byref(dot)))
+        torch.cuda.synchronize()
+        t_adj = time.perf_counter() - t0
+        out['fwd_adj_value'] = N * K / (t_fwd + t_adj)
+        out['adj_ms_per_step'] = 1e3 * t_adj / K
+
+    if not args.no_cpu_baseline:
+        upd, info = cpu_baseline(conf, cosmo.Omega_m, args.cpu_slab)
+        out['cpu_baseline'] = {'value': upd, 'unit': 'particle-updates/s', 'cores': info['cores'],
+                               'kind': 'port', 'sample': info['sample'],
+                               't_step_s': info['t_step_s']}
+        out['speedup_vs_cpu'] = value / upd
+
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,181 @@
+/*
+ * pmx.h -- C ABI of the MI355X (gfx950) particle-mesh kernels behind pmwd_amd.
+ *
+ * Plain pointers and sizes only.  Every device pointer is a HIP device
+ * allocation owned by the caller (torch in pmwd_amd); the context owns only
+ * its scratch.  All work is enqueued on the context's stream; nothing blocks
+ * the host except the entry points documented as returning host values.
+ * Return value: 0 on success, a PMX_E* code otherwise; pmx_last_error()
+ * returns the message of the last failure on the calling thread.
+ *
+ * Layouts (pmwd's): pmid (n,3) int16 row-major, disp/vel/acc (n,3) float32
+ * row-major, meshes float32 C-order (n0,n1,n2) unless a "padded" layout is
+ * named, complex meshes interleaved float32 pairs (n0,n1,n2/2+1).
+ *
+ * Each declaration names the reference interface it replaces; paths are in
+ * eelregit/pmwd.
+ */
+#ifndef PMX_H
+#define PMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PMX_OK = 0,
+  PMX_EINVAL = 1,   /* bad argument (shape, size, null pointer)          */
+  PMX_ENOMEM = 2,   /* device allocation failed                         */
+  PMX_EHIP = 3,     /* HIP runtime or launch error                      */
+  PMX_EUNSUP = 4    /* valid request outside what the kernels implement  */
+};
+
+typedef struct pmx_ctx pmx_ctx;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int pmx_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* pmx_last_error(void);
+
+/* ------------------------------------------------------------ context ---
+ * One context per (mesh shape, cell size, device); it holds FFT twiddles,
+ * k tables, three complex mesh buffers and the scatter binning scratch.
+ * Replaces the static `Configuration` jit aux data (pmwd/configuration.py:21)
+ * and the XLA buffers of gravity (pmwd/gravity.py:47-72).                   */
+int pmx_ctx_create(int n0, int n1, int n2, double cell_size, void* stream,
+                   pmx_ctx** out);
+int pmx_ctx_destroy(pmx_ctx* ctx);
+int pmx_ctx_set_stream(pmx_ctx* ctx, void* stream);
+/* Device bytes held by the context (host value). */
+long long pmx_ctx_bytes(const pmx_ctx* ctx);
+
+/* ------------------------------------------------------- cell keys/sort ---
+ * key[i] = C-order raveled index (strides of Particles.raveled_id,
+ * pmwd/particles.py:156-182) of particle i's lower CIC corner, computed
+ * bit-exactly as enmesh (pmwd/pm_util.py:119-137).                          */
+int pmx_cell_keys(pmx_ctx* ctx, const int16_t* pmid, const float* disp,
+                  long long n, uint32_t* key);
+/* perm = stable ascending argsort of key (n entries, keys < 2^31).  No pmwd
+ * counterpart: the permutation that orders particles by cell.             */
+int pmx_sort_perm(pmx_ctx* ctx, const uint32_t* key, long long n,
+                  uint32_t* perm);
+/* tile[i] = raveled index of the scatter tile holding the lower corner,
+ * and the tile extent used by pmx_scatter_mass (host values).              */
+int pmx_tile_keys(pmx_ctx* ctx, const int16_t* pmid, const float* disp,
+                  long long n, uint32_t* tile);
+int pmx_tile_shape(const pmx_ctx* ctx, int tile_shape[3]);
+
+/* ----------------------------------------------------------- scatter ---
+ * Default-argument CIC mass assignment (pmwd/scatter.py:33-83 with
+ * mesh=None, offset=0, cell_size=None): mesh[c] (+)= sum val*w.  Particles
+ * are binned by mesh tile and accumulated in LDS; no float atomics on
+ * global memory.  mesh is (n0,n1,n2) C-order; accumulate=0 overwrites it.  */
+int pmx_scatter_mass(pmx_ctx* ctx, const int16_t* pmid, const float* disp,
+                     long long n, float val, float* mesh, int accumulate);
+
+/* Generic CIC geometry for non-default arguments: enmesh's a2 branch
+ * (pmwd/pm_util.py:99-118) when out_cell_size > 0, offsets b12, a target
+ * shape (indices outside are dropped, pmwd/gather.py:75), channels.        */
+typedef struct {
+  int mesh_shape[3];      /* periodic shape s1 = conf.mesh_shape           */
+  double cell_size;       /* a1 = conf.cell_size                           */
+  double offset[3];       /* b12                                           */
+  double out_cell_size;   /* a2, or <= 0 for None                          */
+  int out_shape[3];       /* s2, target mesh shape                         */
+} pmx_cic_geom;
+
+/* pmwd/scatter.py:33-83 (any arguments): mesh (out_shape + (nchan,)) +=
+ * val * w.  val is (n,nchan) or NULL for the scalar val_scalar.  Uses
+ * global float atomics (not the hot path).                                  */
+int pmx_scatter_cic(pmx_ctx* ctx, const pmx_cic_geom* g, const int16_t* pmid,
+                    const float* disp, long long n, const float* val,
+                    float val_scalar, int nchan, float* mesh);
+/* pmwd/gather.py:33-77: out (n,nchan) = val + sum mesh[c]*w.  val is
+ * (n,nchan) or NULL for val_scalar.                                        */
+int pmx_gather_cic(pmx_ctx* ctx, const pmx_cic_geom* g, const int16_t* pmid,
+                   const float* disp, long long n, const float* mesh,
+                   int nchan, const float* val, float val_scalar, float* out);
+/* pmwd/scatter.py:86-148 (_scatter_chunk_adj/_scatter_bwd):
+ * disp_cot (n,3) = sum_j (mesh_cot[c_j].val) grad w_j / cell,
+ * val_cot (n,nchan) = sum_j mesh_cot[c_j] w_j (either output may be NULL). */
+int pmx_scatter_cic_adj(pmx_ctx* ctx, const pmx_cic_geom* g,
+                        const int16_t* pmid, const float* disp, long long n,
+                        const float* mesh_cot, const float* val,
+                        float val_scalar, int nchan, float* disp_cot,
+                        float* val_cot);
+/* pmwd/gather.py:80-142 (_gather_chunk_adj/_gather_bwd):
+ * disp_cot (n,3) = sum_j (val_cot.mesh[c_j]) grad w_j / cell and
+ * mesh_cot += scatter(val_cot * w) (either output may be NULL).            */
+int pmx_gather_cic_adj(pmx_ctx* ctx, const pmx_cic_geom* g,
+                       const int16_t* pmid, const float* disp, long long n,
+                       const float* mesh, const float* val_cot, int nchan,
+                       float* disp_cot, float* mesh_cot);
+
+/* --------------------------------------------------------------- FFT ---
+ * Hand-written Stockham FFTs.  pmx_rfftn: jnp.fft.rfftn, backward norm
+ * (pmwd/pm_util.py:281); pmx_irfftn: jnp.fft.irfftn(s=mesh), scaled 1/M
+ * (pmwd/pm_util.py:336).  in/out may not alias.                             */
+int pmx_rfftn(pmx_ctx* ctx, const float* in, float* out_c);
+int pmx_irfftn(pmx_ctx* ctx, const float* in_c, float* out);
+
+/* ----------------------------------------------------------- gravity ---
+ * Force meshes of pmwd/gravity.py:51-67: g_d = irfftn(neg_grad(k_d,
+ * laplace(k, rfftn((dens - 1) * coef)))), coef = 1.5*Omega_m in float32.
+ * grad is 3 meshes (3, n0, n1, n2) C-order.                                */
+int pmx_poisson_grad(pmx_ctx* ctx, const float* dens, float coef, float* grad);
+/* pmwd/gravity.py:47-72 with default scatter/gather arguments:
+ * acc (n,3) for particles of mass val = mesh_size/ptcl_num.                */
+int pmx_gravity(pmx_ctx* ctx, const int16_t* pmid, const float* disp,
+                long long n, float val, float coef, float* acc);
+
+/* ------------------------------------------------------- time stepping ---
+ * pmwd/nbody.py:39-77 drift/kick in float32 with factors computed in
+ * float64 by the caller and rounded to float32.                             */
+int pmx_kick(pmx_ctx* ctx, long long n, float* vel, const float* acc, float k);
+int pmx_drift(pmx_ctx* ctx, long long n, float* disp, const float* vel, float d);
+/* One KDK step of pmwd/nbody.py:121-140 (default symp_splits): vel += acc*k1;
+ * disp += vel*d; acc = gravity(disp); vel += acc*k2.                       */
+int pmx_nbody_step(pmx_ctx* ctx, const int16_t* pmid, float* disp, float* vel,
+                   float* acc, long long n, float val, float coef, float k1,
+                   float d, float k2);
+
+/* ------------------------------------------------------------ adjoint ---
+ * force_adj of pmwd/nbody.py:108-118: acc = gravity(disp) and
+ * acc_cot = VJP_disp(gravity)[vel_cot]; *dot (host) = sum(vel_cot*acc) in
+ * float64, from which the Omega_m cotangent sum(vel_cot*acc)/Omega_m
+ * follows (acc is proportional to Omega_m).  dot may be NULL.              */
+int pmx_gravity_vjp(pmx_ctx* ctx, const int16_t* pmid, const float* disp,
+                    long long n, float val, float coef, const float* vel_cot,
+                    float* acc, float* acc_cot, double* dot);
+/* kick_adj of pmwd/nbody.py:80-99: vel += acc*k; disp_cot -= acc_cot*k;
+ * *dot (host) = sum(vel_cot*acc) in float64 (may be NULL).                 */
+int pmx_kick_adj(pmx_ctx* ctx, long long n, float* vel, const float* acc,
+                 float* disp_cot, const float* acc_cot, const float* vel_cot,
+                 float k, double* dot);
+/* drift_adj of pmwd/nbody.py:49-67: disp += vel*d; vel_cot -= disp_cot*d;
+ * *dot (host) = sum(disp_cot*vel) in float64 (may be NULL).                */
+int pmx_drift_adj(pmx_ctx* ctx, long long n, float* disp, const float* vel,
+                  const float* disp_cot, float* vel_cot, float d, double* dot);
+
+/* float64 dot product of two float32 arrays of length n (host value).      */
+int pmx_dot(pmx_ctx* ctx, const float* a, const float* b, long long n, double* out);
+
+/* ------------------------------------------------------------- timing ---
+ * Kernel timing with HIP events recorded on the context stream around each
+ * launch of the hot path, grouped in PMX_NTIMER classes (bin, scan, fill,
+ * deposit, fold, z-r2c, y-fwd, poisson-x, y-inv, z-c2r, gather, other).
+ * pmx_timing_read synchronises the stream and returns, per class, the summed
+ * milliseconds and the number of launches since the last read.  No pmwd
+ * counterpart: measurement support.                                         */
+#define PMX_NTIMER_CLASSES 12
+int pmx_timing_enable(pmx_ctx* ctx, int on);
+int pmx_timing_read(pmx_ctx* ctx, double* ms, long long* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PMX_H */

@@ -1,0 +1,178 @@
+"""ctypes binding of libpmx.so (the C ABI declared in include/pmx.h).
+
+torch is imported first so that the HIP runtime torch ships is the one the
+library binds to (same soname), and device pointers/streams are shared.
+There is no fallback: if the library is missing or cannot run, the call
+raises.
+"""
+import ctypes
+import os
+import threading
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libpmx.so')
+
+_lib = None
+_lock = threading.Lock()
+
+c_int, c_ll, c_f, c_d, c_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_float, ctypes.c_double, ctypes.c_void_p
+
+
+class PmxError(RuntimeError):
+    """Error returned by the pmx C ABI."""
+
+
+class CicGeom(ctypes.Structure):
+    """pmx_cic_geom (include/pmx.h)."""
+    _fields_ = [('mesh_shape', c_int * 3), ('cell_size', c_d), ('offset', c_d * 3),
+                ('out_cell_size', c_d), ('out_shape', c_int * 3)]
+
+
+# name -> argtypes (all return int status unless listed in _RESTYPE)
+_SIGS = {
+    'pmx_version': [],
+    'pmx_last_error': [],
+    'pmx_ctx_create': [c_int, c_int, c_int, c_d, c_p, ctypes.POINTER(c_p)],
+    'pmx_ctx_destroy': [c_p],
+    'pmx_ctx_set_stream': [c_p, c_p],
+    'pmx_ctx_bytes': [c_p],
+    'pmx_cell_keys': [c_p, c_p, c_p, c_ll, c_p],
+    'pmx_sort_perm': [c_p, c_p, c_ll, c_p],
+    'pmx_tile_keys': [c_p, c_p, c_p, c_ll, c_p],
+    'pmx_tile_shape': [c_p, ctypes.POINTER(c_int * 3)],
+    'pmx_scatter_mass': [c_p, c_p, c_p, c_ll, c_f, c_p, c_int],
+    'pmx_scatter_cic': [c_p, ctypes.POINTER(CicGeom), c_p, c_p, c_ll, c_p, c_f, c_int, c_p],
+    'pmx_gather_cic': [c_p, ctypes.POINTER(CicGeom), c_p, c_p, c_ll, c_p, c_int, c_p, c_f, c_p],
+    'pmx_scatter_cic_adj': [c_p, ctypes.POINTER(CicGeom), c_p, c_p, c_ll, c_p, c_p, c_f, c_int,
+                            c_p, c_p],
+    'pmx_gather_cic_adj': [c_p, ctypes.POINTER(CicGeom), c_p, c_p, c_ll, c_p, c_p, c_int, c_p, c_p],
+    'pmx_rfftn': [c_p, c_p, c_p],
+    'pmx_irfftn': [c_p, c_p, c_p],
+    'pmx_poisson_grad': [c_p, c_p, c_f, c_p],
+    'pmx_gravity': [c_p, c_p, c_p, c_ll, c_f, c_f, c_p],
+    'pmx_kick': [c_p, c_ll, c_p, c_p, c_f],
+    'pmx_drift': [c_p, c_ll, c_p, c_p, c_f],
+    'pmx_nbody_step': [c_p, c_p, c_p, c_p, c_p, c_ll, c_f, c_f, c_f, c_f, c_f],
+    'pmx_gravity_vjp': [c_p, c_p, c_p, c_ll, c_f, c_f, c_p, c_p, c_p, ctypes.POINTER(c_d)],
+    'pmx_kick_adj': [c_p, c_ll, c_p, c_p, c_p, c_p, c_p, c_f, ctypes.POINTER(c_d)],
+    'pmx_drift_adj': [c_p, c_ll, c_p, c_p, c_p, c_p, c_f, ctypes.POINTER(c_d)],
+    'pmx_dot': [c_p, c_p, c_p, c_ll, ctypes.POINTER(c_d)],
+    'pmx_timing_enable': [c_p, c_int],
+    'pmx_timing_read': [c_p, ctypes.POINTER(c_d), ctypes.POINTER(c_ll)],
+}
+
+TIMER_CLASSES = ('bin', 'scan', 'fill', 'deposit', 'fold', 'z_r2c', 'y_fwd', 'poisson_x',
+                 'y_inv', 'z_c2r', 'gather', 'other')
+
+
+def timing_read(ctx):
+    """{class: (ms, launches)} since the last read (synchronises the stream)."""
+    n = len(TIMER_CLASSES)
+    ms = (c_d * n)()
+    cnt = (c_ll * n)()
+    check(lib().pmx_timing_read(ctx.h, ms, cnt))
+    return {k: (ms[i], cnt[i]) for i, k in enumerate(TIMER_CLASSES)}
+_RESTYPE = {'pmx_last_error': ctypes.c_char_p, 'pmx_ctx_bytes': c_ll}
+
+
+def lib():
+    """The loaded library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise PmxError(f'{LIB_PATH} not found: build it with '
+                               '`python -m pmwd_amd.csrc.build` (no CPU fallback exists)')
+            L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+            for name, args in _SIGS.items():
+                fn = getattr(L, name)
+                fn.argtypes = args
+                fn.restype = _RESTYPE.get(name, c_int)
+            _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().pmx_last_error().decode(errors='replace')
+        raise PmxError(f'pmx error {rc}: {msg}')
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def require_device(t, name='tensor'):
+    if not t.is_cuda:
+        raise PmxError(f'{name} must be on a GPU device (got {t.device}); '
+                       'the pmx kernels have no CPU path')
+    return t
+
+
+class Context:
+    """Owner of a pmx_ctx for one (mesh shape, cell size, device)."""
+
+    def __init__(self, mesh_shape, cell_size, device):
+        self.mesh_shape = tuple(int(s) for s in mesh_shape)
+        self.cell_size = float(cell_size)
+        self.device = torch.device(device)
+        h = c_p()
+        with torch.cuda.device(self.device):
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+            check(lib().pmx_ctx_create(*self.mesh_shape, self.cell_size, c_p(stream),
+                                       ctypes.byref(h)))
+        self.h = h
+
+    def bind(self):
+        """Point the context at the current torch stream; returns the handle."""
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(lib().pmx_ctx_set_stream(self.h, c_p(stream)))
+        return self.h
+
+    @property
+    def nbytes(self):
+        return int(lib().pmx_ctx_bytes(self.h))
+
+    def close(self):
+        if self.h:
+            lib().pmx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_ctx_cache = {}
+
+
+def context(mesh_shape, cell_size, device):
+    """Cached Context for the mesh of a configuration on a device."""
+    device = torch.device(device)
+    if device.index is None:
+        device = torch.device(device.type, torch.cuda.current_device())
+    key = (tuple(int(s) for s in mesh_shape), float(cell_size), device.index)
+    ctx = _ctx_cache.get(key)
+    if ctx is None:
+        ctx = Context(mesh_shape, cell_size, device)
+        _ctx_cache[key] = ctx
+    return ctx
+
+
+def clear_contexts():
+    for c in _ctx_cache.values():
+        c.close()
+    _ctx_cache.clear()

@@ -1,0 +1,725 @@
+// Cloud-in-cell scatter/gather, gravity and time stepping (gfx950).
+//
+// Hot scatter (pmx_scatter_mass): particles are binned by the mesh tile that
+// holds their lower CIC corner (count with wave-aggregated integer atomics,
+// exclusive scan, fill), then one workgroup per tile accumulates its
+// particles into an LDS copy of the tile plus a +1 halo shell and writes the
+// tile interior with plain stores and the shell to a halo buffer; a fold
+// pass adds each tile's incoming shells.  No float atomics touch global
+// memory on this path.
+#include "pmx_internal.h"
+
+#include <math.h>
+
+int pmx_poisson_ctx(pmx_ctx* c, float coef);
+int pmx_poisson_adj_ctx(pmx_ctx* c, float scale_out);
+int pmx_red_finish(pmx_ctx* c, int nblk, double* out);
+
+// ----------------------------------------------------------- helpers --
+__device__ __forceinline__ void load_ptcl(const int16_t* __restrict__ pmid,
+                                          const float* __restrict__ disp, long long i,
+                                          int pm[3], float dp[3]) {
+  pm[0] = pmid[3 * i + 0];
+  pm[1] = pmid[3 * i + 1];
+  pm[2] = pmid[3 * i + 2];
+  dp[0] = disp[3 * i + 0];
+  dp[1] = disp[3 * i + 1];
+  dp[2] = disp[3 * i + 2];
+}
+
+__device__ __forceinline__ void cic_setup(const int pm[3], const float dp[3], const MeshGeom& g,
+                                          CicAxis ax[3]) {
+  ax[0] = pmx_cic_axis(pm[0], dp[0], g.cell, g.n0);
+  ax[1] = pmx_cic_axis(pm[1], dp[1], g.cell, g.n1);
+  ax[2] = pmx_cic_axis(pm[2], dp[2], g.cell, g.n2);
+}
+
+__device__ __forceinline__ uint32_t tile_of(const CicAxis ax[3], const TileGeom& tg) {
+  return ((uint32_t)(ax[0].lo / tg.tx) * tg.nty + (uint32_t)(ax[1].lo / tg.ty)) * tg.ntz +
+         (uint32_t)(ax[2].lo / tg.tz);
+}
+
+// Wave-aggregated atomicAdd(&ctr[key], 1): returns this lane's slot.
+// Every lane of the wave must call it (inactive lanes pass active=false).
+__device__ __forceinline__ uint32_t agg_slot(uint32_t* __restrict__ ctr, uint32_t key, bool active) {
+  unsigned long long pending = __ballot(active);
+  const int lane = __lane_id();
+  uint32_t slot = 0;
+  while (pending) {
+    int leader = __ffsll((long long)pending) - 1;
+    uint32_t lkey = __shfl(key, leader);
+    unsigned long long peers = __ballot(active && key == lkey) & pending;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&ctr[lkey], (uint32_t)__popcll(peers));
+    base = __shfl(base, leader);
+    if (active && key == lkey) slot = base + (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+    pending &= ~peers;
+  }
+  return slot;
+}
+
+__global__ void fill_u32_k(uint32_t* __restrict__ p, long long n, uint32_t v) {
+  for (long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x; i < n;
+       i += (long long)gridDim.x * PMX_TPB)
+    p[i] = v;
+}
+
+static int fill_u32(pmx_ctx* c, uint32_t* p, long long n, uint32_t v) {
+  unsigned nb = pmx_blocks(n);
+  if (nb > 4096) nb = 4096;
+  pmx_tic(c, PMX_T_BIN);
+  fill_u32_k<<<nb, PMX_TPB, 0, c->stream>>>(p, n, v);
+  pmx_toc(c, PMX_T_BIN);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+// ------------------------------------------------------------- keys --
+__global__ void cell_keys_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp,
+                            long long n, MeshGeom g, uint32_t* __restrict__ key) {
+  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  if (i >= n) return;
+  int pm[3];
+  float dp[3];
+  load_ptcl(pmid, disp, i, pm, dp);
+  CicAxis ax[3];
+  cic_setup(pm, dp, g, ax);
+  key[i] = ((uint32_t)ax[0].lo * g.n1 + ax[1].lo) * g.n2 + ax[2].lo;
+}
+
+__global__ void tile_keys_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp,
+                            long long n, MeshGeom g, TileGeom tg, uint32_t* __restrict__ tile) {
+  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  if (i >= n) return;
+  int pm[3];
+  float dp[3];
+  load_ptcl(pmid, disp, i, pm, dp);
+  CicAxis ax[3];
+  cic_setup(pm, dp, g, ax);
+  tile[i] = tile_of(ax, tg);
+}
+
+extern "C" int pmx_cell_keys(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
+                             uint32_t* key) {
+  PMX_REQUIRE(c && n >= 0, "pmx_cell_keys: bad argument");
+  if (n == 0) return PMX_OK;
+  PMX_REQUIRE(pmid && disp && key, "pmx_cell_keys: NULL pointer");
+  MeshGeom g = pmx_geom_padded(c);
+  g.rs = c->n[2];
+  cell_keys_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(pmid, disp, n, g, key);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+extern "C" int pmx_tile_keys(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
+                             uint32_t* tile) {
+  PMX_REQUIRE(c && n >= 0, "pmx_tile_keys: bad argument");
+  if (n == 0) return PMX_OK;
+  PMX_REQUIRE(pmid && disp && tile, "pmx_tile_keys: NULL pointer");
+  tile_keys_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(pmid, disp, n, pmx_geom_padded(c), c->tg,
+                                                        tile);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+// ---------------------------------------------------- binned scatter --
+__global__ void bin_count_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp,
+                            long long n, MeshGeom g, TileGeom tg, uint32_t* __restrict__ tile_count,
+                            uint32_t* __restrict__ ptile, uint32_t* __restrict__ prank) {
+  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  bool active = i < n;
+  uint32_t t = 0;
+  if (active) {
+    int pm[3];
+    float dp[3];
+    load_ptcl(pmid, disp, i, pm, dp);
+    CicAxis ax[3];
+    cic_setup(pm, dp, g, ax);
+    t = tile_of(ax, tg);
+  }
+  uint32_t r = agg_slot(tile_count, t, active);
+  if (active) {
+    ptile[i] = t;
+    prank[i] = r;
+  }
+}
+
+__global__ void bin_fill_k(long long n, const uint32_t* __restrict__ ptile,
+                           const uint32_t* __restrict__ prank, const uint32_t* __restrict__ tile_start,
+                           uint32_t* __restrict__ bins) {
+  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  if (i >= n) return;
+  bins[tile_start[ptile[i]] + prank[i]] = (uint32_t)i;
+}
+
+__device__ __forceinline__ void tile_origin(int t, const TileGeom& tg, const MeshGeom& g, int o[3],
+                                            int s[3], int ti[3]) {
+  ti[2] = t % tg.ntz;
+  ti[1] = (t / tg.ntz) % tg.nty;
+  ti[0] = t / (tg.ntz * tg.nty);
+  o[0] = ti[0] * tg.tx;
+  o[1] = ti[1] * tg.ty;
+  o[2] = ti[2] * tg.tz;
+  s[0] = min(tg.tx, g.n0 - o[0]);
+  s[1] = min(tg.ty, g.n1 - o[1]);
+  s[2] = min(tg.tz, g.n2 - o[2]);
+}
+
+__device__ __forceinline__ int shell_index(int qx, int qy, int qz, int sx, int sy, int sz) {
+  if (qx == sx) return qy * (sz + 1) + qz;
+  int offA = (sy + 1) * (sz + 1);
+  if (qy == sy) return offA + qx * (sz + 1) + qz;
+  return offA + sx * (sz + 1) + qx * sy + qy;
+}
+
+// One workgroup per tile: LDS accumulation of val*w (or val_p[chan]*w) of
+// the tile's particles over the (sx+1)(sy+1)(sz+1) cells they touch.
+template <int PERVAL>
+__global__ void __launch_bounds__(PMX_TPB)
+    deposit_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp,
+              const uint32_t* __restrict__ bins, const uint32_t* __restrict__ tile_start,
+              const uint32_t* __restrict__ tile_count, MeshGeom g, TileGeom tg, float val,
+              const float* __restrict__ pval, int vstride, float* __restrict__ mesh, int mrs,
+              int accumulate, float* __restrict__ halo) {
+  extern __shared__ float acc[];
+  int o[3], s[3], ti[3];
+  const int t = blockIdx.x;
+  tile_origin(t, tg, g, o, s, ti);
+  const int LY = s[1] + 1, LZ = s[2] + 1;
+  const int ltot = (s[0] + 1) * LY * LZ;
+  for (int i = threadIdx.x; i < ltot; i += PMX_TPB) acc[i] = 0.f;
+  __syncthreads();
+  const uint32_t start = tile_start[t];
+  const uint32_t cnt = tile_count[t];
+  for (uint32_t q = threadIdx.x; q < cnt; q += PMX_TPB) {
+    long long p = bins[start + q];
+    int pm[3];
+    float dp[3];
+    load_ptcl(pmid, disp, p, pm, dp);
+    CicAxis ax[3];
+    cic_setup(pm, dp, g, ax);
+    float v = PERVAL ? pval[p * vstride] : val;
+    int lx = ax[0].lo - o[0], ly = ax[1].lo - o[1], lz = ax[2].lo - o[2];
+    // binned by this tile, so always inside; the guard keeps LDS in bounds
+    if (lx < 0 || lx >= s[0] || ly < 0 || ly >= s[1] || lz < 0 || lz >= s[2]) continue;
+    int b = (lx * LY + ly) * LZ + lz;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      int li = b + ((j & 1) ? LY * LZ : 0) + ((j & 2) ? LZ : 0) + ((j & 4) ? 1 : 0);
+      atomicAdd(&acc[li], v * pmx_w(ax, j));
+    }
+  }
+  __syncthreads();
+  const int itot = s[0] * s[1] * s[2];
+  for (int i = threadIdx.x; i < itot; i += PMX_TPB) {
+    int lz = i % s[2];
+    int ly = (i / s[2]) % s[1];
+    int lx = i / (s[2] * s[1]);
+    long long m = ((long long)(o[0] + lx) * g.n1 + (o[1] + ly)) * mrs + (o[2] + lz);
+    float a = acc[(lx * LY + ly) * LZ + lz];
+    mesh[m] = accumulate ? mesh[m] + a : a;
+  }
+  const int offA = LY * LZ;
+  const int offB = s[0] * LZ;
+  const int stot = offA + offB + s[0] * s[1];
+  float* hs = halo + (long long)t * tg.shell;
+  for (int i = threadIdx.x; i < stot; i += PMX_TPB) {
+    int qx, qy, qz;
+    if (i < offA) {
+      qx = s[0]; qy = i / LZ; qz = i % LZ;
+    } else if (i < offA + offB) {
+      int k = i - offA;
+      qx = k / LZ; qy = s[1]; qz = k % LZ;
+    } else {
+      int k = i - offA - offB;
+      qx = k / s[1]; qy = k % s[1]; qz = s[2];
+    }
+    hs[i] = acc[(qx * LY + qy) * LZ + qz];
+  }
+}
+
+// Adds to every low-face cell of tile t the shells of the (up to seven)
+// lower neighbour tiles that cover it, in a fixed order.
+__global__ void __launch_bounds__(PMX_TPB)
+    fold_k(float* __restrict__ mesh, int mrs, const float* __restrict__ halo, MeshGeom g,
+           TileGeom tg) {
+  int o[3], s[3], ti[3];
+  const int t = blockIdx.x;
+  tile_origin(t, tg, g, o, s, ti);
+  const int nA = s[1] * s[2];
+  const int nB = (s[0] - 1) * s[2];
+  const int nC = (s[0] - 1) * (s[1] - 1);
+  const int ntot = nA + nB + nC;
+  for (int i = threadIdx.x; i < ntot; i += PMX_TPB) {
+    int l[3];
+    if (i < nA) {
+      l[0] = 0; l[1] = i / s[2]; l[2] = i % s[2];
+    } else if (i < nA + nB) {
+      int k = i - nA;
+      l[0] = 1 + k / s[2]; l[1] = 0; l[2] = k % s[2];
+    } else {
+      int k = i - nA - nB;
+      l[0] = 1 + k / (s[1] - 1); l[1] = 1 + k % (s[1] - 1); l[2] = 0;
+    }
+    float sum = 0.f;
+#pragma unroll
+    for (int S = 1; S < 8; ++S) {
+      if ((S & 1) && l[0] != 0) continue;
+      if ((S & 2) && l[1] != 0) continue;
+      if ((S & 4) && l[2] != 0) continue;
+      int nt[3], ns[3], q[3];
+      nt[0] = (S & 1) ? (ti[0] == 0 ? tg.ntx - 1 : ti[0] - 1) : ti[0];
+      nt[1] = (S & 2) ? (ti[1] == 0 ? tg.nty - 1 : ti[1] - 1) : ti[1];
+      nt[2] = (S & 4) ? (ti[2] == 0 ? tg.ntz - 1 : ti[2] - 1) : ti[2];
+      ns[0] = min(tg.tx, g.n0 - nt[0] * tg.tx);
+      ns[1] = min(tg.ty, g.n1 - nt[1] * tg.ty);
+      ns[2] = min(tg.tz, g.n2 - nt[2] * tg.tz);
+      q[0] = (S & 1) ? ns[0] : l[0];
+      q[1] = (S & 2) ? ns[1] : l[1];
+      q[2] = (S & 4) ? ns[2] : l[2];
+      long long nti = ((long long)nt[0] * tg.nty + nt[1]) * tg.ntz + nt[2];
+      sum += halo[nti * tg.shell + shell_index(q[0], q[1], q[2], ns[0], ns[1], ns[2])];
+    }
+    long long m = ((long long)(o[0] + l[0]) * g.n1 + (o[1] + l[1])) * mrs + (o[2] + l[2]);
+    mesh[m] += sum;
+  }
+}
+
+// Bins the particles by tile (c->bins, c->tile_start, c->tile_count).
+static int bin_particles(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n) {
+  int rc;
+  if ((rc = pmx_ensure_ptcl(c, n))) return rc;
+  if ((rc = fill_u32(c, c->tile_count, c->ntiles + 1, 0u))) return rc;
+  MeshGeom g = pmx_geom_padded(c);
+  if (n > 0) {
+    pmx_tic(c, PMX_T_BIN);
+    bin_count_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(pmid, disp, n, g, c->tg, c->tile_count,
+                                                          c->pkey, c->prank);
+    pmx_toc(c, PMX_T_BIN);
+    PMX_LAUNCH_CHECK();
+  }
+  pmx_tic(c, PMX_T_SCAN);
+  rc = pmx_scan_exclusive(c, c->tile_count, c->tile_start, c->ntiles + 1);
+  pmx_toc(c, PMX_T_SCAN);
+  if (rc) return rc;
+  if (n > 0) {
+    pmx_tic(c, PMX_T_FILL);
+    bin_fill_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(n, c->pkey, c->prank, c->tile_start,
+                                                         c->bins);
+    pmx_toc(c, PMX_T_FILL);
+    PMX_LAUNCH_CHECK();
+  }
+  return PMX_OK;
+}
+
+// Deposits the binned particles into mesh (row stride mrs).
+static int deposit_binned(pmx_ctx* c, const int16_t* pmid, const float* disp, float val,
+                          const float* pval, int vstride, float* mesh, int mrs, int accumulate) {
+  MeshGeom g = pmx_geom_padded(c);
+  const TileGeom& tg = c->tg;
+  size_t lds = (size_t)(tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(float);
+  unsigned nt = (unsigned)c->ntiles;
+  pmx_tic(c, PMX_T_DEPOSIT);
+  if (pval)
+    deposit_k<1><<<nt, PMX_TPB, lds, c->stream>>>(pmid, disp, c->bins, c->tile_start,
+                                                  c->tile_count, g, tg, val, pval, vstride, mesh,
+                                                  mrs, accumulate, c->halo);
+  else
+    deposit_k<0><<<nt, PMX_TPB, lds, c->stream>>>(pmid, disp, c->bins, c->tile_start,
+                                                  c->tile_count, g, tg, val, nullptr, 0, mesh, mrs,
+                                                  accumulate, c->halo);
+  pmx_toc(c, PMX_T_DEPOSIT);
+  PMX_LAUNCH_CHECK();
+  pmx_tic(c, PMX_T_FOLD);
+  fold_k<<<nt, PMX_TPB, 0, c->stream>>>(mesh, mrs, c->halo, g, tg);
+  pmx_toc(c, PMX_T_FOLD);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+extern "C" int pmx_scatter_mass(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
+                                float val, float* mesh, int accumulate) {
+  PMX_REQUIRE(c && mesh && n >= 0, "pmx_scatter_mass: bad argument");
+  PMX_REQUIRE(n == 0 || (pmid && disp), "pmx_scatter_mass: NULL particle arrays");
+  PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_scatter_mass: too many particles");
+  int rc;
+  if ((rc = bin_particles(c, pmid, disp, n))) return rc;
+  return deposit_binned(c, pmid, disp, val, nullptr, 0, mesh, c->n[2], accumulate);
+}
+
+// ----------------------------------------------------------- gather3 --
+// acc_p = sum_j w_j F_d[c_j] for the three force meshes (padded rows).
+// With vcot != NULL also alpha_p = sum_j grad w_j (sum_d v_d F_d[c_j]) / cell
+// (the gather adjoint's disp cotangent, gather.py:80-116) and per-block
+// float64 partial sums of v.acc.
+template <int VJP>
+__global__ void __launch_bounds__(PMX_TPB)
+    gather3_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp, long long n,
+              MeshGeom g, const float* __restrict__ F0, const float* __restrict__ F1,
+              const float* __restrict__ F2, float* __restrict__ acc,
+              const float* __restrict__ vcot, float* __restrict__ alpha,
+              double* __restrict__ part) {
+  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  double dotv = 0;
+  if (i < n) {
+    int pm[3];
+    float dp[3];
+    load_ptcl(pmid, disp, i, pm, dp);
+    CicAxis ax[3];
+    cic_setup(pm, dp, g, ax);
+    float w[8], a0[8], a1[8], a2[8];
+    long long idx[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      idx[j] = pmx_cell_index(ax, j, g.n1, g.rs);
+      w[j] = pmx_w(ax, j);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      a0[j] = F0[idx[j]];
+      a1[j] = F1[idx[j]];
+      a2[j] = F2[idx[j]];
+    }
+    float r0[8], r1[8], r2[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      r0[j] = a0[j] * w[j];
+      r1[j] = a1[j] * w[j];
+      r2[j] = a2[j] * w[j];
+    }
+    float o0 = pmx_tree8(r0), o1 = pmx_tree8(r1), o2 = pmx_tree8(r2);
+    acc[3 * i + 0] = o0;
+    acc[3 * i + 1] = o1;
+    acc[3 * i + 2] = o2;
+    if (VJP) {
+      float v0 = vcot[3 * i + 0], v1 = vcot[3 * i + 1], v2 = vcot[3 * i + 2];
+      float gx[8], gy[8], gz[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        float dc = v0 * a0[j] + v1 * a1[j] + v2 * a2[j];
+        float gr[3];
+        pmx_wgrad(ax, j, gr);
+        gx[j] = dc * gr[0];
+        gy[j] = dc * gr[1];
+        gz[j] = dc * gr[2];
+      }
+      alpha[3 * i + 0] = pmx_tree8(gx) / g.cell;
+      alpha[3 * i + 1] = pmx_tree8(gy) / g.cell;
+      alpha[3 * i + 2] = pmx_tree8(gz) / g.cell;
+      dotv = (double)v0 * o0 + (double)v1 * o1 + (double)v2 * o2;
+    }
+  }
+  if (VJP) {
+    __shared__ double sh[PMX_TPB];
+    sh[threadIdx.x] = dotv;
+    __syncthreads();
+    for (int o = PMX_TPB / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+  }
+}
+
+// alpha_p += val * sum_j grad w_j rho_cot[c_j] / cell (scatter adjoint,
+// scatter.py:86-121, one channel, scalar val).
+__global__ void __launch_bounds__(PMX_TPB)
+    grad_gather1_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp, long long n,
+                   MeshGeom g, const float* __restrict__ R, float val, float* __restrict__ alpha) {
+  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  if (i >= n) return;
+  int pm[3];
+  float dp[3];
+  load_ptcl(pmid, disp, i, pm, dp);
+  CicAxis ax[3];
+  cic_setup(pm, dp, g, ax);
+  float gx[8], gy[8], gz[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    float dc = R[pmx_cell_index(ax, j, g.n1, g.rs)] * val;
+    float gr[3];
+    pmx_wgrad(ax, j, gr);
+    gx[j] = dc * gr[0];
+    gy[j] = dc * gr[1];
+    gz[j] = dc * gr[2];
+  }
+  alpha[3 * i + 0] += pmx_tree8(gx) / g.cell;
+  alpha[3 * i + 1] += pmx_tree8(gy) / g.cell;
+  alpha[3 * i + 2] += pmx_tree8(gz) / g.cell;
+}
+
+// ----------------------------------------------------------- gravity --
+static int gravity_fwd_meshes(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
+                              float val, float coef) {
+  int rc;
+  if ((rc = bin_particles(c, pmid, disp, n))) return rc;
+  if ((rc = deposit_binned(c, pmid, disp, val, nullptr, 0, (float*)c->cbuf[0], c->rs, 0))) return rc;
+  return pmx_poisson_ctx(c, coef);
+}
+
+extern "C" int pmx_gravity(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
+                           float val, float coef, float* acc) {
+  PMX_REQUIRE(c && n >= 0, "pmx_gravity: bad argument");
+  PMX_REQUIRE(n == 0 || (pmid && disp && acc), "pmx_gravity: NULL pointer");
+  PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_gravity: too many particles");
+  int rc;
+  if ((rc = gravity_fwd_meshes(c, pmid, disp, n, val, coef))) return rc;
+  if (n == 0) return PMX_OK;
+  MeshGeom g = pmx_geom_padded(c);
+  pmx_tic(c, PMX_T_GATHER);
+    gather3_k<0><<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(
+      pmid, disp, n, g, (const float*)c->cbuf[0], (const float*)c->cbuf[1],
+      (const float*)c->cbuf[2], acc, nullptr, nullptr, nullptr);
+  pmx_toc(c, PMX_T_GATHER);
+    PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+extern "C" int pmx_gravity_vjp(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
+                               float val, float coef, const float* vel_cot, float* acc,
+                               float* acc_cot, double* dot) {
+  PMX_REQUIRE(c && n >= 0, "pmx_gravity_vjp: bad argument");
+  PMX_REQUIRE(n == 0 || (pmid && disp && acc && acc_cot && vel_cot), "pmx_gravity_vjp: NULL pointer");
+  PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_gravity_vjp: too many particles");
+  int rc;
+  if ((rc = gravity_fwd_meshes(c, pmid, disp, n, val, coef))) return rc;
+  MeshGeom g = pmx_geom_padded(c);
+  int nblk = (int)pmx_blocks(n);
+  if ((rc = pmx_ensure_red(c, nblk))) return rc;
+  if (n > 0) {
+    pmx_tic(c, PMX_T_GATHER);
+    gather3_k<1><<<nblk, PMX_TPB, 0, c->stream>>>(
+        pmid, disp, n, g, (const float*)c->cbuf[0], (const float*)c->cbuf[1],
+        (const float*)c->cbuf[2], acc, vel_cot, acc_cot, c->red);
+    pmx_toc(c, PMX_T_GATHER);
+    PMX_LAUNCH_CHECK();
+  }
+  // the particles did not move: reuse the bins for the three cotangent meshes
+  for (int d = 0; d < 3; ++d)
+    if ((rc = deposit_binned(c, pmid, disp, 0.f, vel_cot + d, 3, (float*)c->cbuf[d], c->rs, 0)))
+      return rc;
+  // rho_cot = -coef * sum_d L_d(mesh_cot_d), L_d = irfftn . (-ik_d) . laplace . rfftn
+  if ((rc = pmx_poisson_adj_ctx(c, (float)(-(double)coef / (double)c->M)))) return rc;
+  if (n > 0) {
+    grad_gather1_k<<<nblk, PMX_TPB, 0, c->stream>>>(pmid, disp, n, g, (const float*)c->cbuf[0],
+                                                    val, acc_cot);
+    PMX_LAUNCH_CHECK();
+  }
+  if (dot) {
+    if (n == 0) {
+      *dot = 0;
+    } else if ((rc = pmx_red_finish(c, nblk, dot))) {
+      return rc;
+    }
+  }
+  return PMX_OK;
+}
+
+// ------------------------------------------------------ time stepping --
+__global__ void axpy3_k(float* __restrict__ y, const float* __restrict__ x, long long n3, float f) {
+  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  if (i < n3) y[i] = y[i] + x[i] * f;
+}
+
+extern "C" int pmx_kick(pmx_ctx* c, long long n, float* vel, const float* acc, float k) {
+  PMX_REQUIRE(c && n >= 0 && (n == 0 || (vel && acc)), "pmx_kick: bad argument");
+  if (n == 0) return PMX_OK;
+  axpy3_k<<<pmx_blocks(3 * n), PMX_TPB, 0, c->stream>>>(vel, acc, 3 * n, k);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+extern "C" int pmx_drift(pmx_ctx* c, long long n, float* disp, const float* vel, float d) {
+  PMX_REQUIRE(c && n >= 0 && (n == 0 || (disp && vel)), "pmx_drift: bad argument");
+  if (n == 0) return PMX_OK;
+  axpy3_k<<<pmx_blocks(3 * n), PMX_TPB, 0, c->stream>>>(disp, vel, 3 * n, d);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+// kick (k1) + drift (d) fused with the tile count of the drifted particles
+__global__ void kick_drift_bin_k(const int16_t* __restrict__ pmid, float* __restrict__ disp,
+                                 float* __restrict__ vel, const float* __restrict__ acc, long long n,
+                                 float k1, float dd, MeshGeom g, TileGeom tg,
+                                 uint32_t* __restrict__ tile_count, uint32_t* __restrict__ ptile,
+                                 uint32_t* __restrict__ prank) {
+  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  bool active = i < n;
+  uint32_t t = 0;
+  if (active) {
+    int pm[3];
+    float dp[3];
+    pm[0] = pmid[3 * i]; pm[1] = pmid[3 * i + 1]; pm[2] = pmid[3 * i + 2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float v = vel[3 * i + a];
+      v = v + acc[3 * i + a] * k1;
+      float x = disp[3 * i + a];
+      x = x + v * dd;
+      vel[3 * i + a] = v;
+      disp[3 * i + a] = x;
+      dp[a] = x;
+    }
+    CicAxis ax[3];
+    cic_setup(pm, dp, g, ax);
+    t = tile_of(ax, tg);
+  }
+  uint32_t r = agg_slot(tile_count, t, active);
+  if (active) {
+    ptile[i] = t;
+    prank[i] = r;
+  }
+}
+
+// gather of the three force meshes fused with the closing kick (k2)
+__global__ void __launch_bounds__(PMX_TPB)
+    gather3_kick_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp, long long n,
+                   MeshGeom g, const float* __restrict__ F0, const float* __restrict__ F1,
+                   const float* __restrict__ F2, float* __restrict__ acc, float* __restrict__ vel,
+                   float k2) {
+  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  if (i >= n) return;
+  int pm[3];
+  float dp[3];
+  load_ptcl(pmid, disp, i, pm, dp);
+  CicAxis ax[3];
+  cic_setup(pm, dp, g, ax);
+  float w[8], r0[8], r1[8], r2[8];
+  long long idx[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    idx[j] = pmx_cell_index(ax, j, g.n1, g.rs);
+    w[j] = pmx_w(ax, j);
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    r0[j] = F0[idx[j]] * w[j];
+    r1[j] = F1[idx[j]] * w[j];
+    r2[j] = F2[idx[j]] * w[j];
+  }
+  float o[3] = {pmx_tree8(r0), pmx_tree8(r1), pmx_tree8(r2)};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    acc[3 * i + a] = o[a];
+    vel[3 * i + a] = vel[3 * i + a] + o[a] * k2;
+  }
+}
+
+extern "C" int pmx_nbody_step(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, float* acc,
+                              long long n, float val, float coef, float k1, float d, float k2) {
+  PMX_REQUIRE(c && n >= 0, "pmx_nbody_step: bad argument");
+  PMX_REQUIRE(n == 0 || (pmid && disp && vel && acc), "pmx_nbody_step: NULL pointer");
+  PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_nbody_step: too many particles");
+  int rc;
+  if ((rc = pmx_ensure_ptcl(c, n))) return rc;
+  if ((rc = fill_u32(c, c->tile_count, c->ntiles + 1, 0u))) return rc;
+  MeshGeom g = pmx_geom_padded(c);
+  if (n > 0) {
+    pmx_tic(c, PMX_T_BIN);
+    kick_drift_bin_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(
+        pmid, disp, vel, acc, n, k1, d, g, c->tg, c->tile_count, c->pkey, c->prank);
+    pmx_toc(c, PMX_T_BIN);
+    PMX_LAUNCH_CHECK();
+  }
+  pmx_tic(c, PMX_T_SCAN);
+  rc = pmx_scan_exclusive(c, c->tile_count, c->tile_start, c->ntiles + 1);
+  pmx_toc(c, PMX_T_SCAN);
+  if (rc) return rc;
+  if (n > 0) {
+    pmx_tic(c, PMX_T_FILL);
+    bin_fill_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(n, c->pkey, c->prank, c->tile_start,
+                                                         c->bins);
+    pmx_toc(c, PMX_T_FILL);
+    PMX_LAUNCH_CHECK();
+  }
+  if ((rc = deposit_binned(c, pmid, disp, val, nullptr, 0, (float*)c->cbuf[0], c->rs, 0))) return rc;
+  if ((rc = pmx_poisson_ctx(c, coef))) return rc;
+  if (n > 0) {
+    pmx_tic(c, PMX_T_GATHER);
+    gather3_kick_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(
+        pmid, disp, n, g, (const float*)c->cbuf[0], (const float*)c->cbuf[1],
+        (const float*)c->cbuf[2], acc, vel, k2);
+    pmx_toc(c, PMX_T_GATHER);
+    PMX_LAUNCH_CHECK();
+  }
+  return PMX_OK;
+}
+
+// kick_adj: vel += acc*k; disp_cot -= acc_cot*k; partial sums of vel_cot.acc
+__global__ void kick_adj_k(long long n3, float* __restrict__ vel, const float* __restrict__ acc,
+                           float* __restrict__ disp_cot, const float* __restrict__ acc_cot,
+                           const float* __restrict__ vel_cot, float k, double* __restrict__ part) {
+  __shared__ double sh[PMX_TPB];
+  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  double s = 0;
+  if (i < n3) {
+    float a = acc[i];
+    vel[i] = vel[i] + a * k;
+    disp_cot[i] = disp_cot[i] - acc_cot[i] * k;
+    s = (double)vel_cot[i] * a;
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = PMX_TPB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// drift_adj: disp += vel*d; vel_cot -= disp_cot*d; partial sums of disp_cot.vel
+__global__ void drift_adj_k(long long n3, float* __restrict__ disp, const float* __restrict__ vel,
+                            const float* __restrict__ disp_cot, float* __restrict__ vel_cot,
+                            float d, double* __restrict__ part) {
+  __shared__ double sh[PMX_TPB];
+  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  double s = 0;
+  if (i < n3) {
+    float v = vel[i];
+    float dc = disp_cot[i];
+    disp[i] = disp[i] + v * d;
+    vel_cot[i] = vel_cot[i] - dc * d;
+    s = (double)dc * v;
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = PMX_TPB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+extern "C" int pmx_kick_adj(pmx_ctx* c, long long n, float* vel, const float* acc, float* disp_cot,
+                            const float* acc_cot, const float* vel_cot, float k, double* dot) {
+  PMX_REQUIRE(c && n >= 0, "pmx_kick_adj: bad argument");
+  PMX_REQUIRE(n == 0 || (vel && acc && disp_cot && acc_cot && vel_cot), "pmx_kick_adj: NULL pointer");
+  if (n == 0) {
+    if (dot) *dot = 0;
+    return PMX_OK;
+  }
+  int nblk = (int)pmx_blocks(3 * n);
+  int rc;
+  if ((rc = pmx_ensure_red(c, nblk))) return rc;
+  kick_adj_k<<<nblk, PMX_TPB, 0, c->stream>>>(3 * n, vel, acc, disp_cot, acc_cot, vel_cot, k, c->red);
+  PMX_LAUNCH_CHECK();
+  if (dot) return pmx_red_finish(c, nblk, dot);
+  return PMX_OK;
+}
+
+extern "C" int pmx_drift_adj(pmx_ctx* c, long long n, float* disp, const float* vel,
+                             const float* disp_cot, float* vel_cot, float d, double* dot) {
+  PMX_REQUIRE(c && n >= 0, "pmx_drift_adj: bad argument");
+  PMX_REQUIRE(n == 0 || (disp && vel && disp_cot && vel_cot), "pmx_drift_adj: NULL pointer");
+  if (n == 0) {
+    if (dot) *dot = 0;
+    return PMX_OK;
+  }
+  int nblk = (int)pmx_blocks(3 * n);
+  int rc;
+  if ((rc = pmx_ensure_red(c, nblk))) return rc;
+  drift_adj_k<<<nblk, PMX_TPB, 0, c->stream>>>(3 * n, disp, vel, disp_cot, vel_cot, d, c->red);
+  PMX_LAUNCH_CHECK();
+  if (dot) return pmx_red_finish(c, nblk, dot);
+  return PMX_OK;
+}

@@ -1,0 +1,433 @@
+// Context, errors, scans and reductions of the pmx library.
+#include "pmx_internal.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+
+static thread_local char g_err[1024] = "";
+
+void pmx_set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+extern "C" const char* pmx_last_error(void) { return g_err; }
+extern "C" int pmx_version(void) { return 100; }
+
+static int dev_alloc(pmx_ctx* c, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    pmx_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    *p = nullptr;
+    return PMX_ENOMEM;
+  }
+  c->bytes += bytes;
+  return PMX_OK;
+}
+
+static void dev_free(pmx_ctx* c, void** p, size_t bytes) {
+  if (*p) {
+    hipFree(*p);
+    c->bytes -= bytes;
+  }
+  *p = nullptr;
+}
+
+// Scatter tile extent: 16 x 16 x 32 cells, clipped to the mesh.
+static TileGeom make_tiles(const int n[3]) {
+  TileGeom t;
+  t.tx = n[0] < 16 ? n[0] : 16;
+  t.ty = n[1] < 16 ? n[1] : 16;
+  t.tz = n[2] < 32 ? n[2] : 32;
+  t.ntx = (n[0] + t.tx - 1) / t.tx;
+  t.nty = (n[1] + t.ty - 1) / t.ty;
+  t.ntz = (n[2] + t.tz - 1) / t.tz;
+  t.shell = (t.tx + 1) * (t.ty + 1) * (t.tz + 1) - t.tx * t.ty * t.tz;
+  return t;
+}
+
+// angular wavenumbers as jnp.fft.fftfreq(n) * (2 pi / cell), float64, then
+// float32 (pm_util.py:185-197); rfft axis uses rfftfreq.
+static void kvec_host(int n, double cell, bool rfft, std::vector<float>& k,
+                      std::vector<float>& km) {
+  int len = rfft ? n / 2 + 1 : n;
+  k.resize(len);
+  km.resize(len);
+  double period = 2 * M_PI / cell;
+  double nyq = M_PI / cell;
+  float nyqf = (float)nyq;
+  float eps = (float)(nyq * (double)FLT_EPSILON);
+  for (int i = 0; i < len; ++i) {
+    long long j = i;
+    if (!rfft && i >= (n + 1) / 2) j = i - n;  // fftfreq ordering
+    double f = (double)j / (double)n;
+    float kf = (float)(f * period);
+    k[i] = kf;
+    bool is_nyq = fabsf(fabsf(kf) - nyqf) <= eps;
+    km[i] = is_nyq ? 0.f : kf;
+  }
+}
+
+extern "C" int pmx_ctx_create(int n0, int n1, int n2, double cell_size, void* stream,
+                              pmx_ctx** out) {
+  PMX_REQUIRE(out != nullptr, "pmx_ctx_create: out is NULL");
+  *out = nullptr;
+  PMX_REQUIRE(n0 > 0 && n1 > 0 && n2 > 0, "pmx_ctx_create: mesh shape must be positive");
+  PMX_REQUIRE(n0 <= 4096 && n1 <= 4096 && n2 <= 4096,
+              "pmx_ctx_create: mesh extent > 4096 is not supported");
+  PMX_REQUIRE((long long)n0 * n1 * n2 < (1LL << 31), "pmx_ctx_create: mesh must have < 2^31 cells");
+  PMX_REQUIRE(cell_size > 0, "pmx_ctx_create: cell_size must be positive");
+  pmx_ctx* c = (pmx_ctx*)calloc(1, sizeof(pmx_ctx));
+  if (!c) {
+    pmx_set_error("pmx_ctx_create: out of host memory");
+    return PMX_ENOMEM;
+  }
+  c->n[0] = n0; c->n[1] = n1; c->n[2] = n2;
+  c->n2h = n2 / 2 + 1;
+  c->rs = 2 * c->n2h;
+  c->M = (long long)n0 * n1 * n2;
+  c->Mc = (long long)n0 * n1 * c->n2h;
+  c->cell = cell_size;
+  c->cellf = (float)cell_size;
+  c->stream = (hipStream_t)stream;
+  hipGetDevice(&c->device);
+
+  int rc = PMX_OK;
+  for (int a = 0; a < 3 && rc == PMX_OK; ++a) {
+    std::vector<float> k, km;
+    kvec_host(c->n[a], cell_size, a == 2, k, km);
+    rc |= dev_alloc(c, (void**)&c->k[a], k.size() * 4);
+    rc |= dev_alloc(c, (void**)&c->km[a], km.size() * 4);
+    if (rc) break;
+    hipMemcpy(c->k[a], k.data(), k.size() * 4, hipMemcpyHostToDevice);
+    hipMemcpy(c->km[a], km.data(), km.size() * 4, hipMemcpyHostToDevice);
+    int n = c->n[a];
+    std::vector<float2> tw(n);
+    for (int j = 0; j < n; ++j) {
+      double ang = -2.0 * M_PI * (double)j / (double)n;
+      tw[j] = make_float2((float)cos(ang), (float)sin(ang));
+    }
+    rc |= dev_alloc(c, (void**)&c->tw[a], n * sizeof(float2));
+    if (rc) break;
+    hipMemcpy(c->tw[a], tw.data(), n * sizeof(float2), hipMemcpyHostToDevice);
+  }
+  for (int b = 0; b < 3 && rc == PMX_OK; ++b)
+    rc |= dev_alloc(c, (void**)&c->cbuf[b], (size_t)c->Mc * sizeof(float2));
+
+  c->tg = make_tiles(c->n);
+  c->ntiles = (long long)c->tg.ntx * c->tg.nty * c->tg.ntz;
+  if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->tile_count, (size_t)(c->ntiles + 1) * 4);
+  if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->tile_start, (size_t)(c->ntiles + 1) * 4);
+  if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->halo, (size_t)c->ntiles * c->tg.shell * 4);
+  if (rc == PMX_OK) rc |= pmx_ensure_red(c, 1024);
+  if (rc != PMX_OK) {
+    pmx_ctx_destroy(c);
+    return rc;
+  }
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    pmx_set_error("pmx_ctx_create: %s", hipGetErrorString(e));
+    pmx_ctx_destroy(c);
+    return PMX_EHIP;
+  }
+  *out = c;
+  return PMX_OK;
+}
+
+void pmx_timer_destroy(pmx_ctx* c);
+
+extern "C" int pmx_ctx_destroy(pmx_ctx* c) {
+  if (!c) return PMX_OK;
+  hipDeviceSynchronize();
+  pmx_timer_destroy(c);
+  for (int a = 0; a < 3; ++a) {
+    if (c->k[a]) hipFree(c->k[a]);
+    if (c->km[a]) hipFree(c->km[a]);
+    if (c->tw[a]) hipFree(c->tw[a]);
+    if (c->cbuf[a]) hipFree(c->cbuf[a]);
+  }
+  void* ps[] = {c->tile_count, c->tile_start, c->halo, c->bins, c->pkey, c->prank, c->red,
+                c->sort_k[0], c->sort_k[1], c->sort_v[0], c->sort_v[1], c->sort_hist,
+                c->scan_tmp};
+  for (void* p : ps)
+    if (p) hipFree(p);
+  if (c->red_host) hipHostFree(c->red_host);
+  free(c);
+  return PMX_OK;
+}
+
+extern "C" int pmx_ctx_set_stream(pmx_ctx* c, void* stream) {
+  PMX_REQUIRE(c != nullptr, "pmx_ctx_set_stream: ctx is NULL");
+  c->stream = (hipStream_t)stream;
+  return PMX_OK;
+}
+
+extern "C" long long pmx_ctx_bytes(const pmx_ctx* c) { return c ? (long long)c->bytes : 0; }
+
+extern "C" int pmx_tile_shape(const pmx_ctx* c, int t[3]) {
+  PMX_REQUIRE(c != nullptr && t != nullptr, "pmx_tile_shape: NULL argument");
+  t[0] = c->tg.tx; t[1] = c->tg.ty; t[2] = c->tg.tz;
+  return PMX_OK;
+}
+
+int pmx_ensure_ptcl(pmx_ctx* c, long long n) {
+  if (n <= c->ptcl_cap) return PMX_OK;
+  long long cap = n + n / 8 + 1024;
+  dev_free(c, (void**)&c->bins, c->ptcl_cap * 4);
+  dev_free(c, (void**)&c->pkey, c->ptcl_cap * 4);
+  dev_free(c, (void**)&c->prank, c->ptcl_cap * 4);
+  c->ptcl_cap = 0;
+  int rc = dev_alloc(c, (void**)&c->bins, cap * 4);
+  rc |= dev_alloc(c, (void**)&c->pkey, cap * 4);
+  rc |= dev_alloc(c, (void**)&c->prank, cap * 4);
+  if (rc) return PMX_ENOMEM;
+  c->ptcl_cap = cap;
+  return PMX_OK;
+}
+
+int pmx_ensure_red(pmx_ctx* c, int nblk) {
+  if (nblk <= c->red_cap) return PMX_OK;
+  dev_free(c, (void**)&c->red, (size_t)c->red_cap * 8);
+  if (c->red_host) hipHostFree(c->red_host);
+  c->red_host = nullptr;
+  c->red_cap = 0;
+  if (dev_alloc(c, (void**)&c->red, (size_t)nblk * 8)) return PMX_ENOMEM;
+  if (hipHostMalloc((void**)&c->red_host, (size_t)nblk * 8, hipHostMallocDefault) != hipSuccess) {
+    pmx_set_error("hipHostMalloc failed");
+    return PMX_ENOMEM;
+  }
+  c->red_cap = nblk;
+  return PMX_OK;
+}
+
+MeshGeom pmx_geom_padded(const pmx_ctx* c) {
+  MeshGeom g;
+  g.n0 = c->n[0]; g.n1 = c->n[1]; g.n2 = c->n[2];
+  g.rs = c->rs;
+  g.cell = c->cellf;
+  g.inv_cell = 1.f / c->cellf;
+  return g;
+}
+
+// ------------------------------------------------------------------ scan --
+// Exclusive prefix sum of uint32, three phases: per-block sums, a
+// single-block scan of the block sums, then per-block scan plus offset.
+#define SCAN_ITEMS 8
+#define SCAN_TILE (PMX_TPB * SCAN_ITEMS)
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
+  int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < PMX_TPB; off <<= 1) {
+    uint32_t x = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += x;
+    __syncthreads();
+  }
+  uint32_t incl = sh[t];
+  *total = sh[PMX_TPB - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void scan_reduce_k(const uint32_t* __restrict__ in, long long n, uint32_t* __restrict__ bsum) {
+  __shared__ uint32_t sh[PMX_TPB];
+  long long base = (long long)blockIdx.x * SCAN_TILE;
+  uint32_t s = 0;
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    long long j = base + (long long)i * PMX_TPB + threadIdx.x;
+    if (j < n) s += in[j];
+  }
+  uint32_t tot;
+  block_exclusive_scan(s, sh, &tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void scan_single_k(uint32_t* __restrict__ data, long long n) {
+  __shared__ uint32_t sh[PMX_TPB];
+  uint32_t carry = 0;
+  for (long long base = 0; base < n; base += PMX_TPB) {
+    long long j = base + threadIdx.x;
+    uint32_t v = j < n ? data[j] : 0;
+    uint32_t tot;
+    uint32_t ex = block_exclusive_scan(v, sh, &tot);
+    if (j < n) data[j] = ex + carry;
+    carry += tot;
+  }
+}
+
+__global__ void scan_apply_k(const uint32_t* __restrict__ in, long long n,
+                             const uint32_t* __restrict__ boff, uint32_t* __restrict__ out) {
+  __shared__ uint32_t sh[PMX_TPB];
+  long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  uint32_t s = 0;
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    long long j = base + i;
+    v[i] = j < n ? in[j] : 0;
+    s += v[i];
+  }
+  uint32_t tot;
+  uint32_t ex = block_exclusive_scan(s, sh, &tot) + boff[blockIdx.x];
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    long long j = base + i;
+    if (j < n) out[j] = ex;
+    ex += v[i];
+  }
+}
+
+int pmx_scan_exclusive(pmx_ctx* c, const uint32_t* in, uint32_t* out, long long n) {
+  long long nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (nb < 1) nb = 1;
+  if (nb > c->scan_tmp_cap) {
+    dev_free(c, (void**)&c->scan_tmp, c->scan_tmp_cap * 4);
+    c->scan_tmp_cap = 0;
+    if (dev_alloc(c, (void**)&c->scan_tmp, nb * 4)) return PMX_ENOMEM;
+    c->scan_tmp_cap = nb;
+  }
+  scan_reduce_k<<<(unsigned)nb, PMX_TPB, 0, c->stream>>>(in, n, c->scan_tmp);
+  scan_single_k<<<1, PMX_TPB, 0, c->stream>>>(c->scan_tmp, nb);
+  scan_apply_k<<<(unsigned)nb, PMX_TPB, 0, c->stream>>>(in, n, c->scan_tmp, out);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+// -------------------------------------------------------------- dot ---
+__global__ void dot_k(const float* __restrict__ a, const float* __restrict__ b, long long n,
+                      double* __restrict__ part) {
+  __shared__ double sh[PMX_TPB];
+  double s = 0;
+  for (long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x; i < n;
+       i += (long long)gridDim.x * PMX_TPB)
+    s += (double)a[i] * (double)b[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = PMX_TPB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+int pmx_red_finish(pmx_ctx* c, int nblk, double* out) {
+  PMX_CHECK_HIP(hipMemcpyAsync(c->red_host, c->red, (size_t)nblk * 8, hipMemcpyDeviceToHost,
+                               c->stream));
+  PMX_CHECK_HIP(hipStreamSynchronize(c->stream));
+  double s = 0;
+  for (int i = 0; i < nblk; ++i) s += c->red_host[i];
+  *out = s;
+  return PMX_OK;
+}
+
+extern "C" int pmx_dot(pmx_ctx* c, const float* a, const float* b, long long n, double* out) {
+  PMX_REQUIRE(c && a && b && out && n >= 0, "pmx_dot: bad argument");
+  int nblk = 512;
+  dot_k<<<nblk, PMX_TPB, 0, c->stream>>>(a, b, n, c->red);
+  PMX_LAUNCH_CHECK();
+  return pmx_red_finish(c, nblk, out);
+}
+
+// ------------------------------------------------------------- timing ---
+#include <vector>
+
+struct PmxTimer {
+  struct Rec {
+    int cls;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t open[PMX_NTIMER];
+  double ms[PMX_NTIMER];
+  long long cnt[PMX_NTIMER];
+};
+
+static hipEvent_t timer_event(PmxTimer* t) {
+  hipEvent_t e = nullptr;
+  if (!t->pool.empty()) {
+    e = t->pool.back();
+    t->pool.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    e = nullptr;
+  }
+  return e;
+}
+
+void pmx_tic(pmx_ctx* c, int cls) {
+  if (!c->timer) return;
+  hipEvent_t e = timer_event(c->timer);
+  if (e && hipEventRecord(e, c->stream) == hipSuccess) c->timer->open[cls] = e;
+  else c->timer->open[cls] = nullptr;
+}
+
+void pmx_toc(pmx_ctx* c, int cls) {
+  if (!c->timer || !c->timer->open[cls]) return;
+  hipEvent_t e = timer_event(c->timer);
+  if (e && hipEventRecord(e, c->stream) == hipSuccess)
+    c->timer->recs.push_back({cls, c->timer->open[cls], e});
+  c->timer->open[cls] = nullptr;
+}
+
+static void timer_free(PmxTimer* t) {
+  for (auto& r : t->recs) {
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
+  for (auto e : t->pool) hipEventDestroy(e);
+  delete t;
+}
+
+extern "C" int pmx_timing_enable(pmx_ctx* c, int on) {
+  PMX_REQUIRE(c != nullptr, "pmx_timing_enable: ctx is NULL");
+  if (on && !c->timer) {
+    c->timer = new PmxTimer();
+    for (int i = 0; i < PMX_NTIMER; ++i) {
+      c->timer->open[i] = nullptr;
+      c->timer->ms[i] = 0;
+      c->timer->cnt[i] = 0;
+    }
+  } else if (!on && c->timer) {
+    hipStreamSynchronize(c->stream);
+    timer_free(c->timer);
+    c->timer = nullptr;
+  }
+  return PMX_OK;
+}
+
+extern "C" int pmx_timing_read(pmx_ctx* c, double* ms, long long* count) {
+  PMX_REQUIRE(c && ms && count, "pmx_timing_read: NULL argument");
+  for (int i = 0; i < PMX_NTIMER; ++i) {
+    ms[i] = 0;
+    count[i] = 0;
+  }
+  if (!c->timer) return PMX_OK;
+  PMX_CHECK_HIP(hipStreamSynchronize(c->stream));
+  PmxTimer* t = c->timer;
+  for (auto& r : t->recs) {
+    float e = 0.f;
+    if (hipEventElapsedTime(&e, r.a, r.b) == hipSuccess) {
+      ms[r.cls] += e;
+      count[r.cls] += 1;
+    }
+    t->pool.push_back(r.a);
+    t->pool.push_back(r.b);
+  }
+  t->recs.clear();
+  return PMX_OK;
+}
+
+void pmx_timer_destroy(pmx_ctx* c) {
+  if (c->timer) {
+    timer_free(c->timer);
+    c->timer = nullptr;
+  }
+}

@@ -1,0 +1,636 @@
+// Hand-written 3D real FFTs and the fused Poisson/gradient solve (gfx950).
+//
+// Layout: a real mesh lives in "padded" rows of rs = 2*(n2/2+1) floats so the
+// r2c/c2r row passes run in place; the complex mesh is (n0, n1, n2h) float2.
+// Each kernel stages P pencils of length n in LDS (pencil-major for the
+// contiguous row passes, interleaved for the strided passes) and runs a
+// Stockham autosort FFT with radix 8/4/2 stages, or a direct DFT for lengths
+// that are not powers of two.
+//
+// Pass order (forward): z r2c (rows, contiguous) -> y (strided) -> x
+// (strided).  The x pass of the Poisson solve does forward x FFT, Green's
+// function, -ik_d and the inverse x FFT for the three components, so the
+// inverse ends with y (strided) and z c2r (rows).
+#include "pmx_internal.h"
+
+#define FFT_TPB 512                  // threads per FFT workgroup
+#define FFT_CAP 8192                 // complex values per workgroup
+#define FFT_MAXV (FFT_CAP / FFT_TPB) // values per thread
+#define PFFT_CAP 4096                // Poisson x pass keeps per-thread state
+#define PFFT_MAXV (PFFT_CAP / FFT_TPB)
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// multiply by -i (forward, DIR<0) or +i (inverse)
+template <int DIR>
+__device__ __forceinline__ float2 mul_mi(float2 a) {
+  return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+template <int DIR>
+__device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int m) {
+  float2 w = tw[m];
+  if (DIR > 0) w.y = -w.y;
+  return w;
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft2(float2* v) {
+  float2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft4(float2* v) {
+  float2 t0 = cadd(v[0], v[2]);
+  float2 t1 = csub(v[0], v[2]);
+  float2 t2 = cadd(v[1], v[3]);
+  float2 t3 = mul_mi<DIR>(csub(v[1], v[3]));
+  v[0] = cadd(t0, t2);
+  v[2] = csub(t0, t2);
+  v[1] = cadd(t1, t3);
+  v[3] = csub(t1, t3);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft8(float2* v) {
+  const float h = 0.70710678118654752440f;
+  float2 e[4] = {v[0], v[2], v[4], v[6]};
+  float2 o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<DIR>(e);
+  dft4<DIR>(o);
+  // w8^q for q = 0..3 with w8 = exp(DIR * i pi/4)
+  float2 o1 = DIR < 0 ? make_float2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x))
+                      : make_float2(h * (o[1].x - o[1].y), h * (o[1].y + o[1].x));
+  float2 o2 = mul_mi<DIR>(o[2]);
+  float2 o3 = DIR < 0 ? make_float2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y))
+                      : make_float2(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y));
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);
+  v[7] = csub(e[3], o3);
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void dftR(float2* v) {
+  if (R == 2) dft2<DIR>(v);
+  else if (R == 4) dft4<DIR>(v);
+  else dft8<DIR>(v);
+}
+
+// LDS layouts of P pencils of length n:
+//   ROWS (pencil-major):  element (p, j) at p*n + j
+//   COLS (interleaved):   element (p, j) at j*P + p
+// Work items are mapped so that consecutive lanes touch consecutive words.
+enum { ROWS = 0, COLS = 1 };
+
+template <int LAY>
+__device__ __forceinline__ int lidx(int p, int j, int n, int P) {
+  return LAY == ROWS ? p * n + j : j * P + p;
+}
+
+// One Stockham stage of radix R (n a power of two, Ns the size of the
+// sub-transforms already done): reads x[j + r n/R], twiddles by
+// w_{Ns R}^{(j mod Ns) r}, and writes the radix-R DFT to
+// (j - j mod Ns) R + j mod Ns + r Ns.  Twiddle powers are built from at most
+// three table loads (w, w^2, w^4) with at most two extra products.
+template <int R, int DIR, int LAY, int MAXV, int TPB>
+__device__ void fft_stage(float2* s, int n, int P, int Ns, int lnb, int lP,
+                          const float2* __restrict__ tw) {
+  const int nb = n / R;
+  const int total = P * nb;
+  const int step = n / (Ns * R);
+  float2 v[MAXV];
+#pragma unroll
+  for (int it = 0; it < MAXV / R; ++it) {
+    int idx = threadIdx.x + it * TPB;
+    if (idx < total) {
+      int p = LAY == ROWS ? (idx >> lnb) : (idx & (P - 1));
+      int j = LAY == ROWS ? (idx & (nb - 1)) : (idx >> lP);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[it * R + r] = s[lidx<LAY>(p, j + r * nb, n, P)];
+      int m = (j & (Ns - 1)) * step;
+      if (m != 0) {
+        float2* u = &v[it * R];
+        float2 w1 = twiddle<DIR>(tw, m);
+        u[1] = cmul(u[1], w1);
+        if (R >= 4) {
+          float2 w2 = twiddle<DIR>(tw, 2 * m);
+          float2 w3 = cmul(w2, w1);
+          u[2] = cmul(u[2], w2);
+          u[3] = cmul(u[3], w3);
+          if (R == 8) {
+            float2 w4 = twiddle<DIR>(tw, 4 * m);
+            u[4] = cmul(u[4], w4);
+            u[5] = cmul(u[5], cmul(w4, w1));
+            u[6] = cmul(u[6], cmul(w4, w2));
+            u[7] = cmul(u[7], cmul(w4, w3));
+          }
+        }
+      }
+      dftR<R, DIR>(&v[it * R]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < MAXV / R; ++it) {
+    int idx = threadIdx.x + it * TPB;
+    if (idx < total) {
+      int p = LAY == ROWS ? (idx >> lnb) : (idx & (P - 1));
+      int j = LAY == ROWS ? (idx & (nb - 1)) : (idx >> lP);
+      int k = j & (Ns - 1);
+      int o = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) s[lidx<LAY>(p, o + r * Ns, n, P)] = v[it * R + r];
+    }
+  }
+  __syncthreads();
+}
+
+// Direct DFT of P pencils of any length n (P*n <= FFT_CAP).
+template <int DIR, int LAY, int MAXV, int TPB>
+__device__ void fft_direct(float2* s, int n, int P, const float2* __restrict__ tw) {
+  float2 v[MAXV];
+  const int total = P * n;
+#pragma unroll
+  for (int it = 0; it < MAXV; ++it) {
+    int idx = threadIdx.x + it * TPB;
+    if (idx < total) {
+      int p = LAY == ROWS ? idx / n : idx % P;
+      int k = LAY == ROWS ? idx % n : idx / P;
+      float2 acc = make_float2(0.f, 0.f);
+      int m = 0;
+      for (int j = 0; j < n; ++j) {
+        float2 w = twiddle<DIR>(tw, m);
+        float2 x = s[lidx<LAY>(p, j, n, P)];
+        acc.x += x.x * w.x - x.y * w.y;
+        acc.y += x.x * w.y + x.y * w.x;
+        m += k;
+        if (m >= n) m -= n;
+      }
+      v[it] = acc;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < MAXV; ++it) {
+    int idx = threadIdx.x + it * TPB;
+    if (idx < total) {
+      int p = LAY == ROWS ? idx / n : idx % P;
+      int k = LAY == ROWS ? idx % n : idx / P;
+      s[lidx<LAY>(p, k, n, P)] = v[it];
+    }
+  }
+  __syncthreads();
+}
+
+// Unnormalised in-LDS FFT of P pencils (exp(DIR 2 pi i jk/n)).  POW2
+// selects the Stockham path (n and P powers of two) or the direct DFT.
+// Callers synchronise before it; it ends with a barrier.
+template <int DIR, int LAY, bool POW2, int MAXV = FFT_MAXV, int TPB = FFT_TPB>
+__device__ void lds_fft(float2* s, int n, int P, const float2* __restrict__ tw) {
+  if (n == 1) return;
+  if (!POW2) {
+    fft_direct<DIR, LAY, MAXV, TPB>(s, n, P, tw);
+    return;
+  }
+  const int lP = __ffs(P) - 1;
+  int Ns = 1;
+  while (Ns < n) {
+    int rem = n / Ns;
+    if (rem % 8 == 0) {
+      fft_stage<8, DIR, LAY, MAXV, TPB>(s, n, P, Ns, __ffs(n / 8) - 1, lP, tw);
+      Ns *= 8;
+    } else if (rem % 4 == 0) {
+      fft_stage<4, DIR, LAY, MAXV, TPB>(s, n, P, Ns, __ffs(n / 4) - 1, lP, tw);
+      Ns *= 4;
+    } else {
+      fft_stage<2, DIR, LAY, MAXV, TPB>(s, n, P, Ns, __ffs(n / 2) - 1, lP, tw);
+      Ns *= 2;
+    }
+  }
+}
+
+static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// pencils per workgroup for length n: a power of two with P*n <= cap
+static int fft_pencils(int n, int cap = FFT_CAP) {
+  int P = 1;
+  while (P * 2 * n <= cap && P * 2 <= 256) P *= 2;
+  return P;
+}
+
+// ------------------------------------------------------------ z passes --
+// In-place r2c of padded rows; PRE applies (x - 1) * coef on load
+// (gravity.py:52-54).
+template <int PRE, bool POW2>
+__global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_k(float* __restrict__ data, long long nrows, int n2,
+                                                   int rs, int P, float coef,
+                                                   const float2* __restrict__ tw) {
+  extern __shared__ float2 s[];
+  long long row0 = (long long)blockIdx.x * P;
+  int np = (int)((nrows - row0) < P ? (nrows - row0) : P);
+  for (int idx = threadIdx.x; idx < np * n2; idx += FFT_TPB) {
+    int p = idx / n2, j = idx - (idx / n2) * n2;
+    float x = data[(row0 + p) * rs + j];
+    if (PRE) {
+      x = x - 1.f;
+      x = x * coef;
+    }
+    s[p * n2 + j] = make_float2(x, 0.f);
+  }
+  __syncthreads();
+  lds_fft<-1, ROWS, POW2>(s, n2, np, tw);
+  const int n2h = n2 / 2 + 1;
+  float2* out = reinterpret_cast<float2*>(data);
+  const int rsc = rs / 2;
+  for (int idx = threadIdx.x; idx < np * n2h; idx += FFT_TPB) {
+    int p = idx / n2h, k = idx - (idx / n2h) * n2h;
+    out[(row0 + p) * rsc + k] = s[p * n2 + k];
+  }
+}
+
+// In-place c2r of padded rows with a final scale.  The full spectrum is
+// rebuilt from the half (X[n-k] = conj X[k]) and the real part of the
+// inverse is kept, which discards Im X[0] and Im X[n/2] as irfft does.
+template <bool POW2>
+__global__ void __launch_bounds__(FFT_TPB, 4) z_c2r_k(float* __restrict__ data, long long nrows, int n2,
+                                                   int rs, int P, float scale,
+                                                   const float2* __restrict__ tw) {
+  extern __shared__ float2 s[];
+  long long row0 = (long long)blockIdx.x * P;
+  int np = (int)((nrows - row0) < P ? (nrows - row0) : P);
+  const float2* in = reinterpret_cast<const float2*>(data);
+  const int rsc = rs / 2;
+  const int half = n2 / 2;
+  for (int idx = threadIdx.x; idx < np * n2; idx += FFT_TPB) {
+    int p = idx / n2, k = idx - (idx / n2) * n2;
+    int kk = k <= half ? k : n2 - k;
+    float2 v = in[(row0 + p) * rsc + kk];
+    if (k > half) v.y = -v.y;
+    s[p * n2 + k] = v;
+  }
+  __syncthreads();
+  lds_fft<1, ROWS, POW2>(s, n2, np, tw);
+  for (int idx = threadIdx.x; idx < np * n2; idx += FFT_TPB) {
+    int p = idx / n2, j = idx - (idx / n2) * n2;
+    data[(row0 + p) * rs + j] = s[p * n2 + j].x * scale;
+  }
+}
+
+// ------------------------------------------------------ strided passes --
+struct Fields3 {
+  float2* f[3];
+};
+
+// In-place FFT along a strided axis of a complex (outer, n, ncol) view:
+// element j of pencil (o, c) is at o*ostride + j*es + c.  A workgroup takes
+// B consecutive columns, so each of its loads is B contiguous float2.
+template <int DIR, bool POW2>
+__global__ void __launch_bounds__(FFT_TPB, 4) strided_fft_k(Fields3 fl, int n, long long es,
+                                                         long long ostride, int ncol, int B,
+                                                         const float2* __restrict__ tw) {
+  extern __shared__ float2 s[];
+  float2* base = fl.f[blockIdx.z] + (long long)blockIdx.y * ostride;
+  int c0 = blockIdx.x * B;
+  int nb = ncol - c0 < B ? ncol - c0 : B;
+  for (int idx = threadIdx.x; idx < n * B; idx += FFT_TPB) {
+    int j = idx / B, b = idx - (idx / B) * B;
+    if (b < nb) s[j * B + b] = base[j * es + c0 + b];
+  }
+  __syncthreads();
+  lds_fft<DIR, COLS, POW2>(s, n, B, tw);
+  for (int idx = threadIdx.x; idx < n * B; idx += FFT_TPB) {
+    int j = idx / B, b = idx - (idx / B) * B;
+    if (b < nb) base[j * es + c0 + b] = s[j * B + b];
+  }
+}
+
+struct KTab {
+  const float* k[3];
+  const float* km[3];
+};
+
+// x pass of the Poisson solve (gravity.py:56-66): forward FFT along x of
+// c0, pot = -src/k^2 (k=0 -> 0), then for d = 0,1,2: g_d = (-i km_d) pot,
+// inverse FFT along x, stored to c_d.  Pencils along axis 0 of (n0,n1,n2h);
+// blockIdx.y = y index, blockIdx.x = block of B kz columns.
+template <bool POW2>
+__global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_k(Fields3 fl, int n0, int n1, int n2h, int B,
+                                                       KTab kt, const float2* __restrict__ tw) {
+  extern __shared__ float2 s[];
+  const long long es = (long long)n1 * n2h;
+  const int y = blockIdx.y;
+  const int c0 = blockIdx.x * B;
+  const int nb = n2h - c0 < B ? n2h - c0 : B;
+  const long long off = (long long)y * n2h + c0;
+  const int total = n0 * B;
+  for (int idx = threadIdx.x; idx < total; idx += FFT_TPB) {
+    int j = idx / B, b = idx - (idx / B) * B;
+    if (b < nb) s[j * B + b] = fl.f[0][off + j * es + b];
+  }
+  __syncthreads();
+  lds_fft<-1, COLS, POW2, PFFT_MAXV>(s, n0, B, tw);
+  const float ky = kt.k[1][y];
+  const float kym = kt.km[1][y];
+  float2 pot[PFFT_MAXV];
+#pragma unroll
+  for (int it = 0; it < PFFT_MAXV; ++it) {
+    int idx = threadIdx.x + it * FFT_TPB;
+    pot[it] = make_float2(0.f, 0.f);
+    if (idx < total) {
+      int j = idx / B, b = idx - (idx / B) * B;
+      if (b < nb) {
+        float2 v = s[j * B + b];
+        float kx = kt.k[0][j], kz = kt.k[2][c0 + b];
+        float k2 = kx * kx;
+        k2 = k2 + ky * ky;
+        k2 = k2 + kz * kz;
+        if (k2 != 0.f) pot[it] = make_float2((-v.x) / k2, (-v.y) / k2);
+      }
+    }
+  }
+  __syncthreads();
+  for (int d = 0; d < 3; ++d) {
+#pragma unroll
+    for (int it = 0; it < PFFT_MAXV; ++it) {
+      int idx = threadIdx.x + it * FFT_TPB;
+      if (idx < total) {
+        int j = idx / B, b = idx - (idx / B) * B;
+        float km = 0.f;
+        if (b < nb) km = d == 0 ? kt.km[0][j] : (d == 1 ? kym : kt.km[2][c0 + b]);
+        s[j * B + b] = make_float2(km * pot[it].y, -(km * pot[it].x));
+      }
+    }
+    __syncthreads();
+    lds_fft<1, COLS, POW2, PFFT_MAXV>(s, n0, B, tw);
+    float2* out = fl.f[d];
+    for (int idx = threadIdx.x; idx < total; idx += FFT_TPB) {
+      int j = idx / B, b = idx - (idx / B) * B;
+      if (b < nb) out[off + j * es + b] = s[j * B + b];
+    }
+    __syncthreads();
+  }
+}
+
+// Adjoint x pass: acc = sum_d (-i km_d) laplace(FFT_x c_d); inverse FFT
+// along x; stored to c0.  Transpose of the three force components.
+template <bool POW2>
+__global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_adj_k(Fields3 fl, int n0, int n1, int n2h, int B,
+                                                           KTab kt, const float2* __restrict__ tw) {
+  extern __shared__ float2 s[];
+  const long long es = (long long)n1 * n2h;
+  const int y = blockIdx.y;
+  const int c0 = blockIdx.x * B;
+  const int nb = n2h - c0 < B ? n2h - c0 : B;
+  const long long off = (long long)y * n2h + c0;
+  const int total = n0 * B;
+  const float ky = kt.k[1][y];
+  const float kym = kt.km[1][y];
+  float2 acc[PFFT_MAXV];
+#pragma unroll
+  for (int it = 0; it < PFFT_MAXV; ++it) acc[it] = make_float2(0.f, 0.f);
+  for (int d = 0; d < 3; ++d) {
+    for (int idx = threadIdx.x; idx < total; idx += FFT_TPB) {
+      int j = idx / B, b = idx - (idx / B) * B;
+      if (b < nb) s[j * B + b] = fl.f[d][off + j * es + b];
+    }
+    __syncthreads();
+    lds_fft<-1, COLS, POW2, PFFT_MAXV>(s, n0, B, tw);
+#pragma unroll
+    for (int it = 0; it < PFFT_MAXV; ++it) {
+      int idx = threadIdx.x + it * FFT_TPB;
+      if (idx < total) {
+        int j = idx / B, b = idx - (idx / B) * B;
+        if (b < nb) {
+          float2 v = s[j * B + b];
+          float kx = kt.k[0][j], kz = kt.k[2][c0 + b];
+          float k2 = kx * kx;
+          k2 = k2 + ky * ky;
+          k2 = k2 + kz * kz;
+          float2 pot = k2 != 0.f ? make_float2((-v.x) / k2, (-v.y) / k2) : make_float2(0.f, 0.f);
+          float km = d == 0 ? kt.km[0][j] : (d == 1 ? kym : kt.km[2][c0 + b]);
+          float2 g = make_float2(km * pot.y, -(km * pot.x));
+          acc[it] = cadd(acc[it], g);
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int it = 0; it < PFFT_MAXV; ++it) {
+    int idx = threadIdx.x + it * FFT_TPB;
+    if (idx < total) s[(idx / B) * B + (idx - (idx / B) * B)] = acc[it];
+  }
+  __syncthreads();
+  lds_fft<1, COLS, POW2, PFFT_MAXV>(s, n0, B, tw);
+  for (int idx = threadIdx.x; idx < total; idx += FFT_TPB) {
+    int j = idx / B, b = idx - (idx / B) * B;
+    if (b < nb) fl.f[0][off + j * es + b] = s[j * B + b];
+  }
+}
+
+// ------------------------------------------------------------- copies --
+// dst rows of dst_rs floats <- src rows of src_rs floats, n2 valid columns
+__global__ void copy_rows_k(float* __restrict__ dst, int dst_rs, const float* __restrict__ src,
+                            int src_rs, long long nrows, int n2) {
+  long long total = nrows * n2;
+  for (long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x; i < total;
+       i += (long long)gridDim.x * PMX_TPB) {
+    long long r = i / n2;
+    int j = (int)(i - r * n2);
+    dst[r * dst_rs + j] = src[r * src_rs + j];
+  }
+}
+
+// ------------------------------------------------------------ drivers --
+static size_t lds_bytes(int n, int P) { return (size_t)P * n * sizeof(float2); }
+
+int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef) {
+  long long nrows = (long long)c->n[0] * c->n[1];
+  int n2 = c->n[2];
+  int P = fft_pencils(n2);
+  unsigned nblk = (unsigned)((nrows + P - 1) / P);
+  size_t lds = lds_bytes(n2, P);
+  bool p2 = is_pow2(n2);
+  pmx_tic(c, PMX_T_ZR2C);
+  if (pre && p2)
+    z_r2c_k<1, true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
+  else if (pre)
+    z_r2c_k<1, false><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
+  else if (p2)
+    z_r2c_k<0, true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
+  else
+    z_r2c_k<0, false><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
+  pmx_toc(c, PMX_T_ZR2C);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+int pmx_fft_c2r_inplace(pmx_ctx* c, float* buf, float scale) {
+  long long nrows = (long long)c->n[0] * c->n[1];
+  int n2 = c->n[2];
+  int P = fft_pencils(n2);
+  unsigned nblk = (unsigned)((nrows + P - 1) / P);
+  size_t lds = lds_bytes(n2, P);
+  pmx_tic(c, PMX_T_ZC2R);
+  if (is_pow2(n2))
+    z_c2r_k<true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2]);
+  else
+    z_c2r_k<false><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2]);
+  pmx_toc(c, PMX_T_ZC2R);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+// axis 1 (y) or axis 0 (x) complex FFT of nf fields in place
+int pmx_fft_strided(pmx_ctx* c, float2* const* f, int nf, int axis, int dir) {
+  Fields3 fl;
+  for (int i = 0; i < 3; ++i) fl.f[i] = f[i < nf ? i : 0];
+  int n = c->n[axis];
+  int B = fft_pencils(n);
+  int ncol = c->n2h;
+  long long es, ostride;
+  unsigned nouter;
+  if (axis == 1) {
+    es = c->n2h;
+    ostride = (long long)c->n[1] * c->n2h;
+    nouter = c->n[0];
+  } else {
+    es = (long long)c->n[1] * c->n2h;
+    ostride = c->n2h;
+    nouter = c->n[1];
+  }
+  dim3 grid((ncol + B - 1) / B, nouter, nf);
+  size_t lds = lds_bytes(n, B);
+  bool p2 = is_pow2(n);
+  int cls = axis == 1 ? (dir < 0 ? PMX_T_YFWD : PMX_T_YINV) : PMX_T_OTHER;
+  pmx_tic(c, cls);
+  if (dir < 0 && p2)
+    strided_fft_k<-1, true><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, c->tw[axis]);
+  else if (dir < 0)
+    strided_fft_k<-1, false><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, c->tw[axis]);
+  else if (p2)
+    strided_fft_k<1, true><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, c->tw[axis]);
+  else
+    strided_fft_k<1, false><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, c->tw[axis]);
+  pmx_toc(c, cls);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+static KTab ktab(const pmx_ctx* c) {
+  KTab kt;
+  for (int a = 0; a < 3; ++a) {
+    kt.k[a] = c->k[a];
+    kt.km[a] = c->km[a];
+  }
+  return kt;
+}
+
+int pmx_poisson_x(pmx_ctx* c, float2* const* f, int adjoint) {
+  Fields3 fl = {{f[0], f[1], f[2]}};
+  int n0 = c->n[0];
+  int B = fft_pencils(n0, PFFT_CAP);
+  dim3 grid((c->n2h + B - 1) / B, c->n[1], 1);
+  size_t lds = lds_bytes(n0, B);
+  bool p2 = is_pow2(n0);
+  pmx_tic(c, PMX_T_POISSON);
+  if (adjoint && p2)
+    poisson_x_adj_k<true><<<grid, FFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, B, ktab(c), c->tw[0]);
+  else if (adjoint)
+    poisson_x_adj_k<false><<<grid, FFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, B, ktab(c), c->tw[0]);
+  else if (p2)
+    poisson_x_k<true><<<grid, FFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, B, ktab(c), c->tw[0]);
+  else
+    poisson_x_k<false><<<grid, FFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, B, ktab(c), c->tw[0]);
+  pmx_toc(c, PMX_T_POISSON);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+// Full forward Poisson on the context buffers: cbuf[0] holds the density in
+// the padded real layout; on return cbuf[0..2] hold the three force meshes
+// (padded real layout), scaled 1/M.
+int pmx_poisson_ctx(pmx_ctx* c, float coef) {
+  int rc;
+  float2* f[3] = {c->cbuf[0], c->cbuf[1], c->cbuf[2]};
+  if ((rc = pmx_fft_r2c_inplace(c, (float*)f[0], 1, coef))) return rc;
+  if ((rc = pmx_fft_strided(c, f, 1, 1, -1))) return rc;
+  if ((rc = pmx_poisson_x(c, f, 0))) return rc;
+  if ((rc = pmx_fft_strided(c, f, 3, 1, 1))) return rc;
+  float scale = (float)(1.0 / (double)c->M);
+  for (int d = 0; d < 3; ++d)
+    if ((rc = pmx_fft_c2r_inplace(c, (float*)f[d], scale))) return rc;
+  return PMX_OK;
+}
+
+// Adjoint Poisson on the context buffers: cbuf[0..2] hold the three
+// cotangent meshes (padded real); on return cbuf[0] holds the density
+// cotangent (padded real) times `scale_out`.
+int pmx_poisson_adj_ctx(pmx_ctx* c, float scale_out) {
+  int rc;
+  float2* f[3] = {c->cbuf[0], c->cbuf[1], c->cbuf[2]};
+  for (int d = 0; d < 3; ++d)
+    if ((rc = pmx_fft_r2c_inplace(c, (float*)f[d], 0, 0.f))) return rc;
+  if ((rc = pmx_fft_strided(c, f, 3, 1, -1))) return rc;
+  if ((rc = pmx_poisson_x(c, f, 1))) return rc;
+  if ((rc = pmx_fft_strided(c, f, 1, 1, 1))) return rc;
+  if ((rc = pmx_fft_c2r_inplace(c, (float*)f[0], scale_out))) return rc;
+  return PMX_OK;
+}
+
+int pmx_copy_rows(pmx_ctx* c, float* dst, int dst_rs, const float* src, int src_rs) {
+  long long nrows = (long long)c->n[0] * c->n[1];
+  long long total = nrows * c->n[2];
+  unsigned nblk = (unsigned)((total + PMX_TPB - 1) / PMX_TPB);
+  if (nblk > 65536u * 4) nblk = 65536u * 4;
+  copy_rows_k<<<nblk, PMX_TPB, 0, c->stream>>>(dst, dst_rs, src, src_rs, nrows, c->n[2]);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+extern "C" int pmx_rfftn(pmx_ctx* c, const float* in, float* out_c) {
+  PMX_REQUIRE(c && in && out_c, "pmx_rfftn: NULL argument");
+  int rc;
+  float2* f[3] = {c->cbuf[0], c->cbuf[1], c->cbuf[2]};
+  if ((rc = pmx_copy_rows(c, (float*)f[0], c->rs, in, c->n[2]))) return rc;
+  if ((rc = pmx_fft_r2c_inplace(c, (float*)f[0], 0, 0.f))) return rc;
+  if ((rc = pmx_fft_strided(c, f, 1, 1, -1))) return rc;
+  if ((rc = pmx_fft_strided(c, f, 1, 0, -1))) return rc;
+  PMX_CHECK_HIP(hipMemcpyAsync(out_c, f[0], (size_t)c->Mc * sizeof(float2),
+                               hipMemcpyDeviceToDevice, c->stream));
+  return PMX_OK;
+}
+
+extern "C" int pmx_irfftn(pmx_ctx* c, const float* in_c, float* out) {
+  PMX_REQUIRE(c && in_c && out, "pmx_irfftn: NULL argument");
+  int rc;
+  float2* f[3] = {c->cbuf[0], c->cbuf[1], c->cbuf[2]};
+  PMX_CHECK_HIP(hipMemcpyAsync(f[0], in_c, (size_t)c->Mc * sizeof(float2),
+                               hipMemcpyDeviceToDevice, c->stream));
+  if ((rc = pmx_fft_strided(c, f, 1, 0, 1))) return rc;
+  if ((rc = pmx_fft_strided(c, f, 1, 1, 1))) return rc;
+  if ((rc = pmx_fft_c2r_inplace(c, (float*)f[0], (float)(1.0 / (double)c->M)))) return rc;
+  return pmx_copy_rows(c, out, c->n[2], (const float*)f[0], c->rs);
+}
+
+extern "C" int pmx_poisson_grad(pmx_ctx* c, const float* dens, float coef, float* grad) {
+  PMX_REQUIRE(c && dens && grad, "pmx_poisson_grad: NULL argument");
+  int rc;
+  if ((rc = pmx_copy_rows(c, (float*)c->cbuf[0], c->rs, dens, c->n[2]))) return rc;
+  if ((rc = pmx_poisson_ctx(c, coef))) return rc;
+  for (int d = 0; d < 3; ++d)
+    if ((rc = pmx_copy_rows(c, grad + (long long)d * c->M, c->n[2], (const float*)c->cbuf[d], c->rs)))
+      return rc;
+  return PMX_OK;
+}

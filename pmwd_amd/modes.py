@@ -1,0 +1,75 @@
+"""White noise and linear modes (pmwd/modes.py:12-117).
+
+``white_noise`` draws float32 normals from ``numpy.random.default_rng(seed)``
+(pmwd's jax.random threefry stream is not reproducible without jax) and
+moves them to the GPU; Fourier modes use the HIP r2c with 'ortho' norm.
+``linear_modes`` multiplies by sqrt(V Plin(|k|)) with |k| formed in float32
+from the float32 wavevectors and Plin evaluated in float64, as pmwd.
+"""
+import math
+
+import numpy as np
+import torch
+
+from .boltzmann import growth
+from .particles import default_device
+from .pm_util import fftfreq, fftfwd, fftinv
+
+
+def white_noise(seed, conf, real=False, unit_abs=False, device=None):
+    """White-noise real or Fourier modes, zero mean, unit variance."""
+    device = torch.device(device) if device is not None else default_device()
+    w = np.random.default_rng(seed).standard_normal(conf.ptcl_grid_shape, dtype=np.float32)
+    modes = torch.as_tensor(w, device=device)
+    if real and not unit_abs:
+        return modes
+    modes = fftfwd(modes, norm='ortho')
+    if unit_abs:
+        modes = modes / torch.abs(modes)
+    if real:
+        modes = fftinv(modes, shape=conf.ptcl_grid_shape, norm='ortho')
+    return modes
+
+
+def _interp(x, xp, fp):
+    """numpy.interp on device tensors (float64), clamped at the ends."""
+    i = torch.searchsorted(xp, x, right=True).clamp(1, len(xp) - 1)
+    x0, x1 = xp[i - 1], xp[i]
+    f0, f1 = fp[i - 1], fp[i]
+    t = (x - x0) / (x1 - x0)
+    out = f0 + t * (f1 - f0)
+    out = torch.where(x <= xp[0], fp[0], out)
+    return torch.where(x >= xp[-1], fp[-1], out)
+
+
+def linear_power_k(k, a, cosmo, conf):
+    """linear_power (boltzmann.py:399-455) on a float64 device tensor of k."""
+    dev = k.device
+    xp = torch.as_tensor(conf.transfer_k, dtype=torch.float64, device=dev)
+    fp = torch.as_tensor(cosmo.transfer, dtype=torch.float64, device=dev)
+    T = _interp(k, xp, fp)
+    P = (0.32 * cosmo.A_s * cosmo.k_pivot * (k / cosmo.k_pivot)**cosmo.n_s
+         * (math.pi * (conf.c / conf.H_0)**2 / cosmo.Omega_m * T)**2)
+    if a is not None:
+        P = P * float(growth(a, cosmo, conf))**2
+    return P
+
+
+def linear_modes(modes, cosmo, conf, a=None, real=False):
+    """sqrt(V Plin(k)) * white-noise modes (modes.py:67-117)."""
+    if cosmo.transfer is None:
+        raise ValueError('Transfer table is empty. Call transfer_integ or boltzmann first.')
+    if not torch.is_complex(modes):
+        modes = fftfwd(modes.to(conf.float_dtype), norm='ortho')
+    kvec = fftfreq(conf.ptcl_grid_shape, conf.ptcl_spacing, dtype=conf.float_dtype,
+                   device=modes.device)
+    k2 = 0
+    for k in kvec:
+        k2 = k2 + k * k
+    k = torch.sqrt(k2).to(torch.float64)
+    P = linear_power_k(k, a, cosmo, conf)
+    fac = torch.sqrt(P * conf.box_vol).to(conf.float_dtype)
+    modes = modes * fac
+    if real:
+        modes = fftinv(modes, shape=conf.ptcl_grid_shape, norm=conf.ptcl_spacing)
+    return modes

@@ -1,0 +1,292 @@
+"""N-body KDK integration and its reverse-time adjoint (pmwd/nbody.py).
+
+Time-step factors are computed on the host in FP64 from the growth table
+(nbody.py:12-36) and rounded to float32 at use, as pmwd does; the particle
+state never leaves the GPU.  With the default ``symp_splits`` (KDK) each step
+is one ``pmx_nbody_step`` call (fused kick+drift+bin, scatter, Poisson,
+fused gather+kick); other splittings run the ops one by one.
+
+The adjoint reconstructs the trajectory backward in time from the final
+state as pmwd's ``nbody_adj`` (nbody.py:226-260) does, optionally snapping
+the reconstruction to sparse forward checkpoints (every ``checkpoint_every``
+steps) to bound the float32 reversal drift.  The cosmology cotangent is for
+Omega_m: time-step factor sensitivities from the growth ODE's forward
+sensitivity, and the force term sum(vel_cot*acc)/Omega_m.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .boltzmann import growth, growth_dOm
+from .cosmology import E2, H_deriv, E2_dOm, H_deriv_dOm
+from .gravity import omega_coef
+
+
+# ------------------------------------------------------ FP64 factors ---
+
+def _G_D(a, cosmo, conf):
+    """Growth factor of ZA canonical velocity in [H_0] (nbody.py:12-14)."""
+    return a**2 * np.sqrt(E2(a, cosmo)) * growth(a, cosmo, conf, deriv=1)
+
+
+def _G_K(a, cosmo, conf):
+    """Growth factor of ZA accelerations in [H_0^2] (nbody.py:17-22)."""
+    return a**3 * E2(a, cosmo) * (growth(a, cosmo, conf, deriv=2)
+                                  + (2 + H_deriv(a, cosmo)) * growth(a, cosmo, conf, deriv=1))
+
+
+def _G_D_dOm(a, cosmo, conf):
+    e2 = E2(a, cosmo)
+    d1 = growth(a, cosmo, conf, deriv=1)
+    return a**2 * (0.5 / np.sqrt(e2) * E2_dOm(a, cosmo) * d1
+                   + np.sqrt(e2) * growth_dOm(a, cosmo, conf, deriv=1))
+
+
+def _G_K_dOm(a, cosmo, conf):
+    e2 = E2(a, cosmo)
+    h = H_deriv(a, cosmo)
+    d1 = growth(a, cosmo, conf, deriv=1)
+    d2 = growth(a, cosmo, conf, deriv=2)
+    dd1 = growth_dOm(a, cosmo, conf, deriv=1)
+    dd2 = growth_dOm(a, cosmo, conf, deriv=2)
+    return a**3 * (E2_dOm(a, cosmo) * (d2 + (2 + h) * d1)
+                   + e2 * (dd2 + H_deriv_dOm(a, cosmo) * d1 + (2 + h) * dd1))
+
+
+def drift_factor(a_vel, a_prev, a_next, cosmo, conf):
+    """Drift time-step factor in [1/H_0] (nbody.py:25-29)."""
+    return (growth(a_next, cosmo, conf) - growth(a_prev, cosmo, conf)) / _G_D(a_vel, cosmo, conf)
+
+
+def kick_factor(a_acc, a_prev, a_next, cosmo, conf):
+    """Kick time-step factor in [1/H_0] (nbody.py:32-36)."""
+    return (_G_D(a_next, cosmo, conf) - _G_D(a_prev, cosmo, conf)) / _G_K(a_acc, cosmo, conf)
+
+
+def drift_factor_dOm(a_vel, a_prev, a_next, cosmo, conf):
+    f = drift_factor(a_vel, a_prev, a_next, cosmo, conf)
+    gd = _G_D(a_vel, cosmo, conf)
+    num = growth_dOm(a_next, cosmo, conf) - growth_dOm(a_prev, cosmo, conf)
+    return num / gd - f * _G_D_dOm(a_vel, cosmo, conf) / gd
+
+
+def kick_factor_dOm(a_acc, a_prev, a_next, cosmo, conf):
+    f = kick_factor(a_acc, a_prev, a_next, cosmo, conf)
+    gk = _G_K(a_acc, cosmo, conf)
+    num = _G_D_dOm(a_next, cosmo, conf) - _G_D_dOm(a_prev, cosmo, conf)
+    return num / gk - f * _G_K_dOm(a_acc, cosmo, conf) / gk
+
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+def _is_kdk(conf):
+    return tuple(tuple(s) for s in conf.symp_splits) == ((0, 0.5), (1, 0.5))
+
+
+def step_ops(a_prev, a_next, conf):
+    """Op list of one step of ``integrate`` (nbody.py:121-140)."""
+    ops = []
+    D = K = 0
+    a_disp = a_vel = a_acc = a_prev
+    for d, k in conf.symp_splits:
+        if d != 0:
+            D += d
+            a_disp_next = a_prev * (1 - D) + a_next * D
+            ops.append(('drift', a_vel, a_disp, a_disp_next))
+            a_disp = a_disp_next
+            ops.append(('force', a_disp))
+            a_acc = a_disp
+        if k != 0:
+            K += k
+            a_vel_next = a_prev * (1 - K) + a_next * K
+            ops.append(('kick', a_acc, a_vel, a_vel_next))
+            a_vel = a_vel_next
+    return ops
+
+
+# ------------------------------------------------------- forward loop ---
+
+def _mass_val(conf):
+    return float(np.float32(conf.mesh_size / conf.ptcl_num))
+
+
+def _force(c, pmid, disp, acc, n, val, coef):
+    _lib.check(_lib.lib().pmx_gravity(c.h, _lib.ptr(pmid), _lib.ptr(disp), n, val, coef,
+                                      _lib.ptr(acc)))
+
+
+def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=None):
+    """Integrates in place; returns acc and the checkpoints {step: (disp, vel)}."""
+    L = _lib.lib()
+    n = pmid.shape[0]
+    c = _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
+    h = c.bind()
+    val, coef = _mass_val(conf), omega_coef(cosmo, conf)
+    acc = torch.empty_like(disp)
+    a_nbody = conf.a_nbody[::-1] if reverse else conf.a_nbody
+    _force(c, pmid, disp, acc, n, val, coef)
+    ckpt = {}
+    kdk = _is_kdk(conf)
+    for i, (a_prev, a_next) in enumerate(zip(a_nbody[:-1], a_nbody[1:])):
+        if checkpoint_every and i % checkpoint_every == 0 and i > 0:
+            ckpt[i] = (disp.clone(), vel.clone())
+        if kdk:
+            a_half = a_prev * (1 - 0.5) + a_next * 0.5
+            k1 = _f32(kick_factor(a_prev, a_prev, a_half, cosmo, conf))
+            d = _f32(drift_factor(a_half, a_prev, a_next, cosmo, conf))
+            k2 = _f32(kick_factor(a_next, a_half, a_next, cosmo, conf))
+            _lib.check(L.pmx_nbody_step(h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel),
+                                        _lib.ptr(acc), n, val, coef, k1, d, k2))
+        else:
+            for op in step_ops(a_prev, a_next, conf):
+                if op[0] == 'drift':
+                    f = _f32(drift_factor(*op[1:], cosmo, conf))
+                    _lib.check(L.pmx_drift(h, n, _lib.ptr(disp), _lib.ptr(vel), f))
+                elif op[0] == 'kick':
+                    f = _f32(kick_factor(*op[1:], cosmo, conf))
+                    _lib.check(L.pmx_kick(h, n, _lib.ptr(vel), _lib.ptr(acc), f))
+                else:
+                    _force(c, pmid, disp, acc, n, val, coef)
+    return acc, ckpt
+
+
+# ------------------------------------------------------------ adjoint ---
+
+def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False, ckpt=None):
+    """Reverse-time adjoint in place (nbody.py:143-162, 226-260).
+
+    On return disp/vel hold the reconstructed initial state and
+    disp_cot/vel_cot the initial-state cotangents; returns (acc, acc_cot,
+    Omega_m cotangent).
+    """
+    L = _lib.lib()
+    n = pmid.shape[0]
+    c = _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
+    h = c.bind()
+    val, coef = _mass_val(conf), omega_coef(cosmo, conf)
+    Om = cosmo.Omega_m
+    acc = torch.empty_like(disp)
+    acc_cot = torch.empty_like(disp)
+    dot = ctypes.c_double(0.0)
+
+    def force_adj():
+        _lib.check(L.pmx_gravity_vjp(h, _lib.ptr(pmid), _lib.ptr(disp), n, val, coef,
+                                     _lib.ptr(vel_cot), _lib.ptr(acc), _lib.ptr(acc_cot),
+                                     ctypes.byref(dot)))
+        return dot.value / Om
+
+    a_nbody = conf.a_nbody[::-1] if reverse else conf.a_nbody
+    nsteps = len(a_nbody) - 1
+    cf = force_adj()
+    om_cot = 0.0
+    for s, (a_prev, a_next) in enumerate(zip(a_nbody[:0:-1], a_nbody[-2::-1])):
+        step = nsteps - 1 - s  # index of the forward step being undone
+        D = K = 0
+        a_disp = a_vel = a_acc = a_prev
+        for d, k in reversed(conf.symp_splits):
+            if k != 0:
+                K += k
+                a_vel_next = a_prev * (1 - K) + a_next * K
+                args = (a_acc, a_vel, a_vel_next)
+                f = _f32(kick_factor(*args, cosmo, conf))
+                dfdO = kick_factor_dOm(*args, cosmo, conf)
+                _lib.check(L.pmx_kick_adj(h, n, _lib.ptr(vel), _lib.ptr(acc), _lib.ptr(disp_cot),
+                                          _lib.ptr(acc_cot), _lib.ptr(vel_cot), f,
+                                          ctypes.byref(dot)))
+                om_cot -= dfdO * dot.value + cf * f
+                a_vel = a_vel_next
+            if d != 0:
+                D += d
+                a_disp_next = a_prev * (1 - D) + a_next * D
+                args = (a_vel, a_disp, a_disp_next)
+                f = _f32(drift_factor(*args, cosmo, conf))
+                dfdO = drift_factor_dOm(*args, cosmo, conf)
+                _lib.check(L.pmx_drift_adj(h, n, _lib.ptr(disp), _lib.ptr(vel),
+                                           _lib.ptr(disp_cot), _lib.ptr(vel_cot), f,
+                                           ctypes.byref(dot)))
+                om_cot -= dfdO * dot.value
+                a_disp = a_disp_next
+                if ckpt and step in ckpt and D == 1:
+                    disp.copy_(ckpt[step][0])
+                cf = force_adj()
+                a_acc = a_disp
+        if ckpt and step in ckpt:
+            vel.copy_(ckpt[step][1])
+    return acc, acc_cot, om_cot
+
+
+class _NBodyFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, disp, vel, om_t, pmid, cosmo, conf, reverse, checkpoint_every):
+        disp_f = disp.detach().clone().contiguous()
+        vel_f = vel.detach().clone().contiguous()
+        acc, ckpt = _run_forward(pmid, disp_f, vel_f, cosmo, conf, reverse, checkpoint_every)
+        ctx.save_for_backward(disp_f, vel_f)
+        ctx.pmid, ctx.cosmo, ctx.conf, ctx.reverse, ctx.ckpt = pmid, cosmo, conf, reverse, ckpt
+        ctx.mark_non_differentiable(acc)
+        return disp_f, vel_f, acc
+
+    @staticmethod
+    def backward(ctx, disp_cot, vel_cot, acc_cot):
+        disp_f, vel_f = ctx.saved_tensors
+        disp = disp_f.clone()
+        vel = vel_f.clone()
+        dc = (disp_cot if disp_cot is not None else torch.zeros_like(disp)).clone().contiguous()
+        vc = (vel_cot if vel_cot is not None else torch.zeros_like(vel)).clone().contiguous()
+        _, _, om_cot = _run_adjoint(ctx.pmid, disp, vel, dc, vc, ctx.cosmo, ctx.conf,
+                                    ctx.reverse, ctx.ckpt)
+        om_grad = None
+        if ctx.needs_input_grad[2]:
+            om_grad = torch.tensor(om_cot, dtype=torch.float64)
+        return dc, vc, om_grad, None, None, None, None, None
+
+
+def nbody(ptcl, obsvbl, cosmo, conf, reverse=False, checkpoint_every=None, omega_m=None):
+    """N-body time integration (nbody.py:215-223).
+
+    Returns ``(ptcl, obsvbl)`` with the final ``disp``, ``vel`` and ``acc``.
+    Differentiable w.r.t. ``ptcl.disp``/``ptcl.vel`` (reverse-time adjoint)
+    and, when ``omega_m`` is a 0-d float64 tensor equal to
+    ``cosmo.Omega_m`` that requires grad, w.r.t. Omega_m.
+    """
+    if cosmo.growth is None:
+        raise ValueError('Growth table is empty. Call growth_integ or boltzmann first.')
+    _lib.require_device(ptcl.disp, 'ptcl.disp')
+    vel = ptcl.vel if ptcl.vel is not None else torch.zeros_like(ptcl.disp)
+    pmid = ptcl.pmid.contiguous()
+    need_grad = torch.is_grad_enabled() and (ptcl.disp.requires_grad or vel.requires_grad or
+                                             (omega_m is not None and omega_m.requires_grad))
+    if need_grad:
+        om_t = omega_m if omega_m is not None else torch.tensor(cosmo.Omega_m, dtype=torch.float64)
+        disp, vel, acc = _NBodyFn.apply(ptcl.disp, vel, om_t, pmid, cosmo, conf, reverse,
+                                        checkpoint_every)
+    else:
+        disp = ptcl.disp.detach().clone().contiguous()
+        vel = vel.detach().clone().contiguous()
+        acc, _ = _run_forward(pmid, disp, vel, cosmo, conf, reverse, None)
+    return ptcl.replace(disp=disp, vel=vel, acc=acc), obsvbl
+
+
+def nbody_adj(ptcl, ptcl_cot, cosmo, conf, reverse=False):
+    """Functional reverse-time adjoint from a final state (nbody.py:251-260).
+
+    ``ptcl`` is the final state, ``ptcl_cot`` its cotangent (disp, vel).
+    Returns ``(ptcl0, ptcl_cot0, cosmo_cot)``: the reconstructed initial
+    state, the initial cotangents (with ``acc`` holding the last force VJP,
+    as pmwd returns it) and a ``Cosmology`` cotangent whose Omega_m entry is
+    set (the other parameters do not enter the N-body step).
+    """
+    pmid = ptcl.pmid.contiguous()
+    disp = ptcl.disp.detach().clone().contiguous()
+    vel = ptcl.vel.detach().clone().contiguous()
+    dc = ptcl_cot.disp.detach().clone().contiguous()
+    vc = ptcl_cot.vel.detach().clone().contiguous()
+    acc, acc_cot, om_cot = _run_adjoint(pmid, disp, vel, dc, vc, cosmo, conf, reverse)
+    cosmo_cot = cosmo.replace(A_s_1e9=0.0, n_s=0.0, Omega_m=om_cot, Omega_b=0.0, h=0.0,
+                              transfer=None, growth=None, growth_dOm=None, varlin=None)
+    return (ptcl.replace(disp=disp, vel=vel, acc=acc),
+            ptcl_cot.replace(disp=dc, vel=vc, acc=acc_cot), cosmo_cot)

@@ -1,0 +1,162 @@
+"""GPU parity of gravity, the KDK integrator and the reverse-time adjoint."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import cosmo as oc, nbody as onb, pm as opm
+from oracle.conf import OConf
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def make_ptcl(shape, mesh_shape, cell, std, seed):
+    pmid, disp = opm.gen_grid(shape, mesh_shape, cell)
+    rng = np.random.default_rng(seed)
+    return pmid, (disp + std * rng.standard_normal(disp.shape)).astype(np.float32)
+
+
+@pytest.mark.parametrize('shape,ratio,std', [((16, 16, 16), 2, 1.5), ((4, 9, 8), 1, 7.0),
+                                             ((7, 8, 6), 1, 7.0), ((32, 32, 32), 1, 0.5)])
+def test_gravity(dev, shape, ratio, std):
+    import pmwd_amd as pa
+    conf = pa.Configuration(1.0, shape, mesh_shape=ratio)
+    cosmo = pa.SimpleLCDM(conf)
+    pmid, disp = make_ptcl(shape, conf.mesh_shape, conf.cell_size, std, 1)
+    ptcl = pa.Particles(conf, torch.as_tensor(pmid, device=dev), torch.as_tensor(disp, device=dev))
+    acc = pa.gravity(None, ptcl, cosmo, conf).cpu().numpy()
+    want = opm.gravity(pmid, disp, 0.3, conf.cell_size, conf.mesh_shape)
+    assert rel_l2(acc, want) < 2e-5
+
+
+@pytest.mark.parametrize('shape,ratio', [((8, 8, 8), 2), ((4, 9, 8), 1), ((7, 8, 6), 1)])
+def test_gravity_vjp(dev, shape, ratio):
+    import pmwd_amd as pa
+    from pmwd_amd import _lib
+    import ctypes
+    conf = pa.Configuration(1.0, shape, mesh_shape=ratio)
+    pmid, disp = make_ptcl(shape, conf.mesh_shape, conf.cell_size, 2.0, 2)
+    vc = np.random.default_rng(5).standard_normal(disp.shape).astype(np.float32)
+    c = _lib.context(conf.mesh_shape, conf.cell_size, dev)
+    P = lambda a: torch.as_tensor(a, device=dev)
+    acc = torch.empty_like(P(disp))
+    ac = torch.empty_like(P(disp))
+    dot = ctypes.c_double()
+    coef = float(np.float32(1.5) * np.float32(0.3))
+    val = float(np.float32(conf.mesh_size / conf.ptcl_num))
+    pm_t, d_t, v_t = P(pmid), P(disp), P(vc)
+    _lib.check(_lib.lib().pmx_gravity_vjp(c.bind(), _lib.ptr(pm_t), _lib.ptr(d_t), len(pmid), val,
+                                          coef, _lib.ptr(v_t), _lib.ptr(acc), _lib.ptr(ac),
+                                          ctypes.byref(dot)))
+    wacc, wdc, wom = opm.gravity_vjp(pmid, disp, 0.3, conf.cell_size, conf.mesh_shape, vc)
+    assert rel_l2(acc.cpu().numpy(), wacc) < 2e-5
+    assert rel_l2(ac.cpu().numpy(), wdc) < 1e-4
+    assert abs(dot.value / 0.3 - wom) < 1e-4 * abs(wom)
+
+
+def test_gravity_vjp_finite_difference(dev):
+    """<v, J dx> from central differences of the GPU gravity matches <J^T v, dx>."""
+    import pmwd_amd as pa
+    conf = pa.Configuration(1.0, (8, 8, 8), mesh_shape=2)
+    cosmo = pa.SimpleLCDM(conf)
+    pmid, disp = make_ptcl((8, 8, 8), conf.mesh_shape, conf.cell_size, 1.0, 3)
+    rng = np.random.default_rng(6)
+    v = rng.standard_normal(disp.shape)
+    dx = rng.standard_normal(disp.shape)
+    P = lambda a: torch.as_tensor(a, device=dev)
+
+    def g(x):
+        return pa.gravity(None, pa.Particles(conf, P(pmid), P(x.astype(np.float32))), cosmo,
+                          conf).cpu().numpy().astype(np.float64)
+    # small enough that few particles cross a CIC kink, large enough for f32
+    eps = 1e-3
+    fd = (np.sum(v * g(disp + eps * dx)) - np.sum(v * g(disp - eps * dx))) / (2 * eps)
+    d = P(disp).requires_grad_(True)
+    acc = pa.gravity(None, pa.Particles(conf, P(pmid), d), cosmo, conf)
+    (gd,) = torch.autograd.grad(acc, d, P(v.astype(np.float32)))
+    an = np.sum(gd.cpu().numpy() * dx)
+    assert abs(fd - an) < 2e-2 * abs(an)
+
+
+def small_setup(shape=(16, 16, 16), ratio=2, a_start=1 / 16, steps_max=1 / 16):
+    import pmwd_amd as pa
+    conf = pa.Configuration(4.0, shape, mesh_shape=ratio, a_start=a_start,
+                            a_nbody_maxstep=steps_max)
+    oconf = OConf(4.0, shape, mesh_shape=ratio, a_start=a_start, a_nbody_maxstep=steps_max)
+    cosmo = pa.boltzmann(pa.SimpleLCDM(conf), conf)
+    ocosmo = oc.boltzmann(oc.SimpleLCDM(oconf), oconf)
+    return pa, conf, oconf, cosmo, ocosmo
+
+
+def test_nbody_forward(dev):
+    pa, conf, oconf, cosmo, ocosmo = small_setup()
+    pmid, disp = make_ptcl(conf.ptcl_grid_shape, conf.mesh_shape, conf.cell_size, 0.5, 4)
+    vel = (0.05 * np.random.default_rng(7).standard_normal(disp.shape)).astype(np.float32)
+    P = lambda a: torch.as_tensor(a, device=dev)
+    ptcl = pa.Particles(conf, P(pmid), P(disp), P(vel))
+    out, _ = pa.nbody(ptcl, None, cosmo, conf)
+    wd, wv, wa = onb.nbody(pmid, disp, vel, ocosmo, oconf)
+    assert rel_l2(out.disp.cpu().numpy(), wd) < 1e-5
+    assert rel_l2(out.vel.cpu().numpy(), wv) < 1e-4
+    assert rel_l2(out.acc.cpu().numpy(), wa) < 1e-4
+
+
+def test_nbody_reversibility(dev):
+    """Forward then reverse returns to the start (adjoint.tex:1529-1549 tolerances)."""
+    pa, conf, oconf, cosmo, ocosmo = small_setup()
+    pmid, disp = make_ptcl(conf.ptcl_grid_shape, conf.mesh_shape, conf.cell_size, 0.5, 5)
+    vel = (0.05 * np.random.default_rng(8).standard_normal(disp.shape)).astype(np.float32)
+    P = lambda a: torch.as_tensor(a, device=dev)
+    ptcl = pa.Particles(conf, P(pmid), P(disp), P(vel))
+    fwd, _ = pa.nbody(ptcl, None, cosmo, conf)
+    back, _ = pa.nbody(fwd, None, cosmo, conf, reverse=True)
+    rd = np.sqrt(np.mean((back.disp.cpu().numpy() - disp)**2)) / np.std(disp - opm.gen_grid(
+        conf.ptcl_grid_shape, conf.mesh_shape, conf.cell_size)[1])
+    assert rd < 1e-3
+
+
+def test_nbody_adjoint_vs_oracle(dev):
+    pa, conf, oconf, cosmo, ocosmo = small_setup(shape=(8, 8, 8), a_start=1 / 4, steps_max=1 / 4)
+    pmid, disp = make_ptcl(conf.ptcl_grid_shape, conf.mesh_shape, conf.cell_size, 0.5, 6)
+    vel = (0.05 * np.random.default_rng(9).standard_normal(disp.shape)).astype(np.float32)
+    rng = np.random.default_rng(10)
+    dcot = rng.standard_normal(disp.shape).astype(np.float32)
+    vcot = rng.standard_normal(disp.shape).astype(np.float32)
+    P = lambda a: torch.as_tensor(a, device=dev)
+    ptcl = pa.Particles(conf, P(pmid), P(disp), P(vel))
+    fin, _ = pa.nbody(ptcl, None, cosmo, conf)
+    p0, pc, cc = pa.nbody_adj(fin, pa.Particles(conf, P(pmid), P(dcot), P(vcot)), cosmo, conf)
+    wd, wv, wa = onb.nbody(pmid, disp, vel, ocosmo, oconf)
+    (r0d, r0v, _), (wdc, wvc, _), wom = onb.nbody_adj(pmid, wd, wv, wa, dcot, vcot, ocosmo, oconf)
+    assert rel_l2(pc.disp.cpu().numpy(), wdc) < 1e-3
+    assert rel_l2(pc.vel.cpu().numpy(), wvc) < 1e-3
+    assert abs(cc.Omega_m - wom) < 1e-3 * abs(wom)
+    assert rel_l2(p0.disp.cpu().numpy(), disp) < 1e-4
+
+
+def test_nbody_autograd_matches_adjoint(dev):
+    pa, conf, oconf, cosmo, ocosmo = small_setup(shape=(8, 8, 8), a_start=1 / 4, steps_max=1 / 4)
+    pmid, disp = make_ptcl(conf.ptcl_grid_shape, conf.mesh_shape, conf.cell_size, 0.5, 6)
+    P = lambda a: torch.as_tensor(a, device=dev)
+    d = P(disp).requires_grad_(True)
+    v = torch.zeros_like(d).requires_grad_(True)
+    om = torch.tensor(cosmo.Omega_m, dtype=torch.float64, requires_grad=True)
+    out, _ = pa.nbody(pa.Particles(conf, P(pmid), d, v), None, cosmo, conf, omega_m=om)
+    dens = pa.scatter(out, conf)
+    loss = (dens**2).mean()
+    gd, gv, gom = torch.autograd.grad(loss, (d, v, om))
+    assert torch.isfinite(gd).all() and torch.isfinite(gv).all()
+    # Omega_m derivative by central differences of the whole pipeline
+    vals = []
+    for s in (1, -1):
+        c2 = pa.boltzmann(pa.SimpleLCDM(conf, Omega_m=0.3 + s * 1e-3), conf)
+        o2, _ = pa.nbody(pa.Particles(conf, P(pmid), P(disp), torch.zeros_like(P(disp))), None,
+                         c2, conf)
+        vals.append(float((pa.scatter(o2, conf).double()**2).mean()))
+    fd = (vals[0] - vals[1]) / 2e-3
+    assert abs(float(gom) - fd) < 0.02 * abs(fd) + 1e-6

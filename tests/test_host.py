@@ -1,0 +1,137 @@
+"""Host-side logic of pmwd_amd (no GPU): configuration, FP64 tables and
+sensitivities, time-step factors, and the C ABI's exported symbols."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import importlib
+
+import pmwd_amd as pa
+from oracle import cosmo as oc, nbody as onb
+from oracle.conf import OConf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+pnb = importlib.import_module('pmwd_amd.nbody')  # the package re-exports nbody()
+
+
+def test_configuration_errors():
+    """configuration.py:155-191 validation."""
+    with pytest.raises(ValueError):
+        pa.Configuration(1.0, (8, 8, 8), mesh_shape=(4, 8, 8))
+    with pytest.raises(ValueError):
+        pa.Configuration(1.0, (8, 8, 8), mesh_shape=(16, 8, 8))
+    with pytest.raises(ValueError):
+        pa.Configuration(1.0, (8, 8, 8), mesh_shape=(8, 8))
+    with pytest.raises(ValueError):
+        pa.Configuration(1.0, (8, 8, 8), symp_splits=((0, 0.5), (1, 0.4)))
+    with pytest.raises(ValueError):
+        pa.Configuration(1.0, (8, 8, 8), pmid_dtype=np.float32)
+
+
+@pytest.mark.parametrize('kw', [dict(), dict(mesh_shape=2, a_start=1 / 32, a_nbody_maxstep=1 / 32),
+                                dict(mesh_shape=1.5)])
+def test_configuration_matches_oracle(kw):
+    c = pa.Configuration(1000 / 64, (64, 64, 64), **kw)
+    o = OConf(1000 / 64, (64, 64, 64), **kw)
+    assert c.mesh_shape == o.mesh_shape
+    assert c.cell_size == o.cell_size
+    np.testing.assert_array_equal(c.a_nbody, o.a_nbody)
+    np.testing.assert_array_equal(c.growth_a, o.growth_a)
+    np.testing.assert_array_equal(c.transfer_k, o.transfer_k)
+    assert c.a_nbody_num == o.a_nbody_num
+
+
+def _cosmos(Om=0.3, **kw):
+    conf = pa.Configuration(4.0, (16, 16, 16), mesh_shape=2, **kw)
+    oconf = OConf(4.0, (16, 16, 16), mesh_shape=2, **kw)
+    return (conf, pa.boltzmann(pa.SimpleLCDM(conf, Omega_m=Om), conf), oconf,
+            oc.boltzmann(oc.SimpleLCDM(oconf, Omega_m=Om), oconf))
+
+
+def test_tables_match_oracle():
+    conf, c, oconf, o = _cosmos()
+    np.testing.assert_allclose(c.transfer, o.transfer, rtol=1e-13)
+    np.testing.assert_allclose(c.growth, o.growth, rtol=1e-10, atol=1e-14)
+    np.testing.assert_allclose(c.varlin, o.varlin, rtol=1e-12)
+    assert abs(c.sigma8 - o.sigma8) < 1e-12
+
+
+def test_from_sigma8():
+    conf = pa.Configuration(4.0, (16, 16, 16))
+    c = pa.boltzmann(pa.Cosmology.from_sigma8(conf, 0.8, n_s=0.96, Omega_m=0.3,
+                                              Omega_b=0.05, h=0.7), conf)
+    assert abs(c.sigma8 - 0.8) < 1e-12
+
+
+def test_growth_sensitivity_finite_difference():
+    conf, c, _, _ = _cosmos()
+    tight = conf.replace(growth_rtol=1e-12, growth_atol=1e-12)
+    h = 1e-5
+    gp = pa.boltzmann(pa.SimpleLCDM(tight, Omega_m=0.3 + h), tight).growth
+    gm = pa.boltzmann(pa.SimpleLCDM(tight, Omega_m=0.3 - h), tight).growth
+    fd = (gp - gm) / (2 * h)
+    # the sensitivity rides on steps chosen for G alone (rtol 1.5e-8), so its
+    # error is absolute, about 1e-7 of the table's O(1e-2..1) scale
+    scale = np.abs(fd).max()
+    np.testing.assert_allclose(c.growth_dOm[:, :, 1:], fd[:, :, 1:], rtol=2e-5, atol=1e-6 * scale)
+
+
+def test_factors_match_oracle():
+    conf, c, oconf, o = _cosmos()
+    for a0, a1 in zip(conf.a_nbody[:-1], conf.a_nbody[1:]):
+        ah = a0 * 0.5 + a1 * 0.5
+        assert pnb.kick_factor(a0, a0, ah, c, conf) == pytest.approx(
+            onb.kick_factor(a0, a0, ah, o, oconf), rel=1e-10)
+        assert pnb.drift_factor(ah, a0, a1, c, conf) == pytest.approx(
+            onb.drift_factor(ah, a0, a1, o, oconf), rel=1e-10)
+
+
+def test_factor_derivatives_finite_difference():
+    conf, c, _, _ = _cosmos()
+    tight = conf.replace(growth_rtol=1e-12, growth_atol=1e-12)
+    h = 1e-5
+    cp = pa.boltzmann(pa.SimpleLCDM(tight, Omega_m=0.3 + h), tight)
+    cm = pa.boltzmann(pa.SimpleLCDM(tight, Omega_m=0.3 - h), tight)
+    for a0, a1 in [(0.1, 0.2), (0.5, 0.75), (0.9, 1.0)]:
+        ah = 0.5 * (a0 + a1)
+        for fn, dfn, args in [(pnb.kick_factor, pnb.kick_factor_dOm, (a0, a0, ah)),
+                              (pnb.drift_factor, pnb.drift_factor_dOm, (ah, a0, a1))]:
+            fd = (fn(*args, cp, tight) - fn(*args, cm, tight)) / (2 * h)
+            assert dfn(*args, c, conf) == pytest.approx(fd, rel=1e-4)
+
+
+def test_library_exports_header_symbols():
+    """libpmx.so loads and exports every function include/pmx.h declares."""
+    from pmwd_amd import _lib
+    hdr = open(os.path.join(ROOT, 'include', 'pmx.h')).read()
+    declared = set(re.findall(r'\b(pmx_[a-z0-9_]+)\s*\(', hdr))
+    assert declared, 'no declarations parsed'
+    L = _lib.lib()
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+    assert declared == set(_lib.exported_symbols())
+    assert L.pmx_version() >= 100
+
+
+def test_library_rejects_bad_arguments():
+    """Argument checks return PMX_EINVAL with a message, before any device work."""
+    from pmwd_amd import _lib
+    L = _lib.lib()
+    out = ctypes.c_void_p()
+    assert L.pmx_ctx_create(0, 4, 4, 1.0, None, ctypes.byref(out)) == 1
+    assert b'positive' in L.pmx_last_error()
+    assert L.pmx_ctx_create(4, 4, 4, -1.0, None, ctypes.byref(out)) == 1
+    assert L.pmx_scatter_mass(None, None, None, 0, 1.0, None, 0) == 1
+
+
+def test_product_does_not_import_oracle():
+    """The product package never imports the oracle."""
+    pkg = os.path.join(ROOT, 'pmwd_amd')
+    for dp, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith('.py'):
+                src = open(os.path.join(dp, f)).read()
+                assert 'oracle' not in re.findall(r'^\s*(?:from|import)\s+(\w+)', src, re.M), f
