@@ -35,7 +35,13 @@ def parse():
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--cpu-slab', type=int, default=64,
                    help='fraction 1/slab of the particles in the CPU baseline sample')
+    p.add_argument('--copy-calib', action='store_true',
+                   help='also run a 1 GiB device copy (counter calibration for profiles)')
     return p.parse_args()
+
+
+def log(msg):
+    print(f'[bench] {msg}', file=sys.stderr, flush=True)
 
 
 def group_bytes(N, M, Mh):
@@ -130,6 +136,16 @@ def main():
     N, M = conf.ptcl_num, conf.mesh_size
     Mh = conf.mesh_shape[0] * conf.mesh_shape[1] * (conf.mesh_shape[2] // 2 + 1)
 
+    if args.copy_calib:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log('copy calibration: 3 x 1 GiB device copies')
+
+    log(f'initial conditions {G}^3 (2LPT)')
     t = time.perf_counter()
     modes = pa.white_noise(0, conf, real=True, device=dev)
     modes = pa.linear_modes(modes, cosmo, conf)
@@ -167,6 +183,7 @@ def main():
         step(i)
     torch.cuda.synchronize()
 
+    log(f'timed forward: {args.steps} steps')
     _lib.check(L.pmx_timing_enable(h, 1))
     _lib.timing_read(ctx)
     torch.cuda.synchronize()
@@ -233,6 +250,7 @@ def main():
     }
 
     if not args.no_adjoint:
+        log(f'timed adjoint: {args.steps} steps')
         dc = torch.randn_like(disp) * 1e-3
         vc = torch.randn_like(vel) * 1e-3
         acc_cot = torch.empty_like(disp)
@@ -265,6 +283,7 @@ def main():
         out['adj_ms_per_step'] = 1e3 * t_adj / K
 
     if not args.no_cpu_baseline:
+        log('cpu baseline sample (numpy oracle)')
         upd, info = cpu_baseline(conf, cosmo.Omega_m, args.cpu_slab)
         out['cpu_baseline'] = {'value': upd, 'unit': 'particle-updates/s', 'cores': info['cores'],
                                'kind': 'port', 'sample': info['sample'],

@@ -144,12 +144,55 @@ __global__ void bin_count_k(const int16_t* __restrict__ pmid, const float* __res
   }
 }
 
-__global__ void bin_fill_k(long long n, const uint32_t* __restrict__ ptile,
-                           const uint32_t* __restrict__ prank, const uint32_t* __restrict__ tile_start,
-                           uint32_t* __restrict__ bins) {
+// Record of a binned particle: d = disp/cell per axis (float32, the value
+// enmesh divides out) and the lower CIC corner relative to the tile origin
+// packed as lx | ly << 10 | lz << 20 (bit pattern stored in .w).
+__global__ void bin_fill_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp,
+                           long long n, MeshGeom g, TileGeom tg, const uint32_t* __restrict__ ptile,
+                           const uint32_t* __restrict__ prank,
+                           const uint32_t* __restrict__ tile_start, float4* __restrict__ rec,
+                           const float* __restrict__ pval, int vstride, int nch,
+                           float* __restrict__ vrec, float* __restrict__ vmax,
+                           uint32_t* __restrict__ pidx) {
   long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
-  if (i >= n) return;
-  bins[tile_start[ptile[i]] + prank[i]] = (uint32_t)i;
+  const bool active = i < n;
+  float vabs[3] = {0.f, 0.f, 0.f};
+  if (active) {
+    int pm[3];
+    float dp[3];
+    load_ptcl(pmid, disp, i, pm, dp);
+    CicAxis ax[3];
+    cic_setup(pm, dp, g, ax);
+    uint32_t t = ptile[i];
+    uint32_t slot = tile_start[t] + prank[i];
+    int lx = ax[0].lo % tg.tx, ly = ax[1].lo % tg.ty, lz = ax[2].lo % tg.tz;
+    uint32_t packed = (uint32_t)lx | ((uint32_t)ly << 10) | ((uint32_t)lz << 20);
+    rec[slot] = make_float4(dp[0] / g.cell, dp[1] / g.cell, dp[2] / g.cell, __uint_as_float(packed));
+    pidx[slot] = (uint32_t)i;
+    for (int ch = 0; ch < nch; ++ch) {
+      float v = pval[i * vstride + ch];
+      vrec[(long long)ch * n + slot] = v;
+      vabs[ch] = fabsf(v);
+    }
+  }
+  // max |v| per channel: reduce over the wave, then one atomic per wave only
+  // when it raises the running maximum (non-negative floats order like
+  // their bit patterns)
+  for (int ch = 0; ch < nch; ++ch) {
+    float m = vabs[ch];
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if (__lane_id() == 0) {
+      unsigned* vm = reinterpret_cast<unsigned*>(&vmax[ch]);
+      if (__float_as_uint(m) > __atomic_load_n(vm, __ATOMIC_RELAXED)) atomicMax(vm, __float_as_uint(m));
+    }
+  }
+}
+
+// per-axis fractions from d = disp/cell exactly as pmx_cic_axis forms them
+__device__ __forceinline__ void frac_of(float d, float& w0, float& w1) {
+  int i = (int)floorf(d);
+  w0 = 1.f - fabsf(d - (float)i);
+  w1 = 1.f - fabsf(d - (float)(i + 1));
 }
 
 __device__ __forceinline__ void tile_origin(int t, const TileGeom& tg, const MeshGeom& g, int o[3],
@@ -172,58 +215,95 @@ __device__ __forceinline__ int shell_index(int qx, int qy, int qz, int sx, int s
   return offA + sx * (sz + 1) + qx * sy + qy;
 }
 
-// One workgroup per tile: LDS accumulation of val*w (or val_p[chan]*w) of
-// the tile's particles over the (sx+1)(sy+1)(sz+1) cells they touch.
+// Fixed-point scale for tile accumulation: 2^k with |v| * 2^k < 2^30, so a
+// sum of up to 2^33 contributions fits a signed 64-bit integer; resolution
+// is 2^-30 of the largest |v|.  LDS float atomics serialise on gfx950 while
+// 64-bit integer LDS atomics run at full rate, and integer sums make the
+// accumulation order-independent (bitwise deterministic).
+__device__ __forceinline__ double fixed_scale(float vmax) {
+  int e = 0;
+  if (vmax > 0.f) frexpf(vmax, &e);  // vmax < 2^e
+  int k = 30 - e;
+  if (k > 300) k = 300;
+  return ldexp(1.0, k);
+}
+
+// One workgroup per tile: LDS accumulation of val*w (or vrec[slot]*w) of
+// the tile's binned records over the (sx+1)(sy+1)(sz+1) cells they touch;
+// each thread streams DEP_U coalesced records per round.  The interior is
+// stored with plain stores, the +1 shell to the tile's halo slot.
+#define DEP_TPB 512
+#define DEP_U 4
 template <int PERVAL>
-__global__ void __launch_bounds__(PMX_TPB)
-    deposit_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp,
-              const uint32_t* __restrict__ bins, const uint32_t* __restrict__ tile_start,
+__global__ void __launch_bounds__(DEP_TPB)
+    deposit_k(const float4* __restrict__ rec, const float* __restrict__ vrec,
+              const float* __restrict__ vmaxp, const uint32_t* __restrict__ tile_start,
               const uint32_t* __restrict__ tile_count, MeshGeom g, TileGeom tg, float val,
-              const float* __restrict__ pval, int vstride, float* __restrict__ mesh, int mrs,
-              int accumulate, float* __restrict__ halo) {
-  extern __shared__ float acc[];
+              float* __restrict__ mesh, int mrs, int accumulate, float* __restrict__ halo) {
+  extern __shared__ unsigned long long acc[];
   int o[3], s[3], ti[3];
-  const int t = blockIdx.x;
+  const int t = (int)pmx_xcd_block(blockIdx.x, gridDim.x);
   tile_origin(t, tg, g, o, s, ti);
   const int LY = s[1] + 1, LZ = s[2] + 1;
   const int ltot = (s[0] + 1) * LY * LZ;
-  for (int i = threadIdx.x; i < ltot; i += PMX_TPB) acc[i] = 0.f;
+  for (int i = threadIdx.x; i < ltot; i += DEP_TPB) acc[i] = 0ull;
+  const double scale = fixed_scale(PERVAL ? *vmaxp : fabsf(val));
+  const double inv = 1.0 / scale;
   __syncthreads();
   const uint32_t start = tile_start[t];
   const uint32_t cnt = tile_count[t];
-  for (uint32_t q = threadIdx.x; q < cnt; q += PMX_TPB) {
-    long long p = bins[start + q];
-    int pm[3];
-    float dp[3];
-    load_ptcl(pmid, disp, p, pm, dp);
-    CicAxis ax[3];
-    cic_setup(pm, dp, g, ax);
-    float v = PERVAL ? pval[p * vstride] : val;
-    int lx = ax[0].lo - o[0], ly = ax[1].lo - o[1], lz = ax[2].lo - o[2];
-    // binned by this tile, so always inside; the guard keeps LDS in bounds
-    if (lx < 0 || lx >= s[0] || ly < 0 || ly >= s[1] || lz < 0 || lz >= s[2]) continue;
-    int b = (lx * LY + ly) * LZ + lz;
+  for (uint32_t q0 = 0; q0 < cnt; q0 += DEP_TPB * DEP_U) {
+    float4 r[DEP_U];
+    float v[DEP_U];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      int li = b + ((j & 1) ? LY * LZ : 0) + ((j & 2) ? LZ : 0) + ((j & 4) ? 1 : 0);
-      atomicAdd(&acc[li], v * pmx_w(ax, j));
+    for (int u = 0; u < DEP_U; ++u) {
+      uint32_t q = q0 + u * DEP_TPB + threadIdx.x;
+      if (q < cnt) {
+        r[u] = rec[start + q];
+        v[u] = PERVAL ? vrec[start + q] : val;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < DEP_U; ++u) {
+      uint32_t q = q0 + u * DEP_TPB + threadIdx.x;
+      if (q >= cnt) continue;
+      uint32_t pk = __float_as_uint(r[u].w);
+      int lx = pk & 1023, ly = (pk >> 10) & 1023, lz = (pk >> 20) & 1023;
+      // binned by this tile, so always inside; the guard keeps LDS in bounds
+      if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
+      float a0, a1, b0, b1, c0, c1;
+      frac_of(r[u].x, a0, a1);
+      frac_of(r[u].y, b0, b1);
+      frac_of(r[u].z, c0, c1);
+      int b = (lx * LY + ly) * LZ + lz;
+      float vv = v[u];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        float f0 = (j & 1) ? a1 : a0;
+        float f1 = (j & 2) ? b1 : b0;
+        float f2 = (j & 4) ? c1 : c0;
+        int li = b + ((j & 1) ? LY * LZ : 0) + ((j & 2) ? LZ : 0) + ((j & 4) ? 1 : 0);
+        float x = vv * ((f0 * f1) * f2);
+        long long qx = (long long)rint((double)x * scale);
+        atomicAdd(&acc[li], (unsigned long long)qx);
+      }
     }
   }
   __syncthreads();
   const int itot = s[0] * s[1] * s[2];
-  for (int i = threadIdx.x; i < itot; i += PMX_TPB) {
+  for (int i = threadIdx.x; i < itot; i += DEP_TPB) {
     int lz = i % s[2];
     int ly = (i / s[2]) % s[1];
     int lx = i / (s[2] * s[1]);
     long long m = ((long long)(o[0] + lx) * g.n1 + (o[1] + ly)) * mrs + (o[2] + lz);
-    float a = acc[(lx * LY + ly) * LZ + lz];
+    float a = (float)((double)(long long)acc[(lx * LY + ly) * LZ + lz] * inv);
     mesh[m] = accumulate ? mesh[m] + a : a;
   }
   const int offA = LY * LZ;
   const int offB = s[0] * LZ;
   const int stot = offA + offB + s[0] * s[1];
   float* hs = halo + (long long)t * tg.shell;
-  for (int i = threadIdx.x; i < stot; i += PMX_TPB) {
+  for (int i = threadIdx.x; i < stot; i += DEP_TPB) {
     int qx, qy, qz;
     if (i < offA) {
       qx = s[0]; qy = i / LZ; qz = i % LZ;
@@ -234,7 +314,74 @@ __global__ void __launch_bounds__(PMX_TPB)
       int k = i - offA - offB;
       qx = k / s[1]; qy = k % s[1]; qz = s[2];
     }
-    hs[i] = acc[(qx * LY + qy) * LZ + qz];
+    hs[i] = (float)((double)(long long)acc[(qx * LY + qy) * LZ + qz] * inv);
+  }
+}
+
+// One workgroup per tile: the tile's three force meshes plus the +1 layer
+// (periodic) are staged in LDS, then each binned record gathers its eight
+// neighbours from LDS: acc = sum_j F[c_j] w_j in the tree order of
+// pmx_tree8, written to the particle's index; KICK also closes the step,
+// vel += acc * k2.  Same arithmetic as a per-particle gather.
+#define TG_TPB 512
+template <int KICK>
+__global__ void __launch_bounds__(TG_TPB)
+    tile_gather_k(const float4* __restrict__ rec, const uint32_t* __restrict__ pidx,
+                  const uint32_t* __restrict__ tile_start, const uint32_t* __restrict__ tile_count,
+                  MeshGeom g, TileGeom tg, const float* __restrict__ F0,
+                  const float* __restrict__ F1, const float* __restrict__ F2,
+                  float* __restrict__ acc, float* __restrict__ vel, float k2) {
+  extern __shared__ float sh[];
+  int o[3], s[3], ti[3];
+  const int t = (int)pmx_xcd_block(blockIdx.x, gridDim.x);
+  tile_origin(t, tg, g, o, s, ti);
+  const int LY = s[1] + 1, LZ = s[2] + 1;
+  const int ltot = (s[0] + 1) * LY * LZ;
+  for (int i = threadIdx.x; i < ltot; i += TG_TPB) {
+    int qz = i % LZ;
+    int qy = (i / LZ) % LY;
+    int qx = i / (LZ * LY);
+    int x = o[0] + qx, y = o[1] + qy, z = o[2] + qz;
+    if (x >= g.n0) x -= g.n0;
+    if (y >= g.n1) y -= g.n1;
+    if (z >= g.n2) z -= g.n2;
+    long long m = ((long long)x * g.n1 + y) * g.rs + z;
+    sh[i] = F0[m];
+    sh[ltot + i] = F1[m];
+    sh[2 * ltot + i] = F2[m];
+  }
+  __syncthreads();
+  const uint32_t start = tile_start[t];
+  const uint32_t cnt = tile_count[t];
+  for (uint32_t q = threadIdx.x; q < cnt; q += TG_TPB) {
+    float4 r = rec[start + q];
+    long long p = pidx[start + q];
+    uint32_t pk = __float_as_uint(r.w);
+    int lx = pk & 1023, ly = (pk >> 10) & 1023, lz = (pk >> 20) & 1023;
+    if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
+    float a0, a1, b0, b1, c0, c1;
+    frac_of(r.x, a0, a1);
+    frac_of(r.y, b0, b1);
+    frac_of(r.z, c0, c1);
+    int b = (lx * LY + ly) * LZ + lz;
+    float v0[8], v1[8], v2[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      float f0 = (j & 1) ? a1 : a0;
+      float f1 = (j & 2) ? b1 : b0;
+      float f2 = (j & 4) ? c1 : c0;
+      float w = (f0 * f1) * f2;
+      int li = b + ((j & 1) ? LY * LZ : 0) + ((j & 2) ? LZ : 0) + ((j & 4) ? 1 : 0);
+      v0[j] = sh[li] * w;
+      v1[j] = sh[ltot + li] * w;
+      v2[j] = sh[2 * ltot + li] * w;
+    }
+    float o3[3] = {pmx_tree8(v0), pmx_tree8(v1), pmx_tree8(v2)};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      acc[3 * p + a] = o3[a];
+      if (KICK) vel[3 * p + a] = vel[3 * p + a] + o3[a] * k2;
+    }
   }
 }
 
@@ -285,7 +432,37 @@ __global__ void __launch_bounds__(PMX_TPB)
   }
 }
 
-// Bins the particles by tile (c->bins, c->tile_start, c->tile_count).
+// Writes the binned records (and nch per-particle values, channel-major, to
+// c->vrec) once the tile counts are scanned.
+static int bin_fill(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
+                    const float* pval, int vstride, int nch) {
+  if (n == 0) return PMX_OK;
+  if (nch > 0) {
+    int rc = pmx_ensure_vrec(c, n);
+    if (rc) return rc;
+  }
+  MeshGeom g = pmx_geom_padded(c);
+  if (nch > 0) {
+    int rc = fill_u32(c, reinterpret_cast<uint32_t*>(c->vmax), 3, 0u);
+    if (rc) return rc;
+  }
+  pmx_tic(c, PMX_T_FILL);
+  bin_fill_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(pmid, disp, n, g, c->tg, c->pkey, c->prank,
+                                                       c->tile_start, c->rec, pval, vstride, nch,
+                                                       c->vrec, c->vmax, c->pidx);
+  pmx_toc(c, PMX_T_FILL);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+static int scan_tiles(pmx_ctx* c) {
+  pmx_tic(c, PMX_T_SCAN);
+  int rc = pmx_scan_exclusive(c, c->tile_count, c->tile_start, c->ntiles + 1);
+  pmx_toc(c, PMX_T_SCAN);
+  return rc;
+}
+
+// Bins the particles by tile (c->rec, c->tile_start, c->tile_count).
 static int bin_particles(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n) {
   int rc;
   if ((rc = pmx_ensure_ptcl(c, n))) return rc;
@@ -298,36 +475,27 @@ static int bin_particles(pmx_ctx* c, const int16_t* pmid, const float* disp, lon
     pmx_toc(c, PMX_T_BIN);
     PMX_LAUNCH_CHECK();
   }
-  pmx_tic(c, PMX_T_SCAN);
-  rc = pmx_scan_exclusive(c, c->tile_count, c->tile_start, c->ntiles + 1);
-  pmx_toc(c, PMX_T_SCAN);
-  if (rc) return rc;
-  if (n > 0) {
-    pmx_tic(c, PMX_T_FILL);
-    bin_fill_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(n, c->pkey, c->prank, c->tile_start,
-                                                         c->bins);
-    pmx_toc(c, PMX_T_FILL);
-    PMX_LAUNCH_CHECK();
-  }
-  return PMX_OK;
+  if ((rc = scan_tiles(c))) return rc;
+  return bin_fill(c, pmid, disp, n, nullptr, 0, 0);
 }
 
-// Deposits the binned particles into mesh (row stride mrs).
-static int deposit_binned(pmx_ctx* c, const int16_t* pmid, const float* disp, float val,
-                          const float* pval, int vstride, float* mesh, int mrs, int accumulate) {
+// Deposits the binned records into mesh (row stride mrs), with the scalar
+// value val or the per-record values of channel ch of c->vrec (ch >= 0).
+static int deposit_binned(pmx_ctx* c, float val, int ch, long long n, float* mesh, int mrs,
+                          int accumulate) {
   MeshGeom g = pmx_geom_padded(c);
   const TileGeom& tg = c->tg;
-  size_t lds = (size_t)(tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(float);
+  size_t lds = (size_t)(tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(unsigned long long);
   unsigned nt = (unsigned)c->ntiles;
   pmx_tic(c, PMX_T_DEPOSIT);
-  if (pval)
-    deposit_k<1><<<nt, PMX_TPB, lds, c->stream>>>(pmid, disp, c->bins, c->tile_start,
-                                                  c->tile_count, g, tg, val, pval, vstride, mesh,
+  if (ch >= 0)
+    deposit_k<1><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, c->vrec + (long long)ch * n, c->vmax + ch,
+                                                  c->tile_start, c->tile_count, g, tg, val, mesh,
                                                   mrs, accumulate, c->halo);
   else
-    deposit_k<0><<<nt, PMX_TPB, lds, c->stream>>>(pmid, disp, c->bins, c->tile_start,
-                                                  c->tile_count, g, tg, val, nullptr, 0, mesh, mrs,
-                                                  accumulate, c->halo);
+    deposit_k<0><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, nullptr, nullptr, c->tile_start,
+                                                  c->tile_count, g, tg, val, mesh, mrs, accumulate,
+                                                  c->halo);
   pmx_toc(c, PMX_T_DEPOSIT);
   PMX_LAUNCH_CHECK();
   pmx_tic(c, PMX_T_FOLD);
@@ -344,7 +512,7 @@ extern "C" int pmx_scatter_mass(pmx_ctx* c, const int16_t* pmid, const float* di
   PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_scatter_mass: too many particles");
   int rc;
   if ((rc = bin_particles(c, pmid, disp, n))) return rc;
-  return deposit_binned(c, pmid, disp, val, nullptr, 0, mesh, c->n[2], accumulate);
+  return deposit_binned(c, val, -1, n, mesh, c->n[2], accumulate);
 }
 
 // ----------------------------------------------------------- gather3 --
@@ -359,7 +527,7 @@ __global__ void __launch_bounds__(PMX_TPB)
               const float* __restrict__ F2, float* __restrict__ acc,
               const float* __restrict__ vcot, float* __restrict__ alpha,
               double* __restrict__ part) {
-  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  long long i = (long long)pmx_xcd_block(blockIdx.x, gridDim.x) * PMX_TPB + threadIdx.x;
   double dotv = 0;
   if (i < n) {
     int pm[3];
@@ -426,7 +594,7 @@ __global__ void __launch_bounds__(PMX_TPB)
 __global__ void __launch_bounds__(PMX_TPB)
     grad_gather1_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp, long long n,
                    MeshGeom g, const float* __restrict__ R, float val, float* __restrict__ alpha) {
-  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  long long i = (long long)pmx_xcd_block(blockIdx.x, gridDim.x) * PMX_TPB + threadIdx.x;
   if (i >= n) return;
   int pm[3];
   float dp[3];
@@ -448,12 +616,34 @@ __global__ void __launch_bounds__(PMX_TPB)
   alpha[3 * i + 2] += pmx_tree8(gz) / g.cell;
 }
 
+// Tile-staged gather of the force meshes in c->cbuf (padded) for the
+// binned particles; with vel also the closing kick.
+static int tile_gather(pmx_ctx* c, float* acc, float* vel, float k2) {
+  MeshGeom g = pmx_geom_padded(c);
+  const TileGeom& tg = c->tg;
+  size_t lds = (size_t)3 * (tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(float);
+  unsigned nt = (unsigned)c->ntiles;
+  const float* F0 = (const float*)c->cbuf[0];
+  const float* F1 = (const float*)c->cbuf[1];
+  const float* F2 = (const float*)c->cbuf[2];
+  pmx_tic(c, PMX_T_GATHER);
+  if (vel)
+    tile_gather_k<1><<<nt, TG_TPB, lds, c->stream>>>(c->rec, c->pidx, c->tile_start, c->tile_count,
+                                                     g, tg, F0, F1, F2, acc, vel, k2);
+  else
+    tile_gather_k<0><<<nt, TG_TPB, lds, c->stream>>>(c->rec, c->pidx, c->tile_start, c->tile_count,
+                                                     g, tg, F0, F1, F2, acc, nullptr, 0.f);
+  pmx_toc(c, PMX_T_GATHER);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
 // ----------------------------------------------------------- gravity --
 static int gravity_fwd_meshes(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
                               float val, float coef) {
   int rc;
   if ((rc = bin_particles(c, pmid, disp, n))) return rc;
-  if ((rc = deposit_binned(c, pmid, disp, val, nullptr, 0, (float*)c->cbuf[0], c->rs, 0))) return rc;
+  if ((rc = deposit_binned(c, val, -1, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
   return pmx_poisson_ctx(c, coef);
 }
 
@@ -465,14 +655,7 @@ extern "C" int pmx_gravity(pmx_ctx* c, const int16_t* pmid, const float* disp, l
   int rc;
   if ((rc = gravity_fwd_meshes(c, pmid, disp, n, val, coef))) return rc;
   if (n == 0) return PMX_OK;
-  MeshGeom g = pmx_geom_padded(c);
-  pmx_tic(c, PMX_T_GATHER);
-    gather3_k<0><<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(
-      pmid, disp, n, g, (const float*)c->cbuf[0], (const float*)c->cbuf[1],
-      (const float*)c->cbuf[2], acc, nullptr, nullptr, nullptr);
-  pmx_toc(c, PMX_T_GATHER);
-    PMX_LAUNCH_CHECK();
-  return PMX_OK;
+  return tile_gather(c, acc, nullptr, 0.f);
 }
 
 extern "C" int pmx_gravity_vjp(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
@@ -494,9 +677,11 @@ extern "C" int pmx_gravity_vjp(pmx_ctx* c, const int16_t* pmid, const float* dis
     pmx_toc(c, PMX_T_GATHER);
     PMX_LAUNCH_CHECK();
   }
-  // the particles did not move: reuse the bins for the three cotangent meshes
+  // the particles did not move: reuse the tile counts, refill the records
+  // with the three cotangent components, and deposit one mesh per component
+  if ((rc = bin_fill(c, pmid, disp, n, vel_cot, 3, 3))) return rc;
   for (int d = 0; d < 3; ++d)
-    if ((rc = deposit_binned(c, pmid, disp, 0.f, vel_cot + d, 3, (float*)c->cbuf[d], c->rs, 0)))
+    if ((rc = deposit_binned(c, 0.f, d, n, (float*)c->cbuf[d], c->rs, 0)))
       return rc;
   // rho_cot = -coef * sum_d L_d(mesh_cot_d), L_d = irfftn . (-ik_d) . laplace . rfftn
   if ((rc = pmx_poisson_adj_ctx(c, (float)(-(double)coef / (double)c->M)))) return rc;
@@ -577,7 +762,7 @@ __global__ void __launch_bounds__(PMX_TPB)
                    MeshGeom g, const float* __restrict__ F0, const float* __restrict__ F1,
                    const float* __restrict__ F2, float* __restrict__ acc, float* __restrict__ vel,
                    float k2) {
-  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  long long i = (long long)pmx_xcd_block(blockIdx.x, gridDim.x) * PMX_TPB + threadIdx.x;
   if (i >= n) return;
   int pm[3];
   float dp[3];
@@ -621,27 +806,11 @@ extern "C" int pmx_nbody_step(pmx_ctx* c, const int16_t* pmid, float* disp, floa
     pmx_toc(c, PMX_T_BIN);
     PMX_LAUNCH_CHECK();
   }
-  pmx_tic(c, PMX_T_SCAN);
-  rc = pmx_scan_exclusive(c, c->tile_count, c->tile_start, c->ntiles + 1);
-  pmx_toc(c, PMX_T_SCAN);
-  if (rc) return rc;
-  if (n > 0) {
-    pmx_tic(c, PMX_T_FILL);
-    bin_fill_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(n, c->pkey, c->prank, c->tile_start,
-                                                         c->bins);
-    pmx_toc(c, PMX_T_FILL);
-    PMX_LAUNCH_CHECK();
-  }
-  if ((rc = deposit_binned(c, pmid, disp, val, nullptr, 0, (float*)c->cbuf[0], c->rs, 0))) return rc;
+  if ((rc = scan_tiles(c))) return rc;
+  if ((rc = bin_fill(c, pmid, disp, n, nullptr, 0, 0))) return rc;
+  if ((rc = deposit_binned(c, val, -1, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
   if ((rc = pmx_poisson_ctx(c, coef))) return rc;
-  if (n > 0) {
-    pmx_tic(c, PMX_T_GATHER);
-    gather3_kick_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(
-        pmid, disp, n, g, (const float*)c->cbuf[0], (const float*)c->cbuf[1],
-        (const float*)c->cbuf[2], acc, vel, k2);
-    pmx_toc(c, PMX_T_GATHER);
-    PMX_LAUNCH_CHECK();
-  }
+  if (n > 0 && (rc = tile_gather(c, acc, vel, k2))) return rc;
   return PMX_OK;
 }
 

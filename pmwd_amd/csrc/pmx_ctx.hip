@@ -41,12 +41,13 @@ static void dev_free(pmx_ctx* c, void** p, size_t bytes) {
   *p = nullptr;
 }
 
-// Scatter tile extent: 16 x 16 x 32 cells, clipped to the mesh.
+// Scatter tile extent: 16^3 cells, clipped to the mesh (the 17^3 LDS block
+// of 64-bit accumulators is 39 KB, four workgroups per CU).
 static TileGeom make_tiles(const int n[3]) {
   TileGeom t;
   t.tx = n[0] < 16 ? n[0] : 16;
   t.ty = n[1] < 16 ? n[1] : 16;
-  t.tz = n[2] < 32 ? n[2] : 32;
+  t.tz = n[2] < 16 ? n[2] : 16;
   t.ntx = (n[0] + t.tx - 1) / t.tx;
   t.nty = (n[1] + t.ty - 1) / t.ty;
   t.ntz = (n[2] + t.tz - 1) / t.tz;
@@ -92,9 +93,12 @@ extern "C" int pmx_ctx_create(int n0, int n1, int n2, double cell_size, void* st
   }
   c->n[0] = n0; c->n[1] = n1; c->n[2] = n2;
   c->n2h = n2 / 2 + 1;
-  c->rs = 2 * c->n2h;
+  // 16 complex = one 128-byte cache line: column blocks of the strided passes
+  // and rows of the row passes then start on line boundaries
+  c->n2p = c->n2h <= 16 ? c->n2h : (c->n2h + 15) / 16 * 16;
+  c->rs = 2 * c->n2p;
   c->M = (long long)n0 * n1 * n2;
-  c->Mc = (long long)n0 * n1 * c->n2h;
+  c->Mc = (long long)n0 * n1 * c->n2p;
   c->cell = cell_size;
   c->cellf = (float)cell_size;
   c->stream = (hipStream_t)stream;
@@ -127,6 +131,7 @@ extern "C" int pmx_ctx_create(int n0, int n1, int n2, double cell_size, void* st
   if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->tile_count, (size_t)(c->ntiles + 1) * 4);
   if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->tile_start, (size_t)(c->ntiles + 1) * 4);
   if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->halo, (size_t)c->ntiles * c->tg.shell * 4);
+  if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->vmax, 3 * sizeof(float));
   if (rc == PMX_OK) rc |= pmx_ensure_red(c, 1024);
   if (rc != PMX_OK) {
     pmx_ctx_destroy(c);
@@ -154,7 +159,7 @@ extern "C" int pmx_ctx_destroy(pmx_ctx* c) {
     if (c->tw[a]) hipFree(c->tw[a]);
     if (c->cbuf[a]) hipFree(c->cbuf[a]);
   }
-  void* ps[] = {c->tile_count, c->tile_start, c->halo, c->bins, c->pkey, c->prank, c->red,
+  void* ps[] = {c->tile_count, c->tile_start, c->halo, c->rec, c->pkey, c->prank, c->pidx, c->vrec, c->vmax, c->red,
                 c->sort_k[0], c->sort_k[1], c->sort_v[0], c->sort_v[1], c->sort_hist,
                 c->scan_tmp};
   for (void* p : ps)
@@ -181,15 +186,27 @@ extern "C" int pmx_tile_shape(const pmx_ctx* c, int t[3]) {
 int pmx_ensure_ptcl(pmx_ctx* c, long long n) {
   if (n <= c->ptcl_cap) return PMX_OK;
   long long cap = n + n / 8 + 1024;
-  dev_free(c, (void**)&c->bins, c->ptcl_cap * 4);
+  dev_free(c, (void**)&c->rec, c->ptcl_cap * 16);
   dev_free(c, (void**)&c->pkey, c->ptcl_cap * 4);
   dev_free(c, (void**)&c->prank, c->ptcl_cap * 4);
+  dev_free(c, (void**)&c->pidx, c->ptcl_cap * 4);
   c->ptcl_cap = 0;
-  int rc = dev_alloc(c, (void**)&c->bins, cap * 4);
+  int rc = dev_alloc(c, (void**)&c->rec, cap * 16);
   rc |= dev_alloc(c, (void**)&c->pkey, cap * 4);
   rc |= dev_alloc(c, (void**)&c->prank, cap * 4);
+  rc |= dev_alloc(c, (void**)&c->pidx, cap * 4);
   if (rc) return PMX_ENOMEM;
   c->ptcl_cap = cap;
+  return PMX_OK;
+}
+
+int pmx_ensure_vrec(pmx_ctx* c, long long n) {
+  if (3 * n <= c->vrec_cap) return PMX_OK;
+  long long cap = 3 * (n + n / 8 + 1024);
+  dev_free(c, (void**)&c->vrec, c->vrec_cap * 4);
+  c->vrec_cap = 0;
+  if (dev_alloc(c, (void**)&c->vrec, cap * 4)) return PMX_ENOMEM;
+  c->vrec_cap = cap;
   return PMX_OK;
 }
 

@@ -1,7 +1,7 @@
 // Hand-written 3D real FFTs and the fused Poisson/gradient solve (gfx950).
 //
 // Layout: a real mesh lives in "padded" rows of rs = 2*(n2/2+1) floats so the
-// r2c/c2r row passes run in place; the complex mesh is (n0, n1, n2h) float2.
+// r2c/c2r row passes run in place; the complex mesh is (n0, n1, n2h) float2 in rows of n2p >= n2h.
 // Each kernel stages P pencils of length n in LDS (pencil-major for the
 // contiguous row passes, interleaved for the strided passes) and runs a
 // Stockham autosort FFT with radix 8/4/2 stages, or a direct DFT for lengths
@@ -325,14 +325,14 @@ struct KTab {
 // inverse FFT along x, stored to c_d.  Pencils along axis 0 of (n0,n1,n2h);
 // blockIdx.y = y index, blockIdx.x = block of B kz columns.
 template <bool POW2>
-__global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_k(Fields3 fl, int n0, int n1, int n2h, int B,
+__global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_k(Fields3 fl, int n0, int n1, int n2h, int n2p, int B,
                                                        KTab kt, const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
-  const long long es = (long long)n1 * n2h;
+  const long long es = (long long)n1 * n2p;
   const int y = blockIdx.y;
   const int c0 = blockIdx.x * B;
   const int nb = n2h - c0 < B ? n2h - c0 : B;
-  const long long off = (long long)y * n2h + c0;
+  const long long off = (long long)y * n2p + c0;
   const int total = n0 * B;
   for (int idx = threadIdx.x; idx < total; idx += FFT_TPB) {
     int j = idx / B, b = idx - (idx / B) * B;
@@ -385,14 +385,14 @@ __global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_k(Fields3 fl, int n0, in
 // Adjoint x pass: acc = sum_d (-i km_d) laplace(FFT_x c_d); inverse FFT
 // along x; stored to c0.  Transpose of the three force components.
 template <bool POW2>
-__global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_adj_k(Fields3 fl, int n0, int n1, int n2h, int B,
+__global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_adj_k(Fields3 fl, int n0, int n1, int n2h, int n2p, int B,
                                                            KTab kt, const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
-  const long long es = (long long)n1 * n2h;
+  const long long es = (long long)n1 * n2p;
   const int y = blockIdx.y;
   const int c0 = blockIdx.x * B;
   const int nb = n2h - c0 < B ? n2h - c0 : B;
-  const long long off = (long long)y * n2h + c0;
+  const long long off = (long long)y * n2p + c0;
   const int total = n0 * B;
   const float ky = kt.k[1][y];
   const float kym = kt.km[1][y];
@@ -502,12 +502,12 @@ int pmx_fft_strided(pmx_ctx* c, float2* const* f, int nf, int axis, int dir) {
   long long es, ostride;
   unsigned nouter;
   if (axis == 1) {
-    es = c->n2h;
-    ostride = (long long)c->n[1] * c->n2h;
+    es = c->n2p;
+    ostride = (long long)c->n[1] * c->n2p;
     nouter = c->n[0];
   } else {
-    es = (long long)c->n[1] * c->n2h;
-    ostride = c->n2h;
+    es = (long long)c->n[1] * c->n2p;
+    ostride = c->n2p;
     nouter = c->n[1];
   }
   dim3 grid((ncol + B - 1) / B, nouter, nf);
@@ -546,13 +546,13 @@ int pmx_poisson_x(pmx_ctx* c, float2* const* f, int adjoint) {
   bool p2 = is_pow2(n0);
   pmx_tic(c, PMX_T_POISSON);
   if (adjoint && p2)
-    poisson_x_adj_k<true><<<grid, FFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, B, ktab(c), c->tw[0]);
+    poisson_x_adj_k<true><<<grid, FFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, ktab(c), c->tw[0]);
   else if (adjoint)
-    poisson_x_adj_k<false><<<grid, FFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, B, ktab(c), c->tw[0]);
+    poisson_x_adj_k<false><<<grid, FFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, ktab(c), c->tw[0]);
   else if (p2)
-    poisson_x_k<true><<<grid, FFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, B, ktab(c), c->tw[0]);
+    poisson_x_k<true><<<grid, FFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, ktab(c), c->tw[0]);
   else
-    poisson_x_k<false><<<grid, FFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, B, ktab(c), c->tw[0]);
+    poisson_x_k<false><<<grid, FFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, ktab(c), c->tw[0]);
   pmx_toc(c, PMX_T_POISSON);
   PMX_LAUNCH_CHECK();
   return PMX_OK;
@@ -589,12 +589,14 @@ int pmx_poisson_adj_ctx(pmx_ctx* c, float scale_out) {
   return PMX_OK;
 }
 
-int pmx_copy_rows(pmx_ctx* c, float* dst, int dst_rs, const float* src, int src_rs) {
+// copies n0*n1 rows of ncols floats between row strides (ncols <= 0: n2)
+int pmx_copy_rows(pmx_ctx* c, float* dst, int dst_rs, const float* src, int src_rs, int ncols = 0) {
+  if (ncols <= 0) ncols = c->n[2];
   long long nrows = (long long)c->n[0] * c->n[1];
-  long long total = nrows * c->n[2];
+  long long total = nrows * ncols;
   unsigned nblk = (unsigned)((total + PMX_TPB - 1) / PMX_TPB);
   if (nblk > 65536u * 4) nblk = 65536u * 4;
-  copy_rows_k<<<nblk, PMX_TPB, 0, c->stream>>>(dst, dst_rs, src, src_rs, nrows, c->n[2]);
+  copy_rows_k<<<nblk, PMX_TPB, 0, c->stream>>>(dst, dst_rs, src, src_rs, nrows, ncols);
   PMX_LAUNCH_CHECK();
   return PMX_OK;
 }
@@ -607,17 +609,14 @@ extern "C" int pmx_rfftn(pmx_ctx* c, const float* in, float* out_c) {
   if ((rc = pmx_fft_r2c_inplace(c, (float*)f[0], 0, 0.f))) return rc;
   if ((rc = pmx_fft_strided(c, f, 1, 1, -1))) return rc;
   if ((rc = pmx_fft_strided(c, f, 1, 0, -1))) return rc;
-  PMX_CHECK_HIP(hipMemcpyAsync(out_c, f[0], (size_t)c->Mc * sizeof(float2),
-                               hipMemcpyDeviceToDevice, c->stream));
-  return PMX_OK;
+  return pmx_copy_rows(c, out_c, 2 * c->n2h, (const float*)f[0], c->rs, 2 * c->n2h);
 }
 
 extern "C" int pmx_irfftn(pmx_ctx* c, const float* in_c, float* out) {
   PMX_REQUIRE(c && in_c && out, "pmx_irfftn: NULL argument");
   int rc;
   float2* f[3] = {c->cbuf[0], c->cbuf[1], c->cbuf[2]};
-  PMX_CHECK_HIP(hipMemcpyAsync(f[0], in_c, (size_t)c->Mc * sizeof(float2),
-                               hipMemcpyDeviceToDevice, c->stream));
+  if ((rc = pmx_copy_rows(c, (float*)f[0], c->rs, in_c, 2 * c->n2h, 2 * c->n2h))) return rc;
   if ((rc = pmx_fft_strided(c, f, 1, 0, 1))) return rc;
   if ((rc = pmx_fft_strided(c, f, 1, 1, 1))) return rc;
   if ((rc = pmx_fft_c2r_inplace(c, (float*)f[0], (float)(1.0 / (double)c->M)))) return rc;

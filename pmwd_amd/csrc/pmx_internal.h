@@ -83,9 +83,10 @@ struct pmx_ctx {
   PmxTimer* timer;     // NULL unless timing is enabled
   int n[3];
   int n2h;             // n2/2+1 complex columns of the r2c layout
-  int rs;              // padded real row stride = 2*n2h
+  int n2p;             // complex row stride: n2h rounded up to 16 (128 B) above 16
+  int rs;              // padded real row stride = 2*n2p floats
   long long M;         // real cells
-  long long Mc;        // complex cells n0*n1*n2h
+  long long Mc;        // complex cells held, n0*n1*n2p
   double cell;         // float64 cell size (fftfreq, pm_util.py:187-195)
   float cellf;         // float32 cell size
   hipStream_t stream;
@@ -108,9 +109,13 @@ struct pmx_ctx {
   uint32_t* tile_start;
   float* halo;
   long long ptcl_cap;
-  uint32_t* bins;
-  uint32_t* pkey;
-  uint32_t* prank;
+  float4* rec;         // binned particle records (d per axis, packed local cell)
+  uint32_t* pkey;      // tile of each particle
+  uint32_t* prank;     // rank of each particle within its tile
+  uint32_t* pidx;      // particle index of each record
+  float* vrec;         // up to 3 channels of per-record values (adjoint)
+  float* vmax;         // max |value| per channel of vrec (fixed-point scale)
+  long long vrec_cap;
 
   // reductions (float64 partial sums)
   double* red;
@@ -130,6 +135,7 @@ struct pmx_ctx {
 };
 
 int pmx_ensure_ptcl(pmx_ctx* c, long long n);
+int pmx_ensure_vrec(pmx_ctx* c, long long n);
 int pmx_ensure_red(pmx_ctx* c, int nblk);
 int pmx_scan_exclusive(pmx_ctx* c, const uint32_t* in, uint32_t* out, long long n);
 MeshGeom pmx_geom_padded(const pmx_ctx* c);
@@ -142,6 +148,17 @@ static inline unsigned pmx_blocks(long long n, int tpb = PMX_TPB) {
 }
 
 // ------------------------------------------------------- device helpers --
+// Workgroups are dispatched round-robin over the 8 XCDs (WG b runs on XCD
+// b mod 8), each with its own L2.  This maps the hardware block index to a
+// logical one so that each XCD works on one contiguous 1/8 of the logical
+// range, keeping neighbouring work (and its shared cache lines) in one L2.
+#define PMX_NXCD 8
+__device__ __forceinline__ unsigned pmx_xcd_block(unsigned b, unsigned nb) {
+  unsigned q = nb / PMX_NXCD, r = nb % PMX_NXCD;
+  unsigned k = b % PMX_NXCD, i = b / PMX_NXCD;
+  return (k < r ? k * (q + 1) : r * (q + 1) + (k - r) * q) + i;
+}
+
 __device__ __forceinline__ int pmx_wrap(int i, int n) {
   int r = i % n;
   return r < 0 ? r + n : r;
