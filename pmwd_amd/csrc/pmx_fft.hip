@@ -16,6 +16,8 @@
 #define FFT_TPB 512                  // threads per FFT workgroup
 #define FFT_CAP 8192                 // complex values per workgroup
 #define FFT_MAXV (FFT_CAP / FFT_TPB) // values per thread
+#define RFFT_CAP 4096                // row passes: fewer values per thread
+#define RFFT_MAXV (RFFT_CAP / FFT_TPB)
 #define PFFT_CAP 4096                // Poisson x pass keeps per-thread state
 #define PFFT_MAXV (PFFT_CAP / FFT_TPB)
 
@@ -88,14 +90,21 @@ __device__ __forceinline__ void dftR(float2* v) {
 }
 
 // LDS layouts of P pencils of length n:
-//   ROWS (pencil-major):  element (p, j) at p*n + j
+//   ROWS (pencil-major):  element (p, j) at p*(n + n/8) + j + j/8; the gap
+//                         after every 8 elements spreads the stride-R stage
+//                         writes over the banks
 //   COLS (interleaved):   element (p, j) at j*P + p
 // Work items are mapped so that consecutive lanes touch consecutive words.
 enum { ROWS = 0, COLS = 1 };
 
 template <int LAY>
 __device__ __forceinline__ int lidx(int p, int j, int n, int P) {
-  return LAY == ROWS ? p * n + j : j * P + p;
+  return LAY == ROWS ? p * (n + (n >> 3)) + j + (j >> 3) : j * P + p;
+}
+
+// LDS elements needed for P pencils of length n in a layout
+__host__ __device__ inline int lds_elems(int lay, int n, int P) {
+  return lay == ROWS ? P * (n + (n >> 3)) : P * n;
 }
 
 // One Stockham stage of radix R (n a power of two, Ns the size of the
@@ -105,7 +114,7 @@ __device__ __forceinline__ int lidx(int p, int j, int n, int P) {
 // three table loads (w, w^2, w^4) with at most two extra products.
 template <int R, int DIR, int LAY, int MAXV, int TPB>
 __device__ void fft_stage(float2* s, int n, int P, int Ns, int lnb, int lP,
-                          const float2* __restrict__ tw) {
+                          const float2* __restrict__ tw, int tws) {
   const int nb = n / R;
   const int total = P * nb;
   const int step = n / (Ns * R);
@@ -118,7 +127,7 @@ __device__ void fft_stage(float2* s, int n, int P, int Ns, int lnb, int lP,
       int j = LAY == ROWS ? (idx & (nb - 1)) : (idx >> lP);
 #pragma unroll
       for (int r = 0; r < R; ++r) v[it * R + r] = s[lidx<LAY>(p, j + r * nb, n, P)];
-      int m = (j & (Ns - 1)) * step;
+      int m = (j & (Ns - 1)) * step * tws;
       if (m != 0) {
         float2* u = &v[it * R];
         float2 w1 = twiddle<DIR>(tw, m);
@@ -158,7 +167,7 @@ __device__ void fft_stage(float2* s, int n, int P, int Ns, int lnb, int lP,
 
 // Direct DFT of P pencils of any length n (P*n <= FFT_CAP).
 template <int DIR, int LAY, int MAXV, int TPB>
-__device__ void fft_direct(float2* s, int n, int P, const float2* __restrict__ tw) {
+__device__ void fft_direct(float2* s, int n, int P, const float2* __restrict__ tw, int tws) {
   float2 v[MAXV];
   const int total = P * n;
 #pragma unroll
@@ -170,7 +179,7 @@ __device__ void fft_direct(float2* s, int n, int P, const float2* __restrict__ t
       float2 acc = make_float2(0.f, 0.f);
       int m = 0;
       for (int j = 0; j < n; ++j) {
-        float2 w = twiddle<DIR>(tw, m);
+        float2 w = twiddle<DIR>(tw, m * tws);
         float2 x = s[lidx<LAY>(p, j, n, P)];
         acc.x += x.x * w.x - x.y * w.y;
         acc.y += x.x * w.y + x.y * w.x;
@@ -194,13 +203,14 @@ __device__ void fft_direct(float2* s, int n, int P, const float2* __restrict__ t
 }
 
 // Unnormalised in-LDS FFT of P pencils (exp(DIR 2 pi i jk/n)).  POW2
-// selects the Stockham path (n and P powers of two) or the direct DFT.
+// selects the Stockham path (n and P powers of two) or the direct DFT.  tw
+// is a table of exp(-2 pi i m/(n tws)); tws > 1 reuses a longer table.
 // Callers synchronise before it; it ends with a barrier.
 template <int DIR, int LAY, bool POW2, int MAXV = FFT_MAXV, int TPB = FFT_TPB>
-__device__ void lds_fft(float2* s, int n, int P, const float2* __restrict__ tw) {
+__device__ void lds_fft(float2* s, int n, int P, const float2* __restrict__ tw, int tws = 1) {
   if (n == 1) return;
   if (!POW2) {
-    fft_direct<DIR, LAY, MAXV, TPB>(s, n, P, tw);
+    fft_direct<DIR, LAY, MAXV, TPB>(s, n, P, tw, tws);
     return;
   }
   const int lP = __ffs(P) - 1;
@@ -208,13 +218,13 @@ __device__ void lds_fft(float2* s, int n, int P, const float2* __restrict__ tw) 
   while (Ns < n) {
     int rem = n / Ns;
     if (rem % 8 == 0) {
-      fft_stage<8, DIR, LAY, MAXV, TPB>(s, n, P, Ns, __ffs(n / 8) - 1, lP, tw);
+      fft_stage<8, DIR, LAY, MAXV, TPB>(s, n, P, Ns, __ffs(n / 8) - 1, lP, tw, tws);
       Ns *= 8;
     } else if (rem % 4 == 0) {
-      fft_stage<4, DIR, LAY, MAXV, TPB>(s, n, P, Ns, __ffs(n / 4) - 1, lP, tw);
+      fft_stage<4, DIR, LAY, MAXV, TPB>(s, n, P, Ns, __ffs(n / 4) - 1, lP, tw, tws);
       Ns *= 4;
     } else {
-      fft_stage<2, DIR, LAY, MAXV, TPB>(s, n, P, Ns, __ffs(n / 2) - 1, lP, tw);
+      fft_stage<2, DIR, LAY, MAXV, TPB>(s, n, P, Ns, __ffs(n / 2) - 1, lP, tw, tws);
       Ns *= 2;
     }
   }
@@ -230,12 +240,12 @@ static int fft_pencils(int n, int cap = FFT_CAP) {
 }
 
 // ------------------------------------------------------------ z passes --
-// In-place r2c of padded rows; PRE applies (x - 1) * coef on load
-// (gravity.py:52-54).
+// In-place r2c of padded rows as a length-n2 complex FFT of the real row
+// (any n2); PRE applies (x - 1) * coef on load (gravity.py:52-54).
 template <int PRE, bool POW2>
 __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_k(float* __restrict__ data, long long nrows, int n2,
-                                                   int rs, int P, float coef,
-                                                   const float2* __restrict__ tw) {
+                                                      int rs, int P, float coef,
+                                                      const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
   long long row0 = (long long)blockIdx.x * P;
   int np = (int)((nrows - row0) < P ? (nrows - row0) : P);
@@ -246,26 +256,26 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_k(float* __restrict__ data, 
       x = x - 1.f;
       x = x * coef;
     }
-    s[p * n2 + j] = make_float2(x, 0.f);
+    s[lidx<ROWS>(p, j, n2, P)] = make_float2(x, 0.f);
   }
   __syncthreads();
-  lds_fft<-1, ROWS, POW2>(s, n2, np, tw);
+  lds_fft<-1, ROWS, POW2, RFFT_MAXV>(s, n2, np, tw);
   const int n2h = n2 / 2 + 1;
   float2* out = reinterpret_cast<float2*>(data);
   const int rsc = rs / 2;
   for (int idx = threadIdx.x; idx < np * n2h; idx += FFT_TPB) {
     int p = idx / n2h, k = idx - (idx / n2h) * n2h;
-    out[(row0 + p) * rsc + k] = s[p * n2 + k];
+    out[(row0 + p) * rsc + k] = s[lidx<ROWS>(p, k, n2, P)];
   }
 }
 
-// In-place c2r of padded rows with a final scale.  The full spectrum is
-// rebuilt from the half (X[n-k] = conj X[k]) and the real part of the
-// inverse is kept, which discards Im X[0] and Im X[n/2] as irfft does.
+// In-place c2r of padded rows with a final scale (any n2).  The full
+// spectrum is rebuilt from the half (X[n-k] = conj X[k]) and the real part
+// of the inverse is kept, which discards Im X[0] and Im X[n/2] as irfft does.
 template <bool POW2>
 __global__ void __launch_bounds__(FFT_TPB, 4) z_c2r_k(float* __restrict__ data, long long nrows, int n2,
-                                                   int rs, int P, float scale,
-                                                   const float2* __restrict__ tw) {
+                                                      int rs, int P, float scale,
+                                                      const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
   long long row0 = (long long)blockIdx.x * P;
   int np = (int)((nrows - row0) < P ? (nrows - row0) : P);
@@ -277,13 +287,93 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_c2r_k(float* __restrict__ data, 
     int kk = k <= half ? k : n2 - k;
     float2 v = in[(row0 + p) * rsc + kk];
     if (k > half) v.y = -v.y;
-    s[p * n2 + k] = v;
+    s[lidx<ROWS>(p, k, n2, P)] = v;
   }
   __syncthreads();
-  lds_fft<1, ROWS, POW2>(s, n2, np, tw);
+  lds_fft<1, ROWS, POW2, RFFT_MAXV>(s, n2, np, tw);
   for (int idx = threadIdx.x; idx < np * n2; idx += FFT_TPB) {
     int p = idx / n2, j = idx - (idx / n2) * n2;
-    data[(row0 + p) * rs + j] = s[p * n2 + j].x * scale;
+    data[(row0 + p) * rs + j] = s[lidx<ROWS>(p, j, n2, P)].x * scale;
+  }
+}
+
+// Half-length r2c for even n2 with H = n2/2 a power of two: the row is read
+// as H complex values z[m] = x[2m] + i x[2m+1], Z = FFT_H(z), and for
+// k = 0..H:  X[k] = E + w^k O,  E = (Z[k] + conj Z[H-k])/2,
+// O = -i (Z[k] - conj Z[H-k])/2,  w = exp(-2 pi i/n2)  (Z[H] = Z[0]).
+// PRE applies (x - 1) * coef on load.
+template <int PRE>
+__global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_half_k(float* __restrict__ data, long long nrows,
+                                                           int n2, int rs, int P, float coef,
+                                                           const float2* __restrict__ tw) {
+  extern __shared__ float2 s[];
+  const int H = n2 >> 1;
+  const int lH = __ffs(H) - 1;
+  long long row0 = (long long)blockIdx.x * P;
+  int np = (int)((nrows - row0) < P ? (nrows - row0) : P);
+  float2* d2 = reinterpret_cast<float2*>(data);
+  const int rsc = rs / 2;
+  for (int idx = threadIdx.x; idx < np * H; idx += FFT_TPB) {
+    int p = idx >> lH, m = idx & (H - 1);
+    float2 v = d2[(row0 + p) * rsc + m];
+    if (PRE) {
+      v.x = v.x - 1.f;
+      v.x = v.x * coef;
+      v.y = v.y - 1.f;
+      v.y = v.y * coef;
+    }
+    s[lidx<ROWS>(p, m, H, P)] = v;
+  }
+  __syncthreads();
+  lds_fft<-1, ROWS, true, RFFT_MAXV>(s, H, np, tw, 2);
+  const int H1 = H + 1;
+  for (int idx = threadIdx.x; idx < np * H1; idx += FFT_TPB) {
+    int p = idx / H1, k = idx - (idx / H1) * H1;
+    float2 zk = s[lidx<ROWS>(p, k == H ? 0 : k, H, P)];
+    float2 zm = s[lidx<ROWS>(p, k == 0 ? 0 : H - k, H, P)];
+    zm.y = -zm.y;
+    float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
+    float2 dd = csub(zk, zm);
+    float2 o = make_float2(0.5f * dd.y, -0.5f * dd.x);
+    d2[(row0 + p) * rsc + k] = cadd(e, cmul(tw[k], o));
+  }
+}
+
+// Half-length c2r (inverse of the above) with a final scale:
+// Z[k] = (X[k] + conj X[H-k]) + i w^-k (X[k] - conj X[H-k]) for k < H with
+// Im X[0] = Im X[H] = 0 (as irfft), z = IFFT_H(Z), x[2m] + i x[2m+1] = z[m].
+__global__ void __launch_bounds__(FFT_TPB, 4) z_c2r_half_k(float* __restrict__ data, long long nrows,
+                                                           int n2, int rs, int P, float scale,
+                                                           const float2* __restrict__ tw) {
+  extern __shared__ float2 s[];
+  const int H = n2 >> 1;
+  const int lH = __ffs(H) - 1;
+  long long row0 = (long long)blockIdx.x * P;
+  int np = (int)((nrows - row0) < P ? (nrows - row0) : P);
+  float2* d2 = reinterpret_cast<float2*>(data);
+  const int rsc = rs / 2;
+  for (int idx = threadIdx.x; idx < np * H; idx += FFT_TPB) {
+    int p = idx >> lH, k = idx & (H - 1);
+    float2 a = d2[(row0 + p) * rsc + k];
+    float2 b = d2[(row0 + p) * rsc + (H - k)];
+    if (k == 0) {
+      a.y = 0.f;
+      b.y = 0.f;
+    }
+    b.y = -b.y;
+    float2 sum = cadd(a, b);
+    float2 dif = csub(a, b);
+    float2 w = tw[k];
+    w.y = -w.y;
+    float2 t = cmul(w, dif);
+    s[lidx<ROWS>(p, k, H, P)] = make_float2(sum.x - t.y, sum.y + t.x);
+  }
+  __syncthreads();
+  lds_fft<1, ROWS, true, RFFT_MAXV>(s, H, np, tw, 2);
+  for (int idx = threadIdx.x; idx < np * H; idx += FFT_TPB) {
+    int p = idx >> lH, m = idx & (H - 1);
+    float2 z = s[lidx<ROWS>(p, m, H, P)];
+    d2[(row0 + p) * rsc + m] = make_float2(z.x * scale, z.y * scale);
   }
 }
 
@@ -453,24 +543,46 @@ __global__ void copy_rows_k(float* __restrict__ dst, int dst_rs, const float* __
 }
 
 // ------------------------------------------------------------ drivers --
-static size_t lds_bytes(int n, int P) { return (size_t)P * n * sizeof(float2); }
+static size_t lds_bytes(int n, int P, int lay = COLS) {
+  return (size_t)lds_elems(lay, n, P) * sizeof(float2);
+}
+
+// rows per workgroup for the row passes: P*n <= RFFT_CAP and the swizzled
+// ROWS block within FFT_CAP elements (64 KB)
+static int row_pencils(int n) {
+  int P = 1;
+  while (P * 2 * n <= RFFT_CAP && lds_elems(ROWS, n, P * 2) <= FFT_CAP && P * 2 <= 256) P *= 2;
+  return P;
+}
+
+static bool half_ok(int n2) { return n2 >= 4 && (n2 & 1) == 0 && is_pow2(n2 / 2); }
 
 int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef) {
   long long nrows = (long long)c->n[0] * c->n[1];
   int n2 = c->n[2];
-  int P = fft_pencils(n2);
-  unsigned nblk = (unsigned)((nrows + P - 1) / P);
-  size_t lds = lds_bytes(n2, P);
-  bool p2 = is_pow2(n2);
   pmx_tic(c, PMX_T_ZR2C);
-  if (pre && p2)
-    z_r2c_k<1, true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
-  else if (pre)
-    z_r2c_k<1, false><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
-  else if (p2)
-    z_r2c_k<0, true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
-  else
-    z_r2c_k<0, false><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
+  if (half_ok(n2)) {
+    int P = row_pencils(n2 / 2);
+    unsigned nblk = (unsigned)((nrows + P - 1) / P);
+    size_t lds = lds_bytes(n2 / 2, P, ROWS);
+    if (pre)
+      z_r2c_half_k<1><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
+    else
+      z_r2c_half_k<0><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
+  } else {
+    int P = row_pencils(n2);
+    unsigned nblk = (unsigned)((nrows + P - 1) / P);
+    size_t lds = lds_bytes(n2, P, ROWS);
+    bool p2 = is_pow2(n2);
+    if (pre && p2)
+      z_r2c_k<1, true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
+    else if (pre)
+      z_r2c_k<1, false><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
+    else if (p2)
+      z_r2c_k<0, true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
+    else
+      z_r2c_k<0, false><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
+  }
   pmx_toc(c, PMX_T_ZR2C);
   PMX_LAUNCH_CHECK();
   return PMX_OK;
@@ -479,14 +591,21 @@ int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef) {
 int pmx_fft_c2r_inplace(pmx_ctx* c, float* buf, float scale) {
   long long nrows = (long long)c->n[0] * c->n[1];
   int n2 = c->n[2];
-  int P = fft_pencils(n2);
-  unsigned nblk = (unsigned)((nrows + P - 1) / P);
-  size_t lds = lds_bytes(n2, P);
   pmx_tic(c, PMX_T_ZC2R);
-  if (is_pow2(n2))
-    z_c2r_k<true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2]);
-  else
-    z_c2r_k<false><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2]);
+  if (half_ok(n2)) {
+    int P = row_pencils(n2 / 2);
+    unsigned nblk = (unsigned)((nrows + P - 1) / P);
+    z_c2r_half_k<<<nblk, FFT_TPB, lds_bytes(n2 / 2, P, ROWS), c->stream>>>(buf, nrows, n2, c->rs, P,
+                                                                        scale, c->tw[2]);
+  } else {
+    int P = row_pencils(n2);
+    unsigned nblk = (unsigned)((nrows + P - 1) / P);
+    size_t lds = lds_bytes(n2, P, ROWS);
+    if (is_pow2(n2))
+      z_c2r_k<true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2]);
+    else
+      z_c2r_k<false><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2]);
+  }
   pmx_toc(c, PMX_T_ZC2R);
   PMX_LAUNCH_CHECK();
   return PMX_OK;
