@@ -36,7 +36,7 @@ def parse():
     p.add_argument('--cpu-slab', type=int, default=64,
                    help='fraction 1/slab of the particles in the CPU baseline sample')
     p.add_argument('--copy-calib', action='store_true',
-                   help='also run a 1 GiB device copy (counter calibration for profiles)')
+                   help='also run 1 GiB device copies (counter calibration for profiles)')
     return p.parse_args()
 
 
@@ -44,24 +44,56 @@ def log(msg):
     print(f'[bench] {msg}', file=sys.stderr, flush=True)
 
 
-def group_bytes(N, M, Mh):
-    """Algorithmic bytes per step of each kernel group."""
+def kernel_bytes(N, mesh_shape, tile):
+    """HBM bytes per step of each timed kernel class: (algorithmic, by design).
+
+    Algorithmic bytes count each logical input read once and each output
+    written once: pmid 6 B (int16 x3) and disp/vel/acc 12 B per particle,
+    4 B per real and 8 B per complex (r2c layout) mesh cell.  Scatter is
+    18N + 4M, gather (3 components) 30N + 12M, kick or drift 36N; fused
+    kernels count shared operands once.  An FFT pass reads its input and
+    writes its output once.  Binning scratch (tile ranks, records, halo
+    shells) is not algorithmic; the second figure adds what the kernels move
+    by design: 16 B records, 4 B particle indices, 4 B tile and rank words,
+    the +1 halo shells.
+    """
+    n0, n1, n2 = mesh_shape
+    M = n0 * n1 * n2
+    Mh = n0 * n1 * (n2 // 2 + 1)
+    tx, ty, tz = tile
+    ntiles = -(-n0 // tx) * -(-n1 // ty) * -(-n2 // tz)
+    shell = (ty + 1) * (tz + 1) + tx * (tz + 1) + tx * ty
+    faces = ty * tz + (tx - 1) * tz + (tx - 1) * (ty - 1)
     return {
-        # fused kick+drift (read vel, acc, disp; write vel, disp) + CIC scatter
-        # (read pmid, write the density mesh)
-        'kick_drift_scatter': 60 * N + 6 * N + 4 * M,
-        # r2c (4M + 8Mh) + Poisson: Green, -ik, 3 c2r (8Mh + 12M)
-        'fft_poisson': 4 * M + 8 * Mh + 8 * Mh + 12 * M,
-        # 3-component gather (18N + 12M + 12N) fused with the closing kick
-        # (read and write vel, 24N)
-        'gather_kick': 18 * N + 12 * M + 12 * N + 24 * N,
+        # kick + drift (read vel, acc, disp; write vel, disp) + scatter's pmid
+        # read; by design also the tile and rank words
+        'bin': (66 * N, 74 * N),
+        'scan': (0, 8 * ntiles),
+        # read pmid, disp, tile, rank; write record + particle index
+        'fill': (0, 46 * N),
+        # write the density mesh; by design read records, write halo shells
+        'deposit': (4 * M, 16 * N + 4 * M + 4 * shell * ntiles),
+        'fold': (0, (4 * shell + 8 * faces) * ntiles),
+        'z_r2c': (4 * M + 8 * Mh,) * 2,
+        'y_fwd': (16 * Mh,) * 2,
+        # read one complex mesh, write three
+        'poisson_x': (32 * Mh,) * 2,
+        'y_inv': (48 * Mh,) * 2,
+        # three launches, each reads a complex and writes a real mesh
+        'z_c2r': (3 * (8 * Mh + 4 * M),) * 2,
+        # gather 30N + 12M and the closing kick (read and write vel);
+        # by design records + indices replace pmid + disp
+        'gather': (54 * N + 12 * M, 56 * N + 12 * M),
     }
 
 
 GROUPS = {
-    'kick_drift_scatter': ('bin', 'scan', 'fill', 'deposit', 'fold'),
-    'fft_poisson': ('z_r2c', 'y_fwd', 'poisson_x', 'y_inv', 'z_c2r'),
-    'gather_kick': ('gather',),
+    # kick+drift fused with the scatter (disp read once): 66N + 4M
+    'kick_drift_scatter': (('bin', 'scan', 'fill', 'deposit', 'fold'), (66, 4, 0)),
+    # 3D r2c 4M + 8Mh plus Poisson (Green, -ik, 3 c2r) 8Mh + 12M
+    'fft_poisson': (('z_r2c', 'y_fwd', 'poisson_x', 'y_inv', 'z_c2r'), (0, 16, 16)),
+    # 3-component gather fused with the closing kick: 54N + 12M
+    'gather_kick': (('gather',), (54, 12, 0)),
 }
 
 
@@ -141,9 +173,11 @@ def main():
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
         torch.cuda.synchronize()
         del src, dst
-        log('copy calibration: 3 x 1 GiB device copies')
+        log('copy calibration: 3 x 1 GiB runtime copies, 3 x 1 GiB elementwise neg')
 
     log(f'initial conditions {G}^3 (2LPT)')
     t = time.perf_counter()
@@ -199,25 +233,40 @@ def main():
     ms_per_step = 1e3 * t_fwd / K
     value = N * K / t_fwd
 
-    # kernel groups: time per step and algorithmic bytes per step
-    gb = group_bytes(N, M, Mh)
+    # per kernel class: time per step, launches, algorithmic bytes per launch
+    ts = (ctypes.c_int * 3)()
+    _lib.check(L.pmx_tile_shape(h, ctypes.byref(ts)))
+    kb = kernel_bytes(N, conf.mesh_shape, tuple(ts))
+    kernels = {}
+    for k, (ms_tot, cnt) in timing.items():
+        if not cnt:
+            continue
+        e = dict(ms_per_step=ms_tot / K, launches_per_step=cnt / K,
+                 ms_per_launch=ms_tot / cnt)
+        if k in kb:
+            alg, impl = kb[k]
+            e['alg_bytes_per_launch'] = alg * K / cnt
+            e['impl_bytes_per_launch'] = impl * K / cnt
+            e['alg_gbs'] = e['alg_bytes_per_launch'] / (e['ms_per_launch'] * 1e-3) / 1e9
+            e['impl_gbs'] = e['impl_bytes_per_launch'] / (e['ms_per_launch'] * 1e-3) / 1e9
+        kernels[k] = e
     groups = {}
-    for gname, classes in GROUPS.items():
-        ms = sum(timing[c][0] for c in classes) / K
-        groups[gname] = dict(ms=ms, bytes=gb[gname],
-                             gbs=gb[gname] / (ms * 1e-3) / 1e9 if ms > 0 else None)
+    for gname, (classes, (cn, cm, cmh)) in GROUPS.items():
+        ms = sum(kernels[c]['ms_per_step'] for c in classes if c in kernels)
+        b = cn * N + cm * M + cmh * Mh
+        groups[gname] = dict(ms=ms, alg_bytes=b, gbs=b / (ms * 1e-3) / 1e9 if ms > 0 else None)
         groups[gname]['frac'] = groups[gname]['gbs'] / HBM_PEAK_GBS if ms > 0 else None
-    kernels = {k: dict(ms_per_step=v[0] / K, launches_per_step=v[1] / K) for k, v in timing.items()
-               if v[1]}
-    dom = max(groups, key=lambda g: groups[g]['ms'])
+    dom = max((k for k in kernels if k in kb and kb[k][0]), key=lambda k: kernels[k]['ms_per_step'])
+    # whole KDK step: read pmid, disp, vel, acc, write disp, vel, acc (78N),
+    # density write+read (8M), r2c + 3 c2r complex traffic (16Mh + 24M)
     B_step = 78 * N + 8 * M + 16 * Mh + 24 * M
 
     traffic = None
     tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
-                         f'traffic_{dom}_{G}.json')
+                         f'traffic_{G}.json')
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get('bytes_per_step')
+            traffic = json.load(f).get('kernels', {}).get(dom, {}).get('hbm_bytes_per_launch')
 
     out = {
         'metric': 'particle-updates/s fwd & fwd+adj at 512^3 (1 GPU); p50/p99 mixed-size latency',
@@ -238,10 +287,10 @@ def main():
                    'ptcl_grid': G, 'mesh': conf.mesh_shape[0], 'box_mpc_h': args.box,
                    'parallelism': 'single GPU'},
         'roofline': {'bound': 'hbm', 'kernel': dom,
-                     'achieved': groups[dom]['gbs'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': groups[dom]['frac'], 'traffic': traffic,
-                     'algorithmic_bytes_per_step': groups[dom]['bytes'],
-                     'ms_per_step': groups[dom]['ms']},
+                     'achieved': kernels[dom]['alg_gbs'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': kernels[dom]['alg_gbs'] / HBM_PEAK_GBS, 'traffic': traffic,
+                     'algorithmic_bytes_per_launch': kernels[dom]['alg_bytes_per_launch'],
+                     'ms_per_launch': kernels[dom]['ms_per_launch']},
         'whole_step_roofline': {'bytes_per_step': B_step,
                                 'frac': B_step / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         'kernel_groups': groups,
@@ -265,6 +314,8 @@ def main():
         _lib.check(L.pmx_gravity_vjp(h, _lib.ptr(pmid), _lib.ptr(disp), N, val, coef, _lib.ptr(vc),
                                      _lib.ptr(acc), _lib.ptr(acc_cot), ctypes.byref(dot)))
         torch.cuda.synchronize()
+        _lib.check(L.pmx_timing_enable(h, 1))
+        _lib.timing_read(ctx)
         t0 = time.perf_counter()
         for i in range(K):
             ka, d, kb = adj_sched[i % nsteps_sched]
@@ -279,8 +330,12 @@ def main():
                                       _lib.ptr(acc_cot), _lib.ptr(vc), kb, ctypes.byref(dot)))
         torch.cuda.synchronize()
         t_adj = time.perf_counter() - t0
+        atiming = _lib.timing_read(ctx)
+        _lib.check(L.pmx_timing_enable(h, 0))
         out['fwd_adj_value'] = N * K / (t_fwd + t_adj)
         out['adj_ms_per_step'] = 1e3 * t_adj / K
+        out['adj_kernels'] = {k: dict(ms_per_step=v[0] / K, launches_per_step=v[1] / K)
+                              for k, v in atiming.items() if v[1]}
 
     if not args.no_cpu_baseline:
         log('cpu baseline sample (numpy oracle)')

@@ -166,12 +166,13 @@ int pmx_dot(pmx_ctx* ctx, const float* a, const float* b, long long n, double* o
 
 /* ------------------------------------------------------------- timing ---
  * Kernel timing with HIP events recorded on the context stream around each
- * launch of the hot path, grouped in PMX_NTIMER classes (bin, scan, fill,
- * deposit, fold, z-r2c, y-fwd, poisson-x, y-inv, z-c2r, gather, other).
+ * launch of the hot path, grouped in PMX_NTIMER_CLASSES classes (bin, scan,
+ * fill, deposit, fold, z-r2c, y-fwd, poisson-x, y-inv, z-c2r, gather,
+ * vjp-gather, grad-gather, kick/drift-adjoint, other).
  * pmx_timing_read synchronises the stream and returns, per class, the summed
  * milliseconds and the number of launches since the last read.  No pmwd
  * counterpart: measurement support.                                         */
-#define PMX_NTIMER_CLASSES 12
+#define PMX_NTIMER_CLASSES 15
 int pmx_timing_enable(pmx_ctx* ctx, int on);
 int pmx_timing_read(pmx_ctx* ctx, double* ms, long long* count);
 

@@ -64,7 +64,7 @@ _SIGS = {
 }
 
 TIMER_CLASSES = ('bin', 'scan', 'fill', 'deposit', 'fold', 'z_r2c', 'y_fwd', 'poisson_x',
-                 'y_inv', 'z_c2r', 'gather', 'other')
+                 'y_inv', 'z_c2r', 'gather', 'vjp_gather', 'grad_gather', 'kd_adj', 'other')
 
 
 def timing_read(ctx):

@@ -67,9 +67,9 @@ __global__ void fill_u32_k(uint32_t* __restrict__ p, long long n, uint32_t v) {
 static int fill_u32(pmx_ctx* c, uint32_t* p, long long n, uint32_t v) {
   unsigned nb = pmx_blocks(n);
   if (nb > 4096) nb = 4096;
-  pmx_tic(c, PMX_T_BIN);
+  pmx_tic(c, PMX_T_OTHER);
   fill_u32_k<<<nb, PMX_TPB, 0, c->stream>>>(p, n, v);
-  pmx_toc(c, PMX_T_BIN);
+  pmx_toc(c, PMX_T_OTHER);
   PMX_LAUNCH_CHECK();
   return PMX_OK;
 }
@@ -670,11 +670,11 @@ extern "C" int pmx_gravity_vjp(pmx_ctx* c, const int16_t* pmid, const float* dis
   int nblk = (int)pmx_blocks(n);
   if ((rc = pmx_ensure_red(c, nblk))) return rc;
   if (n > 0) {
-    pmx_tic(c, PMX_T_GATHER);
+    pmx_tic(c, PMX_T_VJPG);
     gather3_k<1><<<nblk, PMX_TPB, 0, c->stream>>>(
         pmid, disp, n, g, (const float*)c->cbuf[0], (const float*)c->cbuf[1],
         (const float*)c->cbuf[2], acc, vel_cot, acc_cot, c->red);
-    pmx_toc(c, PMX_T_GATHER);
+    pmx_toc(c, PMX_T_VJPG);
     PMX_LAUNCH_CHECK();
   }
   // the particles did not move: reuse the tile counts, refill the records
@@ -686,8 +686,10 @@ extern "C" int pmx_gravity_vjp(pmx_ctx* c, const int16_t* pmid, const float* dis
   // rho_cot = -coef * sum_d L_d(mesh_cot_d), L_d = irfftn . (-ik_d) . laplace . rfftn
   if ((rc = pmx_poisson_adj_ctx(c, (float)(-(double)coef / (double)c->M)))) return rc;
   if (n > 0) {
+    pmx_tic(c, PMX_T_GRADG);
     grad_gather1_k<<<nblk, PMX_TPB, 0, c->stream>>>(pmid, disp, n, g, (const float*)c->cbuf[0],
                                                     val, acc_cot);
+    pmx_toc(c, PMX_T_GRADG);
     PMX_LAUNCH_CHECK();
   }
   if (dot) {
@@ -870,7 +872,9 @@ extern "C" int pmx_kick_adj(pmx_ctx* c, long long n, float* vel, const float* ac
   int nblk = (int)pmx_blocks(3 * n);
   int rc;
   if ((rc = pmx_ensure_red(c, nblk))) return rc;
+  pmx_tic(c, PMX_T_KDADJ);
   kick_adj_k<<<nblk, PMX_TPB, 0, c->stream>>>(3 * n, vel, acc, disp_cot, acc_cot, vel_cot, k, c->red);
+  pmx_toc(c, PMX_T_KDADJ);
   PMX_LAUNCH_CHECK();
   if (dot) return pmx_red_finish(c, nblk, dot);
   return PMX_OK;
@@ -887,7 +891,9 @@ extern "C" int pmx_drift_adj(pmx_ctx* c, long long n, float* disp, const float* 
   int nblk = (int)pmx_blocks(3 * n);
   int rc;
   if ((rc = pmx_ensure_red(c, nblk))) return rc;
+  pmx_tic(c, PMX_T_KDADJ);
   drift_adj_k<<<nblk, PMX_TPB, 0, c->stream>>>(3 * n, disp, vel, disp_cot, vel_cot, d, c->red);
+  pmx_toc(c, PMX_T_KDADJ);
   PMX_LAUNCH_CHECK();
   if (dot) return pmx_red_finish(c, nblk, dot);
   return PMX_OK;

@@ -70,9 +70,13 @@ enum {
   PMX_T_YINV,        // y inverse pass
   PMX_T_ZC2R,        // z c2r row pass
   PMX_T_GATHER,      // 3-component gather (+ fused kick)
+  PMX_T_VJPG,        // gather with the gather-adjoint disp cotangent
+  PMX_T_GRADG,       // scatter-adjoint disp cotangent (weight gradients)
+  PMX_T_KDADJ,       // reverse-time kick/drift with cotangent updates
   PMX_T_OTHER,
   PMX_NTIMER
 };
+static_assert(PMX_NTIMER == PMX_NTIMER_CLASSES, "timer classes out of step with pmx.h");
 
 struct PmxTimer;
 void pmx_tic(pmx_ctx* c, int cls);
