@@ -307,27 +307,30 @@ def main():
         adj_sched = []
         a_rev = list(zip(a[:0:-1], a[-2::-1]))
         for a_prev, a_next in a_rev:
-            ah = a_prev * 0.5 + a_next * 0.5
+            ah = a_prev * (1 - 0.5) + a_next * 0.5
             adj_sched.append((nb._f32(nb.kick_factor(a_prev, a_prev, ah, cosmo, conf)),
                               nb._f32(nb.drift_factor(ah, a_prev, a_next, cosmo, conf)),
                               nb._f32(nb.kick_factor(a_next, ah, a_next, cosmo, conf))))
+        dots = (ctypes.c_double * 2)()
         _lib.check(L.pmx_gravity_vjp(h, _lib.ptr(pmid), _lib.ptr(disp), N, val, coef, _lib.ptr(vc),
                                      _lib.ptr(acc), _lib.ptr(acc_cot), ctypes.byref(dot)))
         torch.cuda.synchronize()
         _lib.check(L.pmx_timing_enable(h, 1))
         _lib.timing_read(ctx)
         t0 = time.perf_counter()
+        kp = 0.0
         for i in range(K):
             ka, d, kb = adj_sched[i % nsteps_sched]
-            _lib.check(L.pmx_kick_adj(h, N, _lib.ptr(vel), _lib.ptr(acc), _lib.ptr(dc),
-                                      _lib.ptr(acc_cot), _lib.ptr(vc), ka, ctypes.byref(dot)))
-            _lib.check(L.pmx_drift_adj(h, N, _lib.ptr(disp), _lib.ptr(vel), _lib.ptr(dc),
-                                       _lib.ptr(vc), d, ctypes.byref(dot)))
-            _lib.check(L.pmx_gravity_vjp(h, _lib.ptr(pmid), _lib.ptr(disp), N, val, coef,
-                                         _lib.ptr(vc), _lib.ptr(acc), _lib.ptr(acc_cot),
-                                         ctypes.byref(dot)))
-            _lib.check(L.pmx_kick_adj(h, N, _lib.ptr(vel), _lib.ptr(acc), _lib.ptr(dc),
-                                      _lib.ptr(acc_cot), _lib.ptr(vc), kb, ctypes.byref(dot)))
+            # one reverse KDK step (closing kick deferred into the next call),
+            # with the two float64 dots the Omega_m cotangent needs read back
+            # as nbody's adjoint does
+            _lib.check(L.pmx_nbody_step_adj(h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel),
+                                            _lib.ptr(acc), _lib.ptr(dc), _lib.ptr(vc),
+                                            _lib.ptr(acc_cot), N, val, coef, kp, ka, d, None,
+                                            dots))
+            kp = kb
+        _lib.check(L.pmx_kick_adj(h, N, _lib.ptr(vel), _lib.ptr(acc), _lib.ptr(dc),
+                                  _lib.ptr(acc_cot), _lib.ptr(vc), kp, None))
         torch.cuda.synchronize()
         t_adj = time.perf_counter() - t0
         atiming = _lib.timing_read(ctx)

@@ -161,6 +161,27 @@ int pmx_kick_adj(pmx_ctx* ctx, long long n, float* vel, const float* acc,
 int pmx_drift_adj(pmx_ctx* ctx, long long n, float* disp, const float* vel,
                   const float* disp_cot, float* vel_cot, float d, double* dot);
 
+/* One reverse-time KDK step of pmwd/nbody.py:143-162 (default symp_splits:
+ * kick_adj ka, drift_adj d, force_adj, kick_adj kb) with its closing kick
+ * left pending, in place:
+ *   [vel += acc*kp; disp_cot -= acc_cot*kp]  the previous step's closing
+ *                                            kick_adj, skipped when kp == 0;
+ *   vel += acc*ka; disp_cot -= acc_cot*ka; disp += vel*d; vel_cot -= disp_cot*d;
+ *   [disp = disp_snap when non-NULL: a stored forward state];
+ *   acc = gravity(disp); acc_cot = VJP_disp(gravity)[vel_cot].
+ * The caller passes this step's kb as the next call's kp, and applies the
+ * last one with pmx_kick_adj.  Each kick is rounded separately, so results
+ * equal the four separate calls bit for bit.  On entry acc/acc_cot hold the
+ * previous force and its VJP (pmx_gravity_vjp or the previous step).
+ * dots (host, may be NULL): dots[0] = sum(disp_cot*vel) of the drift
+ * adjoint, dots[1] = sum(vel_cot*acc) of the new force, which is also the
+ * dot of every kick adjoint that uses that force.                          */
+int pmx_nbody_step_adj(pmx_ctx* ctx, const int16_t* pmid, float* disp,
+                       float* vel, float* acc, float* disp_cot, float* vel_cot,
+                       float* acc_cot, long long n, float val, float coef,
+                       float kp, float ka, float d, const float* disp_snap,
+                       double* dots);
+
 /* float64 dot product of two float32 arrays of length n (host value).      */
 int pmx_dot(pmx_ctx* ctx, const float* a, const float* b, long long n, double* out);
 

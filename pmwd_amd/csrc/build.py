@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, 'libpmx.so')
-SOURCES = ['pmx_ctx.hip', 'pmx_fft.hip', 'pmx_cic.hip', 'pmx_generic.hip', 'pmx_sort.hip']
+SOURCES = ['pmx_ctx.hip', 'pmx_fft.hip', 'pmx_cic.hip', 'pmx_adjoint.hip', 'pmx_generic.hip',
+           'pmx_sort.hip']
 
 HIPCC = os.environ.get('HIPCC', shutil.which('hipcc') or '/opt/rocm/bin/hipcc')
 

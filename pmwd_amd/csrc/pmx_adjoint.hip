@@ -1,0 +1,398 @@
+// Reverse-time adjoint of the KDK step (gfx950): the force VJP
+// (pmwd/nbody.py:108-118 over gravity.py:47-72, i.e. gather.py:80-144 and
+// scatter.py:86-150 around the adjoint Poisson solve), the reverse kick and
+// drift (nbody.py:49-99), and one fused reverse step (nbody.py:143-162)
+// whose closing kick is deferred into the next step's opening kernel, where
+// it streams in particle order.
+//
+// Everything per particle runs over the tile-binned records: one workgroup
+// per tile stages the meshes it reads (tile + 1 layer) in LDS, reads its
+// records in order and writes each particle's results to its index.  Sums
+// for the cosmology cotangent are float64 per-block partials reduced on the
+// device in a fixed order.
+#include "pmx_internal.h"
+
+#include <math.h>
+
+#define ADJ_TPB 512
+
+// Fixed-order block sum of s (wave butterflies, then wave sums in order)
+// stored to part[blockIdx.x].
+template <int TPB>
+__device__ __forceinline__ void block_sum_store(double s, double* __restrict__ part) {
+  __shared__ double wsum[TPB / 64];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (__lane_id() == 0) wsum[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < TPB / 64; ++w) t += wsum[w];
+    part[blockIdx.x] = t;
+  }
+}
+
+// Running max of |v| per channel: wave max, then one atomic per wave only
+// when it raises the stored maximum (non-negative floats order like their
+// bit patterns).
+__device__ __forceinline__ void wave_max_store(float m, float* vmax) {
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if (__lane_id() == 0) {
+    unsigned* vm = reinterpret_cast<unsigned*>(vmax);
+    if (__float_as_uint(m) > __atomic_load_n(vm, __ATOMIC_RELAXED)) atomicMax(vm, __float_as_uint(m));
+  }
+}
+
+// ------------------------------------------------ tile-staged VJP gather --
+// Per record (particle p) with the three force meshes F staged in LDS:
+//   acc_p     = sum_j F[c_j] w_j                         (gather, tree order)
+//   acc_cot_p = sum_j (sum_d v_d F_d[c_j]) grad w_j / cell, v = vel_cot_p
+//               (the gather adjoint's disp cotangent, gather.py:80-116)
+// and the record's three vel_cot components are staged channel-major in
+// vrec (slot order) for the cotangent deposit, with their |max| in vmax.
+// part[block] = sum over the tile of vel_cot_p . acc_p (float64).
+__global__ void __launch_bounds__(ADJ_TPB)
+    tile_vjp_gather_k(const float4* __restrict__ rec, const uint32_t* __restrict__ pidx,
+                      const uint32_t* __restrict__ tile_start,
+                      const uint32_t* __restrict__ tile_count, MeshGeom g, TileGeom tg,
+                      const float* __restrict__ F0, const float* __restrict__ F1,
+                      const float* __restrict__ F2, const float* __restrict__ vel_cot,
+                      float* __restrict__ acc, float* __restrict__ acc_cot,
+                      float* __restrict__ vrec, long long vstride, float* __restrict__ vmax,
+                      double* __restrict__ part) {
+  extern __shared__ float sh[];
+  int o[3], s[3], ti[3];
+  const int t = (int)pmx_xcd_block(blockIdx.x, gridDim.x);
+  tile_origin(t, tg, g, o, s, ti);
+  const int LY = s[1] + 1, LZ = s[2] + 1;
+  const int ltot = (s[0] + 1) * LY * LZ;
+  const float* F[3] = {F0, F1, F2};
+  pmx_stage_tile<3, ADJ_TPB>(sh, o, LY, LZ, ltot, g, F);
+  __syncthreads();
+  const uint32_t start = tile_start[t];
+  const uint32_t cnt = tile_count[t];
+  double dotv = 0;
+  float vm0 = 0.f, vm1 = 0.f, vm2 = 0.f;
+  for (uint32_t q = threadIdx.x; q < cnt; q += ADJ_TPB) {
+    const float4 r = rec[start + q];
+    const long long p = pidx[start + q];
+    int lx, ly, lz;
+    pmx_unpack_local(r.w, lx, ly, lz);
+    if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
+    float wa[2], wb[2], wc[2], sa[2], sb[2], sc[2];
+    frac_sign_of(r.x, wa[0], wa[1], sa[0], sa[1]);
+    frac_sign_of(r.y, wb[0], wb[1], sb[0], sb[1]);
+    frac_sign_of(r.z, wc[0], wc[1], sc[0], sc[1]);
+    const int b = (lx * LY + ly) * LZ + lz;
+    const float v0 = vel_cot[3 * p + 0], v1 = vel_cot[3 * p + 1], v2 = vel_cot[3 * p + 2];
+    float r0[8], r1[8], r2[8], gx[8], gy[8], gz[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int ja = j & 1, jb = (j >> 1) & 1, jc = (j >> 2) & 1;
+      const float f0 = wa[ja], f1 = wb[jb], f2 = wc[jc];
+      const float w = (f0 * f1) * f2;
+      const int li = b + (ja ? LY * LZ : 0) + (jb ? LZ : 0) + jc;
+      const float a0 = sh[li], a1 = sh[ltot + li], a2 = sh[2 * ltot + li];
+      r0[j] = a0 * w;
+      r1[j] = a1 * w;
+      r2[j] = a2 * w;
+      const float dc = v0 * a0 + v1 * a1 + v2 * a2;
+      gx[j] = dc * (sa[ja] * (f1 * f2));
+      gy[j] = dc * (sb[jb] * (f2 * f0));
+      gz[j] = dc * (sc[jc] * (f0 * f1));
+    }
+    const float o0 = pmx_tree8(r0), o1 = pmx_tree8(r1), o2 = pmx_tree8(r2);
+    acc[3 * p + 0] = o0;
+    acc[3 * p + 1] = o1;
+    acc[3 * p + 2] = o2;
+    acc_cot[3 * p + 0] = pmx_tree8(gx) / g.cell;
+    acc_cot[3 * p + 1] = pmx_tree8(gy) / g.cell;
+    acc_cot[3 * p + 2] = pmx_tree8(gz) / g.cell;
+    vrec[start + q] = v0;
+    vrec[vstride + start + q] = v1;
+    vrec[2 * vstride + start + q] = v2;
+    vm0 = fmaxf(vm0, fabsf(v0));
+    vm1 = fmaxf(vm1, fabsf(v1));
+    vm2 = fmaxf(vm2, fabsf(v2));
+    dotv += (double)v0 * o0 + (double)v1 * o1 + (double)v2 * o2;
+  }
+  wave_max_store(vm0, vmax + 0);
+  wave_max_store(vm1, vmax + 1);
+  wave_max_store(vm2, vmax + 2);
+  block_sum_store<ADJ_TPB>(dotv, part);
+}
+
+// ----------------------------------------- tile-staged gradient gather --
+// Per record with the density cotangent R staged in LDS (scatter adjoint,
+// scatter.py:86-121, scalar val):
+//   acc_cot_p += sum_j (R[c_j] val) grad w_j / cell
+__global__ void __launch_bounds__(ADJ_TPB)
+    tile_grad_gather_k(const float4* __restrict__ rec, const uint32_t* __restrict__ pidx,
+                       const uint32_t* __restrict__ tile_start,
+                       const uint32_t* __restrict__ tile_count, MeshGeom g, TileGeom tg,
+                       const float* __restrict__ R, float val, float* __restrict__ acc_cot) {
+  extern __shared__ float sh[];
+  int o[3], s[3], ti[3];
+  const int t = (int)pmx_xcd_block(blockIdx.x, gridDim.x);
+  tile_origin(t, tg, g, o, s, ti);
+  const int LY = s[1] + 1, LZ = s[2] + 1;
+  const int ltot = (s[0] + 1) * LY * LZ;
+  const float* F[1] = {R};
+  pmx_stage_tile<1, ADJ_TPB>(sh, o, LY, LZ, ltot, g, F);
+  __syncthreads();
+  const uint32_t start = tile_start[t];
+  const uint32_t cnt = tile_count[t];
+  for (uint32_t q = threadIdx.x; q < cnt; q += ADJ_TPB) {
+    const float4 r = rec[start + q];
+    const long long p = pidx[start + q];
+    int lx, ly, lz;
+    pmx_unpack_local(r.w, lx, ly, lz);
+    if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
+    float wa[2], wb[2], wc[2], sa[2], sb[2], sc[2];
+    frac_sign_of(r.x, wa[0], wa[1], sa[0], sa[1]);
+    frac_sign_of(r.y, wb[0], wb[1], sb[0], sb[1]);
+    frac_sign_of(r.z, wc[0], wc[1], sc[0], sc[1]);
+    const int b = (lx * LY + ly) * LZ + lz;
+    float gx[8], gy[8], gz[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int ja = j & 1, jb = (j >> 1) & 1, jc = (j >> 2) & 1;
+      const float f0 = wa[ja], f1 = wb[jb], f2 = wc[jc];
+      const int li = b + (ja ? LY * LZ : 0) + (jb ? LZ : 0) + jc;
+      const float dc = sh[li] * val;
+      gx[j] = dc * (sa[ja] * (f1 * f2));
+      gy[j] = dc * (sb[jb] * (f2 * f0));
+      gz[j] = dc * (sc[jc] * (f0 * f1));
+    }
+    acc_cot[3 * p + 0] = acc_cot[3 * p + 0] + pmx_tree8(gx) / g.cell;
+    acc_cot[3 * p + 1] = acc_cot[3 * p + 1] + pmx_tree8(gy) / g.cell;
+    acc_cot[3 * p + 2] = acc_cot[3 * p + 2] + pmx_tree8(gz) / g.cell;
+  }
+}
+
+// With the particles binned for the current disp and the force meshes in
+// c->cbuf: acc and acc_cot (force VJP of vel_cot) and per-tile partials of
+// vel_cot.acc at c->red + red_off (ntiles of them).
+static int force_vjp_binned(pmx_ctx* c, long long n, float coef, float val, const float* vel_cot,
+                            float* acc, float* acc_cot, long long red_off) {
+  int rc;
+  if ((rc = pmx_ensure_vrec(c, n))) return rc;
+  if ((rc = pmx_fill_u32(c, reinterpret_cast<uint32_t*>(c->vmax), 3, 0u))) return rc;
+  const MeshGeom g = pmx_geom_padded(c);
+  const TileGeom& tg = c->tg;
+  const unsigned nt = (unsigned)c->ntiles;
+  const size_t cells = (size_t)(tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1);
+  pmx_tic(c, PMX_T_VJPG);
+  tile_vjp_gather_k<<<nt, ADJ_TPB, 3 * cells * sizeof(float), c->stream>>>(
+      c->rec, c->pidx, c->tile_start, c->tile_count, g, tg, (const float*)c->cbuf[0],
+      (const float*)c->cbuf[1], (const float*)c->cbuf[2], vel_cot, acc, acc_cot, c->vrec, n,
+      c->vmax, c->red + red_off);
+  pmx_toc(c, PMX_T_VJPG);
+  PMX_LAUNCH_CHECK();
+  // the cotangent of each force mesh: scatter(vel_cot_d * w)
+  for (int d = 0; d < 3; ++d)
+    if ((rc = pmx_deposit_binned(c, 0.f, d, n, (float*)c->cbuf[d], c->rs, 0))) return rc;
+  // density cotangent -coef * sum_d L_d(mesh_cot_d), L_d = irfftn (-ik_d) laplace rfftn
+  if ((rc = pmx_poisson_adj_ctx(c, (float)(-(double)coef / (double)c->M)))) return rc;
+  pmx_tic(c, PMX_T_GRADG);
+  tile_grad_gather_k<<<nt, ADJ_TPB, cells * sizeof(float), c->stream>>>(
+      c->rec, c->pidx, c->tile_start, c->tile_count, g, tg, (const float*)c->cbuf[0], val,
+      acc_cot);
+  pmx_toc(c, PMX_T_GRADG);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+extern "C" int pmx_gravity_vjp(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
+                               float val, float coef, const float* vel_cot, float* acc,
+                               float* acc_cot, double* dot) {
+  PMX_REQUIRE(c && n >= 0, "pmx_gravity_vjp: bad argument");
+  PMX_REQUIRE(n == 0 || (pmid && disp && acc && acc_cot && vel_cot), "pmx_gravity_vjp: NULL pointer");
+  PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_gravity_vjp: too many particles");
+  int rc;
+  if (n == 0) {
+    if (dot) *dot = 0;
+    return PMX_OK;
+  }
+  if ((rc = pmx_ensure_red(c, c->ntiles))) return rc;
+  if ((rc = pmx_gravity_fwd_meshes(c, pmid, disp, n, val, coef))) return rc;
+  if ((rc = force_vjp_binned(c, n, coef, val, vel_cot, acc, acc_cot, 0))) return rc;
+  if (!dot) return PMX_OK;
+  if ((rc = pmx_red_reduce(c, 0, c->ntiles, 0))) return rc;
+  return pmx_red_fetch(c, 1, dot);
+}
+
+// ------------------------------------------------ reverse kick / drift --
+// kick_adj (nbody.py:80-99): vel += acc*k; disp_cot -= acc_cot*k;
+// partials of vel_cot.acc
+__global__ void __launch_bounds__(PMX_TPB)
+    kick_adj_k(long long n3, float* __restrict__ vel, const float* __restrict__ acc,
+               float* __restrict__ disp_cot, const float* __restrict__ acc_cot,
+               const float* __restrict__ vel_cot, float k, double* __restrict__ part) {
+  double s = 0;
+  for (long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x; i < n3;
+       i += (long long)gridDim.x * PMX_TPB) {
+    const float a = acc[i];
+    vel[i] = vel[i] + a * k;
+    disp_cot[i] = disp_cot[i] - acc_cot[i] * k;
+    s += (double)vel_cot[i] * a;
+  }
+  block_sum_store<PMX_TPB>(s, part);
+}
+
+// drift_adj (nbody.py:49-67): disp += vel*d; vel_cot -= disp_cot*d;
+// partials of disp_cot.vel
+__global__ void __launch_bounds__(PMX_TPB)
+    drift_adj_k(long long n3, float* __restrict__ disp, const float* __restrict__ vel,
+                const float* __restrict__ disp_cot, float* __restrict__ vel_cot, float d,
+                double* __restrict__ part) {
+  double s = 0;
+  for (long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x; i < n3;
+       i += (long long)gridDim.x * PMX_TPB) {
+    const float v = vel[i];
+    const float dc = disp_cot[i];
+    disp[i] = disp[i] + v * d;
+    vel_cot[i] = vel_cot[i] - dc * d;
+    s += (double)dc * v;
+  }
+  block_sum_store<PMX_TPB>(s, part);
+}
+
+#define ADJ_GRID_CAP 65536
+
+extern "C" int pmx_kick_adj(pmx_ctx* c, long long n, float* vel, const float* acc, float* disp_cot,
+                            const float* acc_cot, const float* vel_cot, float k, double* dot) {
+  PMX_REQUIRE(c && n >= 0, "pmx_kick_adj: bad argument");
+  PMX_REQUIRE(n == 0 || (vel && acc && disp_cot && acc_cot && vel_cot), "pmx_kick_adj: NULL pointer");
+  if (n == 0) {
+    if (dot) *dot = 0;
+    return PMX_OK;
+  }
+  const unsigned nblk = pmx_blocks_capped(3 * n, ADJ_GRID_CAP);
+  int rc;
+  if ((rc = pmx_ensure_red(c, nblk))) return rc;
+  pmx_tic(c, PMX_T_KDADJ);
+  kick_adj_k<<<nblk, PMX_TPB, 0, c->stream>>>(3 * n, vel, acc, disp_cot, acc_cot, vel_cot, k, c->red);
+  pmx_toc(c, PMX_T_KDADJ);
+  PMX_LAUNCH_CHECK();
+  if (dot) return pmx_red_finish(c, nblk, dot);
+  return PMX_OK;
+}
+
+extern "C" int pmx_drift_adj(pmx_ctx* c, long long n, float* disp, const float* vel,
+                             const float* disp_cot, float* vel_cot, float d, double* dot) {
+  PMX_REQUIRE(c && n >= 0, "pmx_drift_adj: bad argument");
+  PMX_REQUIRE(n == 0 || (disp && vel && disp_cot && vel_cot), "pmx_drift_adj: NULL pointer");
+  if (n == 0) {
+    if (dot) *dot = 0;
+    return PMX_OK;
+  }
+  const unsigned nblk = pmx_blocks_capped(3 * n, ADJ_GRID_CAP);
+  int rc;
+  if ((rc = pmx_ensure_red(c, nblk))) return rc;
+  pmx_tic(c, PMX_T_KDADJ);
+  drift_adj_k<<<nblk, PMX_TPB, 0, c->stream>>>(3 * n, disp, vel, disp_cot, vel_cot, d, c->red);
+  pmx_toc(c, PMX_T_KDADJ);
+  PMX_LAUNCH_CHECK();
+  if (dot) return pmx_red_finish(c, nblk, dot);
+  return PMX_OK;
+}
+
+// ------------------------------------------------- fused reverse step --
+// Reverse kicks and drift fused with the tile count of the new positions:
+//   [vel += acc*kp; disp_cot -= acc_cot*kp]   (HAS_KP: the previous step's
+//                                             closing kick_adj, still pending)
+//   vel += acc*ka; disp_cot -= acc_cot*ka; disp += vel*d; vel_cot -= disp_cot*d
+// (disp replaced by snap when given, a stored checkpoint), with per-block
+// partials of disp_cot.vel (the drift's cosmology term).  A kick's own
+// vel_cot.acc equals the dot of the force VJP that produced acc.
+template <int HAS_KP>
+__global__ void __launch_bounds__(PMX_TPB)
+    kd_adj_bin_k(const int16_t* __restrict__ pmid, float* __restrict__ disp, float* __restrict__ vel,
+                 const float* __restrict__ acc, float* __restrict__ disp_cot,
+                 float* __restrict__ vel_cot, const float* __restrict__ acc_cot,
+                 const float* __restrict__ snap, long long n, float kp, float ka, float d,
+                 MeshGeom g, TileGeom tg, uint32_t* __restrict__ tile_count,
+                 uint32_t* __restrict__ ptile, uint32_t* __restrict__ prank,
+                 double* __restrict__ part) {
+  const long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  const bool active = i < n;
+  double s = 0;
+  uint32_t t = 0;
+  if (active) {
+    int pm[3];
+    float dp[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      pm[a] = pmid[3 * i + a];
+      const float ac = acc[3 * i + a], acot = acc_cot[3 * i + a];
+      float v = vel[3 * i + a];
+      float dc = disp_cot[3 * i + a];
+      if (HAS_KP) {
+        v = v + ac * kp;
+        dc = dc - acot * kp;
+      }
+      v = v + ac * ka;
+      dc = dc - acot * ka;
+      float x = disp[3 * i + a];
+      x = x + v * d;
+      float vc = vel_cot[3 * i + a];
+      vc = vc - dc * d;
+      s += (double)dc * v;
+      if (snap) x = snap[3 * i + a];
+      vel[3 * i + a] = v;
+      disp_cot[3 * i + a] = dc;
+      disp[3 * i + a] = x;
+      vel_cot[3 * i + a] = vc;
+      dp[a] = x;
+    }
+    CicAxis ax[3];
+    cic_setup(pm, dp, g, ax);
+    t = tile_of(ax, tg);
+  }
+  const uint32_t r = agg_slot(tile_count, t, active);
+  if (active) {
+    ptile[i] = t;
+    prank[i] = r;
+  }
+  block_sum_store<PMX_TPB>(s, part);
+}
+
+extern "C" int pmx_nbody_step_adj(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel,
+                                  float* acc, float* disp_cot, float* vel_cot, float* acc_cot,
+                                  long long n, float val, float coef, float kp, float ka, float d,
+                                  const float* disp_snap, double* dots) {
+  PMX_REQUIRE(c && n >= 0, "pmx_nbody_step_adj: bad argument");
+  PMX_REQUIRE(n == 0 || (pmid && disp && vel && acc && disp_cot && vel_cot && acc_cot),
+              "pmx_nbody_step_adj: NULL pointer");
+  PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_nbody_step_adj: too many particles");
+  if (n == 0) {
+    if (dots) dots[0] = dots[1] = 0;
+    return PMX_OK;
+  }
+  int rc;
+  if ((rc = pmx_ensure_ptcl(c, n))) return rc;
+  const unsigned nb1 = pmx_blocks(n);
+  if ((rc = pmx_ensure_red(c, (long long)nb1 + c->ntiles))) return rc;
+  if ((rc = pmx_fill_u32(c, c->tile_count, c->ntiles + 1, 0u))) return rc;
+  const MeshGeom g = pmx_geom_padded(c);
+  pmx_tic(c, PMX_T_KDADJ);
+  if (kp != 0.f)
+    kd_adj_bin_k<1><<<nb1, PMX_TPB, 0, c->stream>>>(pmid, disp, vel, acc, disp_cot, vel_cot,
+                                                    acc_cot, disp_snap, n, kp, ka, d, g, c->tg,
+                                                    c->tile_count, c->pkey, c->prank, c->red);
+  else
+    kd_adj_bin_k<0><<<nb1, PMX_TPB, 0, c->stream>>>(pmid, disp, vel, acc, disp_cot, vel_cot,
+                                                    acc_cot, disp_snap, n, 0.f, ka, d, g, c->tg,
+                                                    c->tile_count, c->pkey, c->prank, c->red);
+  pmx_toc(c, PMX_T_KDADJ);
+  PMX_LAUNCH_CHECK();
+  if ((rc = pmx_scan_tiles(c))) return rc;
+  if ((rc = pmx_bin_fill(c, pmid, disp, n, nullptr, 0, 0))) return rc;
+  if ((rc = pmx_deposit_binned(c, val, -1, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
+  if ((rc = pmx_poisson_ctx(c, coef))) return rc;
+  if ((rc = force_vjp_binned(c, n, coef, val, vel_cot, acc, acc_cot, nb1))) return rc;
+  if (!dots) return PMX_OK;
+  if ((rc = pmx_red_reduce(c, 0, nb1, 0))) return rc;
+  if ((rc = pmx_red_reduce(c, nb1, c->ntiles, 1))) return rc;
+  return pmx_red_fetch(c, 2, dots);
+}

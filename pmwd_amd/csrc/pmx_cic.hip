@@ -6,57 +6,11 @@
 // particles into an LDS copy of the tile plus a +1 halo shell and writes the
 // tile interior with plain stores and the shell to a halo buffer; a fold
 // pass adds each tile's incoming shells.  No float atomics touch global
-// memory on this path.
+// memory on this path.  The tile-staged gather and the forward KDK step
+// live here too; the reverse-time adjoint is in pmx_adjoint.hip.
 #include "pmx_internal.h"
 
 #include <math.h>
-
-int pmx_poisson_ctx(pmx_ctx* c, float coef);
-int pmx_poisson_adj_ctx(pmx_ctx* c, float scale_out);
-int pmx_red_finish(pmx_ctx* c, int nblk, double* out);
-
-// ----------------------------------------------------------- helpers --
-__device__ __forceinline__ void load_ptcl(const int16_t* __restrict__ pmid,
-                                          const float* __restrict__ disp, long long i,
-                                          int pm[3], float dp[3]) {
-  pm[0] = pmid[3 * i + 0];
-  pm[1] = pmid[3 * i + 1];
-  pm[2] = pmid[3 * i + 2];
-  dp[0] = disp[3 * i + 0];
-  dp[1] = disp[3 * i + 1];
-  dp[2] = disp[3 * i + 2];
-}
-
-__device__ __forceinline__ void cic_setup(const int pm[3], const float dp[3], const MeshGeom& g,
-                                          CicAxis ax[3]) {
-  ax[0] = pmx_cic_axis(pm[0], dp[0], g.cell, g.n0);
-  ax[1] = pmx_cic_axis(pm[1], dp[1], g.cell, g.n1);
-  ax[2] = pmx_cic_axis(pm[2], dp[2], g.cell, g.n2);
-}
-
-__device__ __forceinline__ uint32_t tile_of(const CicAxis ax[3], const TileGeom& tg) {
-  return ((uint32_t)(ax[0].lo / tg.tx) * tg.nty + (uint32_t)(ax[1].lo / tg.ty)) * tg.ntz +
-         (uint32_t)(ax[2].lo / tg.tz);
-}
-
-// Wave-aggregated atomicAdd(&ctr[key], 1): returns this lane's slot.
-// Every lane of the wave must call it (inactive lanes pass active=false).
-__device__ __forceinline__ uint32_t agg_slot(uint32_t* __restrict__ ctr, uint32_t key, bool active) {
-  unsigned long long pending = __ballot(active);
-  const int lane = __lane_id();
-  uint32_t slot = 0;
-  while (pending) {
-    int leader = __ffsll((long long)pending) - 1;
-    uint32_t lkey = __shfl(key, leader);
-    unsigned long long peers = __ballot(active && key == lkey) & pending;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&ctr[lkey], (uint32_t)__popcll(peers));
-    base = __shfl(base, leader);
-    if (active && key == lkey) slot = base + (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
-    pending &= ~peers;
-  }
-  return slot;
-}
 
 __global__ void fill_u32_k(uint32_t* __restrict__ p, long long n, uint32_t v) {
   for (long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x; i < n;
@@ -64,7 +18,7 @@ __global__ void fill_u32_k(uint32_t* __restrict__ p, long long n, uint32_t v) {
     p[i] = v;
 }
 
-static int fill_u32(pmx_ctx* c, uint32_t* p, long long n, uint32_t v) {
+int pmx_fill_u32(pmx_ctx* c, uint32_t* p, long long n, uint32_t v) {
   unsigned nb = pmx_blocks(n);
   if (nb > 4096) nb = 4096;
   pmx_tic(c, PMX_T_OTHER);
@@ -186,26 +140,6 @@ __global__ void bin_fill_k(const int16_t* __restrict__ pmid, const float* __rest
       if (__float_as_uint(m) > __atomic_load_n(vm, __ATOMIC_RELAXED)) atomicMax(vm, __float_as_uint(m));
     }
   }
-}
-
-// per-axis fractions from d = disp/cell exactly as pmx_cic_axis forms them
-__device__ __forceinline__ void frac_of(float d, float& w0, float& w1) {
-  int i = (int)floorf(d);
-  w0 = 1.f - fabsf(d - (float)i);
-  w1 = 1.f - fabsf(d - (float)(i + 1));
-}
-
-__device__ __forceinline__ void tile_origin(int t, const TileGeom& tg, const MeshGeom& g, int o[3],
-                                            int s[3], int ti[3]) {
-  ti[2] = t % tg.ntz;
-  ti[1] = (t / tg.ntz) % tg.nty;
-  ti[0] = t / (tg.ntz * tg.nty);
-  o[0] = ti[0] * tg.tx;
-  o[1] = ti[1] * tg.ty;
-  o[2] = ti[2] * tg.tz;
-  s[0] = min(tg.tx, g.n0 - o[0]);
-  s[1] = min(tg.ty, g.n1 - o[1]);
-  s[2] = min(tg.tz, g.n2 - o[2]);
 }
 
 __device__ __forceinline__ int shell_index(int qx, int qy, int qz, int sx, int sy, int sz) {
@@ -337,19 +271,8 @@ __global__ void __launch_bounds__(TG_TPB)
   tile_origin(t, tg, g, o, s, ti);
   const int LY = s[1] + 1, LZ = s[2] + 1;
   const int ltot = (s[0] + 1) * LY * LZ;
-  for (int i = threadIdx.x; i < ltot; i += TG_TPB) {
-    int qz = i % LZ;
-    int qy = (i / LZ) % LY;
-    int qx = i / (LZ * LY);
-    int x = o[0] + qx, y = o[1] + qy, z = o[2] + qz;
-    if (x >= g.n0) x -= g.n0;
-    if (y >= g.n1) y -= g.n1;
-    if (z >= g.n2) z -= g.n2;
-    long long m = ((long long)x * g.n1 + y) * g.rs + z;
-    sh[i] = F0[m];
-    sh[ltot + i] = F1[m];
-    sh[2 * ltot + i] = F2[m];
-  }
+  const float* F[3] = {F0, F1, F2};
+  pmx_stage_tile<3, TG_TPB>(sh, o, LY, LZ, ltot, g, F);
   __syncthreads();
   const uint32_t start = tile_start[t];
   const uint32_t cnt = tile_count[t];
@@ -434,7 +357,7 @@ __global__ void __launch_bounds__(PMX_TPB)
 
 // Writes the binned records (and nch per-particle values, channel-major, to
 // c->vrec) once the tile counts are scanned.
-static int bin_fill(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
+int pmx_bin_fill(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
                     const float* pval, int vstride, int nch) {
   if (n == 0) return PMX_OK;
   if (nch > 0) {
@@ -443,7 +366,7 @@ static int bin_fill(pmx_ctx* c, const int16_t* pmid, const float* disp, long lon
   }
   MeshGeom g = pmx_geom_padded(c);
   if (nch > 0) {
-    int rc = fill_u32(c, reinterpret_cast<uint32_t*>(c->vmax), 3, 0u);
+    int rc = pmx_fill_u32(c, reinterpret_cast<uint32_t*>(c->vmax), 3, 0u);
     if (rc) return rc;
   }
   pmx_tic(c, PMX_T_FILL);
@@ -455,7 +378,7 @@ static int bin_fill(pmx_ctx* c, const int16_t* pmid, const float* disp, long lon
   return PMX_OK;
 }
 
-static int scan_tiles(pmx_ctx* c) {
+int pmx_scan_tiles(pmx_ctx* c) {
   pmx_tic(c, PMX_T_SCAN);
   int rc = pmx_scan_exclusive(c, c->tile_count, c->tile_start, c->ntiles + 1);
   pmx_toc(c, PMX_T_SCAN);
@@ -463,10 +386,10 @@ static int scan_tiles(pmx_ctx* c) {
 }
 
 // Bins the particles by tile (c->rec, c->tile_start, c->tile_count).
-static int bin_particles(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n) {
+int pmx_bin_particles(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n) {
   int rc;
   if ((rc = pmx_ensure_ptcl(c, n))) return rc;
-  if ((rc = fill_u32(c, c->tile_count, c->ntiles + 1, 0u))) return rc;
+  if ((rc = pmx_fill_u32(c, c->tile_count, c->ntiles + 1, 0u))) return rc;
   MeshGeom g = pmx_geom_padded(c);
   if (n > 0) {
     pmx_tic(c, PMX_T_BIN);
@@ -475,13 +398,13 @@ static int bin_particles(pmx_ctx* c, const int16_t* pmid, const float* disp, lon
     pmx_toc(c, PMX_T_BIN);
     PMX_LAUNCH_CHECK();
   }
-  if ((rc = scan_tiles(c))) return rc;
-  return bin_fill(c, pmid, disp, n, nullptr, 0, 0);
+  if ((rc = pmx_scan_tiles(c))) return rc;
+  return pmx_bin_fill(c, pmid, disp, n, nullptr, 0, 0);
 }
 
 // Deposits the binned records into mesh (row stride mrs), with the scalar
 // value val or the per-record values of channel ch of c->vrec (ch >= 0).
-static int deposit_binned(pmx_ctx* c, float val, int ch, long long n, float* mesh, int mrs,
+int pmx_deposit_binned(pmx_ctx* c, float val, int ch, long long n, float* mesh, int mrs,
                           int accumulate) {
   MeshGeom g = pmx_geom_padded(c);
   const TileGeom& tg = c->tg;
@@ -511,114 +434,13 @@ extern "C" int pmx_scatter_mass(pmx_ctx* c, const int16_t* pmid, const float* di
   PMX_REQUIRE(n == 0 || (pmid && disp), "pmx_scatter_mass: NULL particle arrays");
   PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_scatter_mass: too many particles");
   int rc;
-  if ((rc = bin_particles(c, pmid, disp, n))) return rc;
-  return deposit_binned(c, val, -1, n, mesh, c->n[2], accumulate);
-}
-
-// ----------------------------------------------------------- gather3 --
-// acc_p = sum_j w_j F_d[c_j] for the three force meshes (padded rows).
-// With vcot != NULL also alpha_p = sum_j grad w_j (sum_d v_d F_d[c_j]) / cell
-// (the gather adjoint's disp cotangent, gather.py:80-116) and per-block
-// float64 partial sums of v.acc.
-template <int VJP>
-__global__ void __launch_bounds__(PMX_TPB)
-    gather3_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp, long long n,
-              MeshGeom g, const float* __restrict__ F0, const float* __restrict__ F1,
-              const float* __restrict__ F2, float* __restrict__ acc,
-              const float* __restrict__ vcot, float* __restrict__ alpha,
-              double* __restrict__ part) {
-  long long i = (long long)pmx_xcd_block(blockIdx.x, gridDim.x) * PMX_TPB + threadIdx.x;
-  double dotv = 0;
-  if (i < n) {
-    int pm[3];
-    float dp[3];
-    load_ptcl(pmid, disp, i, pm, dp);
-    CicAxis ax[3];
-    cic_setup(pm, dp, g, ax);
-    float w[8], a0[8], a1[8], a2[8];
-    long long idx[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      idx[j] = pmx_cell_index(ax, j, g.n1, g.rs);
-      w[j] = pmx_w(ax, j);
-    }
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      a0[j] = F0[idx[j]];
-      a1[j] = F1[idx[j]];
-      a2[j] = F2[idx[j]];
-    }
-    float r0[8], r1[8], r2[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      r0[j] = a0[j] * w[j];
-      r1[j] = a1[j] * w[j];
-      r2[j] = a2[j] * w[j];
-    }
-    float o0 = pmx_tree8(r0), o1 = pmx_tree8(r1), o2 = pmx_tree8(r2);
-    acc[3 * i + 0] = o0;
-    acc[3 * i + 1] = o1;
-    acc[3 * i + 2] = o2;
-    if (VJP) {
-      float v0 = vcot[3 * i + 0], v1 = vcot[3 * i + 1], v2 = vcot[3 * i + 2];
-      float gx[8], gy[8], gz[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        float dc = v0 * a0[j] + v1 * a1[j] + v2 * a2[j];
-        float gr[3];
-        pmx_wgrad(ax, j, gr);
-        gx[j] = dc * gr[0];
-        gy[j] = dc * gr[1];
-        gz[j] = dc * gr[2];
-      }
-      alpha[3 * i + 0] = pmx_tree8(gx) / g.cell;
-      alpha[3 * i + 1] = pmx_tree8(gy) / g.cell;
-      alpha[3 * i + 2] = pmx_tree8(gz) / g.cell;
-      dotv = (double)v0 * o0 + (double)v1 * o1 + (double)v2 * o2;
-    }
-  }
-  if (VJP) {
-    __shared__ double sh[PMX_TPB];
-    sh[threadIdx.x] = dotv;
-    __syncthreads();
-    for (int o = PMX_TPB / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
-  }
-}
-
-// alpha_p += val * sum_j grad w_j rho_cot[c_j] / cell (scatter adjoint,
-// scatter.py:86-121, one channel, scalar val).
-__global__ void __launch_bounds__(PMX_TPB)
-    grad_gather1_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp, long long n,
-                   MeshGeom g, const float* __restrict__ R, float val, float* __restrict__ alpha) {
-  long long i = (long long)pmx_xcd_block(blockIdx.x, gridDim.x) * PMX_TPB + threadIdx.x;
-  if (i >= n) return;
-  int pm[3];
-  float dp[3];
-  load_ptcl(pmid, disp, i, pm, dp);
-  CicAxis ax[3];
-  cic_setup(pm, dp, g, ax);
-  float gx[8], gy[8], gz[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    float dc = R[pmx_cell_index(ax, j, g.n1, g.rs)] * val;
-    float gr[3];
-    pmx_wgrad(ax, j, gr);
-    gx[j] = dc * gr[0];
-    gy[j] = dc * gr[1];
-    gz[j] = dc * gr[2];
-  }
-  alpha[3 * i + 0] += pmx_tree8(gx) / g.cell;
-  alpha[3 * i + 1] += pmx_tree8(gy) / g.cell;
-  alpha[3 * i + 2] += pmx_tree8(gz) / g.cell;
+  if ((rc = pmx_bin_particles(c, pmid, disp, n))) return rc;
+  return pmx_deposit_binned(c, val, -1, n, mesh, c->n[2], accumulate);
 }
 
 // Tile-staged gather of the force meshes in c->cbuf (padded) for the
 // binned particles; with vel also the closing kick.
-static int tile_gather(pmx_ctx* c, float* acc, float* vel, float k2) {
+int pmx_tile_gather(pmx_ctx* c, float* acc, float* vel, float k2) {
   MeshGeom g = pmx_geom_padded(c);
   const TileGeom& tg = c->tg;
   size_t lds = (size_t)3 * (tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(float);
@@ -639,11 +461,11 @@ static int tile_gather(pmx_ctx* c, float* acc, float* vel, float k2) {
 }
 
 // ----------------------------------------------------------- gravity --
-static int gravity_fwd_meshes(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
+int pmx_gravity_fwd_meshes(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
                               float val, float coef) {
   int rc;
-  if ((rc = bin_particles(c, pmid, disp, n))) return rc;
-  if ((rc = deposit_binned(c, val, -1, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
+  if ((rc = pmx_bin_particles(c, pmid, disp, n))) return rc;
+  if ((rc = pmx_deposit_binned(c, val, -1, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
   return pmx_poisson_ctx(c, coef);
 }
 
@@ -653,53 +475,9 @@ extern "C" int pmx_gravity(pmx_ctx* c, const int16_t* pmid, const float* disp, l
   PMX_REQUIRE(n == 0 || (pmid && disp && acc), "pmx_gravity: NULL pointer");
   PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_gravity: too many particles");
   int rc;
-  if ((rc = gravity_fwd_meshes(c, pmid, disp, n, val, coef))) return rc;
+  if ((rc = pmx_gravity_fwd_meshes(c, pmid, disp, n, val, coef))) return rc;
   if (n == 0) return PMX_OK;
-  return tile_gather(c, acc, nullptr, 0.f);
-}
-
-extern "C" int pmx_gravity_vjp(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
-                               float val, float coef, const float* vel_cot, float* acc,
-                               float* acc_cot, double* dot) {
-  PMX_REQUIRE(c && n >= 0, "pmx_gravity_vjp: bad argument");
-  PMX_REQUIRE(n == 0 || (pmid && disp && acc && acc_cot && vel_cot), "pmx_gravity_vjp: NULL pointer");
-  PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_gravity_vjp: too many particles");
-  int rc;
-  if ((rc = gravity_fwd_meshes(c, pmid, disp, n, val, coef))) return rc;
-  MeshGeom g = pmx_geom_padded(c);
-  int nblk = (int)pmx_blocks(n);
-  if ((rc = pmx_ensure_red(c, nblk))) return rc;
-  if (n > 0) {
-    pmx_tic(c, PMX_T_VJPG);
-    gather3_k<1><<<nblk, PMX_TPB, 0, c->stream>>>(
-        pmid, disp, n, g, (const float*)c->cbuf[0], (const float*)c->cbuf[1],
-        (const float*)c->cbuf[2], acc, vel_cot, acc_cot, c->red);
-    pmx_toc(c, PMX_T_VJPG);
-    PMX_LAUNCH_CHECK();
-  }
-  // the particles did not move: reuse the tile counts, refill the records
-  // with the three cotangent components, and deposit one mesh per component
-  if ((rc = bin_fill(c, pmid, disp, n, vel_cot, 3, 3))) return rc;
-  for (int d = 0; d < 3; ++d)
-    if ((rc = deposit_binned(c, 0.f, d, n, (float*)c->cbuf[d], c->rs, 0)))
-      return rc;
-  // rho_cot = -coef * sum_d L_d(mesh_cot_d), L_d = irfftn . (-ik_d) . laplace . rfftn
-  if ((rc = pmx_poisson_adj_ctx(c, (float)(-(double)coef / (double)c->M)))) return rc;
-  if (n > 0) {
-    pmx_tic(c, PMX_T_GRADG);
-    grad_gather1_k<<<nblk, PMX_TPB, 0, c->stream>>>(pmid, disp, n, g, (const float*)c->cbuf[0],
-                                                    val, acc_cot);
-    pmx_toc(c, PMX_T_GRADG);
-    PMX_LAUNCH_CHECK();
-  }
-  if (dot) {
-    if (n == 0) {
-      *dot = 0;
-    } else if ((rc = pmx_red_finish(c, nblk, dot))) {
-      return rc;
-    }
-  }
-  return PMX_OK;
+  return pmx_tile_gather(c, acc, nullptr, 0.f);
 }
 
 // ------------------------------------------------------ time stepping --
@@ -758,40 +536,6 @@ __global__ void kick_drift_bin_k(const int16_t* __restrict__ pmid, float* __rest
   }
 }
 
-// gather of the three force meshes fused with the closing kick (k2)
-__global__ void __launch_bounds__(PMX_TPB)
-    gather3_kick_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp, long long n,
-                   MeshGeom g, const float* __restrict__ F0, const float* __restrict__ F1,
-                   const float* __restrict__ F2, float* __restrict__ acc, float* __restrict__ vel,
-                   float k2) {
-  long long i = (long long)pmx_xcd_block(blockIdx.x, gridDim.x) * PMX_TPB + threadIdx.x;
-  if (i >= n) return;
-  int pm[3];
-  float dp[3];
-  load_ptcl(pmid, disp, i, pm, dp);
-  CicAxis ax[3];
-  cic_setup(pm, dp, g, ax);
-  float w[8], r0[8], r1[8], r2[8];
-  long long idx[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    idx[j] = pmx_cell_index(ax, j, g.n1, g.rs);
-    w[j] = pmx_w(ax, j);
-  }
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    r0[j] = F0[idx[j]] * w[j];
-    r1[j] = F1[idx[j]] * w[j];
-    r2[j] = F2[idx[j]] * w[j];
-  }
-  float o[3] = {pmx_tree8(r0), pmx_tree8(r1), pmx_tree8(r2)};
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    acc[3 * i + a] = o[a];
-    vel[3 * i + a] = vel[3 * i + a] + o[a] * k2;
-  }
-}
-
 extern "C" int pmx_nbody_step(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, float* acc,
                               long long n, float val, float coef, float k1, float d, float k2) {
   PMX_REQUIRE(c && n >= 0, "pmx_nbody_step: bad argument");
@@ -799,7 +543,7 @@ extern "C" int pmx_nbody_step(pmx_ctx* c, const int16_t* pmid, float* disp, floa
   PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_nbody_step: too many particles");
   int rc;
   if ((rc = pmx_ensure_ptcl(c, n))) return rc;
-  if ((rc = fill_u32(c, c->tile_count, c->ntiles + 1, 0u))) return rc;
+  if ((rc = pmx_fill_u32(c, c->tile_count, c->ntiles + 1, 0u))) return rc;
   MeshGeom g = pmx_geom_padded(c);
   if (n > 0) {
     pmx_tic(c, PMX_T_BIN);
@@ -808,93 +552,10 @@ extern "C" int pmx_nbody_step(pmx_ctx* c, const int16_t* pmid, float* disp, floa
     pmx_toc(c, PMX_T_BIN);
     PMX_LAUNCH_CHECK();
   }
-  if ((rc = scan_tiles(c))) return rc;
-  if ((rc = bin_fill(c, pmid, disp, n, nullptr, 0, 0))) return rc;
-  if ((rc = deposit_binned(c, val, -1, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
+  if ((rc = pmx_scan_tiles(c))) return rc;
+  if ((rc = pmx_bin_fill(c, pmid, disp, n, nullptr, 0, 0))) return rc;
+  if ((rc = pmx_deposit_binned(c, val, -1, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
   if ((rc = pmx_poisson_ctx(c, coef))) return rc;
-  if (n > 0 && (rc = tile_gather(c, acc, vel, k2))) return rc;
-  return PMX_OK;
-}
-
-// kick_adj: vel += acc*k; disp_cot -= acc_cot*k; partial sums of vel_cot.acc
-__global__ void kick_adj_k(long long n3, float* __restrict__ vel, const float* __restrict__ acc,
-                           float* __restrict__ disp_cot, const float* __restrict__ acc_cot,
-                           const float* __restrict__ vel_cot, float k, double* __restrict__ part) {
-  __shared__ double sh[PMX_TPB];
-  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
-  double s = 0;
-  if (i < n3) {
-    float a = acc[i];
-    vel[i] = vel[i] + a * k;
-    disp_cot[i] = disp_cot[i] - acc_cot[i] * k;
-    s = (double)vel_cot[i] * a;
-  }
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int o = PMX_TPB / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
-}
-
-// drift_adj: disp += vel*d; vel_cot -= disp_cot*d; partial sums of disp_cot.vel
-__global__ void drift_adj_k(long long n3, float* __restrict__ disp, const float* __restrict__ vel,
-                            const float* __restrict__ disp_cot, float* __restrict__ vel_cot,
-                            float d, double* __restrict__ part) {
-  __shared__ double sh[PMX_TPB];
-  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
-  double s = 0;
-  if (i < n3) {
-    float v = vel[i];
-    float dc = disp_cot[i];
-    disp[i] = disp[i] + v * d;
-    vel_cot[i] = vel_cot[i] - dc * d;
-    s = (double)dc * v;
-  }
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int o = PMX_TPB / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
-}
-
-extern "C" int pmx_kick_adj(pmx_ctx* c, long long n, float* vel, const float* acc, float* disp_cot,
-                            const float* acc_cot, const float* vel_cot, float k, double* dot) {
-  PMX_REQUIRE(c && n >= 0, "pmx_kick_adj: bad argument");
-  PMX_REQUIRE(n == 0 || (vel && acc && disp_cot && acc_cot && vel_cot), "pmx_kick_adj: NULL pointer");
-  if (n == 0) {
-    if (dot) *dot = 0;
-    return PMX_OK;
-  }
-  int nblk = (int)pmx_blocks(3 * n);
-  int rc;
-  if ((rc = pmx_ensure_red(c, nblk))) return rc;
-  pmx_tic(c, PMX_T_KDADJ);
-  kick_adj_k<<<nblk, PMX_TPB, 0, c->stream>>>(3 * n, vel, acc, disp_cot, acc_cot, vel_cot, k, c->red);
-  pmx_toc(c, PMX_T_KDADJ);
-  PMX_LAUNCH_CHECK();
-  if (dot) return pmx_red_finish(c, nblk, dot);
-  return PMX_OK;
-}
-
-extern "C" int pmx_drift_adj(pmx_ctx* c, long long n, float* disp, const float* vel,
-                             const float* disp_cot, float* vel_cot, float d, double* dot) {
-  PMX_REQUIRE(c && n >= 0, "pmx_drift_adj: bad argument");
-  PMX_REQUIRE(n == 0 || (disp && vel && disp_cot && vel_cot), "pmx_drift_adj: NULL pointer");
-  if (n == 0) {
-    if (dot) *dot = 0;
-    return PMX_OK;
-  }
-  int nblk = (int)pmx_blocks(3 * n);
-  int rc;
-  if ((rc = pmx_ensure_red(c, nblk))) return rc;
-  pmx_tic(c, PMX_T_KDADJ);
-  drift_adj_k<<<nblk, PMX_TPB, 0, c->stream>>>(3 * n, disp, vel, disp_cot, vel_cot, d, c->red);
-  pmx_toc(c, PMX_T_KDADJ);
-  PMX_LAUNCH_CHECK();
-  if (dot) return pmx_red_finish(c, nblk, dot);
+  if (n > 0 && (rc = pmx_tile_gather(c, acc, vel, k2))) return rc;
   return PMX_OK;
 }

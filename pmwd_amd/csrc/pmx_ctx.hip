@@ -132,7 +132,13 @@ extern "C" int pmx_ctx_create(int n0, int n1, int n2, double cell_size, void* st
   if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->tile_start, (size_t)(c->ntiles + 1) * 4);
   if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->halo, (size_t)c->ntiles * c->tg.shell * 4);
   if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->vmax, 3 * sizeof(float));
-  if (rc == PMX_OK) rc |= pmx_ensure_red(c, 1024);
+  if (rc == PMX_OK) rc |= pmx_ensure_red(c, 4096);
+  if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->red_out, PMX_RED_SLOTS * sizeof(double));
+  if (rc == PMX_OK && hipHostMalloc((void**)&c->red_host, PMX_RED_SLOTS * sizeof(double),
+                                    hipHostMallocDefault) != hipSuccess) {
+    pmx_set_error("pmx_ctx_create: hipHostMalloc failed");
+    rc = PMX_ENOMEM;
+  }
   if (rc != PMX_OK) {
     pmx_ctx_destroy(c);
     return rc;
@@ -160,7 +166,7 @@ extern "C" int pmx_ctx_destroy(pmx_ctx* c) {
     if (c->cbuf[a]) hipFree(c->cbuf[a]);
   }
   void* ps[] = {c->tile_count, c->tile_start, c->halo, c->rec, c->pkey, c->prank, c->pidx, c->vrec, c->vmax, c->red,
-                c->sort_k[0], c->sort_k[1], c->sort_v[0], c->sort_v[1], c->sort_hist,
+                c->red_out, c->sort_k[0], c->sort_k[1], c->sort_v[0], c->sort_v[1], c->sort_hist,
                 c->scan_tmp};
   for (void* p : ps)
     if (p) hipFree(p);
@@ -210,17 +216,11 @@ int pmx_ensure_vrec(pmx_ctx* c, long long n) {
   return PMX_OK;
 }
 
-int pmx_ensure_red(pmx_ctx* c, int nblk) {
+int pmx_ensure_red(pmx_ctx* c, long long nblk) {
   if (nblk <= c->red_cap) return PMX_OK;
   dev_free(c, (void**)&c->red, (size_t)c->red_cap * 8);
-  if (c->red_host) hipHostFree(c->red_host);
-  c->red_host = nullptr;
   c->red_cap = 0;
   if (dev_alloc(c, (void**)&c->red, (size_t)nblk * 8)) return PMX_ENOMEM;
-  if (hipHostMalloc((void**)&c->red_host, (size_t)nblk * 8, hipHostMallocDefault) != hipSuccess) {
-    pmx_set_error("hipHostMalloc failed");
-    return PMX_ENOMEM;
-  }
   c->red_cap = nblk;
   return PMX_OK;
 }
@@ -335,14 +335,44 @@ __global__ void dot_k(const float* __restrict__ a, const float* __restrict__ b, 
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-int pmx_red_finish(pmx_ctx* c, int nblk, double* out) {
-  PMX_CHECK_HIP(hipMemcpyAsync(c->red_host, c->red, (size_t)nblk * 8, hipMemcpyDeviceToHost,
+// One workgroup sums part[0, nblk) in a fixed order (strided per thread,
+// then a tree), so the result does not depend on timing.
+#define RED_TPB 1024
+__global__ void __launch_bounds__(RED_TPB)
+    red_sum_k(const double* __restrict__ part, long long nblk, double* __restrict__ out) {
+  __shared__ double sh[RED_TPB];
+  double s = 0;
+  for (long long i = threadIdx.x; i < nblk; i += RED_TPB) s += part[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = RED_TPB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
+int pmx_red_reduce(pmx_ctx* c, long long off, long long nblk, int slot) {
+  PMX_REQUIRE(slot >= 0 && slot < PMX_RED_SLOTS && off >= 0 && off + nblk <= c->red_cap,
+              "pmx_red_reduce: bad range");
+  red_sum_k<<<1, RED_TPB, 0, c->stream>>>(c->red + off, nblk, c->red_out + slot);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+int pmx_red_fetch(pmx_ctx* c, int k, double* out) {
+  PMX_REQUIRE(k > 0 && k <= PMX_RED_SLOTS, "pmx_red_fetch: bad count");
+  PMX_CHECK_HIP(hipMemcpyAsync(c->red_host, c->red_out, (size_t)k * 8, hipMemcpyDeviceToHost,
                                c->stream));
   PMX_CHECK_HIP(hipStreamSynchronize(c->stream));
-  double s = 0;
-  for (int i = 0; i < nblk; ++i) s += c->red_host[i];
-  *out = s;
+  for (int i = 0; i < k; ++i) out[i] = c->red_host[i];
   return PMX_OK;
+}
+
+int pmx_red_finish(pmx_ctx* c, long long nblk, double* out) {
+  int rc = pmx_red_reduce(c, 0, nblk, 0);
+  if (rc) return rc;
+  return pmx_red_fetch(c, 1, out);
 }
 
 extern "C" int pmx_dot(pmx_ctx* c, const float* a, const float* b, long long n, double* out) {

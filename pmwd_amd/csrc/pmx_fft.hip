@@ -111,38 +111,50 @@ __host__ __device__ inline int lds_elems(int lay, int n, int P) {
 // sub-transforms already done): reads x[j + r n/R], twiddles by
 // w_{Ns R}^{(j mod Ns) r}, and writes the radix-R DFT to
 // (j - j mod Ns) R + j mod Ns + r Ns.  Twiddle powers are built from at most
-// three table loads (w, w^2, w^4) with at most two extra products.
+// three table reads (w, w^2, w^4) with at most two extra products; all reads
+// of a stage are issued before the arithmetic.
 template <int R, int DIR, int LAY, int MAXV, int TPB>
-__device__ void fft_stage(float2* s, int n, int P, int Ns, int lnb, int lP,
-                          const float2* __restrict__ tw, int tws) {
+__device__ __forceinline__ void fft_stage(float2* s, int n, int P, int Ns, int lnb, int lP,
+                                          const float2* __restrict__ tw, int tws) {
+  constexpr int NG = MAXV / R;
   const int nb = n / R;
   const int total = P * nb;
   const int step = n / (Ns * R);
   float2 v[MAXV];
+  float2 w1[NG], w2[NG], w4[NG];
+  int mm[NG];
 #pragma unroll
-  for (int it = 0; it < MAXV / R; ++it) {
+  for (int it = 0; it < NG; ++it) {
     int idx = threadIdx.x + it * TPB;
+    mm[it] = 0;
     if (idx < total) {
       int p = LAY == ROWS ? (idx >> lnb) : (idx & (P - 1));
       int j = LAY == ROWS ? (idx & (nb - 1)) : (idx >> lP);
+      int m = (j & (Ns - 1)) * step * tws;
+      mm[it] = m;
+      w1[it] = twiddle<DIR>(tw, m);
+      if (R >= 4) w2[it] = twiddle<DIR>(tw, 2 * m);
+      if (R == 8) w4[it] = twiddle<DIR>(tw, 4 * m);
 #pragma unroll
       for (int r = 0; r < R; ++r) v[it * R + r] = s[lidx<LAY>(p, j + r * nb, n, P)];
-      int m = (j & (Ns - 1)) * step * tws;
-      if (m != 0) {
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < NG; ++it) {
+    int idx = threadIdx.x + it * TPB;
+    if (idx < total) {
+      if (mm[it] != 0) {
         float2* u = &v[it * R];
-        float2 w1 = twiddle<DIR>(tw, m);
-        u[1] = cmul(u[1], w1);
+        u[1] = cmul(u[1], w1[it]);
         if (R >= 4) {
-          float2 w2 = twiddle<DIR>(tw, 2 * m);
-          float2 w3 = cmul(w2, w1);
-          u[2] = cmul(u[2], w2);
+          float2 w3 = cmul(w2[it], w1[it]);
+          u[2] = cmul(u[2], w2[it]);
           u[3] = cmul(u[3], w3);
           if (R == 8) {
-            float2 w4 = twiddle<DIR>(tw, 4 * m);
-            u[4] = cmul(u[4], w4);
-            u[5] = cmul(u[5], cmul(w4, w1));
-            u[6] = cmul(u[6], cmul(w4, w2));
-            u[7] = cmul(u[7], cmul(w4, w3));
+            u[4] = cmul(u[4], w4[it]);
+            u[5] = cmul(u[5], cmul(w4[it], w1[it]));
+            u[6] = cmul(u[6], cmul(w4[it], w2[it]));
+            u[7] = cmul(u[7], cmul(w4[it], w3));
           }
         }
       }
@@ -151,7 +163,7 @@ __device__ void fft_stage(float2* s, int n, int P, int Ns, int lnb, int lP,
   }
   __syncthreads();
 #pragma unroll
-  for (int it = 0; it < MAXV / R; ++it) {
+  for (int it = 0; it < NG; ++it) {
     int idx = threadIdx.x + it * TPB;
     if (idx < total) {
       int p = LAY == ROWS ? (idx >> lnb) : (idx & (P - 1));
@@ -167,7 +179,8 @@ __device__ void fft_stage(float2* s, int n, int P, int Ns, int lnb, int lP,
 
 // Direct DFT of P pencils of any length n (P*n <= FFT_CAP).
 template <int DIR, int LAY, int MAXV, int TPB>
-__device__ void fft_direct(float2* s, int n, int P, const float2* __restrict__ tw, int tws) {
+__device__ __forceinline__ void fft_direct(float2* s, int n, int P, const float2* __restrict__ tw,
+                                           int tws) {
   float2 v[MAXV];
   const int total = P * n;
 #pragma unroll
@@ -207,7 +220,8 @@ __device__ void fft_direct(float2* s, int n, int P, const float2* __restrict__ t
 // is a table of exp(-2 pi i m/(n tws)); tws > 1 reuses a longer table.
 // Callers synchronise before it; it ends with a barrier.
 template <int DIR, int LAY, bool POW2, int MAXV = FFT_MAXV, int TPB = FFT_TPB>
-__device__ void lds_fft(float2* s, int n, int P, const float2* __restrict__ tw, int tws = 1) {
+__device__ __forceinline__ void lds_fft(float2* s, int n, int P, const float2* __restrict__ tw,
+                                        int tws = 1) {
   if (n == 1) return;
   if (!POW2) {
     fft_direct<DIR, LAY, MAXV, TPB>(s, n, P, tw, tws);
@@ -239,34 +253,76 @@ static int fft_pencils(int n, int cap = FFT_CAP) {
   return P;
 }
 
+// -------------------------------------------------- batched transfers --
+// Global <-> LDS traffic goes through registers in unrolled batches of up to
+// MAXV values per thread, so a thread's loads are all in flight at once
+// instead of waiting out the memory latency one element at a time.
+template <int MAXV, int TPB, class LD, class ST>
+__device__ __forceinline__ void batch_copy(int total, LD ld, ST st) {
+  float2 r[MAXV];
+#pragma unroll
+  for (int it = 0; it < MAXV; ++it) {
+    int idx = threadIdx.x + it * TPB;
+    if (idx < total) r[it] = ld(idx);
+  }
+#pragma unroll
+  for (int it = 0; it < MAXV; ++it) {
+    int idx = threadIdx.x + it * TPB;
+    if (idx < total) st(idx, r[it]);
+  }
+}
+
+// Copies the twiddle table tw[0, n) to LDS (no barrier).
+template <int TPB>
+__device__ __forceinline__ void stage_twiddles(float2* stw, const float2* __restrict__ tw, int n) {
+  for (int base = 0; base < n; base += 4 * TPB)
+    batch_copy<4, TPB>(n - base, [&](int i) { return tw[base + i]; },
+                       [&](int i, float2 v) { stw[base + i] = v; });
+}
+
 // ------------------------------------------------------------ z passes --
 // In-place r2c of padded rows as a length-n2 complex FFT of the real row
 // (any n2); PRE applies (x - 1) * coef on load (gravity.py:52-54).
+// LDS: the row block, then the n2-entry twiddle table.
 template <int PRE, bool POW2>
 __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_k(float* __restrict__ data, long long nrows, int n2,
                                                       int rs, int P, float coef,
                                                       const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
+  float2* stw = s + lds_elems(ROWS, n2, P);
   long long row0 = (long long)blockIdx.x * P;
   int np = (int)((nrows - row0) < P ? (nrows - row0) : P);
-  for (int idx = threadIdx.x; idx < np * n2; idx += FFT_TPB) {
-    int p = idx / n2, j = idx - (idx / n2) * n2;
-    float x = data[(row0 + p) * rs + j];
-    if (PRE) {
-      x = x - 1.f;
-      x = x * coef;
-    }
-    s[lidx<ROWS>(p, j, n2, P)] = make_float2(x, 0.f);
-  }
+  stage_twiddles<FFT_TPB>(stw, tw, n2);
+  batch_copy<RFFT_MAXV, FFT_TPB>(
+      np * n2,
+      [&](int idx) {
+        int p = idx / n2, j = idx - (idx / n2) * n2;
+        float x = data[(row0 + p) * rs + j];
+        if (PRE) {
+          x = x - 1.f;
+          x = x * coef;
+        }
+        return make_float2(x, 0.f);
+      },
+      [&](int idx, float2 v) {
+        int p = idx / n2, j = idx - (idx / n2) * n2;
+        s[lidx<ROWS>(p, j, n2, P)] = v;
+      });
   __syncthreads();
-  lds_fft<-1, ROWS, POW2, RFFT_MAXV>(s, n2, np, tw);
+  lds_fft<-1, ROWS, POW2, RFFT_MAXV>(s, n2, np, stw);
   const int n2h = n2 / 2 + 1;
   float2* out = reinterpret_cast<float2*>(data);
   const int rsc = rs / 2;
-  for (int idx = threadIdx.x; idx < np * n2h; idx += FFT_TPB) {
-    int p = idx / n2h, k = idx - (idx / n2h) * n2h;
-    out[(row0 + p) * rsc + k] = s[lidx<ROWS>(p, k, n2, P)];
-  }
+  batch_copy<RFFT_MAXV, FFT_TPB>(
+      np * n2h,
+      [&](int idx) {
+        int p = idx / n2h, k = idx - (idx / n2h) * n2h;
+        return s[lidx<ROWS>(p, k, n2, P)];
+      },
+      [&](int idx, float2 v) {
+        int p = idx / n2h, k = idx - (idx / n2h) * n2h;
+        out[(row0 + p) * rsc + k] = v;
+      });
 }
 
 // In-place c2r of padded rows with a final scale (any n2).  The full
@@ -277,24 +333,38 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_c2r_k(float* __restrict__ data, 
                                                       int rs, int P, float scale,
                                                       const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
+  float2* stw = s + lds_elems(ROWS, n2, P);
   long long row0 = (long long)blockIdx.x * P;
   int np = (int)((nrows - row0) < P ? (nrows - row0) : P);
   const float2* in = reinterpret_cast<const float2*>(data);
   const int rsc = rs / 2;
   const int half = n2 / 2;
-  for (int idx = threadIdx.x; idx < np * n2; idx += FFT_TPB) {
-    int p = idx / n2, k = idx - (idx / n2) * n2;
-    int kk = k <= half ? k : n2 - k;
-    float2 v = in[(row0 + p) * rsc + kk];
-    if (k > half) v.y = -v.y;
-    s[lidx<ROWS>(p, k, n2, P)] = v;
-  }
+  stage_twiddles<FFT_TPB>(stw, tw, n2);
+  batch_copy<RFFT_MAXV, FFT_TPB>(
+      np * n2,
+      [&](int idx) {
+        int p = idx / n2, k = idx - (idx / n2) * n2;
+        int kk = k <= half ? k : n2 - k;
+        float2 v = in[(row0 + p) * rsc + kk];
+        if (k > half) v.y = -v.y;
+        return v;
+      },
+      [&](int idx, float2 v) {
+        int p = idx / n2, k = idx - (idx / n2) * n2;
+        s[lidx<ROWS>(p, k, n2, P)] = v;
+      });
   __syncthreads();
-  lds_fft<1, ROWS, POW2, RFFT_MAXV>(s, n2, np, tw);
-  for (int idx = threadIdx.x; idx < np * n2; idx += FFT_TPB) {
-    int p = idx / n2, j = idx - (idx / n2) * n2;
-    data[(row0 + p) * rs + j] = s[lidx<ROWS>(p, j, n2, P)].x * scale;
-  }
+  lds_fft<1, ROWS, POW2, RFFT_MAXV>(s, n2, np, stw);
+  batch_copy<RFFT_MAXV, FFT_TPB>(
+      np * n2,
+      [&](int idx) {
+        int p = idx / n2, j = idx - (idx / n2) * n2;
+        return s[lidx<ROWS>(p, j, n2, P)];
+      },
+      [&](int idx, float2 v) {
+        int p = idx / n2, j = idx - (idx / n2) * n2;
+        data[(row0 + p) * rs + j] = v.x * scale;
+      });
 }
 
 // Half-length r2c for even n2 with H = n2/2 a power of two: the row is read
@@ -309,34 +379,49 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_half_k(float* __restrict__ d
   extern __shared__ float2 s[];
   const int H = n2 >> 1;
   const int lH = __ffs(H) - 1;
+  float2* stw = s + lds_elems(ROWS, H, P);
   long long row0 = (long long)blockIdx.x * P;
   int np = (int)((nrows - row0) < P ? (nrows - row0) : P);
   float2* d2 = reinterpret_cast<float2*>(data);
   const int rsc = rs / 2;
-  for (int idx = threadIdx.x; idx < np * H; idx += FFT_TPB) {
-    int p = idx >> lH, m = idx & (H - 1);
-    float2 v = d2[(row0 + p) * rsc + m];
-    if (PRE) {
-      v.x = v.x - 1.f;
-      v.x = v.x * coef;
-      v.y = v.y - 1.f;
-      v.y = v.y * coef;
-    }
-    s[lidx<ROWS>(p, m, H, P)] = v;
-  }
+  stage_twiddles<FFT_TPB>(stw, tw, n2);
+  batch_copy<RFFT_MAXV, FFT_TPB>(
+      np * H,
+      [&](int idx) {
+        int p = idx >> lH, m = idx & (H - 1);
+        float2 v = d2[(row0 + p) * rsc + m];
+        if (PRE) {
+          v.x = v.x - 1.f;
+          v.x = v.x * coef;
+          v.y = v.y - 1.f;
+          v.y = v.y * coef;
+        }
+        return v;
+      },
+      [&](int idx, float2 v) {
+        int p = idx >> lH, m = idx & (H - 1);
+        s[lidx<ROWS>(p, m, H, P)] = v;
+      });
   __syncthreads();
-  lds_fft<-1, ROWS, true, RFFT_MAXV>(s, H, np, tw, 2);
+  lds_fft<-1, ROWS, true, RFFT_MAXV>(s, H, np, stw, 2);
   const int H1 = H + 1;
-  for (int idx = threadIdx.x; idx < np * H1; idx += FFT_TPB) {
-    int p = idx / H1, k = idx - (idx / H1) * H1;
-    float2 zk = s[lidx<ROWS>(p, k == H ? 0 : k, H, P)];
-    float2 zm = s[lidx<ROWS>(p, k == 0 ? 0 : H - k, H, P)];
-    zm.y = -zm.y;
-    float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
-    float2 dd = csub(zk, zm);
-    float2 o = make_float2(0.5f * dd.y, -0.5f * dd.x);
-    d2[(row0 + p) * rsc + k] = cadd(e, cmul(tw[k], o));
-  }
+  // np*(H+1) <= RFFT_MAXV*FFT_TPB + np: one extra value per thread at most
+  batch_copy<RFFT_MAXV + 1, FFT_TPB>(
+      np * H1,
+      [&](int idx) {
+        int p = idx / H1, k = idx - (idx / H1) * H1;
+        float2 zk = s[lidx<ROWS>(p, k == H ? 0 : k, H, P)];
+        float2 zm = s[lidx<ROWS>(p, k == 0 ? 0 : H - k, H, P)];
+        zm.y = -zm.y;
+        float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
+        float2 dd = csub(zk, zm);
+        float2 o = make_float2(0.5f * dd.y, -0.5f * dd.x);
+        return cadd(e, cmul(stw[k], o));
+      },
+      [&](int idx, float2 v) {
+        int p = idx / H1, k = idx - (idx / H1) * H1;
+        d2[(row0 + p) * rsc + k] = v;
+      });
 }
 
 // Half-length c2r (inverse of the above) with a final scale:
@@ -348,33 +433,56 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_c2r_half_k(float* __restrict__ d
   extern __shared__ float2 s[];
   const int H = n2 >> 1;
   const int lH = __ffs(H) - 1;
+  float2* stw = s + lds_elems(ROWS, H, P);
   long long row0 = (long long)blockIdx.x * P;
   int np = (int)((nrows - row0) < P ? (nrows - row0) : P);
   float2* d2 = reinterpret_cast<float2*>(data);
   const int rsc = rs / 2;
-  for (int idx = threadIdx.x; idx < np * H; idx += FFT_TPB) {
-    int p = idx >> lH, k = idx & (H - 1);
-    float2 a = d2[(row0 + p) * rsc + k];
-    float2 b = d2[(row0 + p) * rsc + (H - k)];
-    if (k == 0) {
-      a.y = 0.f;
-      b.y = 0.f;
+  stage_twiddles<FFT_TPB>(stw, tw, n2);
+  // X[k] and X[H-k] per thread: both loads of the batch are issued first
+  float2 xa[RFFT_MAXV], xb[RFFT_MAXV];
+  const int total = np * H;
+#pragma unroll
+  for (int it = 0; it < RFFT_MAXV; ++it) {
+    int idx = threadIdx.x + it * FFT_TPB;
+    if (idx < total) {
+      int p = idx >> lH, k = idx & (H - 1);
+      xa[it] = d2[(row0 + p) * rsc + k];
+      xb[it] = d2[(row0 + p) * rsc + (H - k)];
     }
-    b.y = -b.y;
-    float2 sum = cadd(a, b);
-    float2 dif = csub(a, b);
-    float2 w = tw[k];
-    w.y = -w.y;
-    float2 t = cmul(w, dif);
-    s[lidx<ROWS>(p, k, H, P)] = make_float2(sum.x - t.y, sum.y + t.x);
+  }
+  __syncthreads();  // the twiddle table is read below
+#pragma unroll
+  for (int it = 0; it < RFFT_MAXV; ++it) {
+    int idx = threadIdx.x + it * FFT_TPB;
+    if (idx < total) {
+      int p = idx >> lH, k = idx & (H - 1);
+      float2 a = xa[it], b = xb[it];
+      if (k == 0) {
+        a.y = 0.f;
+        b.y = 0.f;
+      }
+      b.y = -b.y;
+      float2 sum = cadd(a, b);
+      float2 dif = csub(a, b);
+      float2 w = stw[k];
+      w.y = -w.y;
+      float2 t = cmul(w, dif);
+      s[lidx<ROWS>(p, k, H, P)] = make_float2(sum.x - t.y, sum.y + t.x);
+    }
   }
   __syncthreads();
-  lds_fft<1, ROWS, true, RFFT_MAXV>(s, H, np, tw, 2);
-  for (int idx = threadIdx.x; idx < np * H; idx += FFT_TPB) {
-    int p = idx >> lH, m = idx & (H - 1);
-    float2 z = s[lidx<ROWS>(p, m, H, P)];
-    d2[(row0 + p) * rsc + m] = make_float2(z.x * scale, z.y * scale);
-  }
+  lds_fft<1, ROWS, true, RFFT_MAXV>(s, H, np, stw, 2);
+  batch_copy<RFFT_MAXV, FFT_TPB>(
+      total,
+      [&](int idx) {
+        int p = idx >> lH, m = idx & (H - 1);
+        return s[lidx<ROWS>(p, m, H, P)];
+      },
+      [&](int idx, float2 z) {
+        int p = idx >> lH, m = idx & (H - 1);
+        d2[(row0 + p) * rsc + m] = make_float2(z.x * scale, z.y * scale);
+      });
 }
 
 // ------------------------------------------------------ strided passes --
@@ -385,24 +493,33 @@ struct Fields3 {
 // In-place FFT along a strided axis of a complex (outer, n, ncol) view:
 // element j of pencil (o, c) is at o*ostride + j*es + c.  A workgroup takes
 // B consecutive columns, so each of its loads is B contiguous float2.
+// LDS: the n x B block, then the n-entry twiddle table.
 template <int DIR, bool POW2>
 __global__ void __launch_bounds__(FFT_TPB, 4) strided_fft_k(Fields3 fl, int n, long long es,
                                                          long long ostride, int ncol, int B,
                                                          const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
+  float2* stw = s + n * B;
   float2* base = fl.f[blockIdx.z] + (long long)blockIdx.y * ostride;
+  const int lB = __ffs(B) - 1;
   int c0 = blockIdx.x * B;
   int nb = ncol - c0 < B ? ncol - c0 : B;
-  for (int idx = threadIdx.x; idx < n * B; idx += FFT_TPB) {
-    int j = idx / B, b = idx - (idx / B) * B;
-    if (b < nb) s[j * B + b] = base[j * es + c0 + b];
-  }
+  stage_twiddles<FFT_TPB>(stw, tw, n);
+  batch_copy<FFT_MAXV, FFT_TPB>(
+      n * B,
+      [&](int idx) {
+        int j = idx >> lB, b = idx & (B - 1);
+        return b < nb ? base[j * es + c0 + b] : make_float2(0.f, 0.f);
+      },
+      [&](int idx, float2 v) { s[idx] = v; });
   __syncthreads();
-  lds_fft<DIR, COLS, POW2>(s, n, B, tw);
-  for (int idx = threadIdx.x; idx < n * B; idx += FFT_TPB) {
-    int j = idx / B, b = idx - (idx / B) * B;
-    if (b < nb) base[j * es + c0 + b] = s[j * B + b];
-  }
+  lds_fft<DIR, COLS, POW2>(s, n, B, stw);
+  batch_copy<FFT_MAXV, FFT_TPB>(
+      n * B, [&](int idx) { return s[idx]; },
+      [&](int idx, float2 v) {
+        int j = idx >> lB, b = idx & (B - 1);
+        if (b < nb) base[j * es + c0 + b] = v;
+      });
 }
 
 struct KTab {
@@ -414,95 +531,122 @@ struct KTab {
 // c0, pot = -src/k^2 (k=0 -> 0), then for d = 0,1,2: g_d = (-i km_d) pot,
 // inverse FFT along x, stored to c_d.  Pencils along axis 0 of (n0,n1,n2h);
 // blockIdx.y = y index, blockIdx.x = block of B kz columns.
+// LDS: the n0 x B block, the potential (each thread reads back only what it
+// wrote), then the n0-entry twiddle table.
 template <bool POW2>
 __global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_k(Fields3 fl, int n0, int n1, int n2h, int n2p, int B,
                                                        KTab kt, const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
+  const int total = n0 * B;
+  float2* sp = s + total;
+  float2* stw = sp + total;
   const long long es = (long long)n1 * n2p;
+  const int lB = __ffs(B) - 1;
   const int y = blockIdx.y;
   const int c0 = blockIdx.x * B;
   const int nb = n2h - c0 < B ? n2h - c0 : B;
   const long long off = (long long)y * n2p + c0;
-  const int total = n0 * B;
-  for (int idx = threadIdx.x; idx < total; idx += FFT_TPB) {
-    int j = idx / B, b = idx - (idx / B) * B;
-    if (b < nb) s[j * B + b] = fl.f[0][off + j * es + b];
-  }
+  stage_twiddles<FFT_TPB>(stw, tw, n0);
+  batch_copy<PFFT_MAXV, FFT_TPB>(
+      total,
+      [&](int idx) {
+        int j = idx >> lB, b = idx & (B - 1);
+        return b < nb ? fl.f[0][off + j * es + b] : make_float2(0.f, 0.f);
+      },
+      [&](int idx, float2 v) { s[idx] = v; });
   __syncthreads();
-  lds_fft<-1, COLS, POW2, PFFT_MAXV>(s, n0, B, tw);
+  lds_fft<-1, COLS, POW2, PFFT_MAXV>(s, n0, B, stw);
   const float ky = kt.k[1][y];
   const float kym = kt.km[1][y];
-  float2 pot[PFFT_MAXV];
 #pragma unroll
   for (int it = 0; it < PFFT_MAXV; ++it) {
     int idx = threadIdx.x + it * FFT_TPB;
-    pot[it] = make_float2(0.f, 0.f);
     if (idx < total) {
-      int j = idx / B, b = idx - (idx / B) * B;
+      int j = idx >> lB, b = idx & (B - 1);
+      float2 pot = make_float2(0.f, 0.f);
       if (b < nb) {
-        float2 v = s[j * B + b];
+        float2 v = s[idx];
         float kx = kt.k[0][j], kz = kt.k[2][c0 + b];
         float k2 = kx * kx;
         k2 = k2 + ky * ky;
         k2 = k2 + kz * kz;
-        if (k2 != 0.f) pot[it] = make_float2((-v.x) / k2, (-v.y) / k2);
+        if (k2 != 0.f) pot = make_float2((-v.x) / k2, (-v.y) / k2);
       }
+      sp[idx] = pot;
     }
   }
   __syncthreads();
+#pragma unroll 1
   for (int d = 0; d < 3; ++d) {
 #pragma unroll
     for (int it = 0; it < PFFT_MAXV; ++it) {
       int idx = threadIdx.x + it * FFT_TPB;
       if (idx < total) {
-        int j = idx / B, b = idx - (idx / B) * B;
+        int j = idx >> lB, b = idx & (B - 1);
         float km = 0.f;
         if (b < nb) km = d == 0 ? kt.km[0][j] : (d == 1 ? kym : kt.km[2][c0 + b]);
-        s[j * B + b] = make_float2(km * pot[it].y, -(km * pot[it].x));
+        float2 pot = sp[idx];
+        s[idx] = make_float2(km * pot.y, -(km * pot.x));
       }
     }
     __syncthreads();
-    lds_fft<1, COLS, POW2, PFFT_MAXV>(s, n0, B, tw);
-    float2* out = fl.f[d];
-    for (int idx = threadIdx.x; idx < total; idx += FFT_TPB) {
-      int j = idx / B, b = idx - (idx / B) * B;
-      if (b < nb) out[off + j * es + b] = s[j * B + b];
-    }
+    lds_fft<1, COLS, POW2, PFFT_MAXV>(s, n0, B, stw);
+    // selects rather than indexes, so the argument struct stays in registers
+    float2* out = d == 0 ? fl.f[0] : (d == 1 ? fl.f[1] : fl.f[2]);
+    batch_copy<PFFT_MAXV, FFT_TPB>(
+        total, [&](int idx) { return s[idx]; },
+        [&](int idx, float2 v) {
+          int j = idx >> lB, b = idx & (B - 1);
+          if (b < nb) out[off + j * es + b] = v;
+        });
     __syncthreads();
   }
 }
 
 // Adjoint x pass: acc = sum_d (-i km_d) laplace(FFT_x c_d); inverse FFT
-// along x; stored to c0.  Transpose of the three force components.
+// along x; stored to c0.  Transpose of the three force components.  LDS as
+// poisson_x_k, the second block holding the running sum.
 template <bool POW2>
 __global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_adj_k(Fields3 fl, int n0, int n1, int n2h, int n2p, int B,
                                                            KTab kt, const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
+  const int total = n0 * B;
+  float2* sa = s + total;
+  float2* stw = sa + total;
   const long long es = (long long)n1 * n2p;
+  const int lB = __ffs(B) - 1;
   const int y = blockIdx.y;
   const int c0 = blockIdx.x * B;
   const int nb = n2h - c0 < B ? n2h - c0 : B;
   const long long off = (long long)y * n2p + c0;
-  const int total = n0 * B;
   const float ky = kt.k[1][y];
   const float kym = kt.km[1][y];
-  float2 acc[PFFT_MAXV];
+  stage_twiddles<FFT_TPB>(stw, tw, n0);
 #pragma unroll
-  for (int it = 0; it < PFFT_MAXV; ++it) acc[it] = make_float2(0.f, 0.f);
+  for (int it = 0; it < PFFT_MAXV; ++it) {
+    int idx = threadIdx.x + it * FFT_TPB;
+    if (idx < total) sa[idx] = make_float2(0.f, 0.f);
+  }
+#pragma unroll 1
   for (int d = 0; d < 3; ++d) {
-    for (int idx = threadIdx.x; idx < total; idx += FFT_TPB) {
-      int j = idx / B, b = idx - (idx / B) * B;
-      if (b < nb) s[j * B + b] = fl.f[d][off + j * es + b];
-    }
+    // selects rather than indexes, so the argument struct stays in registers
+    const float2* in = d == 0 ? fl.f[0] : (d == 1 ? fl.f[1] : fl.f[2]);
+    batch_copy<PFFT_MAXV, FFT_TPB>(
+        total,
+        [&](int idx) {
+          int j = idx >> lB, b = idx & (B - 1);
+          return b < nb ? in[off + j * es + b] : make_float2(0.f, 0.f);
+        },
+        [&](int idx, float2 v) { s[idx] = v; });
     __syncthreads();
-    lds_fft<-1, COLS, POW2, PFFT_MAXV>(s, n0, B, tw);
+    lds_fft<-1, COLS, POW2, PFFT_MAXV>(s, n0, B, stw);
 #pragma unroll
     for (int it = 0; it < PFFT_MAXV; ++it) {
       int idx = threadIdx.x + it * FFT_TPB;
       if (idx < total) {
-        int j = idx / B, b = idx - (idx / B) * B;
+        int j = idx >> lB, b = idx & (B - 1);
         if (b < nb) {
-          float2 v = s[j * B + b];
+          float2 v = s[idx];
           float kx = kt.k[0][j], kz = kt.k[2][c0 + b];
           float k2 = kx * kx;
           k2 = k2 + ky * ky;
@@ -510,7 +654,7 @@ __global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_adj_k(Fields3 fl, int n0
           float2 pot = k2 != 0.f ? make_float2((-v.x) / k2, (-v.y) / k2) : make_float2(0.f, 0.f);
           float km = d == 0 ? kt.km[0][j] : (d == 1 ? kym : kt.km[2][c0 + b]);
           float2 g = make_float2(km * pot.y, -(km * pot.x));
-          acc[it] = cadd(acc[it], g);
+          sa[idx] = cadd(sa[idx], g);
         }
       }
     }
@@ -519,14 +663,16 @@ __global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_adj_k(Fields3 fl, int n0
 #pragma unroll
   for (int it = 0; it < PFFT_MAXV; ++it) {
     int idx = threadIdx.x + it * FFT_TPB;
-    if (idx < total) s[(idx / B) * B + (idx - (idx / B) * B)] = acc[it];
+    if (idx < total) s[idx] = sa[idx];
   }
   __syncthreads();
-  lds_fft<1, COLS, POW2, PFFT_MAXV>(s, n0, B, tw);
-  for (int idx = threadIdx.x; idx < total; idx += FFT_TPB) {
-    int j = idx / B, b = idx - (idx / B) * B;
-    if (b < nb) fl.f[0][off + j * es + b] = s[j * B + b];
-  }
+  lds_fft<1, COLS, POW2, PFFT_MAXV>(s, n0, B, stw);
+  batch_copy<PFFT_MAXV, FFT_TPB>(
+      total, [&](int idx) { return s[idx]; },
+      [&](int idx, float2 v) {
+        int j = idx >> lB, b = idx & (B - 1);
+        if (b < nb) fl.f[0][off + j * es + b] = v;
+      });
 }
 
 // ------------------------------------------------------------- copies --
@@ -547,6 +693,9 @@ static size_t lds_bytes(int n, int P, int lay = COLS) {
   return (size_t)lds_elems(lay, n, P) * sizeof(float2);
 }
 
+// the twiddle table staged after the data block
+static size_t tw_bytes(int n) { return (size_t)n * sizeof(float2); }
+
 // rows per workgroup for the row passes: P*n <= RFFT_CAP and the swizzled
 // ROWS block within FFT_CAP elements (64 KB)
 static int row_pencils(int n) {
@@ -564,7 +713,7 @@ int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef) {
   if (half_ok(n2)) {
     int P = row_pencils(n2 / 2);
     unsigned nblk = (unsigned)((nrows + P - 1) / P);
-    size_t lds = lds_bytes(n2 / 2, P, ROWS);
+    size_t lds = lds_bytes(n2 / 2, P, ROWS) + tw_bytes(n2);
     if (pre)
       z_r2c_half_k<1><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
     else
@@ -572,7 +721,7 @@ int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef) {
   } else {
     int P = row_pencils(n2);
     unsigned nblk = (unsigned)((nrows + P - 1) / P);
-    size_t lds = lds_bytes(n2, P, ROWS);
+    size_t lds = lds_bytes(n2, P, ROWS) + tw_bytes(n2);
     bool p2 = is_pow2(n2);
     if (pre && p2)
       z_r2c_k<1, true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
@@ -595,12 +744,12 @@ int pmx_fft_c2r_inplace(pmx_ctx* c, float* buf, float scale) {
   if (half_ok(n2)) {
     int P = row_pencils(n2 / 2);
     unsigned nblk = (unsigned)((nrows + P - 1) / P);
-    z_c2r_half_k<<<nblk, FFT_TPB, lds_bytes(n2 / 2, P, ROWS), c->stream>>>(buf, nrows, n2, c->rs, P,
-                                                                        scale, c->tw[2]);
+    z_c2r_half_k<<<nblk, FFT_TPB, lds_bytes(n2 / 2, P, ROWS) + tw_bytes(n2), c->stream>>>(
+        buf, nrows, n2, c->rs, P, scale, c->tw[2]);
   } else {
     int P = row_pencils(n2);
     unsigned nblk = (unsigned)((nrows + P - 1) / P);
-    size_t lds = lds_bytes(n2, P, ROWS);
+    size_t lds = lds_bytes(n2, P, ROWS) + tw_bytes(n2);
     if (is_pow2(n2))
       z_c2r_k<true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2]);
     else
@@ -630,7 +779,7 @@ int pmx_fft_strided(pmx_ctx* c, float2* const* f, int nf, int axis, int dir) {
     nouter = c->n[1];
   }
   dim3 grid((ncol + B - 1) / B, nouter, nf);
-  size_t lds = lds_bytes(n, B);
+  size_t lds = lds_bytes(n, B) + tw_bytes(n);
   bool p2 = is_pow2(n);
   int cls = axis == 1 ? (dir < 0 ? PMX_T_YFWD : PMX_T_YINV) : PMX_T_OTHER;
   pmx_tic(c, cls);
@@ -661,7 +810,7 @@ int pmx_poisson_x(pmx_ctx* c, float2* const* f, int adjoint) {
   int n0 = c->n[0];
   int B = fft_pencils(n0, PFFT_CAP);
   dim3 grid((c->n2h + B - 1) / B, c->n[1], 1);
-  size_t lds = lds_bytes(n0, B);
+  size_t lds = 2 * lds_bytes(n0, B) + tw_bytes(n0);
   bool p2 = is_pow2(n0);
   pmx_tic(c, PMX_T_POISSON);
   if (adjoint && p2)

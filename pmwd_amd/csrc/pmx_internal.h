@@ -123,8 +123,9 @@ struct pmx_ctx {
 
   // reductions (float64 partial sums)
   double* red;
-  int red_cap;
-  double* red_host;    // pinned
+  long long red_cap;
+  double* red_out;     // PMX_RED_SLOTS reduced sums (device)
+  double* red_host;    // PMX_RED_SLOTS pinned host doubles
 
   // radix sort scratch (lazily sized)
   long long sort_cap;
@@ -140,7 +141,7 @@ struct pmx_ctx {
 
 int pmx_ensure_ptcl(pmx_ctx* c, long long n);
 int pmx_ensure_vrec(pmx_ctx* c, long long n);
-int pmx_ensure_red(pmx_ctx* c, int nblk);
+int pmx_ensure_red(pmx_ctx* c, long long nblk);
 int pmx_scan_exclusive(pmx_ctx* c, const uint32_t* in, uint32_t* out, long long n);
 MeshGeom pmx_geom_padded(const pmx_ctx* c);
 
@@ -235,4 +236,132 @@ __device__ __forceinline__ long long pmx_cell_index(const CicAxis* ax, int j, in
 // fixed tree order used for every 8-neighbour sum
 __device__ __forceinline__ float pmx_tree8(const float v[8]) {
   return ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+}
+
+// --------------------------------------------------- particle helpers --
+__device__ __forceinline__ void load_ptcl(const int16_t* __restrict__ pmid,
+                                          const float* __restrict__ disp, long long i,
+                                          int pm[3], float dp[3]) {
+  pm[0] = pmid[3 * i + 0];
+  pm[1] = pmid[3 * i + 1];
+  pm[2] = pmid[3 * i + 2];
+  dp[0] = disp[3 * i + 0];
+  dp[1] = disp[3 * i + 1];
+  dp[2] = disp[3 * i + 2];
+}
+
+__device__ __forceinline__ void cic_setup(const int pm[3], const float dp[3], const MeshGeom& g,
+                                          CicAxis ax[3]) {
+  ax[0] = pmx_cic_axis(pm[0], dp[0], g.cell, g.n0);
+  ax[1] = pmx_cic_axis(pm[1], dp[1], g.cell, g.n1);
+  ax[2] = pmx_cic_axis(pm[2], dp[2], g.cell, g.n2);
+}
+
+__device__ __forceinline__ uint32_t tile_of(const CicAxis ax[3], const TileGeom& tg) {
+  return ((uint32_t)(ax[0].lo / tg.tx) * tg.nty + (uint32_t)(ax[1].lo / tg.ty)) * tg.ntz +
+         (uint32_t)(ax[2].lo / tg.tz);
+}
+
+// Wave-aggregated atomicAdd(&ctr[key], 1): returns this lane's slot.
+// Every lane of the wave must call it (inactive lanes pass active=false).
+__device__ __forceinline__ uint32_t agg_slot(uint32_t* __restrict__ ctr, uint32_t key, bool active) {
+  unsigned long long pending = __ballot(active);
+  const int lane = __lane_id();
+  uint32_t slot = 0;
+  while (pending) {
+    int leader = __ffsll((long long)pending) - 1;
+    uint32_t lkey = __shfl(key, leader);
+    unsigned long long peers = __ballot(active && key == lkey) & pending;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&ctr[lkey], (uint32_t)__popcll(peers));
+    base = __shfl(base, leader);
+    if (active && key == lkey) slot = base + (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+    pending &= ~peers;
+  }
+  return slot;
+}
+
+// per-axis fractions from d = disp/cell exactly as pmx_cic_axis forms them
+__device__ __forceinline__ void frac_of(float d, float& w0, float& w1) {
+  int i = (int)floorf(d);
+  w0 = 1.f - fabsf(d - (float)i);
+  w1 = 1.f - fabsf(d - (float)(i + 1));
+}
+
+__device__ __forceinline__ void tile_origin(int t, const TileGeom& tg, const MeshGeom& g, int o[3],
+                                            int s[3], int ti[3]) {
+  ti[2] = t % tg.ntz;
+  ti[1] = (t / tg.ntz) % tg.nty;
+  ti[0] = t / (tg.ntz * tg.nty);
+  o[0] = ti[0] * tg.tx;
+  o[1] = ti[1] * tg.ty;
+  o[2] = ti[2] * tg.tz;
+  s[0] = min(tg.tx, g.n0 - o[0]);
+  s[1] = min(tg.ty, g.n1 - o[1]);
+  s[2] = min(tg.tz, g.n2 - o[2]);
+}
+
+// fractions and the sign(-d2) factors of their gradients (pmx_cic_axis)
+__device__ __forceinline__ void frac_sign_of(float d, float& w0, float& w1, float& s0, float& s1) {
+  int i = (int)floorf(d);
+  float d20 = d - (float)i;
+  float d21 = d - (float)(i + 1);
+  w0 = 1.f - fabsf(d20);
+  w1 = 1.f - fabsf(d21);
+  s0 = pmx_sign(-d20);
+  s1 = pmx_sign(-d21);
+}
+
+// Stages NC padded meshes' values over tile t's cells plus the +1 layer
+// (periodic wrap) into LDS, component-major: sh[c*ltot + (qx*LY + qy)*LZ + qz].
+template <int NC, int TPB>
+__device__ __forceinline__ void pmx_stage_tile(float* sh, const int o[3], int LY, int LZ, int ltot,
+                                               const MeshGeom& g, const float* const* F) {
+  for (int i = threadIdx.x; i < ltot; i += TPB) {
+    int qz = i % LZ;
+    int qy = (i / LZ) % LY;
+    int qx = i / (LZ * LY);
+    int x = o[0] + qx, y = o[1] + qy, z = o[2] + qz;
+    if (x >= g.n0) x -= g.n0;
+    if (y >= g.n1) y -= g.n1;
+    if (z >= g.n2) z -= g.n2;
+    long long m = ((long long)x * g.n1 + y) * g.rs + z;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) sh[c * ltot + i] = F[c][m];
+  }
+}
+
+// Unpacks a binned record's tile-local lower corner (lx | ly << 10 | lz << 20).
+__device__ __forceinline__ void pmx_unpack_local(float w, int& lx, int& ly, int& lz) {
+  uint32_t pk = __float_as_uint(w);
+  lx = pk & 1023;
+  ly = (pk >> 10) & 1023;
+  lz = (pk >> 20) & 1023;
+}
+
+// ------------------------------------------------- shared host drivers --
+int pmx_fill_u32(pmx_ctx* c, uint32_t* p, long long n, uint32_t v);
+int pmx_scan_tiles(pmx_ctx* c);
+int pmx_bin_fill(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
+                 const float* pval, int vstride, int nch);
+int pmx_bin_particles(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n);
+int pmx_deposit_binned(pmx_ctx* c, float val, int ch, long long n, float* mesh, int mrs,
+                       int accumulate);
+int pmx_poisson_ctx(pmx_ctx* c, float coef);
+int pmx_poisson_adj_ctx(pmx_ctx* c, float scale_out);
+// bins the particles, deposits the density and solves for the three force
+// meshes in c->cbuf (padded); then the tile-staged gather of those meshes
+int pmx_gravity_fwd_meshes(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
+                           float val, float coef);
+int pmx_tile_gather(pmx_ctx* c, float* acc, float* vel, float k2);
+// float64 partial sums: reduce red[off, off+nblk) in a fixed order into
+// device slot `slot` (< PMX_RED_SLOTS), then fetch slots [0, k) to the host
+#define PMX_RED_SLOTS 8
+int pmx_red_reduce(pmx_ctx* c, long long off, long long nblk, int slot);
+int pmx_red_fetch(pmx_ctx* c, int k, double* out);
+int pmx_red_finish(pmx_ctx* c, long long nblk, double* out);
+// blocks of a capped grid-stride launch over n items
+static inline unsigned pmx_blocks_capped(long long n, unsigned cap, int tpb = PMX_TPB) {
+  unsigned b = pmx_blocks(n, tpb);
+  return b < cap ? b : cap;
 }

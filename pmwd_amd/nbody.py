@@ -156,12 +156,14 @@ def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=N
 
 # ------------------------------------------------------------ adjoint ---
 
-def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False, ckpt=None):
+def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False, ckpt=None,
+                 fused=True):
     """Reverse-time adjoint in place (nbody.py:143-162, 226-260).
 
     On return disp/vel hold the reconstructed initial state and
     disp_cot/vel_cot the initial-state cotangents; returns (acc, acc_cot,
-    Omega_m cotangent).
+    Omega_m cotangent).  KDK steps run as one fused library call each unless
+    ``fused`` is False (then op by op, as for any other splitting).
     """
     L = _lib.lib()
     n = pmid.shape[0]
@@ -182,9 +184,40 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
     a_nbody = conf.a_nbody[::-1] if reverse else conf.a_nbody
     nsteps = len(a_nbody) - 1
     cf = force_adj()
+    dot_force = dot.value
     om_cot = 0.0
+    kdk = fused and _is_kdk(conf)
+    dots = (ctypes.c_double * 2)()
+    kp = 0.0  # closing kick of the previous KDK step, applied by the next call
     for s, (a_prev, a_next) in enumerate(zip(a_nbody[:0:-1], a_nbody[-2::-1])):
         step = nsteps - 1 - s  # index of the forward step being undone
+        if kdk:
+            # kick_adj, drift_adj, force_adj in one library call, the closing
+            # kick_adj deferred into the next call; the kick adjoints'
+            # sum(vel_cot*acc) is the dot of the force VJP that produced acc
+            a_half = a_prev * (1 - 0.5) + a_next * 0.5
+            ka_args = (a_prev, a_prev, a_half)
+            d_args = (a_half, a_prev, a_next)
+            kb_args = (a_next, a_half, a_next)
+            fa = _f32(kick_factor(*ka_args, cosmo, conf))
+            fd = _f32(drift_factor(*d_args, cosmo, conf))
+            fb = _f32(kick_factor(*kb_args, cosmo, conf))
+            snap = ckpt[step][0] if ckpt and step in ckpt else None
+            _lib.check(L.pmx_nbody_step_adj(
+                h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel), _lib.ptr(acc),
+                _lib.ptr(disp_cot), _lib.ptr(vel_cot), _lib.ptr(acc_cot), n, val, coef,
+                kp, fa, fd, _lib.ptr(snap), dots))
+            om_cot -= kick_factor_dOm(*ka_args, cosmo, conf) * dot_force + cf * fa
+            om_cot -= drift_factor_dOm(*d_args, cosmo, conf) * dots[0]
+            dot_force = dots[1]
+            cf = dot_force / Om
+            om_cot -= kick_factor_dOm(*kb_args, cosmo, conf) * dot_force + cf * fb
+            kp = fb
+            if ckpt and step in ckpt:
+                _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
+                kp = 0.0
+                vel.copy_(ckpt[step][1])
+            continue
         D = K = 0
         a_disp = a_vel = a_acc = a_prev
         for d, k in reversed(conf.symp_splits):
@@ -216,7 +249,14 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
                 a_acc = a_disp
         if ckpt and step in ckpt:
             vel.copy_(ckpt[step][1])
+    if kp != 0.0:
+        _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
     return acc, acc_cot, om_cot
+
+
+def _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, k):
+    _lib.check(L.pmx_kick_adj(h, n, _lib.ptr(vel), _lib.ptr(acc), _lib.ptr(disp_cot),
+                              _lib.ptr(acc_cot), _lib.ptr(vel_cot), k, None))
 
 
 class _NBodyFn(torch.autograd.Function):

@@ -160,3 +160,104 @@ def test_nbody_autograd_matches_adjoint(dev):
         vals.append(float((pa.scatter(o2, conf).double()**2).mean()))
     fd = (vals[0] - vals[1]) / 2e-3
     assert abs(float(gom) - fd) < 0.02 * abs(fd) + 1e-6
+
+
+@pytest.mark.parametrize('shape,ratio,snap', [((16, 16, 16), 2, False), ((12, 10, 8), 1, True),
+                                              ((32, 32, 32), 1, False)])
+def test_nbody_step_adj_matches_separate_ops(dev, shape, ratio, snap):
+    """Two reverse KDK steps of pmx_nbody_step_adj (the first step's closing
+    kick deferred into the second call, the last applied by pmx_kick_adj)
+    equal kick_adj, drift_adj, gravity_vjp, kick_adj called one by one
+    (nbody.py:143-162): bitwise for every array, and the dots match in
+    float64."""
+    import ctypes
+    import pmwd_amd as pa
+    from pmwd_amd import _lib
+    L = _lib.lib()
+    conf = pa.Configuration(1.0, shape, mesh_shape=ratio)
+    c = _lib.context(conf.mesh_shape, conf.cell_size, dev)
+    h = c.bind()
+    pmid, disp = make_ptcl(shape, conf.mesh_shape, conf.cell_size, 2.5, 11)
+    rng = np.random.default_rng(12)
+    f32 = lambda: rng.standard_normal(disp.shape).astype(np.float32)
+    P = lambda a: torch.as_tensor(a, device=dev).contiguous()
+    vel, dcot, vcot = f32(), f32(), f32()
+    snap_t = P(disp + f32()) if snap else None
+    n = len(pmid)
+    val = float(np.float32(conf.mesh_size / conf.ptcl_num))
+    coef = float(np.float32(0.45))
+    steps = [(0.37, -0.81, 0.29), (0.21, -0.64, 0.33)]
+    dot = ctypes.c_double()
+
+    def start():
+        st = dict(pmid=P(pmid), disp=P(disp), vel=P(vel), dc=P(dcot), vc=P(vcot))
+        st['acc'] = torch.empty_like(st['disp'])
+        st['ac'] = torch.empty_like(st['disp'])
+        _lib.check(L.pmx_gravity_vjp(h, _lib.ptr(st['pmid']), _lib.ptr(st['disp']), n, val, coef,
+                                     _lib.ptr(st['vc']), _lib.ptr(st['acc']), _lib.ptr(st['ac']),
+                                     ctypes.byref(dot)))
+        return st, dot.value
+
+    a, dot0 = start()
+    p = lambda k: _lib.ptr(a[k])
+    want_dots, kick_dots = [], []
+    for i, (ka, d, kb) in enumerate(steps):
+        _lib.check(L.pmx_kick_adj(h, n, p('vel'), p('acc'), p('dc'), p('ac'), p('vc'), ka,
+                                  ctypes.byref(dot)))
+        kick_dots.append(dot.value)
+        _lib.check(L.pmx_drift_adj(h, n, p('disp'), p('vel'), p('dc'), p('vc'), d,
+                                   ctypes.byref(dot)))
+        ddot = dot.value
+        if snap and i == 0:
+            a['disp'].copy_(snap_t)
+        _lib.check(L.pmx_gravity_vjp(h, p('pmid'), p('disp'), n, val, coef, p('vc'), p('acc'),
+                                     p('ac'), ctypes.byref(dot)))
+        want_dots.append((ddot, dot.value))
+        _lib.check(L.pmx_kick_adj(h, n, p('vel'), p('acc'), p('dc'), p('ac'), p('vc'), kb,
+                                  ctypes.byref(dot)))
+        kick_dots.append(dot.value)
+
+    b, _ = start()
+    q = lambda k: _lib.ptr(b[k])
+    dots = (ctypes.c_double * 2)()
+    kp = 0.0
+    for i, (ka, d, kb) in enumerate(steps):
+        _lib.check(L.pmx_nbody_step_adj(h, q('pmid'), q('disp'), q('vel'), q('acc'), q('dc'),
+                                        q('vc'), q('ac'), n, val, coef, kp, ka, d,
+                                        _lib.ptr(snap_t) if i == 0 else None, dots))
+        for got, want in zip(dots, want_dots[i]):
+            assert abs(got - want) <= 1e-12 * abs(want)
+        kp = kb
+    _lib.check(L.pmx_kick_adj(h, n, q('vel'), q('acc'), q('dc'), q('ac'), q('vc'), kp, None))
+    for k in ('disp', 'vel', 'acc', 'dc', 'vc', 'ac'):
+        np.testing.assert_array_equal(b[k].cpu().numpy(), a[k].cpu().numpy(), err_msg=k)
+    # every kick adjoint's dot is the dot of the force it uses
+    force_dots = [dot0, want_dots[0][1], want_dots[0][1], want_dots[1][1]]
+    for kd, fd in zip(kick_dots, force_dots):
+        assert abs(kd - fd) <= 1e-12 * abs(fd)
+
+
+@pytest.mark.parametrize('every', [None, 1, 2])
+def test_adjoint_driver_fused_matches_op_by_op(dev, every):
+    """nbody's reverse-time adjoint with the fused KDK call (pending closing
+    kicks, checkpoint snaps) equals the op-by-op loop bit for bit."""
+    import importlib
+    nbm = importlib.import_module('pmwd_amd.nbody')
+    pa_, conf, oconf, cosmo, ocosmo = small_setup(shape=(8, 8, 8), a_start=1 / 8, steps_max=1 / 8)
+    pmid, disp = make_ptcl(conf.ptcl_grid_shape, conf.mesh_shape, conf.cell_size, 0.5, 21)
+    P = lambda a: torch.as_tensor(a, device=dev).contiguous()
+    rng = np.random.default_rng(22)
+    d0, v0 = P(disp), P(0.1 * rng.standard_normal(disp.shape).astype(np.float32))
+    dc0 = P(rng.standard_normal(disp.shape).astype(np.float32))
+    vc0 = P(rng.standard_normal(disp.shape).astype(np.float32))
+    pm = P(pmid)
+    d, v = d0.clone(), v0.clone()
+    _, ckpt = nbm._run_forward(pm, d, v, cosmo, conf, checkpoint_every=every)
+    outs = []
+    for fused in (True, False):
+        st = [d.clone(), v.clone(), dc0.clone(), vc0.clone()]
+        acc, acc_cot, om = nbm._run_adjoint(pm, *st, cosmo, conf, ckpt=ckpt, fused=fused)
+        outs.append((st + [acc, acc_cot], om))
+    for x, y in zip(outs[0][0], outs[1][0]):
+        np.testing.assert_array_equal(x.cpu().numpy(), y.cpu().numpy())
+    assert abs(outs[0][1] - outs[1][1]) <= 1e-10 * abs(outs[1][1]) + 1e-10
