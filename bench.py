@@ -33,6 +33,8 @@ def parse():
     p.add_argument('--box', type=float, default=1000.0)
     p.add_argument('--no-adjoint', action='store_true')
     p.add_argument('--no-cpu-baseline', action='store_true')
+    p.add_argument('--no-fft-yardstick', action='store_true',
+                   help='skip timing rocFFT (torch.fft) next to the library FFTs')
     p.add_argument('--cpu-slab', type=int, default=64,
                    help='fraction 1/slab of the particles in the CPU baseline sample')
     p.add_argument('--copy-calib', action='store_true',
@@ -139,6 +141,59 @@ def cpu_baseline(conf, cosmo_om, slab):
                                        f'(scipy.fft, {cores} workers) + kick/drift/scatter/gather '
                                        f'of 1/{slab} of the particles ({ns}, numpy, 1 thread), '
                                        f'particle part scaled x{slab}')
+
+
+def fft_yardstick(conf, ctx, h, L, _lib, dev, reps=5):
+    """rocFFT (torch.fft.rfftn/irfftn) beside the library's pmx_rfftn/irfftn on
+    one mesh: mean ms per transform (HIP events on the library stream), the
+    library's three passes alone (timer classes, without the layout copies
+    between contiguous and padded rows), and the relative L2 difference."""
+    import torch
+    shape = conf.mesh_shape
+    x = torch.randn(shape, device=dev)
+    nh = shape[2] // 2 + 1
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        b.synchronize()
+        return a.elapsed_time(b) / reps
+
+    # rocFFT's r2c result is strided (not C-contiguous) and a c2r may use its
+    # input as scratch: every consumer gets its own contiguous copy
+    yr = torch.fft.rfftn(x).contiguous()
+    xr = torch.fft.irfftn(yr.clone(), s=shape)
+    yt = yr.clone()
+    roc_r2c = timed(lambda: torch.fft.rfftn(x))
+    roc_c2r = timed(lambda: torch.fft.irfftn(yt, s=shape))
+    yo = torch.empty(shape[0], shape[1], nh, dtype=torch.complex64, device=dev)
+    xo = torch.empty(shape, device=dev)
+    r2c = lambda: _lib.check(L.pmx_rfftn(h, _lib.ptr(x), _lib.ptr(yo)))
+    c2r = lambda: _lib.check(L.pmx_irfftn(h, _lib.ptr(yr), _lib.ptr(xo)))
+    our_r2c = timed(r2c)
+    our_c2r = timed(c2r)
+    passes = {}
+    for name, fn in (('r2c', r2c), ('c2r', c2r)):
+        _lib.check(L.pmx_timing_enable(h, 1))
+        _lib.timing_read(ctx)
+        for _ in range(reps):
+            fn()
+        t = _lib.timing_read(ctx)
+        _lib.check(L.pmx_timing_enable(h, 0))
+        passes[name] = sum(t[k][0] for k in ('z_r2c', 'y_fwd', 'y_inv', 'z_c2r', 'other')) / reps
+    r2c()
+    c2r()
+    torch.cuda.synchronize()
+    rel = lambda a, b: float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+    return {'mesh': list(shape), 'rocfft_r2c_ms': roc_r2c, 'rocfft_c2r_ms': roc_c2r,
+            'pmx_r2c_ms': our_r2c, 'pmx_c2r_ms': our_c2r,
+            'pmx_r2c_passes_ms': passes['r2c'], 'pmx_c2r_passes_ms': passes['c2r'],
+            'r2c_rel_l2_vs_rocfft': rel(yo, yr), 'c2r_rel_l2_vs_rocfft': rel(xo, xr)}
 
 
 def main():
@@ -297,6 +352,10 @@ def main():
         'kernels': kernels,
         't_ic_s': t_ic,
     }
+
+    if not args.no_fft_yardstick:
+        log('fft yardstick: rocFFT (torch.fft) vs library r2c/c2r')
+        out['fft_yardstick'] = fft_yardstick(conf, ctx, h, L, _lib, dev)
 
     if not args.no_adjoint:
         log(f'timed adjoint: {args.steps} steps')

@@ -15,6 +15,7 @@
 #include <math.h>
 
 #define ADJ_TPB 512
+#define ADJ_U 4
 
 // Fixed-order block sum of s (wave butterflies, then wave sums in order)
 // stored to part[blockIdx.x].
@@ -72,48 +73,72 @@ __global__ void __launch_bounds__(ADJ_TPB)
   const uint32_t cnt = tile_count[t];
   double dotv = 0;
   float vm0 = 0.f, vm1 = 0.f, vm2 = 0.f;
-  for (uint32_t q = threadIdx.x; q < cnt; q += ADJ_TPB) {
-    const float4 r = rec[start + q];
-    const long long p = pidx[start + q];
-    int lx, ly, lz;
-    pmx_unpack_local(r.w, lx, ly, lz);
-    if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
-    float wa[2], wb[2], wc[2], sa[2], sb[2], sc[2];
-    frac_sign_of(r.x, wa[0], wa[1], sa[0], sa[1]);
-    frac_sign_of(r.y, wb[0], wb[1], sb[0], sb[1]);
-    frac_sign_of(r.z, wc[0], wc[1], sc[0], sc[1]);
-    const int b = (lx * LY + ly) * LZ + lz;
-    const float v0 = vel_cot[3 * p + 0], v1 = vel_cot[3 * p + 1], v2 = vel_cot[3 * p + 2];
-    float r0[8], r1[8], r2[8], gx[8], gy[8], gz[8];
+  // ADJ_U records per thread per round, loads issued before the arithmetic
+  for (uint32_t q0 = 0; q0 < cnt; q0 += ADJ_TPB * ADJ_U) {
+    float4 rr[ADJ_U];
+    long long pp[ADJ_U];
+    float vc[ADJ_U][3];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int ja = j & 1, jb = (j >> 1) & 1, jc = (j >> 2) & 1;
-      const float f0 = wa[ja], f1 = wb[jb], f2 = wc[jc];
-      const float w = (f0 * f1) * f2;
-      const int li = b + (ja ? LY * LZ : 0) + (jb ? LZ : 0) + jc;
-      const float a0 = sh[li], a1 = sh[ltot + li], a2 = sh[2 * ltot + li];
-      r0[j] = a0 * w;
-      r1[j] = a1 * w;
-      r2[j] = a2 * w;
-      const float dc = v0 * a0 + v1 * a1 + v2 * a2;
-      gx[j] = dc * (sa[ja] * (f1 * f2));
-      gy[j] = dc * (sb[jb] * (f2 * f0));
-      gz[j] = dc * (sc[jc] * (f0 * f1));
+    for (int u = 0; u < ADJ_U; ++u) {
+      const uint32_t q = q0 + u * ADJ_TPB + threadIdx.x;
+      pp[u] = -1;
+      if (q < cnt) {
+        rr[u] = rec[start + q];
+        pp[u] = pidx[start + q];
+      }
     }
-    const float o0 = pmx_tree8(r0), o1 = pmx_tree8(r1), o2 = pmx_tree8(r2);
-    acc[3 * p + 0] = o0;
-    acc[3 * p + 1] = o1;
-    acc[3 * p + 2] = o2;
-    acc_cot[3 * p + 0] = pmx_tree8(gx) / g.cell;
-    acc_cot[3 * p + 1] = pmx_tree8(gy) / g.cell;
-    acc_cot[3 * p + 2] = pmx_tree8(gz) / g.cell;
-    vrec[start + q] = v0;
-    vrec[vstride + start + q] = v1;
-    vrec[2 * vstride + start + q] = v2;
-    vm0 = fmaxf(vm0, fabsf(v0));
-    vm1 = fmaxf(vm1, fabsf(v1));
-    vm2 = fmaxf(vm2, fabsf(v2));
-    dotv += (double)v0 * o0 + (double)v1 * o1 + (double)v2 * o2;
+#pragma unroll
+    for (int u = 0; u < ADJ_U; ++u)
+      if (pp[u] >= 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) vc[u][a] = vel_cot[3 * pp[u] + a];
+      }
+#pragma unroll
+    for (int u = 0; u < ADJ_U; ++u) {
+      if (pp[u] < 0) continue;
+      const uint32_t q = q0 + u * ADJ_TPB + threadIdx.x;
+      const float4 r = rr[u];
+      const long long p = pp[u];
+      int lx, ly, lz;
+      pmx_unpack_local(r.w, lx, ly, lz);
+      if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
+      float wa[2], wb[2], wc[2], sa[2], sb[2], sc[2];
+      frac_sign_of(r.x, wa[0], wa[1], sa[0], sa[1]);
+      frac_sign_of(r.y, wb[0], wb[1], sb[0], sb[1]);
+      frac_sign_of(r.z, wc[0], wc[1], sc[0], sc[1]);
+      const int b = (lx * LY + ly) * LZ + lz;
+      const float v0 = vc[u][0], v1 = vc[u][1], v2 = vc[u][2];
+      float r0[8], r1[8], r2[8], gx[8], gy[8], gz[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int ja = j & 1, jb = (j >> 1) & 1, jc = (j >> 2) & 1;
+        const float f0 = wa[ja], f1 = wb[jb], f2 = wc[jc];
+        const float w = (f0 * f1) * f2;
+        const int li = b + (ja ? LY * LZ : 0) + (jb ? LZ : 0) + jc;
+        const float a0 = sh[li], a1 = sh[ltot + li], a2 = sh[2 * ltot + li];
+        r0[j] = a0 * w;
+        r1[j] = a1 * w;
+        r2[j] = a2 * w;
+        const float dc = v0 * a0 + v1 * a1 + v2 * a2;
+        gx[j] = dc * (sa[ja] * (f1 * f2));
+        gy[j] = dc * (sb[jb] * (f2 * f0));
+        gz[j] = dc * (sc[jc] * (f0 * f1));
+      }
+      const float o0 = pmx_tree8(r0), o1 = pmx_tree8(r1), o2 = pmx_tree8(r2);
+      acc[3 * p + 0] = o0;
+      acc[3 * p + 1] = o1;
+      acc[3 * p + 2] = o2;
+      acc_cot[3 * p + 0] = pmx_tree8(gx) / g.cell;
+      acc_cot[3 * p + 1] = pmx_tree8(gy) / g.cell;
+      acc_cot[3 * p + 2] = pmx_tree8(gz) / g.cell;
+      vrec[start + q] = v0;
+      vrec[vstride + start + q] = v1;
+      vrec[2 * vstride + start + q] = v2;
+      vm0 = fmaxf(vm0, fabsf(v0));
+      vm1 = fmaxf(vm1, fabsf(v1));
+      vm2 = fmaxf(vm2, fabsf(v2));
+      dotv += (double)v0 * o0 + (double)v1 * o1 + (double)v2 * o2;
+    }
   }
   wave_max_store(vm0, vmax + 0);
   wave_max_store(vm1, vmax + 1);
@@ -141,31 +166,53 @@ __global__ void __launch_bounds__(ADJ_TPB)
   __syncthreads();
   const uint32_t start = tile_start[t];
   const uint32_t cnt = tile_count[t];
-  for (uint32_t q = threadIdx.x; q < cnt; q += ADJ_TPB) {
-    const float4 r = rec[start + q];
-    const long long p = pidx[start + q];
-    int lx, ly, lz;
-    pmx_unpack_local(r.w, lx, ly, lz);
-    if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
-    float wa[2], wb[2], wc[2], sa[2], sb[2], sc[2];
-    frac_sign_of(r.x, wa[0], wa[1], sa[0], sa[1]);
-    frac_sign_of(r.y, wb[0], wb[1], sb[0], sb[1]);
-    frac_sign_of(r.z, wc[0], wc[1], sc[0], sc[1]);
-    const int b = (lx * LY + ly) * LZ + lz;
-    float gx[8], gy[8], gz[8];
+  for (uint32_t q0 = 0; q0 < cnt; q0 += ADJ_TPB * ADJ_U) {
+    float4 rr[ADJ_U];
+    long long pp[ADJ_U];
+    float ac[ADJ_U][3];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int ja = j & 1, jb = (j >> 1) & 1, jc = (j >> 2) & 1;
-      const float f0 = wa[ja], f1 = wb[jb], f2 = wc[jc];
-      const int li = b + (ja ? LY * LZ : 0) + (jb ? LZ : 0) + jc;
-      const float dc = sh[li] * val;
-      gx[j] = dc * (sa[ja] * (f1 * f2));
-      gy[j] = dc * (sb[jb] * (f2 * f0));
-      gz[j] = dc * (sc[jc] * (f0 * f1));
+    for (int u = 0; u < ADJ_U; ++u) {
+      const uint32_t q = q0 + u * ADJ_TPB + threadIdx.x;
+      pp[u] = -1;
+      if (q < cnt) {
+        rr[u] = rec[start + q];
+        pp[u] = pidx[start + q];
+      }
     }
-    acc_cot[3 * p + 0] = acc_cot[3 * p + 0] + pmx_tree8(gx) / g.cell;
-    acc_cot[3 * p + 1] = acc_cot[3 * p + 1] + pmx_tree8(gy) / g.cell;
-    acc_cot[3 * p + 2] = acc_cot[3 * p + 2] + pmx_tree8(gz) / g.cell;
+#pragma unroll
+    for (int u = 0; u < ADJ_U; ++u)
+      if (pp[u] >= 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ac[u][a] = acc_cot[3 * pp[u] + a];
+      }
+#pragma unroll
+    for (int u = 0; u < ADJ_U; ++u) {
+      if (pp[u] < 0) continue;
+      const float4 r = rr[u];
+      const long long p = pp[u];
+      int lx, ly, lz;
+      pmx_unpack_local(r.w, lx, ly, lz);
+      if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
+      float wa[2], wb[2], wc[2], sa[2], sb[2], sc[2];
+      frac_sign_of(r.x, wa[0], wa[1], sa[0], sa[1]);
+      frac_sign_of(r.y, wb[0], wb[1], sb[0], sb[1]);
+      frac_sign_of(r.z, wc[0], wc[1], sc[0], sc[1]);
+      const int b = (lx * LY + ly) * LZ + lz;
+      float gx[8], gy[8], gz[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int ja = j & 1, jb = (j >> 1) & 1, jc = (j >> 2) & 1;
+        const float f0 = wa[ja], f1 = wb[jb], f2 = wc[jc];
+        const int li = b + (ja ? LY * LZ : 0) + (jb ? LZ : 0) + jc;
+        const float dc = sh[li] * val;
+        gx[j] = dc * (sa[ja] * (f1 * f2));
+        gy[j] = dc * (sb[jb] * (f2 * f0));
+        gz[j] = dc * (sc[jc] * (f0 * f1));
+      }
+      acc_cot[3 * p + 0] = ac[u][0] + pmx_tree8(gx) / g.cell;
+      acc_cot[3 * p + 1] = ac[u][1] + pmx_tree8(gy) / g.cell;
+      acc_cot[3 * p + 2] = ac[u][2] + pmx_tree8(gz) / g.cell;
+    }
   }
 }
 
@@ -314,8 +361,17 @@ __global__ void __launch_bounds__(PMX_TPB)
                  MeshGeom g, TileGeom tg, uint32_t* __restrict__ tile_count,
                  uint32_t* __restrict__ ptile, uint32_t* __restrict__ prank,
                  double* __restrict__ part) {
-  const long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
-  const bool active = i < n;
+  constexpr int T3 = 3 * PMX_TPB;
+  // vel | acc | disp | disp_cot | vel_cot | acc_cot, staged through LDS
+  __shared__ __align__(16) float sh[6 * T3];
+  __shared__ __align__(16) int16_t sp[T3];
+  const long long b0 = (long long)blockIdx.x * PMX_TPB;
+  const int cnt = (int)(n - b0 < PMX_TPB ? n - b0 : PMX_TPB);
+  pmx_stage_in3<6, PMX_TPB>(sh, {{vel, acc, disp, disp_cot, vel_cot, acc_cot}}, b0, cnt);
+  pmx_stage_pmid<PMX_TPB>(sp, pmid, b0, cnt);
+  __syncthreads();
+  const int q = threadIdx.x;
+  const bool active = q < cnt;
   double s = 0;
   uint32_t t = 0;
   if (active) {
@@ -323,26 +379,27 @@ __global__ void __launch_bounds__(PMX_TPB)
     float dp[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      pm[a] = pmid[3 * i + a];
-      const float ac = acc[3 * i + a], acot = acc_cot[3 * i + a];
-      float v = vel[3 * i + a];
-      float dc = disp_cot[3 * i + a];
+      const int e = 3 * q + a;
+      pm[a] = sp[e];
+      const float ac = sh[T3 + e], acot = sh[5 * T3 + e];
+      float v = sh[e];
+      float dc = sh[3 * T3 + e];
       if (HAS_KP) {
         v = v + ac * kp;
         dc = dc - acot * kp;
       }
       v = v + ac * ka;
       dc = dc - acot * ka;
-      float x = disp[3 * i + a];
+      float x = sh[2 * T3 + e];
       x = x + v * d;
-      float vc = vel_cot[3 * i + a];
+      float vc = sh[4 * T3 + e];
       vc = vc - dc * d;
       s += (double)dc * v;
-      if (snap) x = snap[3 * i + a];
-      vel[3 * i + a] = v;
-      disp_cot[3 * i + a] = dc;
-      disp[3 * i + a] = x;
-      vel_cot[3 * i + a] = vc;
+      if (snap) x = snap[3 * (b0 + q) + a];
+      sh[e] = v;
+      sh[2 * T3 + e] = x;
+      sh[3 * T3 + e] = dc;
+      sh[4 * T3 + e] = vc;
       dp[a] = x;
     }
     CicAxis ax[3];
@@ -351,9 +408,12 @@ __global__ void __launch_bounds__(PMX_TPB)
   }
   const uint32_t r = agg_slot(tile_count, t, active);
   if (active) {
-    ptile[i] = t;
-    prank[i] = r;
+    ptile[b0 + q] = t;
+    prank[b0 + q] = r;
   }
+  __syncthreads();
+  pmx_stage_out3<1, PMX_TPB>({{vel}}, sh, b0, cnt);
+  pmx_stage_out3<3, PMX_TPB>({{disp, disp_cot, vel_cot}}, sh + 2 * T3, b0, cnt);
   block_sum_store<PMX_TPB>(s, part);
 }
 

@@ -77,44 +77,57 @@ extern "C" int pmx_tile_keys(pmx_ctx* c, const int16_t* pmid, const float* disp,
 }
 
 // ---------------------------------------------------- binned scatter --
-__global__ void bin_count_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp,
-                            long long n, MeshGeom g, TileGeom tg, uint32_t* __restrict__ tile_count,
-                            uint32_t* __restrict__ ptile, uint32_t* __restrict__ prank) {
-  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
-  bool active = i < n;
+__global__ void __launch_bounds__(PMX_TPB)
+    bin_count_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp, long long n,
+                MeshGeom g, TileGeom tg, uint32_t* __restrict__ tile_count,
+                uint32_t* __restrict__ ptile, uint32_t* __restrict__ prank) {
+  __shared__ __align__(16) float sd[3 * PMX_TPB];
+  __shared__ __align__(16) int16_t sp[3 * PMX_TPB];
+  const long long b0 = (long long)blockIdx.x * PMX_TPB;
+  const int cnt = (int)(n - b0 < PMX_TPB ? n - b0 : PMX_TPB);
+  pmx_stage_in3<1, PMX_TPB>(sd, {{disp}}, b0, cnt);
+  pmx_stage_pmid<PMX_TPB>(sp, pmid, b0, cnt);
+  __syncthreads();
+  const int q = threadIdx.x;
+  const bool active = q < cnt;
   uint32_t t = 0;
   if (active) {
-    int pm[3];
-    float dp[3];
-    load_ptcl(pmid, disp, i, pm, dp);
+    const int pm[3] = {sp[3 * q], sp[3 * q + 1], sp[3 * q + 2]};
+    const float dp[3] = {sd[3 * q], sd[3 * q + 1], sd[3 * q + 2]};
     CicAxis ax[3];
     cic_setup(pm, dp, g, ax);
     t = tile_of(ax, tg);
   }
-  uint32_t r = agg_slot(tile_count, t, active);
+  const uint32_t r = agg_slot(tile_count, t, active);
   if (active) {
-    ptile[i] = t;
-    prank[i] = r;
+    ptile[b0 + q] = t;
+    prank[b0 + q] = r;
   }
 }
 
 // Record of a binned particle: d = disp/cell per axis (float32, the value
 // enmesh divides out) and the lower CIC corner relative to the tile origin
 // packed as lx | ly << 10 | lz << 20 (bit pattern stored in .w).
-__global__ void bin_fill_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp,
-                           long long n, MeshGeom g, TileGeom tg, const uint32_t* __restrict__ ptile,
-                           const uint32_t* __restrict__ prank,
-                           const uint32_t* __restrict__ tile_start, float4* __restrict__ rec,
-                           const float* __restrict__ pval, int vstride, int nch,
-                           float* __restrict__ vrec, float* __restrict__ vmax,
-                           uint32_t* __restrict__ pidx) {
-  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
-  const bool active = i < n;
+__global__ void __launch_bounds__(PMX_TPB)
+    bin_fill_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp, long long n,
+               MeshGeom g, TileGeom tg, const uint32_t* __restrict__ ptile,
+               const uint32_t* __restrict__ prank, const uint32_t* __restrict__ tile_start,
+               float4* __restrict__ rec, const float* __restrict__ pval, int vstride, int nch,
+               float* __restrict__ vrec, float* __restrict__ vmax, uint32_t* __restrict__ pidx) {
+  __shared__ __align__(16) float sd[3 * PMX_TPB];
+  __shared__ __align__(16) int16_t sp[3 * PMX_TPB];
+  const long long b0 = (long long)blockIdx.x * PMX_TPB;
+  const int cnt = (int)(n - b0 < PMX_TPB ? n - b0 : PMX_TPB);
+  pmx_stage_in3<1, PMX_TPB>(sd, {{disp}}, b0, cnt);
+  pmx_stage_pmid<PMX_TPB>(sp, pmid, b0, cnt);
+  __syncthreads();
+  const int q = threadIdx.x;
+  const long long i = b0 + q;
+  const bool active = q < cnt;
   float vabs[3] = {0.f, 0.f, 0.f};
   if (active) {
-    int pm[3];
-    float dp[3];
-    load_ptcl(pmid, disp, i, pm, dp);
+    const int pm[3] = {sp[3 * q], sp[3 * q + 1], sp[3 * q + 2]};
+    const float dp[3] = {sd[3 * q], sd[3 * q + 1], sd[3 * q + 2]};
     CicAxis ax[3];
     cic_setup(pm, dp, g, ax);
     uint32_t t = ptile[i];
@@ -258,6 +271,7 @@ __global__ void __launch_bounds__(DEP_TPB)
 // pmx_tree8, written to the particle's index; KICK also closes the step,
 // vel += acc * k2.  Same arithmetic as a per-particle gather.
 #define TG_TPB 512
+#define TG_U 4
 template <int KICK>
 __global__ void __launch_bounds__(TG_TPB)
     tile_gather_k(const float4* __restrict__ rec, const uint32_t* __restrict__ pidx,
@@ -276,34 +290,59 @@ __global__ void __launch_bounds__(TG_TPB)
   __syncthreads();
   const uint32_t start = tile_start[t];
   const uint32_t cnt = tile_count[t];
-  for (uint32_t q = threadIdx.x; q < cnt; q += TG_TPB) {
-    float4 r = rec[start + q];
-    long long p = pidx[start + q];
-    uint32_t pk = __float_as_uint(r.w);
-    int lx = pk & 1023, ly = (pk >> 10) & 1023, lz = (pk >> 20) & 1023;
-    if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
-    float a0, a1, b0, b1, c0, c1;
-    frac_of(r.x, a0, a1);
-    frac_of(r.y, b0, b1);
-    frac_of(r.z, c0, c1);
-    int b = (lx * LY + ly) * LZ + lz;
-    float v0[8], v1[8], v2[8];
+  // TG_U records per thread per round: their loads (record, index, then
+  // velocity) are all issued before the arithmetic
+  for (uint32_t q0 = 0; q0 < cnt; q0 += TG_TPB * TG_U) {
+    float4 r[TG_U];
+    long long pp[TG_U];
+    float vv[TG_U][3];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      float f0 = (j & 1) ? a1 : a0;
-      float f1 = (j & 2) ? b1 : b0;
-      float f2 = (j & 4) ? c1 : c0;
-      float w = (f0 * f1) * f2;
-      int li = b + ((j & 1) ? LY * LZ : 0) + ((j & 2) ? LZ : 0) + ((j & 4) ? 1 : 0);
-      v0[j] = sh[li] * w;
-      v1[j] = sh[ltot + li] * w;
-      v2[j] = sh[2 * ltot + li] * w;
+    for (int u = 0; u < TG_U; ++u) {
+      const uint32_t q = q0 + u * TG_TPB + threadIdx.x;
+      pp[u] = -1;
+      if (q < cnt) {
+        r[u] = rec[start + q];
+        pp[u] = pidx[start + q];
+      }
     }
-    float o3[3] = {pmx_tree8(v0), pmx_tree8(v1), pmx_tree8(v2)};
+    if (KICK) {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      acc[3 * p + a] = o3[a];
-      if (KICK) vel[3 * p + a] = vel[3 * p + a] + o3[a] * k2;
+      for (int u = 0; u < TG_U; ++u)
+        if (pp[u] >= 0) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) vv[u][a] = vel[3 * pp[u] + a];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < TG_U; ++u) {
+      if (pp[u] < 0) continue;
+      const long long p = pp[u];
+      int lx, ly, lz;
+      pmx_unpack_local(r[u].w, lx, ly, lz);
+      if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
+      float a0, a1, b0, b1, c0, c1;
+      frac_of(r[u].x, a0, a1);
+      frac_of(r[u].y, b0, b1);
+      frac_of(r[u].z, c0, c1);
+      const int b = (lx * LY + ly) * LZ + lz;
+      float v0[8], v1[8], v2[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        float f0 = (j & 1) ? a1 : a0;
+        float f1 = (j & 2) ? b1 : b0;
+        float f2 = (j & 4) ? c1 : c0;
+        float w = (f0 * f1) * f2;
+        int li = b + ((j & 1) ? LY * LZ : 0) + ((j & 2) ? LZ : 0) + ((j & 4) ? 1 : 0);
+        v0[j] = sh[li] * w;
+        v1[j] = sh[ltot + li] * w;
+        v2[j] = sh[2 * ltot + li] * w;
+      }
+      const float o3[3] = {pmx_tree8(v0), pmx_tree8(v1), pmx_tree8(v2)};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        acc[3 * p + a] = o3[a];
+        if (KICK) vel[3 * p + a] = vv[u][a] + o3[a] * k2;
+      }
     }
   }
 }
@@ -502,38 +541,50 @@ extern "C" int pmx_drift(pmx_ctx* c, long long n, float* disp, const float* vel,
   return PMX_OK;
 }
 
-// kick (k1) + drift (d) fused with the tile count of the drifted particles
-__global__ void kick_drift_bin_k(const int16_t* __restrict__ pmid, float* __restrict__ disp,
-                                 float* __restrict__ vel, const float* __restrict__ acc, long long n,
-                                 float k1, float dd, MeshGeom g, TileGeom tg,
-                                 uint32_t* __restrict__ tile_count, uint32_t* __restrict__ ptile,
-                                 uint32_t* __restrict__ prank) {
-  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
-  bool active = i < n;
+// kick (k1) + drift (d) fused with the tile count of the drifted particles;
+// the block's vel, acc and disp rows are staged through LDS
+__global__ void __launch_bounds__(PMX_TPB)
+    kick_drift_bin_k(const int16_t* __restrict__ pmid, float* __restrict__ disp,
+                     float* __restrict__ vel, const float* __restrict__ acc, long long n, float k1,
+                     float dd, MeshGeom g, TileGeom tg, uint32_t* __restrict__ tile_count,
+                     uint32_t* __restrict__ ptile, uint32_t* __restrict__ prank) {
+  constexpr int T3 = 3 * PMX_TPB;
+  __shared__ __align__(16) float sh[3 * T3];  // vel | acc | disp
+  __shared__ __align__(16) int16_t sp[T3];
+  const long long b0 = (long long)blockIdx.x * PMX_TPB;
+  const int cnt = (int)(n - b0 < PMX_TPB ? n - b0 : PMX_TPB);
+  pmx_stage_in3<3, PMX_TPB>(sh, {{vel, acc, disp}}, b0, cnt);
+  pmx_stage_pmid<PMX_TPB>(sp, pmid, b0, cnt);
+  __syncthreads();
+  const int q = threadIdx.x;
+  const bool active = q < cnt;
   uint32_t t = 0;
   if (active) {
     int pm[3];
     float dp[3];
-    pm[0] = pmid[3 * i]; pm[1] = pmid[3 * i + 1]; pm[2] = pmid[3 * i + 2];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      float v = vel[3 * i + a];
-      v = v + acc[3 * i + a] * k1;
-      float x = disp[3 * i + a];
+      pm[a] = sp[3 * q + a];
+      float v = sh[3 * q + a];
+      v = v + sh[T3 + 3 * q + a] * k1;
+      float x = sh[2 * T3 + 3 * q + a];
       x = x + v * dd;
-      vel[3 * i + a] = v;
-      disp[3 * i + a] = x;
+      sh[3 * q + a] = v;
+      sh[2 * T3 + 3 * q + a] = x;
       dp[a] = x;
     }
     CicAxis ax[3];
     cic_setup(pm, dp, g, ax);
     t = tile_of(ax, tg);
   }
-  uint32_t r = agg_slot(tile_count, t, active);
+  const uint32_t r = agg_slot(tile_count, t, active);
   if (active) {
-    ptile[i] = t;
-    prank[i] = r;
+    ptile[b0 + q] = t;
+    prank[b0 + q] = r;
   }
+  __syncthreads();
+  pmx_stage_out3<1, PMX_TPB>({{vel}}, sh, b0, cnt);
+  pmx_stage_out3<1, PMX_TPB>({{disp}}, sh + 2 * T3, b0, cnt);
 }
 
 extern "C" int pmx_nbody_step(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, float* acc,

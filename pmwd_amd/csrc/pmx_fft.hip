@@ -490,6 +490,19 @@ struct Fields3 {
   float2* f[3];
 };
 
+// Block coordinates of a (gx, gy, gz) grid after the XCD-contiguous remap
+// of the linear block index: neighbouring column blocks, which share the
+// 128-byte lines of a row, then run on one XCD close together and meet in
+// its L2.
+__device__ __forceinline__ void xcd_block3(int& bx, int& by, int& bz) {
+  const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const unsigned l = pmx_xcd_block(lin, nb);
+  bx = l % gridDim.x;
+  by = (l / gridDim.x) % gridDim.y;
+  bz = l / (gridDim.x * gridDim.y);
+}
+
 // In-place FFT along a strided axis of a complex (outer, n, ncol) view:
 // element j of pencil (o, c) is at o*ostride + j*es + c.  A workgroup takes
 // B consecutive columns, so each of its loads is B contiguous float2.
@@ -500,9 +513,11 @@ __global__ void __launch_bounds__(FFT_TPB, 4) strided_fft_k(Fields3 fl, int n, l
                                                          const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
   float2* stw = s + n * B;
-  float2* base = fl.f[blockIdx.z] + (long long)blockIdx.y * ostride;
+  int bx, by, bz;
+  xcd_block3(bx, by, bz);
+  float2* base = (bz == 0 ? fl.f[0] : (bz == 1 ? fl.f[1] : fl.f[2])) + (long long)by * ostride;
   const int lB = __ffs(B) - 1;
-  int c0 = blockIdx.x * B;
+  int c0 = bx * B;
   int nb = ncol - c0 < B ? ncol - c0 : B;
   stage_twiddles<FFT_TPB>(stw, tw, n);
   batch_copy<FFT_MAXV, FFT_TPB>(
@@ -542,8 +557,9 @@ __global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_k(Fields3 fl, int n0, in
   float2* stw = sp + total;
   const long long es = (long long)n1 * n2p;
   const int lB = __ffs(B) - 1;
-  const int y = blockIdx.y;
-  const int c0 = blockIdx.x * B;
+  int bx, y, bz;
+  xcd_block3(bx, y, bz);
+  const int c0 = bx * B;
   const int nb = n2h - c0 < B ? n2h - c0 : B;
   const long long off = (long long)y * n2p + c0;
   stage_twiddles<FFT_TPB>(stw, tw, n0);
@@ -615,8 +631,9 @@ __global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_adj_k(Fields3 fl, int n0
   float2* stw = sa + total;
   const long long es = (long long)n1 * n2p;
   const int lB = __ffs(B) - 1;
-  const int y = blockIdx.y;
-  const int c0 = blockIdx.x * B;
+  int bx, y, bz;
+  xcd_block3(bx, y, bz);
+  const int c0 = bx * B;
   const int nb = n2h - c0 < B ? n2h - c0 : B;
   const long long off = (long long)y * n2p + c0;
   const float ky = kt.k[1][y];

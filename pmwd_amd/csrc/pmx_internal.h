@@ -264,21 +264,101 @@ __device__ __forceinline__ uint32_t tile_of(const CicAxis ax[3], const TileGeom&
 
 // Wave-aggregated atomicAdd(&ctr[key], 1): returns this lane's slot.
 // Every lane of the wave must call it (inactive lanes pass active=false).
+// The lanes are grouped by key with ballots first; then each group's leader
+// adds the group size, all leaders in one instruction, so the wave waits for
+// a single atomic round trip however many keys it holds.
 __device__ __forceinline__ uint32_t agg_slot(uint32_t* __restrict__ ctr, uint32_t key, bool active) {
   unsigned long long pending = __ballot(active);
   const int lane = __lane_id();
-  uint32_t slot = 0;
+  int my_leader = lane;
+  uint32_t my_rank = 0, group = 0;
   while (pending) {
-    int leader = __ffsll((long long)pending) - 1;
-    uint32_t lkey = __shfl(key, leader);
-    unsigned long long peers = __ballot(active && key == lkey) & pending;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&ctr[lkey], (uint32_t)__popcll(peers));
-    base = __shfl(base, leader);
-    if (active && key == lkey) slot = base + (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+    const int leader = __ffsll((long long)pending) - 1;
+    const uint32_t lkey = __shfl(key, leader);
+    const unsigned long long peers = __ballot(active && key == lkey) & pending;
+    if (active && key == lkey) {
+      my_leader = leader;
+      my_rank = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+    }
+    if (lane == leader) group = (uint32_t)__popcll(peers);
     pending &= ~peers;
   }
-  return slot;
+  uint32_t base = 0;
+  if (group) base = atomicAdd(&ctr[key], group);
+  base = __shfl(base, my_leader);
+  return base + my_rank;
+}
+
+// ------------------------------------------------ AoS staging through LDS --
+// A block of TPB consecutive particles holds 3*TPB floats of each (n,3)
+// array.  These move whole blocks between HBM and LDS with 16-byte accesses
+// (3*TPB/4 lanes) when the block is full and its rows are 16-byte aligned,
+// else element by element; each thread then works on its particle's three
+// values in LDS.
+template <int TPB>
+__device__ __forceinline__ bool pmx_block_aligned(const void* p) {
+  return ((uintptr_t)p & 15) == 0;
+}
+
+// NS row pointers handed over by value (kept in registers)
+template <int NS, class T>
+struct PmxPtrs {
+  T* p[NS];
+};
+
+template <int NS, int TPB>
+__device__ __forceinline__ void pmx_stage_in3(float* sh, PmxPtrs<NS, const float> g, long long b0,
+                                              int cnt) {
+  bool vec = cnt == TPB;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) vec = vec && pmx_block_aligned<TPB>(g.p[k] + 3 * b0);
+  if (vec) {
+    if (threadIdx.x < 3 * TPB / 4) {
+      float4 r[NS];
+#pragma unroll
+      for (int k = 0; k < NS; ++k) r[k] = reinterpret_cast<const float4*>(g.p[k] + 3 * b0)[threadIdx.x];
+#pragma unroll
+      for (int k = 0; k < NS; ++k) reinterpret_cast<float4*>(sh + k * 3 * TPB)[threadIdx.x] = r[k];
+    }
+  } else {
+    for (int i = threadIdx.x; i < 3 * cnt; i += TPB) {
+#pragma unroll
+      for (int k = 0; k < NS; ++k) sh[k * 3 * TPB + i] = g.p[k][3 * b0 + i];
+    }
+  }
+}
+
+template <int NS, int TPB>
+__device__ __forceinline__ void pmx_stage_out3(PmxPtrs<NS, float> g, const float* sh, long long b0,
+                                               int cnt) {
+  bool vec = cnt == TPB;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) vec = vec && pmx_block_aligned<TPB>(g.p[k] + 3 * b0);
+  if (vec) {
+    if (threadIdx.x < 3 * TPB / 4) {
+#pragma unroll
+      for (int k = 0; k < NS; ++k)
+        reinterpret_cast<float4*>(g.p[k] + 3 * b0)[threadIdx.x] =
+            reinterpret_cast<const float4*>(sh + k * 3 * TPB)[threadIdx.x];
+    }
+  } else {
+    for (int i = threadIdx.x; i < 3 * cnt; i += TPB) {
+#pragma unroll
+      for (int k = 0; k < NS; ++k) g.p[k][3 * b0 + i] = sh[k * 3 * TPB + i];
+    }
+  }
+}
+
+// pmid (n,3) int16: 6*TPB bytes per block, 16-byte accesses by 3*TPB/8 lanes
+template <int TPB>
+__device__ __forceinline__ void pmx_stage_pmid(int16_t* sp, const int16_t* pmid, long long b0, int cnt) {
+  const int16_t* src = pmid + 3 * b0;
+  if (cnt == TPB && pmx_block_aligned<TPB>(src)) {
+    if (threadIdx.x < 3 * TPB / 8)
+      reinterpret_cast<float4*>(sp)[threadIdx.x] = reinterpret_cast<const float4*>(src)[threadIdx.x];
+  } else {
+    for (int i = threadIdx.x; i < 3 * cnt; i += TPB) sp[i] = src[i];
+  }
 }
 
 // per-axis fractions from d = disp/cell exactly as pmx_cic_axis forms them
@@ -312,22 +392,42 @@ __device__ __forceinline__ void frac_sign_of(float d, float& w0, float& w1, floa
   s1 = pmx_sign(-d21);
 }
 
+// Largest tile edge of the scatter binning, and the cells of a tile plus
+// its +1 layer.
+#define PMX_TILE_MAX 16
+#define PMX_TILE_CELLS ((PMX_TILE_MAX + 1) * (PMX_TILE_MAX + 1) * (PMX_TILE_MAX + 1))
+
 // Stages NC padded meshes' values over tile t's cells plus the +1 layer
 // (periodic wrap) into LDS, component-major: sh[c*ltot + (qx*LY + qy)*LZ + qz].
+// All of a thread's loads are issued before its LDS stores.
 template <int NC, int TPB>
 __device__ __forceinline__ void pmx_stage_tile(float* sh, const int o[3], int LY, int LZ, int ltot,
                                                const MeshGeom& g, const float* const* F) {
-  for (int i = threadIdx.x; i < ltot; i += TPB) {
-    int qz = i % LZ;
-    int qy = (i / LZ) % LY;
-    int qx = i / (LZ * LY);
-    int x = o[0] + qx, y = o[1] + qy, z = o[2] + qz;
-    if (x >= g.n0) x -= g.n0;
-    if (y >= g.n1) y -= g.n1;
-    if (z >= g.n2) z -= g.n2;
-    long long m = ((long long)x * g.n1 + y) * g.rs + z;
+  constexpr int NI = (PMX_TILE_CELLS + TPB - 1) / TPB;
+  float r[NI][NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) sh[c * ltot + i] = F[c][m];
+  for (int it = 0; it < NI; ++it) {
+    const int i = threadIdx.x + it * TPB;
+    if (i < ltot) {
+      int qz = i % LZ;
+      int qy = (i / LZ) % LY;
+      int qx = i / (LZ * LY);
+      int x = o[0] + qx, y = o[1] + qy, z = o[2] + qz;
+      if (x >= g.n0) x -= g.n0;
+      if (y >= g.n1) y -= g.n1;
+      if (z >= g.n2) z -= g.n2;
+      const long long m = ((long long)x * g.n1 + y) * g.rs + z;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) r[it][c] = F[c][m];
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < NI; ++it) {
+    const int i = threadIdx.x + it * TPB;
+    if (i < ltot) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) sh[c * ltot + i] = r[it][c];
+    }
   }
 }
 

@@ -9,6 +9,7 @@ import numpy as np
 import torch
 
 from .boltzmann import growth
+from .cosmo_param import check_theta, host_value, lpt_growth_derivs
 from .cosmology import E2
 from .gravity import laplace, neg_grad, nyquist_mask
 from .particles import Particles
@@ -40,8 +41,12 @@ def _L(kvec, pot, conf):
     return L
 
 
-def lpt(modes, cosmo, conf):
-    """LPT at ``conf.lpt_order`` (lpt.py:136-212); returns (ptcl, obsvbl)."""
+def lpt(modes, cosmo, conf, theta=None):
+    """LPT at ``conf.lpt_order`` (lpt.py:136-212); returns (ptcl, obsvbl).
+
+    ``theta`` ({'Omega_m': t}) makes the growth factors differentiable with
+    respect to Omega_m; see ``cosmo_param``.
+    """
     if conf.dim != 3:
         raise ValueError(f'dim={conf.dim} not supported')
     if conf.lpt_order not in (0, 1, 2, 3):
@@ -59,10 +64,16 @@ def lpt(modes, cosmo, conf):
     ptcl = Particles.gen_grid(conf, vel=True, device=dev)
     disp, vel = ptcl.disp.clone(), ptcl.vel.clone()
     a = conf.a_start
+    theta = check_theta(theta, cosmo)
     for order in range(1, 1 + conf.lpt_order):
         D = float(np.float32(growth(a, cosmo, conf, order=order)))
         dD = growth(a, cosmo, conf, order=order, deriv=1)
         a2HDp = float(np.float32(a**2 * np.sqrt(E2(a, cosmo)) * dD))
+        if theta is not None and 'Omega_m' in theta:
+            _, D_om, _, v_om = lpt_growth_derivs(a, cosmo, conf, order)
+            f32 = lambda x: torch.tensor(x, dtype=conf.float_dtype, device=dev)
+            D = host_value(f32(D), theta, torch.tensor(D_om, dtype=torch.float64))
+            a2HDp = host_value(f32(a2HDp), theta, torch.tensor(v_om, dtype=torch.float64))
         for i, k in enumerate(kvec):
             g = neg_grad(k, pots[order - 1], conf.ptcl_spacing)
             g = fftinv(g, shape=conf.ptcl_grid_shape).to(conf.float_dtype).reshape(-1)

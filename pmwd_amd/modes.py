@@ -12,6 +12,7 @@ import numpy as np
 import torch
 
 from .boltzmann import growth
+from .cosmo_param import check_theta, host_value, modes_factor_derivs
 from .particles import default_device
 from .pm_util import fftfreq, fftfwd, fftinv
 
@@ -55,8 +56,13 @@ def linear_power_k(k, a, cosmo, conf):
     return P
 
 
-def linear_modes(modes, cosmo, conf, a=None, real=False):
-    """sqrt(V Plin(k)) * white-noise modes (modes.py:67-117)."""
+def linear_modes(modes, cosmo, conf, a=None, real=False, theta=None):
+    """sqrt(V Plin(k)) * white-noise modes (modes.py:67-117).
+
+    ``theta`` ({'Omega_m': t, 'sigma8': t} of 0-d float64 tensors) makes the
+    result differentiable with respect to those parameters (A_s follows
+    sigma8 as in Cosmology.from_sigma8); see ``cosmo_param``.
+    """
     if cosmo.transfer is None:
         raise ValueError('Transfer table is empty. Call transfer_integ or boltzmann first.')
     if not torch.is_complex(modes):
@@ -69,6 +75,10 @@ def linear_modes(modes, cosmo, conf, a=None, real=False):
     k = torch.sqrt(k2).to(torch.float64)
     P = linear_power_k(k, a, cosmo, conf)
     fac = torch.sqrt(P * conf.box_vol).to(conf.float_dtype)
+    theta = check_theta(theta, cosmo)
+    if theta is not None:
+        d_om, d_s8 = modes_factor_derivs(k, a, cosmo, conf, linear_power_k)
+        fac = host_value(fac, theta, d_om, d_s8)
     modes = modes * fac
     if real:
         modes = fftinv(modes, shape=conf.ptcl_grid_shape, norm=conf.ptcl_spacing)

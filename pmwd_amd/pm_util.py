@@ -80,6 +80,52 @@ def _irfftn_raw(f, shape):
     return out
 
 
+def _interior(n2):
+    """Last-axis bins of the r2c layout that stand for two full-spectrum bins
+    (all but k=0 and, for even n2, k=n2/2)."""
+    return slice(1, (n2 + 1) // 2)
+
+
+def r2c_adjoint(g, shape, irfftn):
+    """Adjoint of the unnormalised real-to-complex transform: N irfftn(g) with
+    the interior last-axis bins halved.  With irfftn dropping the imaginary
+    parts of the self-conjugate bins, this is Re(A^H g) for real inputs."""
+    g = g.clone()
+    g[..., _interior(shape[-1])] *= 0.5
+    return irfftn(g.contiguous(), shape) * math.prod(shape)
+
+
+def c2r_adjoint(y, shape, rfftn):
+    """Adjoint of the 1/N-scaled complex-to-real transform: rfftn(y)/N with the
+    interior last-axis bins doubled (the conjugate-Wirtinger gradient torch
+    expects for complex inputs)."""
+    g = rfftn(y.contiguous()) * (1 / math.prod(shape))
+    g[..., _interior(shape[-1])] *= 2
+    return g
+
+
+class _RFFTn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, f):
+        ctx.shape = tuple(f.shape)
+        return _rfftn_raw(f)
+
+    @staticmethod
+    def backward(ctx, g):
+        return r2c_adjoint(g, ctx.shape, _irfftn_raw)
+
+
+class _IRFFTn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, f, shape):
+        ctx.shape = shape
+        return _irfftn_raw(f, shape)
+
+    @staticmethod
+    def backward(ctx, y):
+        return c2r_adjoint(y, ctx.shape, _rfftn_raw), None
+
+
 def fftfwd(f, shape=None, axes=None, norm=None):
     """rfftn of a real field (pm_util.py:236-289)."""
     if torch.is_complex(f):
@@ -88,7 +134,7 @@ def fftfwd(f, shape=None, axes=None, norm=None):
         raise NotImplementedError('only the full 3D transform is implemented')
     if shape is not None and tuple(shape) != tuple(f.shape):
         raise NotImplementedError('zero-padding/cropping (s != shape) is not implemented')
-    out = _rfftn_raw(f)
+    out = _RFFTn.apply(f)
     N = math.prod(f.shape)
     if norm in (None, 'backward'):
         return out
@@ -108,7 +154,7 @@ def fftinv(f, shape=None, axes=None, norm=None):
     if shape is None:
         shape = (f.shape[0], f.shape[1], 2 * (f.shape[2] - 1))
     shape = tuple(int(s) for s in shape)
-    out = _irfftn_raw(f, shape)
+    out = _IRFFTn.apply(f, shape)
     N = math.prod(shape)
     if norm in (None, 'backward'):
         return out

@@ -97,3 +97,57 @@ def test_poisson_plane_wave(dev):
     for d in range(3):
         want = -1.5 * Om * A * np.sin(phase) * kv[d] / k2
         np.testing.assert_allclose(out[d].cpu().numpy(), want, atol=2e-7 + 2e-5 * np.abs(want).max())
+
+
+@pytest.mark.parametrize('n', [256, 512])
+def test_fft_production_sizes_vs_rocfft(dev, n):
+    """r2c/c2r at the benchmark's mesh sizes against torch.fft (rocFFT) on the
+    same device, float32 both."""
+    import torch
+    from pmwd_amd.pm_util import fftfwd, fftinv
+    shape = (n, n, n)
+    x = torch.randn(shape, generator=torch.Generator(device=dev).manual_seed(n), device=dev)
+    ref = torch.fft.rfftn(x).contiguous()
+    rel = lambda a, b: float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+    assert rel(fftfwd(x), ref) < 1e-6
+    assert rel(fftinv(ref, shape=shape), torch.fft.irfftn(ref.clone(), s=shape)) < 1e-6
+
+
+def _poisson_torch(dens, coef, cell):
+    """gravity.py:51-67 in plain torch float32 (rocFFT): the three force meshes."""
+    import torch
+    shape = dens.shape
+    ks = []
+    for d, n in enumerate(shape):
+        f = (torch.fft.rfftfreq(n, d=cell) if d == 2 else torch.fft.fftfreq(n, d=cell))
+        k = (2 * np.pi * f.double()).float().to(dens.device)
+        ks.append(k.reshape([-1 if i == d else 1 for i in range(3)]))
+    src = torch.fft.rfftn((dens - 1) * coef)
+    k2 = ks[0]**2 + ks[1]**2 + ks[2]**2
+    pot = torch.where(k2 != 0, -src / torch.where(k2 != 0, k2, 1), 0)
+    out = []
+    for d, n in enumerate(shape):
+        knyq = np.pi / cell
+        kd = torch.where((ks[d].abs() - knyq).abs() <= knyq * np.finfo(np.float32).eps,
+                         0, ks[d])
+        out.append(torch.fft.irfftn(-1j * kd * pot, s=shape))
+    return torch.stack(out)
+
+
+@pytest.mark.parametrize('n,cell', [(256, 2.0), (512, 1.953125)])
+def test_poisson_production_sizes_vs_torch(dev, n, cell):
+    """The fused Poisson solve at the benchmark's sizes against a plain
+    torch float32 restatement of gravity.py:51-67 on rocFFT."""
+    import torch
+    from pmwd_amd import _lib
+    shape = (n, n, n)
+    g = torch.Generator(device=dev).manual_seed(5)
+    dens = 1 + 0.5 * torch.randn(shape, generator=g, device=dev)
+    coef = float(np.float32(1.5) * np.float32(0.3))
+    c = _lib.context(shape, cell, dev)
+    out = torch.empty((3,) + shape, dtype=torch.float32, device=dev)
+    _lib.check(_lib.lib().pmx_poisson_grad(c.bind(), _lib.ptr(dens.contiguous()), coef,
+                                           _lib.ptr(out)))
+    want = _poisson_torch(dens, coef, cell)
+    rel = float(torch.linalg.vector_norm(out - want) / torch.linalg.vector_norm(want))
+    assert rel < 2e-6

@@ -135,3 +135,39 @@ def test_product_does_not_import_oracle():
             if f.endswith('.py'):
                 src = open(os.path.join(dp, f)).read()
                 assert 'oracle' not in re.findall(r'^\s*(?:from|import)\s+(\w+)', src, re.M), f
+
+
+@pytest.mark.parametrize('shape', [(4, 6, 8), (5, 4, 7), (3, 3, 2)])
+def test_fft_adjoint_formulas_match_torch_autograd(shape):
+    """The r2c/c2r adjoints used for the library FFTs' backward equal torch's
+    own autograd through torch.fft (CPU), for even and odd last axes."""
+    import torch
+    from pmwd_amd.pm_util import c2r_adjoint, r2c_adjoint
+    g = torch.Generator().manual_seed(3)
+    x = torch.randn(shape, dtype=torch.float64, generator=g, requires_grad=True)
+    hshape = shape[:-1] + (shape[-1] // 2 + 1,)
+    gc = torch.complex(torch.randn(hshape, dtype=torch.float64, generator=g),
+                       torch.randn(hshape, dtype=torch.float64, generator=g))
+    (want,) = torch.autograd.grad(torch.fft.rfftn(x), x, gc)
+    got = r2c_adjoint(gc, shape, lambda h, s: torch.fft.irfftn(h, s=s))
+    torch.testing.assert_close(got, want, rtol=1e-12, atol=1e-12)
+    h = gc.clone().requires_grad_(True)
+    y = torch.randn(shape, dtype=torch.float64, generator=g)
+    (want,) = torch.autograd.grad(torch.fft.irfftn(h, s=shape), h, y)
+    got = c2r_adjoint(y, shape, torch.fft.rfftn)
+    torch.testing.assert_close(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_fixed_sigma8_renormalisation():
+    """at_fixed_sigma8 moves Omega_m and keeps sigma8 (from_sigma8 chain)."""
+    import pmwd_amd as pa
+    from pmwd_amd.cosmo_param import at_fixed_sigma8
+    conf = pa.Configuration(4.0, (16, 16, 16), mesh_shape=2)
+    c = pa.boltzmann(pa.Cosmology.from_sigma8(conf, 0.8, n_s=0.96, Omega_m=0.3,
+                                              Omega_b=0.05, h=0.7), conf)
+    c2 = at_fixed_sigma8(c, conf, 0.31)
+    assert c2.Omega_m == 0.31
+    assert abs(c2.sigma8 - c.sigma8) < 1e-12
+    ref = pa.boltzmann(pa.Cosmology.from_sigma8(conf, c.sigma8, n_s=0.96, Omega_m=0.31,
+                                                Omega_b=0.05, h=0.7), conf)
+    assert abs(c2.A_s_1e9 / ref.A_s_1e9 - 1) < 1e-10
