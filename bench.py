@@ -33,6 +33,10 @@ def parse():
     p.add_argument('--box', type=float, default=1000.0)
     p.add_argument('--no-adjoint', action='store_true')
     p.add_argument('--no-cpu-baseline', action='store_true')
+    p.add_argument('--no-serve', action='store_true', help='skip the serving load')
+    p.add_argument('--serve-requests', type=int, default=40)
+    p.add_argument('--serve-rate', type=float, default=2.0, help='Poisson arrivals per second')
+    p.add_argument('--serve-slots', type=int, default=8)
     p.add_argument('--no-fft-yardstick', action='store_true',
                    help='skip timing rocFFT (torch.fft) next to the library FFTs')
     p.add_argument('--cpu-slab', type=int, default=64,
@@ -141,6 +145,38 @@ def cpu_baseline(conf, cosmo_om, slab):
                                        f'(scipy.fft, {cores} workers) + kick/drift/scatter/gather '
                                        f'of 1/{slab} of the particles ({ns}, numpy, 1 thread), '
                                        f'particle part scaled x{slab}')
+
+
+def serving_bench(dev, n, rate, slots):
+    """p50/p99 latency of the serving mix (pmwd_amd.serving.c5_mix: Poisson
+    arrivals; 40% 64^3/128^3, 40% 128^3/256^3, 20% 256^3/512^3 particles/mesh;
+    half with gradients; 31 steps) on one GPU after one warm-up request of
+    each size and kind (contexts resident, as in steady state)."""
+    import time
+    import numpy as np
+    from pmwd_amd import serving as sv
+    server = sv.Server(dev, max_slots=slots)
+    warm = [sv.SimRequest(seed=1, ptcl_grid=g, mesh_ratio=2, box=2.0 * g, Omega_m=0.3,
+                          sigma8=0.8, grad=gr) for g in (64, 128, 256) for gr in (False, True)]
+    for f in [server.submit(r) for r in warm]:
+        f.result()
+    sched = sv.c5_mix(n, rate, seed=0)
+    t0 = time.perf_counter()
+    res = sv.run_load(server, sched, t0)
+    t_end = max(r.t_done for r in res)
+    server.close()
+    out = sv.latency_stats(res, t0, t_end)
+    by = {}
+    for g in (64, 128, 256):
+        lat = [r.latency for r in res if r.request.ptcl_grid == g]
+        if lat:
+            by[f'{g}^3/{2 * g}^3'] = {'n': len(lat), 'p50_s': float(np.percentile(lat, 50)),
+                                      'p99_s': float(np.percentile(lat, 99))}
+    out.update(rate_per_s=rate, max_slots=slots, peak_concurrent=server.peak_active,
+               hbm_budget_gb=server.hbm_budget / 1e9, by_size=by,
+               mix='40% 64^3/128^3, 40% 128^3/256^3, 20% 256^3/512^3; half forward+gradient; '
+                   'Omega_m~U[0.25,0.35], sigma8~U[0.7,0.9]; 31 KDK steps; Poisson arrivals')
+    return out
 
 
 def fft_yardstick(conf, ctx, h, L, _lib, dev, reps=5):
@@ -398,6 +434,10 @@ def main():
         out['adj_ms_per_step'] = 1e3 * t_adj / K
         out['adj_kernels'] = {k: dict(ms_per_step=v[0] / K, launches_per_step=v[1] / K)
                               for k, v in atiming.items() if v[1]}
+
+    if not args.no_serve:
+        log(f'serving: {args.serve_requests} requests at {args.serve_rate}/s')
+        out['serving'] = serving_bench(dev, args.serve_requests, args.serve_rate, args.serve_slots)
 
     if not args.no_cpu_baseline:
         log('cpu baseline sample (numpy oracle)')

@@ -5,6 +5,7 @@ library binds to (same soname), and device pointers/streams are shared.
 There is no fallback: if the library is missing or cannot run, the call
 raises.
 """
+import contextlib
 import ctypes
 import os
 import threading
@@ -159,6 +160,22 @@ class Context:
 
 
 _ctx_cache = {}
+_tls = threading.local()
+
+
+@contextlib.contextmanager
+def context_scope(provider):
+    """Within the scope, ``context()`` on this thread returns
+    ``provider(key, make)`` instead of the process-wide cache: ``key`` is
+    (mesh shape, cell size, device index) and ``make()`` builds a new Context.
+    Concurrent users of one mesh shape (one per stream) each get their own
+    scratch this way."""
+    prev = getattr(_tls, 'provider', None)
+    _tls.provider = provider
+    try:
+        yield
+    finally:
+        _tls.provider = prev
 
 
 def context(mesh_shape, cell_size, device):
@@ -167,6 +184,9 @@ def context(mesh_shape, cell_size, device):
     if device.index is None:
         device = torch.device(device.type, torch.cuda.current_device())
     key = (tuple(int(s) for s in mesh_shape), float(cell_size), device.index)
+    provider = getattr(_tls, 'provider', None)
+    if provider is not None:
+        return provider(key, lambda: Context(mesh_shape, cell_size, device))
     ctx = _ctx_cache.get(key)
     if ctx is None:
         ctx = Context(mesh_shape, cell_size, device)

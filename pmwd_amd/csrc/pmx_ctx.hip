@@ -143,7 +143,10 @@ extern "C" int pmx_ctx_create(int n0, int n1, int n2, double cell_size, void* st
     pmx_ctx_destroy(c);
     return rc;
   }
-  hipError_t e = hipDeviceSynchronize();
+  // the table uploads are synchronous copies; surface any error of the set
+  // up without waiting for other streams' work
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) {
     pmx_set_error("pmx_ctx_create: %s", hipGetErrorString(e));
     pmx_ctx_destroy(c);

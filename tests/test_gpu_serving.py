@@ -1,0 +1,57 @@
+"""Serving: each request's result is the same alone and in a concurrent mix."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests():
+    from pmwd_amd.serving import SimRequest
+    kw = dict(a_start=1 / 8, a_nbody_maxstep=1 / 8)
+    return [SimRequest(seed=11, ptcl_grid=16, mesh_ratio=2, box=32.0, Omega_m=0.28, sigma8=0.75,
+                       grad=True, **kw),
+            SimRequest(seed=12, ptcl_grid=8, mesh_ratio=2, box=16.0, Omega_m=0.33, sigma8=0.85, **kw),
+            SimRequest(seed=13, ptcl_grid=16, mesh_ratio=1, box=32.0, Omega_m=0.30, sigma8=0.80, **kw),
+            SimRequest(seed=14, ptcl_grid=8, mesh_ratio=2, box=16.0, Omega_m=0.26, sigma8=0.72,
+                       grad=True, **kw),
+            SimRequest(seed=15, ptcl_grid=32, mesh_ratio=2, box=64.0, Omega_m=0.31, sigma8=0.88, **kw),
+            SimRequest(seed=16, ptcl_grid=16, mesh_ratio=2, box=32.0, Omega_m=0.34, sigma8=0.78,
+                       grad=True, **kw)]
+
+
+def test_results_same_alone_and_in_the_mix(dev):
+    from pmwd_amd.serving import Server
+    reqs = _requests()
+    solo = Server(dev, max_slots=1)
+    alone = [solo.submit(r).result() for r in reqs]
+    solo.close()
+    assert solo.peak_active == 1
+    mix_server = Server(dev, max_slots=len(reqs))
+    futs = [mix_server.submit(r) for r in reqs]
+    mix = [f.result() for f in futs]
+    mix_server.close()
+    assert mix_server.peak_active > 1
+    for a, m in zip(alone, mix):
+        assert a.request == m.request
+        np.testing.assert_array_equal(a.dens, m.dens)
+        assert np.isfinite(a.dens).all() and abs(a.dens.mean() - 1) < 1e-5
+        if a.request.grad:
+            assert a.loss == m.loss
+            np.testing.assert_array_equal(a.grad_modes, m.grad_modes)
+            # float64 tile partial sums may be added in another order
+            assert abs(a.grad_Omega_m - m.grad_Omega_m) <= 1e-9 * abs(a.grad_Omega_m)
+            assert a.grad_sigma8 == m.grad_sigma8 or \
+                abs(a.grad_sigma8 - m.grad_sigma8) <= 1e-9 * abs(a.grad_sigma8)
+
+
+def test_server_admits_under_a_tight_budget(dev):
+    """With room for one request at a time the server still completes them
+    all (serially), evicting idle contexts as needed."""
+    from pmwd_amd.serving import Server, request_bytes
+    reqs = _requests()[:3]
+    budget = max(request_bytes(r) for r in reqs) + 1
+    s = Server(dev, hbm_budget=budget, max_slots=3)
+    res = [f.result() for f in [s.submit(r) for r in reqs]]
+    s.close()
+    assert len(res) == 3 and s.peak_active == 1

@@ -171,3 +171,22 @@ def test_fixed_sigma8_renormalisation():
     ref = pa.boltzmann(pa.Cosmology.from_sigma8(conf, c.sigma8, n_s=0.96, Omega_m=0.31,
                                                 Omega_b=0.05, h=0.7), conf)
     assert abs(c2.A_s_1e9 / ref.A_s_1e9 - 1) < 1e-10
+
+
+def test_serving_load_mix():
+    """The serving load generator follows the stated mix and ranges."""
+    from pmwd_amd.serving import c5_mix, context_bytes, request_bytes
+    sched = c5_mix(4000, rate=2.0, seed=1)
+    t = np.array([s[0] for s in sched])
+    assert np.all(np.diff(t) > 0)
+    assert abs(np.mean(np.diff(t)) - 0.5) < 0.03
+    g = np.array([r.ptcl_grid for _, r in sched])
+    for size, frac in ((64, 0.4), (128, 0.4), (256, 0.2)):
+        assert abs(np.mean(g == size) - frac) < 0.03
+    assert abs(np.mean([r.grad for _, r in sched]) - 0.5) < 0.03
+    assert all(0.25 <= r.Omega_m <= 0.35 and 0.7 <= r.sigma8 <= 0.9 for _, r in sched)
+    assert all(r.mesh_ratio == 2 and r.box == 2.0 * r.ptcl_grid for _, r in sched)
+    small = next(r for _, r in sched if r.ptcl_grid == 64)
+    big = next(r for _, r in sched if r.ptcl_grid == 256)
+    assert request_bytes(big) > request_bytes(small)
+    assert context_bytes((512, 512, 512), 256**3, True) > context_bytes((512, 512, 512), 256**3, False)
