@@ -18,8 +18,9 @@
 #define FFT_MAXV (FFT_CAP / FFT_TPB) // values per thread
 #define RFFT_CAP 4096                // row passes: fewer values per thread
 #define RFFT_MAXV (RFFT_CAP / FFT_TPB)
-#define PFFT_CAP 4096                // Poisson x pass keeps per-thread state
-#define PFFT_MAXV (PFFT_CAP / FFT_TPB)
+#define PFFT_TPB 1024                // Poisson x pass: 16 columns of 512 per WG,
+#define PFFT_CAP 8192                // whole 128-byte segments per row
+#define PFFT_MAXV (PFFT_CAP / PFFT_TPB)
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -549,7 +550,7 @@ struct KTab {
 // LDS: the n0 x B block, the potential (each thread reads back only what it
 // wrote), then the n0-entry twiddle table.
 template <bool POW2>
-__global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_k(Fields3 fl, int n0, int n1, int n2h, int n2p, int B,
+__global__ void __launch_bounds__(PFFT_TPB) poisson_x_k(Fields3 fl, int n0, int n1, int n2h, int n2p, int B,
                                                        KTab kt, const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
   const int total = n0 * B;
@@ -562,8 +563,8 @@ __global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_k(Fields3 fl, int n0, in
   const int c0 = bx * B;
   const int nb = n2h - c0 < B ? n2h - c0 : B;
   const long long off = (long long)y * n2p + c0;
-  stage_twiddles<FFT_TPB>(stw, tw, n0);
-  batch_copy<PFFT_MAXV, FFT_TPB>(
+  stage_twiddles<PFFT_TPB>(stw, tw, n0);
+  batch_copy<PFFT_MAXV, PFFT_TPB>(
       total,
       [&](int idx) {
         int j = idx >> lB, b = idx & (B - 1);
@@ -571,12 +572,12 @@ __global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_k(Fields3 fl, int n0, in
       },
       [&](int idx, float2 v) { s[idx] = v; });
   __syncthreads();
-  lds_fft<-1, COLS, POW2, PFFT_MAXV>(s, n0, B, stw);
+  lds_fft<-1, COLS, POW2, PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
   const float ky = kt.k[1][y];
   const float kym = kt.km[1][y];
 #pragma unroll
   for (int it = 0; it < PFFT_MAXV; ++it) {
-    int idx = threadIdx.x + it * FFT_TPB;
+    int idx = threadIdx.x + it * PFFT_TPB;
     if (idx < total) {
       int j = idx >> lB, b = idx & (B - 1);
       float2 pot = make_float2(0.f, 0.f);
@@ -596,7 +597,7 @@ __global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_k(Fields3 fl, int n0, in
   for (int d = 0; d < 3; ++d) {
 #pragma unroll
     for (int it = 0; it < PFFT_MAXV; ++it) {
-      int idx = threadIdx.x + it * FFT_TPB;
+      int idx = threadIdx.x + it * PFFT_TPB;
       if (idx < total) {
         int j = idx >> lB, b = idx & (B - 1);
         float km = 0.f;
@@ -606,10 +607,10 @@ __global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_k(Fields3 fl, int n0, in
       }
     }
     __syncthreads();
-    lds_fft<1, COLS, POW2, PFFT_MAXV>(s, n0, B, stw);
+    lds_fft<1, COLS, POW2, PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
     // selects rather than indexes, so the argument struct stays in registers
     float2* out = d == 0 ? fl.f[0] : (d == 1 ? fl.f[1] : fl.f[2]);
-    batch_copy<PFFT_MAXV, FFT_TPB>(
+    batch_copy<PFFT_MAXV, PFFT_TPB>(
         total, [&](int idx) { return s[idx]; },
         [&](int idx, float2 v) {
           int j = idx >> lB, b = idx & (B - 1);
@@ -623,7 +624,7 @@ __global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_k(Fields3 fl, int n0, in
 // along x; stored to c0.  Transpose of the three force components.  LDS as
 // poisson_x_k, the second block holding the running sum.
 template <bool POW2>
-__global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_adj_k(Fields3 fl, int n0, int n1, int n2h, int n2p, int B,
+__global__ void __launch_bounds__(PFFT_TPB) poisson_x_adj_k(Fields3 fl, int n0, int n1, int n2h, int n2p, int B,
                                                            KTab kt, const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
   const int total = n0 * B;
@@ -638,17 +639,17 @@ __global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_adj_k(Fields3 fl, int n0
   const long long off = (long long)y * n2p + c0;
   const float ky = kt.k[1][y];
   const float kym = kt.km[1][y];
-  stage_twiddles<FFT_TPB>(stw, tw, n0);
+  stage_twiddles<PFFT_TPB>(stw, tw, n0);
 #pragma unroll
   for (int it = 0; it < PFFT_MAXV; ++it) {
-    int idx = threadIdx.x + it * FFT_TPB;
+    int idx = threadIdx.x + it * PFFT_TPB;
     if (idx < total) sa[idx] = make_float2(0.f, 0.f);
   }
 #pragma unroll 1
   for (int d = 0; d < 3; ++d) {
     // selects rather than indexes, so the argument struct stays in registers
     const float2* in = d == 0 ? fl.f[0] : (d == 1 ? fl.f[1] : fl.f[2]);
-    batch_copy<PFFT_MAXV, FFT_TPB>(
+    batch_copy<PFFT_MAXV, PFFT_TPB>(
         total,
         [&](int idx) {
           int j = idx >> lB, b = idx & (B - 1);
@@ -656,10 +657,10 @@ __global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_adj_k(Fields3 fl, int n0
         },
         [&](int idx, float2 v) { s[idx] = v; });
     __syncthreads();
-    lds_fft<-1, COLS, POW2, PFFT_MAXV>(s, n0, B, stw);
+    lds_fft<-1, COLS, POW2, PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
 #pragma unroll
     for (int it = 0; it < PFFT_MAXV; ++it) {
-      int idx = threadIdx.x + it * FFT_TPB;
+      int idx = threadIdx.x + it * PFFT_TPB;
       if (idx < total) {
         int j = idx >> lB, b = idx & (B - 1);
         if (b < nb) {
@@ -679,12 +680,12 @@ __global__ void __launch_bounds__(FFT_TPB, 4) poisson_x_adj_k(Fields3 fl, int n0
   }
 #pragma unroll
   for (int it = 0; it < PFFT_MAXV; ++it) {
-    int idx = threadIdx.x + it * FFT_TPB;
+    int idx = threadIdx.x + it * PFFT_TPB;
     if (idx < total) s[idx] = sa[idx];
   }
   __syncthreads();
-  lds_fft<1, COLS, POW2, PFFT_MAXV>(s, n0, B, stw);
-  batch_copy<PFFT_MAXV, FFT_TPB>(
+  lds_fft<1, COLS, POW2, PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
+  batch_copy<PFFT_MAXV, PFFT_TPB>(
       total, [&](int idx) { return s[idx]; },
       [&](int idx, float2 v) {
         int j = idx >> lB, b = idx & (B - 1);
@@ -706,6 +707,16 @@ __global__ void copy_rows_k(float* __restrict__ dst, int dst_rs, const float* __
 }
 
 // ------------------------------------------------------------ drivers --
+#define PMX_LDS_MAX (160 * 1024)     // LDS a gfx950 workgroup may allocate
+
+// A pencil block must fit the per-thread batches (total <= cap) and LDS.
+static int fft_fits(const char* who, int n, int total, int cap, size_t lds) {
+  PMX_REQUIRE_CODE(total <= cap && lds <= PMX_LDS_MAX, PMX_EUNSUP,
+                   "%s: axis of length %d needs %d values and %zu bytes of LDS per workgroup "
+                   "(limits %d and %d)", who, n, total, lds, cap, PMX_LDS_MAX);
+  return PMX_OK;
+}
+
 static size_t lds_bytes(int n, int P, int lay = COLS) {
   return (size_t)lds_elems(lay, n, P) * sizeof(float2);
 }
@@ -723,23 +734,38 @@ static int row_pencils(int n) {
 
 static bool half_ok(int n2) { return n2 >= 4 && (n2 & 1) == 0 && is_pow2(n2 / 2); }
 
+// rows per workgroup, LDS bytes and pencil length of the z pass of n2
+struct RowPlan {
+  bool half;
+  int P, len;
+  size_t lds;
+};
+
+static RowPlan row_plan(int n2) {
+  RowPlan r;
+  r.half = half_ok(n2);
+  r.len = r.half ? n2 / 2 : n2;
+  r.P = row_pencils(r.len);
+  r.lds = lds_bytes(r.len, r.P, ROWS) + tw_bytes(n2);
+  return r;
+}
+
 int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef) {
-  long long nrows = (long long)c->n[0] * c->n[1];
-  int n2 = c->n[2];
+  const long long nrows = (long long)c->n[0] * c->n[1];
+  const int n2 = c->n[2];
+  const RowPlan r = row_plan(n2);
+  if (int rc = fft_fits("z r2c", n2, r.len * r.P, RFFT_CAP, r.lds)) return rc;
+  const unsigned nblk = (unsigned)((nrows + r.P - 1) / r.P);
+  const int P = r.P;
+  const size_t lds = r.lds;
   pmx_tic(c, PMX_T_ZR2C);
-  if (half_ok(n2)) {
-    int P = row_pencils(n2 / 2);
-    unsigned nblk = (unsigned)((nrows + P - 1) / P);
-    size_t lds = lds_bytes(n2 / 2, P, ROWS) + tw_bytes(n2);
+  if (r.half) {
     if (pre)
       z_r2c_half_k<1><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
     else
       z_r2c_half_k<0><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
   } else {
-    int P = row_pencils(n2);
-    unsigned nblk = (unsigned)((nrows + P - 1) / P);
-    size_t lds = lds_bytes(n2, P, ROWS) + tw_bytes(n2);
-    bool p2 = is_pow2(n2);
+    const bool p2 = is_pow2(n2);
     if (pre && p2)
       z_r2c_k<1, true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
     else if (pre)
@@ -755,23 +781,20 @@ int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef) {
 }
 
 int pmx_fft_c2r_inplace(pmx_ctx* c, float* buf, float scale) {
-  long long nrows = (long long)c->n[0] * c->n[1];
-  int n2 = c->n[2];
+  const long long nrows = (long long)c->n[0] * c->n[1];
+  const int n2 = c->n[2];
+  const RowPlan r = row_plan(n2);
+  if (int rc = fft_fits("z c2r", n2, r.len * r.P, RFFT_CAP, r.lds)) return rc;
+  const unsigned nblk = (unsigned)((nrows + r.P - 1) / r.P);
+  const int P = r.P;
+  const size_t lds = r.lds;
   pmx_tic(c, PMX_T_ZC2R);
-  if (half_ok(n2)) {
-    int P = row_pencils(n2 / 2);
-    unsigned nblk = (unsigned)((nrows + P - 1) / P);
-    z_c2r_half_k<<<nblk, FFT_TPB, lds_bytes(n2 / 2, P, ROWS) + tw_bytes(n2), c->stream>>>(
-        buf, nrows, n2, c->rs, P, scale, c->tw[2]);
-  } else {
-    int P = row_pencils(n2);
-    unsigned nblk = (unsigned)((nrows + P - 1) / P);
-    size_t lds = lds_bytes(n2, P, ROWS) + tw_bytes(n2);
-    if (is_pow2(n2))
-      z_c2r_k<true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2]);
-    else
-      z_c2r_k<false><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2]);
-  }
+  if (r.half)
+    z_c2r_half_k<<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2]);
+  else if (is_pow2(n2))
+    z_c2r_k<true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2]);
+  else
+    z_c2r_k<false><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2]);
   pmx_toc(c, PMX_T_ZC2R);
   PMX_LAUNCH_CHECK();
   return PMX_OK;
@@ -797,6 +820,7 @@ int pmx_fft_strided(pmx_ctx* c, float2* const* f, int nf, int axis, int dir) {
   }
   dim3 grid((ncol + B - 1) / B, nouter, nf);
   size_t lds = lds_bytes(n, B) + tw_bytes(n);
+  if (int rc = fft_fits("strided FFT", n, n * B, FFT_CAP, lds)) return rc;
   bool p2 = is_pow2(n);
   int cls = axis == 1 ? (dir < 0 ? PMX_T_YFWD : PMX_T_YINV) : PMX_T_OTHER;
   pmx_tic(c, cls);
@@ -828,16 +852,17 @@ int pmx_poisson_x(pmx_ctx* c, float2* const* f, int adjoint) {
   int B = fft_pencils(n0, PFFT_CAP);
   dim3 grid((c->n2h + B - 1) / B, c->n[1], 1);
   size_t lds = 2 * lds_bytes(n0, B) + tw_bytes(n0);
+  if (int rc = fft_fits("Poisson x pass", n0, n0 * B, PFFT_CAP, lds)) return rc;
   bool p2 = is_pow2(n0);
   pmx_tic(c, PMX_T_POISSON);
   if (adjoint && p2)
-    poisson_x_adj_k<true><<<grid, FFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, ktab(c), c->tw[0]);
+    poisson_x_adj_k<true><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, ktab(c), c->tw[0]);
   else if (adjoint)
-    poisson_x_adj_k<false><<<grid, FFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, ktab(c), c->tw[0]);
+    poisson_x_adj_k<false><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, ktab(c), c->tw[0]);
   else if (p2)
-    poisson_x_k<true><<<grid, FFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, ktab(c), c->tw[0]);
+    poisson_x_k<true><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, ktab(c), c->tw[0]);
   else
-    poisson_x_k<false><<<grid, FFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, ktab(c), c->tw[0]);
+    poisson_x_k<false><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, ktab(c), c->tw[0]);
   pmx_toc(c, PMX_T_POISSON);
   PMX_LAUNCH_CHECK();
   return PMX_OK;

@@ -30,6 +30,14 @@ void pmx_set_error(const char* fmt, ...);
     }                                                                            \
   } while (0)
 
+#define PMX_REQUIRE_CODE(cond, code, ...)                                        \
+  do {                                                                           \
+    if (!(cond)) {                                                               \
+      pmx_set_error(__VA_ARGS__);                                                \
+      return code;                                                               \
+    }                                                                            \
+  } while (0)
+
 #define PMX_LAUNCH_CHECK()                                                       \
   do {                                                                           \
     hipError_t _e = hipGetLastError();                                           \

@@ -151,3 +151,20 @@ def test_poisson_production_sizes_vs_torch(dev, n, cell):
     want = _poisson_torch(dens, coef, cell)
     rel = float(torch.linalg.vector_norm(out - want) / torch.linalg.vector_norm(want))
     assert rel < 2e-6
+
+
+def test_poisson_longest_x_axis(dev):
+    """The longest supported axis (4096) in the Poisson x pass, whose pencil
+    block and twiddles use the whole 160 KB of a workgroup's LDS."""
+    import torch
+    from pmwd_amd import _lib
+    shape, cell = (4096, 2, 4), 1.0
+    g = torch.Generator(device=dev).manual_seed(6)
+    dens = 1 + 0.5 * torch.randn(shape, generator=g, device=dev)
+    coef = float(np.float32(0.45))
+    c = _lib.context(shape, cell, dev)
+    out = torch.empty((3,) + shape, dtype=torch.float32, device=dev)
+    _lib.check(_lib.lib().pmx_poisson_grad(c.bind(), _lib.ptr(dens), coef, _lib.ptr(out)))
+    want = _poisson_torch(dens, coef, cell)
+    rel = float(torch.linalg.vector_norm(out - want) / torch.linalg.vector_norm(want))
+    assert rel < 2e-6
