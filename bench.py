@@ -34,6 +34,10 @@ def parse():
     p.add_argument('--no-adjoint', action='store_true')
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--no-serve', action='store_true', help='skip the serving load')
+    p.add_argument('--full-adjoint', type=int, default=0, metavar='G',
+                   help='instead of the default run: forward plus gradient of a density MSE '
+                        'for G^3 particles on a G^3 mesh (box 2000*G/1024 Mpc/h, 31 steps, '
+                        'reversal checkpoints every 8 steps), with peak HBM')
     p.add_argument('--serve-requests', type=int, default=40)
     p.add_argument('--serve-rate', type=float, default=2.0, help='Poisson arrivals per second')
     p.add_argument('--serve-slots', type=int, default=8)
@@ -232,6 +236,71 @@ def fft_yardstick(conf, ctx, h, L, _lib, dev, reps=5):
             'r2c_rel_l2_vs_rocfft': rel(yo, yr), 'c2r_rel_l2_vs_rocfft': rel(xo, xr)}
 
 
+def full_adjoint(G, dev):
+    """Forward plus gradient at G^3 particles on a G^3 mesh (the 1024^3
+    configuration at G=1024): white modes (seed 0) -> linear_modes -> 2LPT ->
+    31 KDK steps -> density; loss = mean((rho - target)^2) against a seeded
+    target density; gradients with respect to the modes, Omega_m and sigma8
+    through the reverse-time adjoint with checkpoints every 8 steps.  The
+    modes -> 2LPT part is recomputed in the backward (torch checkpoint) rather
+    than kept.  Reports wall times and peak device bytes (torch allocations
+    plus the library contexts)."""
+    import time
+    import torch
+    import torch.utils.checkpoint as tcp
+    import pmwd_amd as pa
+    from pmwd_amd import _lib
+    conf = pa.Configuration(2000.0 / 1024, (G, G, G), mesh_shape=1, a_start=1 / 32,
+                            a_nbody_maxstep=1 / 32)
+    torch.cuda.reset_peak_memory_stats(dev)
+    t0 = time.perf_counter()
+    cosmo = pa.boltzmann(pa.Cosmology.from_sigma8(conf, 0.8, n_s=0.96, Omega_m=0.3,
+                                                  Omega_b=0.05, h=0.7), conf)
+    white = pa.white_noise(0, conf, real=True, device=dev).requires_grad_(True)
+    gen = torch.Generator(device=dev).manual_seed(1)
+    target = 1 + 0.5 * torch.randn(conf.mesh_shape, generator=gen, device=dev)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    theta = {'Omega_m': torch.tensor(cosmo.Omega_m, dtype=torch.float64, requires_grad=True),
+             'sigma8': torch.tensor(cosmo.sigma8, dtype=torch.float64, requires_grad=True)}
+
+    def ics(w, om, s8):
+        th = {'Omega_m': om, 'sigma8': s8}
+        ptcl, _ = pa.lpt(pa.linear_modes(w, cosmo, conf, theta=th), cosmo, conf, theta=th)
+        return ptcl.disp, ptcl.vel, ptcl.pmid
+
+    t0 = time.perf_counter()
+    disp, vel, pmid = tcp.checkpoint(ics, white, theta['Omega_m'], theta['sigma8'],
+                                     use_reentrant=False)
+    ptcl = pa.Particles(conf, pmid, disp, vel)
+    del disp, vel
+    ptcl, _ = pa.nbody(ptcl, None, cosmo, conf, checkpoint_every=8, omega_m=theta['Omega_m'],
+                       checkpoint_device='cpu')
+    dens = pa.scatter(ptcl, conf)
+    loss = ((dens.double() - target.double())**2).mean()
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g_w, g_om, g_s8 = torch.autograd.grad(loss, (white, theta['Omega_m'], theta['sigma8']))
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    ctx_bytes = sum(c.nbytes for c in _lib._ctx_cache.values())
+    peak = torch.cuda.max_memory_allocated(dev) + ctx_bytes
+    steps = len(conf.a_nbody) - 1
+    n = conf.ptcl_num
+    return {'workload': f'{G}^3 particles on a {G}^3 mesh, box {2000.0 * G / 1024:g} Mpc/h, '
+                        f'{steps} KDK steps from a=1/32, 2LPT, density MSE against a seeded '
+                        f'target, gradients w.r.t. modes, Omega_m, sigma8, reversal '
+                        f'checkpoints every 8 steps in pinned host memory',
+            't_setup_s': t_setup, 't_forward_s': t_fwd, 't_backward_s': t_bwd,
+            'fwd_updates_per_s': n * steps / t_fwd,
+            'fwd_adj_updates_per_s': n * steps / (t_fwd + t_bwd),
+            'peak_hbm_gb': peak / 1e9, 'torch_peak_gb': torch.cuda.max_memory_allocated(dev) / 1e9,
+            'library_context_gb': ctx_bytes / 1e9, 'loss': float(loss.detach()),
+            'grad_modes_norm': float(g_w.double().norm()), 'grad_Omega_m': float(g_om),
+            'grad_sigma8': float(g_s8)}
+
+
 def main():
     args = parse()
     world = int(os.environ.get('WORLD_SIZE', '1'))
@@ -250,6 +319,10 @@ def main():
 
     dev = torch.device('cuda:0')
     torch.cuda.set_device(dev)
+    if args.full_adjoint:
+        log(f'full forward + gradient at {args.full_adjoint}^3')
+        print(json.dumps({'full_adjoint': full_adjoint(args.full_adjoint, dev)}))
+        return
     G = args.grid
     spacing = args.box / G
     conf = pa.Configuration(spacing, (G, G, G), mesh_shape=args.mesh_ratio, a_start=1 / 32,

@@ -194,6 +194,20 @@ def context(mesh_shape, cell_size, device):
     return ctx
 
 
+def context_any_cell(mesh_shape, device):
+    """A context of this mesh shape for work that does not depend on the cell
+    size (the FFTs): an existing one of any cell size when there is one."""
+    device = torch.device(device)
+    if device.index is None:
+        device = torch.device(device.type, torch.cuda.current_device())
+    shape = tuple(int(s) for s in mesh_shape)
+    if getattr(_tls, 'provider', None) is None:
+        for (s, _cell, idx), ctx in _ctx_cache.items():
+            if s == shape and idx == device.index:
+                return ctx
+    return context(shape, 1.0, device)
+
+
 def clear_contexts():
     for c in _ctx_cache.values():
         c.close()

@@ -119,7 +119,19 @@ def _force(c, pmid, disp, acc, n, val, coef):
                                       _lib.ptr(acc)))
 
 
-def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=None):
+def _store(t, where):
+    """A copy of t for a reversal checkpoint: on the device, or in pinned
+    host memory when ``where`` is 'cpu' (copied back when the adjoint needs
+    it)."""
+    if where is None or torch.device(where) == t.device:
+        return t.clone()
+    out = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    out.copy_(t, non_blocking=True)
+    return out
+
+
+def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=None,
+                 checkpoint_device=None):
     """Integrates in place; returns acc and the checkpoints {step: (disp, vel)}."""
     L = _lib.lib()
     n = pmid.shape[0]
@@ -133,7 +145,7 @@ def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=N
     kdk = _is_kdk(conf)
     for i, (a_prev, a_next) in enumerate(zip(a_nbody[:-1], a_nbody[1:])):
         if checkpoint_every and i % checkpoint_every == 0 and i > 0:
-            ckpt[i] = (disp.clone(), vel.clone())
+            ckpt[i] = (_store(disp, checkpoint_device), _store(vel, checkpoint_device))
         if kdk:
             a_half = a_prev * (1 - 0.5) + a_next * 0.5
             k1 = _f32(kick_factor(a_prev, a_prev, a_half, cosmo, conf))
@@ -202,7 +214,7 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
             fa = _f32(kick_factor(*ka_args, cosmo, conf))
             fd = _f32(drift_factor(*d_args, cosmo, conf))
             fb = _f32(kick_factor(*kb_args, cosmo, conf))
-            snap = ckpt[step][0] if ckpt and step in ckpt else None
+            snap = ckpt[step][0].to(disp.device) if ckpt and step in ckpt else None
             _lib.check(L.pmx_nbody_step_adj(
                 h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel), _lib.ptr(acc),
                 _lib.ptr(disp_cot), _lib.ptr(vel_cot), _lib.ptr(acc_cot), n, val, coef,
@@ -216,7 +228,7 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
             if ckpt and step in ckpt:
                 _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
                 kp = 0.0
-                vel.copy_(ckpt[step][1])
+                vel.copy_(ckpt[step][1].to(vel.device))
             continue
         D = K = 0
         a_disp = a_vel = a_acc = a_prev
@@ -244,11 +256,11 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
                 om_cot -= dfdO * dot.value
                 a_disp = a_disp_next
                 if ckpt and step in ckpt and D == 1:
-                    disp.copy_(ckpt[step][0])
+                    disp.copy_(ckpt[step][0].to(disp.device))
                 cf = force_adj()
                 a_acc = a_disp
         if ckpt and step in ckpt:
-            vel.copy_(ckpt[step][1])
+            vel.copy_(ckpt[step][1].to(vel.device))
     if kp != 0.0:
         _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
     return acc, acc_cot, om_cot
@@ -261,10 +273,12 @@ def _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, k):
 
 class _NBodyFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, disp, vel, om_t, pmid, cosmo, conf, reverse, checkpoint_every):
+    def forward(ctx, disp, vel, om_t, pmid, cosmo, conf, reverse, checkpoint_every,
+                checkpoint_device):
         disp_f = disp.detach().clone().contiguous()
         vel_f = vel.detach().clone().contiguous()
-        acc, ckpt = _run_forward(pmid, disp_f, vel_f, cosmo, conf, reverse, checkpoint_every)
+        acc, ckpt = _run_forward(pmid, disp_f, vel_f, cosmo, conf, reverse, checkpoint_every,
+                                 checkpoint_device)
         ctx.save_for_backward(disp_f, vel_f)
         ctx.pmid, ctx.cosmo, ctx.conf, ctx.reverse, ctx.ckpt = pmid, cosmo, conf, reverse, ckpt
         ctx.mark_non_differentiable(acc)
@@ -282,16 +296,20 @@ class _NBodyFn(torch.autograd.Function):
         om_grad = None
         if ctx.needs_input_grad[2]:
             om_grad = torch.tensor(om_cot, dtype=torch.float64)
-        return dc, vc, om_grad, None, None, None, None, None
+        return dc, vc, om_grad, None, None, None, None, None, None
 
 
-def nbody(ptcl, obsvbl, cosmo, conf, reverse=False, checkpoint_every=None, omega_m=None):
+def nbody(ptcl, obsvbl, cosmo, conf, reverse=False, checkpoint_every=None, omega_m=None,
+          checkpoint_device=None):
     """N-body time integration (nbody.py:215-223).
 
     Returns ``(ptcl, obsvbl)`` with the final ``disp``, ``vel`` and ``acc``.
     Differentiable w.r.t. ``ptcl.disp``/``ptcl.vel`` (reverse-time adjoint)
     and, when ``omega_m`` is a 0-d float64 tensor equal to
-    ``cosmo.Omega_m`` that requires grad, w.r.t. Omega_m.
+    ``cosmo.Omega_m`` that requires grad, w.r.t. Omega_m.  With
+    ``checkpoint_every`` the adjoint restarts from states stored every that
+    many steps, kept on the device or, with ``checkpoint_device='cpu'``, in
+    pinned host memory.
     """
     if cosmo.growth is None:
         raise ValueError('Growth table is empty. Call growth_integ or boltzmann first.')
@@ -303,7 +321,7 @@ def nbody(ptcl, obsvbl, cosmo, conf, reverse=False, checkpoint_every=None, omega
     if need_grad:
         om_t = omega_m if omega_m is not None else torch.tensor(cosmo.Omega_m, dtype=torch.float64)
         disp, vel, acc = _NBodyFn.apply(ptcl.disp, vel, om_t, pmid, cosmo, conf, reverse,
-                                        checkpoint_every)
+                                        checkpoint_every, checkpoint_device)
     else:
         disp = ptcl.disp.detach().clone().contiguous()
         vel = vel.detach().clone().contiguous()

@@ -42,8 +42,8 @@ def fftfreq(shape, spacing, dtype=torch.float64, sparse=True, device=None):
 
 
 def _ctx_for(shape, device):
-    # the FFT only needs the shape; any positive cell size keys the context
-    return _lib.context(shape, 1.0, device)
+    # the FFT only needs the shape: any context of this shape serves
+    return _lib.context_any_cell(shape, device)
 
 
 def _check_field(f, complex_):
