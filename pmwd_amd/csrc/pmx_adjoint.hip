@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(ADJ_TPB)
 static int force_vjp_binned(pmx_ctx* c, long long n, float coef, float val, const float* vel_cot,
                             float* acc, float* acc_cot, long long red_off) {
   int rc;
-  if ((rc = pmx_ensure_vrec(c, n))) return rc;
+  if ((rc = pmx_ensure_vrec(c))) return rc;
   if ((rc = pmx_fill_u32(c, reinterpret_cast<uint32_t*>(c->vmax), 3, 0u))) return rc;
   const MeshGeom g = pmx_geom_padded(c);
   const TileGeom& tg = c->tg;
@@ -231,13 +231,13 @@ static int force_vjp_binned(pmx_ctx* c, long long n, float coef, float val, cons
   pmx_tic(c, PMX_T_VJPG);
   tile_vjp_gather_k<<<nt, ADJ_TPB, 3 * cells * sizeof(float), c->stream>>>(
       c->rec, c->pidx, c->tile_start, c->tile_count, g, tg, (const float*)c->cbuf[0],
-      (const float*)c->cbuf[1], (const float*)c->cbuf[2], vel_cot, acc, acc_cot, c->vrec, n,
-      c->vmax, c->red + red_off);
+      (const float*)c->cbuf[1], (const float*)c->cbuf[2], vel_cot, acc, acc_cot, c->vrec,
+      c->rec_cap, c->vmax, c->red + red_off);
   pmx_toc(c, PMX_T_VJPG);
   PMX_LAUNCH_CHECK();
   // the cotangent of each force mesh: scatter(vel_cot_d * w)
-  for (int d = 0; d < 3; ++d)
-    if ((rc = pmx_deposit_binned(c, 0.f, d, n, (float*)c->cbuf[d], c->rs, 0))) return rc;
+  float* meshes[3] = {(float*)c->cbuf[0], (float*)c->cbuf[1], (float*)c->cbuf[2]};
+  if ((rc = pmx_deposit_binned3(c, n, meshes, c->rs))) return rc;
   // density cotangent -coef * sum_d L_d(mesh_cot_d), L_d = irfftn (-ik_d) laplace rfftn
   if ((rc = pmx_poisson_adj_ctx(c, (float)(-(double)coef / (double)c->M)))) return rc;
   pmx_tic(c, PMX_T_GRADG);
@@ -352,14 +352,13 @@ extern "C" int pmx_drift_adj(pmx_ctx* c, long long n, float* disp, const float* 
 // (disp replaced by snap when given, a stored checkpoint), with per-block
 // partials of disp_cot.vel (the drift's cosmology term).  A kick's own
 // vel_cot.acc equals the dot of the force VJP that produced acc.
-template <int HAS_KP>
+template <int HAS_KP, bool CAP>
 __global__ void __launch_bounds__(PMX_TPB)
     kd_adj_bin_k(const int16_t* __restrict__ pmid, float* __restrict__ disp, float* __restrict__ vel,
                  const float* __restrict__ acc, float* __restrict__ disp_cot,
                  float* __restrict__ vel_cot, const float* __restrict__ acc_cot,
                  const float* __restrict__ snap, long long n, float kp, float ka, float d,
-                 MeshGeom g, TileGeom tg, uint32_t* __restrict__ tile_count,
-                 uint32_t* __restrict__ ptile, uint32_t* __restrict__ prank,
+                 MeshGeom g, TileGeom tg, uint32_t* __restrict__ tile_count, BinOut o,
                  double* __restrict__ part) {
   constexpr int T3 = 3 * PMX_TPB;
   // vel | acc | disp | disp_cot | vel_cot | acc_cot, staged through LDS
@@ -374,9 +373,10 @@ __global__ void __launch_bounds__(PMX_TPB)
   const bool active = q < cnt;
   double s = 0;
   uint32_t t = 0;
+  CicAxis ax[3];
+  float dp[3];
   if (active) {
     int pm[3];
-    float dp[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const int e = 3 * q + a;
@@ -402,15 +402,11 @@ __global__ void __launch_bounds__(PMX_TPB)
       sh[4 * T3 + e] = vc;
       dp[a] = x;
     }
-    CicAxis ax[3];
     cic_setup(pm, dp, g, ax);
     t = tile_of(ax, tg);
   }
   const uint32_t r = agg_slot(tile_count, t, active);
-  if (active) {
-    ptile[b0 + q] = t;
-    prank[b0 + q] = r;
-  }
+  if (active) pmx_bin_emit<CAP>(o, b0 + q, t, r, ax, tg);
   __syncthreads();
   pmx_stage_out3<1, PMX_TPB>({{vel}}, sh, b0, cnt);
   pmx_stage_out3<3, PMX_TPB>({{disp, disp_cot, vel_cot}}, sh + 2 * T3, b0, cnt);
@@ -429,26 +425,30 @@ extern "C" int pmx_nbody_step_adj(pmx_ctx* c, const int16_t* pmid, float* disp, 
     if (dots) dots[0] = dots[1] = 0;
     return PMX_OK;
   }
-  int rc;
-  if ((rc = pmx_ensure_ptcl(c, n))) return rc;
+  int rc, cap;
   const unsigned nb1 = pmx_blocks(n);
   if ((rc = pmx_ensure_red(c, (long long)nb1 + c->ntiles))) return rc;
-  if ((rc = pmx_fill_u32(c, c->tile_count, c->ntiles + 1, 0u))) return rc;
+  if ((rc = pmx_bin_begin(c, n, &cap))) return rc;
   const MeshGeom g = pmx_geom_padded(c);
+  const BinOut o = pmx_bin_out(c);
+#define KD_ADJ_LAUNCH(KP, CAPV, KPV)                                                          \
+  kd_adj_bin_k<KP, CAPV><<<nb1, PMX_TPB, 0, c->stream>>>(pmid, disp, vel, acc, disp_cot,       \
+                                                         vel_cot, acc_cot, disp_snap, n, KPV, ka, \
+                                                         d, g, c->tg, c->tile_count, o, c->red)
   pmx_tic(c, PMX_T_KDADJ);
-  if (kp != 0.f)
-    kd_adj_bin_k<1><<<nb1, PMX_TPB, 0, c->stream>>>(pmid, disp, vel, acc, disp_cot, vel_cot,
-                                                    acc_cot, disp_snap, n, kp, ka, d, g, c->tg,
-                                                    c->tile_count, c->pkey, c->prank, c->red);
-  else
-    kd_adj_bin_k<0><<<nb1, PMX_TPB, 0, c->stream>>>(pmid, disp, vel, acc, disp_cot, vel_cot,
-                                                    acc_cot, disp_snap, n, 0.f, ka, d, g, c->tg,
-                                                    c->tile_count, c->pkey, c->prank, c->red);
+  if (kp != 0.f) {
+    if (cap) KD_ADJ_LAUNCH(1, true, kp);
+    else KD_ADJ_LAUNCH(1, false, kp);
+  } else {
+    if (cap) KD_ADJ_LAUNCH(0, true, 0.f);
+    else KD_ADJ_LAUNCH(0, false, 0.f);
+  }
+#undef KD_ADJ_LAUNCH
   pmx_toc(c, PMX_T_KDADJ);
   PMX_LAUNCH_CHECK();
-  if ((rc = pmx_scan_tiles(c))) return rc;
-  if ((rc = pmx_bin_fill(c, pmid, disp, n, nullptr, 0, 0))) return rc;
-  if ((rc = pmx_deposit_binned(c, val, -1, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
+  // disp now holds the binned positions (disp_snap when given)
+  if ((rc = pmx_bin_end(c, pmid, disp, n, cap))) return rc;
+  if ((rc = pmx_deposit_binned(c, val, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
   if ((rc = pmx_poisson_ctx(c, coef))) return rc;
   if ((rc = force_vjp_binned(c, n, coef, val, vel_cot, acc, acc_cot, nb1))) return rc;
   if (!dots) return PMX_OK;

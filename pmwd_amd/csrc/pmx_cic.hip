@@ -12,17 +12,19 @@
 
 #include <math.h>
 
-__global__ void fill_u32_k(uint32_t* __restrict__ p, long long n, uint32_t v) {
+__global__ void fill_u32_k(uint32_t* __restrict__ p, long long n, uint32_t v,
+                           const uint32_t* __restrict__ cond) {
+  if (cond && *cond == 0) return;
   for (long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x; i < n;
        i += (long long)gridDim.x * PMX_TPB)
     p[i] = v;
 }
 
-int pmx_fill_u32(pmx_ctx* c, uint32_t* p, long long n, uint32_t v) {
+int pmx_fill_u32(pmx_ctx* c, uint32_t* p, long long n, uint32_t v, const uint32_t* cond) {
   unsigned nb = pmx_blocks(n);
   if (nb > 4096) nb = 4096;
   pmx_tic(c, PMX_T_OTHER);
-  fill_u32_k<<<nb, PMX_TPB, 0, c->stream>>>(p, n, v);
+  fill_u32_k<<<nb, PMX_TPB, 0, c->stream>>>(p, n, v, cond);
   pmx_toc(c, PMX_T_OTHER);
   PMX_LAUNCH_CHECK();
   return PMX_OK;
@@ -77,81 +79,72 @@ extern "C" int pmx_tile_keys(pmx_ctx* c, const int16_t* pmid, const float* disp,
 }
 
 // ---------------------------------------------------- binned scatter --
+// The binning kernels below run over blocks of PMX_TPB particles, grid-
+// striding when the grid is capped (the exact re-binning after a capacity
+// overflow, launched before the device knows whether it is needed, exits at
+// once unless *cond != 0).
+template <bool CAP>
 __global__ void __launch_bounds__(PMX_TPB)
     bin_count_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp, long long n,
-                MeshGeom g, TileGeom tg, uint32_t* __restrict__ tile_count,
-                uint32_t* __restrict__ ptile, uint32_t* __restrict__ prank) {
+                MeshGeom g, TileGeom tg, uint32_t* __restrict__ tile_count, BinOut o,
+                const uint32_t* __restrict__ cond) {
+  if (cond && *cond == 0) return;
   __shared__ __align__(16) float sd[3 * PMX_TPB];
   __shared__ __align__(16) int16_t sp[3 * PMX_TPB];
-  const long long b0 = (long long)blockIdx.x * PMX_TPB;
-  const int cnt = (int)(n - b0 < PMX_TPB ? n - b0 : PMX_TPB);
-  pmx_stage_in3<1, PMX_TPB>(sd, {{disp}}, b0, cnt);
-  pmx_stage_pmid<PMX_TPB>(sp, pmid, b0, cnt);
-  __syncthreads();
-  const int q = threadIdx.x;
-  const bool active = q < cnt;
-  uint32_t t = 0;
-  if (active) {
-    const int pm[3] = {sp[3 * q], sp[3 * q + 1], sp[3 * q + 2]};
-    const float dp[3] = {sd[3 * q], sd[3 * q + 1], sd[3 * q + 2]};
+  for (long long b0 = (long long)blockIdx.x * PMX_TPB; b0 < n;
+       b0 += (long long)gridDim.x * PMX_TPB) {
+    const int cnt = (int)(n - b0 < PMX_TPB ? n - b0 : PMX_TPB);
+    pmx_stage_in3<1, PMX_TPB>(sd, {{disp}}, b0, cnt);
+    pmx_stage_pmid<PMX_TPB>(sp, pmid, b0, cnt);
+    __syncthreads();
+    const int q = threadIdx.x;
+    const bool active = q < cnt;
+    uint32_t t = 0;
     CicAxis ax[3];
-    cic_setup(pm, dp, g, ax);
-    t = tile_of(ax, tg);
-  }
-  const uint32_t r = agg_slot(tile_count, t, active);
-  if (active) {
-    ptile[b0 + q] = t;
-    prank[b0 + q] = r;
+    float dp[3];
+    if (active) {
+      const int pm[3] = {sp[3 * q], sp[3 * q + 1], sp[3 * q + 2]};
+      dp[0] = sd[3 * q];
+      dp[1] = sd[3 * q + 1];
+      dp[2] = sd[3 * q + 2];
+      cic_setup(pm, dp, g, ax);
+      t = tile_of(ax, tg);
+    }
+    const uint32_t r = agg_slot(tile_count, t, active);
+    if (active) pmx_bin_emit<CAP>(o, b0 + q, t, r, ax, tg);
+    __syncthreads();
   }
 }
 
-// Record of a binned particle: d = disp/cell per axis (float32, the value
-// enmesh divides out) and the lower CIC corner relative to the tile origin
-// packed as lx | ly << 10 | lz << 20 (bit pattern stored in .w).
+// Writes the record (pmx_make_record) and index of each particle at
+// tile_start[tile] + rank once the tile counts are scanned.
 __global__ void __launch_bounds__(PMX_TPB)
     bin_fill_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp, long long n,
                MeshGeom g, TileGeom tg, const uint32_t* __restrict__ ptile,
                const uint32_t* __restrict__ prank, const uint32_t* __restrict__ tile_start,
-               float4* __restrict__ rec, const float* __restrict__ pval, int vstride, int nch,
-               float* __restrict__ vrec, float* __restrict__ vmax, uint32_t* __restrict__ pidx) {
+               float4* __restrict__ rec, uint32_t* __restrict__ pidx,
+               const uint32_t* __restrict__ cond) {
+  if (cond && *cond == 0) return;
   __shared__ __align__(16) float sd[3 * PMX_TPB];
   __shared__ __align__(16) int16_t sp[3 * PMX_TPB];
-  const long long b0 = (long long)blockIdx.x * PMX_TPB;
-  const int cnt = (int)(n - b0 < PMX_TPB ? n - b0 : PMX_TPB);
-  pmx_stage_in3<1, PMX_TPB>(sd, {{disp}}, b0, cnt);
-  pmx_stage_pmid<PMX_TPB>(sp, pmid, b0, cnt);
-  __syncthreads();
-  const int q = threadIdx.x;
-  const long long i = b0 + q;
-  const bool active = q < cnt;
-  float vabs[3] = {0.f, 0.f, 0.f};
-  if (active) {
-    const int pm[3] = {sp[3 * q], sp[3 * q + 1], sp[3 * q + 2]};
-    const float dp[3] = {sd[3 * q], sd[3 * q + 1], sd[3 * q + 2]};
-    CicAxis ax[3];
-    cic_setup(pm, dp, g, ax);
-    uint32_t t = ptile[i];
-    uint32_t slot = tile_start[t] + prank[i];
-    int lx = ax[0].lo % tg.tx, ly = ax[1].lo % tg.ty, lz = ax[2].lo % tg.tz;
-    uint32_t packed = (uint32_t)lx | ((uint32_t)ly << 10) | ((uint32_t)lz << 20);
-    rec[slot] = make_float4(dp[0] / g.cell, dp[1] / g.cell, dp[2] / g.cell, __uint_as_float(packed));
-    pidx[slot] = (uint32_t)i;
-    for (int ch = 0; ch < nch; ++ch) {
-      float v = pval[i * vstride + ch];
-      vrec[(long long)ch * n + slot] = v;
-      vabs[ch] = fabsf(v);
+  for (long long b0 = (long long)blockIdx.x * PMX_TPB; b0 < n;
+       b0 += (long long)gridDim.x * PMX_TPB) {
+    const int cnt = (int)(n - b0 < PMX_TPB ? n - b0 : PMX_TPB);
+    pmx_stage_in3<1, PMX_TPB>(sd, {{disp}}, b0, cnt);
+    pmx_stage_pmid<PMX_TPB>(sp, pmid, b0, cnt);
+    __syncthreads();
+    const int q = threadIdx.x;
+    const long long i = b0 + q;
+    if (q < cnt) {
+      const int pm[3] = {sp[3 * q], sp[3 * q + 1], sp[3 * q + 2]};
+      const float dp[3] = {sd[3 * q], sd[3 * q + 1], sd[3 * q + 2]};
+      CicAxis ax[3];
+      cic_setup(pm, dp, g, ax);
+      const uint32_t slot = tile_start[ptile[i]] + prank[i];
+      rec[slot] = pmx_make_record(ax, tg);
+      pidx[slot] = (uint32_t)i;
     }
-  }
-  // max |v| per channel: reduce over the wave, then one atomic per wave only
-  // when it raises the running maximum (non-negative floats order like
-  // their bit patterns)
-  for (int ch = 0; ch < nch; ++ch) {
-    float m = vabs[ch];
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    if (__lane_id() == 0) {
-      unsigned* vm = reinterpret_cast<unsigned*>(&vmax[ch]);
-      if (__float_as_uint(m) > __atomic_load_n(vm, __ATOMIC_RELAXED)) atomicMax(vm, __float_as_uint(m));
-    }
+    __syncthreads();
   }
 }
 
@@ -175,93 +168,106 @@ __device__ __forceinline__ double fixed_scale(float vmax) {
   return ldexp(1.0, k);
 }
 
-// One workgroup per tile: LDS accumulation of val*w (or vrec[slot]*w) of
-// the tile's binned records over the (sx+1)(sy+1)(sz+1) cells they touch;
+// Output meshes of a deposit (one per channel).
+struct DepOut {
+  float* mesh[3];
+};
+
+// One workgroup per tile: LDS accumulation of val*w (or vrec[ch][slot]*w)
+// of the tile's binned records over the (sx+1)(sy+1)(sz+1) cells they touch;
 // each thread streams DEP_U coalesced records per round.  The interior is
-// stored with plain stores, the +1 shell to the tile's halo slot.
+// stored with plain stores, the +1 shell to the tile's halo slot.  With
+// nch > 1 the channels run one after the other in the same workgroup, the
+// records of the tile coming back from L2.
 #define DEP_TPB 512
 #define DEP_U 4
 template <int PERVAL>
 __global__ void __launch_bounds__(DEP_TPB)
-    deposit_k(const float4* __restrict__ rec, const float* __restrict__ vrec,
+    deposit_k(const float4* __restrict__ rec, const float* __restrict__ vrec, long long vstride,
               const float* __restrict__ vmaxp, const uint32_t* __restrict__ tile_start,
               const uint32_t* __restrict__ tile_count, MeshGeom g, TileGeom tg, float val,
-              float* __restrict__ mesh, int mrs, int accumulate, float* __restrict__ halo) {
+              DepOut out, int nch, int mrs, int accumulate, float* __restrict__ halo,
+              long long hstride) {
   extern __shared__ unsigned long long acc[];
   int o[3], s[3], ti[3];
   const int t = (int)pmx_xcd_block(blockIdx.x, gridDim.x);
   tile_origin(t, tg, g, o, s, ti);
   const int LY = s[1] + 1, LZ = s[2] + 1;
   const int ltot = (s[0] + 1) * LY * LZ;
-  for (int i = threadIdx.x; i < ltot; i += DEP_TPB) acc[i] = 0ull;
-  const double scale = fixed_scale(PERVAL ? *vmaxp : fabsf(val));
-  const double inv = 1.0 / scale;
-  __syncthreads();
   const uint32_t start = tile_start[t];
   const uint32_t cnt = tile_count[t];
-  for (uint32_t q0 = 0; q0 < cnt; q0 += DEP_TPB * DEP_U) {
-    float4 r[DEP_U];
-    float v[DEP_U];
+  for (int ch = 0; ch < nch; ++ch) {
+    for (int i = threadIdx.x; i < ltot; i += DEP_TPB) acc[i] = 0ull;
+    const double scale = fixed_scale(PERVAL ? vmaxp[ch] : fabsf(val));
+    const double inv = 1.0 / scale;
+    const float* vr = PERVAL ? vrec + (long long)ch * vstride : nullptr;
+    __syncthreads();
+    for (uint32_t q0 = 0; q0 < cnt; q0 += DEP_TPB * DEP_U) {
+      float4 r[DEP_U];
+      float v[DEP_U];
 #pragma unroll
-    for (int u = 0; u < DEP_U; ++u) {
-      uint32_t q = q0 + u * DEP_TPB + threadIdx.x;
-      if (q < cnt) {
-        r[u] = rec[start + q];
-        v[u] = PERVAL ? vrec[start + q] : val;
+      for (int u = 0; u < DEP_U; ++u) {
+        uint32_t q = q0 + u * DEP_TPB + threadIdx.x;
+        if (q < cnt) {
+          r[u] = rec[start + q];
+          v[u] = PERVAL ? vr[start + q] : val;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < DEP_U; ++u) {
+        uint32_t q = q0 + u * DEP_TPB + threadIdx.x;
+        if (q >= cnt) continue;
+        int lx, ly, lz;
+        pmx_unpack_local(r[u].w, lx, ly, lz);
+        // binned by this tile, so always inside; the guard keeps LDS in bounds
+        if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
+        float a0, a1, b0, b1, c0, c1;
+        frac_of(r[u].x, a0, a1);
+        frac_of(r[u].y, b0, b1);
+        frac_of(r[u].z, c0, c1);
+        int b = (lx * LY + ly) * LZ + lz;
+        float vv = v[u];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          float f0 = (j & 1) ? a1 : a0;
+          float f1 = (j & 2) ? b1 : b0;
+          float f2 = (j & 4) ? c1 : c0;
+          int li = b + ((j & 1) ? LY * LZ : 0) + ((j & 2) ? LZ : 0) + ((j & 4) ? 1 : 0);
+          float x = vv * ((f0 * f1) * f2);
+          long long qx = (long long)rint((double)x * scale);
+          atomicAdd(&acc[li], (unsigned long long)qx);
+        }
       }
     }
-#pragma unroll
-    for (int u = 0; u < DEP_U; ++u) {
-      uint32_t q = q0 + u * DEP_TPB + threadIdx.x;
-      if (q >= cnt) continue;
-      uint32_t pk = __float_as_uint(r[u].w);
-      int lx = pk & 1023, ly = (pk >> 10) & 1023, lz = (pk >> 20) & 1023;
-      // binned by this tile, so always inside; the guard keeps LDS in bounds
-      if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
-      float a0, a1, b0, b1, c0, c1;
-      frac_of(r[u].x, a0, a1);
-      frac_of(r[u].y, b0, b1);
-      frac_of(r[u].z, c0, c1);
-      int b = (lx * LY + ly) * LZ + lz;
-      float vv = v[u];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        float f0 = (j & 1) ? a1 : a0;
-        float f1 = (j & 2) ? b1 : b0;
-        float f2 = (j & 4) ? c1 : c0;
-        int li = b + ((j & 1) ? LY * LZ : 0) + ((j & 2) ? LZ : 0) + ((j & 4) ? 1 : 0);
-        float x = vv * ((f0 * f1) * f2);
-        long long qx = (long long)rint((double)x * scale);
-        atomicAdd(&acc[li], (unsigned long long)qx);
+    __syncthreads();
+    float* mesh = ch == 0 ? out.mesh[0] : (ch == 1 ? out.mesh[1] : out.mesh[2]);
+    const int itot = s[0] * s[1] * s[2];
+    for (int i = threadIdx.x; i < itot; i += DEP_TPB) {
+      int lz = i % s[2];
+      int ly = (i / s[2]) % s[1];
+      int lx = i / (s[2] * s[1]);
+      long long m = ((long long)(o[0] + lx) * g.n1 + (o[1] + ly)) * mrs + (o[2] + lz);
+      float a = (float)((double)(long long)acc[(lx * LY + ly) * LZ + lz] * inv);
+      mesh[m] = accumulate ? mesh[m] + a : a;
+    }
+    const int offA = LY * LZ;
+    const int offB = s[0] * LZ;
+    const int stot = offA + offB + s[0] * s[1];
+    float* hs = halo + ch * hstride + (long long)t * tg.shell;
+    for (int i = threadIdx.x; i < stot; i += DEP_TPB) {
+      int qx, qy, qz;
+      if (i < offA) {
+        qx = s[0]; qy = i / LZ; qz = i % LZ;
+      } else if (i < offA + offB) {
+        int k = i - offA;
+        qx = k / LZ; qy = s[1]; qz = k % LZ;
+      } else {
+        int k = i - offA - offB;
+        qx = k / s[1]; qy = k % s[1]; qz = s[2];
       }
+      hs[i] = (float)((double)(long long)acc[(qx * LY + qy) * LZ + qz] * inv);
     }
-  }
-  __syncthreads();
-  const int itot = s[0] * s[1] * s[2];
-  for (int i = threadIdx.x; i < itot; i += DEP_TPB) {
-    int lz = i % s[2];
-    int ly = (i / s[2]) % s[1];
-    int lx = i / (s[2] * s[1]);
-    long long m = ((long long)(o[0] + lx) * g.n1 + (o[1] + ly)) * mrs + (o[2] + lz);
-    float a = (float)((double)(long long)acc[(lx * LY + ly) * LZ + lz] * inv);
-    mesh[m] = accumulate ? mesh[m] + a : a;
-  }
-  const int offA = LY * LZ;
-  const int offB = s[0] * LZ;
-  const int stot = offA + offB + s[0] * s[1];
-  float* hs = halo + (long long)t * tg.shell;
-  for (int i = threadIdx.x; i < stot; i += DEP_TPB) {
-    int qx, qy, qz;
-    if (i < offA) {
-      qx = s[0]; qy = i / LZ; qz = i % LZ;
-    } else if (i < offA + offB) {
-      int k = i - offA;
-      qx = k / LZ; qy = s[1]; qz = k % LZ;
-    } else {
-      int k = i - offA - offB;
-      qx = k / s[1]; qy = k % s[1]; qz = s[2];
-    }
-    hs[i] = (float)((double)(long long)acc[(qx * LY + qy) * LZ + qz] * inv);
+    __syncthreads();  // acc is cleared for the next channel
   }
 }
 
@@ -348,10 +354,11 @@ __global__ void __launch_bounds__(TG_TPB)
 }
 
 // Adds to every low-face cell of tile t the shells of the (up to seven)
-// lower neighbour tiles that cover it, in a fixed order.
+// lower neighbour tiles that cover it, in a fixed order; each of nch
+// channels has its mesh and its halo slice (hstride floats apart).
 __global__ void __launch_bounds__(PMX_TPB)
-    fold_k(float* __restrict__ mesh, int mrs, const float* __restrict__ halo, MeshGeom g,
-           TileGeom tg) {
+    fold_k(DepOut out, int nch, int mrs, const float* __restrict__ halo_all, long long hstride,
+           MeshGeom g, TileGeom tg) {
   int o[3], s[3], ti[3];
   const int t = blockIdx.x;
   tile_origin(t, tg, g, o, s, ti);
@@ -359,7 +366,10 @@ __global__ void __launch_bounds__(PMX_TPB)
   const int nB = (s[0] - 1) * s[2];
   const int nC = (s[0] - 1) * (s[1] - 1);
   const int ntot = nA + nB + nC;
+  for (int ch = 0; ch < nch; ++ch)
   for (int i = threadIdx.x; i < ntot; i += PMX_TPB) {
+    float* mesh = ch == 0 ? out.mesh[0] : (ch == 1 ? out.mesh[1] : out.mesh[2]);
+    const float* halo = halo_all + ch * hstride;
     int l[3];
     if (i < nA) {
       l[0] = 0; l[1] = i / s[2]; l[2] = i % s[2];
@@ -394,77 +404,145 @@ __global__ void __launch_bounds__(PMX_TPB)
   }
 }
 
-// Writes the binned records (and nch per-particle values, channel-major, to
-// c->vrec) once the tile counts are scanned.
+// Writes the binned records once the tile counts are scanned.
 int pmx_bin_fill(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
-                    const float* pval, int vstride, int nch) {
+                 const uint32_t* cond) {
   if (n == 0) return PMX_OK;
-  if (nch > 0) {
-    int rc = pmx_ensure_vrec(c, n);
-    if (rc) return rc;
-  }
-  MeshGeom g = pmx_geom_padded(c);
-  if (nch > 0) {
-    int rc = pmx_fill_u32(c, reinterpret_cast<uint32_t*>(c->vmax), 3, 0u);
-    if (rc) return rc;
-  }
+  const MeshGeom g = pmx_geom_padded(c);
+  const unsigned nb = cond ? pmx_blocks_capped(n, PMX_REBIN_GRID) : pmx_blocks(n);
   pmx_tic(c, PMX_T_FILL);
-  bin_fill_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(pmid, disp, n, g, c->tg, c->pkey, c->prank,
-                                                       c->tile_start, c->rec, pval, vstride, nch,
-                                                       c->vrec, c->vmax, c->pidx);
+  bin_fill_k<<<nb, PMX_TPB, 0, c->stream>>>(pmid, disp, n, g, c->tg, c->pkey, c->prank,
+                                            c->tile_start, c->rec, c->pidx, cond);
   pmx_toc(c, PMX_T_FILL);
   PMX_LAUNCH_CHECK();
   return PMX_OK;
 }
 
-int pmx_scan_tiles(pmx_ctx* c) {
+int pmx_scan_tiles(pmx_ctx* c, const uint32_t* cond) {
   pmx_tic(c, PMX_T_SCAN);
-  int rc = pmx_scan_exclusive(c, c->tile_count, c->tile_start, c->ntiles + 1);
+  int rc = pmx_scan_exclusive(c, c->tile_count, c->tile_start, c->ntiles + 1, cond);
   pmx_toc(c, PMX_T_SCAN);
   return rc;
 }
 
-// Bins the particles by tile (c->rec, c->tile_start, c->tile_count).
-int pmx_bin_particles(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n) {
+// Capacities from the counts of the last binning: count + slack per tile
+// (0 for the end slot), then the counts and the overflow flag are cleared.
+__global__ void tile_caps_k(uint32_t* __restrict__ count, uint32_t* __restrict__ cap, long long nt,
+                            uint32_t* __restrict__ flag) {
+  const long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  if (i <= nt) {
+    const uint32_t k = count[i];
+    cap[i] = i < nt ? k + pmx_tile_slack(k) : 0u;
+    count[i] = 0u;
+  }
+  if (i == 0) flag[0] = 0u;
+}
+
+int pmx_bin_begin(pmx_ctx* c, long long n, int* cap) {
   int rc;
   if ((rc = pmx_ensure_ptcl(c, n))) return rc;
-  if ((rc = pmx_fill_u32(c, c->tile_count, c->ntiles + 1, 0u))) return rc;
-  MeshGeom g = pmx_geom_padded(c);
-  if (n > 0) {
+  // the capacities of caps_n particles fit the record slots and 32-bit slots
+  *cap = c->caps_valid && c->caps_n <= c->ptcl_cap && c->rec_cap < (1LL << 32) - 1;
+  c->caps_valid = 0;  // until pmx_bin_end has exact counts again
+  if (!*cap) return pmx_fill_u32(c, c->tile_count, c->ntiles + 1, 0u);
+  pmx_tic(c, PMX_T_SCAN);
+  tile_caps_k<<<pmx_blocks(c->ntiles + 1), PMX_TPB, 0, c->stream>>>(c->tile_count, c->tile_cap,
+                                                                    c->ntiles, c->flags);
+  PMX_LAUNCH_CHECK();
+  rc = pmx_scan_exclusive(c, c->tile_cap, c->tile_start, c->ntiles + 1);
+  pmx_toc(c, PMX_T_SCAN);
+  return rc;
+}
+
+BinOut pmx_bin_out(pmx_ctx* c) {
+  return BinOut{c->pkey, c->prank, c->tile_start, c->rec, c->pidx, c->flags};
+}
+
+int pmx_bin_end(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n, int cap) {
+  int rc;
+  if (!cap) {
+    if ((rc = pmx_scan_tiles(c))) return rc;
+    if ((rc = pmx_bin_fill(c, pmid, disp, n))) return rc;
+  } else if (n > 0) {
+    // a tile overflowed its capacity: count, scan and fill exactly, all
+    // conditional on the device-side flag
+    const uint32_t* f = c->flags;
+    const MeshGeom g = pmx_geom_padded(c);
+    if ((rc = pmx_fill_u32(c, c->tile_count, c->ntiles + 1, 0u, f))) return rc;
     pmx_tic(c, PMX_T_BIN);
-    bin_count_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(pmid, disp, n, g, c->tg, c->tile_count,
-                                                          c->pkey, c->prank);
+    bin_count_k<false><<<pmx_blocks_capped(n, PMX_REBIN_GRID), PMX_TPB, 0, c->stream>>>(
+        pmid, disp, n, g, c->tg, c->tile_count, pmx_bin_out(c), f);
+    pmx_toc(c, PMX_T_BIN);
+    PMX_LAUNCH_CHECK();
+    if ((rc = pmx_scan_tiles(c, f))) return rc;
+    if ((rc = pmx_bin_fill(c, pmid, disp, n, f))) return rc;
+  }
+  c->caps_valid = 1;
+  c->caps_n = n;
+  return PMX_OK;
+}
+
+// Bins the particles by tile (c->rec, c->tile_start, c->tile_count).
+int pmx_bin_particles(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n) {
+  int rc, cap;
+  if ((rc = pmx_bin_begin(c, n, &cap))) return rc;
+  if (n > 0) {
+    const MeshGeom g = pmx_geom_padded(c);
+    const BinOut o = pmx_bin_out(c);
+    pmx_tic(c, PMX_T_BIN);
+    if (cap)
+      bin_count_k<true><<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(pmid, disp, n, g, c->tg,
+                                                                  c->tile_count, o, nullptr);
+    else
+      bin_count_k<false><<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(pmid, disp, n, g, c->tg,
+                                                                   c->tile_count, o, nullptr);
     pmx_toc(c, PMX_T_BIN);
     PMX_LAUNCH_CHECK();
   }
-  if ((rc = pmx_scan_tiles(c))) return rc;
-  return pmx_bin_fill(c, pmid, disp, n, nullptr, 0, 0);
+  return pmx_bin_end(c, pmid, disp, n, cap);
 }
 
 // Deposits the binned records into mesh (row stride mrs), with the scalar
 // value val or the per-record values of channel ch of c->vrec (ch >= 0).
-int pmx_deposit_binned(pmx_ctx* c, float val, int ch, long long n, float* mesh, int mrs,
-                          int accumulate) {
-  MeshGeom g = pmx_geom_padded(c);
+static int deposit_run(pmx_ctx* c, float val, int perval, int nch, long long n, DepOut out,
+                       int mrs, int accumulate) {
+  const MeshGeom g = pmx_geom_padded(c);
   const TileGeom& tg = c->tg;
-  size_t lds = (size_t)(tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(unsigned long long);
-  unsigned nt = (unsigned)c->ntiles;
+  const size_t lds = (size_t)(tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(unsigned long long);
+  const unsigned nt = (unsigned)c->ntiles;
+  const long long hstride = c->ntiles * tg.shell;
+  int rc;
+  if ((rc = pmx_ensure_halo(c, nch))) return rc;
   pmx_tic(c, PMX_T_DEPOSIT);
-  if (ch >= 0)
-    deposit_k<1><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, c->vrec + (long long)ch * n, c->vmax + ch,
-                                                  c->tile_start, c->tile_count, g, tg, val, mesh,
-                                                  mrs, accumulate, c->halo);
+  if (perval)
+    deposit_k<1><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, c->vrec, c->rec_cap, c->vmax,
+                                                  c->tile_start, c->tile_count, g, tg, val, out,
+                                                  nch, mrs, accumulate, c->halo, hstride);
   else
-    deposit_k<0><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, nullptr, nullptr, c->tile_start,
-                                                  c->tile_count, g, tg, val, mesh, mrs, accumulate,
-                                                  c->halo);
+    deposit_k<0><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, nullptr, 0, nullptr, c->tile_start,
+                                                  c->tile_count, g, tg, val, out, nch, mrs,
+                                                  accumulate, c->halo, hstride);
   pmx_toc(c, PMX_T_DEPOSIT);
   PMX_LAUNCH_CHECK();
   pmx_tic(c, PMX_T_FOLD);
-  fold_k<<<nt, PMX_TPB, 0, c->stream>>>(mesh, mrs, c->halo, g, tg);
+  fold_k<<<nt, PMX_TPB, 0, c->stream>>>(out, nch, mrs, c->halo, hstride, g, tg);
   pmx_toc(c, PMX_T_FOLD);
   PMX_LAUNCH_CHECK();
   return PMX_OK;
+}
+
+// Deposits the binned records into mesh (row stride mrs) with the scalar
+// value val.
+int pmx_deposit_binned(pmx_ctx* c, float val, long long n, float* mesh, int mrs, int accumulate) {
+  DepOut out = {{mesh, mesh, mesh}};
+  return deposit_run(c, val, 0, 1, n, out, mrs, accumulate);
+}
+
+// Deposits the three per-record value channels of c->vrec into three meshes
+// (row stride mrs), overwriting them.
+int pmx_deposit_binned3(pmx_ctx* c, long long n, float* const* mesh, int mrs) {
+  DepOut out = {{mesh[0], mesh[1], mesh[2]}};
+  return deposit_run(c, 0.f, 1, 3, n, out, mrs, 0);
 }
 
 extern "C" int pmx_scatter_mass(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
@@ -474,7 +552,7 @@ extern "C" int pmx_scatter_mass(pmx_ctx* c, const int16_t* pmid, const float* di
   PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_scatter_mass: too many particles");
   int rc;
   if ((rc = pmx_bin_particles(c, pmid, disp, n))) return rc;
-  return pmx_deposit_binned(c, val, -1, n, mesh, c->n[2], accumulate);
+  return pmx_deposit_binned(c, val, n, mesh, c->n[2], accumulate);
 }
 
 // Tile-staged gather of the force meshes in c->cbuf (padded) for the
@@ -504,7 +582,7 @@ int pmx_gravity_fwd_meshes(pmx_ctx* c, const int16_t* pmid, const float* disp, l
                               float val, float coef) {
   int rc;
   if ((rc = pmx_bin_particles(c, pmid, disp, n))) return rc;
-  if ((rc = pmx_deposit_binned(c, val, -1, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
+  if ((rc = pmx_deposit_binned(c, val, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
   return pmx_poisson_ctx(c, coef);
 }
 
@@ -541,13 +619,14 @@ extern "C" int pmx_drift(pmx_ctx* c, long long n, float* disp, const float* vel,
   return PMX_OK;
 }
 
-// kick (k1) + drift (d) fused with the tile count of the drifted particles;
+// kick (k1) + drift (d) fused with the binning of the drifted particles;
 // the block's vel, acc and disp rows are staged through LDS
+template <bool CAP>
 __global__ void __launch_bounds__(PMX_TPB)
     kick_drift_bin_k(const int16_t* __restrict__ pmid, float* __restrict__ disp,
                      float* __restrict__ vel, const float* __restrict__ acc, long long n, float k1,
                      float dd, MeshGeom g, TileGeom tg, uint32_t* __restrict__ tile_count,
-                     uint32_t* __restrict__ ptile, uint32_t* __restrict__ prank) {
+                     BinOut o) {
   constexpr int T3 = 3 * PMX_TPB;
   __shared__ __align__(16) float sh[3 * T3];  // vel | acc | disp
   __shared__ __align__(16) int16_t sp[T3];
@@ -559,9 +638,10 @@ __global__ void __launch_bounds__(PMX_TPB)
   const int q = threadIdx.x;
   const bool active = q < cnt;
   uint32_t t = 0;
+  CicAxis ax[3];
+  float dp[3];
   if (active) {
     int pm[3];
-    float dp[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       pm[a] = sp[3 * q + a];
@@ -573,15 +653,11 @@ __global__ void __launch_bounds__(PMX_TPB)
       sh[2 * T3 + 3 * q + a] = x;
       dp[a] = x;
     }
-    CicAxis ax[3];
     cic_setup(pm, dp, g, ax);
     t = tile_of(ax, tg);
   }
   const uint32_t r = agg_slot(tile_count, t, active);
-  if (active) {
-    ptile[b0 + q] = t;
-    prank[b0 + q] = r;
-  }
+  if (active) pmx_bin_emit<CAP>(o, b0 + q, t, r, ax, tg);
   __syncthreads();
   pmx_stage_out3<1, PMX_TPB>({{vel}}, sh, b0, cnt);
   pmx_stage_out3<1, PMX_TPB>({{disp}}, sh + 2 * T3, b0, cnt);
@@ -592,20 +668,23 @@ extern "C" int pmx_nbody_step(pmx_ctx* c, const int16_t* pmid, float* disp, floa
   PMX_REQUIRE(c && n >= 0, "pmx_nbody_step: bad argument");
   PMX_REQUIRE(n == 0 || (pmid && disp && vel && acc), "pmx_nbody_step: NULL pointer");
   PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_nbody_step: too many particles");
-  int rc;
-  if ((rc = pmx_ensure_ptcl(c, n))) return rc;
-  if ((rc = pmx_fill_u32(c, c->tile_count, c->ntiles + 1, 0u))) return rc;
+  int rc, cap;
+  if ((rc = pmx_bin_begin(c, n, &cap))) return rc;
   MeshGeom g = pmx_geom_padded(c);
   if (n > 0) {
+    const BinOut o = pmx_bin_out(c);
     pmx_tic(c, PMX_T_BIN);
-    kick_drift_bin_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(
-        pmid, disp, vel, acc, n, k1, d, g, c->tg, c->tile_count, c->pkey, c->prank);
+    if (cap)
+      kick_drift_bin_k<true><<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(
+          pmid, disp, vel, acc, n, k1, d, g, c->tg, c->tile_count, o);
+    else
+      kick_drift_bin_k<false><<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(
+          pmid, disp, vel, acc, n, k1, d, g, c->tg, c->tile_count, o);
     pmx_toc(c, PMX_T_BIN);
     PMX_LAUNCH_CHECK();
   }
-  if ((rc = pmx_scan_tiles(c))) return rc;
-  if ((rc = pmx_bin_fill(c, pmid, disp, n, nullptr, 0, 0))) return rc;
-  if ((rc = pmx_deposit_binned(c, val, -1, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
+  if ((rc = pmx_bin_end(c, pmid, disp, n, cap))) return rc;
+  if ((rc = pmx_deposit_binned(c, val, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
   if ((rc = pmx_poisson_ctx(c, coef))) return rc;
   if (n > 0 && (rc = pmx_tile_gather(c, acc, vel, k2))) return rc;
   return PMX_OK;

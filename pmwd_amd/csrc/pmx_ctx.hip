@@ -52,6 +52,10 @@ static TileGeom make_tiles(const int n[3]) {
   t.nty = (n[1] + t.ty - 1) / t.ty;
   t.ntz = (n[2] + t.tz - 1) / t.tz;
   t.shell = (t.tx + 1) * (t.ty + 1) * (t.tz + 1) - t.tx * t.ty * t.tz;
+  // exact for extents <= 16 and mesh indices < 4096 (pmx_tdiv)
+  t.mx = ((1u << 20) + t.tx - 1) / t.tx;
+  t.my = ((1u << 20) + t.ty - 1) / t.ty;
+  t.mz = ((1u << 20) + t.tz - 1) / t.tz;
   return t;
 }
 
@@ -130,7 +134,9 @@ extern "C" int pmx_ctx_create(int n0, int n1, int n2, double cell_size, void* st
   c->ntiles = (long long)c->tg.ntx * c->tg.nty * c->tg.ntz;
   if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->tile_count, (size_t)(c->ntiles + 1) * 4);
   if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->tile_start, (size_t)(c->ntiles + 1) * 4);
-  if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->halo, (size_t)c->ntiles * c->tg.shell * 4);
+  if (rc == PMX_OK) rc |= pmx_ensure_halo(c, 1);
+  if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->tile_cap, (size_t)(c->ntiles + 1) * 4);
+  if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->flags, 4 * sizeof(uint32_t));
   if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->vmax, 3 * sizeof(float));
   if (rc == PMX_OK) rc |= pmx_ensure_red(c, 4096);
   if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->red_out, PMX_RED_SLOTS * sizeof(double));
@@ -168,7 +174,8 @@ extern "C" int pmx_ctx_destroy(pmx_ctx* c) {
     if (c->tw[a]) hipFree(c->tw[a]);
     if (c->cbuf[a]) hipFree(c->cbuf[a]);
   }
-  void* ps[] = {c->tile_count, c->tile_start, c->halo, c->rec, c->pkey, c->prank, c->pidx, c->vrec, c->vmax, c->red,
+  void* ps[] = {c->tile_count, c->tile_start, c->tile_cap, c->flags, c->halo, c->rec, c->pkey,
+                c->prank, c->pidx, c->vrec, c->vmax, c->red,
                 c->red_out, c->sort_k[0], c->sort_k[1], c->sort_v[0], c->sort_v[1], c->sort_hist,
                 c->scan_tmp};
   for (void* p : ps)
@@ -194,24 +201,39 @@ extern "C" int pmx_tile_shape(const pmx_ctx* c, int t[3]) {
 
 int pmx_ensure_ptcl(pmx_ctx* c, long long n) {
   if (n <= c->ptcl_cap) return PMX_OK;
-  long long cap = n + n / 8 + 1024;
-  dev_free(c, (void**)&c->rec, c->ptcl_cap * 16);
+  const long long cap = n + n / 16 + 1024;
+  // records also hold the per-tile slack of the capacity binning
+  const long long rcap = pmx_record_cap(c, cap);
+  dev_free(c, (void**)&c->rec, c->rec_cap * 16);
+  dev_free(c, (void**)&c->pidx, c->rec_cap * 4);
   dev_free(c, (void**)&c->pkey, c->ptcl_cap * 4);
   dev_free(c, (void**)&c->prank, c->ptcl_cap * 4);
-  dev_free(c, (void**)&c->pidx, c->ptcl_cap * 4);
-  c->ptcl_cap = 0;
-  int rc = dev_alloc(c, (void**)&c->rec, cap * 16);
+  dev_free(c, (void**)&c->vrec, c->vrec_cap * 4);
+  c->ptcl_cap = c->rec_cap = c->vrec_cap = 0;
+  c->caps_valid = 0;
+  int rc = dev_alloc(c, (void**)&c->rec, rcap * 16);
+  rc |= dev_alloc(c, (void**)&c->pidx, rcap * 4);
   rc |= dev_alloc(c, (void**)&c->pkey, cap * 4);
   rc |= dev_alloc(c, (void**)&c->prank, cap * 4);
-  rc |= dev_alloc(c, (void**)&c->pidx, cap * 4);
   if (rc) return PMX_ENOMEM;
   c->ptcl_cap = cap;
+  c->rec_cap = rcap;
   return PMX_OK;
 }
 
-int pmx_ensure_vrec(pmx_ctx* c, long long n) {
-  if (3 * n <= c->vrec_cap) return PMX_OK;
-  long long cap = 3 * (n + n / 8 + 1024);
+int pmx_ensure_halo(pmx_ctx* c, int nch) {
+  if (nch <= c->halo_nch) return PMX_OK;
+  dev_free(c, (void**)&c->halo, (size_t)c->halo_nch * c->ntiles * c->tg.shell * 4);
+  c->halo_nch = 0;
+  if (dev_alloc(c, (void**)&c->halo, (size_t)nch * c->ntiles * c->tg.shell * 4)) return PMX_ENOMEM;
+  c->halo_nch = nch;
+  return PMX_OK;
+}
+
+// three channels of per-record values, rec_cap floats apart
+int pmx_ensure_vrec(pmx_ctx* c) {
+  const long long cap = 3 * c->rec_cap;
+  if (cap <= c->vrec_cap) return PMX_OK;
   dev_free(c, (void**)&c->vrec, c->vrec_cap * 4);
   c->vrec_cap = 0;
   if (dev_alloc(c, (void**)&c->vrec, cap * 4)) return PMX_ENOMEM;
@@ -259,7 +281,11 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
   return incl - v;
 }
 
-__global__ void scan_reduce_k(const uint32_t* __restrict__ in, long long n, uint32_t* __restrict__ bsum) {
+// The scan kernels return at once when cond is given and *cond == 0 (a
+// device-side decision, no host round trip).
+__global__ void scan_reduce_k(const uint32_t* __restrict__ in, long long n, uint32_t* __restrict__ bsum,
+                              const uint32_t* __restrict__ cond) {
+  if (cond && *cond == 0) return;
   __shared__ uint32_t sh[PMX_TPB];
   long long base = (long long)blockIdx.x * SCAN_TILE;
   uint32_t s = 0;
@@ -272,7 +298,9 @@ __global__ void scan_reduce_k(const uint32_t* __restrict__ in, long long n, uint
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ void scan_single_k(uint32_t* __restrict__ data, long long n) {
+__global__ void scan_single_k(uint32_t* __restrict__ data, long long n,
+                              const uint32_t* __restrict__ cond) {
+  if (cond && *cond == 0) return;
   __shared__ uint32_t sh[PMX_TPB];
   uint32_t carry = 0;
   for (long long base = 0; base < n; base += PMX_TPB) {
@@ -286,7 +314,9 @@ __global__ void scan_single_k(uint32_t* __restrict__ data, long long n) {
 }
 
 __global__ void scan_apply_k(const uint32_t* __restrict__ in, long long n,
-                             const uint32_t* __restrict__ boff, uint32_t* __restrict__ out) {
+                             const uint32_t* __restrict__ boff, uint32_t* __restrict__ out,
+                             const uint32_t* __restrict__ cond) {
+  if (cond && *cond == 0) return;
   __shared__ uint32_t sh[PMX_TPB];
   long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
   uint32_t v[SCAN_ITEMS];
@@ -305,7 +335,8 @@ __global__ void scan_apply_k(const uint32_t* __restrict__ in, long long n,
   }
 }
 
-int pmx_scan_exclusive(pmx_ctx* c, const uint32_t* in, uint32_t* out, long long n) {
+int pmx_scan_exclusive(pmx_ctx* c, const uint32_t* in, uint32_t* out, long long n,
+                       const uint32_t* cond) {
   long long nb = (n + SCAN_TILE - 1) / SCAN_TILE;
   if (nb < 1) nb = 1;
   if (nb > c->scan_tmp_cap) {
@@ -314,9 +345,9 @@ int pmx_scan_exclusive(pmx_ctx* c, const uint32_t* in, uint32_t* out, long long 
     if (dev_alloc(c, (void**)&c->scan_tmp, nb * 4)) return PMX_ENOMEM;
     c->scan_tmp_cap = nb;
   }
-  scan_reduce_k<<<(unsigned)nb, PMX_TPB, 0, c->stream>>>(in, n, c->scan_tmp);
-  scan_single_k<<<1, PMX_TPB, 0, c->stream>>>(c->scan_tmp, nb);
-  scan_apply_k<<<(unsigned)nb, PMX_TPB, 0, c->stream>>>(in, n, c->scan_tmp, out);
+  scan_reduce_k<<<(unsigned)nb, PMX_TPB, 0, c->stream>>>(in, n, c->scan_tmp, cond);
+  scan_single_k<<<1, PMX_TPB, 0, c->stream>>>(c->scan_tmp, nb, cond);
+  scan_apply_k<<<(unsigned)nb, PMX_TPB, 0, c->stream>>>(in, n, c->scan_tmp, out, cond);
   PMX_LAUNCH_CHECK();
   return PMX_OK;
 }

@@ -18,8 +18,8 @@
 #define FFT_MAXV (FFT_CAP / FFT_TPB) // values per thread
 #define RFFT_CAP 4096                // row passes: fewer values per thread
 #define RFFT_MAXV (RFFT_CAP / FFT_TPB)
-#define PFFT_TPB 1024                // Poisson x pass: 16 columns of 512 per WG,
-#define PFFT_CAP 8192                // whole 128-byte segments per row
+#define PFFT_TPB 512                 // Poisson x pass: 8 columns of 512 per WG,
+#define PFFT_CAP 4096                // two such blocks (68 KB of LDS each) per CU
 #define PFFT_MAXV (PFFT_CAP / PFFT_TPB)
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -544,18 +544,23 @@ struct KTab {
 };
 
 // x pass of the Poisson solve (gravity.py:56-66): forward FFT along x of
-// c0, pot = -src/k^2 (k=0 -> 0), then for d = 0,1,2: g_d = (-i km_d) pot,
-// inverse FFT along x, stored to c_d.  Pencils along axis 0 of (n0,n1,n2h);
+// c0, pot = -src/k^2 (k=0 -> 0), g_d = (-i km_d) pot, inverse FFT along x,
+// stored to c_d.  km_1 (one value per y row) and km_2 (one per kz column)
+// are constant along x, so the y and z components share one inverse
+// transform h = IFFT_x(-i pot): c_1 = km_1 h and c_2 = km_2 h.  Only the x
+// component needs a transform of its own; the two inverse transforms run
+// side by side as 2B pencils.  Pencils along axis 0 of (n0,n1,n2h);
 // blockIdx.y = y index, blockIdx.x = block of B kz columns.
-// LDS: the n0 x B block, the potential (each thread reads back only what it
-// wrote), then the n0-entry twiddle table.
+// LDS: 2 n0 B values (the forward block, then the two inverse transforms
+// interleaved: column b of the x component at j 2B + b, of h at j 2B + B + b),
+// then the n0-entry twiddle table.
 template <bool POW2>
 __global__ void __launch_bounds__(PFFT_TPB) poisson_x_k(Fields3 fl, int n0, int n1, int n2h, int n2p, int B,
                                                        KTab kt, const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
   const int total = n0 * B;
-  float2* sp = s + total;
-  float2* stw = sp + total;
+  const int B2 = 2 * B;
+  float2* stw = s + 2 * total;
   const long long es = (long long)n1 * n2p;
   const int lB = __ffs(B) - 1;
   int bx, y, bz;
@@ -575,61 +580,76 @@ __global__ void __launch_bounds__(PFFT_TPB) poisson_x_k(Fields3 fl, int n0, int 
   lds_fft<-1, COLS, POW2, PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
   const float ky = kt.k[1][y];
   const float kym = kt.km[1][y];
+  float2 h[PFFT_MAXV];
 #pragma unroll
   for (int it = 0; it < PFFT_MAXV; ++it) {
     int idx = threadIdx.x + it * PFFT_TPB;
+    h[it] = make_float2(0.f, 0.f);
     if (idx < total) {
       int j = idx >> lB, b = idx & (B - 1);
-      float2 pot = make_float2(0.f, 0.f);
       if (b < nb) {
         float2 v = s[idx];
         float kx = kt.k[0][j], kz = kt.k[2][c0 + b];
         float k2 = kx * kx;
         k2 = k2 + ky * ky;
         k2 = k2 + kz * kz;
-        if (k2 != 0.f) pot = make_float2((-v.x) / k2, (-v.y) / k2);
+        // pot = -v / k^2, h = -i pot = (pot.y, -pot.x)
+        if (k2 != 0.f) h[it] = make_float2((-v.y) / k2, -((-v.x) / k2));
       }
-      sp[idx] = pot;
     }
   }
   __syncthreads();
-#pragma unroll 1
-  for (int d = 0; d < 3; ++d) {
 #pragma unroll
-    for (int it = 0; it < PFFT_MAXV; ++it) {
-      int idx = threadIdx.x + it * PFFT_TPB;
-      if (idx < total) {
-        int j = idx >> lB, b = idx & (B - 1);
-        float km = 0.f;
-        if (b < nb) km = d == 0 ? kt.km[0][j] : (d == 1 ? kym : kt.km[2][c0 + b]);
-        float2 pot = sp[idx];
-        s[idx] = make_float2(km * pot.y, -(km * pot.x));
+  for (int it = 0; it < PFFT_MAXV; ++it) {
+    int idx = threadIdx.x + it * PFFT_TPB;
+    if (idx < total) {
+      int j = idx >> lB, b = idx & (B - 1);
+      float kmx = kt.km[0][j];
+      s[j * B2 + b] = make_float2(kmx * h[it].x, kmx * h[it].y);
+      s[j * B2 + B + b] = h[it];
+    }
+  }
+  __syncthreads();
+  lds_fft<1, COLS, POW2, 2 * PFFT_MAXV, PFFT_TPB>(s, n0, B2, stw);
+  float2 r[2 * PFFT_MAXV];
+#pragma unroll
+  for (int it = 0; it < 2 * PFFT_MAXV; ++it) {
+    int idx = threadIdx.x + it * PFFT_TPB;
+    if (idx < 2 * total) r[it] = s[idx];
+  }
+#pragma unroll
+  for (int it = 0; it < 2 * PFFT_MAXV; ++it) {
+    int idx = threadIdx.x + it * PFFT_TPB;
+    if (idx < 2 * total) {
+      int j = idx >> (lB + 1), p = idx & (B2 - 1);
+      int b = p < B ? p : p - B;
+      if (b < nb) {
+        long long o = off + j * es + b;
+        if (p < B) {
+          fl.f[0][o] = r[it];
+        } else {
+          float kmz = kt.km[2][c0 + b];
+          fl.f[1][o] = make_float2(kym * r[it].x, kym * r[it].y);
+          fl.f[2][o] = make_float2(kmz * r[it].x, kmz * r[it].y);
+        }
       }
     }
-    __syncthreads();
-    lds_fft<1, COLS, POW2, PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
-    // selects rather than indexes, so the argument struct stays in registers
-    float2* out = d == 0 ? fl.f[0] : (d == 1 ? fl.f[1] : fl.f[2]);
-    batch_copy<PFFT_MAXV, PFFT_TPB>(
-        total, [&](int idx) { return s[idx]; },
-        [&](int idx, float2 v) {
-          int j = idx >> lB, b = idx & (B - 1);
-          if (b < nb) out[off + j * es + b] = v;
-        });
-    __syncthreads();
   }
 }
 
 // Adjoint x pass: acc = sum_d (-i km_d) laplace(FFT_x c_d); inverse FFT
-// along x; stored to c0.  Transpose of the three force components.  LDS as
-// poisson_x_k, the second block holding the running sum.
+// along x; stored to c0.  Transpose of the three force components.  As km_1
+// and km_2 are constant along x, FFT_x(km_1 c_1 + km_2 c_2) replaces the two
+// transforms of c_1 and c_2: the block holds c_0 and that combination side
+// by side as 2B pencils (j 2B + b and j 2B + B + b) for one forward pass,
+// then the sum in B pencils for the inverse.  LDS as poisson_x_k.
 template <bool POW2>
 __global__ void __launch_bounds__(PFFT_TPB) poisson_x_adj_k(Fields3 fl, int n0, int n1, int n2h, int n2p, int B,
                                                            KTab kt, const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
   const int total = n0 * B;
-  float2* sa = s + total;
-  float2* stw = sa + total;
+  const int B2 = 2 * B;
+  float2* stw = s + 2 * total;
   const long long es = (long long)n1 * n2p;
   const int lB = __ffs(B) - 1;
   int bx, y, bz;
@@ -640,48 +660,54 @@ __global__ void __launch_bounds__(PFFT_TPB) poisson_x_adj_k(Fields3 fl, int n0, 
   const float ky = kt.k[1][y];
   const float kym = kt.km[1][y];
   stage_twiddles<PFFT_TPB>(stw, tw, n0);
+  batch_copy<2 * PFFT_MAXV, PFFT_TPB>(
+      2 * total,
+      [&](int idx) {
+        int j = idx >> (lB + 1), p = idx & (B2 - 1);
+        int b = p < B ? p : p - B;
+        float2 v = make_float2(0.f, 0.f);
+        if (b < nb) {
+          long long o = off + j * es + b;
+          if (p < B) {
+            v = fl.f[0][o];
+          } else {
+            float kmz = kt.km[2][c0 + b];
+            float2 a = fl.f[1][o], c = fl.f[2][o];
+            v = make_float2(kym * a.x + kmz * c.x, kym * a.y + kmz * c.y);
+          }
+        }
+        return v;
+      },
+      [&](int idx, float2 v) { s[idx] = v; });
+  __syncthreads();
+  lds_fft<-1, COLS, POW2, 2 * PFFT_MAXV, PFFT_TPB>(s, n0, B2, stw);
+  float2 acc[PFFT_MAXV];
 #pragma unroll
   for (int it = 0; it < PFFT_MAXV; ++it) {
     int idx = threadIdx.x + it * PFFT_TPB;
-    if (idx < total) sa[idx] = make_float2(0.f, 0.f);
-  }
-#pragma unroll 1
-  for (int d = 0; d < 3; ++d) {
-    // selects rather than indexes, so the argument struct stays in registers
-    const float2* in = d == 0 ? fl.f[0] : (d == 1 ? fl.f[1] : fl.f[2]);
-    batch_copy<PFFT_MAXV, PFFT_TPB>(
-        total,
-        [&](int idx) {
-          int j = idx >> lB, b = idx & (B - 1);
-          return b < nb ? in[off + j * es + b] : make_float2(0.f, 0.f);
-        },
-        [&](int idx, float2 v) { s[idx] = v; });
-    __syncthreads();
-    lds_fft<-1, COLS, POW2, PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
-#pragma unroll
-    for (int it = 0; it < PFFT_MAXV; ++it) {
-      int idx = threadIdx.x + it * PFFT_TPB;
-      if (idx < total) {
-        int j = idx >> lB, b = idx & (B - 1);
-        if (b < nb) {
-          float2 v = s[idx];
-          float kx = kt.k[0][j], kz = kt.k[2][c0 + b];
-          float k2 = kx * kx;
-          k2 = k2 + ky * ky;
-          k2 = k2 + kz * kz;
-          float2 pot = k2 != 0.f ? make_float2((-v.x) / k2, (-v.y) / k2) : make_float2(0.f, 0.f);
-          float km = d == 0 ? kt.km[0][j] : (d == 1 ? kym : kt.km[2][c0 + b]);
-          float2 g = make_float2(km * pot.y, -(km * pot.x));
-          sa[idx] = cadd(sa[idx], g);
+    acc[it] = make_float2(0.f, 0.f);
+    if (idx < total) {
+      int j = idx >> lB, b = idx & (B - 1);
+      if (b < nb) {
+        float kx = kt.k[0][j], kz = kt.k[2][c0 + b];
+        float k2 = kx * kx;
+        k2 = k2 + ky * ky;
+        k2 = k2 + kz * kz;
+        if (k2 != 0.f) {
+          float kmx = kt.km[0][j];
+          float2 x0 = s[j * B2 + b], yz = s[j * B2 + B + b];
+          float sx = kmx * x0.x + yz.x, sy = kmx * x0.y + yz.y;
+          // pot = -sum / k^2, acc = -i pot = (pot.y, -pot.x)
+          acc[it] = make_float2((-sy) / k2, -((-sx) / k2));
         }
       }
     }
-    __syncthreads();
   }
+  __syncthreads();
 #pragma unroll
   for (int it = 0; it < PFFT_MAXV; ++it) {
     int idx = threadIdx.x + it * PFFT_TPB;
-    if (idx < total) s[idx] = sa[idx];
+    if (idx < total) s[idx] = acc[it];
   }
   __syncthreads();
   lds_fft<1, COLS, POW2, PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);

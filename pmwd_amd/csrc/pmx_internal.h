@@ -62,7 +62,14 @@ struct TileGeom {
   int tx, ty, tz;      // tile extent (clipped to the mesh)
   int ntx, nty, ntz;   // tiles per axis
   int shell;           // floats reserved per tile for the +1 halo shell
+  // ceil(2^20 / extent): lo / t == (lo * m) >> 20 exactly for lo < 4096 and
+  // t <= 16 (error below lo / 2^20 <= 1/256 < 1/t), a 24-bit multiply
+  unsigned mx, my, mz;
 };
+
+__device__ __forceinline__ int pmx_tdiv(int lo, unsigned m) {
+  return (int)(__umul24((unsigned)lo, m) >> 20);
+}
 
 // ---------------------------------------------------------- kernel timing --
 // Optional per-kernel-class timing with HIP events on the context stream.
@@ -119,7 +126,17 @@ struct pmx_ctx {
   long long ntiles;
   uint32_t* tile_count;
   uint32_t* tile_start;
-  float* halo;
+  // Capacity binning: each tile gets room for its count of the last binning
+  // plus slack (tile_cap), tile_start the scan of the capacities, and the
+  // binning kernel writes the records itself; flags[0] is set on the device
+  // when a tile overflows, which runs the exact count/scan/fill after it.
+  uint32_t* tile_cap;
+  uint32_t* flags;
+  int caps_valid;      // host: tile_count holds the exact counts of caps_n particles
+  long long caps_n;
+  long long rec_cap;   // record slots of rec/pidx (and the vrec channel stride)
+  float* halo;         // nch shells per tile, channel-major
+  int halo_nch;        // channels the halo buffer holds
   long long ptcl_cap;
   float4* rec;         // binned particle records (d per axis, packed local cell)
   uint32_t* pkey;      // tile of each particle
@@ -148,9 +165,17 @@ struct pmx_ctx {
 };
 
 int pmx_ensure_ptcl(pmx_ctx* c, long long n);
-int pmx_ensure_vrec(pmx_ctx* c, long long n);
+int pmx_ensure_vrec(pmx_ctx* c);
 int pmx_ensure_red(pmx_ctx* c, long long nblk);
-int pmx_scan_exclusive(pmx_ctx* c, const uint32_t* in, uint32_t* out, long long n);
+int pmx_scan_exclusive(pmx_ctx* c, const uint32_t* in, uint32_t* out, long long n,
+                       const uint32_t* cond = nullptr);
+
+// Slack of a tile's capacity over its last count, and the record slots the
+// capacities of n particles can take (sum over tiles of count + slack).
+__host__ __device__ inline uint32_t pmx_tile_slack(uint32_t count) { return (count >> 3) + 32; }
+inline long long pmx_record_cap(const pmx_ctx* c, long long n) {
+  return n + n / 8 + 33 * (c->ntiles + 1);
+}
 MeshGeom pmx_geom_padded(const pmx_ctx* c);
 
 // grid helpers
@@ -172,7 +197,11 @@ __device__ __forceinline__ unsigned pmx_xcd_block(unsigned b, unsigned nb) {
   return (k < r ? k * (q + 1) : r * (q + 1) + (k - r) * q) + i;
 }
 
+// i mod n in [0, n); one period either side without a division
 __device__ __forceinline__ int pmx_wrap(int i, int n) {
+  if ((unsigned)i < (unsigned)n) return i;
+  if (i < 0 && i >= -n) return i + n;
+  if (i >= n && i - n < n) return i - n;
   int r = i % n;
   return r < 0 ? r + n : r;
 }
@@ -189,6 +218,7 @@ struct CicAxis {
   int lo, hi;
   float w0, w1;
   float s0, s1;
+  float d;             // disp / cell
 };
 
 __device__ __forceinline__ float pmx_sign(float x) {
@@ -198,6 +228,7 @@ __device__ __forceinline__ float pmx_sign(float x) {
 __device__ __forceinline__ CicAxis pmx_cic_axis(int pm, float dp, float cell, int n) {
   CicAxis a;
   float d = dp / cell;
+  a.d = d;
   float fl = floorf(d);
   int i = (int)fl;
   float d20 = d - (float)i;
@@ -266,8 +297,50 @@ __device__ __forceinline__ void cic_setup(const int pm[3], const float dp[3], co
 }
 
 __device__ __forceinline__ uint32_t tile_of(const CicAxis ax[3], const TileGeom& tg) {
-  return ((uint32_t)(ax[0].lo / tg.tx) * tg.nty + (uint32_t)(ax[1].lo / tg.ty)) * tg.ntz +
-         (uint32_t)(ax[2].lo / tg.tz);
+  return ((uint32_t)pmx_tdiv(ax[0].lo, tg.mx) * tg.nty + (uint32_t)pmx_tdiv(ax[1].lo, tg.my)) *
+             tg.ntz +
+         (uint32_t)pmx_tdiv(ax[2].lo, tg.mz);
+}
+
+// Record of a binned particle: d = disp/cell per axis (float32, the value
+// enmesh divides out) and the lower CIC corner relative to the tile origin
+// packed as lx | ly << 10 | lz << 20 (bit pattern stored in .w).
+__device__ __forceinline__ float4 pmx_make_record(const CicAxis ax[3], const TileGeom& tg) {
+  const int lx = ax[0].lo - pmx_tdiv(ax[0].lo, tg.mx) * tg.tx;
+  const int ly = ax[1].lo - pmx_tdiv(ax[1].lo, tg.my) * tg.ty;
+  const int lz = ax[2].lo - pmx_tdiv(ax[2].lo, tg.mz) * tg.tz;
+  const uint32_t packed = (uint32_t)lx | ((uint32_t)ly << 10) | ((uint32_t)lz << 20);
+  return make_float4(ax[0].d, ax[1].d, ax[2].d, __uint_as_float(packed));
+}
+
+// Where a binning kernel puts particle i of tile t with rank r.  Exact mode
+// (CAP false): the tile and rank, for the fill after the scan of the counts.
+// Capacity mode: the record itself at start[t] + r when r is below the
+// tile's capacity start[t+1] - start[t], else the overflow flag.
+struct BinOut {
+  uint32_t* ptile;
+  uint32_t* prank;
+  const uint32_t* start;
+  float4* rec;
+  uint32_t* pidx;
+  uint32_t* flag;
+};
+
+template <bool CAP>
+__device__ __forceinline__ void pmx_bin_emit(const BinOut& o, long long i, uint32_t t, uint32_t r,
+                                             const CicAxis ax[3], const TileGeom& tg) {
+  if (!CAP) {
+    o.ptile[i] = t;
+    o.prank[i] = r;
+    return;
+  }
+  const uint32_t b = o.start[t], e = o.start[t + 1];
+  if (r < e - b) {
+    o.rec[b + r] = pmx_make_record(ax, tg);
+    o.pidx[b + r] = (uint32_t)i;
+  } else {
+    o.flag[0] = 1u;
+  }
 }
 
 // Wave-aggregated atomicAdd(&ctr[key], 1): returns this lane's slot.
@@ -281,8 +354,8 @@ __device__ __forceinline__ uint32_t agg_slot(uint32_t* __restrict__ ctr, uint32_
   int my_leader = lane;
   uint32_t my_rank = 0, group = 0;
   while (pending) {
-    const int leader = __ffsll((long long)pending) - 1;
-    const uint32_t lkey = __shfl(key, leader);
+    const int leader = __ffsll((long long)pending) - 1;  // wave-uniform
+    const uint32_t lkey = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
     const unsigned long long peers = __ballot(active && key == lkey) & pending;
     if (active && key == lkey) {
       my_leader = leader;
@@ -448,13 +521,24 @@ __device__ __forceinline__ void pmx_unpack_local(float w, int& lx, int& ly, int&
 }
 
 // ------------------------------------------------- shared host drivers --
-int pmx_fill_u32(pmx_ctx* c, uint32_t* p, long long n, uint32_t v);
-int pmx_scan_tiles(pmx_ctx* c);
+// with cond, the fill, scan and binning kernels do nothing unless *cond != 0
+int pmx_fill_u32(pmx_ctx* c, uint32_t* p, long long n, uint32_t v, const uint32_t* cond = nullptr);
+int pmx_scan_tiles(pmx_ctx* c, const uint32_t* cond = nullptr);
 int pmx_bin_fill(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
-                 const float* pval, int vstride, int nch);
+                 const uint32_t* cond = nullptr);
 int pmx_bin_particles(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n);
-int pmx_deposit_binned(pmx_ctx* c, float val, int ch, long long n, float* mesh, int mrs,
-                       int accumulate);
+// A binning pass in three parts: pmx_bin_begin zeroes the counts and, when
+// the counts of the last pass can size the tiles, sets up the capacity
+// layout (returns *cap = 1); the caller's binning kernel then counts the
+// particles and emits them (pmx_bin_out gives its BinOut); pmx_bin_end
+// finishes the exact path, or re-bins exactly on the device when a tile
+// overflowed, and records the counts for the next pass.
+int pmx_bin_begin(pmx_ctx* c, long long n, int* cap);
+BinOut pmx_bin_out(pmx_ctx* c);
+int pmx_bin_end(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n, int cap);
+int pmx_deposit_binned(pmx_ctx* c, float val, long long n, float* mesh, int mrs, int accumulate);
+int pmx_deposit_binned3(pmx_ctx* c, long long n, float* const* mesh, int mrs);
+int pmx_ensure_halo(pmx_ctx* c, int nch);
 int pmx_poisson_ctx(pmx_ctx* c, float coef);
 int pmx_poisson_adj_ctx(pmx_ctx* c, float scale_out);
 // bins the particles, deposits the density and solves for the three force
@@ -468,6 +552,8 @@ int pmx_tile_gather(pmx_ctx* c, float* acc, float* vel, float k2);
 int pmx_red_reduce(pmx_ctx* c, long long off, long long nblk, int slot);
 int pmx_red_fetch(pmx_ctx* c, int k, double* out);
 int pmx_red_finish(pmx_ctx* c, long long nblk, double* out);
+// grid of the exact re-binning launched after a capacity pass
+#define PMX_REBIN_GRID 4096u
 // blocks of a capped grid-stride launch over n items
 static inline unsigned pmx_blocks_capped(long long n, unsigned cap, int tpb = PMX_TPB) {
   unsigned b = pmx_blocks(n, tpb);
