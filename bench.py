@@ -54,18 +54,22 @@ def log(msg):
     print(f'[bench] {msg}', file=sys.stderr, flush=True)
 
 
-def kernel_bytes(N, mesh_shape, tile):
-    """HBM bytes per step of each timed kernel class: (algorithmic, by design).
+def kernel_bytes(N, mesh_shape, tile, K):
+    """HBM bytes of each timed kernel class over a K-step pmx_nbody_steps
+    call: (algorithmic, by design).
 
     Algorithmic bytes count each logical input read once and each output
     written once: pmid 6 B (int16 x3) and disp/vel/acc 12 B per particle,
-    4 B per real and 8 B per complex (r2c layout) mesh cell.  Scatter is
-    18N + 4M, gather (3 components) 30N + 12M, kick or drift 36N; fused
-    kernels count shared operands once.  An FFT pass reads its input and
-    writes its output once.  Binning scratch (tile ranks, records, halo
-    shells) is not algorithmic; the second figure adds what the kernels move
-    by design: 16 B records, 4 B particle indices, 4 B tile and rank words,
-    the +1 halo shells.
+    4 B per real and 8 B per complex (r2c layout) mesh cell.  An FFT pass
+    reads its input and writes its output once.  Per step the scatter reads
+    the positions (pmid, disp: 18N) and writes the density (4M); the fused
+    step kernel reads pmid, disp, vel and the three force meshes and writes
+    disp and vel (54N + 12M) -- gather, closing kick, next opening kick and
+    drift; acc is written once, by the last step (which writes acc, vel and
+    disp: 66N + 12M).  The first kick/drift/bin of the call reads vel, acc,
+    disp, pmid and writes vel, disp (66N).  The second figure is what the
+    kernels move by design: tile-ordered particles of 34 B (s0, s1, pm_z)
+    in place of pmid/disp/vel, tile and rank words, the +1 halo shells.
     """
     n0, n1, n2 = mesh_shape
     M = n0 * n1 * n2
@@ -75,35 +79,33 @@ def kernel_bytes(N, mesh_shape, tile):
     shell = (ty + 1) * (tz + 1) + tx * (tz + 1) + tx * ty
     faces = ty * tz + (tx - 1) * tz + (tx - 1) * (ty - 1)
     return {
-        # kick + drift (read vel, acc, disp; write vel, disp) + scatter's pmid
-        # read; by design also the tile and rank words
+        # first kick + drift + exact count: read vel, acc, disp, pmid, write
+        # vel, disp; by design also the tile and rank words
         'bin': (66 * N, 74 * N),
-        'scan': (0, 8 * ntiles),
-        # read pmid, disp, tile, rank; write record + particle index
-        'fill': (0, 46 * N),
-        # write the density mesh; by design read records, write halo shells
-        'deposit': (4 * M, 16 * N + 4 * M + 4 * shell * ntiles),
-        'fold': (0, (4 * shell + 8 * faces) * ntiles),
-        'z_r2c': (4 * M + 8 * Mh,) * 2,
-        'y_fwd': (16 * Mh,) * 2,
+        'scan': (0, 8 * ntiles * (2 * K - 1)),
+        # first fill: read pmid, disp, vel, tile, rank; write 34 B per particle
+        'fill': (0, 72 * N),
+        'deposit': (K * (18 * N + 4 * M), K * (18 * N + 4 * M + 4 * shell * ntiles)),
+        'fold': (0, K * (4 * shell + 8 * faces) * ntiles),
+        'z_r2c': (K * (4 * M + 8 * Mh),) * 2,
+        'y_fwd': (K * 16 * Mh,) * 2,
         # read one complex mesh, write three
-        'poisson_x': (32 * Mh,) * 2,
-        'y_inv': (48 * Mh,) * 2,
+        'poisson_x': (K * 32 * Mh,) * 2,
+        'y_inv': (K * 48 * Mh,) * 2,
         # three launches, each reads a complex and writes a real mesh
-        'z_c2r': (3 * (8 * Mh + 4 * M),) * 2,
-        # gather 30N + 12M and the closing kick (read and write vel);
-        # by design records + indices replace pmid + disp
-        'gather': (54 * N + 12 * M, 56 * N + 12 * M),
+        'z_c2r': (K * 3 * (8 * Mh + 4 * M),) * 2,
+        'gather': ((K - 1) * (54 * N + 12 * M) + 66 * N + 12 * M,
+                   (K - 1) * (68 * N + 12 * M) + 70 * N + 12 * M),
     }
 
 
 GROUPS = {
-    # kick+drift fused with the scatter (disp read once): 66N + 4M
-    'kick_drift_scatter': (('bin', 'scan', 'fill', 'deposit', 'fold'), (66, 4, 0)),
-    # 3D r2c 4M + 8Mh plus Poisson (Green, -ik, 3 c2r) 8Mh + 12M
-    'fft_poisson': (('z_r2c', 'y_fwd', 'poisson_x', 'y_inv', 'z_c2r'), (0, 16, 16)),
-    # 3-component gather fused with the closing kick: 54N + 12M
-    'gather_kick': (('gather',), (54, 12, 0)),
+    # first kick/drift/bin (once per call) and the scatter of every step
+    'scatter': ('bin', 'scan', 'fill', 'deposit', 'fold'),
+    # 3D r2c plus Poisson (Green, -ik, 3 c2r)
+    'fft_poisson': ('z_r2c', 'y_fwd', 'poisson_x', 'y_inv', 'z_c2r'),
+    # gather, closing kick, next opening kick and drift, move to the new tile
+    'step_gather_kick_drift': ('gather',),
 }
 
 
@@ -371,14 +373,17 @@ def main():
                       nb._f32(nb.drift_factor(ah, a0, a1, cosmo, conf)),
                       nb._f32(nb.kick_factor(a1, ah, a1, cosmo, conf))))
 
-    def step(i):
-        k1, d, k2 = sched[i % nsteps_sched]
-        _lib.check(L.pmx_nbody_step(h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel),
-                                    _lib.ptr(acc), N, val, coef, k1, d, k2))
+    def steps(i0, k):
+        """k KDK steps of the schedule from step i0 in one pmx_nbody_steps call"""
+        f = np.array([sched[i % nsteps_sched] for i in range(i0, i0 + k)], dtype=np.float32)
+        k1, d, k2 = (np.ascontiguousarray(f[:, m]) for m in range(3))
+        P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+        _lib.check(L.pmx_nbody_steps(h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel),
+                                     _lib.ptr(acc), N, val, coef, k, P(k1), P(d), P(k2)))
 
     _lib.check(L.pmx_gravity(h, _lib.ptr(pmid), _lib.ptr(disp), N, val, coef, _lib.ptr(acc)))
-    for i in range(args.warmup):
-        step(i)
+    if args.warmup:
+        steps(0, args.warmup)
     torch.cuda.synchronize()
 
     log(f'timed forward: {args.steps} steps')
@@ -386,8 +391,7 @@ def main():
     _lib.timing_read(ctx)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(args.warmup, args.warmup + args.steps):
-        step(i)
+    steps(args.warmup, args.steps)
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
     timing = _lib.timing_read(ctx)
@@ -400,7 +404,7 @@ def main():
     # per kernel class: time per step, launches, algorithmic bytes per launch
     ts = (ctypes.c_int * 3)()
     _lib.check(L.pmx_tile_shape(h, ctypes.byref(ts)))
-    kb = kernel_bytes(N, conf.mesh_shape, tuple(ts))
+    kb = kernel_bytes(N, conf.mesh_shape, tuple(ts), K)
     kernels = {}
     for k, (ms_tot, cnt) in timing.items():
         if not cnt:
@@ -409,21 +413,20 @@ def main():
                  ms_per_launch=ms_tot / cnt)
         if k in kb:
             alg, impl = kb[k]
-            e['alg_bytes_per_launch'] = alg * K / cnt
-            e['impl_bytes_per_launch'] = impl * K / cnt
-            e['alg_gbs'] = e['alg_bytes_per_launch'] / (e['ms_per_launch'] * 1e-3) / 1e9
-            e['impl_gbs'] = e['impl_bytes_per_launch'] / (e['ms_per_launch'] * 1e-3) / 1e9
+            e['alg_bytes_per_launch'] = alg / cnt
+            e['impl_bytes_per_launch'] = impl / cnt
+            e['alg_gbs'] = alg / (ms_tot * 1e-3) / 1e9
+            e['impl_gbs'] = impl / (ms_tot * 1e-3) / 1e9
         kernels[k] = e
     groups = {}
-    for gname, (classes, (cn, cm, cmh)) in GROUPS.items():
+    for gname, classes in GROUPS.items():
         ms = sum(kernels[c]['ms_per_step'] for c in classes if c in kernels)
-        b = cn * N + cm * M + cmh * Mh
+        b = sum(kb[c][0] for c in classes) / K
         groups[gname] = dict(ms=ms, alg_bytes=b, gbs=b / (ms * 1e-3) / 1e9 if ms > 0 else None)
         groups[gname]['frac'] = groups[gname]['gbs'] / HBM_PEAK_GBS if ms > 0 else None
     dom = max((k for k in kernels if k in kb and kb[k][0]), key=lambda k: kernels[k]['ms_per_step'])
-    # whole KDK step: read pmid, disp, vel, acc, write disp, vel, acc (78N),
-    # density write+read (8M), r2c + 3 c2r complex traffic (16Mh + 24M)
-    B_step = 78 * N + 8 * M + 16 * Mh + 24 * M
+    # whole KDK step: the algorithmic bytes of all classes per step
+    B_step = sum(v[0] for v in kb.values()) / K
 
     traffic = None
     tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',

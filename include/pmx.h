@@ -142,6 +142,14 @@ int pmx_drift(pmx_ctx* ctx, long long n, float* disp, const float* vel, float d)
 int pmx_nbody_step(pmx_ctx* ctx, const int16_t* pmid, float* disp, float* vel,
                    float* acc, long long n, float val, float coef, float k1,
                    float d, float k2);
+/* nsteps KDK steps of pmwd/nbody.py:121-140 (the loop of nbody, :186-201)
+ * with factors k1[s], d[s], k2[s] (host arrays): bitwise the result of
+ * nsteps pmx_nbody_step calls.  Between steps the particles stay on the
+ * device in tile order (two extra copies of about 34 bytes per particle);
+ * disp, vel and acc are written once, at the end.                          */
+int pmx_nbody_steps(pmx_ctx* ctx, const int16_t* pmid, float* disp, float* vel,
+                    float* acc, long long n, float val, float coef, int nsteps,
+                    const float* k1, const float* d, const float* k2);
 
 /* ------------------------------------------------------------ adjoint ---
  * force_adj of pmwd/nbody.py:108-118: acc = gravity(disp) and

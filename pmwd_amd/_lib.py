@@ -56,6 +56,7 @@ _SIGS = {
     'pmx_kick': [c_p, c_ll, c_p, c_p, c_f],
     'pmx_drift': [c_p, c_ll, c_p, c_p, c_f],
     'pmx_nbody_step': [c_p, c_p, c_p, c_p, c_p, c_ll, c_f, c_f, c_f, c_f, c_f],
+    'pmx_nbody_steps': [c_p, c_p, c_p, c_p, c_p, c_ll, c_f, c_f, c_int, c_p, c_p, c_p],
     'pmx_gravity_vjp': [c_p, c_p, c_p, c_ll, c_f, c_f, c_p, c_p, c_p, ctypes.POINTER(c_d)],
     'pmx_kick_adj': [c_p, c_ll, c_p, c_p, c_p, c_p, c_p, c_f, ctypes.POINTER(c_d)],
     'pmx_drift_adj': [c_p, c_ll, c_p, c_p, c_p, c_p, c_f, ctypes.POINTER(c_d)],

@@ -178,12 +178,15 @@ struct DepOut {
 // each thread streams DEP_U coalesced records per round.  The interior is
 // stored with plain stores, the +1 shell to the tile's halo slot.  With
 // nch > 1 the channels run one after the other in the same workgroup, the
-// records of the tile coming back from L2.
+// records of the tile coming back from L2.  STATE reads tile-ordered
+// particles (rec = their s0, pz their pm_z) and forms d and the corner
+// itself.
 #define DEP_TPB 512
 #define DEP_U 4
-template <int PERVAL>
+template <int PERVAL, int STATE = 0>
 __global__ void __launch_bounds__(DEP_TPB)
-    deposit_k(const float4* __restrict__ rec, const float* __restrict__ vrec, long long vstride,
+    deposit_k(const float4* __restrict__ rec, const uint16_t* __restrict__ pz,
+              const float* __restrict__ vrec, long long vstride,
               const float* __restrict__ vmaxp, const uint32_t* __restrict__ tile_start,
               const uint32_t* __restrict__ tile_count, MeshGeom g, TileGeom tg, float val,
               DepOut out, int nch, int mrs, int accumulate, float* __restrict__ halo,
@@ -205,11 +208,13 @@ __global__ void __launch_bounds__(DEP_TPB)
     for (uint32_t q0 = 0; q0 < cnt; q0 += DEP_TPB * DEP_U) {
       float4 r[DEP_U];
       float v[DEP_U];
+      uint16_t zz[DEP_U];
 #pragma unroll
       for (int u = 0; u < DEP_U; ++u) {
         uint32_t q = q0 + u * DEP_TPB + threadIdx.x;
         if (q < cnt) {
           r[u] = rec[start + q];
+          if (STATE) zz[u] = pz[start + q];
           v[u] = PERVAL ? vr[start + q] : val;
         }
       }
@@ -218,9 +223,21 @@ __global__ void __launch_bounds__(DEP_TPB)
         uint32_t q = q0 + u * DEP_TPB + threadIdx.x;
         if (q >= cnt) continue;
         int lx, ly, lz;
-        pmx_unpack_local(r[u].w, lx, ly, lz);
-        // binned by this tile, so always inside; the guard keeps LDS in bounds
-        if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
+        if (STATE) {
+          float d[3];
+          int l[3];
+          // binned by this tile, so always inside; the guard keeps LDS in bounds
+          if (!pmx_state_local(r[u], zz[u], g, o, s, d, l)) continue;
+          r[u].x = d[0];
+          r[u].y = d[1];
+          r[u].z = d[2];
+          lx = l[0];
+          ly = l[1];
+          lz = l[2];
+        } else {
+          pmx_unpack_local(r[u].w, lx, ly, lz);
+          if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
+        }
         float a0, a1, b0, b1, c0, c1;
         frac_of(r[u].x, a0, a1);
         frac_of(r[u].y, b0, b1);
@@ -326,24 +343,8 @@ __global__ void __launch_bounds__(TG_TPB)
       int lx, ly, lz;
       pmx_unpack_local(r[u].w, lx, ly, lz);
       if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
-      float a0, a1, b0, b1, c0, c1;
-      frac_of(r[u].x, a0, a1);
-      frac_of(r[u].y, b0, b1);
-      frac_of(r[u].z, c0, c1);
-      const int b = (lx * LY + ly) * LZ + lz;
-      float v0[8], v1[8], v2[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        float f0 = (j & 1) ? a1 : a0;
-        float f1 = (j & 2) ? b1 : b0;
-        float f2 = (j & 4) ? c1 : c0;
-        float w = (f0 * f1) * f2;
-        int li = b + ((j & 1) ? LY * LZ : 0) + ((j & 2) ? LZ : 0) + ((j & 4) ? 1 : 0);
-        v0[j] = sh[li] * w;
-        v1[j] = sh[ltot + li] * w;
-        v2[j] = sh[2 * ltot + li] * w;
-      }
-      const float o3[3] = {pmx_tree8(v0), pmx_tree8(v1), pmx_tree8(v2)};
+      float o3[3];
+      pmx_tile_acc(sh, ltot, LY, LZ, r[u].x, r[u].y, r[u].z, lx, ly, lz, o3);
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         acc[3 * p + a] = o3[a];
@@ -515,13 +516,13 @@ static int deposit_run(pmx_ctx* c, float val, int perval, int nch, long long n, 
   if ((rc = pmx_ensure_halo(c, nch))) return rc;
   pmx_tic(c, PMX_T_DEPOSIT);
   if (perval)
-    deposit_k<1><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, c->vrec, c->rec_cap, c->vmax,
+    deposit_k<1><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, nullptr, c->vrec, c->rec_cap, c->vmax,
                                                   c->tile_start, c->tile_count, g, tg, val, out,
                                                   nch, mrs, accumulate, c->halo, hstride);
   else
-    deposit_k<0><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, nullptr, 0, nullptr, c->tile_start,
-                                                  c->tile_count, g, tg, val, out, nch, mrs,
-                                                  accumulate, c->halo, hstride);
+    deposit_k<0><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, nullptr, nullptr, 0, nullptr,
+                                                  c->tile_start, c->tile_count, g, tg, val, out,
+                                                  nch, mrs, accumulate, c->halo, hstride);
   pmx_toc(c, PMX_T_DEPOSIT);
   PMX_LAUNCH_CHECK();
   pmx_tic(c, PMX_T_FOLD);
@@ -543,6 +544,28 @@ int pmx_deposit_binned(pmx_ctx* c, float val, long long n, float* mesh, int mrs,
 int pmx_deposit_binned3(pmx_ctx* c, long long n, float* const* mesh, int mrs) {
   DepOut out = {{mesh[0], mesh[1], mesh[2]}};
   return deposit_run(c, 0.f, 1, 3, n, out, mrs, 0);
+}
+
+// Deposits val per tile-ordered particle (st, tile ranges start/count) into
+// mesh (row stride mrs), overwriting it.
+int pmx_deposit_state(pmx_ctx* c, const PState& st, const uint32_t* start, const uint32_t* count,
+                      float val, float* mesh, int mrs) {
+  const MeshGeom g = pmx_geom_padded(c);
+  const TileGeom& tg = c->tg;
+  const size_t lds = (size_t)(tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(unsigned long long);
+  const unsigned nt = (unsigned)c->ntiles;
+  const long long hstride = c->ntiles * tg.shell;
+  DepOut out = {{mesh, mesh, mesh}};
+  pmx_tic(c, PMX_T_DEPOSIT);
+  deposit_k<0, 1><<<nt, DEP_TPB, lds, c->stream>>>(st.s0, st.pz, nullptr, 0, nullptr, start, count,
+                                                   g, tg, val, out, 1, mrs, 0, c->halo, hstride);
+  pmx_toc(c, PMX_T_DEPOSIT);
+  PMX_LAUNCH_CHECK();
+  pmx_tic(c, PMX_T_FOLD);
+  fold_k<<<nt, PMX_TPB, 0, c->stream>>>(out, 1, mrs, c->halo, hstride, g, tg);
+  pmx_toc(c, PMX_T_FOLD);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
 }
 
 extern "C" int pmx_scatter_mass(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
@@ -663,6 +686,21 @@ __global__ void __launch_bounds__(PMX_TPB)
   pmx_stage_out3<1, PMX_TPB>({{disp}}, sh + 2 * T3, b0, cnt);
 }
 
+int pmx_kick_drift_bin(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, const float* acc,
+                       long long n, float k1, float d, uint32_t* count, const BinOut& o, int cap) {
+  const MeshGeom g = pmx_geom_padded(c);
+  pmx_tic(c, PMX_T_BIN);
+  if (cap)
+    kick_drift_bin_k<true><<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(pmid, disp, vel, acc, n, k1, d,
+                                                                     g, c->tg, count, o);
+  else
+    kick_drift_bin_k<false><<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(pmid, disp, vel, acc, n, k1,
+                                                                      d, g, c->tg, count, o);
+  pmx_toc(c, PMX_T_BIN);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
 extern "C" int pmx_nbody_step(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, float* acc,
                               long long n, float val, float coef, float k1, float d, float k2) {
   PMX_REQUIRE(c && n >= 0, "pmx_nbody_step: bad argument");
@@ -670,19 +708,9 @@ extern "C" int pmx_nbody_step(pmx_ctx* c, const int16_t* pmid, float* disp, floa
   PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_nbody_step: too many particles");
   int rc, cap;
   if ((rc = pmx_bin_begin(c, n, &cap))) return rc;
-  MeshGeom g = pmx_geom_padded(c);
-  if (n > 0) {
-    const BinOut o = pmx_bin_out(c);
-    pmx_tic(c, PMX_T_BIN);
-    if (cap)
-      kick_drift_bin_k<true><<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(
-          pmid, disp, vel, acc, n, k1, d, g, c->tg, c->tile_count, o);
-    else
-      kick_drift_bin_k<false><<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(
-          pmid, disp, vel, acc, n, k1, d, g, c->tg, c->tile_count, o);
-    pmx_toc(c, PMX_T_BIN);
-    PMX_LAUNCH_CHECK();
-  }
+  if (n > 0 && (rc = pmx_kick_drift_bin(c, pmid, disp, vel, acc, n, k1, d, c->tile_count,
+                                         pmx_bin_out(c), cap)))
+    return rc;
   if ((rc = pmx_bin_end(c, pmid, disp, n, cap))) return rc;
   if ((rc = pmx_deposit_binned(c, val, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
   if ((rc = pmx_poisson_ctx(c, coef))) return rc;

@@ -180,6 +180,13 @@ extern "C" int pmx_ctx_destroy(pmx_ctx* c) {
                 c->scan_tmp};
   for (void* p : ps)
     if (p) hipFree(p);
+  for (int b = 0; b < 2; ++b) {
+    void* qs[] = {c->pl[b].s0, c->pl[b].s1, c->pl[b].pz, c->pl_count[b], c->pl_start[b]};
+    for (void* p : qs)
+      if (p) hipFree(p);
+  }
+  if (c->pl_tile) hipFree(c->pl_tile);
+  if (c->pl_rank) hipFree(c->pl_rank);
   if (c->red_host) hipHostFree(c->red_host);
   free(c);
   return PMX_OK;
@@ -227,6 +234,34 @@ int pmx_ensure_halo(pmx_ctx* c, int nch) {
   c->halo_nch = 0;
   if (dev_alloc(c, (void**)&c->halo, (size_t)nch * c->ntiles * c->tg.shell * 4)) return PMX_ENOMEM;
   c->halo_nch = nch;
+  return PMX_OK;
+}
+
+// the pipeline's tile-ordered particle copies (rec_cap slots each)
+int pmx_ensure_state(pmx_ctx* c) {
+  const long long cap = c->rec_cap;
+  if (cap <= c->pl_cap) return PMX_OK;
+  const long long old = c->pl_cap;
+  for (int b = 0; b < 2; ++b) {
+    dev_free(c, (void**)&c->pl[b].s0, old * 16);
+    dev_free(c, (void**)&c->pl[b].s1, old * 16);
+    dev_free(c, (void**)&c->pl[b].pz, old * 2);
+  }
+  dev_free(c, (void**)&c->pl_tile, old * 4);
+  dev_free(c, (void**)&c->pl_rank, old * 4);
+  c->pl_cap = 0;
+  int rc = PMX_OK;
+  for (int b = 0; b < 2; ++b) {
+    rc |= dev_alloc(c, (void**)&c->pl[b].s0, cap * 16);
+    rc |= dev_alloc(c, (void**)&c->pl[b].s1, cap * 16);
+    rc |= dev_alloc(c, (void**)&c->pl[b].pz, cap * 2);
+    if (!c->pl_count[b]) rc |= dev_alloc(c, (void**)&c->pl_count[b], (size_t)(c->ntiles + 1) * 4);
+    if (!c->pl_start[b]) rc |= dev_alloc(c, (void**)&c->pl_start[b], (size_t)(c->ntiles + 1) * 4);
+  }
+  rc |= dev_alloc(c, (void**)&c->pl_tile, cap * 4);
+  rc |= dev_alloc(c, (void**)&c->pl_rank, cap * 4);
+  if (rc) return PMX_ENOMEM;
+  c->pl_cap = cap;
   return PMX_OK;
 }
 

@@ -71,6 +71,16 @@ __device__ __forceinline__ int pmx_tdiv(int lo, unsigned m) {
   return (int)(__umul24((unsigned)lo, m) >> 20);
 }
 
+// A particle held in tile order by the multi-step pipeline: s0 = (disp,
+// pm_x | pm_y << 16), s1 = (vel, index in the caller's arrays), pz = pm_z,
+// pm taken mod the mesh extent per axis (the CIC corner wraps, so only pm
+// mod n enters).
+struct PState {
+  float4* s0;
+  float4* s1;
+  uint16_t* pz;
+};
+
 // ---------------------------------------------------------- kernel timing --
 // Optional per-kernel-class timing with HIP events on the context stream.
 enum {
@@ -135,6 +145,14 @@ struct pmx_ctx {
   int caps_valid;      // host: tile_count holds the exact counts of caps_n particles
   long long caps_n;
   long long rec_cap;   // record slots of rec/pidx (and the vrec channel stride)
+  // multi-step pipeline: two tile-ordered particle copies of rec_cap slots,
+  // their tile counts and starts, and per-slot tile/rank of an exact re-bin
+  PState pl[2];
+  uint32_t* pl_count[2];
+  uint32_t* pl_start[2];
+  uint32_t* pl_tile;
+  uint32_t* pl_rank;
+  long long pl_cap;    // slots the pipeline buffers hold
   float* halo;         // nch shells per tile, channel-major
   int halo_nch;        // channels the halo buffer holds
   long long ptcl_cap;
@@ -512,6 +530,63 @@ __device__ __forceinline__ void pmx_stage_tile(float* sh, const int o[3], int LY
   }
 }
 
+// acc = sum_j F[c_j] w_j over the eight corners of a binned particle (d =
+// disp/cell per axis, lower corner (lx, ly, lz) in the staged tile of three
+// channels ltot floats apart, rows LY x LZ), summed in the tree order of
+// pmx_tree8 -- the arithmetic of a per-particle gather.
+__device__ __forceinline__ void pmx_tile_acc(const float* sh, int ltot, int LY, int LZ,
+                                             float dx, float dy, float dz, int lx, int ly, int lz,
+                                             float out[3]) {
+  float a0, a1, b0, b1, c0, c1;
+  frac_of(dx, a0, a1);
+  frac_of(dy, b0, b1);
+  frac_of(dz, c0, c1);
+  const int b = (lx * LY + ly) * LZ + lz;
+  float v0[8], v1[8], v2[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    float f0 = (j & 1) ? a1 : a0;
+    float f1 = (j & 2) ? b1 : b0;
+    float f2 = (j & 4) ? c1 : c0;
+    float w = (f0 * f1) * f2;
+    int li = b + ((j & 1) ? LY * LZ : 0) + ((j & 2) ? LZ : 0) + ((j & 4) ? 1 : 0);
+    v0[j] = sh[li] * w;
+    v1[j] = sh[ltot + li] * w;
+    v2[j] = sh[2 * ltot + li] * w;
+  }
+  out[0] = pmx_tree8(v0);
+  out[1] = pmx_tree8(v1);
+  out[2] = pmx_tree8(v2);
+}
+
+// ------------------------------------------------ tile-ordered particles --
+__device__ __forceinline__ void pmx_state_pm(float4 s0, uint16_t pz, int pm[3]) {
+  const uint32_t b = __float_as_uint(s0.w);
+  pm[0] = (int)(b & 0xffffu);
+  pm[1] = (int)(b >> 16);
+  pm[2] = (int)pz;
+}
+
+// d = disp/cell and the lower corner relative to tile origin o of a state
+// entry (the values pmx_make_record packs); false when the corner is not in
+// the tile of extent s.
+__device__ __forceinline__ bool pmx_state_local(float4 s0, uint16_t pz, const MeshGeom& g,
+                                                const int o[3], const int s[3], float d[3],
+                                                int l[3]) {
+  int pm[3];
+  pmx_state_pm(s0, pz, pm);
+  const float dp[3] = {s0.x, s0.y, s0.z};
+  CicAxis ax[3];
+  cic_setup(pm, dp, g, ax);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    d[a] = ax[a].d;
+    l[a] = ax[a].lo - o[a];
+  }
+  return (unsigned)l[0] < (unsigned)s[0] && (unsigned)l[1] < (unsigned)s[1] &&
+         (unsigned)l[2] < (unsigned)s[2];
+}
+
 // Unpacks a binned record's tile-local lower corner (lx | ly << 10 | lz << 20).
 __device__ __forceinline__ void pmx_unpack_local(float w, int& lx, int& ly, int& lz) {
   uint32_t pk = __float_as_uint(w);
@@ -537,6 +612,13 @@ int pmx_bin_begin(pmx_ctx* c, long long n, int* cap);
 BinOut pmx_bin_out(pmx_ctx* c);
 int pmx_bin_end(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n, int cap);
 int pmx_deposit_binned(pmx_ctx* c, float val, long long n, float* mesh, int mrs, int accumulate);
+int pmx_deposit_state(pmx_ctx* c, const PState& st, const uint32_t* start, const uint32_t* count,
+                      float val, float* mesh, int mrs);
+// kick (k1) and drift (d) of the caller's arrays, binned into count/o (cap:
+// capacity mode)
+int pmx_kick_drift_bin(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, const float* acc,
+                       long long n, float k1, float d, uint32_t* count, const BinOut& o, int cap);
+int pmx_ensure_state(pmx_ctx* c);
 int pmx_deposit_binned3(pmx_ctx* c, long long n, float* const* mesh, int mrs);
 int pmx_ensure_halo(pmx_ctx* c, int nch);
 int pmx_poisson_ctx(pmx_ctx* c, float coef);

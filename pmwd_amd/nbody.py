@@ -130,9 +130,23 @@ def _store(t, where):
     return out
 
 
+def _kdk_factors(a_prev, a_next, cosmo, conf):
+    """float32 k1, d, k2 of one KDK step (nbody.py:205-212)."""
+    a_half = a_prev * (1 - 0.5) + a_next * 0.5
+    return (_f32(kick_factor(a_prev, a_prev, a_half, cosmo, conf)),
+            _f32(drift_factor(a_half, a_prev, a_next, cosmo, conf)),
+            _f32(kick_factor(a_next, a_half, a_next, cosmo, conf)))
+
+
 def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=None,
-                 checkpoint_device=None):
-    """Integrates in place; returns acc and the checkpoints {step: (disp, vel)}."""
+                 checkpoint_device=None, pipeline=True):
+    """Integrates in place; returns acc and the checkpoints {step: (disp, vel)}.
+
+    KDK steps between checkpoints run as one ``pmx_nbody_steps`` call (the
+    particles kept in tile order on the device between steps), or one
+    ``pmx_nbody_step`` call per step when ``pipeline`` is False; both give
+    the same bits.
+    """
     L = _lib.lib()
     n = pmid.shape[0]
     c = _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
@@ -142,27 +156,42 @@ def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=N
     a_nbody = conf.a_nbody[::-1] if reverse else conf.a_nbody
     _force(c, pmid, disp, acc, n, val, coef)
     ckpt = {}
-    kdk = _is_kdk(conf)
-    for i, (a_prev, a_next) in enumerate(zip(a_nbody[:-1], a_nbody[1:])):
+    steps = list(zip(a_nbody[:-1], a_nbody[1:]))
+    if _is_kdk(conf):
+        i = 0
+        while i < len(steps):
+            if checkpoint_every and i % checkpoint_every == 0 and i > 0:
+                ckpt[i] = (_store(disp, checkpoint_device), _store(vel, checkpoint_device))
+            j = min(len(steps), (i // checkpoint_every + 1) * checkpoint_every
+                    if checkpoint_every else len(steps))
+            if not pipeline:
+                j = i + 1
+            f = np.array([_kdk_factors(ap, an, cosmo, conf) for ap, an in steps[i:j]],
+                         dtype=np.float32).reshape(-1, 3)
+            k1, d, k2 = (np.ascontiguousarray(f[:, m]) for m in range(3))
+            if j - i == 1:
+                _lib.check(L.pmx_nbody_step(h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel),
+                                            _lib.ptr(acc), n, val, coef, float(k1[0]),
+                                            float(d[0]), float(k2[0])))
+            else:
+                P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+                _lib.check(L.pmx_nbody_steps(h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel),
+                                             _lib.ptr(acc), n, val, coef, j - i, P(k1), P(d),
+                                             P(k2)))
+            i = j
+        return acc, ckpt
+    for i, (a_prev, a_next) in enumerate(steps):
         if checkpoint_every and i % checkpoint_every == 0 and i > 0:
             ckpt[i] = (_store(disp, checkpoint_device), _store(vel, checkpoint_device))
-        if kdk:
-            a_half = a_prev * (1 - 0.5) + a_next * 0.5
-            k1 = _f32(kick_factor(a_prev, a_prev, a_half, cosmo, conf))
-            d = _f32(drift_factor(a_half, a_prev, a_next, cosmo, conf))
-            k2 = _f32(kick_factor(a_next, a_half, a_next, cosmo, conf))
-            _lib.check(L.pmx_nbody_step(h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel),
-                                        _lib.ptr(acc), n, val, coef, k1, d, k2))
-        else:
-            for op in step_ops(a_prev, a_next, conf):
-                if op[0] == 'drift':
-                    f = _f32(drift_factor(*op[1:], cosmo, conf))
-                    _lib.check(L.pmx_drift(h, n, _lib.ptr(disp), _lib.ptr(vel), f))
-                elif op[0] == 'kick':
-                    f = _f32(kick_factor(*op[1:], cosmo, conf))
-                    _lib.check(L.pmx_kick(h, n, _lib.ptr(vel), _lib.ptr(acc), f))
-                else:
-                    _force(c, pmid, disp, acc, n, val, coef)
+        for op in step_ops(a_prev, a_next, conf):
+            if op[0] == 'drift':
+                f = _f32(drift_factor(*op[1:], cosmo, conf))
+                _lib.check(L.pmx_drift(h, n, _lib.ptr(disp), _lib.ptr(vel), f))
+            elif op[0] == 'kick':
+                f = _f32(kick_factor(*op[1:], cosmo, conf))
+                _lib.check(L.pmx_kick(h, n, _lib.ptr(vel), _lib.ptr(acc), f))
+            else:
+                _force(c, pmid, disp, acc, n, val, coef)
     return acc, ckpt
 
 

@@ -1,0 +1,130 @@
+"""pmx_nbody_steps: several KDK steps with the particles kept in tile order
+on the device must give, bit for bit, what as many pmx_nbody_step calls
+give -- including steps where tiles overflow their capacity and the device
+redoes the step exactly."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import pm as opm
+
+pytestmark = pytest.mark.gpu
+
+
+def particles(shape, ratio, cell, seed, std=0.7):
+    pmid, disp = opm.gen_grid(shape, tuple(ratio * s for s in shape), cell)
+    rng = np.random.default_rng(seed)
+    return pmid, (disp + std * cell * rng.standard_normal(disp.shape)).astype(np.float32)
+
+
+def factors(k, seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    f = (scale * (0.2 + 0.3 * rng.random((k, 3)))).astype(np.float32)
+    return [np.ascontiguousarray(f[:, m]) for m in range(3)]
+
+
+def run(conf, dev, pmid, disp, vel, k1, d, k2, multi):
+    from pmwd_amd import _lib
+    L = _lib.lib()
+    ctx = _lib.Context(conf.mesh_shape, conf.cell_size, dev)
+    h = ctx.bind()
+    P = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev).clone()
+    pm, x, v = P(pmid), P(disp), P(vel)
+    acc = torch.zeros_like(x)
+    val, coef = 1.0, 0.45
+    _lib.check(L.pmx_gravity(h, _lib.ptr(pm), _lib.ptr(x), len(pmid), val, coef, _lib.ptr(acc)))
+    if multi:
+        H = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        _lib.check(L.pmx_nbody_steps(h, _lib.ptr(pm), _lib.ptr(x), _lib.ptr(v), _lib.ptr(acc),
+                                     len(pmid), val, coef, len(k1), H(k1), H(d), H(k2)))
+    else:
+        for s in range(len(k1)):
+            _lib.check(L.pmx_nbody_step(h, _lib.ptr(pm), _lib.ptr(x), _lib.ptr(v), _lib.ptr(acc),
+                                        len(pmid), val, coef, float(k1[s]), float(d[s]),
+                                        float(k2[s])))
+    torch.cuda.synchronize()
+    return x.cpu().numpy(), v.cpu().numpy(), acc.cpu().numpy()
+
+
+def assert_same(a, b):
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize('shape,ratio,k', [((32, 32, 32), 1, 5), ((20, 24, 12), 2, 4),
+                                           ((7, 8, 6), 1, 3), ((16, 16, 16), 2, 1),
+                                           ((12, 10, 8), 1, 2)])
+def test_steps_match_single_steps(dev, shape, ratio, k):
+    import pmwd_amd as pa
+    conf = pa.Configuration(1.0, shape, mesh_shape=ratio)
+    pmid, disp = particles(shape, ratio, conf.cell_size, 1)
+    vel = (0.5 * np.random.default_rng(2).standard_normal(disp.shape)).astype(np.float32)
+    k1, d, k2 = factors(k, 3)
+    got = run(conf, dev, pmid, disp, vel, k1, d, k2, True)
+    want = run(conf, dev, pmid, disp, vel, k1, d, k2, False)
+    assert_same(got, want)
+
+
+def test_steps_collapse_overflows_capacity(dev):
+    """Particles streaming to the box centre: tiles near it outgrow the
+    capacities sized from the step before, which forces the exact redo."""
+    import pmwd_amd as pa
+    shape, ratio = (32, 32, 32), 1
+    conf = pa.Configuration(1.0, shape, mesh_shape=ratio)
+    pmid, disp = particles(shape, ratio, conf.cell_size, 4, std=0.2)
+    q = pmid.astype(np.float64) * conf.cell_size + disp
+    centre = 0.5 * np.array(shape) * conf.cell_size
+    vel = (0.3 * (centre - q)).astype(np.float32)
+    k1 = np.zeros(6, np.float32)
+    k2 = np.zeros(6, np.float32)
+    d = np.full(6, 0.5, np.float32)
+    got = run(conf, dev, pmid, disp, vel, k1, d, k2, True)
+    want = run(conf, dev, pmid, disp, vel, k1, d, k2, False)
+    assert_same(got, want)
+    # the run did collapse: most particles end within a few cells of the centre
+    x = pmid.astype(np.float64) * conf.cell_size + got[0]
+    r = np.abs(((x - centre) + 16) % 32 - 16).max(axis=1)
+    assert np.mean(r < 6) > 0.5
+
+
+def test_steps_large_moves(dev):
+    """Several cells per step in random directions: most particles change
+    tile every step."""
+    import pmwd_amd as pa
+    shape, ratio = (24, 24, 24), 2
+    conf = pa.Configuration(1.0, shape, mesh_shape=ratio)
+    pmid, disp = particles(shape, ratio, conf.cell_size, 5)
+    vel = (8.0 * np.random.default_rng(6).standard_normal(disp.shape)).astype(np.float32)
+    k1, d, k2 = factors(5, 7)
+    assert_same(run(conf, dev, pmid, disp, vel, k1, d, k2, True),
+                run(conf, dev, pmid, disp, vel, k1, d, k2, False))
+
+
+def test_nbody_driver_pipeline_matches_per_step(dev):
+    """pmwd_amd.nbody's forward loop (with checkpoints between segments)."""
+    import importlib
+    import pmwd_amd as pa
+    nbm = importlib.import_module('pmwd_amd.nbody')
+    conf = pa.Configuration(4.0, (16, 16, 16), mesh_shape=2, a_start=1 / 8,
+                            a_nbody_maxstep=1 / 16)
+    cosmo = pa.boltzmann(pa.SimpleLCDM(conf), conf)
+    pmid, disp = particles((16, 16, 16), 2, conf.cell_size, 8, std=1.0)
+    vel = (0.1 * np.random.default_rng(9).standard_normal(disp.shape)).astype(np.float32)
+    P = lambda a: torch.as_tensor(a, device=dev).clone()
+    outs = []
+    for pipeline in (True, False):
+        for every in (None, 3):
+            d, v = P(disp), P(vel)
+            acc, ckpt = nbm._run_forward(P(pmid), d, v, cosmo, conf, checkpoint_every=every,
+                                         pipeline=pipeline)
+            outs.append((d.cpu().numpy(), v.cpu().numpy(), acc.cpu().numpy(),
+                         {i: (a.cpu().numpy(), b.cpu().numpy()) for i, (a, b) in ckpt.items()}))
+    ref = outs[3]
+    for o in outs[:3]:
+        assert_same(o[:3], ref[:3])
+    for o in (outs[1],):
+        assert o[3].keys() == ref[3].keys()
+        for i in ref[3]:
+            assert_same(o[3][i], ref[3][i])
