@@ -37,7 +37,7 @@ def parse():
     p.add_argument('--full-adjoint', type=int, default=0, metavar='G',
                    help='instead of the default run: forward plus gradient of a density MSE '
                         'for G^3 particles on a G^3 mesh (box 2000*G/1024 Mpc/h, 31 steps, '
-                        'reversal checkpoints every 8 steps), with peak HBM')
+                        'reverse-time adjoint), with device memory per phase')
     p.add_argument('--serve-requests', type=int, default=40)
     p.add_argument('--serve-rate', type=float, default=2.0, help='Poisson arrivals per second')
     p.add_argument('--serve-slots', type=int, default=8)
@@ -243,13 +243,12 @@ def full_adjoint(G, dev):
     configuration at G=1024): white modes (seed 0) -> linear_modes -> 2LPT ->
     31 KDK steps -> density; loss = mean((rho - target)^2) against a seeded
     target density; gradients with respect to the modes, Omega_m and sigma8
-    through the reverse-time adjoint with checkpoints every 8 steps.  The
-    modes -> 2LPT part is recomputed in the backward (torch checkpoint) rather
-    than kept.  Reports wall times and peak device bytes (torch allocations
-    plus the library contexts)."""
+    through the reverse-time adjoint, which reconstructs the trajectory from
+    the final state as pmwd does (no stored states).  The modes -> 2LPT part
+    is recomputed in the backward rather than kept.  Reports wall times and
+    device memory per phase (torch peaks, library contexts, device-wide use)."""
     import time
     import torch
-    import torch.utils.checkpoint as tcp
     import pmwd_amd as pa
     from pmwd_amd import _lib
     conf = pa.Configuration(2000.0 / 1024, (G, G, G), mesh_shape=1, a_start=1 / 32,
@@ -271,41 +270,65 @@ def full_adjoint(G, dev):
         ptcl, _ = pa.lpt(pa.linear_modes(w, cosmo, conf, theta=th), cosmo, conf, theta=th)
         return ptcl.disp, ptcl.vel, ptcl.pmid
 
+    # The initial conditions are recomputed in the backward rather than kept
+    # (as torch.utils.checkpoint would); between the N-body adjoint and that
+    # recomputation the library contexts give back their particle scratch.
     t0 = time.perf_counter()
-    disp, vel, pmid = tcp.checkpoint(ics, white, theta['Omega_m'], theta['sigma8'],
-                                     use_reentrant=False)
-    ptcl = pa.Particles(conf, pmid, disp, vel)
-    del disp, vel
-    ptcl, _ = pa.nbody(ptcl, None, cosmo, conf, checkpoint_every=8, omega_m=theta['Omega_m'],
-                       checkpoint_device='cpu')
+    with torch.no_grad():
+        disp0, vel0, pmid = ics(white, theta['Omega_m'], theta['sigma8'])
+    disp0.requires_grad_(True)
+    vel0.requires_grad_(True)
+    ptcl, _ = pa.nbody(pa.Particles(conf, pmid, disp0, vel0), None, cosmo, conf,
+                       omega_m=theta['Omega_m'])
     dens = pa.scatter(ptcl, conf)
+    del ptcl
     loss = ((dens.double() - target.double())**2).mean()
+    del dens
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
+    def used():
+        free, total = torch.cuda.mem_get_info(dev)
+        return (total - free) / 1e9
+
+    mem = {'library_context_gb_forward': sum(c.nbytes for c in _lib._ctx_cache.values()) / 1e9,
+           'device_used_gb_after_forward': used()}
     t0 = time.perf_counter()
-    g_w, g_om, g_s8 = torch.autograd.grad(loss, (white, theta['Omega_m'], theta['sigma8']))
+    g_d0, g_v0, g_om1 = torch.autograd.grad(loss, (disp0, vel0, theta['Omega_m']))
+    torch.cuda.synchronize()
+    t_nbody_bwd = time.perf_counter() - t0
+    mem['library_context_gb_adjoint'] = sum(c.nbytes for c in _lib._ctx_cache.values()) / 1e9
+    mem['device_used_gb_after_adjoint'] = used()
+    mem['torch_peak_gb_through_adjoint'] = torch.cuda.max_memory_allocated(dev) / 1e9
+    loss_value = float(loss.detach())
+    del disp0, vel0, loss
+    _lib.trim_contexts()
+    torch.cuda.reset_peak_memory_stats(dev)
+    d1, v1, _ = ics(white, theta['Omega_m'], theta['sigma8'])
+    g_w, g_om2, g_s8 = torch.autograd.grad((d1, v1), (white, theta['Omega_m'], theta['sigma8']),
+                                           grad_outputs=(g_d0, g_v0), allow_unused=True)
+    g_om = g_om1 + (g_om2 if g_om2 is not None else 0)
     torch.cuda.synchronize()
     t_bwd = time.perf_counter() - t0
-    ctx_bytes = sum(c.nbytes for c in _lib._ctx_cache.values())
-    peak = torch.cuda.max_memory_allocated(dev) + ctx_bytes
+    mem['torch_peak_gb_ic_backward'] = torch.cuda.max_memory_allocated(dev) / 1e9
+    mem['library_context_gb_ic_backward'] = sum(c.nbytes for c in _lib._ctx_cache.values()) / 1e9
     steps = len(conf.a_nbody) - 1
     n = conf.ptcl_num
     return {'workload': f'{G}^3 particles on a {G}^3 mesh, box {2000.0 * G / 1024:g} Mpc/h, '
                         f'{steps} KDK steps from a=1/32, 2LPT, density MSE against a seeded '
-                        f'target, gradients w.r.t. modes, Omega_m, sigma8, reversal '
-                        f'checkpoints every 8 steps in pinned host memory',
+                        f'target, gradients w.r.t. modes, Omega_m, sigma8 by reverse-time '
+                        f'integration from the final state',
             't_setup_s': t_setup, 't_forward_s': t_fwd, 't_backward_s': t_bwd,
+            't_nbody_backward_s': t_nbody_bwd,
             'fwd_updates_per_s': n * steps / t_fwd,
             'fwd_adj_updates_per_s': n * steps / (t_fwd + t_bwd),
-            'peak_hbm_gb': peak / 1e9, 'torch_peak_gb': torch.cuda.max_memory_allocated(dev) / 1e9,
-            'library_context_gb': ctx_bytes / 1e9, 'loss': float(loss.detach()),
+            'memory': mem, 'loss': loss_value,
             'grad_modes_norm': float(g_w.double().norm()), 'grad_Omega_m': float(g_om),
             'grad_sigma8': float(g_s8)}
 
 
 def main():
     args = parse()
-    world = int(os.environ.get('WORLD_SIZE', '1'))
+    world =int(os.environ.get('WORLD_SIZE', '1'))
     if args.gpus != 1 or world != 1:
         print(f'bench.py: this build runs on one GPU; --gpus {args.gpus} (WORLD_SIZE={world}) '
               'is not supported. The multi-GPU decomposition is a design note only.',

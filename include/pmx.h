@@ -52,6 +52,17 @@ int pmx_ctx_destroy(pmx_ctx* ctx);
 int pmx_ctx_set_stream(pmx_ctx* ctx, void* stream);
 /* Device bytes held by the context (host value). */
 long long pmx_ctx_bytes(const pmx_ctx* ctx);
+/* Frees context scratch once the context's stream has finished; the next
+ * call reallocates what it needs.  what = PMX_TRIM_PIPELINE: the particle
+ * copy of pmx_nbody_steps (34 B per record slot); PMX_TRIM_ALL: every
+ * particle-sized buffer (records, indices, value channels, that copy) and
+ * extra halo channels.  No counterpart in pmwd, where XLA owns every
+ * buffer: for callers that need the memory between phases (an adjoint
+ * after a forward, a backward recomputing its inputs, a serving pool under
+ * pressure).                                                               */
+#define PMX_TRIM_PIPELINE 1
+#define PMX_TRIM_ALL 2
+int pmx_ctx_trim(pmx_ctx* ctx, int what);
 
 /* ------------------------------------------------------- cell keys/sort ---
  * key[i] = C-order raveled index (strides of Particles.raveled_id,

@@ -39,6 +39,7 @@ _SIGS = {
     'pmx_ctx_destroy': [c_p],
     'pmx_ctx_set_stream': [c_p, c_p],
     'pmx_ctx_bytes': [c_p],
+    'pmx_ctx_trim': [c_p, c_int],
     'pmx_cell_keys': [c_p, c_p, c_p, c_ll, c_p],
     'pmx_sort_perm': [c_p, c_p, c_ll, c_p],
     'pmx_tile_keys': [c_p, c_p, c_p, c_ll, c_p],
@@ -124,6 +125,9 @@ def require_device(t, name='tensor'):
     return t
 
 
+TRIM_PIPELINE, TRIM_ALL = 1, 2   # pmx.h PMX_TRIM_*
+
+
 class Context:
     """Owner of a pmx_ctx for one (mesh shape, cell size, device)."""
 
@@ -147,6 +151,12 @@ class Context:
     @property
     def nbytes(self):
         return int(lib().pmx_ctx_bytes(self.h))
+
+    def trim(self, pipeline_only=False):
+        """Free particle-sized scratch (reallocated by the next call that
+        needs it): only the multi-step particle copy, or all of it."""
+        if self.h:
+            check(lib().pmx_ctx_trim(self.h, TRIM_PIPELINE if pipeline_only else TRIM_ALL))
 
     def close(self):
         if self.h:
@@ -207,6 +217,12 @@ def context_any_cell(mesh_shape, device):
             if s == shape and idx == device.index:
                 return ctx
     return context(shape, 1.0, device)
+
+
+def trim_contexts():
+    """Free the particle-sized scratch of every cached context."""
+    for c in _ctx_cache.values():
+        c.trim()
 
 
 def clear_contexts():

@@ -180,13 +180,11 @@ extern "C" int pmx_ctx_destroy(pmx_ctx* c) {
                 c->scan_tmp};
   for (void* p : ps)
     if (p) hipFree(p);
-  for (int b = 0; b < 2; ++b) {
-    void* qs[] = {c->pl[b].s0, c->pl[b].s1, c->pl[b].pz, c->pl_count[b], c->pl_start[b]};
-    for (void* p : qs)
-      if (p) hipFree(p);
-  }
-  if (c->pl_tile) hipFree(c->pl_tile);
-  if (c->pl_rank) hipFree(c->pl_rank);
+  // pl[1], pl_tile and pl_rank alias the scratch freed above
+  void* qs[] = {c->pl[0].s0, c->pl[0].s1, c->pl[0].pz, c->pl_count[0], c->pl_start[0],
+                c->pl_count[1], c->pl_start[1]};
+  for (void* p : qs)
+    if (p) hipFree(p);
   if (c->red_host) hipHostFree(c->red_host);
   free(c);
   return PMX_OK;
@@ -199,6 +197,44 @@ extern "C" int pmx_ctx_set_stream(pmx_ctx* c, void* stream) {
 }
 
 extern "C" long long pmx_ctx_bytes(const pmx_ctx* c) { return c ? (long long)c->bytes : 0; }
+
+extern "C" int pmx_ctx_trim(pmx_ctx* c, int what) {
+  PMX_REQUIRE(c != nullptr, "pmx_ctx_trim: NULL context");
+  PMX_REQUIRE((what & ~(PMX_TRIM_PIPELINE | PMX_TRIM_ALL)) == 0, "pmx_ctx_trim: bad flags");
+  // work still queued on the context's stream may use the scratch
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    pmx_set_error("pmx_ctx_trim: %s", hipGetErrorString(e));
+    return PMX_EHIP;
+  }
+  if (!(what & PMX_TRIM_ALL)) {
+    if (what & PMX_TRIM_PIPELINE) {
+      dev_free(c, (void**)&c->pl[0].s0, c->pl_cap * 16);
+      dev_free(c, (void**)&c->pl[0].s1, c->pl_cap * 16);
+      dev_free(c, (void**)&c->pl[0].pz, c->pl_cap * 2);
+      c->pl_cap = 0;
+    }
+    return PMX_OK;
+  }
+  dev_free(c, (void**)&c->rec, c->rec_cap * 16);
+  dev_free(c, (void**)&c->pidx, c->rec_cap * 4);
+  dev_free(c, (void**)&c->pkey, c->rec_cap * 4);
+  dev_free(c, (void**)&c->prank, c->rec_cap * 4);
+  dev_free(c, (void**)&c->vrec, c->vrec_cap * 4);
+  dev_free(c, (void**)&c->pl[0].s0, c->pl_cap * 16);
+  dev_free(c, (void**)&c->pl[0].s1, c->pl_cap * 16);
+  dev_free(c, (void**)&c->pl[0].pz, c->pl_cap * 2);
+  c->pl[1] = PState{};
+  c->pl_tile = c->pl_rank = nullptr;
+  c->ptcl_cap = c->rec_cap = c->vrec_cap = c->pl_cap = 0;
+  c->caps_valid = 0;
+  if (c->halo_nch > 1) {
+    dev_free(c, (void**)&c->halo, (size_t)c->halo_nch * c->ntiles * c->tg.shell * 4);
+    c->halo_nch = 0;
+    return pmx_ensure_halo(c, 1);
+  }
+  return PMX_OK;
+}
 
 extern "C" int pmx_tile_shape(const pmx_ctx* c, int t[3]) {
   PMX_REQUIRE(c != nullptr && t != nullptr, "pmx_tile_shape: NULL argument");
@@ -213,15 +249,19 @@ int pmx_ensure_ptcl(pmx_ctx* c, long long n) {
   const long long rcap = pmx_record_cap(c, cap);
   dev_free(c, (void**)&c->rec, c->rec_cap * 16);
   dev_free(c, (void**)&c->pidx, c->rec_cap * 4);
-  dev_free(c, (void**)&c->pkey, c->ptcl_cap * 4);
-  dev_free(c, (void**)&c->prank, c->ptcl_cap * 4);
+  dev_free(c, (void**)&c->pkey, c->rec_cap * 4);
+  dev_free(c, (void**)&c->prank, c->rec_cap * 4);
   dev_free(c, (void**)&c->vrec, c->vrec_cap * 4);
-  c->ptcl_cap = c->rec_cap = c->vrec_cap = 0;
+  dev_free(c, (void**)&c->pl[0].s0, c->pl_cap * 16);
+  dev_free(c, (void**)&c->pl[0].s1, c->pl_cap * 16);
+  dev_free(c, (void**)&c->pl[0].pz, c->pl_cap * 2);
+  c->ptcl_cap = c->rec_cap = c->vrec_cap = c->pl_cap = 0;
   c->caps_valid = 0;
+  // pkey/prank hold a tile and rank per particle, or (multi-step re-bin) per record slot
   int rc = dev_alloc(c, (void**)&c->rec, rcap * 16);
   rc |= dev_alloc(c, (void**)&c->pidx, rcap * 4);
-  rc |= dev_alloc(c, (void**)&c->pkey, cap * 4);
-  rc |= dev_alloc(c, (void**)&c->prank, cap * 4);
+  rc |= dev_alloc(c, (void**)&c->pkey, rcap * 4);
+  rc |= dev_alloc(c, (void**)&c->prank, rcap * 4);
   if (rc) return PMX_ENOMEM;
   c->ptcl_cap = cap;
   c->rec_cap = rcap;
@@ -237,37 +277,42 @@ int pmx_ensure_halo(pmx_ctx* c, int nch) {
   return PMX_OK;
 }
 
-// the pipeline's tile-ordered particle copies (rec_cap slots each)
+// The multi-step pipeline's two tile-ordered particle copies of rec_cap
+// slots: the first is its own memory, the second lives in the per-step
+// path's scratch (s0 in rec, s1 in the four vrec channels, pz in pidx), and
+// the exact re-bin's per-slot tile and rank in pkey/prank.  The per-step
+// path rewrites all of these before it reads them.
 int pmx_ensure_state(pmx_ctx* c) {
+  int rc;
+  if ((rc = pmx_ensure_vrec(c))) return rc;
   const long long cap = c->rec_cap;
-  if (cap <= c->pl_cap) return PMX_OK;
-  const long long old = c->pl_cap;
-  for (int b = 0; b < 2; ++b) {
-    dev_free(c, (void**)&c->pl[b].s0, old * 16);
-    dev_free(c, (void**)&c->pl[b].s1, old * 16);
-    dev_free(c, (void**)&c->pl[b].pz, old * 2);
+  if (cap > c->pl_cap) {
+    dev_free(c, (void**)&c->pl[0].s0, c->pl_cap * 16);
+    dev_free(c, (void**)&c->pl[0].s1, c->pl_cap * 16);
+    dev_free(c, (void**)&c->pl[0].pz, c->pl_cap * 2);
+    c->pl_cap = 0;
+    rc = dev_alloc(c, (void**)&c->pl[0].s0, cap * 16);
+    rc |= dev_alloc(c, (void**)&c->pl[0].s1, cap * 16);
+    rc |= dev_alloc(c, (void**)&c->pl[0].pz, cap * 2);
+    for (int b = 0; b < 2; ++b) {
+      if (!c->pl_count[b]) rc |= dev_alloc(c, (void**)&c->pl_count[b], (size_t)(c->ntiles + 1) * 4);
+      if (!c->pl_start[b]) rc |= dev_alloc(c, (void**)&c->pl_start[b], (size_t)(c->ntiles + 1) * 4);
+    }
+    if (rc) return PMX_ENOMEM;
+    c->pl_cap = cap;
   }
-  dev_free(c, (void**)&c->pl_tile, old * 4);
-  dev_free(c, (void**)&c->pl_rank, old * 4);
-  c->pl_cap = 0;
-  int rc = PMX_OK;
-  for (int b = 0; b < 2; ++b) {
-    rc |= dev_alloc(c, (void**)&c->pl[b].s0, cap * 16);
-    rc |= dev_alloc(c, (void**)&c->pl[b].s1, cap * 16);
-    rc |= dev_alloc(c, (void**)&c->pl[b].pz, cap * 2);
-    if (!c->pl_count[b]) rc |= dev_alloc(c, (void**)&c->pl_count[b], (size_t)(c->ntiles + 1) * 4);
-    if (!c->pl_start[b]) rc |= dev_alloc(c, (void**)&c->pl_start[b], (size_t)(c->ntiles + 1) * 4);
-  }
-  rc |= dev_alloc(c, (void**)&c->pl_tile, cap * 4);
-  rc |= dev_alloc(c, (void**)&c->pl_rank, cap * 4);
-  if (rc) return PMX_ENOMEM;
-  c->pl_cap = cap;
+  c->pl[1].s0 = c->rec;
+  c->pl[1].s1 = reinterpret_cast<float4*>(c->vrec);
+  c->pl[1].pz = reinterpret_cast<uint16_t*>(c->pidx);
+  c->pl_tile = c->pkey;
+  c->pl_rank = c->prank;
   return PMX_OK;
 }
 
-// three channels of per-record values, rec_cap floats apart
+// per-record value channels, rec_cap floats apart: three for the adjoint's
+// deposits, a fourth so the four hold a float4 per slot for the pipeline
 int pmx_ensure_vrec(pmx_ctx* c) {
-  const long long cap = 3 * c->rec_cap;
+  const long long cap = 4 * c->rec_cap;
   if (cap <= c->vrec_cap) return PMX_OK;
   dev_free(c, (void**)&c->vrec, c->vrec_cap * 4);
   c->vrec_cap = 0;
@@ -290,7 +335,10 @@ MeshGeom pmx_geom_padded(const pmx_ctx* c) {
   g.n0 = c->n[0]; g.n1 = c->n[1]; g.n2 = c->n[2];
   g.rs = c->rs;
   g.cell = c->cellf;
-  g.inv_cell = 1.f / c->cellf;
+  int e = 0;
+  const float m = frexpf(c->cellf, &e);  // cell = m 2^e, m in [0.5, 1)
+  const float inv = ldexpf(1.f, 1 - e);
+  g.inv_cell = (m == 0.5f && isfinite(inv) && inv >= FLT_MIN) ? inv : 0.f;
   return g;
 }
 

@@ -54,7 +54,9 @@ struct MeshGeom {
   int n0, n1, n2;      // mesh shape (periodic wrap shape)
   int rs;              // row stride in floats along the last axis (>= n2)
   float cell;          // float32 cell size (enmesh a1, pm_util.py:83)
-  float inv_cell;      // unused by bit-exact paths; only for gradients' /cell
+  // 1/cell when cell is a power of two (then x * inv_cell == x / cell for
+  // every float x: both round the same exact value), else 0
+  float inv_cell;
 };
 
 // Scatter binning: the mesh is cut into tiles of TX x TY x TZ cells.
@@ -145,14 +147,15 @@ struct pmx_ctx {
   int caps_valid;      // host: tile_count holds the exact counts of caps_n particles
   long long caps_n;
   long long rec_cap;   // record slots of rec/pidx (and the vrec channel stride)
-  // multi-step pipeline: two tile-ordered particle copies of rec_cap slots,
-  // their tile counts and starts, and per-slot tile/rank of an exact re-bin
+  // multi-step pipeline: two tile-ordered particle copies of rec_cap slots
+  // (the second aliases rec/vrec/pidx), their tile counts and starts, and
+  // the per-slot tile/rank of an exact re-bin (aliases pkey/prank)
   PState pl[2];
   uint32_t* pl_count[2];
   uint32_t* pl_start[2];
   uint32_t* pl_tile;
   uint32_t* pl_rank;
-  long long pl_cap;    // slots the pipeline buffers hold
+  long long pl_cap;    // slots of the pipeline's own copy
   float* halo;         // nch shells per tile, channel-major
   int halo_nch;        // channels the halo buffer holds
   long long ptcl_cap;
@@ -243,9 +246,10 @@ __device__ __forceinline__ float pmx_sign(float x) {
   return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f);
 }
 
-__device__ __forceinline__ CicAxis pmx_cic_axis(int pm, float dp, float cell, int n) {
+__device__ __forceinline__ CicAxis pmx_cic_axis(int pm, float dp, float cell, float inv_cell,
+                                                int n) {
   CicAxis a;
-  float d = dp / cell;
+  float d = inv_cell != 0.f ? dp * inv_cell : dp / cell;
   a.d = d;
   float fl = floorf(d);
   int i = (int)fl;
@@ -309,9 +313,9 @@ __device__ __forceinline__ void load_ptcl(const int16_t* __restrict__ pmid,
 
 __device__ __forceinline__ void cic_setup(const int pm[3], const float dp[3], const MeshGeom& g,
                                           CicAxis ax[3]) {
-  ax[0] = pmx_cic_axis(pm[0], dp[0], g.cell, g.n0);
-  ax[1] = pmx_cic_axis(pm[1], dp[1], g.cell, g.n1);
-  ax[2] = pmx_cic_axis(pm[2], dp[2], g.cell, g.n2);
+  ax[0] = pmx_cic_axis(pm[0], dp[0], g.cell, g.inv_cell, g.n0);
+  ax[1] = pmx_cic_axis(pm[1], dp[1], g.cell, g.inv_cell, g.n1);
+  ax[2] = pmx_cic_axis(pm[2], dp[2], g.cell, g.inv_cell, g.n2);
 }
 
 __device__ __forceinline__ uint32_t tile_of(const CicAxis ax[3], const TileGeom& tg) {

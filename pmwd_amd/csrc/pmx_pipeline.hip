@@ -20,8 +20,15 @@
 // (nbody.py:205-212 for each step).
 #include "pmx_internal.h"
 
+#ifndef PL_TPB
 #define PL_TPB 512
-#define PL_U 2
+#endif
+#ifndef PL_U
+#define PL_U 4
+#endif
+#ifndef PL_MINB
+#define PL_MINB 1
+#endif
 
 enum { PL_CAP = 0, PL_COUNT = 1, PL_WRITE = 2, PL_LAST = 3 };
 
@@ -100,7 +107,7 @@ __global__ void pl_caps_k(const uint32_t* __restrict__ count_in, uint32_t* __res
 //   PL_LAST   acc, vel and disp to the caller's arrays at the particle index.
 // PL_COUNT and PL_WRITE return at once unless the overflow flag is set.
 template <int MODE>
-__global__ void __launch_bounds__(PL_TPB)
+__global__ void __launch_bounds__(PL_TPB, PL_MINB)
     pl_step_k(PState in, const uint32_t* __restrict__ in_start,
               const uint32_t* __restrict__ in_count, MeshGeom g, TileGeom tg,
               const float* __restrict__ F0, const float* __restrict__ F1,

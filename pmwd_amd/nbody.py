@@ -2,9 +2,11 @@
 
 Time-step factors are computed on the host in FP64 from the growth table
 (nbody.py:12-36) and rounded to float32 at use, as pmwd does; the particle
-state never leaves the GPU.  With the default ``symp_splits`` (KDK) each step
-is one ``pmx_nbody_step`` call (fused kick+drift+bin, scatter, Poisson,
-fused gather+kick); other splittings run the ops one by one.
+state never leaves the GPU.  With the default ``symp_splits`` (KDK) the
+steps between checkpoints are one ``pmx_nbody_steps`` call (particles kept
+in tile order on the device; per step a scatter, the Poisson solve and one
+fused gather + closing kick + next kick/drift + re-bin kernel); other
+splittings run the ops one by one.
 
 The adjoint reconstructs the trajectory backward in time from the final
 state as pmwd's ``nbody_adj`` (nbody.py:226-260) does, optionally snapping
@@ -209,6 +211,9 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
     L = _lib.lib()
     n = pmid.shape[0]
     c = _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
+    # the reverse steps bin per step; the forward pipeline's particle copy
+    # gives its memory back to the cotangents
+    c.trim(pipeline_only=True)
     h = c.bind()
     val, coef = _mass_val(conf), omega_coef(cosmo, conf)
     Om = cosmo.Omega_m

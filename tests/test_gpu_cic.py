@@ -36,14 +36,36 @@ SHAPES = [
 ]
 
 
+@pytest.mark.parametrize('spacing', [2.0, 1.7])
 @pytest.mark.parametrize('shape,ratio,std', SHAPES)
-def test_cell_keys_bit_exact(dev, shape, ratio, std):
+def test_cell_keys_bit_exact(dev, shape, ratio, std, spacing):
     from pmwd_amd.pm_util import cell_keys
-    conf = conf_of(shape, ratio, spacing=2.0)
+    conf = conf_of(shape, ratio, spacing=spacing)
     pmid, disp = make_ptcl(shape, conf.mesh_shape, conf.cell_size, std * conf.cell_size, seed=1)
     ptcl = to_ptcl(conf, pmid, disp, dev)
     got = cell_keys(ptcl, conf).cpu().numpy()
     want = opm.cell_keys(pmid, disp, conf.cell_size, conf.mesh_shape)
+    np.testing.assert_array_equal(got, want.astype(np.int64))
+
+
+@pytest.mark.parametrize('spacing,ratio', [(1.0, 2), (1.0, 1), (2.0, 1), (1.5, 2), (0.7, 1)])
+def test_cell_keys_at_cell_boundaries(dev, spacing, ratio):
+    """disp at and one ulp either side of multiples of the cell size: the
+    corner follows float32 disp/cell exactly (a product with the exact
+    reciprocal when the cell is a power of two, a division otherwise)."""
+    from pmwd_amd.pm_util import cell_keys
+    shape = (8, 8, 8)
+    conf = conf_of(shape, ratio, spacing=spacing)
+    pmid, disp = make_ptcl(shape, conf.mesh_shape, conf.cell_size, 0.0)
+    rng = np.random.default_rng(3)
+    k = rng.integers(-40, 40, size=disp.shape).astype(np.float32)
+    base = (k * np.float32(conf.cell_size)).astype(np.float32)
+    step = rng.integers(-1, 2, size=disp.shape)
+    d = np.where(step < 0, np.nextafter(base, np.float32(-np.inf)),
+                 np.where(step > 0, np.nextafter(base, np.float32(np.inf)), base))
+    d = d.astype(np.float32)
+    got = cell_keys(to_ptcl(conf, pmid, d, dev), conf).cpu().numpy()
+    want = opm.cell_keys(pmid, d, conf.cell_size, conf.mesh_shape)
     np.testing.assert_array_equal(got, want.astype(np.int64))
 
 

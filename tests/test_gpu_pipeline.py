@@ -53,12 +53,16 @@ def assert_same(a, b):
         np.testing.assert_array_equal(x, y)
 
 
-@pytest.mark.parametrize('shape,ratio,k', [((32, 32, 32), 1, 5), ((20, 24, 12), 2, 4),
-                                           ((7, 8, 6), 1, 3), ((16, 16, 16), 2, 1),
-                                           ((12, 10, 8), 1, 2)])
-def test_steps_match_single_steps(dev, shape, ratio, k):
+@pytest.mark.parametrize('shape,ratio,k,spacing', [((32, 32, 32), 1, 5, 1.0),
+                                                   ((20, 24, 12), 2, 4, 1.0),
+                                                   ((7, 8, 6), 1, 3, 1.0),
+                                                   ((16, 16, 16), 2, 1, 1.0),
+                                                   ((12, 10, 8), 1, 2, 1.0),
+                                                   ((16, 16, 16), 1, 4, 1.3),
+                                                   ((12, 8, 20), 2, 3, 0.7)])
+def test_steps_match_single_steps(dev, shape, ratio, k, spacing):
     import pmwd_amd as pa
-    conf = pa.Configuration(1.0, shape, mesh_shape=ratio)
+    conf = pa.Configuration(spacing, shape, mesh_shape=ratio)
     pmid, disp = particles(shape, ratio, conf.cell_size, 1)
     vel = (0.5 * np.random.default_rng(2).standard_normal(disp.shape)).astype(np.float32)
     k1, d, k2 = factors(k, 3)
