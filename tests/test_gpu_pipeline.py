@@ -132,3 +132,38 @@ def test_nbody_driver_pipeline_matches_per_step(dev):
         assert o[3].keys() == ref[3].keys()
         for i in ref[3]:
             assert_same(o[3][i], ref[3][i])
+
+
+def test_trim_between_calls(dev):
+    """Freeing the pipeline copy, or all particle scratch, between calls
+    changes nothing: the next call reallocates what it needs."""
+    import pmwd_amd as pa
+    from pmwd_amd import _lib
+    shape, ratio = (16, 16, 16), 2
+    conf = pa.Configuration(1.0, shape, mesh_shape=ratio)
+    pmid, disp = particles(shape, ratio, conf.cell_size, 10)
+    vel = (0.5 * np.random.default_rng(11).standard_normal(disp.shape)).astype(np.float32)
+    k1, d, k2 = factors(3, 12)
+    L = _lib.lib()
+    ctx = _lib.Context(conf.mesh_shape, conf.cell_size, dev)
+    H = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    P = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev).clone()
+    outs = []
+    full = ctx.nbytes
+    for trim in (None, 'pipeline', 'all'):
+        if trim == 'pipeline':
+            ctx.trim(pipeline_only=True)
+            assert ctx.nbytes < full
+        elif trim == 'all':
+            ctx.trim()
+            assert ctx.nbytes < full
+        pm, x, v = P(pmid), P(disp), P(vel)
+        acc = torch.zeros_like(x)
+        h = ctx.bind()
+        _lib.check(L.pmx_nbody_steps(h, _lib.ptr(pm), _lib.ptr(x), _lib.ptr(v), _lib.ptr(acc),
+                                     len(pmid), 1.0, 0.45, 3, H(k1), H(d), H(k2)))
+        torch.cuda.synchronize()
+        outs.append((x.cpu().numpy(), v.cpu().numpy(), acc.cpu().numpy()))
+        full = ctx.nbytes
+    assert_same(outs[1], outs[0])
+    assert_same(outs[2], outs[0])
