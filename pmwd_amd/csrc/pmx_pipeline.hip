@@ -115,19 +115,28 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
               PlUser user) {
   if ((MODE == PL_COUNT || MODE == PL_WRITE) && out.flag[0] == 0u) return;
   extern __shared__ float sh[];
+  // PL_CAP: per round, particles moving to each of the 27 tiles around this
+  // one are counted in LDS (wave-aggregated) and take one contiguous run per
+  // tile from a single global atomic, so a round's stores into a tile are
+  // one run instead of a short run per wave; farther moves rank per wave
+  __shared__ uint32_t s_cnt[27], s_base[27];
   int o[3], s[3], ti[3];
   const int t = (int)pmx_xcd_block(blockIdx.x, gridDim.x);
   tile_origin(t, tg, g, o, s, ti);
   const int LY = s[1] + 1, LZ = s[2] + 1;
   const int ltot = (s[0] + 1) * LY * LZ;
   const float* F[3] = {F0, F1, F2};
+  if (MODE == PL_CAP && threadIdx.x < 27) s_cnt[threadIdx.x] = 0u;
   pmx_stage_tile<3, PL_TPB>(sh, o, LY, LZ, ltot, g, F);
   __syncthreads();
   const uint32_t start = in_start[t];
   const uint32_t cnt = in_count[t];
+  const int ntl[3] = {tg.ntx, tg.nty, tg.ntz};
   for (uint32_t q0 = 0; q0 < cnt; q0 += PL_TPB * PL_U) {
-    float4 a0[PL_U], a1[PL_U];
+    float4 a0[PL_U], a1[PL_U], n0[PL_U], n1[PL_U];
     uint16_t az[PL_U];
+    uint32_t nt[PL_U];
+    int code[PL_U];
 #pragma unroll
     for (int u = 0; u < PL_U; ++u) {
       const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
@@ -140,10 +149,11 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
 #pragma unroll
     for (int u = 0; u < PL_U; ++u) {
       const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
-      const bool active = q < cnt;
-      float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0;
-      uint32_t nt = 0;
-      if (active) {
+      n0[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      n1[u] = n0[u];
+      nt[u] = 0;
+      code[u] = 27;
+      if (q < cnt) {
         float acc[3] = {0.f, 0.f, 0.f};
         float d[3];
         int l[3];
@@ -172,29 +182,89 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
           pmx_state_pm(a0[u], az[u], pm);
           CicAxis ax[3];
           cic_setup(pm, x, g, ax);
-          nt = tile_of(ax, tg);
-          n0 = make_float4(x[0], x[1], x[2], a0[u].w);
-          n1 = make_float4(v[0], v[1], v[2], a1[u].w);
+          nt[u] = tile_of(ax, tg);
+          n0[u] = make_float4(x[0], x[1], x[2], a0[u].w);
+          n1[u] = make_float4(v[0], v[1], v[2], a1[u].w);
+          if (MODE == PL_CAP) {
+            // neighbour code 0..26 of the new tile, 27 when farther
+            const unsigned tm[3] = {tg.mx, tg.my, tg.mz};
+            int cc = 0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              const int dl = pmx_tdiv(ax[a].lo, tm[a]) - ti[a];
+              int ca = 3;
+              if (dl == 0) ca = 1;
+              else if (dl == 1 || dl == 1 - ntl[a]) ca = 2;
+              else if (dl == -1 || dl == ntl[a] - 1) ca = 0;
+              cc = ca == 3 || cc == 27 ? 27 : cc * 3 + ca;
+            }
+            code[u] = cc;
+          }
         }
       }
-      if (MODE == PL_LAST) continue;
-      if (MODE == PL_WRITE) {
-        if (active)
-          pl_put(out.st, out.start[out.ptile[start + q]] + out.prank[start + q], n0, n1, az[u]);
-        continue;
+    }
+    if (MODE == PL_LAST) continue;
+    if (MODE == PL_WRITE) {
+#pragma unroll
+      for (int u = 0; u < PL_U; ++u) {
+        const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
+        if (q < cnt)
+          pl_put(out.st, out.start[out.ptile[start + q]] + out.prank[start + q], n0[u], n1[u],
+                 az[u]);
       }
-      const uint32_t r = agg_slot(out.count, nt, active);
-      if (!active) continue;
-      if (MODE == PL_COUNT) {
-        out.ptile[start + q] = nt;
-        out.prank[start + q] = r;
-      } else {
-        const uint32_t b = out.start[nt], e = out.start[nt + 1];
-        if (r < e - b)
-          pl_put(out.st, b + r, n0, n1, az[u]);
-        else
-          out.flag[0] = 1u;
+      continue;
+    }
+    if (MODE == PL_COUNT) {
+#pragma unroll
+      for (int u = 0; u < PL_U; ++u) {
+        const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
+        const uint32_t r = agg_slot(out.count, nt[u], q < cnt);
+        if (q < cnt) {
+          out.ptile[start + q] = nt[u];
+          out.prank[start + q] = r;
+        }
       }
+      continue;
+    }
+    // PL_CAP: ranks within this round's run per neighbour tile, then the runs
+    uint32_t rk[PL_U];
+#pragma unroll
+    for (int u = 0; u < PL_U; ++u) {
+      const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
+      const bool near = q < cnt && code[u] < 27;
+      const uint32_t rl = agg_slot(s_cnt, near ? (uint32_t)code[u] : 0u, near);
+      const uint32_t rf = agg_slot(out.count, nt[u], q < cnt && code[u] == 27);
+      rk[u] = near ? rl : rf;
+    }
+    __syncthreads();
+    if (threadIdx.x < 27) {
+      const uint32_t k = s_cnt[threadIdx.x];
+      uint32_t base = 0;
+      if (k) {
+        int c3 = threadIdx.x, nb[3];
+#pragma unroll
+        for (int a = 2; a >= 0; --a) {
+          int v = ti[a] + c3 % 3 - 1;
+          nb[a] = v < 0 ? v + ntl[a] : (v >= ntl[a] ? v - ntl[a] : v);
+          c3 /= 3;
+        }
+        const uint32_t tile = ((uint32_t)nb[0] * tg.nty + (uint32_t)nb[1]) * tg.ntz + (uint32_t)nb[2];
+        base = atomicAdd(&out.count[tile], k);
+      }
+      s_base[threadIdx.x] = base;
+      s_cnt[threadIdx.x] = 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PL_U; ++u) {
+      const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
+      if (q >= cnt) continue;
+      const uint32_t r = code[u] < 27 ? s_base[code[u]] + rk[u] : rk[u];
+      const uint32_t b = out.start[nt[u]], e = out.start[nt[u] + 1];
+      if (r < e - b)
+        pl_put(out.st, b + r, n0[u], n1[u], az[u]);
+      else
+        out.flag[0] = 1u;
     }
   }
 }
