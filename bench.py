@@ -38,6 +38,9 @@ def parse():
                    help='instead of the default run: forward plus gradient of a density MSE '
                         'for G^3 particles on a G^3 mesh (box 2000*G/1024 Mpc/h, 31 steps, '
                         'reverse-time adjoint), with device memory per phase')
+    p.add_argument('--full-adjoint-checkpoints', type=int, default=0, metavar='S',
+                   help='with --full-adjoint: snap the reverse run to states stored every S '
+                        'steps in host memory (0: none, pure reversal as pmwd)')
     p.add_argument('--serve-requests', type=int, default=40)
     p.add_argument('--serve-rate', type=float, default=2.0, help='Poisson arrivals per second')
     p.add_argument('--serve-slots', type=int, default=8)
@@ -238,13 +241,14 @@ def fft_yardstick(conf, ctx, h, L, _lib, dev, reps=5):
             'r2c_rel_l2_vs_rocfft': rel(yo, yr), 'c2r_rel_l2_vs_rocfft': rel(xo, xr)}
 
 
-def full_adjoint(G, dev):
+def full_adjoint(G, dev, ckpt_every=0):
     """Forward plus gradient at G^3 particles on a G^3 mesh (the 1024^3
     configuration at G=1024): white modes (seed 0) -> linear_modes -> 2LPT ->
     31 KDK steps -> density; loss = mean((rho - target)^2) against a seeded
     target density; gradients with respect to the modes, Omega_m and sigma8
     through the reverse-time adjoint, which reconstructs the trajectory from
-    the final state as pmwd does (no stored states).  The modes -> 2LPT part
+    the final state, snapped to the states stored every ckpt_every steps in
+    host memory (0: pure reversal as pmwd).  The modes -> 2LPT part
     is recomputed in the backward rather than kept.  Reports wall times and
     device memory per phase (torch peaks, library contexts, device-wide use)."""
     import time
@@ -279,7 +283,8 @@ def full_adjoint(G, dev):
     disp0.requires_grad_(True)
     vel0.requires_grad_(True)
     ptcl, _ = pa.nbody(pa.Particles(conf, pmid, disp0, vel0), None, cosmo, conf,
-                       omega_m=theta['Omega_m'])
+                       omega_m=theta['Omega_m'], checkpoint_every=ckpt_every or None,
+                       checkpoint_device='cpu')
     dens = pa.scatter(ptcl, conf)
     del ptcl
     loss = ((dens.double() - target.double())**2).mean()
@@ -316,7 +321,9 @@ def full_adjoint(G, dev):
     return {'workload': f'{G}^3 particles on a {G}^3 mesh, box {2000.0 * G / 1024:g} Mpc/h, '
                         f'{steps} KDK steps from a=1/32, 2LPT, density MSE against a seeded '
                         f'target, gradients w.r.t. modes, Omega_m, sigma8 by reverse-time '
-                        f'integration from the final state',
+                        f'integration from the final state'
+                        + (f', snapped to states stored every {ckpt_every} steps in host memory'
+                           if ckpt_every else ''),
             't_setup_s': t_setup, 't_forward_s': t_fwd, 't_backward_s': t_bwd,
             't_nbody_backward_s': t_nbody_bwd,
             'fwd_updates_per_s': n * steps / t_fwd,
@@ -346,7 +353,8 @@ def main():
     torch.cuda.set_device(dev)
     if args.full_adjoint:
         log(f'full forward + gradient at {args.full_adjoint}^3')
-        print(json.dumps({'full_adjoint': full_adjoint(args.full_adjoint, dev)}))
+        print(json.dumps({'full_adjoint': full_adjoint(args.full_adjoint, dev,
+                                                       args.full_adjoint_checkpoints)}))
         return
     G = args.grid
     spacing = args.box / G

@@ -121,14 +121,18 @@ def _force(c, pmid, disp, acc, n, val, coef):
                                       _lib.ptr(acc)))
 
 
+_PIN_MAX = 4 << 30  # larger host checkpoints are pageable (pinned memory is limited)
+
+
 def _store(t, where):
-    """A copy of t for a reversal checkpoint: on the device, or in pinned
-    host memory when ``where`` is 'cpu' (copied back when the adjoint needs
-    it)."""
+    """A copy of t for a reversal checkpoint: on the device, or in host
+    memory when ``where`` is 'cpu' (pinned up to 4 GiB per array, pageable
+    beyond; copied back when the adjoint needs it)."""
     if where is None or torch.device(where) == t.device:
         return t.clone()
-    out = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
-    out.copy_(t, non_blocking=True)
+    pin = t.numel() * t.element_size() <= _PIN_MAX
+    out = torch.empty(t.shape, dtype=t.dtype, pin_memory=pin)
+    out.copy_(t, non_blocking=pin)
     return out
 
 
@@ -343,7 +347,7 @@ def nbody(ptcl, obsvbl, cosmo, conf, reverse=False, checkpoint_every=None, omega
     ``cosmo.Omega_m`` that requires grad, w.r.t. Omega_m.  With
     ``checkpoint_every`` the adjoint restarts from states stored every that
     many steps, kept on the device or, with ``checkpoint_device='cpu'``, in
-    pinned host memory.
+    host memory.
     """
     if cosmo.growth is None:
         raise ValueError('Growth table is empty. Call growth_integ or boltzmann first.')
