@@ -513,26 +513,26 @@ def main():
             adj_sched.append((nb._f32(nb.kick_factor(a_prev, a_prev, ah, cosmo, conf)),
                               nb._f32(nb.drift_factor(ah, a_prev, a_next, cosmo, conf)),
                               nb._f32(nb.kick_factor(a_next, ah, a_next, cosmo, conf))))
-        dots = (ctypes.c_double * 2)()
         _lib.check(L.pmx_gravity_vjp(h, _lib.ptr(pmid), _lib.ptr(disp), N, val, coef, _lib.ptr(vc),
                                      _lib.ptr(acc), _lib.ptr(acc_cot), ctypes.byref(dot)))
         torch.cuda.synchronize()
         _lib.check(L.pmx_timing_enable(h, 1))
         _lib.timing_read(ctx)
         t0 = time.perf_counter()
-        kp = 0.0
-        for i in range(K):
-            ka, d, kb = adj_sched[i % nsteps_sched]
-            # one reverse KDK step (closing kick deferred into the next call),
-            # with the two float64 dots the Omega_m cotangent needs read back
-            # as nbody's adjoint does
-            _lib.check(L.pmx_nbody_step_adj(h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel),
-                                            _lib.ptr(acc), _lib.ptr(dc), _lib.ptr(vc),
-                                            _lib.ptr(acc_cot), N, val, coef, kp, ka, d, None,
-                                            dots))
-            kp = kb
+        # K reverse KDK steps in one pmx_nbody_steps_adj call (particles and
+        # cotangents kept in tile order), the per-step float64 dots the
+        # Omega_m cotangent needs read back at the end as nbody's adjoint
+        # does, then the pending closing kick
+        f = np.array([adj_sched[i % nsteps_sched] for i in range(K)], dtype=np.float32)
+        fa, fd, fb = (np.ascontiguousarray(f[:, m]) for m in range(3))
+        adots = (ctypes.c_double * (2 * K))()
+        P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+        _lib.check(L.pmx_nbody_steps_adj(h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel),
+                                         _lib.ptr(acc), _lib.ptr(dc), _lib.ptr(vc),
+                                         _lib.ptr(acc_cot), N, val, coef, 0.0, K, P(fa), P(fd),
+                                         P(fb), None, adots))
         _lib.check(L.pmx_kick_adj(h, N, _lib.ptr(vel), _lib.ptr(acc), _lib.ptr(dc),
-                                  _lib.ptr(acc_cot), _lib.ptr(vc), kp, None))
+                                  _lib.ptr(acc_cot), _lib.ptr(vc), float(fb[-1]), None))
         torch.cuda.synchronize()
         t_adj = time.perf_counter() - t0
         atiming = _lib.timing_read(ctx)

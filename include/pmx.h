@@ -200,6 +200,20 @@ int pmx_nbody_step_adj(pmx_ctx* ctx, const int16_t* pmid, float* disp,
                        float* acc_cot, long long n, float val, float coef,
                        float kp, float ka, float d, const float* disp_snap,
                        double* dots);
+/* nsteps reverse KDK steps of integrate_adj (pmwd/nbody.py:143-162, the
+ * loop of nbody_adj :226-260): bitwise the arrays of the chained calls
+ * pmx_nbody_step_adj(kp, ka[0], d[0]), pmx_nbody_step_adj(kb[0], ka[1],
+ * d[1]), ..., with disp_snap (may be NULL) given to the last; the last
+ * closing kick kb[nsteps-1] stays pending for the caller, as there.
+ * dots[2s], dots[2s+1] (host, 2*nsteps doubles) are step s's two dots,
+ * the same sums in another grouping.  Between steps the particles and
+ * their cotangents stay on the device in tile order (about 100 extra
+ * bytes per particle).                                                     */
+int pmx_nbody_steps_adj(pmx_ctx* ctx, const int16_t* pmid, float* disp,
+                        float* vel, float* acc, float* disp_cot, float* vel_cot,
+                        float* acc_cot, long long n, float val, float coef,
+                        float kp, int nsteps, const float* ka, const float* d,
+                        const float* kb, const float* disp_snap, double* dots);
 
 /* float64 dot product of two float32 arrays of length n (host value).      */
 int pmx_dot(pmx_ctx* ctx, const float* a, const float* b, long long n, double* out);

@@ -63,6 +63,8 @@ _SIGS = {
     'pmx_drift_adj': [c_p, c_ll, c_p, c_p, c_p, c_p, c_f, ctypes.POINTER(c_d)],
     'pmx_nbody_step_adj': [c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_ll, c_f, c_f, c_f, c_f, c_f,
                            c_p, ctypes.POINTER(c_d)],
+    'pmx_nbody_steps_adj': [c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_ll, c_f, c_f, c_f, c_int,
+                            c_p, c_p, c_p, c_p, ctypes.POINTER(c_d)],
     'pmx_dot': [c_p, c_p, c_p, c_ll, ctypes.POINTER(c_d)],
     'pmx_timing_enable': [c_p, c_int],
     'pmx_timing_read': [c_p, ctypes.POINTER(c_d), ctypes.POINTER(c_ll)],

@@ -17,32 +17,6 @@
 #define ADJ_TPB 512
 #define ADJ_U 4
 
-// Fixed-order block sum of s (wave butterflies, then wave sums in order)
-// stored to part[blockIdx.x].
-template <int TPB>
-__device__ __forceinline__ void block_sum_store(double s, double* __restrict__ part) {
-  __shared__ double wsum[TPB / 64];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-  if (__lane_id() == 0) wsum[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0;
-    for (int w = 0; w < TPB / 64; ++w) t += wsum[w];
-    part[blockIdx.x] = t;
-  }
-}
-
-// Running max of |v| per channel: wave max, then one atomic per wave only
-// when it raises the stored maximum (non-negative floats order like their
-// bit patterns).
-__device__ __forceinline__ void wave_max_store(float m, float* vmax) {
-  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-  if (__lane_id() == 0) {
-    unsigned* vm = reinterpret_cast<unsigned*>(vmax);
-    if (__float_as_uint(m) > __atomic_load_n(vm, __ATOMIC_RELAXED)) atomicMax(vm, __float_as_uint(m));
-  }
-}
-
 // ------------------------------------------------ tile-staged VJP gather --
 // Per record (particle p) with the three force meshes F staged in LDS:
 //   acc_p     = sum_j F[c_j] w_j                         (gather, tree order)
@@ -99,38 +73,19 @@ __global__ void __launch_bounds__(ADJ_TPB)
       const uint32_t q = q0 + u * ADJ_TPB + threadIdx.x;
       const float4 r = rr[u];
       const long long p = pp[u];
-      int lx, ly, lz;
-      pmx_unpack_local(r.w, lx, ly, lz);
-      if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
-      float wa[2], wb[2], wc[2], sa[2], sb[2], sc[2];
-      frac_sign_of(r.x, wa[0], wa[1], sa[0], sa[1]);
-      frac_sign_of(r.y, wb[0], wb[1], sb[0], sb[1]);
-      frac_sign_of(r.z, wc[0], wc[1], sc[0], sc[1]);
-      const int b = (lx * LY + ly) * LZ + lz;
+      int l[3];
+      pmx_unpack_local(r.w, l[0], l[1], l[2]);
+      if (l[0] >= s[0] || l[1] >= s[1] || l[2] >= s[2]) continue;
+      const float d[3] = {r.x, r.y, r.z};
       const float v0 = vc[u][0], v1 = vc[u][1], v2 = vc[u][2];
-      float r0[8], r1[8], r2[8], gx[8], gy[8], gz[8];
+      float o[3], ot[3];
+      pmx_vjp_one(sh, ltot, LY, LZ, d, l, vc[u], g.cell, o, ot);
+      const float o0 = o[0], o1 = o[1], o2 = o[2];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int ja = j & 1, jb = (j >> 1) & 1, jc = (j >> 2) & 1;
-        const float f0 = wa[ja], f1 = wb[jb], f2 = wc[jc];
-        const float w = (f0 * f1) * f2;
-        const int li = b + (ja ? LY * LZ : 0) + (jb ? LZ : 0) + jc;
-        const float a0 = sh[li], a1 = sh[ltot + li], a2 = sh[2 * ltot + li];
-        r0[j] = a0 * w;
-        r1[j] = a1 * w;
-        r2[j] = a2 * w;
-        const float dc = v0 * a0 + v1 * a1 + v2 * a2;
-        gx[j] = dc * (sa[ja] * (f1 * f2));
-        gy[j] = dc * (sb[jb] * (f2 * f0));
-        gz[j] = dc * (sc[jc] * (f0 * f1));
+      for (int a = 0; a < 3; ++a) {
+        acc[3 * p + a] = o[a];
+        acc_cot[3 * p + a] = ot[a];
       }
-      const float o0 = pmx_tree8(r0), o1 = pmx_tree8(r1), o2 = pmx_tree8(r2);
-      acc[3 * p + 0] = o0;
-      acc[3 * p + 1] = o1;
-      acc[3 * p + 2] = o2;
-      acc_cot[3 * p + 0] = pmx_tree8(gx) / g.cell;
-      acc_cot[3 * p + 1] = pmx_tree8(gy) / g.cell;
-      acc_cot[3 * p + 2] = pmx_tree8(gz) / g.cell;
       vrec[start + q] = v0;
       vrec[vstride + start + q] = v1;
       vrec[2 * vstride + start + q] = v2;
@@ -190,28 +145,14 @@ __global__ void __launch_bounds__(ADJ_TPB)
       if (pp[u] < 0) continue;
       const float4 r = rr[u];
       const long long p = pp[u];
-      int lx, ly, lz;
-      pmx_unpack_local(r.w, lx, ly, lz);
-      if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
-      float wa[2], wb[2], wc[2], sa[2], sb[2], sc[2];
-      frac_sign_of(r.x, wa[0], wa[1], sa[0], sa[1]);
-      frac_sign_of(r.y, wb[0], wb[1], sb[0], sb[1]);
-      frac_sign_of(r.z, wc[0], wc[1], sc[0], sc[1]);
-      const int b = (lx * LY + ly) * LZ + lz;
-      float gx[8], gy[8], gz[8];
+      int l[3];
+      pmx_unpack_local(r.w, l[0], l[1], l[2]);
+      if (l[0] >= s[0] || l[1] >= s[1] || l[2] >= s[2]) continue;
+      const float d[3] = {r.x, r.y, r.z};
+      float gr[3];
+      pmx_grad_one(sh, LY, LZ, d, l, val, g.cell, gr);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int ja = j & 1, jb = (j >> 1) & 1, jc = (j >> 2) & 1;
-        const float f0 = wa[ja], f1 = wb[jb], f2 = wc[jc];
-        const int li = b + (ja ? LY * LZ : 0) + (jb ? LZ : 0) + jc;
-        const float dc = sh[li] * val;
-        gx[j] = dc * (sa[ja] * (f1 * f2));
-        gy[j] = dc * (sb[jb] * (f2 * f0));
-        gz[j] = dc * (sc[jc] * (f0 * f1));
-      }
-      acc_cot[3 * p + 0] = ac[u][0] + pmx_tree8(gx) / g.cell;
-      acc_cot[3 * p + 1] = ac[u][1] + pmx_tree8(gy) / g.cell;
-      acc_cot[3 * p + 2] = ac[u][2] + pmx_tree8(gz) / g.cell;
+      for (int a = 0; a < 3; ++a) acc_cot[3 * p + a] = ac[u][a] + gr[a];
     }
   }
 }
@@ -413,6 +354,30 @@ __global__ void __launch_bounds__(PMX_TPB)
   block_sum_store<PMX_TPB>(s, part);
 }
 
+int pmx_kd_adj_bin(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, const float* acc,
+                   float* disp_cot, float* vel_cot, const float* acc_cot, const float* disp_snap,
+                   long long n, float kp, float ka, float d, uint32_t* count, const BinOut& o,
+                   int cap, double* part) {
+  const MeshGeom g = pmx_geom_padded(c);
+  const unsigned nb1 = pmx_blocks(n);
+#define KD_ADJ_LAUNCH(KP, CAPV, KPV)                                                          \
+  kd_adj_bin_k<KP, CAPV><<<nb1, PMX_TPB, 0, c->stream>>>(pmid, disp, vel, acc, disp_cot,       \
+                                                         vel_cot, acc_cot, disp_snap, n, KPV, ka, \
+                                                         d, g, c->tg, count, o, part)
+  pmx_tic(c, PMX_T_KDADJ);
+  if (kp != 0.f) {
+    if (cap) KD_ADJ_LAUNCH(1, true, kp);
+    else KD_ADJ_LAUNCH(1, false, kp);
+  } else {
+    if (cap) KD_ADJ_LAUNCH(0, true, 0.f);
+    else KD_ADJ_LAUNCH(0, false, 0.f);
+  }
+#undef KD_ADJ_LAUNCH
+  pmx_toc(c, PMX_T_KDADJ);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
 extern "C" int pmx_nbody_step_adj(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel,
                                   float* acc, float* disp_cot, float* vel_cot, float* acc_cot,
                                   long long n, float val, float coef, float kp, float ka, float d,
@@ -429,23 +394,10 @@ extern "C" int pmx_nbody_step_adj(pmx_ctx* c, const int16_t* pmid, float* disp, 
   const unsigned nb1 = pmx_blocks(n);
   if ((rc = pmx_ensure_red(c, (long long)nb1 + c->ntiles))) return rc;
   if ((rc = pmx_bin_begin(c, n, &cap))) return rc;
-  const MeshGeom g = pmx_geom_padded(c);
   const BinOut o = pmx_bin_out(c);
-#define KD_ADJ_LAUNCH(KP, CAPV, KPV)                                                          \
-  kd_adj_bin_k<KP, CAPV><<<nb1, PMX_TPB, 0, c->stream>>>(pmid, disp, vel, acc, disp_cot,       \
-                                                         vel_cot, acc_cot, disp_snap, n, KPV, ka, \
-                                                         d, g, c->tg, c->tile_count, o, c->red)
-  pmx_tic(c, PMX_T_KDADJ);
-  if (kp != 0.f) {
-    if (cap) KD_ADJ_LAUNCH(1, true, kp);
-    else KD_ADJ_LAUNCH(1, false, kp);
-  } else {
-    if (cap) KD_ADJ_LAUNCH(0, true, 0.f);
-    else KD_ADJ_LAUNCH(0, false, 0.f);
-  }
-#undef KD_ADJ_LAUNCH
-  pmx_toc(c, PMX_T_KDADJ);
-  PMX_LAUNCH_CHECK();
+  if ((rc = pmx_kd_adj_bin(c, pmid, disp, vel, acc, disp_cot, vel_cot, acc_cot, disp_snap, n, kp,
+                            ka, d, c->tile_count, o, cap, c->red)))
+    return rc;
   // disp now holds the binned positions (disp_snap when given)
   if ((rc = pmx_bin_end(c, pmid, disp, n, cap))) return rc;
   if ((rc = pmx_deposit_binned(c, val, n, (float*)c->cbuf[0], c->rs, 0))) return rc;

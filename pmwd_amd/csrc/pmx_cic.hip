@@ -168,10 +168,6 @@ __device__ __forceinline__ double fixed_scale(float vmax) {
   return ldexp(1.0, k);
 }
 
-// Output meshes of a deposit (one per channel).
-struct DepOut {
-  float* mesh[3];
-};
 
 // One workgroup per tile: LDS accumulation of val*w (or vrec[ch][slot]*w)
 // of the tile's binned records over the (sx+1)(sy+1)(sz+1) cells they touch;
@@ -186,7 +182,7 @@ struct DepOut {
 template <int PERVAL, int STATE = 0>
 __global__ void __launch_bounds__(DEP_TPB)
     deposit_k(const float4* __restrict__ rec, const uint16_t* __restrict__ pz,
-              const float* __restrict__ vrec, long long vstride,
+              const float* __restrict__ vrec, long long vstride, int vstep,
               const float* __restrict__ vmaxp, const uint32_t* __restrict__ tile_start,
               const uint32_t* __restrict__ tile_count, MeshGeom g, TileGeom tg, float val,
               DepOut out, int nch, int mrs, int accumulate, float* __restrict__ halo,
@@ -215,7 +211,7 @@ __global__ void __launch_bounds__(DEP_TPB)
         if (q < cnt) {
           r[u] = rec[start + q];
           if (STATE) zz[u] = pz[start + q];
-          v[u] = PERVAL ? vr[start + q] : val;
+          v[u] = PERVAL ? vr[(long long)(start + q) * vstep] : val;
         }
       }
 #pragma unroll
@@ -516,11 +512,12 @@ static int deposit_run(pmx_ctx* c, float val, int perval, int nch, long long n, 
   if ((rc = pmx_ensure_halo(c, nch))) return rc;
   pmx_tic(c, PMX_T_DEPOSIT);
   if (perval)
-    deposit_k<1><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, nullptr, c->vrec, c->rec_cap, c->vmax,
-                                                  c->tile_start, c->tile_count, g, tg, val, out,
-                                                  nch, mrs, accumulate, c->halo, hstride);
+    deposit_k<1><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, nullptr, c->vrec, c->rec_cap, 1,
+                                                  c->vmax, c->tile_start, c->tile_count, g, tg,
+                                                  val, out, nch, mrs, accumulate, c->halo,
+                                                  hstride);
   else
-    deposit_k<0><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, nullptr, nullptr, 0, nullptr,
+    deposit_k<0><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, nullptr, nullptr, 0, 1, nullptr,
                                                   c->tile_start, c->tile_count, g, tg, val, out,
                                                   nch, mrs, accumulate, c->halo, hstride);
   pmx_toc(c, PMX_T_DEPOSIT);
@@ -550,19 +547,36 @@ int pmx_deposit_binned3(pmx_ctx* c, long long n, float* const* mesh, int mrs) {
 // mesh (row stride mrs), overwriting it.
 int pmx_deposit_state(pmx_ctx* c, const PState& st, const uint32_t* start, const uint32_t* count,
                       float val, float* mesh, int mrs) {
+  DepOut out = {{mesh, mesh, mesh}};
+  return pmx_deposit_state_vals(c, st, start, count, val, nullptr, 0, 0, 1, out, mrs);
+}
+
+// Deposits tile-ordered particles into nch meshes (row stride mrs),
+// overwriting them: val each, or (vals) channel ch of slot i at
+// vals[ch * vstride + i * vstep] with the fixed-point scale from c->vmax.
+int pmx_deposit_state_vals(pmx_ctx* c, const PState& st, const uint32_t* start,
+                           const uint32_t* count, float val, const float* vals, long long vstride,
+                           int vstep, int nch, DepOut out, int mrs) {
   const MeshGeom g = pmx_geom_padded(c);
   const TileGeom& tg = c->tg;
   const size_t lds = (size_t)(tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(unsigned long long);
   const unsigned nt = (unsigned)c->ntiles;
   const long long hstride = c->ntiles * tg.shell;
-  DepOut out = {{mesh, mesh, mesh}};
+  int rc;
+  if ((rc = pmx_ensure_halo(c, nch))) return rc;
   pmx_tic(c, PMX_T_DEPOSIT);
-  deposit_k<0, 1><<<nt, DEP_TPB, lds, c->stream>>>(st.s0, st.pz, nullptr, 0, nullptr, start, count,
-                                                   g, tg, val, out, 1, mrs, 0, c->halo, hstride);
+  if (vals)
+    deposit_k<1, 1><<<nt, DEP_TPB, lds, c->stream>>>(st.s0, st.pz, vals, vstride, vstep, c->vmax,
+                                                     start, count, g, tg, 0.f, out, nch, mrs, 0,
+                                                     c->halo, hstride);
+  else
+    deposit_k<0, 1><<<nt, DEP_TPB, lds, c->stream>>>(st.s0, st.pz, nullptr, 0, 1, nullptr, start,
+                                                     count, g, tg, val, out, nch, mrs, 0, c->halo,
+                                                     hstride);
   pmx_toc(c, PMX_T_DEPOSIT);
   PMX_LAUNCH_CHECK();
   pmx_tic(c, PMX_T_FOLD);
-  fold_k<<<nt, PMX_TPB, 0, c->stream>>>(out, 1, mrs, c->halo, hstride, g, tg);
+  fold_k<<<nt, PMX_TPB, 0, c->stream>>>(out, nch, mrs, c->halo, hstride, g, tg);
   pmx_toc(c, PMX_T_FOLD);
   PMX_LAUNCH_CHECK();
   return PMX_OK;

@@ -41,6 +41,8 @@ static void dev_free(pmx_ctx* c, void** p, size_t bytes) {
   *p = nullptr;
 }
 
+static void free_adj_state(pmx_ctx* c);
+
 // Scatter tile extent: 16^3 cells, clipped to the mesh (the 17^3 LDS block
 // of 64-bit accumulators is 39 KB, four workgroups per CU).
 static TileGeom make_tiles(const int n[3]) {
@@ -180,9 +182,10 @@ extern "C" int pmx_ctx_destroy(pmx_ctx* c) {
                 c->scan_tmp};
   for (void* p : ps)
     if (p) hipFree(p);
-  // pl[1], pl_tile and pl_rank alias the scratch freed above
+  // pl[1], al[1], pl_tile and pl_rank alias the scratch freed above
   void* qs[] = {c->pl[0].s0, c->pl[0].s1, c->pl[0].pz, c->pl_count[0], c->pl_start[0],
-                c->pl_count[1], c->pl_start[1]};
+                c->pl_count[1], c->pl_start[1], c->al[0].p.s0, c->al[0].p.s1, c->al[0].p.pz,
+                c->al[0].c0, c->al[0].c1, c->abuf[0], c->abuf[1], c->abuf[2], c->al_dots};
   for (void* p : qs)
     if (p) hipFree(p);
   if (c->red_host) hipHostFree(c->red_host);
@@ -213,9 +216,11 @@ extern "C" int pmx_ctx_trim(pmx_ctx* c, int what) {
       dev_free(c, (void**)&c->pl[0].s1, c->pl_cap * 16);
       dev_free(c, (void**)&c->pl[0].pz, c->pl_cap * 2);
       c->pl_cap = 0;
+      free_adj_state(c);
     }
     return PMX_OK;
   }
+  free_adj_state(c);
   dev_free(c, (void**)&c->rec, c->rec_cap * 16);
   dev_free(c, (void**)&c->pidx, c->rec_cap * 4);
   dev_free(c, (void**)&c->pkey, c->rec_cap * 4);
@@ -255,6 +260,7 @@ int pmx_ensure_ptcl(pmx_ctx* c, long long n) {
   dev_free(c, (void**)&c->pl[0].s0, c->pl_cap * 16);
   dev_free(c, (void**)&c->pl[0].s1, c->pl_cap * 16);
   dev_free(c, (void**)&c->pl[0].pz, c->pl_cap * 2);
+  free_adj_state(c);
   c->ptcl_cap = c->rec_cap = c->vrec_cap = c->pl_cap = 0;
   c->caps_valid = 0;
   // pkey/prank hold a tile and rank per particle, or (multi-step re-bin) per record slot
@@ -307,6 +313,61 @@ int pmx_ensure_state(pmx_ctx* c) {
   c->pl_tile = c->pkey;
   c->pl_rank = c->prank;
   return PMX_OK;
+}
+
+// The reverse-time pipeline's own particle copy and per-slot gather
+// results (rec_cap slots each), its per-step dot products, and its second
+// copy in the per-step scratch and the forward pipeline's copy: p.s0 in rec,
+// p.s1 in the four vrec channels, p.pz in pidx, c0/c1 in the forward copy's
+// s0/s1.
+int pmx_ensure_adj_state(pmx_ctx* c, int nsteps) {
+  int rc;
+  if ((rc = pmx_ensure_state(c))) return rc;
+  const long long cap = c->rec_cap;
+  if (cap > c->al_cap) {
+    AState& a = c->al[0];
+    const long long old = c->al_cap;
+    dev_free(c, (void**)&a.p.s0, old * 16);
+    dev_free(c, (void**)&a.p.s1, old * 16);
+    dev_free(c, (void**)&a.p.pz, old * 2);
+    dev_free(c, (void**)&a.c0, old * 16);
+    dev_free(c, (void**)&a.c1, old * 16);
+    c->al_cap = 0;
+    rc = dev_alloc(c, (void**)&a.p.s0, cap * 16);
+    rc |= dev_alloc(c, (void**)&a.p.s1, cap * 16);
+    rc |= dev_alloc(c, (void**)&a.p.pz, cap * 2);
+    rc |= dev_alloc(c, (void**)&a.c0, cap * 16);
+    rc |= dev_alloc(c, (void**)&a.c1, cap * 16);
+    if (rc) return PMX_ENOMEM;
+    c->al_cap = cap;
+  }
+  for (int b = 0; b < 3; ++b)
+    if (!c->abuf[b] && dev_alloc(c, (void**)&c->abuf[b], (size_t)c->Mc * sizeof(float2)))
+      return PMX_ENOMEM;
+  if (2LL * nsteps > c->al_dots_cap) {
+    dev_free(c, (void**)&c->al_dots, c->al_dots_cap * 8);
+    c->al_dots_cap = 0;
+    const long long k = 2LL * nsteps + 64;
+    if (dev_alloc(c, (void**)&c->al_dots, k * 8)) return PMX_ENOMEM;
+    c->al_dots_cap = k;
+  }
+  c->al[1].p = c->pl[1];
+  c->al[1].c0 = c->pl[0].s0;
+  c->al[1].c1 = c->pl[0].s1;
+  return PMX_OK;
+}
+
+static void free_adj_state(pmx_ctx* c) {
+  AState& a = c->al[0];
+  const long long cap = c->al_cap;
+  dev_free(c, (void**)&a.p.s0, cap * 16);
+  dev_free(c, (void**)&a.p.s1, cap * 16);
+  dev_free(c, (void**)&a.p.pz, cap * 2);
+  dev_free(c, (void**)&a.c0, cap * 16);
+  dev_free(c, (void**)&a.c1, cap * 16);
+  for (int b = 0; b < 3; ++b) dev_free(c, (void**)&c->abuf[b], (size_t)c->Mc * sizeof(float2));
+  c->al_cap = 0;
+  c->al[1] = AState{};
 }
 
 // per-record value channels, rec_cap floats apart: three for the adjoint's
@@ -473,6 +534,13 @@ int pmx_red_reduce(pmx_ctx* c, long long off, long long nblk, int slot) {
   PMX_REQUIRE(slot >= 0 && slot < PMX_RED_SLOTS && off >= 0 && off + nblk <= c->red_cap,
               "pmx_red_reduce: bad range");
   red_sum_k<<<1, RED_TPB, 0, c->stream>>>(c->red + off, nblk, c->red_out + slot);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+int pmx_red_reduce_to(pmx_ctx* c, long long off, long long nblk, double* dst) {
+  PMX_REQUIRE(dst && off >= 0 && off + nblk <= c->red_cap, "pmx_red_reduce_to: bad range");
+  red_sum_k<<<1, RED_TPB, 0, c->stream>>>(c->red + off, nblk, dst);
   PMX_LAUNCH_CHECK();
   return PMX_OK;
 }

@@ -914,8 +914,13 @@ int pmx_poisson_ctx(pmx_ctx* c, float coef) {
 // cotangent meshes (padded real); on return cbuf[0] holds the density
 // cotangent (padded real) times `scale_out`.
 int pmx_poisson_adj_ctx(pmx_ctx* c, float scale_out) {
-  int rc;
   float2* f[3] = {c->cbuf[0], c->cbuf[1], c->cbuf[2]};
+  return pmx_poisson_adj_bufs(c, f, scale_out);
+}
+
+// The same on three other mesh buffers of the context's layout.
+int pmx_poisson_adj_bufs(pmx_ctx* c, float2* const* f, float scale_out) {
+  int rc;
   for (int d = 0; d < 3; ++d)
     if ((rc = pmx_fft_r2c_inplace(c, (float*)f[d], 0, 0.f))) return rc;
   if ((rc = pmx_fft_strided(c, f, 3, 1, -1))) return rc;

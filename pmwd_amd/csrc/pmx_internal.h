@@ -83,6 +83,14 @@ struct PState {
   uint16_t* pz;
 };
 
+// The reverse-time pipeline's particle: a PState plus c0 = (disp_cot, 0)
+// and c1 = (vel_cot, 0).
+struct AState {
+  PState p;
+  float4* c0;
+  float4* c1;
+};
+
 // ---------------------------------------------------------- kernel timing --
 // Optional per-kernel-class timing with HIP events on the context stream.
 enum {
@@ -156,6 +164,13 @@ struct pmx_ctx {
   uint32_t* pl_tile;
   uint32_t* pl_rank;
   long long pl_cap;    // slots of the pipeline's own copy
+  // reverse-time pipeline: its own particle copy (the second aliases rec,
+  // vrec, pidx and the forward pipeline's copy) and three mesh buffers
+  AState al[2];
+  long long al_cap;
+  float2* abuf[3];     // the reverse pipeline's cotangent meshes (F stays in cbuf)
+  double* al_dots;     // per-step dot products of one reverse run (device)
+  long long al_dots_cap;
   float* halo;         // nch shells per tile, channel-major
   int halo_nch;        // channels the halo buffer holds
   long long ptcl_cap;
@@ -563,6 +578,121 @@ __device__ __forceinline__ void pmx_tile_acc(const float* sh, int ltot, int LY, 
   out[2] = pmx_tree8(v2);
 }
 
+// Force VJP of one binned particle over a staged tile of the three force
+// meshes (gather.py:80-116 for three components): acc = sum_j F[c_j] w_j and
+// acot = sum_j (v . F[c_j]) grad w_j / cell, v = vel_cot, each in the tree
+// order of pmx_tree8.
+__device__ __forceinline__ void pmx_vjp_one(const float* sh, int ltot, int LY, int LZ,
+                                            const float d[3], const int l[3], const float v[3],
+                                            float cell, float acc[3], float acot[3]) {
+  float wa[2], wb[2], wc[2], sa[2], sb[2], sc[2];
+  frac_sign_of(d[0], wa[0], wa[1], sa[0], sa[1]);
+  frac_sign_of(d[1], wb[0], wb[1], sb[0], sb[1]);
+  frac_sign_of(d[2], wc[0], wc[1], sc[0], sc[1]);
+  const int b = (l[0] * LY + l[1]) * LZ + l[2];
+  float r0[8], r1[8], r2[8], gx[8], gy[8], gz[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int ja = j & 1, jb = (j >> 1) & 1, jc = (j >> 2) & 1;
+    const float f0 = wa[ja], f1 = wb[jb], f2 = wc[jc];
+    const float w = (f0 * f1) * f2;
+    const int li = b + (ja ? LY * LZ : 0) + (jb ? LZ : 0) + jc;
+    const float a0 = sh[li], a1 = sh[ltot + li], a2 = sh[2 * ltot + li];
+    r0[j] = a0 * w;
+    r1[j] = a1 * w;
+    r2[j] = a2 * w;
+    const float dc = v[0] * a0 + v[1] * a1 + v[2] * a2;
+    gx[j] = dc * (sa[ja] * (f1 * f2));
+    gy[j] = dc * (sb[jb] * (f2 * f0));
+    gz[j] = dc * (sc[jc] * (f0 * f1));
+  }
+  acc[0] = pmx_tree8(r0);
+  acc[1] = pmx_tree8(r1);
+  acc[2] = pmx_tree8(r2);
+  acot[0] = pmx_tree8(gx) / cell;
+  acot[1] = pmx_tree8(gy) / cell;
+  acot[2] = pmx_tree8(gz) / cell;
+}
+
+// Scatter-adjoint disp cotangent of one binned particle over a staged tile
+// of the density cotangent R (scatter.py:86-121, scalar val):
+// sum_j (R[c_j] val) grad w_j / cell per axis, tree order.
+__device__ __forceinline__ void pmx_grad_one(const float* sh, int LY, int LZ, const float d[3],
+                                             const int l[3], float val, float cell,
+                                             float out[3]) {
+  float wa[2], wb[2], wc[2], sa[2], sb[2], sc[2];
+  frac_sign_of(d[0], wa[0], wa[1], sa[0], sa[1]);
+  frac_sign_of(d[1], wb[0], wb[1], sb[0], sb[1]);
+  frac_sign_of(d[2], wc[0], wc[1], sc[0], sc[1]);
+  const int b = (l[0] * LY + l[1]) * LZ + l[2];
+  float gx[8], gy[8], gz[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int ja = j & 1, jb = (j >> 1) & 1, jc = (j >> 2) & 1;
+    const float f0 = wa[ja], f1 = wb[jb], f2 = wc[jc];
+    const int li = b + (ja ? LY * LZ : 0) + (jb ? LZ : 0) + jc;
+    const float dc = sh[li] * val;
+    gx[j] = dc * (sa[ja] * (f1 * f2));
+    gy[j] = dc * (sb[jb] * (f2 * f0));
+    gz[j] = dc * (sc[jc] * (f0 * f1));
+  }
+  out[0] = pmx_tree8(gx) / cell;
+  out[1] = pmx_tree8(gy) / cell;
+  out[2] = pmx_tree8(gz) / cell;
+}
+
+// Fixed-order block sum of s (wave butterflies, then wave sums in order)
+// stored to part[blockIdx.x].
+template <int TPB>
+__device__ __forceinline__ void block_sum_store(double s, double* __restrict__ part) {
+  __shared__ double wsum[TPB / 64];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (__lane_id() == 0) wsum[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < TPB / 64; ++w) t += wsum[w];
+    part[blockIdx.x] = t;
+  }
+}
+
+// Two fixed-order block sums at once (one barrier), stored to
+// pa[blockIdx.x] and pb[blockIdx.x].
+template <int TPB>
+__device__ __forceinline__ void block_sum2_store(double a, double b, double* __restrict__ pa,
+                                                 double* __restrict__ pb) {
+  __shared__ double wsum2[2][TPB / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+  }
+  if (__lane_id() == 0) {
+    wsum2[0][threadIdx.x >> 6] = a;
+    wsum2[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ta = 0, tb = 0;
+    for (int w = 0; w < TPB / 64; ++w) {
+      ta += wsum2[0][w];
+      tb += wsum2[1][w];
+    }
+    pa[blockIdx.x] = ta;
+    pb[blockIdx.x] = tb;
+  }
+}
+
+// Running max of |v| per channel: wave max, then one atomic per wave only
+// when it raises the stored maximum (non-negative floats order like their
+// bit patterns).
+__device__ __forceinline__ void wave_max_store(float m, float* vmax) {
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if (__lane_id() == 0) {
+    unsigned* vm = reinterpret_cast<unsigned*>(vmax);
+    if (__float_as_uint(m) > __atomic_load_n(vm, __ATOMIC_RELAXED)) atomicMax(vm, __float_as_uint(m));
+  }
+}
+
 // ------------------------------------------------ tile-ordered particles --
 __device__ __forceinline__ void pmx_state_pm(float4 s0, uint16_t pz, int pm[3]) {
   const uint32_t b = __float_as_uint(s0.w);
@@ -589,6 +719,53 @@ __device__ __forceinline__ bool pmx_state_local(float4 s0, uint16_t pz, const Me
   }
   return (unsigned)l[0] < (unsigned)s[0] && (unsigned)l[1] < (unsigned)s[1] &&
          (unsigned)l[2] < (unsigned)s[2];
+}
+
+// Moves between the tile-ordered copies of the multi-step pipelines.  A
+// workgroup owns one source tile; per round its particles bound for each of
+// the 27 tiles around it (code (cx*3 + cy)*3 + cz, c = 0/1/2 for -1/0/+1
+// with periodic wrap) are counted in LDS and take one contiguous run of the
+// destination tile from a single global atomic; code 27 (farther) ranks per
+// wave on the global counters.
+__device__ __forceinline__ int pl_nbr_code(const CicAxis ax[3], const int ti[3],
+                                           const TileGeom& tg) {
+  const unsigned tm[3] = {tg.mx, tg.my, tg.mz};
+  const int ntl[3] = {tg.ntx, tg.nty, tg.ntz};
+  int cc = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int dl = pmx_tdiv(ax[a].lo, tm[a]) - ti[a];
+    int ca = 3;
+    if (dl == 0) ca = 1;
+    else if (dl == 1 || dl == 1 - ntl[a]) ca = 2;
+    else if (dl == -1 || dl == ntl[a] - 1) ca = 0;
+    cc = ca == 3 || cc == 27 ? 27 : cc * 3 + ca;
+  }
+  return cc;
+}
+
+// Threads 0..26: this round's LDS count of neighbour code t becomes a run
+// base in count[] of that tile (one atomic), and the count is cleared.
+// Call between two barriers.
+__device__ __forceinline__ void pl_round_bases(uint32_t* s_cnt, uint32_t* s_base, const int ti[3],
+                                               const TileGeom& tg, uint32_t* count) {
+  if (threadIdx.x >= 27) return;
+  const int ntl[3] = {tg.ntx, tg.nty, tg.ntz};
+  const uint32_t k = s_cnt[threadIdx.x];
+  uint32_t base = 0;
+  if (k) {
+    int c3 = threadIdx.x, nb[3];
+#pragma unroll
+    for (int a = 2; a >= 0; --a) {
+      const int v = ti[a] + c3 % 3 - 1;
+      nb[a] = v < 0 ? v + ntl[a] : (v >= ntl[a] ? v - ntl[a] : v);
+      c3 /= 3;
+    }
+    const uint32_t tile = ((uint32_t)nb[0] * tg.nty + (uint32_t)nb[1]) * tg.ntz + (uint32_t)nb[2];
+    base = atomicAdd(&count[tile], k);
+  }
+  s_base[threadIdx.x] = base;
+  s_cnt[threadIdx.x] = 0u;
 }
 
 // Unpacks a binned record's tile-local lower corner (lx | ly << 10 | lz << 20).
@@ -618,15 +795,31 @@ int pmx_bin_end(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
 int pmx_deposit_binned(pmx_ctx* c, float val, long long n, float* mesh, int mrs, int accumulate);
 int pmx_deposit_state(pmx_ctx* c, const PState& st, const uint32_t* start, const uint32_t* count,
                       float val, float* mesh, int mrs);
+// Output meshes of a deposit (one per channel).
+struct DepOut {
+  float* mesh[3];
+};
+int pmx_deposit_state_vals(pmx_ctx* c, const PState& st, const uint32_t* start,
+                           const uint32_t* count, float val, const float* vals, long long vstride,
+                           int vstep, int nch, DepOut out, int mrs);
 // kick (k1) and drift (d) of the caller's arrays, binned into count/o (cap:
 // capacity mode)
 int pmx_kick_drift_bin(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, const float* acc,
                        long long n, float k1, float d, uint32_t* count, const BinOut& o, int cap);
 int pmx_ensure_state(pmx_ctx* c);
+int pmx_ensure_adj_state(pmx_ctx* c, int nsteps);
+// reverse kick(s) and drift of the caller's arrays (pending closing kick kp
+// first when nonzero), binned into count/o; per-block partials of the drift
+// dot product at part[0, pmx_blocks(n))
+int pmx_kd_adj_bin(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, const float* acc,
+                   float* disp_cot, float* vel_cot, const float* acc_cot, const float* disp_snap,
+                   long long n, float kp, float ka, float d, uint32_t* count, const BinOut& o,
+                   int cap, double* part);
 int pmx_deposit_binned3(pmx_ctx* c, long long n, float* const* mesh, int mrs);
 int pmx_ensure_halo(pmx_ctx* c, int nch);
 int pmx_poisson_ctx(pmx_ctx* c, float coef);
 int pmx_poisson_adj_ctx(pmx_ctx* c, float scale_out);
+int pmx_poisson_adj_bufs(pmx_ctx* c, float2* const* f, float scale_out);
 // bins the particles, deposits the density and solves for the three force
 // meshes in c->cbuf (padded); then the tile-staged gather of those meshes
 int pmx_gravity_fwd_meshes(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
@@ -636,6 +829,8 @@ int pmx_tile_gather(pmx_ctx* c, float* acc, float* vel, float k2);
 // device slot `slot` (< PMX_RED_SLOTS), then fetch slots [0, k) to the host
 #define PMX_RED_SLOTS 8
 int pmx_red_reduce(pmx_ctx* c, long long off, long long nblk, int slot);
+// the same into a device double
+int pmx_red_reduce_to(pmx_ctx* c, long long off, long long nblk, double* dst);
 int pmx_red_fetch(pmx_ctx* c, int k, double* out);
 int pmx_red_finish(pmx_ctx* c, long long nblk, double* out);
 // grid of the exact re-binning launched after a capacity pass

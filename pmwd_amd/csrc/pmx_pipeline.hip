@@ -115,10 +115,8 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
               PlUser user) {
   if ((MODE == PL_COUNT || MODE == PL_WRITE) && out.flag[0] == 0u) return;
   extern __shared__ float sh[];
-  // PL_CAP: per round, particles moving to each of the 27 tiles around this
-  // one are counted in LDS (wave-aggregated) and take one contiguous run per
-  // tile from a single global atomic, so a round's stores into a tile are
-  // one run instead of a short run per wave; farther moves rank per wave
+  // PL_CAP: runs per neighbour tile and round (pl_nbr_code, pl_round_bases),
+  // so a round's stores into a tile are one run instead of one per wave
   __shared__ uint32_t s_cnt[27], s_base[27];
   int o[3], s[3], ti[3];
   const int t = (int)pmx_xcd_block(blockIdx.x, gridDim.x);
@@ -131,7 +129,6 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
   __syncthreads();
   const uint32_t start = in_start[t];
   const uint32_t cnt = in_count[t];
-  const int ntl[3] = {tg.ntx, tg.nty, tg.ntz};
   for (uint32_t q0 = 0; q0 < cnt; q0 += PL_TPB * PL_U) {
     float4 a0[PL_U], a1[PL_U], n0[PL_U], n1[PL_U];
     uint16_t az[PL_U];
@@ -185,21 +182,7 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
           nt[u] = tile_of(ax, tg);
           n0[u] = make_float4(x[0], x[1], x[2], a0[u].w);
           n1[u] = make_float4(v[0], v[1], v[2], a1[u].w);
-          if (MODE == PL_CAP) {
-            // neighbour code 0..26 of the new tile, 27 when farther
-            const unsigned tm[3] = {tg.mx, tg.my, tg.mz};
-            int cc = 0;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              const int dl = pmx_tdiv(ax[a].lo, tm[a]) - ti[a];
-              int ca = 3;
-              if (dl == 0) ca = 1;
-              else if (dl == 1 || dl == 1 - ntl[a]) ca = 2;
-              else if (dl == -1 || dl == ntl[a] - 1) ca = 0;
-              cc = ca == 3 || cc == 27 ? 27 : cc * 3 + ca;
-            }
-            code[u] = cc;
-          }
+          if (MODE == PL_CAP) code[u] = pl_nbr_code(ax, ti, tg);
         }
       }
     }
@@ -237,23 +220,7 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
       rk[u] = near ? rl : rf;
     }
     __syncthreads();
-    if (threadIdx.x < 27) {
-      const uint32_t k = s_cnt[threadIdx.x];
-      uint32_t base = 0;
-      if (k) {
-        int c3 = threadIdx.x, nb[3];
-#pragma unroll
-        for (int a = 2; a >= 0; --a) {
-          int v = ti[a] + c3 % 3 - 1;
-          nb[a] = v < 0 ? v + ntl[a] : (v >= ntl[a] ? v - ntl[a] : v);
-          c3 /= 3;
-        }
-        const uint32_t tile = ((uint32_t)nb[0] * tg.nty + (uint32_t)nb[1]) * tg.ntz + (uint32_t)nb[2];
-        base = atomicAdd(&out.count[tile], k);
-      }
-      s_base[threadIdx.x] = base;
-      s_cnt[threadIdx.x] = 0u;
-    }
+    pl_round_bases(s_cnt, s_base, ti, tg, out.count);
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < PL_U; ++u) {
@@ -354,5 +321,379 @@ extern "C" int pmx_nbody_steps(pmx_ctx* c, const int16_t* pmid, float* disp, flo
     PMX_LAUNCH_CHECK();
     A = B;
   }
+  return PMX_OK;
+}
+
+// ------------------------------------------------ reverse-time pipeline --
+// pmx_nbody_steps_adj: nsteps reverse KDK steps (pmx_nbody_step_adj chained
+// with its pending closing kick) with the particles and their cotangents in
+// tile order between steps.  Per step, over the copy binned at this step's
+// drifted positions:
+//   density deposit, Poisson: the force meshes F stay in the context buffers;
+//   three-channel deposit of vel_cot and the adjoint Poisson solve into three
+//   more buffers: the density cotangent R;
+//   al_step_k  with F and R staged: acc and the gather part of acc_cot (F),
+//              the scatter part (R), sum vel_cot.acc; then this step's
+//              closing kick adjoint, the next step's opening kick and drift
+//              adjoints, |vel_cot| maxima for the next deposit, and the move
+//              to the other copy (last step: the caller's arrays).
+// Float operations are those of the per-step path in the same order, so
+// every array is bitwise what the chained calls give; the float64 dot
+// products sum the same terms in another grouping.
+
+__device__ __forceinline__ void al_put(const AState& st, uint32_t slot, float4 s0, float4 s1,
+                                       uint16_t pz, float4 c0, float4 c1) {
+  st.p.s0[slot] = s0;
+  st.p.s1[slot] = s1;
+  st.p.pz[slot] = pz;
+  st.c0[slot] = c0;
+  st.c1[slot] = c1;
+}
+
+// Writes particle i (after the first reverse kick and drift) with its
+// cotangents to slot start[ptile[i]] + prank[i] of st; |vel_cot| maxima per
+// channel to vmax.
+__global__ void __launch_bounds__(PMX_TPB)
+    al_fill_k(const int16_t* __restrict__ pmid, const float* __restrict__ disp,
+              const float* __restrict__ vel, const float* __restrict__ disp_cot,
+              const float* __restrict__ vel_cot, long long n, MeshGeom g,
+              const uint32_t* __restrict__ ptile, const uint32_t* __restrict__ prank,
+              const uint32_t* __restrict__ start, AState st, float* __restrict__ vmax) {
+  constexpr int T3 = 3 * PMX_TPB;
+  __shared__ __align__(16) float sh[4 * T3];  // disp | vel | disp_cot | vel_cot
+  __shared__ __align__(16) int16_t sp[T3];
+  const long long b0 = (long long)blockIdx.x * PMX_TPB;
+  const int cnt = (int)(n - b0 < PMX_TPB ? n - b0 : PMX_TPB);
+  pmx_stage_in3<4, PMX_TPB>(sh, {{disp, vel, disp_cot, vel_cot}}, b0, cnt);
+  pmx_stage_pmid<PMX_TPB>(sp, pmid, b0, cnt);
+  __syncthreads();
+  const int q = threadIdx.x;
+  float vm[3] = {0.f, 0.f, 0.f};
+  if (q < cnt) {
+    const long long i = b0 + q;
+    const uint32_t px = (uint32_t)pmx_wrap(sp[3 * q], g.n0);
+    const uint32_t py = (uint32_t)pmx_wrap(sp[3 * q + 1], g.n1);
+    const uint16_t pz = (uint16_t)pmx_wrap(sp[3 * q + 2], g.n2);
+    const float* e = sh + 3 * q;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) vm[a] = fabsf(e[3 * T3 + a]);
+    al_put(st, start[ptile[i]] + prank[i],
+           make_float4(e[0], e[1], e[2], __uint_as_float(px | (py << 16))),
+           make_float4(e[T3], e[T3 + 1], e[T3 + 2], __uint_as_float((uint32_t)i)), pz,
+           make_float4(e[2 * T3], e[2 * T3 + 1], e[2 * T3 + 2], 0.f),
+           make_float4(e[3 * T3], e[3 * T3 + 1], e[3 * T3 + 2], 0.f));
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) wave_max_store(vm[a], vmax + a);
+}
+
+#define AL_TPB 512
+#ifndef AL_U
+#define AL_U 1
+#endif
+
+// Destination of a reverse pipeline step.
+struct AlOut {
+  AState st;
+  uint32_t* count;
+  const uint32_t* start;
+  uint32_t* ptile;
+  uint32_t* prank;
+  uint32_t* flag;
+};
+
+// The caller's arrays (last step).
+struct AlUser {
+  float* disp;
+  float* vel;
+  float* acc;
+  float* disp_cot;
+  float* vel_cot;
+  float* acc_cot;
+};
+
+// One workgroup per tile with the three force meshes F and the density
+// cotangent R staged: acc and acc_cot of the force VJP (pmx_vjp_one, then
+// + pmx_grad_one as the per-step path adds them); pf[block] = sum
+// vel_cot.acc.  Then (not LAST) the closing kick adjoint kp of this step
+// (skipped when 0), the next step's kick adjoint ka and drift adjoint dd
+// (position replaced by snap[index] when snap is given), pd[block] = the
+// drift adjoint's sum disp_cot.vel, |vel_cot| maxima to vmax, and the move
+// to the other copy as pl_step_k's modes; LAST writes the caller's arrays.
+// Partials and maxima come from PL_CAP and PL_LAST only.
+// (two workgroups per CU: at most 128 VGPRs, as the 79 KB of LDS allows)
+template <int MODE>
+__global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4)))
+    al_step_k(AState in, const uint32_t* __restrict__ in_start,
+              const uint32_t* __restrict__ in_count, MeshGeom g, TileGeom tg,
+              const float* __restrict__ F0, const float* __restrict__ F1,
+              const float* __restrict__ F2, const float* __restrict__ R, float val, float kp,
+              float ka, float dd, const float* __restrict__ snap, AlOut out, AlUser user,
+              float* __restrict__ vmax, double* __restrict__ pf, double* __restrict__ pd) {
+  if ((MODE == PL_COUNT || MODE == PL_WRITE) && out.flag[0] == 0u) return;
+  extern __shared__ float sh[];
+  __shared__ uint32_t s_cnt[27], s_base[27];
+  int o[3], s[3], ti[3];
+  const int t = (int)pmx_xcd_block(blockIdx.x, gridDim.x);
+  tile_origin(t, tg, g, o, s, ti);
+  const int LY = s[1] + 1, LZ = s[2] + 1;
+  const int ltot = (s[0] + 1) * LY * LZ;
+  const float* F[4] = {F0, F1, F2, R};
+  if (MODE == PL_CAP && threadIdx.x < 27) s_cnt[threadIdx.x] = 0u;
+  pmx_stage_tile<4, AL_TPB>(sh, o, LY, LZ, ltot, g, F);
+  __syncthreads();
+  const uint32_t start = in_start[t];
+  const uint32_t cnt = in_count[t];
+  double fsum = 0, dsum = 0;
+  float vm[3] = {0.f, 0.f, 0.f};
+  for (uint32_t q0 = 0; q0 < cnt; q0 += AL_TPB * AL_U) {
+    float4 a0[AL_U], a1[AL_U], c0[AL_U], c1[AL_U];
+    uint16_t az[AL_U];
+#pragma unroll
+    for (int u = 0; u < AL_U; ++u) {
+      const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
+      if (q < cnt) {
+        const uint32_t sl = start + q;
+        a0[u] = in.p.s0[sl];
+        a1[u] = in.p.s1[sl];
+        c0[u] = in.c0[sl];
+        c1[u] = in.c1[sl];
+        az[u] = in.p.pz[sl];
+      }
+    }
+    float4 n0[AL_U], n1[AL_U], m0[AL_U], m1[AL_U];
+    uint32_t nt[AL_U];
+    int code[AL_U];
+#pragma unroll
+    for (int u = 0; u < AL_U; ++u) {
+      const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
+      nt[u] = 0;
+      code[u] = 27;
+      if (q >= cnt) continue;
+      float d[3];
+      int l[3];
+      float ac[3] = {0.f, 0.f, 0.f}, at[3] = {0.f, 0.f, 0.f}, gr[3] = {0.f, 0.f, 0.f};
+      float x[3] = {a0[u].x, a0[u].y, a0[u].z}, v[3] = {a1[u].x, a1[u].y, a1[u].z};
+      float dc[3] = {c0[u].x, c0[u].y, c0[u].z}, vc[3] = {c1[u].x, c1[u].y, c1[u].z};
+      // binned by this tile, so always inside; the guard keeps LDS in bounds
+      if (pmx_state_local(a0[u], az[u], g, o, s, d, l)) {
+        pmx_vjp_one(sh, ltot, LY, LZ, d, l, vc, g.cell, ac, at);
+        pmx_grad_one(sh + 3 * ltot, LY, LZ, d, l, val, g.cell, gr);
+      }
+      if (MODE == PL_CAP || MODE == PL_LAST)
+        fsum += (double)vc[0] * ac[0] + (double)vc[1] * ac[1] + (double)vc[2] * ac[2];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) at[a] = at[a] + gr[a];
+      if (MODE == PL_LAST) {
+        const long long i = __float_as_uint(a1[u].w);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          user.disp[3 * i + a] = x[a];
+          user.vel[3 * i + a] = v[a];
+          user.disp_cot[3 * i + a] = dc[a];
+          user.vel_cot[3 * i + a] = vc[a];
+          user.acc[3 * i + a] = ac[a];
+          user.acc_cot[3 * i + a] = at[a];
+        }
+        continue;
+      }
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (kp != 0.f) {
+          v[a] = v[a] + ac[a] * kp;
+          dc[a] = dc[a] - at[a] * kp;
+        }
+        v[a] = v[a] + ac[a] * ka;
+        dc[a] = dc[a] - at[a] * ka;
+        x[a] = x[a] + v[a] * dd;
+        vc[a] = vc[a] - dc[a] * dd;
+        dsum += (double)dc[a] * v[a];
+        vm[a] = fmaxf(vm[a], fabsf(vc[a]));
+      }
+      if (snap) {
+        const long long i = __float_as_uint(a1[u].w);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) x[a] = snap[3 * i + a];
+      }
+      int pm[3];
+      pmx_state_pm(a0[u], az[u], pm);
+      CicAxis ax[3];
+      cic_setup(pm, x, g, ax);
+      nt[u] = tile_of(ax, tg);
+      if (MODE == PL_CAP) code[u] = pl_nbr_code(ax, ti, tg);
+      n0[u] = make_float4(x[0], x[1], x[2], a0[u].w);
+      n1[u] = make_float4(v[0], v[1], v[2], a1[u].w);
+      m0[u] = make_float4(dc[0], dc[1], dc[2], 0.f);
+      m1[u] = make_float4(vc[0], vc[1], vc[2], 0.f);
+    }
+    if (MODE == PL_LAST) continue;
+    if (MODE == PL_WRITE) {
+#pragma unroll
+      for (int u = 0; u < AL_U; ++u) {
+        const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
+        if (q < cnt)
+          al_put(out.st, out.start[out.ptile[start + q]] + out.prank[start + q], n0[u], n1[u],
+                 az[u], m0[u], m1[u]);
+      }
+      continue;
+    }
+    if (MODE == PL_COUNT) {
+#pragma unroll
+      for (int u = 0; u < AL_U; ++u) {
+        const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
+        const uint32_t r = agg_slot(out.count, nt[u], q < cnt);
+        if (q < cnt) {
+          out.ptile[start + q] = nt[u];
+          out.prank[start + q] = r;
+        }
+      }
+      continue;
+    }
+    uint32_t rk[AL_U];
+#pragma unroll
+    for (int u = 0; u < AL_U; ++u) {
+      const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
+      const bool near = q < cnt && code[u] < 27;
+      const uint32_t rl = agg_slot(s_cnt, near ? (uint32_t)code[u] : 0u, near);
+      const uint32_t rf = agg_slot(out.count, nt[u], q < cnt && code[u] == 27);
+      rk[u] = near ? rl : rf;
+    }
+    __syncthreads();
+    pl_round_bases(s_cnt, s_base, ti, tg, out.count);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < AL_U; ++u) {
+      const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
+      if (q >= cnt) continue;
+      const uint32_t r = code[u] < 27 ? s_base[code[u]] + rk[u] : rk[u];
+      const uint32_t b = out.start[nt[u]], e = out.start[nt[u] + 1];
+      if (r < e - b)
+        al_put(out.st, b + r, n0[u], n1[u], az[u], m0[u], m1[u]);
+      else
+        out.flag[0] = 1u;
+    }
+  }
+  if (MODE == PL_CAP) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) wave_max_store(vm[a], vmax + a);
+    block_sum2_store<AL_TPB>(fsum, dsum, pf, pd);
+  }
+  if (MODE == PL_LAST) block_sum_store<AL_TPB>(fsum, pf);
+}
+
+template <int MODE>
+static void al_launch(pmx_ctx* c, int src, float val, float kp, float ka, float dd,
+                      const float* snap, AlOut out, AlUser user) {
+  const MeshGeom g = pmx_geom_padded(c);
+  const TileGeom& tg = c->tg;
+  const size_t lds = (size_t)4 * (tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(float);
+  al_step_k<MODE><<<(unsigned)c->ntiles, AL_TPB, lds, c->stream>>>(
+      c->al[src], c->pl_start[src], c->pl_count[src], g, tg, (const float*)c->cbuf[0],
+      (const float*)c->cbuf[1], (const float*)c->cbuf[2], (const float*)c->abuf[0], val, kp, ka,
+      dd, snap, out, user, c->vmax, c->red, c->red + c->ntiles);
+}
+
+extern "C" int pmx_nbody_steps_adj(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel,
+                                   float* acc, float* disp_cot, float* vel_cot, float* acc_cot,
+                                   long long n, float val, float coef, float kp, int nsteps,
+                                   const float* ka, const float* d, const float* kb,
+                                   const float* disp_snap, double* dots) {
+  PMX_REQUIRE(c && n >= 0 && nsteps >= 0, "pmx_nbody_steps_adj: bad argument");
+  PMX_REQUIRE(n == 0 || (pmid && disp && vel && acc && disp_cot && vel_cot && acc_cot),
+              "pmx_nbody_steps_adj: NULL pointer");
+  PMX_REQUIRE(nsteps == 0 || (ka && d && kb), "pmx_nbody_steps_adj: NULL factor array");
+  PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_nbody_steps_adj: too many particles");
+  if (nsteps == 0) return PMX_OK;
+  int rc;
+  if (n == 0) {
+    if (dots)
+      for (int s = 0; s < 2 * nsteps; ++s) dots[s] = 0;
+    return PMX_OK;
+  }
+  if ((rc = pmx_ensure_ptcl(c, n))) return rc;
+  if ((rc = pmx_ensure_adj_state(c, nsteps))) return rc;
+  PMX_REQUIRE(c->rec_cap < (1LL << 32) - 1, "pmx_nbody_steps_adj: too many particles");
+  const long long nt = c->ntiles;
+  const unsigned nb1 = pmx_blocks(n);
+  if ((rc = pmx_ensure_red(c, nb1 > 2 * nt ? (long long)nb1 : 2 * nt))) return rc;
+  const MeshGeom g = pmx_geom_padded(c);
+  const float adj_scale = (float)(-(double)coef / (double)c->M);
+  uint32_t* vmax_u = reinterpret_cast<uint32_t*>(c->vmax);
+  int A = 0;
+
+  // first step's closing kick of the step before (kp), opening kick and drift
+  // adjoints on the caller's arrays, exact binning, fill copy A
+  if ((rc = pmx_fill_u32(c, c->pl_count[A], nt + 1, 0u))) return rc;
+  const BinOut bo = {c->pkey, c->prank, nullptr, nullptr, nullptr, c->flags};
+  if ((rc = pmx_kd_adj_bin(c, pmid, disp, vel, acc, disp_cot, vel_cot, acc_cot,
+                            nsteps == 1 ? disp_snap : nullptr, n, kp, ka[0], d[0],
+                            c->pl_count[A], bo, 0, c->red)))
+    return rc;
+  if ((rc = pmx_red_reduce_to(c, 0, nb1, c->al_dots))) return rc;
+  pmx_tic(c, PMX_T_SCAN);
+  rc = pmx_scan_exclusive(c, c->pl_count[A], c->pl_start[A], nt + 1);
+  pmx_toc(c, PMX_T_SCAN);
+  if (rc) return rc;
+  if ((rc = pmx_fill_u32(c, vmax_u, 3, 0u))) return rc;
+  pmx_tic(c, PMX_T_FILL);
+  al_fill_k<<<nb1, PMX_TPB, 0, c->stream>>>(pmid, disp, vel, disp_cot, vel_cot, n, g, c->pkey,
+                                            c->prank, c->pl_start[A], c->al[A], c->vmax);
+  pmx_toc(c, PMX_T_FILL);
+  PMX_LAUNCH_CHECK();
+
+  const AlUser user = {disp, vel, acc, disp_cot, vel_cot, acc_cot};
+  for (int s = 0; s < nsteps; ++s) {
+    // the force meshes at this step's positions (kept in cbuf)
+    if ((rc = pmx_deposit_state(c, c->al[A].p, c->pl_start[A], c->pl_count[A], val,
+                                (float*)c->cbuf[0], c->rs)))
+      return rc;
+    if ((rc = pmx_poisson_ctx(c, coef))) return rc;
+    // the cotangents of the three force meshes, then of the density (abuf[0])
+    DepOut out3 = {{(float*)c->abuf[0], (float*)c->abuf[1], (float*)c->abuf[2]}};
+    if ((rc = pmx_deposit_state_vals(c, c->al[A].p, c->pl_start[A], c->pl_count[A], 0.f,
+                                     reinterpret_cast<const float*>(c->al[A].c1), 1, 4, 3, out3,
+                                     c->rs)))
+      return rc;
+    if ((rc = pmx_poisson_adj_bufs(c, c->abuf, adj_scale))) return rc;
+    if (s + 1 == nsteps) {
+      pmx_tic(c, PMX_T_GRADG);
+      al_launch<PL_LAST>(c, A, val, 0.f, 0.f, 0.f, nullptr, AlOut{}, user);
+      pmx_toc(c, PMX_T_GRADG);
+      PMX_LAUNCH_CHECK();
+      if ((rc = pmx_red_reduce_to(c, 0, nt, c->al_dots + 2 * s + 1))) return rc;
+      break;
+    }
+    const int B = 1 - A;
+    pmx_tic(c, PMX_T_SCAN);
+    pl_caps_k<<<pmx_blocks(nt + 1), PMX_TPB, 0, c->stream>>>(c->pl_count[A], c->tile_cap,
+                                                             c->pl_count[B], nt, c->flags);
+    PMX_LAUNCH_CHECK();
+    rc = pmx_scan_exclusive(c, c->tile_cap, c->pl_start[B], nt + 1);
+    pmx_toc(c, PMX_T_SCAN);
+    if (rc) return rc;
+    if ((rc = pmx_fill_u32(c, vmax_u, 3, 0u))) return rc;
+    const AlOut out = {c->al[B], c->pl_count[B], c->pl_start[B], c->pkey, c->prank, c->flags};
+    const float* snap = s + 2 == nsteps ? disp_snap : nullptr;
+    pmx_tic(c, PMX_T_GRADG);
+    al_launch<PL_CAP>(c, A, val, kb[s], ka[s + 1], d[s + 1], snap, out, user);
+    pmx_toc(c, PMX_T_GRADG);
+    PMX_LAUNCH_CHECK();
+    if ((rc = pmx_red_reduce_to(c, 0, nt, c->al_dots + 2 * s + 1))) return rc;
+    if ((rc = pmx_red_reduce_to(c, nt, nt, c->al_dots + 2 * (s + 1)))) return rc;
+    // a tile overflowed: redo the move exactly (skipped on the device otherwise)
+    if ((rc = pmx_fill_u32(c, c->pl_count[B], nt + 1, 0u, c->flags))) return rc;
+    pmx_tic(c, PMX_T_BIN);
+    al_launch<PL_COUNT>(c, A, val, kb[s], ka[s + 1], d[s + 1], snap, out, user);
+    PMX_LAUNCH_CHECK();
+    if ((rc = pmx_scan_exclusive(c, c->pl_count[B], c->pl_start[B], nt + 1, c->flags))) return rc;
+    al_launch<PL_WRITE>(c, A, val, kb[s], ka[s + 1], d[s + 1], snap, out, user);
+    pmx_toc(c, PMX_T_BIN);
+    PMX_LAUNCH_CHECK();
+    A = B;
+  }
+  if (!dots) return PMX_OK;
+  PMX_CHECK_HIP(hipMemcpyAsync(dots, c->al_dots, (size_t)2 * nsteps * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream));
+  PMX_CHECK_HIP(hipStreamSynchronize(c->stream));
   return PMX_OK;
 }

@@ -203,21 +203,36 @@ def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=N
 
 # ------------------------------------------------------------ adjoint ---
 
+def _adj_pipeline_fits(n, conf, device):
+    """Whether the device has room for the reverse pipeline's particle
+    copies (about 132 B per record slot, 1.2 slots per particle)."""
+    free, _ = torch.cuda.mem_get_info(device)
+    return 132 * 1.2 * n + (2 << 30) < free
+
+
 def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False, ckpt=None,
-                 fused=True):
+                 fused=True, pipeline=None):
     """Reverse-time adjoint in place (nbody.py:143-162, 226-260).
 
     On return disp/vel hold the reconstructed initial state and
     disp_cot/vel_cot the initial-state cotangents; returns (acc, acc_cot,
-    Omega_m cotangent).  KDK steps run as one fused library call each unless
-    ``fused`` is False (then op by op, as for any other splitting).
+    Omega_m cotangent).  KDK steps between checkpoints run as one
+    ``pmx_nbody_steps_adj`` call (particles and cotangents kept in tile
+    order on the device) when ``pipeline`` is True -- by default when the
+    device has room -- else as one ``pmx_nbody_step_adj`` call per step;
+    both give the same arrays.  With ``fused`` False every splitting runs op
+    by op.
     """
     L = _lib.lib()
     n = pmid.shape[0]
     c = _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
-    # the reverse steps bin per step; the forward pipeline's particle copy
-    # gives its memory back to the cotangents
-    c.trim(pipeline_only=True)
+    kdk = fused and _is_kdk(conf)
+    if pipeline is None:
+        pipeline = kdk and _adj_pipeline_fits(n, conf, disp.device)
+    if not pipeline:
+        # the reverse steps bin per step; the forward pipeline's particle
+        # copy gives its memory back to the cotangents
+        c.trim(pipeline_only=True)
     h = c.bind()
     val, coef = _mass_val(conf), omega_coef(cosmo, conf)
     Om = cosmo.Omega_m
@@ -236,10 +251,51 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
     cf = force_adj()
     dot_force = dot.value
     om_cot = 0.0
-    kdk = fused and _is_kdk(conf)
     dots = (ctypes.c_double * 2)()
     kp = 0.0  # closing kick of the previous KDK step, applied by the next call
-    for s, (a_prev, a_next) in enumerate(zip(a_nbody[:0:-1], a_nbody[-2::-1])):
+    rev = list(zip(a_nbody[:0:-1], a_nbody[-2::-1]))
+    if kdk and pipeline:
+        s = 0
+        while s < nsteps:
+            # one library call per run of steps ending at a checkpoint (snapped)
+            e = s
+            while e < nsteps - 1 and not (ckpt and nsteps - 1 - e in ckpt):
+                e += 1
+            args = []
+            for a_prev, a_next in rev[s:e + 1]:
+                a_half = a_prev * (1 - 0.5) + a_next * 0.5
+                args.append(((a_prev, a_prev, a_half), (a_half, a_prev, a_next),
+                             (a_next, a_half, a_next)))
+            f = np.array([[_f32(kick_factor(*ka, cosmo, conf)),
+                           _f32(drift_factor(*da, cosmo, conf)),
+                           _f32(kick_factor(*kb, cosmo, conf))] for ka, da, kb in args],
+                         dtype=np.float32)
+            fa, fd, fb = (np.ascontiguousarray(f[:, m]) for m in range(3))
+            step_e = nsteps - 1 - e
+            snap = ckpt[step_e][0].to(disp.device) if ckpt and step_e in ckpt else None
+            k = e - s + 1
+            dd = (ctypes.c_double * (2 * k))()
+            P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+            _lib.check(L.pmx_nbody_steps_adj(
+                h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel), _lib.ptr(acc),
+                _lib.ptr(disp_cot), _lib.ptr(vel_cot), _lib.ptr(acc_cot), n, val, coef, kp, k,
+                P(fa), P(fd), P(fb), _lib.ptr(snap), dd))
+            for j, (ka_args, d_args, kb_args) in enumerate(args):
+                om_cot -= kick_factor_dOm(*ka_args, cosmo, conf) * dot_force + cf * float(fa[j])
+                om_cot -= drift_factor_dOm(*d_args, cosmo, conf) * dd[2 * j]
+                dot_force = dd[2 * j + 1]
+                cf = dot_force / Om
+                om_cot -= kick_factor_dOm(*kb_args, cosmo, conf) * dot_force + cf * float(fb[j])
+            kp = float(fb[-1])
+            if ckpt and step_e in ckpt:
+                _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
+                kp = 0.0
+                vel.copy_(ckpt[step_e][1].to(vel.device))
+            s = e + 1
+        if kp != 0.0:
+            _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
+        return acc, acc_cot, om_cot
+    for s, (a_prev, a_next) in enumerate(rev):
         step = nsteps - 1 - s  # index of the forward step being undone
         if kdk:
             # kick_adj, drift_adj, force_adj in one library call, the closing

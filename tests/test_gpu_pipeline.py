@@ -167,3 +167,110 @@ def test_trim_between_calls(dev):
         full = ctx.nbytes
     assert_same(outs[1], outs[0])
     assert_same(outs[2], outs[0])
+
+
+def run_adj(conf, dev, pmid, arrs, kp, ka, d, kb, multi, snap=None, coef=0.45):
+    """Reverse steps from state arrs = (disp, vel, disp_cot, vel_cot): the
+    force VJP at the start, then pmx_nbody_steps_adj or the chained
+    pmx_nbody_step_adj calls; returns the six arrays and the dots."""
+    from pmwd_amd import _lib
+    L = _lib.lib()
+    ctx = _lib.Context(conf.mesh_shape, conf.cell_size, dev)
+    h = ctx.bind()
+    P = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev).clone()
+    pm = P(pmid)
+    x, v, dc, vc = (P(a) for a in arrs)
+    acc, ac = torch.empty_like(x), torch.empty_like(x)
+    n, val = len(pmid), 1.0
+    dot = ctypes.c_double()
+    _lib.check(L.pmx_gravity_vjp(h, _lib.ptr(pm), _lib.ptr(x), n, val, coef, _lib.ptr(vc),
+                                 _lib.ptr(acc), _lib.ptr(ac), ctypes.byref(dot)))
+    sn = P(snap) if snap is not None else None
+    k = len(ka)
+    dots = np.zeros(2 * k)
+    if multi:
+        H = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        dd = (ctypes.c_double * (2 * k))()
+        _lib.check(L.pmx_nbody_steps_adj(h, _lib.ptr(pm), _lib.ptr(x), _lib.ptr(v), _lib.ptr(acc),
+                                         _lib.ptr(dc), _lib.ptr(vc), _lib.ptr(ac), n, val, coef,
+                                         kp, k, H(ka), H(d), H(kb), _lib.ptr(sn), dd))
+        dots[:] = list(dd)
+    else:
+        dd = (ctypes.c_double * 2)()
+        kpi = kp
+        for s in range(k):
+            _lib.check(L.pmx_nbody_step_adj(h, _lib.ptr(pm), _lib.ptr(x), _lib.ptr(v),
+                                            _lib.ptr(acc), _lib.ptr(dc), _lib.ptr(vc),
+                                            _lib.ptr(ac), n, val, coef, kpi, float(ka[s]),
+                                            float(d[s]), _lib.ptr(sn) if s == k - 1 else None,
+                                            dd))
+            dots[2 * s:2 * s + 2] = list(dd)
+            kpi = float(kb[s])
+    torch.cuda.synchronize()
+    return [t.cpu().numpy() for t in (x, v, dc, vc, acc, ac)], dots
+
+
+@pytest.mark.parametrize('shape,ratio,k,spacing,kp,snap', [
+    ((16, 16, 16), 2, 4, 1.0, 0.3, False), ((32, 32, 32), 1, 3, 1.0, 0.0, True),
+    ((12, 10, 8), 1, 1, 1.0, 0.2, True), ((7, 8, 6), 1, 2, 1.3, 0.0, False),
+    ((12, 8, 20), 2, 3, 0.7, 0.25, True)])
+def test_steps_adj_match_step_adj(dev, shape, ratio, k, spacing, kp, snap):
+    import pmwd_amd as pa
+    conf = pa.Configuration(spacing, shape, mesh_shape=ratio)
+    pmid, disp = particles(shape, ratio, conf.cell_size, 20)
+    rng = np.random.default_rng(21)
+    arrs = [disp] + [(0.5 * rng.standard_normal(disp.shape)).astype(np.float32)
+                     for _ in range(3)]
+    ka, d, kb = factors(k, 22)
+    d = -d  # reverse time
+    sn = (disp + 0.3 * rng.standard_normal(disp.shape)).astype(np.float32) if snap else None
+    got, gd = run_adj(conf, dev, pmid, arrs, kp, ka, d, kb, True, sn)
+    want, wd = run_adj(conf, dev, pmid, arrs, kp, ka, d, kb, False, sn)
+    assert_same(got, want)
+    np.testing.assert_allclose(gd, wd, rtol=1e-12, atol=1e-300)
+
+
+def test_steps_adj_collapse_overflows_capacity(dev):
+    import pmwd_amd as pa
+    shape, ratio = (32, 32, 32), 1
+    conf = pa.Configuration(1.0, shape, mesh_shape=ratio)
+    pmid, disp = particles(shape, ratio, conf.cell_size, 23, std=0.2)
+    q = pmid.astype(np.float64) * conf.cell_size + disp
+    centre = 0.5 * np.array(shape) * conf.cell_size
+    vel = (0.3 * (centre - q)).astype(np.float32)
+    rng = np.random.default_rng(24)
+    arrs = [disp, vel] + [rng.standard_normal(disp.shape).astype(np.float32) for _ in range(2)]
+    ka = np.zeros(5, np.float32)
+    kb = np.zeros(5, np.float32)
+    d = np.full(5, 0.5, np.float32)
+    got, gd = run_adj(conf, dev, pmid, arrs, 0.0, ka, d, kb, True)
+    want, wd = run_adj(conf, dev, pmid, arrs, 0.0, ka, d, kb, False)
+    assert_same(got, want)
+    np.testing.assert_allclose(gd, wd, rtol=1e-12, atol=1e-300)
+
+
+def test_adjoint_driver_pipeline_matches_per_step(dev):
+    """pmwd_amd.nbody's reverse loop, with and without checkpoint snaps."""
+    import importlib
+    import pmwd_amd as pa
+    nbm = importlib.import_module('pmwd_amd.nbody')
+    conf = pa.Configuration(4.0, (16, 16, 16), mesh_shape=2, a_start=1 / 8,
+                            a_nbody_maxstep=1 / 16)
+    cosmo = pa.boltzmann(pa.SimpleLCDM(conf), conf)
+    pmid, disp = particles((16, 16, 16), 2, conf.cell_size, 25, std=1.0)
+    rng = np.random.default_rng(26)
+    vel = (0.1 * rng.standard_normal(disp.shape)).astype(np.float32)
+    dc0 = rng.standard_normal(disp.shape).astype(np.float32)
+    vc0 = rng.standard_normal(disp.shape).astype(np.float32)
+    P = lambda a: torch.as_tensor(a, device=dev).clone()
+    for every in (None, 3):
+        d, v = P(disp), P(vel)
+        _, ckpt = nbm._run_forward(P(pmid), d, v, cosmo, conf, checkpoint_every=every)
+        outs = []
+        for pipeline in (True, False):
+            x, vv, dc, vc = d.clone(), v.clone(), P(dc0), P(vc0)
+            acc, ac, om = nbm._run_adjoint(P(pmid), x, vv, dc, vc, cosmo, conf, ckpt=ckpt,
+                                           pipeline=pipeline)
+            outs.append(([t.cpu().numpy() for t in (x, vv, dc, vc, acc, ac)], om))
+        assert_same(outs[0][0], outs[1][0])
+        assert abs(outs[0][1] - outs[1][1]) <= 1e-10 * abs(outs[1][1])
