@@ -179,6 +179,61 @@ __device__ __forceinline__ double fixed_scale(float vmax) {
 // itself.
 #define DEP_TPB 512
 #define DEP_U 4
+
+// Adds v*w_j for the eight corners of a particle (d = disp/cell, lower corner
+// at LDS cell b) to the fixed-point tile accumulator.
+__device__ __forceinline__ void dep_one(unsigned long long* acc, float dx, float dy, float dz,
+                                        int b, int LY, int LZ, float vv, double scale) {
+  float a0, a1, b0, b1, c0, c1;
+  frac_of(dx, a0, a1);
+  frac_of(dy, b0, b1);
+  frac_of(dz, c0, c1);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    float f0 = (j & 1) ? a1 : a0;
+    float f1 = (j & 2) ? b1 : b0;
+    float f2 = (j & 4) ? c1 : c0;
+    int li = b + ((j & 1) ? LY * LZ : 0) + ((j & 2) ? LZ : 0) + ((j & 4) ? 1 : 0);
+    float x = vv * ((f0 * f1) * f2);
+    long long qx = (long long)rint((double)x * scale);
+    atomicAdd(&acc[li], (unsigned long long)qx);
+  }
+}
+
+// Converts the accumulated tile back to float: the interior to the mesh
+// (row stride mrs), the +1 shell to the tile's halo slot hs.
+template <int TPB>
+__device__ __forceinline__ void dep_store(const unsigned long long* acc, float* mesh, float* hs,
+                                          const int o[3], const int s[3], int LY, int LZ,
+                                          const MeshGeom& g, int mrs, int accumulate,
+                                          double inv) {
+  const int itot = s[0] * s[1] * s[2];
+  for (int i = threadIdx.x; i < itot; i += TPB) {
+    int lz = i % s[2];
+    int ly = (i / s[2]) % s[1];
+    int lx = i / (s[2] * s[1]);
+    long long m = ((long long)(o[0] + lx) * g.n1 + (o[1] + ly)) * mrs + (o[2] + lz);
+    float a = (float)((double)(long long)acc[(lx * LY + ly) * LZ + lz] * inv);
+    mesh[m] = accumulate ? mesh[m] + a : a;
+  }
+  const int offA = LY * LZ;
+  const int offB = s[0] * LZ;
+  const int stot = offA + offB + s[0] * s[1];
+  for (int i = threadIdx.x; i < stot; i += TPB) {
+    int qx, qy, qz;
+    if (i < offA) {
+      qx = s[0]; qy = i / LZ; qz = i % LZ;
+    } else if (i < offA + offB) {
+      int k = i - offA;
+      qx = k / LZ; qy = s[1]; qz = k % LZ;
+    } else {
+      int k = i - offA - offB;
+      qx = k / s[1]; qy = k % s[1]; qz = s[2];
+    }
+    hs[i] = (float)((double)(long long)acc[(qx * LY + qy) * LZ + qz] * inv);
+  }
+}
+
 template <int PERVAL, int STATE = 0>
 __global__ void __launch_bounds__(DEP_TPB)
     deposit_k(const float4* __restrict__ rec, const uint16_t* __restrict__ pz,
@@ -234,52 +289,13 @@ __global__ void __launch_bounds__(DEP_TPB)
           pmx_unpack_local(r[u].w, lx, ly, lz);
           if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
         }
-        float a0, a1, b0, b1, c0, c1;
-        frac_of(r[u].x, a0, a1);
-        frac_of(r[u].y, b0, b1);
-        frac_of(r[u].z, c0, c1);
-        int b = (lx * LY + ly) * LZ + lz;
-        float vv = v[u];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          float f0 = (j & 1) ? a1 : a0;
-          float f1 = (j & 2) ? b1 : b0;
-          float f2 = (j & 4) ? c1 : c0;
-          int li = b + ((j & 1) ? LY * LZ : 0) + ((j & 2) ? LZ : 0) + ((j & 4) ? 1 : 0);
-          float x = vv * ((f0 * f1) * f2);
-          long long qx = (long long)rint((double)x * scale);
-          atomicAdd(&acc[li], (unsigned long long)qx);
-        }
+        dep_one(acc, r[u].x, r[u].y, r[u].z, (lx * LY + ly) * LZ + lz, LY, LZ, v[u], scale);
       }
     }
     __syncthreads();
     float* mesh = ch == 0 ? out.mesh[0] : (ch == 1 ? out.mesh[1] : out.mesh[2]);
-    const int itot = s[0] * s[1] * s[2];
-    for (int i = threadIdx.x; i < itot; i += DEP_TPB) {
-      int lz = i % s[2];
-      int ly = (i / s[2]) % s[1];
-      int lx = i / (s[2] * s[1]);
-      long long m = ((long long)(o[0] + lx) * g.n1 + (o[1] + ly)) * mrs + (o[2] + lz);
-      float a = (float)((double)(long long)acc[(lx * LY + ly) * LZ + lz] * inv);
-      mesh[m] = accumulate ? mesh[m] + a : a;
-    }
-    const int offA = LY * LZ;
-    const int offB = s[0] * LZ;
-    const int stot = offA + offB + s[0] * s[1];
-    float* hs = halo + ch * hstride + (long long)t * tg.shell;
-    for (int i = threadIdx.x; i < stot; i += DEP_TPB) {
-      int qx, qy, qz;
-      if (i < offA) {
-        qx = s[0]; qy = i / LZ; qz = i % LZ;
-      } else if (i < offA + offB) {
-        int k = i - offA;
-        qx = k / LZ; qy = s[1]; qz = k % LZ;
-      } else {
-        int k = i - offA - offB;
-        qx = k / s[1]; qy = k % s[1]; qz = s[2];
-      }
-      hs[i] = (float)((double)(long long)acc[(qx * LY + qy) * LZ + qz] * inv);
-    }
+    dep_store<DEP_TPB>(acc, mesh, halo + ch * hstride + (long long)t * tg.shell, o, s, LY, LZ, g,
+                       mrs, accumulate, inv);
     __syncthreads();  // acc is cleared for the next channel
   }
 }
