@@ -406,11 +406,13 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_half_k(float* __restrict__ d
   __syncthreads();
   lds_fft<-1, ROWS, true, RFFT_MAXV>(s, H, np, stw, 2);
   const int H1 = H + 1;
+  // idx / H1 as a multiply-high: exact while idx * H1 < 2^32 (idx < 2^13 here)
+  const unsigned mH1 = (unsigned)((0x100000000ull + H1 - 1) / H1);
   // np*(H+1) <= RFFT_MAXV*FFT_TPB + np: one extra value per thread at most
   batch_copy<RFFT_MAXV + 1, FFT_TPB>(
       np * H1,
       [&](int idx) {
-        int p = idx / H1, k = idx - (idx / H1) * H1;
+        int p = (int)__umulhi((unsigned)idx, mH1), k = idx - p * H1;
         float2 zk = s[lidx<ROWS>(p, k == H ? 0 : k, H, P)];
         float2 zm = s[lidx<ROWS>(p, k == 0 ? 0 : H - k, H, P)];
         zm.y = -zm.y;
@@ -420,7 +422,7 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_half_k(float* __restrict__ d
         return cadd(e, cmul(stw[k], o));
       },
       [&](int idx, float2 v) {
-        int p = idx / H1, k = idx - (idx / H1) * H1;
+        int p = (int)__umulhi((unsigned)idx, mH1), k = idx - p * H1;
         d2[(row0 + p) * rsc + k] = v;
       });
 }
