@@ -92,9 +92,12 @@ def kernel_bytes(N, mesh_shape, tile, K):
         'fold': (0, K * (4 * shell + 8 * faces) * ntiles),
         'z_r2c': (K * (4 * M + 8 * Mh),) * 2,
         'y_fwd': (K * 16 * Mh,) * 2,
-        # read one complex mesh, write three
-        'poisson_x': (K * 32 * Mh,) * 2,
-        'y_inv': (K * 48 * Mh,) * 2,
+        # read one complex mesh, write two (the x component and the
+        # transform shared by the y and z components)
+        'poisson_x': (K * 24 * Mh,) * 2,
+        # x component in place; the shared transform read once, the y and z
+        # components written
+        'y_inv': (K * 40 * Mh,) * 2,
         # three launches, each reads a complex and writes a real mesh
         'z_c2r': (K * 3 * (8 * Mh + 4 * M),) * 2,
         'gather': ((K - 1) * (54 * N + 12 * M) + 66 * N + 12 * M,

@@ -546,12 +546,13 @@ struct KTab {
 };
 
 // x pass of the Poisson solve (gravity.py:56-66): forward FFT along x of
-// c0, pot = -src/k^2 (k=0 -> 0), g_d = (-i km_d) pot, inverse FFT along x,
-// stored to c_d.  km_1 (one value per y row) and km_2 (one per kz column)
-// are constant along x, so the y and z components share one inverse
-// transform h = IFFT_x(-i pot): c_1 = km_1 h and c_2 = km_2 h.  Only the x
-// component needs a transform of its own; the two inverse transforms run
-// side by side as 2B pencils.  Pencils along axis 0 of (n0,n1,n2h);
+// c0, pot = -src/k^2 (k=0 -> 0), g_d = (-i km_d) pot, inverse FFT along x.
+// km_1 (one value per y row) and km_2 (one per kz column) are constant along
+// x, so the y and z components share one inverse transform h =
+// IFFT_x(-i pot), stored to c_1 (y_inv_pair_k scales it by km_1 and km_2 for
+// the two components); the x component (-i km_0 pot) is transformed on its
+// own and stored to c_0.  The two inverse transforms run side by side as 2B
+// pencils.  Pencils along axis 0 of (n0,n1,n2h);
 // blockIdx.y = y index, blockIdx.x = block of B kz columns.
 // LDS: 2 n0 B values (the forward block, then the two inverse transforms
 // interleaved: column b of the x component at j 2B + b, of h at j 2B + B + b),
@@ -581,7 +582,6 @@ __global__ void __launch_bounds__(PFFT_TPB) poisson_x_k(Fields3 fl, int n0, int 
   __syncthreads();
   lds_fft<-1, COLS, POW2, PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
   const float ky = kt.k[1][y];
-  const float kym = kt.km[1][y];
   float2 h[PFFT_MAXV];
 #pragma unroll
   for (int it = 0; it < PFFT_MAXV; ++it) {
@@ -627,13 +627,7 @@ __global__ void __launch_bounds__(PFFT_TPB) poisson_x_k(Fields3 fl, int n0, int 
       int b = p < B ? p : p - B;
       if (b < nb) {
         long long o = off + j * es + b;
-        if (p < B) {
-          fl.f[0][o] = r[it];
-        } else {
-          float kmz = kt.km[2][c0 + b];
-          fl.f[1][o] = make_float2(kym * r[it].x, kym * r[it].y);
-          fl.f[2][o] = make_float2(kmz * r[it].x, kmz * r[it].y);
-        }
+        (p < B ? fl.f[0] : fl.f[1])[o] = r[it];
       }
     }
   }
@@ -642,9 +636,10 @@ __global__ void __launch_bounds__(PFFT_TPB) poisson_x_k(Fields3 fl, int n0, int 
 // Adjoint x pass: acc = sum_d (-i km_d) laplace(FFT_x c_d); inverse FFT
 // along x; stored to c0.  Transpose of the three force components.  As km_1
 // and km_2 are constant along x, FFT_x(km_1 c_1 + km_2 c_2) replaces the two
-// transforms of c_1 and c_2: the block holds c_0 and that combination side
-// by side as 2B pencils (j 2B + b and j 2B + B + b) for one forward pass,
-// then the sum in B pencils for the inverse.  LDS as poisson_x_k.
+// transforms of c_1 and c_2; y_fwd_pair_k leaves that combination in c_1.
+// The block holds c_0 and c_1 side by side as 2B pencils (j 2B + b and
+// j 2B + B + b) for one forward pass, then the sum in B pencils for the
+// inverse.  LDS as poisson_x_k.
 template <bool POW2>
 __global__ void __launch_bounds__(PFFT_TPB) poisson_x_adj_k(Fields3 fl, int n0, int n1, int n2h, int n2p, int B,
                                                            KTab kt, const float2* __restrict__ tw) {
@@ -660,7 +655,6 @@ __global__ void __launch_bounds__(PFFT_TPB) poisson_x_adj_k(Fields3 fl, int n0, 
   const int nb = n2h - c0 < B ? n2h - c0 : B;
   const long long off = (long long)y * n2p + c0;
   const float ky = kt.k[1][y];
-  const float kym = kt.km[1][y];
   stage_twiddles<PFFT_TPB>(stw, tw, n0);
   batch_copy<2 * PFFT_MAXV, PFFT_TPB>(
       2 * total,
@@ -668,16 +662,7 @@ __global__ void __launch_bounds__(PFFT_TPB) poisson_x_adj_k(Fields3 fl, int n0, 
         int j = idx >> (lB + 1), p = idx & (B2 - 1);
         int b = p < B ? p : p - B;
         float2 v = make_float2(0.f, 0.f);
-        if (b < nb) {
-          long long o = off + j * es + b;
-          if (p < B) {
-            v = fl.f[0][o];
-          } else {
-            float kmz = kt.km[2][c0 + b];
-            float2 a = fl.f[1][o], c = fl.f[2][o];
-            v = make_float2(kym * a.x + kmz * c.x, kym * a.y + kmz * c.y);
-          }
-        }
+        if (b < nb) v = (p < B ? fl.f[0] : fl.f[1])[off + j * es + b];
         return v;
       },
       [&](int idx, float2 v) { s[idx] = v; });
@@ -718,6 +703,102 @@ __global__ void __launch_bounds__(PFFT_TPB) poisson_x_adj_k(Fields3 fl, int n0, 
       [&](int idx, float2 v) {
         int j = idx >> lB, b = idx & (B - 1);
         if (b < nb) fl.f[0][off + j * es + b] = v;
+      });
+}
+
+// y passes of the gradient components, paired (pencils along axis 1 of
+// (n0,n1,n2h); blockIdx.y = x row, blockIdx.x = block of B kz columns; LDS:
+// 2 n1 B values side by side as 2B pencils, then the twiddle table).
+//
+// Inverse: from h = IFFT_x(-i pot) in c_1, the y and z force components
+// IFFT_y(km_1 h) to c_1 and IFFT_y(km_2 h) to c_2 -- the products and
+// transforms of a separate y pass over km_1 h and km_2 h, with h read once.
+template <bool POW2>
+__global__ void __launch_bounds__(FFT_TPB, 4) y_inv_pair_k(Fields3 fl, int n1, long long es,
+                                                         long long ostride, int ncol, int B,
+                                                         KTab kt, const float2* __restrict__ tw) {
+  extern __shared__ float2 s[];
+  const int B2 = 2 * B;
+  const int total = n1 * B;
+  float2* stw = s + 2 * total;
+  int bx, by, bz;
+  xcd_block3(bx, by, bz);
+  const long long off = (long long)by * ostride;
+  const int lB = __ffs(B) - 1;
+  const int c0 = bx * B;
+  const int nb = ncol - c0 < B ? ncol - c0 : B;
+  stage_twiddles<FFT_TPB>(stw, tw, n1);
+  float2 h[FFT_MAXV / 2];
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV / 2; ++it) {
+    const int idx = threadIdx.x + it * FFT_TPB;
+    h[it] = make_float2(0.f, 0.f);
+    if (idx < total) {
+      const int j = idx >> lB, b = idx & (B - 1);
+      if (b < nb) h[it] = fl.f[1][off + j * es + c0 + b];
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV / 2; ++it) {
+    const int idx = threadIdx.x + it * FFT_TPB;
+    if (idx < total) {
+      const int j = idx >> lB, b = idx & (B - 1);
+      const float kmy = kt.km[1][j], kmz = b < nb ? kt.km[2][c0 + b] : 0.f;
+      s[j * B2 + b] = make_float2(kmy * h[it].x, kmy * h[it].y);
+      s[j * B2 + B + b] = make_float2(kmz * h[it].x, kmz * h[it].y);
+    }
+  }
+  __syncthreads();
+  lds_fft<1, COLS, POW2>(s, n1, B2, stw);
+  batch_copy<FFT_MAXV, FFT_TPB>(
+      2 * total, [&](int idx) { return s[idx]; },
+      [&](int idx, float2 v) {
+        const int j = idx >> (lB + 1), p = idx & (B2 - 1);
+        const int b = p < B ? p : p - B;
+        if (b < nb) (p < B ? fl.f[1] : fl.f[2])[off + j * es + c0 + b] = v;
+      });
+}
+
+// Forward (adjoint path): FFT_y of c_1 and c_2 side by side, then their
+// combination km_1 FFT_y(c_1) + km_2 FFT_y(c_2) to c_1 for poisson_x_adj_k
+// -- the products of the x pass that used to form it, moved here.
+template <bool POW2>
+__global__ void __launch_bounds__(FFT_TPB, 4) y_fwd_pair_k(Fields3 fl, int n1, long long es,
+                                                         long long ostride, int ncol, int B,
+                                                         KTab kt, const float2* __restrict__ tw) {
+  extern __shared__ float2 s[];
+  const int B2 = 2 * B;
+  const int total = n1 * B;
+  float2* stw = s + 2 * total;
+  int bx, by, bz;
+  xcd_block3(bx, by, bz);
+  const long long off = (long long)by * ostride;
+  const int lB = __ffs(B) - 1;
+  const int c0 = bx * B;
+  const int nb = ncol - c0 < B ? ncol - c0 : B;
+  stage_twiddles<FFT_TPB>(stw, tw, n1);
+  batch_copy<FFT_MAXV, FFT_TPB>(
+      2 * total,
+      [&](int idx) {
+        const int j = idx >> (lB + 1), p = idx & (B2 - 1);
+        const int b = p < B ? p : p - B;
+        return b < nb ? (p < B ? fl.f[1] : fl.f[2])[off + j * es + c0 + b]
+                      : make_float2(0.f, 0.f);
+      },
+      [&](int idx, float2 v) { s[idx] = v; });
+  __syncthreads();
+  lds_fft<-1, COLS, POW2>(s, n1, B2, stw);
+  batch_copy<FFT_MAXV / 2, FFT_TPB>(
+      total,
+      [&](int idx) {
+        const int j = idx >> lB, b = idx & (B - 1);
+        const float2 a = s[j * B2 + b], c = s[j * B2 + B + b];
+        const float kmy = kt.km[1][j], kmz = b < nb ? kt.km[2][c0 + b] : 0.f;
+        return make_float2(kmy * a.x + kmz * c.x, kmy * a.y + kmz * c.y);
+      },
+      [&](int idx, float2 v) {
+        const int j = idx >> lB, b = idx & (B - 1);
+        if (b < nb) fl.f[1][off + j * es + c0 + b] = v;
       });
 }
 
@@ -874,6 +955,36 @@ static KTab ktab(const pmx_ctx* c) {
   return kt;
 }
 
+// Paired y pass of the gradient's y and z components: dir > 0 turns h in
+// f[1] into the two force components in f[1] and f[2] (y_inv_pair_k);
+// dir < 0 turns the cotangents in f[1] and f[2] into km_1 FFT_y(f[1]) +
+// km_2 FFT_y(f[2]) in f[1] (y_fwd_pair_k).
+static int pmx_fft_ypair(pmx_ctx* c, float2* const* f, int dir) {
+  Fields3 fl = {{f[0], f[1], f[2]}};
+  const int n = c->n[1];
+  const int B = fft_pencils(n, FFT_CAP / 2);
+  const int ncol = c->n2h;
+  const long long es = c->n2p, ostride = (long long)n * c->n2p;
+  dim3 grid((ncol + B - 1) / B, c->n[0], 1);
+  size_t lds = 2 * lds_bytes(n, B) + tw_bytes(n);
+  if (int rc = fft_fits("paired y FFT", n, 2 * n * B, FFT_CAP, lds)) return rc;
+  const bool p2 = is_pow2(n);
+  const KTab kt = ktab(c);
+  const int cls = dir < 0 ? PMX_T_YFWD : PMX_T_YINV;
+  pmx_tic(c, cls);
+  if (dir < 0 && p2)
+    y_fwd_pair_k<true><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]);
+  else if (dir < 0)
+    y_fwd_pair_k<false><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]);
+  else if (p2)
+    y_inv_pair_k<true><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]);
+  else
+    y_inv_pair_k<false><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]);
+  pmx_toc(c, cls);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
 int pmx_poisson_x(pmx_ctx* c, float2* const* f, int adjoint) {
   Fields3 fl = {{f[0], f[1], f[2]}};
   int n0 = c->n[0];
@@ -905,7 +1016,8 @@ int pmx_poisson_ctx(pmx_ctx* c, float coef) {
   if ((rc = pmx_fft_r2c_inplace(c, (float*)f[0], 1, coef))) return rc;
   if ((rc = pmx_fft_strided(c, f, 1, 1, -1))) return rc;
   if ((rc = pmx_poisson_x(c, f, 0))) return rc;
-  if ((rc = pmx_fft_strided(c, f, 3, 1, 1))) return rc;
+  if ((rc = pmx_fft_strided(c, f, 1, 1, 1))) return rc;
+  if ((rc = pmx_fft_ypair(c, f, 1))) return rc;
   float scale = (float)(1.0 / (double)c->M);
   for (int d = 0; d < 3; ++d)
     if ((rc = pmx_fft_c2r_inplace(c, (float*)f[d], scale))) return rc;
@@ -925,7 +1037,8 @@ int pmx_poisson_adj_bufs(pmx_ctx* c, float2* const* f, float scale_out) {
   int rc;
   for (int d = 0; d < 3; ++d)
     if ((rc = pmx_fft_r2c_inplace(c, (float*)f[d], 0, 0.f))) return rc;
-  if ((rc = pmx_fft_strided(c, f, 3, 1, -1))) return rc;
+  if ((rc = pmx_fft_strided(c, f, 1, 1, -1))) return rc;
+  if ((rc = pmx_fft_ypair(c, f, -1))) return rc;
   if ((rc = pmx_poisson_x(c, f, 1))) return rc;
   if ((rc = pmx_fft_strided(c, f, 1, 1, 1))) return rc;
   if ((rc = pmx_fft_c2r_inplace(c, (float*)f[0], scale_out))) return rc;
