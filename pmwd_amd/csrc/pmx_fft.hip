@@ -22,8 +22,18 @@
 #define PFFT_CAP 4096                // two such blocks (68 KB of LDS each) per CU
 #define PFFT_MAXV (PFFT_CAP / PFFT_TPB)
 
+// Complex products inside the transforms may fuse into FMAs (the file is
+// otherwise built with separate roundings).
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+#pragma clang fp contract(fast)
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a w (DIR < 0) or a conj(w) (DIR > 0)
+template <int DIR>
+__device__ __forceinline__ float2 cmul_dir(float2 a, float2 w) {
+#pragma clang fp contract(fast)
+  return DIR < 0 ? make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x)
+                 : make_float2(a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y);
 }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -104,8 +114,25 @@ __device__ __forceinline__ int lidx(int p, int j, int n, int P) {
 }
 
 // LDS elements needed for P pencils of length n in a layout
-__host__ __device__ inline int lds_elems(int lay, int n, int P) {
+__host__ __device__ constexpr int lds_elems(int lay, int n, int P) {
   return lay == ROWS ? P * (n + (n >> 3)) : P * n;
+}
+
+__host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n >> 1); }
+
+// pencils per workgroup for length n: a power of two with P*n <= cap
+__host__ __device__ constexpr int fft_pencils(int n, int cap = FFT_CAP) {
+  int P = 1;
+  while (P * 2 * n <= cap && P * 2 <= 256) P *= 2;
+  return P;
+}
+
+// rows per workgroup for the row passes: P*n <= RFFT_CAP and the swizzled
+// ROWS block within FFT_CAP elements (64 KB)
+__host__ __device__ constexpr int row_pencils(int n) {
+  int P = 1;
+  while (P * 2 * n <= RFFT_CAP && lds_elems(ROWS, n, P * 2) <= FFT_CAP && P * 2 <= 256) P *= 2;
+  return P;
 }
 
 // One Stockham stage of radix R (n a power of two, Ns the size of the
@@ -245,14 +272,116 @@ __device__ __forceinline__ void lds_fft(float2* s, int n, int P, const float2* _
   }
 }
 
-static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
-
-// pencils per workgroup for length n: a power of two with P*n <= cap
-static int fft_pencils(int n, int cap = FFT_CAP) {
-  int P = 1;
-  while (P * 2 * n <= cap && P * 2 <= 256) P *= 2;
-  return P;
+// ----------------------------------------- compile-time length FFTs --
+// The same Stockham stages with N, P and the stage size NS known at compile
+// time (N and P powers of two): index arithmetic folds to shifts and
+// immediate LDS offsets, and the bounds tests go when the butterflies fill
+// the workgroup.  COLS blocks read every twiddle power from the table (the
+// pencils of a wave share a twiddle, so the reads broadcast); ROWS blocks
+// build w^3, w^5..w^7 from w, w^2, w^4 as lds_fft does.  In both layouts
+// element x of pencil p sits at lidx(p, x); the R inputs of a butterfly are
+// a fixed stride apart, and so are its R outputs except for ROWS stages
+// with 1 < NS < 8, which compute each address.
+template <int R, int DIR, int LAY, int N, int P, int NS, int TPB, int TWS>
+__device__ __forceinline__ void ct_stage(float2* s, const float2* __restrict__ tw) {
+  constexpr int NB = N / R, TOTAL = P * NB;
+  constexpr int NG = (TOTAL + TPB - 1) / TPB;
+  constexpr bool FULL = TOTAL % TPB == 0;
+  constexpr int LP = ilog2c(P), LNB = ilog2c(NB);
+  constexpr int STEP = N / (NS * R) * TWS;
+  constexpr bool SEP_IN = LAY == COLS || NB % 8 == 0;
+  constexpr bool SEP_OUT = LAY == COLS || NS % 8 == 0 || NS == 1;
+  constexpr int SIN = LAY == ROWS ? NB + (NB >> 3) : NB * P;
+  constexpr int SOUT = LAY == ROWS ? NS + (NS >> 3) : NS * P;
+  constexpr bool ALLTW = LAY == COLS;
+  float2 v[NG * R];
+  float2 w[NG][R > 1 ? R - 1 : 1];
+  int ob[NG];
+#pragma unroll
+  for (int it = 0; it < NG; ++it) {
+    const int idx = threadIdx.x + it * TPB;
+    if (FULL || idx < TOTAL) {
+      const int p = LAY == ROWS ? idx >> LNB : idx & (P - 1);
+      const int j = LAY == ROWS ? idx & (NB - 1) : idx >> LP;
+      const int k = j & (NS - 1);
+      ob[it] = lidx<LAY>(p, (j - k) * R + k, N, P);
+      const int ib = lidx<LAY>(p, j, N, P);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        v[it * R + r] = SEP_IN ? s[ib + r * SIN] : s[lidx<LAY>(p, j + r * NB, N, P)];
+      if (NS > 1) {
+        const int m = k * STEP;
+        if (ALLTW) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) w[it][r - 1] = tw[r * m];
+        } else {
+          w[it][0] = tw[m];
+          if constexpr (R >= 4) w[it][1] = tw[2 * m];
+          if constexpr (R == 8) w[it][3] = tw[4 * m];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < NG; ++it) {
+    const int idx = threadIdx.x + it * TPB;
+    if (FULL || idx < TOTAL) {
+      float2* u = &v[it * R];
+      if (NS > 1) {
+        if constexpr (!ALLTW) {
+          if constexpr (R >= 4) w[it][2] = cmul(w[it][1], w[it][0]);
+          if constexpr (R == 8) {
+            w[it][4] = cmul(w[it][3], w[it][0]);
+            w[it][5] = cmul(w[it][3], w[it][1]);
+            w[it][6] = cmul(w[it][3], w[it][2]);
+          }
+        }
+#pragma unroll
+        for (int r = 1; r < R; ++r) u[r] = cmul_dir<DIR>(u[r], w[it][r - 1]);
+      }
+      dftR<R, DIR>(u);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < NG; ++it) {
+    const int idx = threadIdx.x + it * TPB;
+    if (FULL || idx < TOTAL) {
+      if (SEP_OUT) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[ob[it] + r * SOUT] = v[it * R + r];
+      } else {
+        const int p = idx >> LNB, j = idx & (NB - 1);
+        const int k = j & (NS - 1), o = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[lidx<LAY>(p, o + r * NS, N, P)] = v[it * R + r];
+      }
+    }
+  }
+  __syncthreads();
 }
+
+template <int DIR, int LAY, int N, int P, int NS, int TPB, int TWS>
+__device__ __forceinline__ void ct_fft(float2* s, const float2* __restrict__ tw) {
+  if constexpr (NS < N) {
+    constexpr int REM = N / NS;
+    constexpr int R = REM % 8 == 0 ? 8 : (REM % 4 == 0 ? 4 : 2);
+    ct_stage<R, DIR, LAY, N, P, NS, TPB, TWS>(s, tw);
+    ct_fft<DIR, LAY, N, P, NS * R, TPB, TWS>(s, tw);
+  }
+}
+
+// FFT of P pencils of length n: the compile-time path when LN > 0
+// (n = 2^LN, P = 2^LP), else lds_fft.
+template <int DIR, int LAY, bool POW2, int LN, int LP, int MAXV, int TPB, int TWS = 1>
+__device__ __forceinline__ void fft_run(float2* s, int n, int P, const float2* __restrict__ tw) {
+  if constexpr (LN > 0)
+    ct_fft<DIR, LAY, 1 << LN, 1 << LP, 1, TPB, TWS>(s, tw);
+  else
+    lds_fft<DIR, LAY, POW2, MAXV, TPB>(s, n, P, tw, TWS);
+}
+
+static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 // -------------------------------------------------- batched transfers --
 // Global <-> LDS traffic goes through registers in unrolled batches of up to
@@ -372,94 +501,116 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_c2r_k(float* __restrict__ data, 
 // as H complex values z[m] = x[2m] + i x[2m+1], Z = FFT_H(z), and for
 // k = 0..H:  X[k] = E + w^k O,  E = (Z[k] + conj Z[H-k])/2,
 // O = -i (Z[k] - conj Z[H-k])/2,  w = exp(-2 pi i/n2)  (Z[H] = Z[0]).
-// PRE applies (x - 1) * coef on load.
-template <int PRE>
+// PRE applies (x - 1) * coef on load.  LH > 0 fixes H = 2^LH and
+// P = row_pencils(H) at compile time (the launch then holds whole blocks).
+__device__ __forceinline__ float2 r2c_half_x(const float2* s, const float2* stw, int p, int k, int H,
+                                             int P) {
+  float2 zk = s[lidx<ROWS>(p, k & (H - 1), H, P)];
+  float2 zm = s[lidx<ROWS>(p, (H - k) & (H - 1), H, P)];
+  zm.y = -zm.y;
+  float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
+  float2 dd = csub(zk, zm);
+  float2 o = make_float2(0.5f * dd.y, -0.5f * dd.x);
+  return cadd(e, cmul(stw[k], o));
+}
+
+template <int PRE, int LH>
 __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_half_k(float* __restrict__ data, long long nrows,
-                                                           int n2, int rs, int P, float coef,
+                                                           int n2, int rs, int P_, float coef,
                                                            const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
-  const int H = n2 >> 1;
-  const int lH = __ffs(H) - 1;
+  constexpr int PC = LH ? row_pencils(1 << LH) : 1;
+  const int H = LH ? 1 << LH : n2 >> 1;
+  const int lH = LH ? LH : __ffs(H) - 1;
+  const int P = LH ? PC : P_;
   float2* stw = s + lds_elems(ROWS, H, P);
-  long long row0 = (long long)blockIdx.x * P;
-  int np = (int)((nrows - row0) < P ? (nrows - row0) : P);
-  float2* d2 = reinterpret_cast<float2*>(data);
+  const long long row0 = (long long)blockIdx.x * P;
+  const int np = LH ? PC : (int)((nrows - row0) < P ? (nrows - row0) : P);
   const int rsc = rs / 2;
-  stage_twiddles<FFT_TPB>(stw, tw, n2);
-  batch_copy<RFFT_MAXV, FFT_TPB>(
-      np * H,
-      [&](int idx) {
-        int p = idx >> lH, m = idx & (H - 1);
-        float2 v = d2[(row0 + p) * rsc + m];
+  float2* blk = reinterpret_cast<float2*>(data) + row0 * rsc;
+  const int total = np * H;
+  stage_twiddles<FFT_TPB>(stw, tw, 2 * H);
+  {
+    float2 r[RFFT_MAXV];
+#pragma unroll
+    for (int it = 0; it < RFFT_MAXV; ++it) {
+      const int idx = threadIdx.x + it * FFT_TPB;
+      if (idx < total) {
+        const int p = idx >> lH, m = idx & (H - 1);
+        float2 v = blk[__umul24(p, rsc) + m];
         if (PRE) {
           v.x = v.x - 1.f;
           v.x = v.x * coef;
           v.y = v.y - 1.f;
           v.y = v.y * coef;
         }
-        return v;
-      },
-      [&](int idx, float2 v) {
-        int p = idx >> lH, m = idx & (H - 1);
-        s[lidx<ROWS>(p, m, H, P)] = v;
-      });
+        r[it] = v;
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < RFFT_MAXV; ++it) {
+      const int idx = threadIdx.x + it * FFT_TPB;
+      if (idx < total) s[lidx<ROWS>(idx >> lH, idx & (H - 1), H, P)] = r[it];
+    }
+  }
   __syncthreads();
-  lds_fft<-1, ROWS, true, RFFT_MAXV>(s, H, np, stw, 2);
-  const int H1 = H + 1;
-  // idx / H1 as a multiply-high: exact while idx * H1 < 2^32 (idx < 2^13 here)
-  const unsigned mH1 = (unsigned)((0x100000000ull + H1 - 1) / H1);
-  // np*(H+1) <= RFFT_MAXV*FFT_TPB + np: one extra value per thread at most
-  batch_copy<RFFT_MAXV + 1, FFT_TPB>(
-      np * H1,
-      [&](int idx) {
-        int p = (int)__umulhi((unsigned)idx, mH1), k = idx - p * H1;
-        float2 zk = s[lidx<ROWS>(p, k == H ? 0 : k, H, P)];
-        float2 zm = s[lidx<ROWS>(p, k == 0 ? 0 : H - k, H, P)];
-        zm.y = -zm.y;
-        float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
-        float2 dd = csub(zk, zm);
-        float2 o = make_float2(0.5f * dd.y, -0.5f * dd.x);
-        return cadd(e, cmul(stw[k], o));
-      },
-      [&](int idx, float2 v) {
-        int p = (int)__umulhi((unsigned)idx, mH1), k = idx - p * H1;
-        d2[(row0 + p) * rsc + k] = v;
-      });
+  fft_run<-1, ROWS, true, LH, ilog2c(PC), RFFT_MAXV, FFT_TPB, 2>(s, H, np, stw);
+  {
+    float2 r[RFFT_MAXV];
+#pragma unroll
+    for (int it = 0; it < RFFT_MAXV; ++it) {
+      const int idx = threadIdx.x + it * FFT_TPB;
+      if (idx < total) r[it] = r2c_half_x(s, stw, idx >> lH, idx & (H - 1), H, P);
+    }
+#pragma unroll
+    for (int it = 0; it < RFFT_MAXV; ++it) {
+      const int idx = threadIdx.x + it * FFT_TPB;
+      if (idx < total) blk[__umul24(idx >> lH, rsc) + (idx & (H - 1))] = r[it];
+    }
+    // X[H] of each row
+    if ((int)threadIdx.x < np)
+      blk[__umul24(threadIdx.x, rsc) + H] = r2c_half_x(s, stw, threadIdx.x, H, H, P);
+  }
 }
 
 // Half-length c2r (inverse of the above) with a final scale:
 // Z[k] = (X[k] + conj X[H-k]) + i w^-k (X[k] - conj X[H-k]) for k < H with
 // Im X[0] = Im X[H] = 0 (as irfft), z = IFFT_H(Z), x[2m] + i x[2m+1] = z[m].
+// LH as in z_r2c_half_k.
+template <int LH>
 __global__ void __launch_bounds__(FFT_TPB, 4) z_c2r_half_k(float* __restrict__ data, long long nrows,
-                                                           int n2, int rs, int P, float scale,
+                                                           int n2, int rs, int P_, float scale,
                                                            const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
-  const int H = n2 >> 1;
-  const int lH = __ffs(H) - 1;
+  constexpr int PC = LH ? row_pencils(1 << LH) : 1;
+  const int H = LH ? 1 << LH : n2 >> 1;
+  const int lH = LH ? LH : __ffs(H) - 1;
+  const int P = LH ? PC : P_;
   float2* stw = s + lds_elems(ROWS, H, P);
-  long long row0 = (long long)blockIdx.x * P;
-  int np = (int)((nrows - row0) < P ? (nrows - row0) : P);
-  float2* d2 = reinterpret_cast<float2*>(data);
+  const long long row0 = (long long)blockIdx.x * P;
+  const int np = LH ? PC : (int)((nrows - row0) < P ? (nrows - row0) : P);
   const int rsc = rs / 2;
-  stage_twiddles<FFT_TPB>(stw, tw, n2);
+  float2* blk = reinterpret_cast<float2*>(data) + row0 * rsc;
+  const int total = np * H;
+  stage_twiddles<FFT_TPB>(stw, tw, 2 * H);
   // X[k] and X[H-k] per thread: both loads of the batch are issued first
   float2 xa[RFFT_MAXV], xb[RFFT_MAXV];
-  const int total = np * H;
 #pragma unroll
   for (int it = 0; it < RFFT_MAXV; ++it) {
-    int idx = threadIdx.x + it * FFT_TPB;
+    const int idx = threadIdx.x + it * FFT_TPB;
     if (idx < total) {
-      int p = idx >> lH, k = idx & (H - 1);
-      xa[it] = d2[(row0 + p) * rsc + k];
-      xb[it] = d2[(row0 + p) * rsc + (H - k)];
+      const float2* row = blk + __umul24(idx >> lH, rsc);
+      const int k = idx & (H - 1);
+      xa[it] = row[k];
+      xb[it] = row[H - k];
     }
   }
   __syncthreads();  // the twiddle table is read below
 #pragma unroll
   for (int it = 0; it < RFFT_MAXV; ++it) {
-    int idx = threadIdx.x + it * FFT_TPB;
+    const int idx = threadIdx.x + it * FFT_TPB;
     if (idx < total) {
-      int p = idx >> lH, k = idx & (H - 1);
+      const int p = idx >> lH, k = idx & (H - 1);
       float2 a = xa[it], b = xb[it];
       if (k == 0) {
         a.y = 0.f;
@@ -468,24 +619,24 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_c2r_half_k(float* __restrict__ d
       b.y = -b.y;
       float2 sum = cadd(a, b);
       float2 dif = csub(a, b);
-      float2 w = stw[k];
-      w.y = -w.y;
-      float2 t = cmul(w, dif);
+      float2 t = cmul_dir<1>(dif, stw[k]);
       s[lidx<ROWS>(p, k, H, P)] = make_float2(sum.x - t.y, sum.y + t.x);
     }
   }
   __syncthreads();
-  lds_fft<1, ROWS, true, RFFT_MAXV>(s, H, np, stw, 2);
-  batch_copy<RFFT_MAXV, FFT_TPB>(
-      total,
-      [&](int idx) {
-        int p = idx >> lH, m = idx & (H - 1);
-        return s[lidx<ROWS>(p, m, H, P)];
-      },
-      [&](int idx, float2 z) {
-        int p = idx >> lH, m = idx & (H - 1);
-        d2[(row0 + p) * rsc + m] = make_float2(z.x * scale, z.y * scale);
-      });
+  fft_run<1, ROWS, true, LH, ilog2c(PC), RFFT_MAXV, FFT_TPB, 2>(s, H, np, stw);
+  float2 r[RFFT_MAXV];
+#pragma unroll
+  for (int it = 0; it < RFFT_MAXV; ++it) {
+    const int idx = threadIdx.x + it * FFT_TPB;
+    if (idx < total) r[it] = s[lidx<ROWS>(idx >> lH, idx & (H - 1), H, P)];
+  }
+#pragma unroll
+  for (int it = 0; it < RFFT_MAXV; ++it) {
+    const int idx = threadIdx.x + it * FFT_TPB;
+    if (idx < total)
+      blk[__umul24(idx >> lH, rsc) + (idx & (H - 1))] = make_float2(r[it].x * scale, r[it].y * scale);
+  }
 }
 
 // ------------------------------------------------------ strided passes --
@@ -506,38 +657,63 @@ __device__ __forceinline__ void xcd_block3(int& bx, int& by, int& bz) {
   bz = l / (gridDim.x * gridDim.y);
 }
 
+// A thread's share of an n x B block of columns (B a power of two dividing
+// TPB): its it-th element is (row j0 + it*js, column b), at LDS index
+// threadIdx.x + it*TPB in the COLS layout.
+struct ColMap {
+  int b, j0, js;
+};
+template <int TPB>
+__device__ __forceinline__ ColMap col_map(int lB) {
+  ColMap m;
+  m.b = threadIdx.x & ((1 << lB) - 1);
+  m.j0 = threadIdx.x >> lB;
+  m.js = TPB >> lB;
+  return m;
+}
+
 // In-place FFT along a strided axis of a complex (outer, n, ncol) view:
 // element j of pencil (o, c) is at o*ostride + j*es + c.  A workgroup takes
 // B consecutive columns, so each of its loads is B contiguous float2.
-// LDS: the n x B block, then the n-entry twiddle table.
-template <int DIR, bool POW2>
-__global__ void __launch_bounds__(FFT_TPB, 4) strided_fft_k(Fields3 fl, int n, long long es,
-                                                         long long ostride, int ncol, int B,
+// LDS: the n x B block, then the n-entry twiddle table.  LN > 0 fixes
+// n = 2^LN and B = fft_pencils(n) at compile time.
+template <int DIR, bool POW2, int LN>
+__global__ void __launch_bounds__(FFT_TPB, 4) strided_fft_k(Fields3 fl, int n_, long long es,
+                                                         long long ostride, int ncol, int B_,
                                                          const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
+  constexpr int BC = LN ? fft_pencils(1 << LN) : 1;
+  const int n = LN ? 1 << LN : n_;
+  const int B = LN ? BC : B_;
+  const int lB = LN ? ilog2c(BC) : __ffs(B_) - 1;
   float2* stw = s + n * B;
   int bx, by, bz;
   xcd_block3(bx, by, bz);
-  float2* base = (bz == 0 ? fl.f[0] : (bz == 1 ? fl.f[1] : fl.f[2])) + (long long)by * ostride;
-  const int lB = __ffs(B) - 1;
-  int c0 = bx * B;
-  int nb = ncol - c0 < B ? ncol - c0 : B;
+  const int c0 = bx * B;
+  const int nb = ncol - c0 < B ? ncol - c0 : B;
+  const ColMap cm = col_map<FFT_TPB>(lB);
+  const bool live = cm.b < nb;
+  float2* g = (bz == 0 ? fl.f[0] : (bz == 1 ? fl.f[1] : fl.f[2])) + (long long)by * ostride + c0 +
+              cm.b + (long long)cm.j0 * es;
+  const long long gs = (long long)cm.js * es;
   stage_twiddles<FFT_TPB>(stw, tw, n);
-  batch_copy<FFT_MAXV, FFT_TPB>(
-      n * B,
-      [&](int idx) {
-        int j = idx >> lB, b = idx & (B - 1);
-        return b < nb ? base[j * es + c0 + b] : make_float2(0.f, 0.f);
-      },
-      [&](int idx, float2 v) { s[idx] = v; });
+  float2 r[FFT_MAXV];
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV; ++it) {
+    r[it] = make_float2(0.f, 0.f);
+    if (live && cm.j0 + it * cm.js < n) r[it] = g[it * gs];
+  }
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV; ++it)
+    if (cm.j0 + it * cm.js < n) s[threadIdx.x + it * FFT_TPB] = r[it];
   __syncthreads();
-  lds_fft<DIR, COLS, POW2>(s, n, B, stw);
-  batch_copy<FFT_MAXV, FFT_TPB>(
-      n * B, [&](int idx) { return s[idx]; },
-      [&](int idx, float2 v) {
-        int j = idx >> lB, b = idx & (B - 1);
-        if (b < nb) base[j * es + c0 + b] = v;
-      });
+  fft_run<DIR, COLS, POW2, LN, ilog2c(BC), FFT_MAXV, FFT_TPB>(s, n, B, stw);
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV; ++it)
+    if (cm.j0 + it * cm.js < n) r[it] = s[threadIdx.x + it * FFT_TPB];
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV; ++it)
+    if (live && cm.j0 + it * cm.js < n) g[it * gs] = r[it];
 }
 
 struct KTab {
@@ -557,80 +733,91 @@ struct KTab {
 // LDS: 2 n0 B values (the forward block, then the two inverse transforms
 // interleaved: column b of the x component at j 2B + b, of h at j 2B + B + b),
 // then the n0-entry twiddle table.
-template <bool POW2>
-__global__ void __launch_bounds__(PFFT_TPB) poisson_x_k(Fields3 fl, int n0, int n1, int n2h, int n2p, int B,
-                                                       KTab kt, const float2* __restrict__ tw) {
+template <bool POW2, int LN>
+__global__ void __launch_bounds__(PFFT_TPB) poisson_x_k(Fields3 fl, int n0_, int n1, int n2h, int n2p,
+                                                       int B_, KTab kt, const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
-  const int total = n0 * B;
+  constexpr int BC = LN ? fft_pencils(1 << LN, PFFT_CAP) : 1;
+  const int n0 = LN ? 1 << LN : n0_;
+  const int B = LN ? BC : B_;
+  const int lB = LN ? ilog2c(BC) : __ffs(B_) - 1;
   const int B2 = 2 * B;
-  float2* stw = s + 2 * total;
+  float2* stw = s + 2 * n0 * B;
+  float* sk = reinterpret_cast<float*>(stw + n0);
+  float* skm = sk + n0;
   const long long es = (long long)n1 * n2p;
-  const int lB = __ffs(B) - 1;
   int bx, y, bz;
   xcd_block3(bx, y, bz);
   const int c0 = bx * B;
   const int nb = n2h - c0 < B ? n2h - c0 : B;
   const long long off = (long long)y * n2p + c0;
   stage_twiddles<PFFT_TPB>(stw, tw, n0);
-  batch_copy<PFFT_MAXV, PFFT_TPB>(
-      total,
-      [&](int idx) {
-        int j = idx >> lB, b = idx & (B - 1);
-        return b < nb ? fl.f[0][off + j * es + b] : make_float2(0.f, 0.f);
-      },
-      [&](int idx, float2 v) { s[idx] = v; });
+  for (int i = threadIdx.x; i < n0; i += PFFT_TPB) {
+    sk[i] = kt.k[0][i];
+    skm[i] = kt.km[0][i];
+  }
+  const ColMap cm = col_map<PFFT_TPB>(lB);
+  const bool live = cm.b < nb;
+  {
+    const float2* g = fl.f[0] + off + cm.b + (long long)cm.j0 * es;
+    const long long gs = (long long)cm.js * es;
+    float2 r[PFFT_MAXV];
+#pragma unroll
+    for (int it = 0; it < PFFT_MAXV; ++it) {
+      r[it] = make_float2(0.f, 0.f);
+      if (live && cm.j0 + it * cm.js < n0) r[it] = g[it * gs];
+    }
+#pragma unroll
+    for (int it = 0; it < PFFT_MAXV; ++it)
+      if (cm.j0 + it * cm.js < n0) s[threadIdx.x + it * PFFT_TPB] = r[it];
+  }
   __syncthreads();
-  lds_fft<-1, COLS, POW2, PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
+  fft_run<-1, COLS, POW2, LN, ilog2c(BC), PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
   const float ky = kt.k[1][y];
+  const float kz = live ? kt.k[2][c0 + cm.b] : 0.f;
   float2 h[PFFT_MAXV];
 #pragma unroll
   for (int it = 0; it < PFFT_MAXV; ++it) {
-    int idx = threadIdx.x + it * PFFT_TPB;
+    const int j = cm.j0 + it * cm.js;
     h[it] = make_float2(0.f, 0.f);
-    if (idx < total) {
-      int j = idx >> lB, b = idx & (B - 1);
-      if (b < nb) {
-        float2 v = s[idx];
-        float kx = kt.k[0][j], kz = kt.k[2][c0 + b];
-        float k2 = kx * kx;
-        k2 = k2 + ky * ky;
-        k2 = k2 + kz * kz;
-        // pot = -v / k^2, h = -i pot = (pot.y, -pot.x)
-        if (k2 != 0.f) h[it] = make_float2((-v.y) / k2, -((-v.x) / k2));
+    if (live && j < n0) {
+      float2 v = s[threadIdx.x + it * PFFT_TPB];
+      float kx = sk[j];
+      float k2 = kx * kx;
+      k2 = k2 + ky * ky;
+      k2 = k2 + kz * kz;
+      // pot = -v / k^2, h = -i pot = (pot.y, -pot.x)
+      if (k2 != 0.f) {
+        const float q = 1.f / k2;
+        h[it] = make_float2(-(v.y * q), v.x * q);
       }
     }
   }
   __syncthreads();
 #pragma unroll
   for (int it = 0; it < PFFT_MAXV; ++it) {
-    int idx = threadIdx.x + it * PFFT_TPB;
-    if (idx < total) {
-      int j = idx >> lB, b = idx & (B - 1);
-      float kmx = kt.km[0][j];
-      s[j * B2 + b] = make_float2(kmx * h[it].x, kmx * h[it].y);
-      s[j * B2 + B + b] = h[it];
+    const int j = cm.j0 + it * cm.js;
+    if (j < n0) {
+      const float kmx = skm[j];
+      const int ls = j * B2 + cm.b;
+      s[ls] = make_float2(kmx * h[it].x, kmx * h[it].y);
+      s[ls + B] = h[it];
     }
   }
   __syncthreads();
-  lds_fft<1, COLS, POW2, 2 * PFFT_MAXV, PFFT_TPB>(s, n0, B2, stw);
+  fft_run<1, COLS, POW2, LN, ilog2c(BC) + 1, 2 * PFFT_MAXV, PFFT_TPB>(s, n0, B2, stw);
+  const ColMap c2 = col_map<PFFT_TPB>(lB + 1);
+  const int b2 = c2.b & (B - 1);
+  const bool live2 = b2 < nb;
+  float2* q = (c2.b < B ? fl.f[0] : fl.f[1]) + off + b2 + (long long)c2.j0 * es;
+  const long long qs = (long long)c2.js * es;
   float2 r[2 * PFFT_MAXV];
 #pragma unroll
-  for (int it = 0; it < 2 * PFFT_MAXV; ++it) {
-    int idx = threadIdx.x + it * PFFT_TPB;
-    if (idx < 2 * total) r[it] = s[idx];
-  }
+  for (int it = 0; it < 2 * PFFT_MAXV; ++it)
+    if (c2.j0 + it * c2.js < n0) r[it] = s[threadIdx.x + it * PFFT_TPB];
 #pragma unroll
-  for (int it = 0; it < 2 * PFFT_MAXV; ++it) {
-    int idx = threadIdx.x + it * PFFT_TPB;
-    if (idx < 2 * total) {
-      int j = idx >> (lB + 1), p = idx & (B2 - 1);
-      int b = p < B ? p : p - B;
-      if (b < nb) {
-        long long o = off + j * es + b;
-        (p < B ? fl.f[0] : fl.f[1])[o] = r[it];
-      }
-    }
-  }
+  for (int it = 0; it < 2 * PFFT_MAXV; ++it)
+    if (live2 && c2.j0 + it * c2.js < n0) q[it * qs] = r[it];
 }
 
 // Adjoint x pass: acc = sum_d (-i km_d) laplace(FFT_x c_d); inverse FFT
@@ -640,70 +827,86 @@ __global__ void __launch_bounds__(PFFT_TPB) poisson_x_k(Fields3 fl, int n0, int 
 // The block holds c_0 and c_1 side by side as 2B pencils (j 2B + b and
 // j 2B + B + b) for one forward pass, then the sum in B pencils for the
 // inverse.  LDS as poisson_x_k.
-template <bool POW2>
-__global__ void __launch_bounds__(PFFT_TPB) poisson_x_adj_k(Fields3 fl, int n0, int n1, int n2h, int n2p, int B,
-                                                           KTab kt, const float2* __restrict__ tw) {
+template <bool POW2, int LN>
+__global__ void __launch_bounds__(PFFT_TPB) poisson_x_adj_k(Fields3 fl, int n0_, int n1, int n2h, int n2p,
+                                                           int B_, KTab kt, const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
-  const int total = n0 * B;
+  constexpr int BC = LN ? fft_pencils(1 << LN, PFFT_CAP) : 1;
+  const int n0 = LN ? 1 << LN : n0_;
+  const int B = LN ? BC : B_;
+  const int lB = LN ? ilog2c(BC) : __ffs(B_) - 1;
   const int B2 = 2 * B;
-  float2* stw = s + 2 * total;
+  float2* stw = s + 2 * n0 * B;
+  float* sk = reinterpret_cast<float*>(stw + n0);
+  float* skm = sk + n0;
   const long long es = (long long)n1 * n2p;
-  const int lB = __ffs(B) - 1;
   int bx, y, bz;
   xcd_block3(bx, y, bz);
   const int c0 = bx * B;
   const int nb = n2h - c0 < B ? n2h - c0 : B;
   const long long off = (long long)y * n2p + c0;
-  const float ky = kt.k[1][y];
   stage_twiddles<PFFT_TPB>(stw, tw, n0);
-  batch_copy<2 * PFFT_MAXV, PFFT_TPB>(
-      2 * total,
-      [&](int idx) {
-        int j = idx >> (lB + 1), p = idx & (B2 - 1);
-        int b = p < B ? p : p - B;
-        float2 v = make_float2(0.f, 0.f);
-        if (b < nb) v = (p < B ? fl.f[0] : fl.f[1])[off + j * es + b];
-        return v;
-      },
-      [&](int idx, float2 v) { s[idx] = v; });
+  for (int i = threadIdx.x; i < n0; i += PFFT_TPB) {
+    sk[i] = kt.k[0][i];
+    skm[i] = kt.km[0][i];
+  }
+  {
+    const ColMap c2 = col_map<PFFT_TPB>(lB + 1);
+    const int b2 = c2.b & (B - 1);
+    const bool live2 = b2 < nb;
+    const float2* q = (c2.b < B ? fl.f[0] : fl.f[1]) + off + b2 + (long long)c2.j0 * es;
+    const long long qs = (long long)c2.js * es;
+    float2 r[2 * PFFT_MAXV];
+#pragma unroll
+    for (int it = 0; it < 2 * PFFT_MAXV; ++it) {
+      r[it] = make_float2(0.f, 0.f);
+      if (live2 && c2.j0 + it * c2.js < n0) r[it] = q[it * qs];
+    }
+#pragma unroll
+    for (int it = 0; it < 2 * PFFT_MAXV; ++it)
+      if (c2.j0 + it * c2.js < n0) s[threadIdx.x + it * PFFT_TPB] = r[it];
+  }
   __syncthreads();
-  lds_fft<-1, COLS, POW2, 2 * PFFT_MAXV, PFFT_TPB>(s, n0, B2, stw);
+  fft_run<-1, COLS, POW2, LN, ilog2c(BC) + 1, 2 * PFFT_MAXV, PFFT_TPB>(s, n0, B2, stw);
+  const ColMap cm = col_map<PFFT_TPB>(lB);
+  const bool live = cm.b < nb;
+  const float ky = kt.k[1][y];
+  const float kz = live ? kt.k[2][c0 + cm.b] : 0.f;
   float2 acc[PFFT_MAXV];
 #pragma unroll
   for (int it = 0; it < PFFT_MAXV; ++it) {
-    int idx = threadIdx.x + it * PFFT_TPB;
+    const int j = cm.j0 + it * cm.js;
     acc[it] = make_float2(0.f, 0.f);
-    if (idx < total) {
-      int j = idx >> lB, b = idx & (B - 1);
-      if (b < nb) {
-        float kx = kt.k[0][j], kz = kt.k[2][c0 + b];
-        float k2 = kx * kx;
-        k2 = k2 + ky * ky;
-        k2 = k2 + kz * kz;
-        if (k2 != 0.f) {
-          float kmx = kt.km[0][j];
-          float2 x0 = s[j * B2 + b], yz = s[j * B2 + B + b];
-          float sx = kmx * x0.x + yz.x, sy = kmx * x0.y + yz.y;
-          // pot = -sum / k^2, acc = -i pot = (pot.y, -pot.x)
-          acc[it] = make_float2((-sy) / k2, -((-sx) / k2));
-        }
+    if (live && j < n0) {
+      float kx = sk[j];
+      float k2 = kx * kx;
+      k2 = k2 + ky * ky;
+      k2 = k2 + kz * kz;
+      if (k2 != 0.f) {
+        const float kmx = skm[j];
+        const int ls = j * B2 + cm.b;
+        float2 x0 = s[ls], yz = s[ls + B];
+        float sx = kmx * x0.x + yz.x, sy = kmx * x0.y + yz.y;
+        // pot = -sum / k^2, acc = -i pot = (pot.y, -pot.x)
+        const float q = 1.f / k2;
+        acc[it] = make_float2(-(sy * q), sx * q);
       }
     }
   }
   __syncthreads();
 #pragma unroll
-  for (int it = 0; it < PFFT_MAXV; ++it) {
-    int idx = threadIdx.x + it * PFFT_TPB;
-    if (idx < total) s[idx] = acc[it];
-  }
+  for (int it = 0; it < PFFT_MAXV; ++it)
+    if (cm.j0 + it * cm.js < n0) s[threadIdx.x + it * PFFT_TPB] = acc[it];
   __syncthreads();
-  lds_fft<1, COLS, POW2, PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
-  batch_copy<PFFT_MAXV, PFFT_TPB>(
-      total, [&](int idx) { return s[idx]; },
-      [&](int idx, float2 v) {
-        int j = idx >> lB, b = idx & (B - 1);
-        if (b < nb) fl.f[0][off + j * es + b] = v;
-      });
+  fft_run<1, COLS, POW2, LN, ilog2c(BC), PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
+  float2* g = fl.f[0] + off + cm.b + (long long)cm.j0 * es;
+  const long long gs = (long long)cm.js * es;
+#pragma unroll
+  for (int it = 0; it < PFFT_MAXV; ++it)
+    if (cm.j0 + it * cm.js < n0) acc[it] = s[threadIdx.x + it * PFFT_TPB];
+#pragma unroll
+  for (int it = 0; it < PFFT_MAXV; ++it)
+    if (live && cm.j0 + it * cm.js < n0) g[it * gs] = acc[it];
 }
 
 // y passes of the gradient components, paired (pencils along axis 1 of
@@ -713,93 +916,118 @@ __global__ void __launch_bounds__(PFFT_TPB) poisson_x_adj_k(Fields3 fl, int n0, 
 // Inverse: from h = IFFT_x(-i pot) in c_1, the y and z force components
 // IFFT_y(km_1 h) to c_1 and IFFT_y(km_2 h) to c_2 -- the products and
 // transforms of a separate y pass over km_1 h and km_2 h, with h read once.
-template <bool POW2>
-__global__ void __launch_bounds__(FFT_TPB, 4) y_inv_pair_k(Fields3 fl, int n1, long long es,
-                                                         long long ostride, int ncol, int B,
+template <bool POW2, int LN>
+__global__ void __launch_bounds__(FFT_TPB, 4) y_inv_pair_k(Fields3 fl, int n_, long long es,
+                                                         long long ostride, int ncol, int B_,
                                                          KTab kt, const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
+  constexpr int BC = LN ? fft_pencils(1 << LN, FFT_CAP / 2) : 1;
+  const int n = LN ? 1 << LN : n_;
+  const int B = LN ? BC : B_;
+  const int lB = LN ? ilog2c(BC) : __ffs(B_) - 1;
   const int B2 = 2 * B;
-  const int total = n1 * B;
-  float2* stw = s + 2 * total;
+  float2* stw = s + 2 * n * B;
   int bx, by, bz;
   xcd_block3(bx, by, bz);
-  const long long off = (long long)by * ostride;
-  const int lB = __ffs(B) - 1;
   const int c0 = bx * B;
   const int nb = ncol - c0 < B ? ncol - c0 : B;
-  stage_twiddles<FFT_TPB>(stw, tw, n1);
-  float2 h[FFT_MAXV / 2];
+  const long long off = (long long)by * ostride + c0;
+  stage_twiddles<FFT_TPB>(stw, tw, n);
+  {
+    const ColMap cm = col_map<FFT_TPB>(lB);
+    const bool live = cm.b < nb;
+    const float kmz = live ? kt.km[2][c0 + cm.b] : 0.f;
+    const float2* g = fl.f[1] + off + cm.b + (long long)cm.j0 * es;
+    const long long gs = (long long)cm.js * es;
+    float2 h[FFT_MAXV / 2];
 #pragma unroll
-  for (int it = 0; it < FFT_MAXV / 2; ++it) {
-    const int idx = threadIdx.x + it * FFT_TPB;
-    h[it] = make_float2(0.f, 0.f);
-    if (idx < total) {
-      const int j = idx >> lB, b = idx & (B - 1);
-      if (b < nb) h[it] = fl.f[1][off + j * es + c0 + b];
+    for (int it = 0; it < FFT_MAXV / 2; ++it) {
+      h[it] = make_float2(0.f, 0.f);
+      if (live && cm.j0 + it * cm.js < n) h[it] = g[it * gs];
     }
-  }
 #pragma unroll
-  for (int it = 0; it < FFT_MAXV / 2; ++it) {
-    const int idx = threadIdx.x + it * FFT_TPB;
-    if (idx < total) {
-      const int j = idx >> lB, b = idx & (B - 1);
-      const float kmy = kt.km[1][j], kmz = b < nb ? kt.km[2][c0 + b] : 0.f;
-      s[j * B2 + b] = make_float2(kmy * h[it].x, kmy * h[it].y);
-      s[j * B2 + B + b] = make_float2(kmz * h[it].x, kmz * h[it].y);
+    for (int it = 0; it < FFT_MAXV / 2; ++it) {
+      const int j = cm.j0 + it * cm.js;
+      if (j < n) {
+        const float kmy = kt.km[1][j];
+        const int ls = j * B2 + cm.b;
+        s[ls] = make_float2(kmy * h[it].x, kmy * h[it].y);
+        s[ls + B] = make_float2(kmz * h[it].x, kmz * h[it].y);
+      }
     }
   }
   __syncthreads();
-  lds_fft<1, COLS, POW2>(s, n1, B2, stw);
-  batch_copy<FFT_MAXV, FFT_TPB>(
-      2 * total, [&](int idx) { return s[idx]; },
-      [&](int idx, float2 v) {
-        const int j = idx >> (lB + 1), p = idx & (B2 - 1);
-        const int b = p < B ? p : p - B;
-        if (b < nb) (p < B ? fl.f[1] : fl.f[2])[off + j * es + c0 + b] = v;
-      });
+  fft_run<1, COLS, POW2, LN, ilog2c(BC) + 1, FFT_MAXV, FFT_TPB>(s, n, B2, stw);
+  const ColMap c2 = col_map<FFT_TPB>(lB + 1);
+  const int b2 = c2.b & (B - 1);
+  const bool live2 = b2 < nb;
+  float2* q = (c2.b < B ? fl.f[1] : fl.f[2]) + off + b2 + (long long)c2.j0 * es;
+  const long long qs = (long long)c2.js * es;
+  float2 r[FFT_MAXV];
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV; ++it)
+    if (c2.j0 + it * c2.js < n) r[it] = s[threadIdx.x + it * FFT_TPB];
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV; ++it)
+    if (live2 && c2.j0 + it * c2.js < n) q[it * qs] = r[it];
 }
 
 // Forward (adjoint path): FFT_y of c_1 and c_2 side by side, then their
-// combination km_1 FFT_y(c_1) + km_2 FFT_y(c_2) to c_1 for poisson_x_adj_k
-// -- the products of the x pass that used to form it, moved here.
-template <bool POW2>
-__global__ void __launch_bounds__(FFT_TPB, 4) y_fwd_pair_k(Fields3 fl, int n1, long long es,
-                                                         long long ostride, int ncol, int B,
+// combination km_1 FFT_y(c_1) + km_2 FFT_y(c_2) to c_1 for poisson_x_adj_k.
+template <bool POW2, int LN>
+__global__ void __launch_bounds__(FFT_TPB, 4) y_fwd_pair_k(Fields3 fl, int n_, long long es,
+                                                         long long ostride, int ncol, int B_,
                                                          KTab kt, const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
+  constexpr int BC = LN ? fft_pencils(1 << LN, FFT_CAP / 2) : 1;
+  const int n = LN ? 1 << LN : n_;
+  const int B = LN ? BC : B_;
+  const int lB = LN ? ilog2c(BC) : __ffs(B_) - 1;
   const int B2 = 2 * B;
-  const int total = n1 * B;
-  float2* stw = s + 2 * total;
+  float2* stw = s + 2 * n * B;
   int bx, by, bz;
   xcd_block3(bx, by, bz);
-  const long long off = (long long)by * ostride;
-  const int lB = __ffs(B) - 1;
   const int c0 = bx * B;
   const int nb = ncol - c0 < B ? ncol - c0 : B;
-  stage_twiddles<FFT_TPB>(stw, tw, n1);
-  batch_copy<FFT_MAXV, FFT_TPB>(
-      2 * total,
-      [&](int idx) {
-        const int j = idx >> (lB + 1), p = idx & (B2 - 1);
-        const int b = p < B ? p : p - B;
-        return b < nb ? (p < B ? fl.f[1] : fl.f[2])[off + j * es + c0 + b]
-                      : make_float2(0.f, 0.f);
-      },
-      [&](int idx, float2 v) { s[idx] = v; });
+  const long long off = (long long)by * ostride + c0;
+  stage_twiddles<FFT_TPB>(stw, tw, n);
+  {
+    const ColMap c2 = col_map<FFT_TPB>(lB + 1);
+    const int b2 = c2.b & (B - 1);
+    const bool live2 = b2 < nb;
+    const float2* q = (c2.b < B ? fl.f[1] : fl.f[2]) + off + b2 + (long long)c2.j0 * es;
+    const long long qs = (long long)c2.js * es;
+    float2 r[FFT_MAXV];
+#pragma unroll
+    for (int it = 0; it < FFT_MAXV; ++it) {
+      r[it] = make_float2(0.f, 0.f);
+      if (live2 && c2.j0 + it * c2.js < n) r[it] = q[it * qs];
+    }
+#pragma unroll
+    for (int it = 0; it < FFT_MAXV; ++it)
+      if (c2.j0 + it * c2.js < n) s[threadIdx.x + it * FFT_TPB] = r[it];
+  }
   __syncthreads();
-  lds_fft<-1, COLS, POW2>(s, n1, B2, stw);
-  batch_copy<FFT_MAXV / 2, FFT_TPB>(
-      total,
-      [&](int idx) {
-        const int j = idx >> lB, b = idx & (B - 1);
-        const float2 a = s[j * B2 + b], c = s[j * B2 + B + b];
-        const float kmy = kt.km[1][j], kmz = b < nb ? kt.km[2][c0 + b] : 0.f;
-        return make_float2(kmy * a.x + kmz * c.x, kmy * a.y + kmz * c.y);
-      },
-      [&](int idx, float2 v) {
-        const int j = idx >> lB, b = idx & (B - 1);
-        if (b < nb) fl.f[1][off + j * es + c0 + b] = v;
-      });
+  fft_run<-1, COLS, POW2, LN, ilog2c(BC) + 1, FFT_MAXV, FFT_TPB>(s, n, B2, stw);
+  const ColMap cm = col_map<FFT_TPB>(lB);
+  const bool live = cm.b < nb;
+  const float kmz = live ? kt.km[2][c0 + cm.b] : 0.f;
+  float2* g = fl.f[1] + off + cm.b + (long long)cm.j0 * es;
+  const long long gs = (long long)cm.js * es;
+  float2 r[FFT_MAXV / 2];
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV / 2; ++it) {
+    const int j = cm.j0 + it * cm.js;
+    if (j < n) {
+      const int ls = j * B2 + cm.b;
+      const float2 a = s[ls], c = s[ls + B];
+      const float kmy = kt.km[1][j];
+      r[it] = make_float2(kmy * a.x + kmz * c.x, kmy * a.y + kmz * c.y);
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV / 2; ++it)
+    if (live && cm.j0 + it * cm.js < n) g[it * gs] = r[it];
 }
 
 // ------------------------------------------------------------- copies --
@@ -833,13 +1061,6 @@ static size_t lds_bytes(int n, int P, int lay = COLS) {
 // the twiddle table staged after the data block
 static size_t tw_bytes(int n) { return (size_t)n * sizeof(float2); }
 
-// rows per workgroup for the row passes: P*n <= RFFT_CAP and the swizzled
-// ROWS block within FFT_CAP elements (64 KB)
-static int row_pencils(int n) {
-  int P = 1;
-  while (P * 2 * n <= RFFT_CAP && lds_elems(ROWS, n, P * 2) <= FFT_CAP && P * 2 <= 256) P *= 2;
-  return P;
-}
 
 static bool half_ok(int n2) { return n2 >= 4 && (n2 & 1) == 0 && is_pow2(n2 / 2); }
 
@@ -859,6 +1080,27 @@ static RowPlan row_plan(int n2) {
   return r;
 }
 
+// log2 of a length that has compile-time kernel variants (64..1024), else 0
+static int ct_log2(int n) {
+  if (!is_pow2(n) || n < 64 || n > 1024) return 0;
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l;
+}
+
+// CALL(L) with L the compile-time constant for ln (0: generic kernels)
+#define PMX_CT_SWITCH(ln, CALL) \
+  do {                          \
+    switch (ln) {               \
+      case 6: CALL(6); break;   \
+      case 7: CALL(7); break;   \
+      case 8: CALL(8); break;   \
+      case 9: CALL(9); break;   \
+      case 10: CALL(10); break; \
+      default: CALL(0);         \
+    }                           \
+  } while (0)
+
 int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef) {
   const long long nrows = (long long)c->n[0] * c->n[1];
   const int n2 = c->n[2];
@@ -869,10 +1111,14 @@ int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef) {
   const size_t lds = r.lds;
   pmx_tic(c, PMX_T_ZR2C);
   if (r.half) {
-    if (pre)
-      z_r2c_half_k<1><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
-    else
-      z_r2c_half_k<0><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
+    const int lh = nrows % P == 0 ? ct_log2(r.len) : 0;
+#define R2C_HALF(L)                                                                            \
+  if (pre)                                                                                     \
+    z_r2c_half_k<1, L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]); \
+  else                                                                                         \
+    z_r2c_half_k<0, L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2])
+    PMX_CT_SWITCH(lh, R2C_HALF);
+#undef R2C_HALF
   } else {
     const bool p2 = is_pow2(n2);
     if (pre && p2)
@@ -898,12 +1144,17 @@ int pmx_fft_c2r_inplace(pmx_ctx* c, float* buf, float scale) {
   const int P = r.P;
   const size_t lds = r.lds;
   pmx_tic(c, PMX_T_ZC2R);
-  if (r.half)
-    z_c2r_half_k<<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2]);
-  else if (is_pow2(n2))
+  if (r.half) {
+    const int lh = nrows % P == 0 ? ct_log2(r.len) : 0;
+#define C2R_HALF(L) \
+  z_c2r_half_k<L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2])
+    PMX_CT_SWITCH(lh, C2R_HALF);
+#undef C2R_HALF
+  } else if (is_pow2(n2)) {
     z_c2r_k<true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2]);
-  else
+  } else {
     z_c2r_k<false><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2]);
+  }
   pmx_toc(c, PMX_T_ZC2R);
   PMX_LAUNCH_CHECK();
   return PMX_OK;
@@ -930,17 +1181,21 @@ int pmx_fft_strided(pmx_ctx* c, float2* const* f, int nf, int axis, int dir) {
   dim3 grid((ncol + B - 1) / B, nouter, nf);
   size_t lds = lds_bytes(n, B) + tw_bytes(n);
   if (int rc = fft_fits("strided FFT", n, n * B, FFT_CAP, lds)) return rc;
-  bool p2 = is_pow2(n);
-  int cls = axis == 1 ? (dir < 0 ? PMX_T_YFWD : PMX_T_YINV) : PMX_T_OTHER;
+  const bool p2 = is_pow2(n);
+  const int ln = ct_log2(n);
+  const int cls = axis == 1 ? (dir < 0 ? PMX_T_YFWD : PMX_T_YINV) : PMX_T_OTHER;
   pmx_tic(c, cls);
-  if (dir < 0 && p2)
-    strided_fft_k<-1, true><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, c->tw[axis]);
-  else if (dir < 0)
-    strided_fft_k<-1, false><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, c->tw[axis]);
-  else if (p2)
-    strided_fft_k<1, true><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, c->tw[axis]);
-  else
-    strided_fft_k<1, false><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, c->tw[axis]);
+#define STRIDED(L)                                                                                  \
+  if (dir < 0 && (L || p2))                                                                         \
+    strided_fft_k<-1, true, L><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, c->tw[axis]); \
+  else if (dir < 0)                                                                                 \
+    strided_fft_k<-1, false, 0><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, c->tw[axis]); \
+  else if (L || p2)                                                                                 \
+    strided_fft_k<1, true, L><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, c->tw[axis]);  \
+  else                                                                                              \
+    strided_fft_k<1, false, 0><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, c->tw[axis])
+  PMX_CT_SWITCH(ln, STRIDED);
+#undef STRIDED
   pmx_toc(c, cls);
   PMX_LAUNCH_CHECK();
   return PMX_OK;
@@ -967,19 +1222,24 @@ static int pmx_fft_ypair(pmx_ctx* c, float2* const* f, int dir) {
   const long long es = c->n2p, ostride = (long long)n * c->n2p;
   dim3 grid((ncol + B - 1) / B, c->n[0], 1);
   size_t lds = 2 * lds_bytes(n, B) + tw_bytes(n);
+  // 2 n B fits for every axis length the context accepts (<= 4096)
   if (int rc = fft_fits("paired y FFT", n, 2 * n * B, FFT_CAP, lds)) return rc;
-  const bool p2 = is_pow2(n);
   const KTab kt = ktab(c);
+  const bool p2 = is_pow2(n);
+  const int ln = ct_log2(n);
   const int cls = dir < 0 ? PMX_T_YFWD : PMX_T_YINV;
   pmx_tic(c, cls);
-  if (dir < 0 && p2)
-    y_fwd_pair_k<true><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]);
-  else if (dir < 0)
-    y_fwd_pair_k<false><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]);
-  else if (p2)
-    y_inv_pair_k<true><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]);
-  else
-    y_inv_pair_k<false><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]);
+#define YPAIR(L)                                                                                  \
+  if (dir < 0 && (L || p2))                                                                       \
+    y_fwd_pair_k<true, L><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]); \
+  else if (dir < 0)                                                                               \
+    y_fwd_pair_k<false, 0><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]); \
+  else if (L || p2)                                                                               \
+    y_inv_pair_k<true, L><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]); \
+  else                                                                                            \
+    y_inv_pair_k<false, 0><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1])
+  PMX_CT_SWITCH(ln, YPAIR);
+#undef YPAIR
   pmx_toc(c, cls);
   PMX_LAUNCH_CHECK();
   return PMX_OK;
@@ -987,21 +1247,27 @@ static int pmx_fft_ypair(pmx_ctx* c, float2* const* f, int dir) {
 
 int pmx_poisson_x(pmx_ctx* c, float2* const* f, int adjoint) {
   Fields3 fl = {{f[0], f[1], f[2]}};
-  int n0 = c->n[0];
-  int B = fft_pencils(n0, PFFT_CAP);
+  const int n0 = c->n[0];
+  const int B = fft_pencils(n0, PFFT_CAP);
   dim3 grid((c->n2h + B - 1) / B, c->n[1], 1);
-  size_t lds = 2 * lds_bytes(n0, B) + tw_bytes(n0);
+  // the data block, the twiddles, then the k_0 and km_0 tables
+  size_t lds = 2 * lds_bytes(n0, B) + tw_bytes(n0) + 2 * (size_t)n0 * sizeof(float);
   if (int rc = fft_fits("Poisson x pass", n0, n0 * B, PFFT_CAP, lds)) return rc;
-  bool p2 = is_pow2(n0);
+  const bool p2 = is_pow2(n0);
+  const int ln = ct_log2(n0);
+  const KTab kt = ktab(c);
   pmx_tic(c, PMX_T_POISSON);
-  if (adjoint && p2)
-    poisson_x_adj_k<true><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, ktab(c), c->tw[0]);
-  else if (adjoint)
-    poisson_x_adj_k<false><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, ktab(c), c->tw[0]);
-  else if (p2)
-    poisson_x_k<true><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, ktab(c), c->tw[0]);
-  else
-    poisson_x_k<false><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, ktab(c), c->tw[0]);
+#define PX(L)                                                                                             \
+  if (adjoint && (L || p2))                                                                               \
+    poisson_x_adj_k<true, L><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, kt, c->tw[0]); \
+  else if (adjoint)                                                                                       \
+    poisson_x_adj_k<false, 0><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, kt, c->tw[0]); \
+  else if (L || p2)                                                                                       \
+    poisson_x_k<true, L><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, kt, c->tw[0]); \
+  else                                                                                                    \
+    poisson_x_k<false, 0><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, kt, c->tw[0])
+  PMX_CT_SWITCH(ln, PX);
+#undef PX
   pmx_toc(c, PMX_T_POISSON);
   PMX_LAUNCH_CHECK();
   return PMX_OK;

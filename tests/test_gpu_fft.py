@@ -9,7 +9,8 @@ from oracle import pm as opm
 pytestmark = pytest.mark.gpu
 
 FFT_SHAPES = [(8, 8, 8), (16, 32, 64), (4, 9, 7), (7, 8, 5), (12, 10, 6), (64, 64, 64),
-              (2, 4, 1024), (1024, 2, 4), (3, 5, 2), (1, 1, 8), (128, 128, 128)]
+              (2, 4, 1024), (1024, 2, 4), (3, 5, 2), (1, 1, 8), (128, 128, 128),
+              (256, 64, 512), (2, 4, 2048), (3, 256, 8)]
 
 
 def rel_l2(a, b):
@@ -55,8 +56,12 @@ def test_fft_norms(dev):
     assert rel_l2(y.cpu().numpy(), x) < 2e-6
 
 
+# (128, 64, 256) and (64, 1024, 16) run the fixed-length kernel variants
+# (x 64/128, y 64/1024, z half-length 128); (2, 4096, 4) has the longest y
+# axis, one pencil pair per block in the paired y pass
 POISSON = [((16, 16, 16), 2.0), ((4, 9, 8), 7.0), ((7, 8, 6), 7.0), ((32, 16, 24), 1.0),
-           ((64, 64, 64), 3.0), ((1, 1, 2), 1.0)]
+           ((64, 64, 64), 3.0), ((1, 1, 2), 1.0), ((128, 64, 256), 1.0), ((64, 1024, 16), 2.0),
+           ((2, 4096, 4), 1.0)]
 
 
 @pytest.mark.parametrize('shape,cell', POISSON)
