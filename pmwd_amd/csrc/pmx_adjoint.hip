@@ -178,9 +178,9 @@ static int force_vjp_binned(pmx_ctx* c, long long n, float coef, float val, cons
   PMX_LAUNCH_CHECK();
   // the cotangent of each force mesh: scatter(vel_cot_d * w)
   float* meshes[3] = {(float*)c->cbuf[0], (float*)c->cbuf[1], (float*)c->cbuf[2]};
-  if ((rc = pmx_deposit_binned3(c, n, meshes, c->rs))) return rc;
+  if ((rc = pmx_deposit_binned3(c, n, meshes, c->rs, 1))) return rc;
   // density cotangent -coef * sum_d L_d(mesh_cot_d), L_d = irfftn (-ik_d) laplace rfftn
-  if ((rc = pmx_poisson_adj_ctx(c, (float)(-(double)coef / (double)c->M)))) return rc;
+  if ((rc = pmx_poisson_adj_ctx(c, (float)(-(double)coef / (double)c->M), 1))) return rc;
   pmx_tic(c, PMX_T_GRADG);
   tile_grad_gather_k<<<nt, ADJ_TPB, cells * sizeof(float), c->stream>>>(
       c->rec, c->pidx, c->tile_start, c->tile_count, g, tg, (const float*)c->cbuf[0], val,
@@ -400,8 +400,8 @@ extern "C" int pmx_nbody_step_adj(pmx_ctx* c, const int16_t* pmid, float* disp, 
     return rc;
   // disp now holds the binned positions (disp_snap when given)
   if ((rc = pmx_bin_end(c, pmid, disp, n, cap))) return rc;
-  if ((rc = pmx_deposit_binned(c, val, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
-  if ((rc = pmx_poisson_ctx(c, coef))) return rc;
+  if ((rc = pmx_deposit_binned(c, val, n, (float*)c->cbuf[0], c->rs, 0, 1))) return rc;
+  if ((rc = pmx_poisson_ctx(c, coef, 1))) return rc;
   if ((rc = force_vjp_binned(c, n, coef, val, vel_cot, acc, acc_cot, nb1))) return rc;
   if (!dots) return PMX_OK;
   if ((rc = pmx_red_reduce(c, 0, nb1, 0))) return rc;

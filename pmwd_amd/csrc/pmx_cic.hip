@@ -517,8 +517,9 @@ int pmx_bin_particles(pmx_ctx* c, const int16_t* pmid, const float* disp, long l
 
 // Deposits the binned records into mesh (row stride mrs), with the scalar
 // value val or the per-record values of channel ch of c->vrec (ch >= 0).
+// defer: leave the fold to the Poisson solve's row pass when it can take it.
 static int deposit_run(pmx_ctx* c, float val, int perval, int nch, long long n, DepOut out,
-                       int mrs, int accumulate) {
+                       int mrs, int accumulate, int defer) {
   const MeshGeom g = pmx_geom_padded(c);
   const TileGeom& tg = c->tg;
   const size_t lds = (size_t)(tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(unsigned long long);
@@ -538,6 +539,7 @@ static int deposit_run(pmx_ctx* c, float val, int perval, int nch, long long n, 
                                                   nch, mrs, accumulate, c->halo, hstride);
   pmx_toc(c, PMX_T_DEPOSIT);
   PMX_LAUNCH_CHECK();
+  if (defer && !accumulate && pmx_fold_deferrable(c)) return PMX_OK;
   pmx_tic(c, PMX_T_FOLD);
   fold_k<<<nt, PMX_TPB, 0, c->stream>>>(out, nch, mrs, c->halo, hstride, g, tg);
   pmx_toc(c, PMX_T_FOLD);
@@ -547,24 +549,25 @@ static int deposit_run(pmx_ctx* c, float val, int perval, int nch, long long n, 
 
 // Deposits the binned records into mesh (row stride mrs) with the scalar
 // value val.
-int pmx_deposit_binned(pmx_ctx* c, float val, long long n, float* mesh, int mrs, int accumulate) {
+int pmx_deposit_binned(pmx_ctx* c, float val, long long n, float* mesh, int mrs, int accumulate,
+                       int defer) {
   DepOut out = {{mesh, mesh, mesh}};
-  return deposit_run(c, val, 0, 1, n, out, mrs, accumulate);
+  return deposit_run(c, val, 0, 1, n, out, mrs, accumulate, defer);
 }
 
 // Deposits the three per-record value channels of c->vrec into three meshes
 // (row stride mrs), overwriting them.
-int pmx_deposit_binned3(pmx_ctx* c, long long n, float* const* mesh, int mrs) {
+int pmx_deposit_binned3(pmx_ctx* c, long long n, float* const* mesh, int mrs, int defer) {
   DepOut out = {{mesh[0], mesh[1], mesh[2]}};
-  return deposit_run(c, 0.f, 1, 3, n, out, mrs, 0);
+  return deposit_run(c, 0.f, 1, 3, n, out, mrs, 0, defer);
 }
 
 // Deposits val per tile-ordered particle (st, tile ranges start/count) into
 // mesh (row stride mrs), overwriting it.
 int pmx_deposit_state(pmx_ctx* c, const PState& st, const uint32_t* start, const uint32_t* count,
-                      float val, float* mesh, int mrs) {
+                      float val, float* mesh, int mrs, int defer) {
   DepOut out = {{mesh, mesh, mesh}};
-  return pmx_deposit_state_vals(c, st, start, count, val, nullptr, 0, 0, 1, out, mrs);
+  return pmx_deposit_state_vals(c, st, start, count, val, nullptr, 0, 0, 1, out, mrs, defer);
 }
 
 // Deposits tile-ordered particles into nch meshes (row stride mrs),
@@ -572,7 +575,7 @@ int pmx_deposit_state(pmx_ctx* c, const PState& st, const uint32_t* start, const
 // vals[ch * vstride + i * vstep] with the fixed-point scale from c->vmax.
 int pmx_deposit_state_vals(pmx_ctx* c, const PState& st, const uint32_t* start,
                            const uint32_t* count, float val, const float* vals, long long vstride,
-                           int vstep, int nch, DepOut out, int mrs) {
+                           int vstep, int nch, DepOut out, int mrs, int defer) {
   const MeshGeom g = pmx_geom_padded(c);
   const TileGeom& tg = c->tg;
   const size_t lds = (size_t)(tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(unsigned long long);
@@ -591,6 +594,7 @@ int pmx_deposit_state_vals(pmx_ctx* c, const PState& st, const uint32_t* start,
                                                      hstride);
   pmx_toc(c, PMX_T_DEPOSIT);
   PMX_LAUNCH_CHECK();
+  if (defer && pmx_fold_deferrable(c)) return PMX_OK;
   pmx_tic(c, PMX_T_FOLD);
   fold_k<<<nt, PMX_TPB, 0, c->stream>>>(out, nch, mrs, c->halo, hstride, g, tg);
   pmx_toc(c, PMX_T_FOLD);
@@ -635,8 +639,8 @@ int pmx_gravity_fwd_meshes(pmx_ctx* c, const int16_t* pmid, const float* disp, l
                               float val, float coef) {
   int rc;
   if ((rc = pmx_bin_particles(c, pmid, disp, n))) return rc;
-  if ((rc = pmx_deposit_binned(c, val, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
-  return pmx_poisson_ctx(c, coef);
+  if ((rc = pmx_deposit_binned(c, val, n, (float*)c->cbuf[0], c->rs, 0, 1))) return rc;
+  return pmx_poisson_ctx(c, coef, 1);
 }
 
 extern "C" int pmx_gravity(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
@@ -742,8 +746,8 @@ extern "C" int pmx_nbody_step(pmx_ctx* c, const int16_t* pmid, float* disp, floa
                                          pmx_bin_out(c), cap)))
     return rc;
   if ((rc = pmx_bin_end(c, pmid, disp, n, cap))) return rc;
-  if ((rc = pmx_deposit_binned(c, val, n, (float*)c->cbuf[0], c->rs, 0))) return rc;
-  if ((rc = pmx_poisson_ctx(c, coef))) return rc;
+  if ((rc = pmx_deposit_binned(c, val, n, (float*)c->cbuf[0], c->rs, 0, 1))) return rc;
+  if ((rc = pmx_poisson_ctx(c, coef, 1))) return rc;
   if (n > 0 && (rc = pmx_tile_gather(c, acc, vel, k2))) return rc;
   return PMX_OK;
 }

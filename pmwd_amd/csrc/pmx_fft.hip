@@ -514,10 +514,33 @@ __device__ __forceinline__ float2 r2c_half_x(const float2* s, const float2* stw,
   return cadd(e, cmul(stw[k], o));
 }
 
-template <int PRE, int LH>
+// The deposit's halo shells of one channel for a row pass that folds them
+// in on load (16^3 tiles; n1 the y extent of the mesh).
+struct FoldIn {
+  const float* halo;
+  int shell, ntx, nty, ntz, n1;
+};
+
+// Row (x, y), z cells 2m and 2m+1: each cell on a low face of its tile gets
+// the fold's shell sum added, as fold_k adds it to the mesh.
+__device__ __forceinline__ float2 fold_pair(float2 v, const FoldIn& fi, int x, int y, int m) {
+  const int lx = x & 15, ly = y & 15, z = 2 * m, lz = z & 15;
+  if (lx == 0 || ly == 0 || lz == 0)
+    v.x = v.x + pmx_fold_sum16(fi.halo, fi.shell, fi.ntx, fi.nty, fi.ntz, x >> 4, y >> 4, z >> 4,
+                               lx, ly, lz);
+  if (lx == 0 || ly == 0)
+    v.y = v.y + pmx_fold_sum16(fi.halo, fi.shell, fi.ntx, fi.nty, fi.ntz, x >> 4, y >> 4, z >> 4,
+                               lx, ly, lz + 1);
+  return v;
+}
+
+// FOLD (with LH > 0, 16^3 tiles, P | n1): the density is the deposit's tile
+// interiors plus the fold of fi.halo, added here on load.
+template <int PRE, int LH, bool FOLD = false>
 __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_half_k(float* __restrict__ data, long long nrows,
                                                            int n2, int rs, int P_, float coef,
-                                                           const float2* __restrict__ tw) {
+                                                           const float2* __restrict__ tw,
+                                                           FoldIn fi) {
   extern __shared__ float2 s[];
   constexpr int PC = LH ? row_pencils(1 << LH) : 1;
   const int H = LH ? 1 << LH : n2 >> 1;
@@ -530,6 +553,9 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_half_k(float* __restrict__ d
   float2* blk = reinterpret_cast<float2*>(data) + row0 * rsc;
   const int total = np * H;
   stage_twiddles<FFT_TPB>(stw, tw, 2 * H);
+  // the block's rows lie in one x plane when folding
+  const int fx = FOLD ? (int)row0 / fi.n1 : 0;
+  const int fy0 = FOLD ? (int)row0 - fx * fi.n1 : 0;
   {
     float2 r[RFFT_MAXV];
 #pragma unroll
@@ -538,6 +564,7 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_half_k(float* __restrict__ d
       if (idx < total) {
         const int p = idx >> lH, m = idx & (H - 1);
         float2 v = blk[__umul24(p, rsc) + m];
+        if (FOLD) v = fold_pair(v, fi, fx, fy0 + p, m);
         if (PRE) {
           v.x = v.x - 1.f;
           v.x = v.x * coef;
@@ -1101,7 +1128,19 @@ static int ct_log2(int n) {
     }                           \
   } while (0)
 
-int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef) {
+int pmx_fold_deferrable(const pmx_ctx* c) {
+  if (!c->fold_on_load) return 0;
+  const TileGeom& t = c->tg;
+  if (t.tx != 16 || t.ty != 16 || t.tz != 16) return 0;
+  if (c->n[0] % 16 || c->n[1] % 16 || c->n[2] % 16) return 0;
+  const RowPlan r = row_plan(c->n[2]);
+  return r.half && ct_log2(r.len) > 0 && c->n[1] % r.P == 0;
+}
+
+// In-place r2c of a padded real mesh; pre applies (x - 1) * coef.  With
+// halo (one channel's shells; pmx_fold_deferrable must hold) the deposit's
+// fold is added on load.
+int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef, const float* halo = nullptr) {
   const long long nrows = (long long)c->n[0] * c->n[1];
   const int n2 = c->n[2];
   const RowPlan r = row_plan(n2);
@@ -1109,14 +1148,22 @@ int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef) {
   const unsigned nblk = (unsigned)((nrows + r.P - 1) / r.P);
   const int P = r.P;
   const size_t lds = r.lds;
+  const FoldIn fi = {halo, c->tg.shell, c->tg.ntx, c->tg.nty, c->tg.ntz, c->n[1]};
+  PMX_REQUIRE(!halo || pmx_fold_deferrable(c), "z r2c: no fold on load for this mesh");
   pmx_tic(c, PMX_T_ZR2C);
   if (r.half) {
     const int lh = nrows % P == 0 ? ct_log2(r.len) : 0;
-#define R2C_HALF(L)                                                                            \
-  if (pre)                                                                                     \
-    z_r2c_half_k<1, L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]); \
-  else                                                                                         \
-    z_r2c_half_k<0, L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2])
+#define R2C_HALF(L)                                                                                \
+  if (halo && pre)                                                                                 \
+    z_r2c_half_k<1, L, (L > 0)><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef,  \
+                                                                  c->tw[2], fi);                   \
+  else if (halo)                                                                                   \
+    z_r2c_half_k<0, L, (L > 0)><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef,  \
+                                                                  c->tw[2], fi);                   \
+  else if (pre)                                                                                    \
+    z_r2c_half_k<1, L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2], fi); \
+  else                                                                                             \
+    z_r2c_half_k<0, L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2], fi)
     PMX_CT_SWITCH(lh, R2C_HALF);
 #undef R2C_HALF
   } else {
@@ -1274,12 +1321,14 @@ int pmx_poisson_x(pmx_ctx* c, float2* const* f, int adjoint) {
 }
 
 // Full forward Poisson on the context buffers: cbuf[0] holds the density in
-// the padded real layout; on return cbuf[0..2] hold the three force meshes
-// (padded real layout), scaled 1/M.
-int pmx_poisson_ctx(pmx_ctx* c, float coef) {
+// the padded real layout (deferred: its tile interiors, the fold of c->halo
+// still to add); on return cbuf[0..2] hold the three force meshes (padded
+// real layout), scaled 1/M.
+int pmx_poisson_ctx(pmx_ctx* c, float coef, int deferred) {
   int rc;
   float2* f[3] = {c->cbuf[0], c->cbuf[1], c->cbuf[2]};
-  if ((rc = pmx_fft_r2c_inplace(c, (float*)f[0], 1, coef))) return rc;
+  const float* halo = deferred && pmx_fold_deferrable(c) ? c->halo : nullptr;
+  if ((rc = pmx_fft_r2c_inplace(c, (float*)f[0], 1, coef, halo))) return rc;
   if ((rc = pmx_fft_strided(c, f, 1, 1, -1))) return rc;
   if ((rc = pmx_poisson_x(c, f, 0))) return rc;
   if ((rc = pmx_fft_strided(c, f, 1, 1, 1))) return rc;
@@ -1291,18 +1340,22 @@ int pmx_poisson_ctx(pmx_ctx* c, float coef) {
 }
 
 // Adjoint Poisson on the context buffers: cbuf[0..2] hold the three
-// cotangent meshes (padded real); on return cbuf[0] holds the density
-// cotangent (padded real) times `scale_out`.
-int pmx_poisson_adj_ctx(pmx_ctx* c, float scale_out) {
+// cotangent meshes (padded real; deferred: with the folds of c->halo's three
+// channels still to add); on return cbuf[0] holds the density cotangent
+// (padded real) times `scale_out`.
+int pmx_poisson_adj_ctx(pmx_ctx* c, float scale_out, int deferred) {
   float2* f[3] = {c->cbuf[0], c->cbuf[1], c->cbuf[2]};
-  return pmx_poisson_adj_bufs(c, f, scale_out);
+  return pmx_poisson_adj_bufs(c, f, scale_out, deferred);
 }
 
 // The same on three other mesh buffers of the context's layout.
-int pmx_poisson_adj_bufs(pmx_ctx* c, float2* const* f, float scale_out) {
+int pmx_poisson_adj_bufs(pmx_ctx* c, float2* const* f, float scale_out, int deferred) {
   int rc;
+  const bool fold = deferred && pmx_fold_deferrable(c);
+  const long long hstride = c->ntiles * c->tg.shell;
   for (int d = 0; d < 3; ++d)
-    if ((rc = pmx_fft_r2c_inplace(c, (float*)f[d], 0, 0.f))) return rc;
+    if ((rc = pmx_fft_r2c_inplace(c, (float*)f[d], 0, 0.f, fold ? c->halo + d * hstride : nullptr)))
+      return rc;
   if ((rc = pmx_fft_strided(c, f, 1, 1, -1))) return rc;
   if ((rc = pmx_fft_ypair(c, f, -1))) return rc;
   if ((rc = pmx_poisson_x(c, f, 1))) return rc;

@@ -173,6 +173,8 @@ struct pmx_ctx {
   long long al_dots_cap;
   float* halo;         // nch shells per tile, channel-major
   int halo_nch;        // channels the halo buffer holds
+  int fold_on_load;    // the Poisson row pass may take the deposit's fold
+                       // (environment PMX_FOLD_ON_LOAD=0 at creation: never)
   long long ptcl_cap;
   float4* rec;         // binned particle records (d per axis, packed local cell)
   uint32_t* pkey;      // tile of each particle
@@ -499,6 +501,31 @@ __device__ __forceinline__ void tile_origin(int t, const TileGeom& tg, const Mes
   s[2] = min(tg.tz, g.n2 - o[2]);
 }
 
+// The fold of 16^3 tiles at tile-local cell (lx, ly, lz) of tile (tx, ty,
+// tz): the +1 shells of the lower neighbours that cover the cell (S = 1..7,
+// bits 0/1/2 a step down in x/y/z, periodic), summed in that order from 0
+// as fold_k sums them.  Shell layout of a 16^3 tile: the x-high face (17 x
+// 17), the y-high face (16 x 17), then the z-high face (16 x 16).
+__device__ __forceinline__ float pmx_fold_sum16(const float* __restrict__ halo, int shell, int ntx,
+                                                int nty, int ntz, int tx, int ty, int tz, int lx,
+                                                int ly, int lz) {
+  float sum = 0.f;
+#pragma unroll
+  for (int S = 1; S < 8; ++S) {
+    if ((S & 1) && lx != 0) continue;
+    if ((S & 2) && ly != 0) continue;
+    if ((S & 4) && lz != 0) continue;
+    const int nx = (S & 1) ? (tx == 0 ? ntx - 1 : tx - 1) : tx;
+    const int ny = (S & 2) ? (ty == 0 ? nty - 1 : ty - 1) : ty;
+    const int nz = (S & 4) ? (tz == 0 ? ntz - 1 : tz - 1) : tz;
+    const int qz = (S & 4) ? 16 : lz;
+    const int q = (S & 1) ? ((S & 2) ? 16 : ly) * 17 + qz
+                          : ((S & 2) ? 289 + lx * 17 + qz : 561 + lx * 16 + ly);
+    sum += halo[(long long)((nx * nty + ny) * ntz + nz) * shell + q];
+  }
+  return sum;
+}
+
 // fractions and the sign(-d2) factors of their gradients (pmx_cic_axis)
 __device__ __forceinline__ void frac_sign_of(float d, float& w0, float& w1, float& s0, float& s1) {
   int i = (int)floorf(d);
@@ -792,16 +819,23 @@ int pmx_bin_particles(pmx_ctx* c, const int16_t* pmid, const float* disp, long l
 int pmx_bin_begin(pmx_ctx* c, long long n, int* cap);
 BinOut pmx_bin_out(pmx_ctx* c);
 int pmx_bin_end(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n, int cap);
-int pmx_deposit_binned(pmx_ctx* c, float val, long long n, float* mesh, int mrs, int accumulate);
+// The r2c row pass of the Poisson solve can add the deposit's halo shells as
+// it loads the density (16^3 tiles covering the mesh, a fixed-length
+// half-size row pass whose row blocks stay in one x plane).  Deposits made
+// with defer = 1 then leave the fold to it, and the solve is told so with
+// deferred = 1; otherwise both run as before.
+int pmx_fold_deferrable(const pmx_ctx* c);
+int pmx_deposit_binned(pmx_ctx* c, float val, long long n, float* mesh, int mrs, int accumulate,
+                       int defer = 0);
 int pmx_deposit_state(pmx_ctx* c, const PState& st, const uint32_t* start, const uint32_t* count,
-                      float val, float* mesh, int mrs);
+                      float val, float* mesh, int mrs, int defer = 0);
 // Output meshes of a deposit (one per channel).
 struct DepOut {
   float* mesh[3];
 };
 int pmx_deposit_state_vals(pmx_ctx* c, const PState& st, const uint32_t* start,
                            const uint32_t* count, float val, const float* vals, long long vstride,
-                           int vstep, int nch, DepOut out, int mrs);
+                           int vstep, int nch, DepOut out, int mrs, int defer = 0);
 // kick (k1) and drift (d) of the caller's arrays, binned into count/o (cap:
 // capacity mode)
 int pmx_kick_drift_bin(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, const float* acc,
@@ -815,11 +849,11 @@ int pmx_kd_adj_bin(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, con
                    float* disp_cot, float* vel_cot, const float* acc_cot, const float* disp_snap,
                    long long n, float kp, float ka, float d, uint32_t* count, const BinOut& o,
                    int cap, double* part);
-int pmx_deposit_binned3(pmx_ctx* c, long long n, float* const* mesh, int mrs);
+int pmx_deposit_binned3(pmx_ctx* c, long long n, float* const* mesh, int mrs, int defer = 0);
 int pmx_ensure_halo(pmx_ctx* c, int nch);
-int pmx_poisson_ctx(pmx_ctx* c, float coef);
-int pmx_poisson_adj_ctx(pmx_ctx* c, float scale_out);
-int pmx_poisson_adj_bufs(pmx_ctx* c, float2* const* f, float scale_out);
+int pmx_poisson_ctx(pmx_ctx* c, float coef, int deferred = 0);
+int pmx_poisson_adj_ctx(pmx_ctx* c, float scale_out, int deferred = 0);
+int pmx_poisson_adj_bufs(pmx_ctx* c, float2* const* f, float scale_out, int deferred = 0);
 // bins the particles, deposits the density and solves for the three force
 // meshes in c->cbuf (padded); then the tile-staged gather of those meshes
 int pmx_gravity_fwd_meshes(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,

@@ -287,9 +287,9 @@ extern "C" int pmx_nbody_steps(pmx_ctx* c, const int16_t* pmid, float* disp, flo
   const PlUser user = {disp, vel, acc};
   for (int s = 0; s < nsteps; ++s) {
     if ((rc = pmx_deposit_state(c, c->pl[A], c->pl_start[A], c->pl_count[A], val,
-                                (float*)c->cbuf[0], c->rs)))
+                                (float*)c->cbuf[0], c->rs, 1)))
       return rc;
-    if ((rc = pmx_poisson_ctx(c, coef))) return rc;
+    if ((rc = pmx_poisson_ctx(c, coef, 1))) return rc;
     if (s + 1 == nsteps) {
       pmx_tic(c, PMX_T_GATHER);
       pl_launch<PL_LAST>(c, A, PlOut{}, user, k2[s], 0.f, 0.f);
@@ -645,16 +645,16 @@ extern "C" int pmx_nbody_steps_adj(pmx_ctx* c, const int16_t* pmid, float* disp,
   for (int s = 0; s < nsteps; ++s) {
     // the force meshes at this step's positions (kept in cbuf)
     if ((rc = pmx_deposit_state(c, c->al[A].p, c->pl_start[A], c->pl_count[A], val,
-                                (float*)c->cbuf[0], c->rs)))
+                                (float*)c->cbuf[0], c->rs, 1)))
       return rc;
-    if ((rc = pmx_poisson_ctx(c, coef))) return rc;
+    if ((rc = pmx_poisson_ctx(c, coef, 1))) return rc;
     // the cotangents of the three force meshes, then of the density (abuf[0])
     DepOut out3 = {{(float*)c->abuf[0], (float*)c->abuf[1], (float*)c->abuf[2]}};
     if ((rc = pmx_deposit_state_vals(c, c->al[A].p, c->pl_start[A], c->pl_count[A], 0.f,
                                      reinterpret_cast<const float*>(c->al[A].c1), 1, 4, 3, out3,
-                                     c->rs)))
+                                     c->rs, 1)))
       return rc;
-    if ((rc = pmx_poisson_adj_bufs(c, c->abuf, adj_scale))) return rc;
+    if ((rc = pmx_poisson_adj_bufs(c, c->abuf, adj_scale, 1))) return rc;
     if (s + 1 == nsteps) {
       pmx_tic(c, PMX_T_GRADG);
       al_launch<PL_LAST>(c, A, val, 0.f, 0.f, 0.f, nullptr, AlOut{}, user);

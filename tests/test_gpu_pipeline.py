@@ -274,3 +274,32 @@ def test_adjoint_driver_pipeline_matches_per_step(dev):
             outs.append(([t.cpu().numpy() for t in (x, vv, dc, vc, acc, ac)], om))
         assert_same(outs[0][0], outs[1][0])
         assert abs(outs[0][1] - outs[1][1]) <= 1e-10 * abs(outs[1][1])
+
+
+@pytest.mark.parametrize('shape,ratio', [((64, 64, 64), 2), ((32, 48, 64), 4)])
+def test_fold_on_load_matches_fold_kernel(dev, monkeypatch, shape, ratio):
+    """On meshes whose Poisson row pass adds the deposit's halo shells as it
+    loads the density (16^3 tiles, half-length rows of 64..1024 complex
+    values), forward and reverse steps -- chained single steps and the
+    multi-step entry points, the reverse ones with their three cotangent
+    channels -- equal bit for bit the runs that fold with the separate
+    kernel (PMX_FOLD_ON_LOAD=0 when the context is made)."""
+    import pmwd_amd as pa
+    conf = pa.Configuration(1.0, shape, mesh_shape=ratio)
+    pmid, disp = particles(shape, ratio, conf.cell_size, 31, std=0.8)
+    rng = np.random.default_rng(32)
+    vel = (0.5 * rng.standard_normal(disp.shape)).astype(np.float32)
+    k1, d, k2 = factors(2, 33)
+    arrs = [disp] + [(0.5 * rng.standard_normal(disp.shape)).astype(np.float32)
+                     for _ in range(3)]
+    ka, da, kb = factors(2, 34)
+    got = [run(conf, dev, pmid, disp, vel, k1, d, k2, m) for m in (True, False)]
+    got_adj = [run_adj(conf, dev, pmid, arrs, 0.2, ka, -da, kb, m) for m in (True, False)]
+    monkeypatch.setenv('PMX_FOLD_ON_LOAD', '0')
+    want = run(conf, dev, pmid, disp, vel, k1, d, k2, False)
+    want_adj, want_dots = run_adj(conf, dev, pmid, arrs, 0.2, ka, -da, kb, False)
+    for g in got:
+        assert_same(g, want)
+    for g, gd in got_adj:
+        assert_same(g, want_adj)
+        np.testing.assert_allclose(gd, want_dots, rtol=1e-12, atol=1e-300)
