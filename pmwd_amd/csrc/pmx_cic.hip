@@ -12,6 +12,8 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 __global__ void fill_u32_k(uint32_t* __restrict__ p, long long n, uint32_t v,
                            const uint32_t* __restrict__ cond) {
   if (cond && *cond == 0) return;
@@ -181,21 +183,26 @@ __device__ __forceinline__ double fixed_scale(float vmax) {
 #define DEP_U 4
 
 // Adds v*w_j for the eight corners of a particle (d = disp/cell, lower corner
-// at LDS cell b) to the fixed-point tile accumulator.
+// at LDS cell b) to the fixed-point tile accumulator.  The scale is a power
+// of two; up to 2^126 (sf = scale as a float, fq) x * sf is exact in float
+// and |x * sf| < 2^30, so rintf and an int32 conversion give the value
+// rint((double)x * scale) gives, in fewer instructions.
+template <bool FQ>
 __device__ __forceinline__ void dep_one(unsigned long long* acc, float dx, float dy, float dz,
-                                        int b, int LY, int LZ, float vv, double scale) {
+                                        int b, int LY, int LZ, float vv, double scale, float sf) {
   float a0, a1, b0, b1, c0, c1;
   frac_of(dx, a0, a1);
   frac_of(dy, b0, b1);
   frac_of(dz, c0, c1);
+  const int sxy = __umul24(LY, LZ);
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     float f0 = (j & 1) ? a1 : a0;
     float f1 = (j & 2) ? b1 : b0;
     float f2 = (j & 4) ? c1 : c0;
-    int li = b + ((j & 1) ? LY * LZ : 0) + ((j & 2) ? LZ : 0) + ((j & 4) ? 1 : 0);
+    int li = b + ((j & 1) ? sxy : 0) + ((j & 2) ? LZ : 0) + ((j & 4) ? 1 : 0);
     float x = vv * ((f0 * f1) * f2);
-    long long qx = (long long)rint((double)x * scale);
+    long long qx = FQ ? (long long)(int)rintf(x * sf) : (long long)rint((double)x * scale);
     atomicAdd(&acc[li], (unsigned long long)qx);
   }
 }
@@ -208,13 +215,24 @@ __device__ __forceinline__ void dep_store(const unsigned long long* acc, float* 
                                           const MeshGeom& g, int mrs, int accumulate,
                                           double inv) {
   const int itot = s[0] * s[1] * s[2];
-  for (int i = threadIdx.x; i < itot; i += TPB) {
-    int lz = i % s[2];
-    int ly = (i / s[2]) % s[1];
-    int lx = i / (s[2] * s[1]);
-    long long m = ((long long)(o[0] + lx) * g.n1 + (o[1] + ly)) * mrs + (o[2] + lz);
-    float a = (float)((double)(long long)acc[(lx * LY + ly) * LZ + lz] * inv);
-    mesh[m] = accumulate ? mesh[m] + a : a;
+  if (s[0] == 16 && s[1] == 16 && s[2] == 16) {
+    // a full tile: shifts in place of the divisions
+    float* base = mesh + ((long long)o[0] * g.n1 + o[1]) * mrs + o[2];
+    for (int i = threadIdx.x; i < 4096; i += TPB) {
+      const int lz = i & 15, ly = (i >> 4) & 15, lx = i >> 8;
+      float* p = base + ((long long)lx * g.n1 + ly) * mrs + lz;
+      float a = (float)((double)(long long)acc[(lx * 17 + ly) * 17 + lz] * inv);
+      *p = accumulate ? *p + a : a;
+    }
+  } else {
+    for (int i = threadIdx.x; i < itot; i += TPB) {
+      int lz = i % s[2];
+      int ly = (i / s[2]) % s[1];
+      int lx = i / (s[2] * s[1]);
+      long long m = ((long long)(o[0] + lx) * g.n1 + (o[1] + ly)) * mrs + (o[2] + lz);
+      float a = (float)((double)(long long)acc[(lx * LY + ly) * LZ + lz] * inv);
+      mesh[m] = accumulate ? mesh[m] + a : a;
+    }
   }
   const int offA = LY * LZ;
   const int offB = s[0] * LZ;
@@ -254,8 +272,13 @@ __global__ void __launch_bounds__(DEP_TPB)
     for (int i = threadIdx.x; i < ltot; i += DEP_TPB) acc[i] = 0ull;
     const double scale = fixed_scale(PERVAL ? vmaxp[ch] : fabsf(val));
     const double inv = 1.0 / scale;
+    const bool fq = scale <= 0x1p126;
+    const float sf = fq ? (float)scale : 0.f;
     const float* vr = PERVAL ? vrec + (long long)ch * vstride : nullptr;
     __syncthreads();
+    // the particle rounds, with the float or the double fixed-point conversion
+    auto rounds = [&](auto fqc) {
+    constexpr bool FQ = decltype(fqc)::value;
     for (uint32_t q0 = 0; q0 < cnt; q0 += DEP_TPB * DEP_U) {
       float4 r[DEP_U];
       float v[DEP_U];
@@ -289,9 +312,15 @@ __global__ void __launch_bounds__(DEP_TPB)
           pmx_unpack_local(r[u].w, lx, ly, lz);
           if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
         }
-        dep_one(acc, r[u].x, r[u].y, r[u].z, (lx * LY + ly) * LZ + lz, LY, LZ, v[u], scale);
+        dep_one<FQ>(acc, r[u].x, r[u].y, r[u].z, __umul24(__umul24(lx, LY) + ly, LZ) + lz, LY,
+                    LZ, v[u], scale, sf);
       }
     }
+    };
+    if (fq)
+      rounds(std::true_type{});
+    else
+      rounds(std::false_type{});
     __syncthreads();
     float* mesh = ch == 0 ? out.mesh[0] : (ch == 1 ? out.mesh[1] : out.mesh[2]);
     dep_store<DEP_TPB>(acc, mesh, halo + ch * hstride + (long long)t * tg.shell, o, s, LY, LZ, g,

@@ -119,6 +119,20 @@ def test_scatter_mass(dev, shape, ratio, std):
     assert abs(dens.astype(np.float64).sum() - conf.mesh_size) < 1e-4 * conf.mesh_size
 
 
+@pytest.mark.parametrize('val', [1e-33, 3e-20, 1e30])
+def test_scatter_fixed_point_extremes(dev, val):
+    """The fixed-point tile accumulation at the ends of its scale range: a
+    scale of 2^139 (val 1e-33) takes the double-precision conversion, 2^94
+    and 2^-70 the single-precision one."""
+    import pmwd_amd as pa
+    conf = conf_of((16, 16, 16), 2)
+    pmid, disp = make_ptcl((16, 16, 16), conf.mesh_shape, conf.cell_size, 3.0, seed=9)
+    got = pa.scatter(to_ptcl(conf, pmid, disp, dev), conf, val=val).cpu().numpy()
+    want = opm.scatter(pmid, disp, conf.cell_size, conf.mesh_shape, val=val)
+    assert np.all(np.isfinite(got))
+    np.testing.assert_allclose(got, want, rtol=2e-6, atol=2e-6 * val)
+
+
 def test_scatter_accumulates_into_mesh(dev):
     import pmwd_amd as pa
     conf = conf_of((16, 16, 16), 1)
