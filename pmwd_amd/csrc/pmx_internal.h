@@ -549,19 +549,24 @@ template <int NC, int TPB>
 __device__ __forceinline__ void pmx_stage_tile(float* sh, const int o[3], int LY, int LZ, int ltot,
                                                const MeshGeom& g, const float* const* F) {
   constexpr int NI = (PMX_TILE_CELLS + TPB - 1) / TPB;
+  // i / LZ and (i / LZ) / LY as 24-bit multiplies by ceil(2^20 / L): exact
+  // for i < 2^16 and L <= 17 (the error i (M L - 2^20) / 2^20 / L stays
+  // below 1 / L)
+  const unsigned mz = ((1u << 20) + LZ - 1) / LZ, my = ((1u << 20) + LY - 1) / LY;
   float r[NI][NC];
 #pragma unroll
   for (int it = 0; it < NI; ++it) {
     const int i = threadIdx.x + it * TPB;
     if (i < ltot) {
-      int qz = i % LZ;
-      int qy = (i / LZ) % LY;
-      int qx = i / (LZ * LY);
+      const int iz = (int)(__umul24((unsigned)i, mz) >> 20);
+      const int qx = (int)(__umul24((unsigned)iz, my) >> 20);
+      const int qz = i - iz * LZ;
+      const int qy = iz - qx * LY;
       int x = o[0] + qx, y = o[1] + qy, z = o[2] + qz;
       if (x >= g.n0) x -= g.n0;
       if (y >= g.n1) y -= g.n1;
       if (z >= g.n2) z -= g.n2;
-      const long long m = ((long long)x * g.n1 + y) * g.rs + z;
+      const long long m = (long long)(__umul24(x, g.n1) + y) * g.rs + z;
 #pragma unroll
       for (int c = 0; c < NC; ++c) r[it][c] = F[c][m];
     }
@@ -587,7 +592,7 @@ __device__ __forceinline__ void pmx_tile_acc(const float* sh, int ltot, int LY, 
   frac_of(dx, a0, a1);
   frac_of(dy, b0, b1);
   frac_of(dz, c0, c1);
-  const int b = (lx * LY + ly) * LZ + lz;
+  const int b = __umul24(__umul24(lx, LY) + ly, LZ) + lz;
   float v0[8], v1[8], v2[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
@@ -616,7 +621,7 @@ __device__ __forceinline__ void pmx_vjp_one(const float* sh, int ltot, int LY, i
   frac_sign_of(d[0], wa[0], wa[1], sa[0], sa[1]);
   frac_sign_of(d[1], wb[0], wb[1], sb[0], sb[1]);
   frac_sign_of(d[2], wc[0], wc[1], sc[0], sc[1]);
-  const int b = (l[0] * LY + l[1]) * LZ + l[2];
+  const int b = __umul24(__umul24(l[0], LY) + l[1], LZ) + l[2];
   float r0[8], r1[8], r2[8], gx[8], gy[8], gz[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
@@ -651,7 +656,7 @@ __device__ __forceinline__ void pmx_grad_one(const float* sh, int LY, int LZ, co
   frac_sign_of(d[0], wa[0], wa[1], sa[0], sa[1]);
   frac_sign_of(d[1], wb[0], wb[1], sb[0], sb[1]);
   frac_sign_of(d[2], wc[0], wc[1], sc[0], sc[1]);
-  const int b = (l[0] * LY + l[1]) * LZ + l[2];
+  const int b = __umul24(__umul24(l[0], LY) + l[1], LZ) + l[2];
   float gx[8], gy[8], gz[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
