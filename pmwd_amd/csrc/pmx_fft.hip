@@ -19,7 +19,12 @@
 #define RFFT_CAP 4096                // row passes: fewer values per thread
 #define RFFT_MAXV (RFFT_CAP / FFT_TPB)
 #define PFFT_TPB 512                 // Poisson x pass: 8 columns of 512 per WG,
-#define PFFT_CAP 4096                // two such blocks (68 KB of LDS each) per CU
+#ifndef PFFT_CAP                     // four such blocks (40 KB of LDS each) per CU
+#define PFFT_CAP 4096
+#endif
+#ifndef PFFT_WPE                     // waves per SIMD the register budget allows
+#define PFFT_WPE 1
+#endif
 #define PFFT_MAXV (PFFT_CAP / PFFT_TPB)
 
 // Complex products inside the transforms may fuse into FMAs (the file is
@@ -754,22 +759,20 @@ struct KTab {
 // x, so the y and z components share one inverse transform h =
 // IFFT_x(-i pot), stored to c_1 (y_inv_pair_k scales it by km_1 and km_2 for
 // the two components); the x component (-i km_0 pot) is transformed on its
-// own and stored to c_0.  The two inverse transforms run side by side as 2B
-// pencils.  Pencils along axis 0 of (n0,n1,n2h);
-// blockIdx.y = y index, blockIdx.x = block of B kz columns.
-// LDS: 2 n0 B values (the forward block, then the two inverse transforms
-// interleaved: column b of the x component at j 2B + b, of h at j 2B + B + b),
-// then the n0-entry twiddle table.
+// own and stored to c_0.  The two inverse transforms run one after the
+// other in the same LDS block, h kept in registers meanwhile.  Pencils along
+// axis 0 of (n0,n1,n2h); blockIdx.y = y index, blockIdx.x = block of B kz
+// columns.  LDS: the n0 x B block, the n0-entry twiddle table, then the k_0
+// and km_0 tables.
 template <bool POW2, int LN>
-__global__ void __launch_bounds__(PFFT_TPB) poisson_x_k(Fields3 fl, int n0_, int n1, int n2h, int n2p,
+__global__ void __launch_bounds__(PFFT_TPB) __attribute__((amdgpu_waves_per_eu(PFFT_WPE))) poisson_x_k(Fields3 fl, int n0_, int n1, int n2h, int n2p,
                                                        int B_, KTab kt, const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
   constexpr int BC = LN ? fft_pencils(1 << LN, PFFT_CAP) : 1;
   const int n0 = LN ? 1 << LN : n0_;
   const int B = LN ? BC : B_;
   const int lB = LN ? ilog2c(BC) : __ffs(B_) - 1;
-  const int B2 = 2 * B;
-  float2* stw = s + 2 * n0 * B;
+  float2* stw = s + n0 * B;
   float* sk = reinterpret_cast<float*>(stw + n0);
   float* skm = sk + n0;
   const long long es = (long long)n1 * n2p;
@@ -785,19 +788,18 @@ __global__ void __launch_bounds__(PFFT_TPB) poisson_x_k(Fields3 fl, int n0_, int
   }
   const ColMap cm = col_map<PFFT_TPB>(lB);
   const bool live = cm.b < nb;
-  {
-    const float2* g = fl.f[0] + off + cm.b + (long long)cm.j0 * es;
-    const long long gs = (long long)cm.js * es;
-    float2 r[PFFT_MAXV];
+  float2* g0 = fl.f[0] + off + cm.b + (long long)cm.j0 * es;
+  float2* g1 = fl.f[1] + off + cm.b + (long long)cm.j0 * es;
+  const long long gs = (long long)cm.js * es;
+  float2 r[PFFT_MAXV];
 #pragma unroll
-    for (int it = 0; it < PFFT_MAXV; ++it) {
-      r[it] = make_float2(0.f, 0.f);
-      if (live && cm.j0 + it * cm.js < n0) r[it] = g[it * gs];
-    }
-#pragma unroll
-    for (int it = 0; it < PFFT_MAXV; ++it)
-      if (cm.j0 + it * cm.js < n0) s[threadIdx.x + it * PFFT_TPB] = r[it];
+  for (int it = 0; it < PFFT_MAXV; ++it) {
+    r[it] = make_float2(0.f, 0.f);
+    if (live && cm.j0 + it * cm.js < n0) r[it] = g0[it * gs];
   }
+#pragma unroll
+  for (int it = 0; it < PFFT_MAXV; ++it)
+    if (cm.j0 + it * cm.js < n0) s[threadIdx.x + it * PFFT_TPB] = r[it];
   __syncthreads();
   fft_run<-1, COLS, POW2, LN, ilog2c(BC), PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
   const float ky = kt.k[1][y];
@@ -821,49 +823,54 @@ __global__ void __launch_bounds__(PFFT_TPB) poisson_x_k(Fields3 fl, int n0_, int
     }
   }
   __syncthreads();
+  // x component: IFFT_x(km_0 h) to c_0
 #pragma unroll
   for (int it = 0; it < PFFT_MAXV; ++it) {
     const int j = cm.j0 + it * cm.js;
     if (j < n0) {
       const float kmx = skm[j];
-      const int ls = j * B2 + cm.b;
-      s[ls] = make_float2(kmx * h[it].x, kmx * h[it].y);
-      s[ls + B] = h[it];
+      s[threadIdx.x + it * PFFT_TPB] = make_float2(kmx * h[it].x, kmx * h[it].y);
     }
   }
   __syncthreads();
-  fft_run<1, COLS, POW2, LN, ilog2c(BC) + 1, 2 * PFFT_MAXV, PFFT_TPB>(s, n0, B2, stw);
-  const ColMap c2 = col_map<PFFT_TPB>(lB + 1);
-  const int b2 = c2.b & (B - 1);
-  const bool live2 = b2 < nb;
-  float2* q = (c2.b < B ? fl.f[0] : fl.f[1]) + off + b2 + (long long)c2.j0 * es;
-  const long long qs = (long long)c2.js * es;
-  float2 r[2 * PFFT_MAXV];
+  fft_run<1, COLS, POW2, LN, ilog2c(BC), PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
 #pragma unroll
-  for (int it = 0; it < 2 * PFFT_MAXV; ++it)
-    if (c2.j0 + it * c2.js < n0) r[it] = s[threadIdx.x + it * PFFT_TPB];
+  for (int it = 0; it < PFFT_MAXV; ++it)
+    if (cm.j0 + it * cm.js < n0) r[it] = s[threadIdx.x + it * PFFT_TPB];
+  __syncthreads();
+  // h to c_1
 #pragma unroll
-  for (int it = 0; it < 2 * PFFT_MAXV; ++it)
-    if (live2 && c2.j0 + it * c2.js < n0) q[it * qs] = r[it];
+  for (int it = 0; it < PFFT_MAXV; ++it)
+    if (cm.j0 + it * cm.js < n0) s[threadIdx.x + it * PFFT_TPB] = h[it];
+#pragma unroll
+  for (int it = 0; it < PFFT_MAXV; ++it)
+    if (live && cm.j0 + it * cm.js < n0) g0[it * gs] = r[it];
+  __syncthreads();
+  fft_run<1, COLS, POW2, LN, ilog2c(BC), PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
+#pragma unroll
+  for (int it = 0; it < PFFT_MAXV; ++it)
+    if (cm.j0 + it * cm.js < n0) r[it] = s[threadIdx.x + it * PFFT_TPB];
+#pragma unroll
+  for (int it = 0; it < PFFT_MAXV; ++it)
+    if (live && cm.j0 + it * cm.js < n0) g1[it * gs] = r[it];
 }
 
 // Adjoint x pass: acc = sum_d (-i km_d) laplace(FFT_x c_d); inverse FFT
 // along x; stored to c0.  Transpose of the three force components.  As km_1
 // and km_2 are constant along x, FFT_x(km_1 c_1 + km_2 c_2) replaces the two
 // transforms of c_1 and c_2; y_fwd_pair_k leaves that combination in c_1.
-// The block holds c_0 and c_1 side by side as 2B pencils (j 2B + b and
-// j 2B + B + b) for one forward pass, then the sum in B pencils for the
-// inverse.  LDS as poisson_x_k.
+// The forward transforms of c_0 and c_1 run one after the other in the same
+// LDS block (the first result kept in registers), then the inverse of the
+// sum.  LDS as poisson_x_k.
 template <bool POW2, int LN>
-__global__ void __launch_bounds__(PFFT_TPB) poisson_x_adj_k(Fields3 fl, int n0_, int n1, int n2h, int n2p,
+__global__ void __launch_bounds__(PFFT_TPB) __attribute__((amdgpu_waves_per_eu(PFFT_WPE))) poisson_x_adj_k(Fields3 fl, int n0_, int n1, int n2h, int n2p,
                                                            int B_, KTab kt, const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
   constexpr int BC = LN ? fft_pencils(1 << LN, PFFT_CAP) : 1;
   const int n0 = LN ? 1 << LN : n0_;
   const int B = LN ? BC : B_;
   const int lB = LN ? ilog2c(BC) : __ffs(B_) - 1;
-  const int B2 = 2 * B;
-  float2* stw = s + 2 * n0 * B;
+  float2* stw = s + n0 * B;
   float* sk = reinterpret_cast<float*>(stw + n0);
   float* skm = sk + n0;
   const long long es = (long long)n1 * n2p;
@@ -877,26 +884,39 @@ __global__ void __launch_bounds__(PFFT_TPB) poisson_x_adj_k(Fields3 fl, int n0_,
     sk[i] = kt.k[0][i];
     skm[i] = kt.km[0][i];
   }
-  {
-    const ColMap c2 = col_map<PFFT_TPB>(lB + 1);
-    const int b2 = c2.b & (B - 1);
-    const bool live2 = b2 < nb;
-    const float2* q = (c2.b < B ? fl.f[0] : fl.f[1]) + off + b2 + (long long)c2.j0 * es;
-    const long long qs = (long long)c2.js * es;
-    float2 r[2 * PFFT_MAXV];
-#pragma unroll
-    for (int it = 0; it < 2 * PFFT_MAXV; ++it) {
-      r[it] = make_float2(0.f, 0.f);
-      if (live2 && c2.j0 + it * c2.js < n0) r[it] = q[it * qs];
-    }
-#pragma unroll
-    for (int it = 0; it < 2 * PFFT_MAXV; ++it)
-      if (c2.j0 + it * c2.js < n0) s[threadIdx.x + it * PFFT_TPB] = r[it];
-  }
-  __syncthreads();
-  fft_run<-1, COLS, POW2, LN, ilog2c(BC) + 1, 2 * PFFT_MAXV, PFFT_TPB>(s, n0, B2, stw);
   const ColMap cm = col_map<PFFT_TPB>(lB);
   const bool live = cm.b < nb;
+  float2* g0 = fl.f[0] + off + cm.b + (long long)cm.j0 * es;
+  const float2* g1 = fl.f[1] + off + cm.b + (long long)cm.j0 * es;
+  const long long gs = (long long)cm.js * es;
+  float2 x0[PFFT_MAXV];
+#pragma unroll
+  for (int it = 0; it < PFFT_MAXV; ++it) {
+    x0[it] = make_float2(0.f, 0.f);
+    if (live && cm.j0 + it * cm.js < n0) x0[it] = g0[it * gs];
+  }
+#pragma unroll
+  for (int it = 0; it < PFFT_MAXV; ++it)
+    if (cm.j0 + it * cm.js < n0) s[threadIdx.x + it * PFFT_TPB] = x0[it];
+  __syncthreads();
+  fft_run<-1, COLS, POW2, LN, ilog2c(BC), PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
+  {
+    float2 yz[PFFT_MAXV];
+#pragma unroll
+    for (int it = 0; it < PFFT_MAXV; ++it) {
+      yz[it] = make_float2(0.f, 0.f);
+      if (live && cm.j0 + it * cm.js < n0) yz[it] = g1[it * gs];
+    }
+#pragma unroll
+    for (int it = 0; it < PFFT_MAXV; ++it)
+      if (cm.j0 + it * cm.js < n0) x0[it] = s[threadIdx.x + it * PFFT_TPB];
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < PFFT_MAXV; ++it)
+      if (cm.j0 + it * cm.js < n0) s[threadIdx.x + it * PFFT_TPB] = yz[it];
+  }
+  __syncthreads();
+  fft_run<-1, COLS, POW2, LN, ilog2c(BC), PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
   const float ky = kt.k[1][y];
   const float kz = live ? kt.k[2][c0 + cm.b] : 0.f;
   float2 acc[PFFT_MAXV];
@@ -911,9 +931,8 @@ __global__ void __launch_bounds__(PFFT_TPB) poisson_x_adj_k(Fields3 fl, int n0_,
       k2 = k2 + kz * kz;
       if (k2 != 0.f) {
         const float kmx = skm[j];
-        const int ls = j * B2 + cm.b;
-        float2 x0 = s[ls], yz = s[ls + B];
-        float sx = kmx * x0.x + yz.x, sy = kmx * x0.y + yz.y;
+        const float2 w = s[threadIdx.x + it * PFFT_TPB];
+        float sx = kmx * x0[it].x + w.x, sy = kmx * x0[it].y + w.y;
         // pot = -sum / k^2, acc = -i pot = (pot.y, -pot.x)
         const float q = 1.f / k2;
         acc[it] = make_float2(-(sy * q), sx * q);
@@ -926,14 +945,12 @@ __global__ void __launch_bounds__(PFFT_TPB) poisson_x_adj_k(Fields3 fl, int n0_,
     if (cm.j0 + it * cm.js < n0) s[threadIdx.x + it * PFFT_TPB] = acc[it];
   __syncthreads();
   fft_run<1, COLS, POW2, LN, ilog2c(BC), PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
-  float2* g = fl.f[0] + off + cm.b + (long long)cm.j0 * es;
-  const long long gs = (long long)cm.js * es;
 #pragma unroll
   for (int it = 0; it < PFFT_MAXV; ++it)
     if (cm.j0 + it * cm.js < n0) acc[it] = s[threadIdx.x + it * PFFT_TPB];
 #pragma unroll
   for (int it = 0; it < PFFT_MAXV; ++it)
-    if (live && cm.j0 + it * cm.js < n0) g[it * gs] = acc[it];
+    if (live && cm.j0 + it * cm.js < n0) g0[it * gs] = acc[it];
 }
 
 // y passes of the gradient components, paired (pencils along axis 1 of
@@ -1298,7 +1315,7 @@ int pmx_poisson_x(pmx_ctx* c, float2* const* f, int adjoint) {
   const int B = fft_pencils(n0, PFFT_CAP);
   dim3 grid((c->n2h + B - 1) / B, c->n[1], 1);
   // the data block, the twiddles, then the k_0 and km_0 tables
-  size_t lds = 2 * lds_bytes(n0, B) + tw_bytes(n0) + 2 * (size_t)n0 * sizeof(float);
+  size_t lds = lds_bytes(n0, B) + tw_bytes(n0) + 2 * (size_t)n0 * sizeof(float);
   if (int rc = fft_fits("Poisson x pass", n0, n0 * B, PFFT_CAP, lds)) return rc;
   const bool p2 = is_pow2(n0);
   const int ln = ct_log2(n0);
