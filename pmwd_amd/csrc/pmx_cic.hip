@@ -180,7 +180,13 @@ __device__ __forceinline__ double fixed_scale(float vmax) {
 // particles (rec = their s0, pz their pm_z) and forms d and the corner
 // itself.
 #define DEP_TPB 512
-#define DEP_U 4
+#ifndef DEP_U
+#define DEP_U 2
+#endif
+// eight waves per SIMD (at most 64 VGPRs): four 39 KB workgroups per CU
+#ifndef DEP_WPE
+#define DEP_WPE 8
+#endif
 
 // Adds v*w_j for the eight corners of a particle (d = disp/cell, lower corner
 // at LDS cell b) to the fixed-point tile accumulator.  The scale is a power
@@ -253,7 +259,7 @@ __device__ __forceinline__ void dep_store(const unsigned long long* acc, float* 
 }
 
 template <int PERVAL, int STATE = 0>
-__global__ void __launch_bounds__(DEP_TPB)
+__global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DEP_WPE)))
     deposit_k(const float4* __restrict__ rec, const uint16_t* __restrict__ pz,
               const float* __restrict__ vrec, long long vstride, int vstep,
               const float* __restrict__ vmaxp, const uint32_t* __restrict__ tile_start,
