@@ -757,7 +757,7 @@ struct KTab {
 // c0, pot = -src/k^2 (k=0 -> 0), g_d = (-i km_d) pot, inverse FFT along x.
 // km_1 (one value per y row) and km_2 (one per kz column) are constant along
 // x, so the y and z components share one inverse transform h =
-// IFFT_x(-i pot), stored to c_1 (y_inv_pair_k scales it by km_1 and km_2 for
+// IFFT_x(-i pot), stored to c_1 (y_inv_seq_k scales it by km_1 and km_2 for
 // the two components); the x component (-i km_0 pot) is transformed on its
 // own and stored to c_0.  The two inverse transforms run one after the
 // other in the same LDS block, h kept in registers meanwhile.  Pencils along
@@ -858,7 +858,7 @@ __global__ void __launch_bounds__(PFFT_TPB) __attribute__((amdgpu_waves_per_eu(P
 // Adjoint x pass: acc = sum_d (-i km_d) laplace(FFT_x c_d); inverse FFT
 // along x; stored to c0.  Transpose of the three force components.  As km_1
 // and km_2 are constant along x, FFT_x(km_1 c_1 + km_2 c_2) replaces the two
-// transforms of c_1 and c_2; y_fwd_pair_k leaves that combination in c_1.
+// transforms of c_1 and c_2; y_fwd_seq_k leaves that combination in c_1.
 // The forward transforms of c_0 and c_1 run one after the other in the same
 // LDS block (the first result kept in registers), then the inverse of the
 // sum.  LDS as poisson_x_k.
@@ -954,124 +954,140 @@ __global__ void __launch_bounds__(PFFT_TPB) __attribute__((amdgpu_waves_per_eu(P
 }
 
 // y passes of the gradient components, paired (pencils along axis 1 of
-// (n0,n1,n2h); blockIdx.y = x row, blockIdx.x = block of B kz columns; LDS:
-// 2 n1 B values side by side as 2B pencils, then the twiddle table).
+// (n0,n1,n2h); blockIdx.y = x row, blockIdx.x = block of B = fft_pencils(n)
+// kz columns, 128-byte segments at n = 512; LDS: the n x B block, then the
+// twiddle table).  The two transforms run one after the other in the same
+// LDS block.
 //
 // Inverse: from h = IFFT_x(-i pot) in c_1, the y and z force components
 // IFFT_y(km_1 h) to c_1 and IFFT_y(km_2 h) to c_2 -- the products and
-// transforms of a separate y pass over km_1 h and km_2 h, with h read once.
+// transforms of a separate y pass over km_1 h and km_2 h, with h read once
+// and kept in registers.
+//
+// Forward (adjoint path, y_fwd_seq_k): FFT_y of c_1 (kept in registers) and
+// of c_2, then their combination km_1 FFT_y(c_1) + km_2 FFT_y(c_2) to c_1
+// for poisson_x_adj_k.
 template <bool POW2, int LN>
-__global__ void __launch_bounds__(FFT_TPB, 4) y_inv_pair_k(Fields3 fl, int n_, long long es,
-                                                         long long ostride, int ncol, int B_,
-                                                         KTab kt, const float2* __restrict__ tw) {
+__global__ void __launch_bounds__(FFT_TPB, 2) y_inv_seq_k(Fields3 fl, int n_, long long es,
+                                                        long long ostride, int ncol, int B_,
+                                                        KTab kt, const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
-  constexpr int BC = LN ? fft_pencils(1 << LN, FFT_CAP / 2) : 1;
+  constexpr int BC = LN ? fft_pencils(1 << LN) : 1;
   const int n = LN ? 1 << LN : n_;
   const int B = LN ? BC : B_;
   const int lB = LN ? ilog2c(BC) : __ffs(B_) - 1;
-  const int B2 = 2 * B;
-  float2* stw = s + 2 * n * B;
+  float2* stw = s + n * B;
   int bx, by, bz;
   xcd_block3(bx, by, bz);
   const int c0 = bx * B;
   const int nb = ncol - c0 < B ? ncol - c0 : B;
   const long long off = (long long)by * ostride + c0;
   stage_twiddles<FFT_TPB>(stw, tw, n);
-  {
-    const ColMap cm = col_map<FFT_TPB>(lB);
-    const bool live = cm.b < nb;
-    const float kmz = live ? kt.km[2][c0 + cm.b] : 0.f;
-    const float2* g = fl.f[1] + off + cm.b + (long long)cm.j0 * es;
-    const long long gs = (long long)cm.js * es;
-    float2 h[FFT_MAXV / 2];
-#pragma unroll
-    for (int it = 0; it < FFT_MAXV / 2; ++it) {
-      h[it] = make_float2(0.f, 0.f);
-      if (live && cm.j0 + it * cm.js < n) h[it] = g[it * gs];
-    }
-#pragma unroll
-    for (int it = 0; it < FFT_MAXV / 2; ++it) {
-      const int j = cm.j0 + it * cm.js;
-      if (j < n) {
-        const float kmy = kt.km[1][j];
-        const int ls = j * B2 + cm.b;
-        s[ls] = make_float2(kmy * h[it].x, kmy * h[it].y);
-        s[ls + B] = make_float2(kmz * h[it].x, kmz * h[it].y);
-      }
-    }
-  }
-  __syncthreads();
-  fft_run<1, COLS, POW2, LN, ilog2c(BC) + 1, FFT_MAXV, FFT_TPB>(s, n, B2, stw);
-  const ColMap c2 = col_map<FFT_TPB>(lB + 1);
-  const int b2 = c2.b & (B - 1);
-  const bool live2 = b2 < nb;
-  float2* q = (c2.b < B ? fl.f[1] : fl.f[2]) + off + b2 + (long long)c2.j0 * es;
-  const long long qs = (long long)c2.js * es;
-  float2 r[FFT_MAXV];
-#pragma unroll
-  for (int it = 0; it < FFT_MAXV; ++it)
-    if (c2.j0 + it * c2.js < n) r[it] = s[threadIdx.x + it * FFT_TPB];
-#pragma unroll
-  for (int it = 0; it < FFT_MAXV; ++it)
-    if (live2 && c2.j0 + it * c2.js < n) q[it * qs] = r[it];
-}
-
-// Forward (adjoint path): FFT_y of c_1 and c_2 side by side, then their
-// combination km_1 FFT_y(c_1) + km_2 FFT_y(c_2) to c_1 for poisson_x_adj_k.
-template <bool POW2, int LN>
-__global__ void __launch_bounds__(FFT_TPB, 4) y_fwd_pair_k(Fields3 fl, int n_, long long es,
-                                                         long long ostride, int ncol, int B_,
-                                                         KTab kt, const float2* __restrict__ tw) {
-  extern __shared__ float2 s[];
-  constexpr int BC = LN ? fft_pencils(1 << LN, FFT_CAP / 2) : 1;
-  const int n = LN ? 1 << LN : n_;
-  const int B = LN ? BC : B_;
-  const int lB = LN ? ilog2c(BC) : __ffs(B_) - 1;
-  const int B2 = 2 * B;
-  float2* stw = s + 2 * n * B;
-  int bx, by, bz;
-  xcd_block3(bx, by, bz);
-  const int c0 = bx * B;
-  const int nb = ncol - c0 < B ? ncol - c0 : B;
-  const long long off = (long long)by * ostride + c0;
-  stage_twiddles<FFT_TPB>(stw, tw, n);
-  {
-    const ColMap c2 = col_map<FFT_TPB>(lB + 1);
-    const int b2 = c2.b & (B - 1);
-    const bool live2 = b2 < nb;
-    const float2* q = (c2.b < B ? fl.f[1] : fl.f[2]) + off + b2 + (long long)c2.j0 * es;
-    const long long qs = (long long)c2.js * es;
-    float2 r[FFT_MAXV];
-#pragma unroll
-    for (int it = 0; it < FFT_MAXV; ++it) {
-      r[it] = make_float2(0.f, 0.f);
-      if (live2 && c2.j0 + it * c2.js < n) r[it] = q[it * qs];
-    }
-#pragma unroll
-    for (int it = 0; it < FFT_MAXV; ++it)
-      if (c2.j0 + it * c2.js < n) s[threadIdx.x + it * FFT_TPB] = r[it];
-  }
-  __syncthreads();
-  fft_run<-1, COLS, POW2, LN, ilog2c(BC) + 1, FFT_MAXV, FFT_TPB>(s, n, B2, stw);
   const ColMap cm = col_map<FFT_TPB>(lB);
   const bool live = cm.b < nb;
   const float kmz = live ? kt.km[2][c0 + cm.b] : 0.f;
-  float2* g = fl.f[1] + off + cm.b + (long long)cm.j0 * es;
+  float2* g1 = fl.f[1] + off + cm.b + (long long)cm.j0 * es;
+  float2* g2 = fl.f[2] + off + cm.b + (long long)cm.j0 * es;
   const long long gs = (long long)cm.js * es;
-  float2 r[FFT_MAXV / 2];
+  float2 h[FFT_MAXV], r[FFT_MAXV];
 #pragma unroll
-  for (int it = 0; it < FFT_MAXV / 2; ++it) {
+  for (int it = 0; it < FFT_MAXV; ++it) {
+    h[it] = make_float2(0.f, 0.f);
+    if (live && cm.j0 + it * cm.js < n) h[it] = g1[it * gs];
+  }
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV; ++it) {
     const int j = cm.j0 + it * cm.js;
     if (j < n) {
-      const int ls = j * B2 + cm.b;
-      const float2 a = s[ls], c = s[ls + B];
       const float kmy = kt.km[1][j];
-      r[it] = make_float2(kmy * a.x + kmz * c.x, kmy * a.y + kmz * c.y);
+      s[threadIdx.x + it * FFT_TPB] = make_float2(kmy * h[it].x, kmy * h[it].y);
+    }
+  }
+  __syncthreads();
+  fft_run<1, COLS, POW2, LN, ilog2c(BC), FFT_MAXV, FFT_TPB>(s, n, B, stw);
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV; ++it)
+    if (cm.j0 + it * cm.js < n) r[it] = s[threadIdx.x + it * FFT_TPB];
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV; ++it)
+    if (cm.j0 + it * cm.js < n)
+      s[threadIdx.x + it * FFT_TPB] = make_float2(kmz * h[it].x, kmz * h[it].y);
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV; ++it)
+    if (live && cm.j0 + it * cm.js < n) g1[it * gs] = r[it];
+  __syncthreads();
+  fft_run<1, COLS, POW2, LN, ilog2c(BC), FFT_MAXV, FFT_TPB>(s, n, B, stw);
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV; ++it)
+    if (cm.j0 + it * cm.js < n) r[it] = s[threadIdx.x + it * FFT_TPB];
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV; ++it)
+    if (live && cm.j0 + it * cm.js < n) g2[it * gs] = r[it];
+}
+
+template <bool POW2, int LN>
+__global__ void __launch_bounds__(FFT_TPB, 2) y_fwd_seq_k(Fields3 fl, int n_, long long es,
+                                                        long long ostride, int ncol, int B_,
+                                                        KTab kt, const float2* __restrict__ tw) {
+  extern __shared__ float2 s[];
+  constexpr int BC = LN ? fft_pencils(1 << LN) : 1;
+  const int n = LN ? 1 << LN : n_;
+  const int B = LN ? BC : B_;
+  const int lB = LN ? ilog2c(BC) : __ffs(B_) - 1;
+  float2* stw = s + n * B;
+  int bx, by, bz;
+  xcd_block3(bx, by, bz);
+  const int c0 = bx * B;
+  const int nb = ncol - c0 < B ? ncol - c0 : B;
+  const long long off = (long long)by * ostride + c0;
+  stage_twiddles<FFT_TPB>(stw, tw, n);
+  const ColMap cm = col_map<FFT_TPB>(lB);
+  const bool live = cm.b < nb;
+  const float kmz = live ? kt.km[2][c0 + cm.b] : 0.f;
+  float2* g1 = fl.f[1] + off + cm.b + (long long)cm.j0 * es;
+  const float2* g2 = fl.f[2] + off + cm.b + (long long)cm.j0 * es;
+  const long long gs = (long long)cm.js * es;
+  float2 a[FFT_MAXV];
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV; ++it) {
+    a[it] = make_float2(0.f, 0.f);
+    if (live && cm.j0 + it * cm.js < n) a[it] = g1[it * gs];
+  }
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV; ++it)
+    if (cm.j0 + it * cm.js < n) s[threadIdx.x + it * FFT_TPB] = a[it];
+  __syncthreads();
+  fft_run<-1, COLS, POW2, LN, ilog2c(BC), FFT_MAXV, FFT_TPB>(s, n, B, stw);
+  {
+    float2 c[FFT_MAXV];
+#pragma unroll
+    for (int it = 0; it < FFT_MAXV; ++it) {
+      c[it] = make_float2(0.f, 0.f);
+      if (live && cm.j0 + it * cm.js < n) c[it] = g2[it * gs];
+    }
+#pragma unroll
+    for (int it = 0; it < FFT_MAXV; ++it)
+      if (cm.j0 + it * cm.js < n) a[it] = s[threadIdx.x + it * FFT_TPB];
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < FFT_MAXV; ++it)
+      if (cm.j0 + it * cm.js < n) s[threadIdx.x + it * FFT_TPB] = c[it];
+  }
+  __syncthreads();
+  fft_run<-1, COLS, POW2, LN, ilog2c(BC), FFT_MAXV, FFT_TPB>(s, n, B, stw);
+#pragma unroll
+  for (int it = 0; it < FFT_MAXV; ++it) {
+    const int j = cm.j0 + it * cm.js;
+    if (j < n) {
+      const float2 c = s[threadIdx.x + it * FFT_TPB];
+      const float kmy = kt.km[1][j];
+      a[it] = make_float2(kmy * a[it].x + kmz * c.x, kmy * a[it].y + kmz * c.y);
     }
   }
 #pragma unroll
-  for (int it = 0; it < FFT_MAXV / 2; ++it)
-    if (live && cm.j0 + it * cm.js < n) g[it * gs] = r[it];
+  for (int it = 0; it < FFT_MAXV; ++it)
+    if (live && cm.j0 + it * cm.js < n) g1[it * gs] = a[it];
 }
 
 // ------------------------------------------------------------- copies --
@@ -1275,19 +1291,18 @@ static KTab ktab(const pmx_ctx* c) {
 }
 
 // Paired y pass of the gradient's y and z components: dir > 0 turns h in
-// f[1] into the two force components in f[1] and f[2] (y_inv_pair_k);
+// f[1] into the two force components in f[1] and f[2] (y_inv_seq_k);
 // dir < 0 turns the cotangents in f[1] and f[2] into km_1 FFT_y(f[1]) +
-// km_2 FFT_y(f[2]) in f[1] (y_fwd_pair_k).
+// km_2 FFT_y(f[2]) in f[1] (y_fwd_seq_k).
 static int pmx_fft_ypair(pmx_ctx* c, float2* const* f, int dir) {
   Fields3 fl = {{f[0], f[1], f[2]}};
   const int n = c->n[1];
-  const int B = fft_pencils(n, FFT_CAP / 2);
+  const int B = fft_pencils(n);
   const int ncol = c->n2h;
   const long long es = c->n2p, ostride = (long long)n * c->n2p;
   dim3 grid((ncol + B - 1) / B, c->n[0], 1);
-  size_t lds = 2 * lds_bytes(n, B) + tw_bytes(n);
-  // 2 n B fits for every axis length the context accepts (<= 4096)
-  if (int rc = fft_fits("paired y FFT", n, 2 * n * B, FFT_CAP, lds)) return rc;
+  const size_t lds = lds_bytes(n, B) + tw_bytes(n);
+  if (int rc = fft_fits("paired y FFT", n, n * B, FFT_CAP, lds)) return rc;
   const KTab kt = ktab(c);
   const bool p2 = is_pow2(n);
   const int ln = ct_log2(n);
@@ -1295,13 +1310,13 @@ static int pmx_fft_ypair(pmx_ctx* c, float2* const* f, int dir) {
   pmx_tic(c, cls);
 #define YPAIR(L)                                                                                  \
   if (dir < 0 && (L || p2))                                                                       \
-    y_fwd_pair_k<true, L><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]); \
+    y_fwd_seq_k<true, L><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]); \
   else if (dir < 0)                                                                               \
-    y_fwd_pair_k<false, 0><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]); \
+    y_fwd_seq_k<false, 0><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]); \
   else if (L || p2)                                                                               \
-    y_inv_pair_k<true, L><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]); \
+    y_inv_seq_k<true, L><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1]); \
   else                                                                                            \
-    y_inv_pair_k<false, 0><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1])
+    y_inv_seq_k<false, 0><<<grid, FFT_TPB, lds, c->stream>>>(fl, n, es, ostride, ncol, B, kt, c->tw[1])
   PMX_CT_SWITCH(ln, YPAIR);
 #undef YPAIR
   pmx_toc(c, cls);
