@@ -222,7 +222,8 @@ int pmx_dot(pmx_ctx* ctx, const float* a, const float* b, long long n, double* o
  * Kernel timing with HIP events recorded on the context stream around each
  * launch of the hot path, grouped in PMX_NTIMER_CLASSES classes (bin, scan,
  * fill, deposit, fold, z-r2c, y-fwd, poisson-x, y-inv, z-c2r, gather,
- * vjp-gather, grad-gather, kick/drift-adjoint, other).
+ * vjp-gather, grad-gather, kick/drift-adjoint, other).  The fold class is
+ * empty where the Poisson solve's row pass adds the halo shells itself.
  * pmx_timing_read synchronises the stream and returns, per class, the summed
  * milliseconds and the number of launches since the last read.  No pmwd
  * counterpart: measurement support.                                         */
