@@ -87,18 +87,20 @@ class SimResult:
 def context_bytes(mesh_shape, n_ptcl, adjoint):
     """Device bytes a library context of this mesh holds once used for n_ptcl
     particles: three complex mesh buffers, twiddles, k tables, tile arrays
-    and halo shells (three channels with the adjoint), and per record slot
-    (about 1.2 n_ptcl slots) the record, index, tile and rank words, four
-    value channels and the multi-step pipeline's own particle copy."""
+    and halo shells, and per record slot (about 1.2 n_ptcl slots) the record,
+    index, tile and rank words, four value channels and the multi-step
+    pipeline's own particle copy.  The reverse-time pipeline adds three
+    cotangent meshes, a 66-byte particle copy per slot and two more halo
+    channels."""
     n0, n1, n2 = mesh_shape
     n2p = n2 // 2 + 1
     n2p = (n2p + 15) // 16 * 16 if n2p > 16 else n2p
-    mesh = 3 * n0 * n1 * n2p * 8
+    mesh = (6 if adjoint else 3) * n0 * n1 * n2p * 8
     tiles = math.ceil(n0 / 16) * math.ceil(n1 / 16) * math.ceil(n2 / 16)
     tile_bytes = tiles * (40 + 4 * 817 * (3 if adjoint else 1))
     cap = n_ptcl + n_ptcl // 16 + 1024
     slots = cap + cap // 8 + 33 * (tiles + 1)
-    per_slot = 16 + 4 + 8 + 16 + 34
+    per_slot = 16 + 4 + 8 + 16 + 34 + (66 if adjoint else 0)
     return int(mesh + tile_bytes + slots * per_slot + 64 * 2**20)
 
 
