@@ -71,7 +71,7 @@ def kernel_bytes(N, mesh_shape, tile, K, fold_on_load=False):
     drift; acc is written once, by the last step (which writes acc, vel and
     disp: 66N + 12M).  The first kick/drift/bin of the call reads vel, acc,
     disp, pmid and writes vel, disp (66N).  The second figure is what the
-    kernels move by design: tile-ordered particles of 34 B (s0, s1, pm_z)
+    kernels move by design: tile-ordered particles of 32 B (s0, s1)
     in place of pmid/disp/vel, tile and rank words, the +1 halo shells.
     With fold_on_load the Poisson row pass reads those shells as it loads
     the density and no fold pass runs.
@@ -88,9 +88,9 @@ def kernel_bytes(N, mesh_shape, tile, K, fold_on_load=False):
         # vel, disp; by design also the tile and rank words
         'bin': (66 * N, 74 * N),
         'scan': (0, 8 * ntiles * (2 * K - 1)),
-        # first fill: read pmid, disp, vel, tile, rank; write 34 B per particle
-        'fill': (0, 72 * N),
-        'deposit': (K * (18 * N + 4 * M), K * (18 * N + 4 * M + 4 * shell * ntiles)),
+        # first fill: read pmid, disp, vel, tile, rank; write 32 B per particle
+        'fill': (0, 70 * N),
+        'deposit': (K * (18 * N + 4 * M), K * (16 * N + 4 * M + 4 * shell * ntiles)),
         'fold': (0, 0 if fold_on_load else K * (4 * shell + 8 * faces) * ntiles),
         'z_r2c': (K * (4 * M + 8 * Mh),
                   K * (4 * M + 8 * Mh + (4 * shell * ntiles if fold_on_load else 0))),
@@ -104,7 +104,7 @@ def kernel_bytes(N, mesh_shape, tile, K, fold_on_load=False):
         # three launches, each reads a complex and writes a real mesh
         'z_c2r': (K * 3 * (8 * Mh + 4 * M),) * 2,
         'gather': ((K - 1) * (54 * N + 12 * M) + 66 * N + 12 * M,
-                   (K - 1) * (68 * N + 12 * M) + 70 * N + 12 * M),
+                   (K - 1) * (64 * N + 12 * M) + 68 * N + 12 * M),
     }
 
 

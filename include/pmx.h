@@ -54,7 +54,7 @@ int pmx_ctx_set_stream(pmx_ctx* ctx, void* stream);
 long long pmx_ctx_bytes(const pmx_ctx* ctx);
 /* Frees context scratch once the context's stream has finished; the next
  * call reallocates what it needs.  what = PMX_TRIM_PIPELINE: the particle
- * copy of pmx_nbody_steps (34 B per record slot); PMX_TRIM_ALL: every
+ * copy of pmx_nbody_steps (32 B per record slot); PMX_TRIM_ALL: every
  * particle-sized buffer (records, indices, value channels, that copy) and
  * extra halo channels.  No counterpart in pmwd, where XLA owns every
  * buffer: for callers that need the memory between phases (an adjoint
@@ -156,8 +156,9 @@ int pmx_nbody_step(pmx_ctx* ctx, const int16_t* pmid, float* disp, float* vel,
 /* nsteps KDK steps of pmwd/nbody.py:121-140 (the loop of nbody, :186-201)
  * with factors k1[s], d[s], k2[s] (host arrays): bitwise the result of
  * nsteps pmx_nbody_step calls.  Between steps the particles stay on the
- * device in tile order (two extra copies of about 34 bytes per particle);
- * disp, vel and acc are written once, at the end.                          */
+ * device in tile order (two extra copies of about 32 bytes per particle);
+ * disp, vel and acc are written once, at the end.  Meshes with an axis
+ * longer than 1024 cells run as nsteps pmx_nbody_step calls.               */
 int pmx_nbody_steps(pmx_ctx* ctx, const int16_t* pmid, float* disp, float* vel,
                     float* acc, long long n, float val, float coef, int nsteps,
                     const float* k1, const float* d, const float* k2);
@@ -208,7 +209,8 @@ int pmx_nbody_step_adj(pmx_ctx* ctx, const int16_t* pmid, float* disp,
  * dots[2s], dots[2s+1] (host, 2*nsteps doubles) are step s's two dots,
  * the same sums in another grouping.  Between steps the particles and
  * their cotangents stay on the device in tile order (about 100 extra
- * bytes per particle).                                                     */
+ * bytes per particle).  Meshes with an axis longer than 1024 cells run as
+ * the chained calls.                                                       */
 int pmx_nbody_steps_adj(pmx_ctx* ctx, const int16_t* pmid, float* disp,
                         float* vel, float* acc, float* disp_cot, float* vel_cot,
                         float* acc_cot, long long n, float val, float coef,

@@ -177,7 +177,7 @@ __device__ __forceinline__ double fixed_scale(float vmax) {
 // stored with plain stores, the +1 shell to the tile's halo slot.  With
 // nch > 1 the channels run one after the other in the same workgroup, the
 // records of the tile coming back from L2.  STATE reads tile-ordered
-// particles (rec = their s0, pz their pm_z) and forms d and the corner
+// particles (rec = their s0, pm packed in s0.w) and forms d and the corner
 // itself.
 #define DEP_TPB 512
 #ifndef DEP_U
@@ -260,8 +260,7 @@ __device__ __forceinline__ void dep_store(const unsigned long long* acc, float* 
 
 template <int PERVAL, int STATE = 0>
 __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DEP_WPE)))
-    deposit_k(const float4* __restrict__ rec, const uint16_t* __restrict__ pz,
-              const float* __restrict__ vrec, long long vstride, int vstep,
+    deposit_k(const float4* __restrict__ rec, const float* __restrict__ vrec, long long vstride, int vstep,
               const float* __restrict__ vmaxp, const uint32_t* __restrict__ tile_start,
               const uint32_t* __restrict__ tile_count, MeshGeom g, TileGeom tg, float val,
               DepOut out, int nch, int mrs, int accumulate, float* __restrict__ halo,
@@ -288,13 +287,11 @@ __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DE
     for (uint32_t q0 = 0; q0 < cnt; q0 += DEP_TPB * DEP_U) {
       float4 r[DEP_U];
       float v[DEP_U];
-      uint16_t zz[DEP_U];
 #pragma unroll
       for (int u = 0; u < DEP_U; ++u) {
         uint32_t q = q0 + u * DEP_TPB + threadIdx.x;
         if (q < cnt) {
           r[u] = rec[start + q];
-          if (STATE) zz[u] = pz[start + q];
           v[u] = PERVAL ? vr[(long long)(start + q) * vstep] : val;
         }
       }
@@ -307,7 +304,7 @@ __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DE
           float d[3];
           int l[3];
           // binned by this tile, so always inside; the guard keeps LDS in bounds
-          if (!pmx_state_local(r[u], zz[u], g, o, s, d, l)) continue;
+          if (!pmx_state_local(r[u], g, o, s, d, l)) continue;
           r[u].x = d[0];
           r[u].y = d[1];
           r[u].z = d[2];
@@ -564,12 +561,12 @@ static int deposit_run(pmx_ctx* c, float val, int perval, int nch, long long n, 
   if ((rc = pmx_ensure_halo(c, nch))) return rc;
   pmx_tic(c, PMX_T_DEPOSIT);
   if (perval)
-    deposit_k<1><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, nullptr, c->vrec, c->rec_cap, 1,
+    deposit_k<1><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, c->vrec, c->rec_cap, 1,
                                                   c->vmax, c->tile_start, c->tile_count, g, tg,
                                                   val, out, nch, mrs, accumulate, c->halo,
                                                   hstride);
   else
-    deposit_k<0><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, nullptr, nullptr, 0, 1, nullptr,
+    deposit_k<0><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, nullptr, 0, 1, nullptr,
                                                   c->tile_start, c->tile_count, g, tg, val, out,
                                                   nch, mrs, accumulate, c->halo, hstride);
   pmx_toc(c, PMX_T_DEPOSIT);
@@ -620,11 +617,11 @@ int pmx_deposit_state_vals(pmx_ctx* c, const PState& st, const uint32_t* start,
   if ((rc = pmx_ensure_halo(c, nch))) return rc;
   pmx_tic(c, PMX_T_DEPOSIT);
   if (vals)
-    deposit_k<1, 1><<<nt, DEP_TPB, lds, c->stream>>>(st.s0, st.pz, vals, vstride, vstep, c->vmax,
+    deposit_k<1, 1><<<nt, DEP_TPB, lds, c->stream>>>(st.s0, vals, vstride, vstep, c->vmax,
                                                      start, count, g, tg, 0.f, out, nch, mrs, 0,
                                                      c->halo, hstride);
   else
-    deposit_k<0, 1><<<nt, DEP_TPB, lds, c->stream>>>(st.s0, st.pz, nullptr, 0, 1, nullptr, start,
+    deposit_k<0, 1><<<nt, DEP_TPB, lds, c->stream>>>(st.s0, nullptr, 0, 1, nullptr, start,
                                                      count, g, tg, val, out, nch, mrs, 0, c->halo,
                                                      hstride);
   pmx_toc(c, PMX_T_DEPOSIT);

@@ -185,9 +185,9 @@ extern "C" int pmx_ctx_destroy(pmx_ctx* c) {
   for (void* p : ps)
     if (p) hipFree(p);
   // pl[1], al[1], pl_tile and pl_rank alias the scratch freed above
-  void* qs[] = {c->pl[0].s0, c->pl[0].s1, c->pl[0].pz, c->pl_count[0], c->pl_start[0],
-                c->pl_count[1], c->pl_start[1], c->al[0].p.s0, c->al[0].p.s1, c->al[0].p.pz,
-                c->al[0].c0, c->al[0].c1, c->abuf[0], c->abuf[1], c->abuf[2], c->al_dots};
+  void* qs[] = {c->pl[0].s0, c->pl[0].s1, c->pl_count[0], c->pl_start[0], c->pl_count[1],
+                c->pl_start[1], c->al[0].p.s0, c->al[0].p.s1, c->al[0].c0, c->al[0].c1,
+                c->abuf[0], c->abuf[1], c->abuf[2], c->al_dots};
   for (void* p : qs)
     if (p) hipFree(p);
   if (c->red_host) hipHostFree(c->red_host);
@@ -216,7 +216,6 @@ extern "C" int pmx_ctx_trim(pmx_ctx* c, int what) {
     if (what & PMX_TRIM_PIPELINE) {
       dev_free(c, (void**)&c->pl[0].s0, c->pl_cap * 16);
       dev_free(c, (void**)&c->pl[0].s1, c->pl_cap * 16);
-      dev_free(c, (void**)&c->pl[0].pz, c->pl_cap * 2);
       c->pl_cap = 0;
       free_adj_state(c);
     }
@@ -230,7 +229,6 @@ extern "C" int pmx_ctx_trim(pmx_ctx* c, int what) {
   dev_free(c, (void**)&c->vrec, c->vrec_cap * 4);
   dev_free(c, (void**)&c->pl[0].s0, c->pl_cap * 16);
   dev_free(c, (void**)&c->pl[0].s1, c->pl_cap * 16);
-  dev_free(c, (void**)&c->pl[0].pz, c->pl_cap * 2);
   c->pl[1] = PState{};
   c->pl_tile = c->pl_rank = nullptr;
   c->ptcl_cap = c->rec_cap = c->vrec_cap = c->pl_cap = 0;
@@ -261,7 +259,6 @@ int pmx_ensure_ptcl(pmx_ctx* c, long long n) {
   dev_free(c, (void**)&c->vrec, c->vrec_cap * 4);
   dev_free(c, (void**)&c->pl[0].s0, c->pl_cap * 16);
   dev_free(c, (void**)&c->pl[0].s1, c->pl_cap * 16);
-  dev_free(c, (void**)&c->pl[0].pz, c->pl_cap * 2);
   free_adj_state(c);
   c->ptcl_cap = c->rec_cap = c->vrec_cap = c->pl_cap = 0;
   c->caps_valid = 0;
@@ -287,7 +284,7 @@ int pmx_ensure_halo(pmx_ctx* c, int nch) {
 
 // The multi-step pipeline's two tile-ordered particle copies of rec_cap
 // slots: the first is its own memory, the second lives in the per-step
-// path's scratch (s0 in rec, s1 in the four vrec channels, pz in pidx), and
+// path's scratch (s0 in rec, s1 in the four vrec channels), and
 // the exact re-bin's per-slot tile and rank in pkey/prank.  The per-step
 // path rewrites all of these before it reads them.
 int pmx_ensure_state(pmx_ctx* c) {
@@ -297,11 +294,9 @@ int pmx_ensure_state(pmx_ctx* c) {
   if (cap > c->pl_cap) {
     dev_free(c, (void**)&c->pl[0].s0, c->pl_cap * 16);
     dev_free(c, (void**)&c->pl[0].s1, c->pl_cap * 16);
-    dev_free(c, (void**)&c->pl[0].pz, c->pl_cap * 2);
     c->pl_cap = 0;
     rc = dev_alloc(c, (void**)&c->pl[0].s0, cap * 16);
     rc |= dev_alloc(c, (void**)&c->pl[0].s1, cap * 16);
-    rc |= dev_alloc(c, (void**)&c->pl[0].pz, cap * 2);
     for (int b = 0; b < 2; ++b) {
       if (!c->pl_count[b]) rc |= dev_alloc(c, (void**)&c->pl_count[b], (size_t)(c->ntiles + 1) * 4);
       if (!c->pl_start[b]) rc |= dev_alloc(c, (void**)&c->pl_start[b], (size_t)(c->ntiles + 1) * 4);
@@ -311,7 +306,6 @@ int pmx_ensure_state(pmx_ctx* c) {
   }
   c->pl[1].s0 = c->rec;
   c->pl[1].s1 = reinterpret_cast<float4*>(c->vrec);
-  c->pl[1].pz = reinterpret_cast<uint16_t*>(c->pidx);
   c->pl_tile = c->pkey;
   c->pl_rank = c->prank;
   return PMX_OK;
@@ -320,8 +314,7 @@ int pmx_ensure_state(pmx_ctx* c) {
 // The reverse-time pipeline's own particle copy and per-slot gather
 // results (rec_cap slots each), its per-step dot products, and its second
 // copy in the per-step scratch and the forward pipeline's copy: p.s0 in rec,
-// p.s1 in the four vrec channels, p.pz in pidx, c0/c1 in the forward copy's
-// s0/s1.
+// p.s1 in the four vrec channels, c0/c1 in the forward copy's s0/s1.
 int pmx_ensure_adj_state(pmx_ctx* c, int nsteps) {
   int rc;
   if ((rc = pmx_ensure_state(c))) return rc;
@@ -331,13 +324,11 @@ int pmx_ensure_adj_state(pmx_ctx* c, int nsteps) {
     const long long old = c->al_cap;
     dev_free(c, (void**)&a.p.s0, old * 16);
     dev_free(c, (void**)&a.p.s1, old * 16);
-    dev_free(c, (void**)&a.p.pz, old * 2);
     dev_free(c, (void**)&a.c0, old * 16);
     dev_free(c, (void**)&a.c1, old * 16);
     c->al_cap = 0;
     rc = dev_alloc(c, (void**)&a.p.s0, cap * 16);
     rc |= dev_alloc(c, (void**)&a.p.s1, cap * 16);
-    rc |= dev_alloc(c, (void**)&a.p.pz, cap * 2);
     rc |= dev_alloc(c, (void**)&a.c0, cap * 16);
     rc |= dev_alloc(c, (void**)&a.c1, cap * 16);
     if (rc) return PMX_ENOMEM;
@@ -364,7 +355,6 @@ static void free_adj_state(pmx_ctx* c) {
   const long long cap = c->al_cap;
   dev_free(c, (void**)&a.p.s0, cap * 16);
   dev_free(c, (void**)&a.p.s1, cap * 16);
-  dev_free(c, (void**)&a.p.pz, cap * 2);
   dev_free(c, (void**)&a.c0, cap * 16);
   dev_free(c, (void**)&a.c1, cap * 16);
   for (int b = 0; b < 3; ++b) dev_free(c, (void**)&c->abuf[b], (size_t)c->Mc * sizeof(float2));

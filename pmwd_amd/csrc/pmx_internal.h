@@ -74,13 +74,14 @@ __device__ __forceinline__ int pmx_tdiv(int lo, unsigned m) {
 }
 
 // A particle held in tile order by the multi-step pipeline: s0 = (disp,
-// pm_x | pm_y << 16), s1 = (vel, index in the caller's arrays), pz = pm_z,
+// pm_x | pm_y << 10 | pm_z << 20), s1 = (vel, index in the caller's arrays),
 // pm taken mod the mesh extent per axis (the CIC corner wraps, so only pm
-// mod n enters).
+// mod n enters; the pipelines take meshes of at most PMX_PL_MAXN cells per
+// axis, so each fits 10 bits).
+#define PMX_PL_MAXN 1024
 struct PState {
   float4* s0;
   float4* s1;
-  uint16_t* pz;
 };
 
 // The reverse-time pipeline's particle: a PState plus c0 = (disp_cot, 0)
@@ -726,21 +727,24 @@ __device__ __forceinline__ void wave_max_store(float m, float* vmax) {
 }
 
 // ------------------------------------------------ tile-ordered particles --
-__device__ __forceinline__ void pmx_state_pm(float4 s0, uint16_t pz, int pm[3]) {
+__device__ __forceinline__ uint32_t pmx_state_pack(uint32_t px, uint32_t py, uint32_t pz) {
+  return px | (py << 10) | (pz << 20);
+}
+
+__device__ __forceinline__ void pmx_state_pm(float4 s0, int pm[3]) {
   const uint32_t b = __float_as_uint(s0.w);
-  pm[0] = (int)(b & 0xffffu);
-  pm[1] = (int)(b >> 16);
-  pm[2] = (int)pz;
+  pm[0] = (int)(b & 0x3ffu);
+  pm[1] = (int)((b >> 10) & 0x3ffu);
+  pm[2] = (int)(b >> 20);
 }
 
 // d = disp/cell and the lower corner relative to tile origin o of a state
 // entry (the values pmx_make_record packs); false when the corner is not in
 // the tile of extent s.
-__device__ __forceinline__ bool pmx_state_local(float4 s0, uint16_t pz, const MeshGeom& g,
-                                                const int o[3], const int s[3], float d[3],
-                                                int l[3]) {
+__device__ __forceinline__ bool pmx_state_local(float4 s0, const MeshGeom& g, const int o[3],
+                                                const int s[3], float d[3], int l[3]) {
   int pm[3];
-  pmx_state_pm(s0, pz, pm);
+  pmx_state_pm(s0, pm);
   const float dp[3] = {s0.x, s0.y, s0.z};
   CicAxis ax[3];
   cic_setup(pm, dp, g, ax);

@@ -49,11 +49,9 @@ struct PlUser {
   float* acc;
 };
 
-__device__ __forceinline__ void pl_put(const PState& st, uint32_t slot, float4 s0, float4 s1,
-                                       uint16_t pz) {
+__device__ __forceinline__ void pl_put(const PState& st, uint32_t slot, float4 s0, float4 s1) {
   st.s0[slot] = s0;
   st.s1[slot] = s1;
-  st.pz[slot] = pz;
 }
 
 // Writes particle i (after the first kick and drift) to the slot
@@ -76,12 +74,12 @@ __global__ void __launch_bounds__(PMX_TPB)
   const long long i = b0 + q;
   const uint32_t px = (uint32_t)pmx_wrap(sp[3 * q], g.n0);
   const uint32_t py = (uint32_t)pmx_wrap(sp[3 * q + 1], g.n1);
-  const uint16_t pz = (uint16_t)pmx_wrap(sp[3 * q + 2], g.n2);
+  const uint32_t pz = (uint32_t)pmx_wrap(sp[3 * q + 2], g.n2);
   const float4 s0 = make_float4(sh[3 * q], sh[3 * q + 1], sh[3 * q + 2],
-                                __uint_as_float(px | (py << 16)));
+                                __uint_as_float(pmx_state_pack(px, py, pz)));
   const float4 s1 = make_float4(sh[T3 + 3 * q], sh[T3 + 3 * q + 1], sh[T3 + 3 * q + 2],
                                 __uint_as_float((uint32_t)i));
-  pl_put(st, start[ptile[i]] + prank[i], s0, s1, pz);
+  pl_put(st, start[ptile[i]] + prank[i], s0, s1);
 }
 
 // Capacities of the next copy from the counts of the current one; clears
@@ -131,7 +129,6 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
   const uint32_t cnt = in_count[t];
   for (uint32_t q0 = 0; q0 < cnt; q0 += PL_TPB * PL_U) {
     float4 a0[PL_U], a1[PL_U], n0[PL_U], n1[PL_U];
-    uint16_t az[PL_U];
     uint32_t nt[PL_U];
     int code[PL_U];
 #pragma unroll
@@ -140,7 +137,6 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
       if (q < cnt) {
         a0[u] = in.s0[start + q];
         a1[u] = in.s1[start + q];
-        az[u] = in.pz[start + q];
       }
     }
 #pragma unroll
@@ -155,7 +151,7 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
         float d[3];
         int l[3];
         // binned by this tile, so always inside; the guard keeps LDS in bounds
-        if (pmx_state_local(a0[u], az[u], g, o, s, d, l))
+        if (pmx_state_local(a0[u], g, o, s, d, l))
           pmx_tile_acc(sh, ltot, LY, LZ, d[0], d[1], d[2], l[0], l[1], l[2], acc);
         float v[3] = {a1[u].x, a1[u].y, a1[u].z};
         float x[3] = {a0[u].x, a0[u].y, a0[u].z};
@@ -176,7 +172,7 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
             x[a] = x[a] + v[a] * dn;
           }
           int pm[3];
-          pmx_state_pm(a0[u], az[u], pm);
+          pmx_state_pm(a0[u], pm);
           CicAxis ax[3];
           cic_setup(pm, x, g, ax);
           nt[u] = tile_of(ax, tg);
@@ -192,8 +188,7 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
       for (int u = 0; u < PL_U; ++u) {
         const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
         if (q < cnt)
-          pl_put(out.st, out.start[out.ptile[start + q]] + out.prank[start + q], n0[u], n1[u],
-                 az[u]);
+          pl_put(out.st, out.start[out.ptile[start + q]] + out.prank[start + q], n0[u], n1[u]);
       }
       continue;
     }
@@ -229,7 +224,7 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
       const uint32_t r = code[u] < 27 ? s_base[code[u]] + rk[u] : rk[u];
       const uint32_t b = out.start[nt[u]], e = out.start[nt[u] + 1];
       if (r < e - b)
-        pl_put(out.st, b + r, n0[u], n1[u], az[u]);
+        pl_put(out.st, b + r, n0[u], n1[u]);
       else
         out.flag[0] = 1u;
     }
@@ -255,10 +250,11 @@ extern "C" int pmx_nbody_steps(pmx_ctx* c, const int16_t* pmid, float* disp, flo
   PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_nbody_steps: too many particles");
   int rc;
   if (nsteps == 0) return PMX_OK;
-  if (n == 0) {
-    // no particles: the steps solve an empty mesh, as pmx_nbody_step does
+  if (n == 0 || c->n[0] > PMX_PL_MAXN || c->n[1] > PMX_PL_MAXN || c->n[2] > PMX_PL_MAXN) {
+    // no particles (the steps solve an empty mesh), or a mesh axis longer
+    // than the tile-ordered state packs: one step at a time
     for (int s = 0; s < nsteps; ++s)
-      if ((rc = pmx_nbody_step(c, pmid, disp, vel, acc, 0, val, coef, k1[s], d[s], k2[s])))
+      if ((rc = pmx_nbody_step(c, pmid, disp, vel, acc, n, val, coef, k1[s], d[s], k2[s])))
         return rc;
     return PMX_OK;
   }
@@ -342,10 +338,9 @@ extern "C" int pmx_nbody_steps(pmx_ctx* c, const int16_t* pmid, float* disp, flo
 // products sum the same terms in another grouping.
 
 __device__ __forceinline__ void al_put(const AState& st, uint32_t slot, float4 s0, float4 s1,
-                                       uint16_t pz, float4 c0, float4 c1) {
+                                       float4 c0, float4 c1) {
   st.p.s0[slot] = s0;
   st.p.s1[slot] = s1;
-  st.p.pz[slot] = pz;
   st.c0[slot] = c0;
   st.c1[slot] = c1;
 }
@@ -373,13 +368,13 @@ __global__ void __launch_bounds__(PMX_TPB)
     const long long i = b0 + q;
     const uint32_t px = (uint32_t)pmx_wrap(sp[3 * q], g.n0);
     const uint32_t py = (uint32_t)pmx_wrap(sp[3 * q + 1], g.n1);
-    const uint16_t pz = (uint16_t)pmx_wrap(sp[3 * q + 2], g.n2);
+    const uint32_t pz = (uint32_t)pmx_wrap(sp[3 * q + 2], g.n2);
     const float* e = sh + 3 * q;
 #pragma unroll
     for (int a = 0; a < 3; ++a) vm[a] = fabsf(e[3 * T3 + a]);
     al_put(st, start[ptile[i]] + prank[i],
-           make_float4(e[0], e[1], e[2], __uint_as_float(px | (py << 16))),
-           make_float4(e[T3], e[T3 + 1], e[T3 + 2], __uint_as_float((uint32_t)i)), pz,
+           make_float4(e[0], e[1], e[2], __uint_as_float(pmx_state_pack(px, py, pz))),
+           make_float4(e[T3], e[T3 + 1], e[T3 + 2], __uint_as_float((uint32_t)i)),
            make_float4(e[2 * T3], e[2 * T3 + 1], e[2 * T3 + 2], 0.f),
            make_float4(e[3 * T3], e[3 * T3 + 1], e[3 * T3 + 2], 0.f));
   }
@@ -448,7 +443,6 @@ __global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4))
   float vm[3] = {0.f, 0.f, 0.f};
   for (uint32_t q0 = 0; q0 < cnt; q0 += AL_TPB * AL_U) {
     float4 a0[AL_U], a1[AL_U], c0[AL_U], c1[AL_U];
-    uint16_t az[AL_U];
 #pragma unroll
     for (int u = 0; u < AL_U; ++u) {
       const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
@@ -458,7 +452,6 @@ __global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4))
         a1[u] = in.p.s1[sl];
         c0[u] = in.c0[sl];
         c1[u] = in.c1[sl];
-        az[u] = in.p.pz[sl];
       }
     }
     float4 n0[AL_U], n1[AL_U], m0[AL_U], m1[AL_U];
@@ -476,7 +469,7 @@ __global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4))
       float x[3] = {a0[u].x, a0[u].y, a0[u].z}, v[3] = {a1[u].x, a1[u].y, a1[u].z};
       float dc[3] = {c0[u].x, c0[u].y, c0[u].z}, vc[3] = {c1[u].x, c1[u].y, c1[u].z};
       // binned by this tile, so always inside; the guard keeps LDS in bounds
-      if (pmx_state_local(a0[u], az[u], g, o, s, d, l)) {
+      if (pmx_state_local(a0[u], g, o, s, d, l)) {
         pmx_vjp_one(sh, ltot, LY, LZ, d, l, vc, g.cell, ac, at);
         pmx_grad_one(sh + 3 * ltot, LY, LZ, d, l, val, g.cell, gr);
       }
@@ -516,7 +509,7 @@ __global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4))
         for (int a = 0; a < 3; ++a) x[a] = snap[3 * i + a];
       }
       int pm[3];
-      pmx_state_pm(a0[u], az[u], pm);
+      pmx_state_pm(a0[u], pm);
       CicAxis ax[3];
       cic_setup(pm, x, g, ax);
       nt[u] = tile_of(ax, tg);
@@ -533,7 +526,7 @@ __global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4))
         const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
         if (q < cnt)
           al_put(out.st, out.start[out.ptile[start + q]] + out.prank[start + q], n0[u], n1[u],
-                 az[u], m0[u], m1[u]);
+                 m0[u], m1[u]);
       }
       continue;
     }
@@ -568,7 +561,7 @@ __global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4))
       const uint32_t r = code[u] < 27 ? s_base[code[u]] + rk[u] : rk[u];
       const uint32_t b = out.start[nt[u]], e = out.start[nt[u] + 1];
       if (r < e - b)
-        al_put(out.st, b + r, n0[u], n1[u], az[u], m0[u], m1[u]);
+        al_put(out.st, b + r, n0[u], n1[u], m0[u], m1[u]);
       else
         out.flag[0] = 1u;
     }
@@ -608,6 +601,24 @@ extern "C" int pmx_nbody_steps_adj(pmx_ctx* c, const int16_t* pmid, float* disp,
   if (n == 0) {
     if (dots)
       for (int s = 0; s < 2 * nsteps; ++s) dots[s] = 0;
+    return PMX_OK;
+  }
+  if (c->n[0] > PMX_PL_MAXN || c->n[1] > PMX_PL_MAXN || c->n[2] > PMX_PL_MAXN) {
+    // a mesh axis longer than the tile-ordered state packs: one reverse step
+    // at a time, each closing kick pending into the next
+    float kpi = kp;
+    for (int s = 0; s < nsteps; ++s) {
+      double dd[2];
+      if ((rc = pmx_nbody_step_adj(c, pmid, disp, vel, acc, disp_cot, vel_cot, acc_cot, n, val,
+                                   coef, kpi, ka[s], d[s],
+                                   s + 1 == nsteps ? disp_snap : nullptr, dd)))
+        return rc;
+      if (dots) {
+        dots[2 * s] = dd[0];
+        dots[2 * s + 1] = dd[1];
+      }
+      kpi = kb[s];
+    }
     return PMX_OK;
   }
   if ((rc = pmx_ensure_ptcl(c, n))) return rc;

@@ -59,8 +59,11 @@ def assert_same(a, b):
                                                    ((16, 16, 16), 2, 1, 1.0),
                                                    ((12, 10, 8), 1, 2, 1.0),
                                                    ((16, 16, 16), 1, 4, 1.3),
-                                                   ((12, 8, 20), 2, 3, 0.7)])
+                                                   ((12, 8, 20), 2, 3, 0.7),
+                                                   ((1056, 2, 4), 1, 2, 1.0)])
 def test_steps_match_single_steps(dev, shape, ratio, k, spacing):
+    """(1056, 2, 4): an axis beyond the 1024 cells the tile-ordered state
+    packs, which the library runs one step at a time."""
     import pmwd_amd as pa
     conf = pa.Configuration(spacing, shape, mesh_shape=ratio)
     pmid, disp = particles(shape, ratio, conf.cell_size, 1)
@@ -213,7 +216,7 @@ def run_adj(conf, dev, pmid, arrs, kp, ka, d, kb, multi, snap=None, coef=0.45):
 @pytest.mark.parametrize('shape,ratio,k,spacing,kp,snap', [
     ((16, 16, 16), 2, 4, 1.0, 0.3, False), ((32, 32, 32), 1, 3, 1.0, 0.0, True),
     ((12, 10, 8), 1, 1, 1.0, 0.2, True), ((7, 8, 6), 1, 2, 1.3, 0.0, False),
-    ((12, 8, 20), 2, 3, 0.7, 0.25, True)])
+    ((12, 8, 20), 2, 3, 0.7, 0.25, True), ((1056, 2, 4), 1, 2, 1.0, 0.2, True)])
 def test_steps_adj_match_step_adj(dev, shape, ratio, k, spacing, kp, snap):
     import pmwd_amd as pa
     conf = pa.Configuration(spacing, shape, mesh_shape=ratio)
