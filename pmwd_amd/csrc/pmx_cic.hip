@@ -260,8 +260,7 @@ __device__ __forceinline__ void dep_store(const unsigned long long* acc, float* 
 
 template <int PERVAL, int STATE = 0>
 __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DEP_WPE)))
-    deposit_k(const float4* __restrict__ rec, const float* __restrict__ vrec, long long vstride, int vstep,
-              const float* __restrict__ vmaxp, const uint32_t* __restrict__ tile_start,
+    deposit_k(const float4* __restrict__ rec, ValSrc vals, const float* __restrict__ vmaxp, const uint32_t* __restrict__ tile_start,
               const uint32_t* __restrict__ tile_count, MeshGeom g, TileGeom tg, float val,
               DepOut out, int nch, int mrs, int accumulate, float* __restrict__ halo,
               long long hstride) {
@@ -279,7 +278,8 @@ __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DE
     const double inv = 1.0 / scale;
     const bool fq = scale <= 0x1p126;
     const float sf = fq ? (float)scale : 0.f;
-    const float* vr = PERVAL ? vrec + (long long)ch * vstride : nullptr;
+    const float* vr = PERVAL ? vals.p[ch] : nullptr;
+    const int vstep = PERVAL ? vals.step[ch] : 0;
     __syncthreads();
     // the particle rounds, with the float or the double fixed-point conversion
     auto rounds = [&](auto fqc) {
@@ -559,16 +559,21 @@ static int deposit_run(pmx_ctx* c, float val, int perval, int nch, long long n, 
   const long long hstride = c->ntiles * tg.shell;
   int rc;
   if ((rc = pmx_ensure_halo(c, nch))) return rc;
+  // value channel ch of record slot i: vrec[ch * rec_cap + i]
+  ValSrc vs = {};
+  for (int ch = 0; ch < 3; ++ch) {
+    vs.p[ch] = c->vrec ? c->vrec + ch * c->rec_cap : nullptr;
+    vs.step[ch] = 1;
+  }
   pmx_tic(c, PMX_T_DEPOSIT);
   if (perval)
-    deposit_k<1><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, c->vrec, c->rec_cap, 1,
-                                                  c->vmax, c->tile_start, c->tile_count, g, tg,
-                                                  val, out, nch, mrs, accumulate, c->halo,
-                                                  hstride);
+    deposit_k<1><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, vs, c->vmax, c->tile_start,
+                                                  c->tile_count, g, tg, val, out, nch, mrs,
+                                                  accumulate, c->halo, hstride);
   else
-    deposit_k<0><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, nullptr, 0, 1, nullptr,
-                                                  c->tile_start, c->tile_count, g, tg, val, out,
-                                                  nch, mrs, accumulate, c->halo, hstride);
+    deposit_k<0><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, vs, nullptr, c->tile_start,
+                                                  c->tile_count, g, tg, val, out, nch, mrs,
+                                                  accumulate, c->halo, hstride);
   pmx_toc(c, PMX_T_DEPOSIT);
   PMX_LAUNCH_CHECK();
   if (defer && !accumulate && pmx_fold_deferrable(c)) return PMX_OK;
@@ -599,15 +604,15 @@ int pmx_deposit_binned3(pmx_ctx* c, long long n, float* const* mesh, int mrs, in
 int pmx_deposit_state(pmx_ctx* c, const PState& st, const uint32_t* start, const uint32_t* count,
                       float val, float* mesh, int mrs, int defer) {
   DepOut out = {{mesh, mesh, mesh}};
-  return pmx_deposit_state_vals(c, st, start, count, val, nullptr, 0, 0, 1, out, mrs, defer);
+  return pmx_deposit_state_vals(c, st, start, count, val, nullptr, 1, out, mrs, defer);
 }
 
 // Deposits tile-ordered particles into nch meshes (row stride mrs),
 // overwriting them: val each, or (vals) channel ch of slot i at
-// vals[ch * vstride + i * vstep] with the fixed-point scale from c->vmax.
+// vals->p[ch][i * vals->step[ch]] with the fixed-point scale from c->vmax.
 int pmx_deposit_state_vals(pmx_ctx* c, const PState& st, const uint32_t* start,
-                           const uint32_t* count, float val, const float* vals, long long vstride,
-                           int vstep, int nch, DepOut out, int mrs, int defer) {
+                           const uint32_t* count, float val, const ValSrc* vals, int nch,
+                           DepOut out, int mrs, int defer) {
   const MeshGeom g = pmx_geom_padded(c);
   const TileGeom& tg = c->tg;
   const size_t lds = (size_t)(tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(unsigned long long);
@@ -617,13 +622,11 @@ int pmx_deposit_state_vals(pmx_ctx* c, const PState& st, const uint32_t* start,
   if ((rc = pmx_ensure_halo(c, nch))) return rc;
   pmx_tic(c, PMX_T_DEPOSIT);
   if (vals)
-    deposit_k<1, 1><<<nt, DEP_TPB, lds, c->stream>>>(st.s0, vals, vstride, vstep, c->vmax,
-                                                     start, count, g, tg, 0.f, out, nch, mrs, 0,
-                                                     c->halo, hstride);
+    deposit_k<1, 1><<<nt, DEP_TPB, lds, c->stream>>>(st.s0, *vals, c->vmax, start, count, g, tg,
+                                                     0.f, out, nch, mrs, 0, c->halo, hstride);
   else
-    deposit_k<0, 1><<<nt, DEP_TPB, lds, c->stream>>>(st.s0, nullptr, 0, 1, nullptr, start,
-                                                     count, g, tg, val, out, nch, mrs, 0, c->halo,
-                                                     hstride);
+    deposit_k<0, 1><<<nt, DEP_TPB, lds, c->stream>>>(st.s0, ValSrc{}, nullptr, start, count, g,
+                                                     tg, val, out, nch, mrs, 0, c->halo, hstride);
   pmx_toc(c, PMX_T_DEPOSIT);
   PMX_LAUNCH_CHECK();
   if (defer && pmx_fold_deferrable(c)) return PMX_OK;

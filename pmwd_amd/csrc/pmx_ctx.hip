@@ -325,12 +325,12 @@ int pmx_ensure_adj_state(pmx_ctx* c, int nsteps) {
     dev_free(c, (void**)&a.p.s0, old * 16);
     dev_free(c, (void**)&a.p.s1, old * 16);
     dev_free(c, (void**)&a.c0, old * 16);
-    dev_free(c, (void**)&a.c1, old * 16);
+    dev_free(c, (void**)&a.c1, old * 8);
     c->al_cap = 0;
     rc = dev_alloc(c, (void**)&a.p.s0, cap * 16);
     rc |= dev_alloc(c, (void**)&a.p.s1, cap * 16);
     rc |= dev_alloc(c, (void**)&a.c0, cap * 16);
-    rc |= dev_alloc(c, (void**)&a.c1, cap * 16);
+    rc |= dev_alloc(c, (void**)&a.c1, cap * 8);
     if (rc) return PMX_ENOMEM;
     c->al_cap = cap;
   }
@@ -346,7 +346,7 @@ int pmx_ensure_adj_state(pmx_ctx* c, int nsteps) {
   }
   c->al[1].p = c->pl[1];
   c->al[1].c0 = c->pl[0].s0;
-  c->al[1].c1 = c->pl[0].s1;
+  c->al[1].c1 = reinterpret_cast<float2*>(c->pl[0].s1);
   return PMX_OK;
 }
 
@@ -356,7 +356,7 @@ static void free_adj_state(pmx_ctx* c) {
   dev_free(c, (void**)&a.p.s0, cap * 16);
   dev_free(c, (void**)&a.p.s1, cap * 16);
   dev_free(c, (void**)&a.c0, cap * 16);
-  dev_free(c, (void**)&a.c1, cap * 16);
+  dev_free(c, (void**)&a.c1, cap * 8);
   for (int b = 0; b < 3; ++b) dev_free(c, (void**)&c->abuf[b], (size_t)c->Mc * sizeof(float2));
   c->al_cap = 0;
   c->al[1] = AState{};

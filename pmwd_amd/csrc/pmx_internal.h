@@ -84,12 +84,19 @@ struct PState {
   float4* s1;
 };
 
-// The reverse-time pipeline's particle: a PState plus c0 = (disp_cot, 0)
-// and c1 = (vel_cot, 0).
+// The reverse-time pipeline's particle: a PState plus c0 = (disp_cot,
+// vel_cot.x) and c1 = (vel_cot.y, vel_cot.z).
 struct AState {
   PState p;
   float4* c0;
-  float4* c1;
+  float2* c1;
+};
+
+// Per-channel value sources of a deposit: channel ch of slot i is
+// p[ch][i * step[ch]].
+struct ValSrc {
+  const float* p[3];
+  int step[3];
 };
 
 // ---------------------------------------------------------- kernel timing --
@@ -843,8 +850,8 @@ struct DepOut {
   float* mesh[3];
 };
 int pmx_deposit_state_vals(pmx_ctx* c, const PState& st, const uint32_t* start,
-                           const uint32_t* count, float val, const float* vals, long long vstride,
-                           int vstep, int nch, DepOut out, int mrs, int defer = 0);
+                           const uint32_t* count, float val, const ValSrc* vals, int nch,
+                           DepOut out, int mrs, int defer = 0);
 // kick (k1) and drift (d) of the caller's arrays, binned into count/o (cap:
 // capacity mode)
 int pmx_kick_drift_bin(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, const float* acc,

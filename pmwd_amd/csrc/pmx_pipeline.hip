@@ -338,7 +338,7 @@ extern "C" int pmx_nbody_steps(pmx_ctx* c, const int16_t* pmid, float* disp, flo
 // products sum the same terms in another grouping.
 
 __device__ __forceinline__ void al_put(const AState& st, uint32_t slot, float4 s0, float4 s1,
-                                       float4 c0, float4 c1) {
+                                       float4 c0, float2 c1) {
   st.p.s0[slot] = s0;
   st.p.s1[slot] = s1;
   st.c0[slot] = c0;
@@ -375,8 +375,8 @@ __global__ void __launch_bounds__(PMX_TPB)
     al_put(st, start[ptile[i]] + prank[i],
            make_float4(e[0], e[1], e[2], __uint_as_float(pmx_state_pack(px, py, pz))),
            make_float4(e[T3], e[T3 + 1], e[T3 + 2], __uint_as_float((uint32_t)i)),
-           make_float4(e[2 * T3], e[2 * T3 + 1], e[2 * T3 + 2], 0.f),
-           make_float4(e[3 * T3], e[3 * T3 + 1], e[3 * T3 + 2], 0.f));
+           make_float4(e[2 * T3], e[2 * T3 + 1], e[2 * T3 + 2], e[3 * T3]),
+           make_float2(e[3 * T3 + 1], e[3 * T3 + 2]));
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a) wave_max_store(vm[a], vmax + a);
@@ -442,7 +442,8 @@ __global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4))
   double fsum = 0, dsum = 0;
   float vm[3] = {0.f, 0.f, 0.f};
   for (uint32_t q0 = 0; q0 < cnt; q0 += AL_TPB * AL_U) {
-    float4 a0[AL_U], a1[AL_U], c0[AL_U], c1[AL_U];
+    float4 a0[AL_U], a1[AL_U], c0[AL_U];
+    float2 c1[AL_U];
 #pragma unroll
     for (int u = 0; u < AL_U; ++u) {
       const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
@@ -454,7 +455,8 @@ __global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4))
         c1[u] = in.c1[sl];
       }
     }
-    float4 n0[AL_U], n1[AL_U], m0[AL_U], m1[AL_U];
+    float4 n0[AL_U], n1[AL_U], m0[AL_U];
+    float2 m1[AL_U];
     uint32_t nt[AL_U];
     int code[AL_U];
 #pragma unroll
@@ -467,7 +469,7 @@ __global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4))
       int l[3];
       float ac[3] = {0.f, 0.f, 0.f}, at[3] = {0.f, 0.f, 0.f}, gr[3] = {0.f, 0.f, 0.f};
       float x[3] = {a0[u].x, a0[u].y, a0[u].z}, v[3] = {a1[u].x, a1[u].y, a1[u].z};
-      float dc[3] = {c0[u].x, c0[u].y, c0[u].z}, vc[3] = {c1[u].x, c1[u].y, c1[u].z};
+      float dc[3] = {c0[u].x, c0[u].y, c0[u].z}, vc[3] = {c0[u].w, c1[u].x, c1[u].y};
       // binned by this tile, so always inside; the guard keeps LDS in bounds
       if (pmx_state_local(a0[u], g, o, s, d, l)) {
         pmx_vjp_one(sh, ltot, LY, LZ, d, l, vc, g.cell, ac, at);
@@ -516,8 +518,8 @@ __global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4))
       if (MODE == PL_CAP) code[u] = pl_nbr_code(ax, ti, tg);
       n0[u] = make_float4(x[0], x[1], x[2], a0[u].w);
       n1[u] = make_float4(v[0], v[1], v[2], a1[u].w);
-      m0[u] = make_float4(dc[0], dc[1], dc[2], 0.f);
-      m1[u] = make_float4(vc[0], vc[1], vc[2], 0.f);
+      m0[u] = make_float4(dc[0], dc[1], dc[2], vc[0]);
+      m1[u] = make_float2(vc[1], vc[2]);
     }
     if (MODE == PL_LAST) continue;
     if (MODE == PL_WRITE) {
@@ -661,9 +663,11 @@ extern "C" int pmx_nbody_steps_adj(pmx_ctx* c, const int16_t* pmid, float* disp,
     if ((rc = pmx_poisson_ctx(c, coef, 1))) return rc;
     // the cotangents of the three force meshes, then of the density (abuf[0])
     DepOut out3 = {{(float*)c->abuf[0], (float*)c->abuf[1], (float*)c->abuf[2]}};
-    if ((rc = pmx_deposit_state_vals(c, c->al[A].p, c->pl_start[A], c->pl_count[A], 0.f,
-                                     reinterpret_cast<const float*>(c->al[A].c1), 1, 4, 3, out3,
-                                     c->rs, 1)))
+    const float* cv0 = reinterpret_cast<const float*>(c->al[A].c0);
+    const float* cv1 = reinterpret_cast<const float*>(c->al[A].c1);
+    const ValSrc vc = {{cv0 + 3, cv1, cv1 + 1}, {4, 2, 2}};  // vel_cot x, y, z
+    if ((rc = pmx_deposit_state_vals(c, c->al[A].p, c->pl_start[A], c->pl_count[A], 0.f, &vc, 3,
+                                     out3, c->rs, 1)))
       return rc;
     if ((rc = pmx_poisson_adj_bufs(c, c->abuf, adj_scale, 1))) return rc;
     if (s + 1 == nsteps) {

@@ -90,7 +90,7 @@ def context_bytes(mesh_shape, n_ptcl, adjoint):
     and halo shells, and per record slot (about 1.2 n_ptcl slots) the record,
     index, tile and rank words, four value channels and the multi-step
     pipeline's own particle copy.  The reverse-time pipeline adds three
-    cotangent meshes, a 64-byte particle copy per slot and two more halo
+    cotangent meshes, a 56-byte particle copy per slot and two more halo
     channels."""
     n0, n1, n2 = mesh_shape
     n2p = n2 // 2 + 1
@@ -100,7 +100,7 @@ def context_bytes(mesh_shape, n_ptcl, adjoint):
     tile_bytes = tiles * (40 + 4 * 817 * (3 if adjoint else 1))
     cap = n_ptcl + n_ptcl // 16 + 1024
     slots = cap + cap // 8 + 33 * (tiles + 1)
-    per_slot = 16 + 4 + 8 + 16 + 32 + (64 if adjoint else 0)
+    per_slot = 16 + 4 + 8 + 16 + 32 + (56 if adjoint else 0)
     return int(mesh + tile_bytes + slots * per_slot + 64 * 2**20)
 
 
