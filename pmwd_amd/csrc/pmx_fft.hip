@@ -18,8 +18,13 @@
 #define FFT_MAXV (FFT_CAP / FFT_TPB) // values per thread
 #define RFFT_CAP 4096                // row passes: fewer values per thread
 #define RFFT_MAXV (RFFT_CAP / FFT_TPB)
-#define PFFT_TPB 512                 // Poisson x pass: 8 columns of 512 per WG,
-#ifndef PFFT_CAP                     // four such blocks (40 KB of LDS each) per CU
+// Poisson x pass: 8 columns of 512 per 512-thread workgroup, three such
+// workgroups (40 KB of LDS each) per CU; 16 columns per 1024 threads and 8
+// per 256 measured slower
+#ifndef PFFT_TPB
+#define PFFT_TPB 512
+#endif
+#ifndef PFFT_CAP
 #define PFFT_CAP 4096
 #endif
 #ifndef PFFT_WPE                     // waves per SIMD the register budget allows
