@@ -181,7 +181,7 @@ __device__ __forceinline__ double fixed_scale(float vmax) {
 // itself.
 #define DEP_TPB 512
 #ifndef DEP_U
-#define DEP_U 2
+#define DEP_U 1
 #endif
 // eight waves per SIMD (at most 64 VGPRs): four 39 KB workgroups per CU
 #ifndef DEP_WPE
@@ -282,11 +282,13 @@ __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DE
     const int vstep = PERVAL ? vals.step[ch] : 0;
     __syncthreads();
     // the particle rounds, with the float or the double fixed-point conversion
+    // (the next round's records are loaded before this round's arithmetic,
+    // so they are in flight meanwhile)
     auto rounds = [&](auto fqc) {
     constexpr bool FQ = decltype(fqc)::value;
-    for (uint32_t q0 = 0; q0 < cnt; q0 += DEP_TPB * DEP_U) {
-      float4 r[DEP_U];
-      float v[DEP_U];
+    float4 rn[DEP_U];
+    float vn[DEP_U];
+    auto fetch = [&](uint32_t q0, float4* r, float* v) {
 #pragma unroll
       for (int u = 0; u < DEP_U; ++u) {
         uint32_t q = q0 + u * DEP_TPB + threadIdx.x;
@@ -295,6 +297,17 @@ __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DE
           v[u] = PERVAL ? vr[(long long)(start + q) * vstep] : val;
         }
       }
+    };
+    fetch(0, rn, vn);
+    for (uint32_t q0 = 0; q0 < cnt; q0 += DEP_TPB * DEP_U) {
+      float4 r[DEP_U];
+      float v[DEP_U];
+#pragma unroll
+      for (int u = 0; u < DEP_U; ++u) {
+        r[u] = rn[u];
+        v[u] = vn[u];
+      }
+      if (q0 + DEP_TPB * DEP_U < cnt) fetch(q0 + DEP_TPB * DEP_U, rn, vn);
 #pragma unroll
       for (int u = 0; u < DEP_U; ++u) {
         uint32_t q = q0 + u * DEP_TPB + threadIdx.x;

@@ -441,9 +441,11 @@ __global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4))
   const uint32_t cnt = in_count[t];
   double fsum = 0, dsum = 0;
   float vm[3] = {0.f, 0.f, 0.f};
-  for (uint32_t q0 = 0; q0 < cnt; q0 += AL_TPB * AL_U) {
-    float4 a0[AL_U], a1[AL_U], c0[AL_U];
-    float2 c1[AL_U];
+  // the next round's particles are loaded before this round's arithmetic,
+  // so they are in flight meanwhile (5% on the 512^3 reverse step)
+  float4 pa0[AL_U], pa1[AL_U], pc0[AL_U];
+  float2 pc1[AL_U];
+  auto fetch = [&](uint32_t q0, float4* a0, float4* a1, float4* c0, float2* c1) {
 #pragma unroll
     for (int u = 0; u < AL_U; ++u) {
       const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
@@ -455,6 +457,19 @@ __global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4))
         c1[u] = in.c1[sl];
       }
     }
+  };
+  fetch(0, pa0, pa1, pc0, pc1);
+  for (uint32_t q0 = 0; q0 < cnt; q0 += AL_TPB * AL_U) {
+    float4 a0[AL_U], a1[AL_U], c0[AL_U];
+    float2 c1[AL_U];
+#pragma unroll
+    for (int u = 0; u < AL_U; ++u) {
+      a0[u] = pa0[u];
+      a1[u] = pa1[u];
+      c0[u] = pc0[u];
+      c1[u] = pc1[u];
+    }
+    if (q0 + AL_TPB * AL_U < cnt) fetch(q0 + AL_TPB * AL_U, pa0, pa1, pc0, pc1);
     float4 n0[AL_U], n1[AL_U], m0[AL_U];
     float2 m1[AL_U];
     uint32_t nt[AL_U];
