@@ -29,6 +29,9 @@
 #ifndef PL_MINB
 #define PL_MINB 1
 #endif
+// workgroups of the exact re-bin launches, which return at once unless a
+// tile overflowed
+#define PL_REBIN_GRID 2048u
 
 enum { PL_CAP = 0, PL_COUNT = 1, PL_WRITE = 2, PL_LAST = 3 };
 
@@ -116,118 +119,130 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
   // PL_CAP: runs per neighbour tile and round (pl_nbr_code, pl_round_bases),
   // so a round's stores into a tile are one run instead of one per wave
   __shared__ uint32_t s_cnt[27], s_base[27];
-  int o[3], s[3], ti[3];
-  const int t = (int)pmx_xcd_block(blockIdx.x, gridDim.x);
-  tile_origin(t, tg, g, o, s, ti);
-  const int LY = s[1] + 1, LZ = s[2] + 1;
-  const int ltot = (s[0] + 1) * LY * LZ;
-  const float* F[3] = {F0, F1, F2};
-  if (MODE == PL_CAP && threadIdx.x < 27) s_cnt[threadIdx.x] = 0u;
-  pmx_stage_tile<3, PL_TPB>(sh, o, LY, LZ, ltot, g, F);
-  __syncthreads();
-  const uint32_t start = in_start[t];
-  const uint32_t cnt = in_count[t];
-  for (uint32_t q0 = 0; q0 < cnt; q0 += PL_TPB * PL_U) {
-    float4 a0[PL_U], a1[PL_U], n0[PL_U], n1[PL_U];
-    uint32_t nt[PL_U];
-    int code[PL_U];
-#pragma unroll
-    for (int u = 0; u < PL_U; ++u) {
-      const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
-      if (q < cnt) {
-        a0[u] = in.s0[start + q];
-        a1[u] = in.s1[start + q];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < PL_U; ++u) {
-      const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
-      n0[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-      n1[u] = n0[u];
-      nt[u] = 0;
-      code[u] = 27;
-      if (q < cnt) {
-        float acc[3] = {0.f, 0.f, 0.f};
-        float d[3];
-        int l[3];
-        // binned by this tile, so always inside; the guard keeps LDS in bounds
-        if (pmx_state_local(a0[u], g, o, s, d, l))
-          pmx_tile_acc(sh, ltot, LY, LZ, d[0], d[1], d[2], l[0], l[1], l[2], acc);
-        float v[3] = {a1[u].x, a1[u].y, a1[u].z};
-        float x[3] = {a0[u].x, a0[u].y, a0[u].z};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) v[a] = v[a] + acc[a] * k2;
-        if (MODE == PL_LAST) {
-          const long long i = __float_as_uint(a1[u].w);
-#pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            user.acc[3 * i + a] = acc[a];
-            user.vel[3 * i + a] = v[a];
-            user.disp[3 * i + a] = x[a];
-          }
-        } else {
-#pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            v[a] = v[a] + acc[a] * k1n;
-            x[a] = x[a] + v[a] * dn;
-          }
-          int pm[3];
-          pmx_state_pm(a0[u], pm);
-          CicAxis ax[3];
-          cic_setup(pm, x, g, ax);
-          nt[u] = tile_of(ax, tg);
-          n0[u] = make_float4(x[0], x[1], x[2], a0[u].w);
-          n1[u] = make_float4(v[0], v[1], v[2], a1[u].w);
-          if (MODE == PL_CAP) code[u] = pl_nbr_code(ax, ti, tg);
-        }
-      }
-    }
-    if (MODE == PL_LAST) continue;
-    if (MODE == PL_WRITE) {
-#pragma unroll
+  // the exact re-bin (COUNT, WRITE: rare) walks the tiles with a small grid,
+  // so that it costs little when the overflow flag is clear
+  auto tile = [&](const int t) {
+    int o[3], s[3], ti[3];
+    tile_origin(t, tg, g, o, s, ti);
+    const int LY = s[1] + 1, LZ = s[2] + 1;
+    const int ltot = (s[0] + 1) * LY * LZ;
+    const float* F[3] = {F0, F1, F2};
+    if (MODE == PL_CAP && threadIdx.x < 27) s_cnt[threadIdx.x] = 0u;
+    pmx_stage_tile<3, PL_TPB>(sh, o, LY, LZ, ltot, g, F);
+    __syncthreads();
+    const uint32_t start = in_start[t];
+    const uint32_t cnt = in_count[t];
+    for (uint32_t q0 = 0; q0 < cnt; q0 += PL_TPB * PL_U) {
+      float4 a0[PL_U], a1[PL_U], n0[PL_U], n1[PL_U];
+      uint32_t nt[PL_U];
+      int code[PL_U];
+  #pragma unroll
       for (int u = 0; u < PL_U; ++u) {
         const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
-        if (q < cnt)
-          pl_put(out.st, out.start[out.ptile[start + q]] + out.prank[start + q], n0[u], n1[u]);
-      }
-      continue;
-    }
-    if (MODE == PL_COUNT) {
-#pragma unroll
-      for (int u = 0; u < PL_U; ++u) {
-        const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
-        const uint32_t r = agg_slot(out.count, nt[u], q < cnt);
         if (q < cnt) {
-          out.ptile[start + q] = nt[u];
-          out.prank[start + q] = r;
+          a0[u] = in.s0[start + q];
+          a1[u] = in.s1[start + q];
         }
       }
-      continue;
+  #pragma unroll
+      for (int u = 0; u < PL_U; ++u) {
+        const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
+        n0[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        n1[u] = n0[u];
+        nt[u] = 0;
+        code[u] = 27;
+        if (q < cnt) {
+          float acc[3] = {0.f, 0.f, 0.f};
+          float d[3];
+          int l[3];
+          // binned by this tile, so always inside; the guard keeps LDS in bounds
+          if (pmx_state_local(a0[u], g, o, s, d, l))
+            pmx_tile_acc(sh, ltot, LY, LZ, d[0], d[1], d[2], l[0], l[1], l[2], acc);
+          float v[3] = {a1[u].x, a1[u].y, a1[u].z};
+          float x[3] = {a0[u].x, a0[u].y, a0[u].z};
+  #pragma unroll
+          for (int a = 0; a < 3; ++a) v[a] = v[a] + acc[a] * k2;
+          if (MODE == PL_LAST) {
+            const long long i = __float_as_uint(a1[u].w);
+  #pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              user.acc[3 * i + a] = acc[a];
+              user.vel[3 * i + a] = v[a];
+              user.disp[3 * i + a] = x[a];
+            }
+          } else {
+  #pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              v[a] = v[a] + acc[a] * k1n;
+              x[a] = x[a] + v[a] * dn;
+            }
+            int pm[3];
+            pmx_state_pm(a0[u], pm);
+            CicAxis ax[3];
+            cic_setup(pm, x, g, ax);
+            nt[u] = tile_of(ax, tg);
+            n0[u] = make_float4(x[0], x[1], x[2], a0[u].w);
+            n1[u] = make_float4(v[0], v[1], v[2], a1[u].w);
+            if (MODE == PL_CAP) code[u] = pl_nbr_code(ax, ti, tg);
+          }
+        }
+      }
+      if (MODE == PL_LAST) continue;
+      if (MODE == PL_WRITE) {
+  #pragma unroll
+        for (int u = 0; u < PL_U; ++u) {
+          const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
+          if (q < cnt)
+            pl_put(out.st, out.start[out.ptile[start + q]] + out.prank[start + q], n0[u], n1[u]);
+        }
+        continue;
+      }
+      if (MODE == PL_COUNT) {
+  #pragma unroll
+        for (int u = 0; u < PL_U; ++u) {
+          const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
+          const uint32_t r = agg_slot(out.count, nt[u], q < cnt);
+          if (q < cnt) {
+            out.ptile[start + q] = nt[u];
+            out.prank[start + q] = r;
+          }
+        }
+        continue;
+      }
+      // PL_CAP: ranks within this round's run per neighbour tile, then the runs
+      uint32_t rk[PL_U];
+  #pragma unroll
+      for (int u = 0; u < PL_U; ++u) {
+        const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
+        const bool near = q < cnt && code[u] < 27;
+        const uint32_t rl = agg_slot(s_cnt, near ? (uint32_t)code[u] : 0u, near);
+        const uint32_t rf = agg_slot(out.count, nt[u], q < cnt && code[u] == 27);
+        rk[u] = near ? rl : rf;
+      }
+      __syncthreads();
+      pl_round_bases(s_cnt, s_base, ti, tg, out.count);
+      __syncthreads();
+  #pragma unroll
+      for (int u = 0; u < PL_U; ++u) {
+        const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
+        if (q >= cnt) continue;
+        const uint32_t r = code[u] < 27 ? s_base[code[u]] + rk[u] : rk[u];
+        const uint32_t b = out.start[nt[u]], e = out.start[nt[u] + 1];
+        if (r < e - b)
+          pl_put(out.st, b + r, n0[u], n1[u]);
+        else
+          out.flag[0] = 1u;
+      }
     }
-    // PL_CAP: ranks within this round's run per neighbour tile, then the runs
-    uint32_t rk[PL_U];
-#pragma unroll
-    for (int u = 0; u < PL_U; ++u) {
-      const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
-      const bool near = q < cnt && code[u] < 27;
-      const uint32_t rl = agg_slot(s_cnt, near ? (uint32_t)code[u] : 0u, near);
-      const uint32_t rf = agg_slot(out.count, nt[u], q < cnt && code[u] == 27);
-      rk[u] = near ? rl : rf;
+  };
+  if (MODE == PL_COUNT || MODE == PL_WRITE) {
+    const unsigned ntl = (unsigned)tg.ntx * tg.nty * tg.ntz;
+    for (unsigned tb = blockIdx.x; tb < ntl; tb += gridDim.x) {
+      tile((int)pmx_xcd_block(tb, ntl));
+      __syncthreads();
     }
-    __syncthreads();
-    pl_round_bases(s_cnt, s_base, ti, tg, out.count);
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < PL_U; ++u) {
-      const uint32_t q = q0 + u * PL_TPB + threadIdx.x;
-      if (q >= cnt) continue;
-      const uint32_t r = code[u] < 27 ? s_base[code[u]] + rk[u] : rk[u];
-      const uint32_t b = out.start[nt[u]], e = out.start[nt[u] + 1];
-      if (r < e - b)
-        pl_put(out.st, b + r, n0[u], n1[u]);
-      else
-        out.flag[0] = 1u;
-    }
+  } else {
+    tile((int)pmx_xcd_block(blockIdx.x, gridDim.x));
   }
 }
 
@@ -236,7 +251,9 @@ static void pl_launch(pmx_ctx* c, int src, PlOut out, PlUser user, float k2, flo
   const MeshGeom g = pmx_geom_padded(c);
   const TileGeom& tg = c->tg;
   const size_t lds = (size_t)3 * (tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(float);
-  pl_step_k<MODE><<<(unsigned)c->ntiles, PL_TPB, lds, c->stream>>>(
+  const bool rebin = MODE == PL_COUNT || MODE == PL_WRITE;
+  const unsigned grid = rebin && c->ntiles > PL_REBIN_GRID ? PL_REBIN_GRID : (unsigned)c->ntiles;
+  pl_step_k<MODE><<<grid, PL_TPB, lds, c->stream>>>(
       c->pl[src], c->pl_start[src], c->pl_count[src], g, tg, (const float*)c->cbuf[0],
       (const float*)c->cbuf[1], (const float*)c->cbuf[2], k2, k1n, dn, out, user);
 }
@@ -428,160 +445,170 @@ __global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4))
   if ((MODE == PL_COUNT || MODE == PL_WRITE) && out.flag[0] == 0u) return;
   extern __shared__ float sh[];
   __shared__ uint32_t s_cnt[27], s_base[27];
-  int o[3], s[3], ti[3];
-  const int t = (int)pmx_xcd_block(blockIdx.x, gridDim.x);
-  tile_origin(t, tg, g, o, s, ti);
-  const int LY = s[1] + 1, LZ = s[2] + 1;
-  const int ltot = (s[0] + 1) * LY * LZ;
-  const float* F[4] = {F0, F1, F2, R};
-  if (MODE == PL_CAP && threadIdx.x < 27) s_cnt[threadIdx.x] = 0u;
-  pmx_stage_tile<4, AL_TPB>(sh, o, LY, LZ, ltot, g, F);
-  __syncthreads();
-  const uint32_t start = in_start[t];
-  const uint32_t cnt = in_count[t];
   double fsum = 0, dsum = 0;
   float vm[3] = {0.f, 0.f, 0.f};
-  // the next round's particles are loaded before this round's arithmetic,
-  // so they are in flight meanwhile (5% on the 512^3 reverse step)
-  float4 pa0[AL_U], pa1[AL_U], pc0[AL_U];
-  float2 pc1[AL_U];
-  auto fetch = [&](uint32_t q0, float4* a0, float4* a1, float4* c0, float2* c1) {
+  auto tile = [&](const int t) {
+    int o[3], s[3], ti[3];
+    tile_origin(t, tg, g, o, s, ti);
+    const int LY = s[1] + 1, LZ = s[2] + 1;
+    const int ltot = (s[0] + 1) * LY * LZ;
+    const float* F[4] = {F0, F1, F2, R};
+    if (MODE == PL_CAP && threadIdx.x < 27) s_cnt[threadIdx.x] = 0u;
+    pmx_stage_tile<4, AL_TPB>(sh, o, LY, LZ, ltot, g, F);
+    __syncthreads();
+    const uint32_t start = in_start[t];
+    const uint32_t cnt = in_count[t];
+    // the next round's particles are loaded before this round's arithmetic,
+    // so they are in flight meanwhile (5% on the 512^3 reverse step)
+    float4 pa0[AL_U], pa1[AL_U], pc0[AL_U];
+    float2 pc1[AL_U];
+    auto fetch = [&](uint32_t q0, float4* a0, float4* a1, float4* c0, float2* c1) {
 #pragma unroll
-    for (int u = 0; u < AL_U; ++u) {
-      const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
-      if (q < cnt) {
-        const uint32_t sl = start + q;
-        a0[u] = in.p.s0[sl];
-        a1[u] = in.p.s1[sl];
-        c0[u] = in.c0[sl];
-        c1[u] = in.c1[sl];
+      for (int u = 0; u < AL_U; ++u) {
+        const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
+        if (q < cnt) {
+          const uint32_t sl = start + q;
+          a0[u] = in.p.s0[sl];
+          a1[u] = in.p.s1[sl];
+          c0[u] = in.c0[sl];
+          c1[u] = in.c1[sl];
+        }
       }
-    }
-  };
-  fetch(0, pa0, pa1, pc0, pc1);
-  for (uint32_t q0 = 0; q0 < cnt; q0 += AL_TPB * AL_U) {
-    float4 a0[AL_U], a1[AL_U], c0[AL_U];
-    float2 c1[AL_U];
+    };
+    fetch(0, pa0, pa1, pc0, pc1);
+    for (uint32_t q0 = 0; q0 < cnt; q0 += AL_TPB * AL_U) {
+      float4 a0[AL_U], a1[AL_U], c0[AL_U];
+      float2 c1[AL_U];
 #pragma unroll
-    for (int u = 0; u < AL_U; ++u) {
-      a0[u] = pa0[u];
-      a1[u] = pa1[u];
-      c0[u] = pc0[u];
-      c1[u] = pc1[u];
-    }
-    if (q0 + AL_TPB * AL_U < cnt) fetch(q0 + AL_TPB * AL_U, pa0, pa1, pc0, pc1);
-    float4 n0[AL_U], n1[AL_U], m0[AL_U];
-    float2 m1[AL_U];
-    uint32_t nt[AL_U];
-    int code[AL_U];
-#pragma unroll
-    for (int u = 0; u < AL_U; ++u) {
-      const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
-      nt[u] = 0;
-      code[u] = 27;
-      if (q >= cnt) continue;
-      float d[3];
-      int l[3];
-      float ac[3] = {0.f, 0.f, 0.f}, at[3] = {0.f, 0.f, 0.f}, gr[3] = {0.f, 0.f, 0.f};
-      float x[3] = {a0[u].x, a0[u].y, a0[u].z}, v[3] = {a1[u].x, a1[u].y, a1[u].z};
-      float dc[3] = {c0[u].x, c0[u].y, c0[u].z}, vc[3] = {c0[u].w, c1[u].x, c1[u].y};
-      // binned by this tile, so always inside; the guard keeps LDS in bounds
-      if (pmx_state_local(a0[u], g, o, s, d, l)) {
-        pmx_vjp_one(sh, ltot, LY, LZ, d, l, vc, g.cell, ac, at);
-        pmx_grad_one(sh + 3 * ltot, LY, LZ, d, l, val, g.cell, gr);
+      for (int u = 0; u < AL_U; ++u) {
+        a0[u] = pa0[u];
+        a1[u] = pa1[u];
+        c0[u] = pc0[u];
+        c1[u] = pc1[u];
       }
-      if (MODE == PL_CAP || MODE == PL_LAST)
-        fsum += (double)vc[0] * ac[0] + (double)vc[1] * ac[1] + (double)vc[2] * ac[2];
+      if (q0 + AL_TPB * AL_U < cnt) fetch(q0 + AL_TPB * AL_U, pa0, pa1, pc0, pc1);
+      float4 n0[AL_U], n1[AL_U], m0[AL_U];
+      float2 m1[AL_U];
+      uint32_t nt[AL_U];
+      int code[AL_U];
 #pragma unroll
-      for (int a = 0; a < 3; ++a) at[a] = at[a] + gr[a];
-      if (MODE == PL_LAST) {
-        const long long i = __float_as_uint(a1[u].w);
+      for (int u = 0; u < AL_U; ++u) {
+        const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
+        nt[u] = 0;
+        code[u] = 27;
+        if (q >= cnt) continue;
+        float d[3];
+        int l[3];
+        float ac[3] = {0.f, 0.f, 0.f}, at[3] = {0.f, 0.f, 0.f}, gr[3] = {0.f, 0.f, 0.f};
+        float x[3] = {a0[u].x, a0[u].y, a0[u].z}, v[3] = {a1[u].x, a1[u].y, a1[u].z};
+        float dc[3] = {c0[u].x, c0[u].y, c0[u].z}, vc[3] = {c0[u].w, c1[u].x, c1[u].y};
+        // binned by this tile, so always inside; the guard keeps LDS in bounds
+        if (pmx_state_local(a0[u], g, o, s, d, l)) {
+          pmx_vjp_one(sh, ltot, LY, LZ, d, l, vc, g.cell, ac, at);
+          pmx_grad_one(sh + 3 * ltot, LY, LZ, d, l, val, g.cell, gr);
+        }
+        if (MODE == PL_CAP || MODE == PL_LAST)
+          fsum += (double)vc[0] * ac[0] + (double)vc[1] * ac[1] + (double)vc[2] * ac[2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) at[a] = at[a] + gr[a];
+        if (MODE == PL_LAST) {
+          const long long i = __float_as_uint(a1[u].w);
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            user.disp[3 * i + a] = x[a];
+            user.vel[3 * i + a] = v[a];
+            user.disp_cot[3 * i + a] = dc[a];
+            user.vel_cot[3 * i + a] = vc[a];
+            user.acc[3 * i + a] = ac[a];
+            user.acc_cot[3 * i + a] = at[a];
+          }
+          continue;
+        }
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          user.disp[3 * i + a] = x[a];
-          user.vel[3 * i + a] = v[a];
-          user.disp_cot[3 * i + a] = dc[a];
-          user.vel_cot[3 * i + a] = vc[a];
-          user.acc[3 * i + a] = ac[a];
-          user.acc_cot[3 * i + a] = at[a];
+          if (kp != 0.f) {
+            v[a] = v[a] + ac[a] * kp;
+            dc[a] = dc[a] - at[a] * kp;
+          }
+          v[a] = v[a] + ac[a] * ka;
+          dc[a] = dc[a] - at[a] * ka;
+          x[a] = x[a] + v[a] * dd;
+          vc[a] = vc[a] - dc[a] * dd;
+          dsum += (double)dc[a] * v[a];
+          vm[a] = fmaxf(vm[a], fabsf(vc[a]));
+        }
+        if (snap) {
+          const long long i = __float_as_uint(a1[u].w);
+#pragma unroll
+          for (int a = 0; a < 3; ++a) x[a] = snap[3 * i + a];
+        }
+        int pm[3];
+        pmx_state_pm(a0[u], pm);
+        CicAxis ax[3];
+        cic_setup(pm, x, g, ax);
+        nt[u] = tile_of(ax, tg);
+        if (MODE == PL_CAP) code[u] = pl_nbr_code(ax, ti, tg);
+        n0[u] = make_float4(x[0], x[1], x[2], a0[u].w);
+        n1[u] = make_float4(v[0], v[1], v[2], a1[u].w);
+        m0[u] = make_float4(dc[0], dc[1], dc[2], vc[0]);
+        m1[u] = make_float2(vc[1], vc[2]);
+      }
+      if (MODE == PL_LAST) continue;
+      if (MODE == PL_WRITE) {
+#pragma unroll
+        for (int u = 0; u < AL_U; ++u) {
+          const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
+          if (q < cnt)
+            al_put(out.st, out.start[out.ptile[start + q]] + out.prank[start + q], n0[u], n1[u],
+                   m0[u], m1[u]);
         }
         continue;
       }
+      if (MODE == PL_COUNT) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        if (kp != 0.f) {
-          v[a] = v[a] + ac[a] * kp;
-          dc[a] = dc[a] - at[a] * kp;
+        for (int u = 0; u < AL_U; ++u) {
+          const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
+          const uint32_t r = agg_slot(out.count, nt[u], q < cnt);
+          if (q < cnt) {
+            out.ptile[start + q] = nt[u];
+            out.prank[start + q] = r;
+          }
         }
-        v[a] = v[a] + ac[a] * ka;
-        dc[a] = dc[a] - at[a] * ka;
-        x[a] = x[a] + v[a] * dd;
-        vc[a] = vc[a] - dc[a] * dd;
-        dsum += (double)dc[a] * v[a];
-        vm[a] = fmaxf(vm[a], fabsf(vc[a]));
+        continue;
       }
-      if (snap) {
-        const long long i = __float_as_uint(a1[u].w);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) x[a] = snap[3 * i + a];
-      }
-      int pm[3];
-      pmx_state_pm(a0[u], pm);
-      CicAxis ax[3];
-      cic_setup(pm, x, g, ax);
-      nt[u] = tile_of(ax, tg);
-      if (MODE == PL_CAP) code[u] = pl_nbr_code(ax, ti, tg);
-      n0[u] = make_float4(x[0], x[1], x[2], a0[u].w);
-      n1[u] = make_float4(v[0], v[1], v[2], a1[u].w);
-      m0[u] = make_float4(dc[0], dc[1], dc[2], vc[0]);
-      m1[u] = make_float2(vc[1], vc[2]);
-    }
-    if (MODE == PL_LAST) continue;
-    if (MODE == PL_WRITE) {
+      uint32_t rk[AL_U];
 #pragma unroll
       for (int u = 0; u < AL_U; ++u) {
         const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
-        if (q < cnt)
-          al_put(out.st, out.start[out.ptile[start + q]] + out.prank[start + q], n0[u], n1[u],
-                 m0[u], m1[u]);
+        const bool near = q < cnt && code[u] < 27;
+        const uint32_t rl = agg_slot(s_cnt, near ? (uint32_t)code[u] : 0u, near);
+        const uint32_t rf = agg_slot(out.count, nt[u], q < cnt && code[u] == 27);
+        rk[u] = near ? rl : rf;
       }
-      continue;
-    }
-    if (MODE == PL_COUNT) {
+      __syncthreads();
+      pl_round_bases(s_cnt, s_base, ti, tg, out.count);
+      __syncthreads();
 #pragma unroll
       for (int u = 0; u < AL_U; ++u) {
         const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
-        const uint32_t r = agg_slot(out.count, nt[u], q < cnt);
-        if (q < cnt) {
-          out.ptile[start + q] = nt[u];
-          out.prank[start + q] = r;
-        }
+        if (q >= cnt) continue;
+        const uint32_t r = code[u] < 27 ? s_base[code[u]] + rk[u] : rk[u];
+        const uint32_t b = out.start[nt[u]], e = out.start[nt[u] + 1];
+        if (r < e - b)
+          al_put(out.st, b + r, n0[u], n1[u], m0[u], m1[u]);
+        else
+          out.flag[0] = 1u;
       }
-      continue;
     }
-    uint32_t rk[AL_U];
-#pragma unroll
-    for (int u = 0; u < AL_U; ++u) {
-      const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
-      const bool near = q < cnt && code[u] < 27;
-      const uint32_t rl = agg_slot(s_cnt, near ? (uint32_t)code[u] : 0u, near);
-      const uint32_t rf = agg_slot(out.count, nt[u], q < cnt && code[u] == 27);
-      rk[u] = near ? rl : rf;
+  };
+  if (MODE == PL_COUNT || MODE == PL_WRITE) {
+    const unsigned ntl = (unsigned)tg.ntx * tg.nty * tg.ntz;
+    for (unsigned tb = blockIdx.x; tb < ntl; tb += gridDim.x) {
+      tile((int)pmx_xcd_block(tb, ntl));
+      __syncthreads();
     }
-    __syncthreads();
-    pl_round_bases(s_cnt, s_base, ti, tg, out.count);
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < AL_U; ++u) {
-      const uint32_t q = q0 + u * AL_TPB + threadIdx.x;
-      if (q >= cnt) continue;
-      const uint32_t r = code[u] < 27 ? s_base[code[u]] + rk[u] : rk[u];
-      const uint32_t b = out.start[nt[u]], e = out.start[nt[u] + 1];
-      if (r < e - b)
-        al_put(out.st, b + r, n0[u], n1[u], m0[u], m1[u]);
-      else
-        out.flag[0] = 1u;
-    }
+  } else {
+    tile((int)pmx_xcd_block(blockIdx.x, gridDim.x));
   }
   if (MODE == PL_CAP) {
 #pragma unroll
@@ -597,7 +624,9 @@ static void al_launch(pmx_ctx* c, int src, float val, float kp, float ka, float 
   const MeshGeom g = pmx_geom_padded(c);
   const TileGeom& tg = c->tg;
   const size_t lds = (size_t)4 * (tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(float);
-  al_step_k<MODE><<<(unsigned)c->ntiles, AL_TPB, lds, c->stream>>>(
+  const bool rebin = MODE == PL_COUNT || MODE == PL_WRITE;
+  const unsigned grid = rebin && c->ntiles > PL_REBIN_GRID ? PL_REBIN_GRID : (unsigned)c->ntiles;
+  al_step_k<MODE><<<grid, AL_TPB, lds, c->stream>>>(
       c->al[src], c->pl_start[src], c->pl_count[src], g, tg, (const float*)c->cbuf[0],
       (const float*)c->cbuf[1], (const float*)c->cbuf[2], (const float*)c->abuf[0], val, kp, ka,
       dd, snap, out, user, c->vmax, c->red, c->red + c->ntiles);
