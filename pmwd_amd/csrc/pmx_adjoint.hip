@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(ADJ_TPB)
       const float d[3] = {r.x, r.y, r.z};
       const float v0 = vc[u][0], v1 = vc[u][1], v2 = vc[u][2];
       float o[3], ot[3];
-      pmx_vjp_one(sh, ltot, LY, LZ, d, l, vc[u], g.cell, o, ot);
+      pmx_vjp_one(sh, ltot, LY, LZ, pmx_frac_of_d(d), l, vc[u], g.cell, g.inv_cell, o, ot);
       const float o0 = o[0], o1 = o[1], o2 = o[2];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(ADJ_TPB)
       if (l[0] >= s[0] || l[1] >= s[1] || l[2] >= s[2]) continue;
       const float d[3] = {r.x, r.y, r.z};
       float gr[3];
-      pmx_grad_one(sh, LY, LZ, d, l, val, g.cell, gr);
+      pmx_grad_one(sh, LY, LZ, pmx_frac_of_d(d), l, val, g.cell, g.inv_cell, gr);
 #pragma unroll
       for (int a = 0; a < 3; ++a) acc_cot[3 * p + a] = ac[u][a] + gr[a];
     }

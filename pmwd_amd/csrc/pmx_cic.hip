@@ -194,12 +194,10 @@ __device__ __forceinline__ double fixed_scale(float vmax) {
 // and |x * sf| < 2^30, so rintf and an int32 conversion give the value
 // rint((double)x * scale) gives, in fewer instructions.
 template <bool FQ>
-__device__ __forceinline__ void dep_one(unsigned long long* acc, float dx, float dy, float dz,
-                                        int b, int LY, int LZ, float vv, double scale, float sf) {
-  float a0, a1, b0, b1, c0, c1;
-  frac_of(dx, a0, a1);
-  frac_of(dy, b0, b1);
-  frac_of(dz, c0, c1);
+__device__ __forceinline__ void dep_one(unsigned long long* acc, const CicFrac& fr, int b, int LY,
+                                        int LZ, float vv, double scale, float sf) {
+  const float a0 = fr.w[0][0], a1 = fr.w[0][1], b0 = fr.w[1][0], b1 = fr.w[1][1];
+  const float c0 = fr.w[2][0], c1 = fr.w[2][1];
   const int sxy = __umul24(LY, LZ);
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
@@ -313,23 +311,21 @@ __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DE
         uint32_t q = q0 + u * DEP_TPB + threadIdx.x;
         if (q >= cnt) continue;
         int lx, ly, lz;
+        CicFrac fr;
         if (STATE) {
-          float d[3];
           int l[3];
           // binned by this tile, so always inside; the guard keeps LDS in bounds
-          if (!pmx_state_local(r[u], g, o, s, d, l)) continue;
-          r[u].x = d[0];
-          r[u].y = d[1];
-          r[u].z = d[2];
+          if (!pmx_state_local(r[u], g, o, s, fr, l)) continue;
           lx = l[0];
           ly = l[1];
           lz = l[2];
         } else {
           pmx_unpack_local(r[u].w, lx, ly, lz);
           if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
+          const float d[3] = {r[u].x, r[u].y, r[u].z};
+          fr = pmx_frac_of_d(d);
         }
-        dep_one<FQ>(acc, r[u].x, r[u].y, r[u].z, __umul24(__umul24(lx, LY) + ly, LZ) + lz, LY,
-                    LZ, v[u], scale, sf);
+        dep_one<FQ>(acc, fr, __umul24(__umul24(lx, LY) + ly, LZ) + lz, LY, LZ, v[u], scale, sf);
       }
     }
     };
@@ -401,7 +397,8 @@ __global__ void __launch_bounds__(TG_TPB)
       pmx_unpack_local(r[u].w, lx, ly, lz);
       if (lx >= s[0] || ly >= s[1] || lz >= s[2]) continue;
       float o3[3];
-      pmx_tile_acc(sh, ltot, LY, LZ, r[u].x, r[u].y, r[u].z, lx, ly, lz, o3);
+      const float d[3] = {r[u].x, r[u].y, r[u].z};
+      pmx_tile_acc(sh, ltot, LY, LZ, pmx_frac_of_d(d), lx, ly, lz, o3);
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         acc[3 * p + a] = o3[a];

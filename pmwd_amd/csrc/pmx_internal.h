@@ -267,6 +267,11 @@ struct CicAxis {
   float d;             // disp / cell
 };
 
+// x / cell, as a product when inv_cell = 1/cell is exact (inv_cell != 0)
+__device__ __forceinline__ float pmx_div_cell(float x, float cell, float inv_cell) {
+  return inv_cell != 0.f ? x * inv_cell : x / cell;
+}
+
 __device__ __forceinline__ float pmx_sign(float x) {
   return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f);
 }
@@ -274,7 +279,7 @@ __device__ __forceinline__ float pmx_sign(float x) {
 __device__ __forceinline__ CicAxis pmx_cic_axis(int pm, float dp, float cell, float inv_cell,
                                                 int n) {
   CicAxis a;
-  float d = inv_cell != 0.f ? dp * inv_cell : dp / cell;
+  float d = pmx_div_cell(dp, cell, inv_cell);
   a.d = d;
   float fl = floorf(d);
   int i = (int)fl;
@@ -489,13 +494,6 @@ __device__ __forceinline__ void pmx_stage_pmid(int16_t* sp, const int16_t* pmid,
   }
 }
 
-// per-axis fractions from d = disp/cell exactly as pmx_cic_axis forms them
-__device__ __forceinline__ void frac_of(float d, float& w0, float& w1) {
-  int i = (int)floorf(d);
-  w0 = 1.f - fabsf(d - (float)i);
-  w1 = 1.f - fabsf(d - (float)(i + 1));
-}
-
 __device__ __forceinline__ void tile_origin(int t, const TileGeom& tg, const MeshGeom& g, int o[3],
                                             int s[3], int ti[3]) {
   ti[2] = t % tg.ntz;
@@ -534,7 +532,8 @@ __device__ __forceinline__ float pmx_fold_sum16(const float* __restrict__ halo, 
   return sum;
 }
 
-// fractions and the sign(-d2) factors of their gradients (pmx_cic_axis)
+// fractions and the sign(-d2) factors of their gradients from d = disp/cell,
+// exactly as pmx_cic_axis forms them
 __device__ __forceinline__ void frac_sign_of(float d, float& w0, float& w1, float& s0, float& s1) {
   int i = (int)floorf(d);
   float d20 = d - (float)i;
@@ -543,6 +542,34 @@ __device__ __forceinline__ void frac_sign_of(float d, float& w0, float& w1, floa
   w1 = 1.f - fabsf(d21);
   s0 = pmx_sign(-d20);
   s1 = pmx_sign(-d21);
+}
+
+// Per-axis fractions of the nb = 0 / 1 neighbours, w[a][0] and w[a][1], and
+// the sign(-d2) factors of their gradients, sg[a][*] (pmx_cic_axis).  Built
+// from d (records) or taken from the axes cic_setup already formed, with
+// the same values; fields a caller does not read are not computed.
+struct CicFrac {
+  float w[3][2];
+  float sg[3][2];
+};
+
+__device__ __forceinline__ CicFrac pmx_frac_of_d(const float d[3]) {
+  CicFrac f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) frac_sign_of(d[a], f.w[a][0], f.w[a][1], f.sg[a][0], f.sg[a][1]);
+  return f;
+}
+
+__device__ __forceinline__ CicFrac pmx_frac_of_axes(const CicAxis ax[3]) {
+  CicFrac f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    f.w[a][0] = ax[a].w0;
+    f.w[a][1] = ax[a].w1;
+    f.sg[a][0] = ax[a].s0;
+    f.sg[a][1] = ax[a].s1;
+  }
+  return f;
 }
 
 // Largest tile edge of the scatter binning, and the cells of a tile plus
@@ -594,12 +621,10 @@ __device__ __forceinline__ void pmx_stage_tile(float* sh, const int o[3], int LY
 // channels ltot floats apart, rows LY x LZ), summed in the tree order of
 // pmx_tree8 -- the arithmetic of a per-particle gather.
 __device__ __forceinline__ void pmx_tile_acc(const float* sh, int ltot, int LY, int LZ,
-                                             float dx, float dy, float dz, int lx, int ly, int lz,
+                                             const CicFrac& fr, int lx, int ly, int lz,
                                              float out[3]) {
-  float a0, a1, b0, b1, c0, c1;
-  frac_of(dx, a0, a1);
-  frac_of(dy, b0, b1);
-  frac_of(dz, c0, c1);
+  const float a0 = fr.w[0][0], a1 = fr.w[0][1], b0 = fr.w[1][0], b1 = fr.w[1][1];
+  const float c0 = fr.w[2][0], c1 = fr.w[2][1];
   const int b = __umul24(__umul24(lx, LY) + ly, LZ) + lz;
   float v0[8], v1[8], v2[8];
 #pragma unroll
@@ -623,12 +648,15 @@ __device__ __forceinline__ void pmx_tile_acc(const float* sh, int ltot, int LY, 
 // acot = sum_j (v . F[c_j]) grad w_j / cell, v = vel_cot, each in the tree
 // order of pmx_tree8.
 __device__ __forceinline__ void pmx_vjp_one(const float* sh, int ltot, int LY, int LZ,
-                                            const float d[3], const int l[3], const float v[3],
-                                            float cell, float acc[3], float acot[3]) {
-  float wa[2], wb[2], wc[2], sa[2], sb[2], sc[2];
-  frac_sign_of(d[0], wa[0], wa[1], sa[0], sa[1]);
-  frac_sign_of(d[1], wb[0], wb[1], sb[0], sb[1]);
-  frac_sign_of(d[2], wc[0], wc[1], sc[0], sc[1]);
+                                            const CicFrac& fr, const int l[3], const float v[3],
+                                            float cell, float inv_cell, float acc[3],
+                                            float acot[3]) {
+  const float* wa = fr.w[0];
+  const float* wb = fr.w[1];
+  const float* wc = fr.w[2];
+  const float* sa = fr.sg[0];
+  const float* sb = fr.sg[1];
+  const float* sc = fr.sg[2];
   const int b = __umul24(__umul24(l[0], LY) + l[1], LZ) + l[2];
   float r0[8], r1[8], r2[8], gx[8], gy[8], gz[8];
 #pragma unroll
@@ -649,21 +677,23 @@ __device__ __forceinline__ void pmx_vjp_one(const float* sh, int ltot, int LY, i
   acc[0] = pmx_tree8(r0);
   acc[1] = pmx_tree8(r1);
   acc[2] = pmx_tree8(r2);
-  acot[0] = pmx_tree8(gx) / cell;
-  acot[1] = pmx_tree8(gy) / cell;
-  acot[2] = pmx_tree8(gz) / cell;
+  acot[0] = pmx_div_cell(pmx_tree8(gx), cell, inv_cell);
+  acot[1] = pmx_div_cell(pmx_tree8(gy), cell, inv_cell);
+  acot[2] = pmx_div_cell(pmx_tree8(gz), cell, inv_cell);
 }
 
 // Scatter-adjoint disp cotangent of one binned particle over a staged tile
 // of the density cotangent R (scatter.py:86-121, scalar val):
 // sum_j (R[c_j] val) grad w_j / cell per axis, tree order.
-__device__ __forceinline__ void pmx_grad_one(const float* sh, int LY, int LZ, const float d[3],
+__device__ __forceinline__ void pmx_grad_one(const float* sh, int LY, int LZ, const CicFrac& fr,
                                              const int l[3], float val, float cell,
-                                             float out[3]) {
-  float wa[2], wb[2], wc[2], sa[2], sb[2], sc[2];
-  frac_sign_of(d[0], wa[0], wa[1], sa[0], sa[1]);
-  frac_sign_of(d[1], wb[0], wb[1], sb[0], sb[1]);
-  frac_sign_of(d[2], wc[0], wc[1], sc[0], sc[1]);
+                                             float inv_cell, float out[3]) {
+  const float* wa = fr.w[0];
+  const float* wb = fr.w[1];
+  const float* wc = fr.w[2];
+  const float* sa = fr.sg[0];
+  const float* sb = fr.sg[1];
+  const float* sc = fr.sg[2];
   const int b = __umul24(__umul24(l[0], LY) + l[1], LZ) + l[2];
   float gx[8], gy[8], gz[8];
 #pragma unroll
@@ -676,9 +706,9 @@ __device__ __forceinline__ void pmx_grad_one(const float* sh, int LY, int LZ, co
     gy[j] = dc * (sb[jb] * (f2 * f0));
     gz[j] = dc * (sc[jc] * (f0 * f1));
   }
-  out[0] = pmx_tree8(gx) / cell;
-  out[1] = pmx_tree8(gy) / cell;
-  out[2] = pmx_tree8(gz) / cell;
+  out[0] = pmx_div_cell(pmx_tree8(gx), cell, inv_cell);
+  out[1] = pmx_div_cell(pmx_tree8(gy), cell, inv_cell);
+  out[2] = pmx_div_cell(pmx_tree8(gz), cell, inv_cell);
 }
 
 // Fixed-order block sum of s (wave butterflies, then wave sums in order)
@@ -745,21 +775,19 @@ __device__ __forceinline__ void pmx_state_pm(float4 s0, int pm[3]) {
   pm[2] = (int)(b >> 20);
 }
 
-// d = disp/cell and the lower corner relative to tile origin o of a state
-// entry (the values pmx_make_record packs); false when the corner is not in
-// the tile of extent s.
+// CIC fractions and the lower corner relative to tile origin o of a state
+// entry (from the d = disp/cell and corner pmx_make_record packs); false
+// when the corner is not in the tile of extent s.
 __device__ __forceinline__ bool pmx_state_local(float4 s0, const MeshGeom& g, const int o[3],
-                                                const int s[3], float d[3], int l[3]) {
+                                                const int s[3], CicFrac& fr, int l[3]) {
   int pm[3];
   pmx_state_pm(s0, pm);
   const float dp[3] = {s0.x, s0.y, s0.z};
   CicAxis ax[3];
   cic_setup(pm, dp, g, ax);
+  fr = pmx_frac_of_axes(ax);
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    d[a] = ax[a].d;
-    l[a] = ax[a].lo - o[a];
-  }
+  for (int a = 0; a < 3; ++a) l[a] = ax[a].lo - o[a];
   return (unsigned)l[0] < (unsigned)s[0] && (unsigned)l[1] < (unsigned)s[1] &&
          (unsigned)l[2] < (unsigned)s[2];
 }

@@ -153,11 +153,11 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
         code[u] = 27;
         if (q < cnt) {
           float acc[3] = {0.f, 0.f, 0.f};
-          float d[3];
+          CicFrac fr;
           int l[3];
           // binned by this tile, so always inside; the guard keeps LDS in bounds
-          if (pmx_state_local(a0[u], g, o, s, d, l))
-            pmx_tile_acc(sh, ltot, LY, LZ, d[0], d[1], d[2], l[0], l[1], l[2], acc);
+          if (pmx_state_local(a0[u], g, o, s, fr, l))
+            pmx_tile_acc(sh, ltot, LY, LZ, fr, l[0], l[1], l[2], acc);
           float v[3] = {a1[u].x, a1[u].y, a1[u].z};
           float x[3] = {a0[u].x, a0[u].y, a0[u].z};
   #pragma unroll
@@ -497,15 +497,15 @@ __global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4))
         nt[u] = 0;
         code[u] = 27;
         if (q >= cnt) continue;
-        float d[3];
+        CicFrac fr;
         int l[3];
         float ac[3] = {0.f, 0.f, 0.f}, at[3] = {0.f, 0.f, 0.f}, gr[3] = {0.f, 0.f, 0.f};
         float x[3] = {a0[u].x, a0[u].y, a0[u].z}, v[3] = {a1[u].x, a1[u].y, a1[u].z};
         float dc[3] = {c0[u].x, c0[u].y, c0[u].z}, vc[3] = {c0[u].w, c1[u].x, c1[u].y};
         // binned by this tile, so always inside; the guard keeps LDS in bounds
-        if (pmx_state_local(a0[u], g, o, s, d, l)) {
-          pmx_vjp_one(sh, ltot, LY, LZ, d, l, vc, g.cell, ac, at);
-          pmx_grad_one(sh + 3 * ltot, LY, LZ, d, l, val, g.cell, gr);
+        if (pmx_state_local(a0[u], g, o, s, fr, l)) {
+          pmx_vjp_one(sh, ltot, LY, LZ, fr, l, vc, g.cell, g.inv_cell, ac, at);
+          pmx_grad_one(sh + 3 * ltot, LY, LZ, fr, l, val, g.cell, g.inv_cell, gr);
         }
         if (MODE == PL_CAP || MODE == PL_LAST)
           fsum += (double)vc[0] * ac[0] + (double)vc[1] * ac[1] + (double)vc[2] * ac[2];
