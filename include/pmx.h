@@ -64,6 +64,19 @@ long long pmx_ctx_bytes(const pmx_ctx* ctx);
 #define PMX_TRIM_ALL 2
 int pmx_ctx_trim(pmx_ctx* ctx, int what);
 
+/* Device allocator for contexts created after the call (both NULL: hipMalloc
+ * and hipFree, the default).  alloc returns a block of `bytes` on `device`
+ * for use on `stream` (the context's stream then), or NULL, which the
+ * calling entry point reports as PMX_ENOMEM; release takes it back (bytes is
+ * 0 when a context is destroyed).  The context's stream is synchronised
+ * before a block is taken or handed back.  No counterpart in pmwd, where
+ * XLA owns every buffer: for hosts with a caching allocator of their own
+ * (torch's), so that memory a context gives back serves the host's arrays
+ * and the reverse without new device mappings.                            */
+typedef void* (*pmx_alloc_fn)(size_t bytes, int device, void* stream, void* user);
+typedef void (*pmx_free_fn)(void* ptr, size_t bytes, int device, void* stream, void* user);
+int pmx_set_allocator(pmx_alloc_fn alloc, pmx_free_fn release, void* user);
+
 /* ------------------------------------------------------- cell keys/sort ---
  * key[i] = C-order raveled index (strides of Particles.raveled_id,
  * pmwd/particles.py:156-182) of particle i's lower CIC corner, computed

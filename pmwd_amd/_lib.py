@@ -22,7 +22,14 @@ c_int, c_ll, c_f, c_d, c_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_float, ct
 
 
 class PmxError(RuntimeError):
-    """Error returned by the pmx C ABI."""
+    """Error returned by the pmx C ABI; ``rc`` is its pmx_status code."""
+
+    def __init__(self, msg, rc=None):
+        super().__init__(msg)
+        self.rc = rc
+
+
+ENOMEM = 2  # PMX_ENOMEM
 
 
 class CicGeom(ctypes.Structure):
@@ -32,6 +39,33 @@ class CicGeom(ctypes.Structure):
 
 
 # name -> argtypes (all return int status unless listed in _RESTYPE)
+# pmx_alloc_fn / pmx_free_fn (include/pmx.h)
+_ALLOC_T = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p,
+                            ctypes.c_void_p)
+_FREE_T = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p,
+                           ctypes.c_void_p)
+
+
+@_ALLOC_T
+def _torch_alloc(nbytes, device, stream, user):
+    # a block of torch's caching allocator for the context's stream; NULL
+    # (PMX_ENOMEM in the library) when torch has no room
+    try:
+        import torch
+        return torch.cuda.caching_allocator_alloc(int(nbytes), int(device), int(stream or 0))
+    except Exception:
+        return None
+
+
+@_FREE_T
+def _torch_free(ptr, nbytes, device, stream, user):
+    try:
+        import torch
+        torch.cuda.caching_allocator_delete(int(ptr))
+    except Exception:
+        pass
+
+
 _SIGS = {
     'pmx_version': [],
     'pmx_last_error': [],
@@ -40,6 +74,7 @@ _SIGS = {
     'pmx_ctx_set_stream': [c_p, c_p],
     'pmx_ctx_bytes': [c_p],
     'pmx_ctx_trim': [c_p, c_int],
+    'pmx_set_allocator': [_ALLOC_T, _FREE_T, c_p],
     'pmx_cell_keys': [c_p, c_p, c_p, c_ll, c_p],
     'pmx_sort_perm': [c_p, c_p, c_ll, c_p],
     'pmx_tile_keys': [c_p, c_p, c_p, c_ll, c_p],
@@ -99,6 +134,13 @@ def lib():
                 fn = getattr(L, name)
                 fn.argtypes = args
                 fn.restype = _RESTYPE.get(name, c_int)
+            # contexts take their device memory from torch's caching
+            # allocator, so that what one gives back serves the other without
+            # new device mappings (PMX_TORCH_ALLOCATOR=0: hipMalloc)
+            if os.environ.get('PMX_TORCH_ALLOCATOR', '1') != '0':
+                check_rc = L.pmx_set_allocator(_torch_alloc, _torch_free, None)
+                if check_rc != 0:
+                    raise PmxError(f'pmx_set_allocator failed ({check_rc})', check_rc)
             _lib = L
     return _lib
 
@@ -110,7 +152,7 @@ def exported_symbols():
 def check(rc):
     if rc != 0:
         msg = lib().pmx_last_error().decode(errors='replace')
-        raise PmxError(f'pmx error {rc}: {msg}')
+        raise PmxError(f'pmx error {rc}: {msg}', rc)
 
 
 def ptr(t):

@@ -21,12 +21,48 @@ void pmx_set_error(const char* fmt, ...) {
 extern "C" const char* pmx_last_error(void) { return g_err; }
 extern "C" int pmx_version(void) { return 100; }
 
+// Device allocator for contexts created from now on (both NULL: hipMalloc).
+static pmx_alloc_fn g_dalloc = nullptr;
+static pmx_free_fn g_dfree = nullptr;
+static void* g_dalloc_user = nullptr;
+
+extern "C" int pmx_set_allocator(pmx_alloc_fn alloc, pmx_free_fn release, void* user) {
+  PMX_REQUIRE((alloc == nullptr) == (release == nullptr),
+              "pmx_set_allocator: give both functions or neither");
+  g_dalloc = alloc;
+  g_dfree = release;
+  g_dalloc_user = user;
+  return PMX_OK;
+}
+
+// With a caller's allocator the context's stream is drained before a block
+// is taken or handed back, so a block changes owner only when no queued
+// work on the stream can still touch it.
 static int dev_alloc(pmx_ctx* c, void** p, size_t bytes) {
   if (bytes == 0) bytes = 256;
+  if (c->dalloc) {
+    *p = c->dalloc(bytes, c->device, (void*)c->stream, c->dalloc_user);
+    if (!*p) {
+      pmx_set_error("device allocation of %zu bytes failed (caller's allocator)", bytes);
+      return PMX_ENOMEM;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+      (void)hipGetLastError();
+      c->dfree(*p, bytes, c->device, (void*)c->stream, c->dalloc_user);
+      *p = nullptr;
+      pmx_set_error("device allocation: stream synchronisation failed");
+      return PMX_EHIP;
+    }
+    c->bytes += bytes;
+    return PMX_OK;
+  }
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
     pmx_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
     *p = nullptr;
+    // the failure is reported here; clear it so that the next launch check
+    // after a caller's fallback does not see it
+    (void)hipGetLastError();
     return PMX_ENOMEM;
   }
   c->bytes += bytes;
@@ -35,11 +71,19 @@ static int dev_alloc(pmx_ctx* c, void** p, size_t bytes) {
 
 static void dev_free(pmx_ctx* c, void** p, size_t bytes) {
   if (*p) {
-    hipFree(*p);
+    if (c->dfree) {
+      (void)hipStreamSynchronize(c->stream);
+      c->dfree(*p, bytes, c->device, (void*)c->stream, c->dalloc_user);
+    } else {
+      hipFree(*p);
+    }
     c->bytes -= bytes;
   }
   *p = nullptr;
 }
+
+int pmx_dev_alloc(pmx_ctx* c, void** p, size_t bytes) { return dev_alloc(c, p, bytes); }
+void pmx_dev_free(pmx_ctx* c, void** p, size_t bytes) { dev_free(c, p, bytes); }
 
 static void free_adj_state(pmx_ctx* c);
 
@@ -109,6 +153,9 @@ extern "C" int pmx_ctx_create(int n0, int n1, int n2, double cell_size, void* st
   c->cellf = (float)cell_size;
   c->stream = (hipStream_t)stream;
   hipGetDevice(&c->device);
+  c->dalloc = g_dalloc;
+  c->dfree = g_dfree;
+  c->dalloc_user = g_dalloc_user;
 
   int rc = PMX_OK;
   for (int a = 0; a < 3 && rc == PMX_OK; ++a) {
@@ -172,24 +219,30 @@ extern "C" int pmx_ctx_destroy(pmx_ctx* c) {
   if (!c) return PMX_OK;
   hipDeviceSynchronize();
   pmx_timer_destroy(c);
+  // sizes are not needed here: the byte count ends with the context
+  auto rel = [c](void* p) {
+    if (!p) return;
+    if (c->dfree)
+      c->dfree(p, 0, c->device, (void*)c->stream, c->dalloc_user);
+    else
+      hipFree(p);
+  };
   for (int a = 0; a < 3; ++a) {
-    if (c->k[a]) hipFree(c->k[a]);
-    if (c->km[a]) hipFree(c->km[a]);
-    if (c->tw[a]) hipFree(c->tw[a]);
-    if (c->cbuf[a]) hipFree(c->cbuf[a]);
+    rel(c->k[a]);
+    rel(c->km[a]);
+    rel(c->tw[a]);
+    rel(c->cbuf[a]);
   }
   void* ps[] = {c->tile_count, c->tile_start, c->tile_cap, c->flags, c->halo, c->rec, c->pkey,
                 c->prank, c->pidx, c->vrec, c->vmax, c->red,
                 c->red_out, c->sort_k[0], c->sort_k[1], c->sort_v[0], c->sort_v[1], c->sort_hist,
                 c->scan_tmp};
-  for (void* p : ps)
-    if (p) hipFree(p);
+  for (void* p : ps) rel(p);
   // pl[1], al[1], pl_tile and pl_rank alias the scratch freed above
   void* qs[] = {c->pl[0].s0, c->pl[0].s1, c->pl_count[0], c->pl_start[0], c->pl_count[1],
                 c->pl_start[1], c->al[0].p.s0, c->al[0].p.s1, c->al[0].c0, c->al[0].c1,
                 c->abuf[0], c->abuf[1], c->abuf[2], c->al_dots};
-  for (void* p : qs)
-    if (p) hipFree(p);
+  for (void* p : qs) rel(p);
   if (c->red_host) hipHostFree(c->red_host);
   free(c);
   return PMX_OK;
@@ -267,7 +320,14 @@ int pmx_ensure_ptcl(pmx_ctx* c, long long n) {
   rc |= dev_alloc(c, (void**)&c->pidx, rcap * 4);
   rc |= dev_alloc(c, (void**)&c->pkey, rcap * 4);
   rc |= dev_alloc(c, (void**)&c->prank, rcap * 4);
-  if (rc) return PMX_ENOMEM;
+  if (rc) {
+    // give back the part that was allocated (the failed pointers are NULL)
+    dev_free(c, (void**)&c->rec, rcap * 16);
+    dev_free(c, (void**)&c->pidx, rcap * 4);
+    dev_free(c, (void**)&c->pkey, rcap * 4);
+    dev_free(c, (void**)&c->prank, rcap * 4);
+    return PMX_ENOMEM;
+  }
   c->ptcl_cap = cap;
   c->rec_cap = rcap;
   return PMX_OK;
@@ -301,7 +361,11 @@ int pmx_ensure_state(pmx_ctx* c) {
       if (!c->pl_count[b]) rc |= dev_alloc(c, (void**)&c->pl_count[b], (size_t)(c->ntiles + 1) * 4);
       if (!c->pl_start[b]) rc |= dev_alloc(c, (void**)&c->pl_start[b], (size_t)(c->ntiles + 1) * 4);
     }
-    if (rc) return PMX_ENOMEM;
+    if (rc) {
+      dev_free(c, (void**)&c->pl[0].s0, cap * 16);
+      dev_free(c, (void**)&c->pl[0].s1, cap * 16);
+      return PMX_ENOMEM;
+    }
     c->pl_cap = cap;
   }
   c->pl[1].s0 = c->rec;
@@ -331,7 +395,13 @@ int pmx_ensure_adj_state(pmx_ctx* c, int nsteps) {
     rc |= dev_alloc(c, (void**)&a.p.s1, cap * 16);
     rc |= dev_alloc(c, (void**)&a.c0, cap * 16);
     rc |= dev_alloc(c, (void**)&a.c1, cap * 8);
-    if (rc) return PMX_ENOMEM;
+    if (rc) {
+      dev_free(c, (void**)&a.p.s0, cap * 16);
+      dev_free(c, (void**)&a.p.s1, cap * 16);
+      dev_free(c, (void**)&a.c0, cap * 16);
+      dev_free(c, (void**)&a.c1, cap * 8);
+      return PMX_ENOMEM;
+    }
     c->al_cap = cap;
   }
   for (int b = 0; b < 3; ++b)

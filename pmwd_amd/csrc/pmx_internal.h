@@ -138,6 +138,10 @@ struct pmx_ctx {
   float cellf;         // float32 cell size
   hipStream_t stream;
   int device;
+  // device allocator of pmx_set_allocator at creation (NULL: hipMalloc/hipFree)
+  pmx_alloc_fn dalloc;
+  pmx_free_fn dfree;
+  void* dalloc_user;
 
   // k tables (float32 as pmwd: fftfreq(...).astype(f32)); *_m has the Nyquist
   // bin zeroed (gravity.py:38-42)
@@ -213,6 +217,10 @@ struct pmx_ctx {
 int pmx_ensure_ptcl(pmx_ctx* c, long long n);
 int pmx_ensure_vrec(pmx_ctx* c);
 int pmx_ensure_red(pmx_ctx* c, long long nblk);
+// Device memory of a context through its allocator (pmx_set_allocator or
+// hipMalloc), counted in pmx_ctx_bytes; the free takes the allocated size.
+int pmx_dev_alloc(pmx_ctx* c, void** p, size_t bytes);
+void pmx_dev_free(pmx_ctx* c, void** p, size_t bytes);
 int pmx_scan_exclusive(pmx_ctx* c, const uint32_t* in, uint32_t* out, long long n,
                        const uint32_t* cond = nullptr);
 

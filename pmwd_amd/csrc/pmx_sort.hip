@@ -82,27 +82,29 @@ __global__ void max_u32_k(const uint32_t* __restrict__ k, long long n, uint32_t*
 static int ensure_sort(pmx_ctx* c, long long n, long long nhist) {
   if (n > c->sort_cap) {
     for (int b = 0; b < 2; ++b) {
-      if (c->sort_k[b]) hipFree(c->sort_k[b]);
-      if (c->sort_v[b]) hipFree(c->sort_v[b]);
-      c->sort_k[b] = c->sort_v[b] = nullptr;
+      pmx_dev_free(c, (void**)&c->sort_k[b], (size_t)c->sort_cap * 4);
+      pmx_dev_free(c, (void**)&c->sort_v[b], (size_t)c->sort_cap * 4);
     }
-    c->bytes -= (size_t)c->sort_cap * 16;
     c->sort_cap = 0;
+    int rc = PMX_OK;
     for (int b = 0; b < 2; ++b) {
-      PMX_CHECK_HIP(hipMalloc((void**)&c->sort_k[b], (size_t)n * 4));
-      PMX_CHECK_HIP(hipMalloc((void**)&c->sort_v[b], (size_t)n * 4));
+      rc |= pmx_dev_alloc(c, (void**)&c->sort_k[b], (size_t)n * 4);
+      rc |= pmx_dev_alloc(c, (void**)&c->sort_v[b], (size_t)n * 4);
+    }
+    if (rc) {
+      for (int b = 0; b < 2; ++b) {
+        pmx_dev_free(c, (void**)&c->sort_k[b], (size_t)n * 4);
+        pmx_dev_free(c, (void**)&c->sort_v[b], (size_t)n * 4);
+      }
+      return PMX_ENOMEM;
     }
     c->sort_cap = n;
-    c->bytes += (size_t)n * 16;
   }
   if (nhist > c->sort_hist_cap) {
-    if (c->sort_hist) hipFree(c->sort_hist);
-    c->bytes -= (size_t)c->sort_hist_cap * 4;
-    c->sort_hist = nullptr;
+    pmx_dev_free(c, (void**)&c->sort_hist, (size_t)c->sort_hist_cap * 8);
     c->sort_hist_cap = 0;
-    PMX_CHECK_HIP(hipMalloc((void**)&c->sort_hist, (size_t)nhist * 2 * 4));
+    if (pmx_dev_alloc(c, (void**)&c->sort_hist, (size_t)nhist * 8)) return PMX_ENOMEM;
     c->sort_hist_cap = nhist;
-    c->bytes += (size_t)nhist * 8;
   }
   return PMX_OK;
 }

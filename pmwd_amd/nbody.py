@@ -203,11 +203,20 @@ def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=N
 
 # ------------------------------------------------------------ adjoint ---
 
-def _adj_pipeline_fits(n, conf, device):
-    """Whether the device has room for the reverse pipeline's particle
-    copies (about 132 B per record slot, 1.2 slots per particle)."""
+def _adj_pipeline_fits(n, c, device):
+    """Whether the reverse pipeline's particle copies (about 132 B per record
+    slot, 1.2 slots per particle) may fit: in free device memory (after
+    torch's cached blocks are given back, when needed) plus what the context
+    already holds (the forward pipeline's copies and scratch, which the reverse
+    pipeline reuses).  The allocation itself decides; this only skips a
+    hopeless attempt."""
+    need = 132 * 1.2 * n + (2 << 30)
     free, _ = torch.cuda.mem_get_info(device)
-    return 132 * 1.2 * n + (2 << 30) < free
+    if need < free + c.nbytes:
+        return True
+    torch.cuda.empty_cache()
+    free, _ = torch.cuda.mem_get_info(device)
+    return need < free + c.nbytes
 
 
 def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False, ckpt=None,
@@ -219,16 +228,18 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
     Omega_m cotangent).  KDK steps between checkpoints run as one
     ``pmx_nbody_steps_adj`` call (particles and cotangents kept in tile
     order on the device) when ``pipeline`` is True -- by default when the
-    device has room -- else as one ``pmx_nbody_step_adj`` call per step;
-    both give the same arrays.  With ``fused`` False every splitting runs op
-    by op.
+    device has room, falling back to the per-step path when the pipeline's
+    allocation fails (the library reports that before it touches any array)
+    -- else as one ``pmx_nbody_step_adj`` call per step; both give the same
+    arrays.  With ``fused`` False every splitting runs op by op.
     """
     L = _lib.lib()
     n = pmid.shape[0]
     c = _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
     kdk = fused and _is_kdk(conf)
-    if pipeline is None:
-        pipeline = kdk and _adj_pipeline_fits(n, conf, disp.device)
+    auto = pipeline is None
+    if auto:
+        pipeline = kdk and _adj_pipeline_fits(n, c, disp.device)
     if not pipeline:
         # the reverse steps bin per step; the forward pipeline's particle
         # copy gives its memory back to the cotangents
@@ -276,10 +287,16 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
             k = e - s + 1
             dd = (ctypes.c_double * (2 * k))()
             P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-            _lib.check(L.pmx_nbody_steps_adj(
+            rc = L.pmx_nbody_steps_adj(
                 h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel), _lib.ptr(acc),
                 _lib.ptr(disp_cot), _lib.ptr(vel_cot), _lib.ptr(acc_cot), n, val, coef, kp, k,
-                P(fa), P(fd), P(fb), _lib.ptr(snap), dd))
+                P(fa), P(fd), P(fb), _lib.ptr(snap), dd)
+            if rc == _lib.ENOMEM and auto and s == 0:
+                # no room after all; nothing has changed yet
+                c.trim(pipeline_only=True)
+                pipeline = False
+                break
+            _lib.check(rc)
             for j, (ka_args, d_args, kb_args) in enumerate(args):
                 om_cot -= kick_factor_dOm(*ka_args, cosmo, conf) * dot_force + cf * float(fa[j])
                 om_cot -= drift_factor_dOm(*d_args, cosmo, conf) * dd[2 * j]
@@ -292,6 +309,7 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
                 kp = 0.0
                 vel.copy_(ckpt[step_e][1].to(vel.device))
             s = e + 1
+    if kdk and pipeline:
         if kp != 0.0:
             _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
         return acc, acc_cot, om_cot

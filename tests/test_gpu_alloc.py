@@ -1,0 +1,84 @@
+"""GPU: contexts on torch's caching allocator (the host's default,
+pmx_set_allocator) against contexts on hipMalloc."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import pm as opm
+
+pytestmark = pytest.mark.gpu
+
+
+def _hooked_by_default():
+    return os.environ.get('PMX_TORCH_ALLOCATOR', '1') != '0'
+
+
+def _make(conf, hooked, dev):
+    """A fresh Context built with (hooked) or without torch's allocator, and
+    the bytes of torch's pool it took."""
+    from pmwd_amd import _lib
+    L = _lib.lib()
+    if hooked:
+        _lib.check(L.pmx_set_allocator(_lib._torch_alloc, _lib._torch_free, None))
+    else:
+        _lib.check(L.pmx_set_allocator(_lib._ALLOC_T(), _lib._FREE_T(), None))
+    try:
+        torch.cuda.synchronize(dev)
+        before = torch.cuda.memory_allocated(dev)
+        ctx = _lib.Context(conf.mesh_shape, conf.cell_size, dev)
+        took = torch.cuda.memory_allocated(dev) - before
+    finally:
+        if _hooked_by_default():
+            _lib.check(L.pmx_set_allocator(_lib._torch_alloc, _lib._torch_free, None))
+        else:
+            _lib.check(L.pmx_set_allocator(_lib._ALLOC_T(), _lib._FREE_T(), None))
+    return ctx, took
+
+
+def _run(conf, ctx, pmid, disp, vel, dev):
+    import pmwd_amd as pa
+    from pmwd_amd import _lib
+    with _lib.context_scope(lambda key, make: ctx):
+        P = lambda a: torch.as_tensor(a, device=dev)
+        ptcl = pa.Particles(conf, P(pmid), P(disp), P(vel))
+        dens = pa.scatter(ptcl, conf)
+        cosmo = pa.boltzmann(pa.Cosmology.from_sigma8(conf, 0.8, n_s=0.96, Omega_m=0.3,
+                                                      Omega_b=0.05, h=0.7), conf)
+        out, _ = pa.nbody(ptcl, None, cosmo, conf)
+        torch.cuda.synchronize(dev)
+        return dens.cpu().numpy(), out.disp.cpu().numpy(), out.vel.cpu().numpy()
+
+
+def test_torch_allocator_context_matches_hipmalloc(dev):
+    """Same bits either way; the hooked context's memory is torch's, and
+    returns to torch's pool when the scratch is trimmed and the context
+    closed."""
+    import pmwd_amd as pa
+    shape = (32, 32, 32)
+    conf = pa.Configuration(2.0, shape, mesh_shape=1, a_start=1 / 8, a_nbody_maxstep=1 / 8)
+    pmid, disp = opm.gen_grid(shape, conf.mesh_shape, conf.cell_size)
+    rng = np.random.default_rng(4)
+    disp = (disp + 1.5 * rng.standard_normal(disp.shape)).astype(np.float32)
+    vel = (0.1 * rng.standard_normal(disp.shape)).astype(np.float32)
+
+    ctx_h, took_h = _make(conf, True, dev)
+    ctx_p, took_p = _make(conf, False, dev)
+    try:
+        assert took_p == 0
+        assert took_h >= ctx_h.nbytes > 0
+        got_h = _run(conf, ctx_h, pmid, disp, vel, dev)
+        got_p = _run(conf, ctx_p, pmid, disp, vel, dev)
+        for a, b in zip(got_h, got_p):
+            np.testing.assert_array_equal(a, b)
+        # particle scratch went through the hook too, and goes back
+        torch.cuda.synchronize(dev)
+        held = torch.cuda.memory_allocated(dev)
+        grown = ctx_h.nbytes
+        ctx_h.trim()
+        assert ctx_h.nbytes < grown
+        assert torch.cuda.memory_allocated(dev) <= held - (grown - ctx_h.nbytes)
+    finally:
+        ctx_h.close()
+        ctx_p.close()
