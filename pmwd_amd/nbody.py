@@ -203,14 +203,15 @@ def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=N
 
 # ------------------------------------------------------------ adjoint ---
 
-def _adj_pipeline_fits(n, c, device):
+def _adj_pipeline_fits(n, c, device, extra=0):
     """Whether the reverse pipeline's particle copies (about 132 B per record
     slot, 1.2 slots per particle) may fit: in free device memory (after
     torch's cached blocks are given back, when needed) plus what the context
     already holds (the forward pipeline's copies and scratch, which the reverse
-    pipeline reuses).  The allocation itself decides; this only skips a
-    hopeless attempt."""
-    need = 132 * 1.2 * n + (2 << 30)
+    pipeline reuses), beside ``extra`` bytes the caller still has to
+    allocate.  The allocation itself decides; this only skips a hopeless
+    attempt."""
+    need = 132 * 1.2 * n + extra + (2 << 30)
     free, _ = torch.cuda.mem_get_info(device)
     if need < free + c.nbytes:
         return True
@@ -239,7 +240,7 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
     kdk = fused and _is_kdk(conf)
     auto = pipeline is None
     if auto:
-        pipeline = kdk and _adj_pipeline_fits(n, c, disp.device)
+        pipeline = kdk and _adj_pipeline_fits(n, c, disp.device, extra=24 * n)  # acc, acc_cot
     if not pipeline:
         # the reverse steps bin per step; the forward pipeline's particle
         # copy gives its memory back to the cotangents
@@ -399,12 +400,23 @@ class _NBodyFn(torch.autograd.Function):
     @staticmethod
     def backward(ctx, disp_cot, vel_cot, acc_cot):
         disp_f, vel_f = ctx.saved_tensors
+        conf = ctx.conf
+        pipeline = None
+        if _is_kdk(conf):
+            n = ctx.pmid.shape[0]
+            c = _lib.context(conf.mesh_shape, conf.cell_size, disp_f.device)
+            # four working copies plus acc and acc_cot: 72 B per particle
+            if not _adj_pipeline_fits(n, c, disp_f.device, extra=72 * n):
+                # per-step reverse calls: the forward pipeline's copy goes back
+                # to the pool before the working copies are allocated
+                c.trim(pipeline_only=True)
+                pipeline = False
         disp = disp_f.clone()
         vel = vel_f.clone()
         dc = (disp_cot if disp_cot is not None else torch.zeros_like(disp)).clone().contiguous()
         vc = (vel_cot if vel_cot is not None else torch.zeros_like(vel)).clone().contiguous()
-        _, _, om_cot = _run_adjoint(ctx.pmid, disp, vel, dc, vc, ctx.cosmo, ctx.conf,
-                                    ctx.reverse, ctx.ckpt)
+        _, _, om_cot = _run_adjoint(ctx.pmid, disp, vel, dc, vc, ctx.cosmo, conf,
+                                    ctx.reverse, ctx.ckpt, pipeline=pipeline)
         om_grad = None
         if ctx.needs_input_grad[2]:
             om_grad = torch.tensor(om_cot, dtype=torch.float64)
