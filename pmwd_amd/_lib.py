@@ -5,6 +5,7 @@ library binds to (same soname), and device pointers/streams are shared.
 There is no fallback: if the library is missing or cannot run, the call
 raises.
 """
+import atexit
 import contextlib
 import ctypes
 import os
@@ -204,7 +205,11 @@ class Context:
 
     def close(self):
         if self.h:
-            lib().pmx_ctx_destroy(self.h)
+            # at interpreter exit (after _close_at_exit) the device memory is
+            # left to the process end: the allocator callbacks a destroy
+            # would call may already be gone
+            if _exiting is False:
+                lib().pmx_ctx_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -273,3 +278,19 @@ def clear_contexts():
     for c in _ctx_cache.values():
         c.close()
     _ctx_cache.clear()
+
+
+_exiting = False
+
+
+@atexit.register
+def _close_at_exit():
+    """Destroys the cached contexts while torch and the allocator callbacks
+    are alive (atexit handlers run before module teardown, and this one,
+    registered after torch's, before torch's)."""
+    global _exiting
+    try:
+        if _lib is not None:
+            clear_contexts()
+    finally:
+        _exiting = True

@@ -82,3 +82,84 @@ def test_torch_allocator_context_matches_hipmalloc(dev):
     finally:
         ctx_h.close()
         ctx_p.close()
+
+
+def test_adjoint_falls_back_when_pipeline_allocation_fails(dev, monkeypatch):
+    """The reverse pipeline's allocation fails (an allocator that refuses
+    one request, armed just before pmx_nbody_steps_adj): the library gives
+    back what it took and returns PMX_ENOMEM before touching any array, and
+    the adjoint driver runs the per-step path instead, with the same
+    arrays as the pipeline."""
+    import importlib
+    import pmwd_amd as pa
+    from pmwd_amd import _lib
+    nbm = importlib.import_module('pmwd_amd.nbody')
+    L = _lib.lib()
+    shape = (16, 16, 16)
+    conf = pa.Configuration(4.0, shape, mesh_shape=2, a_start=1 / 8, a_nbody_maxstep=1 / 16)
+    cosmo = pa.boltzmann(pa.SimpleLCDM(conf), conf)
+    pmid, disp = opm.gen_grid(shape, conf.mesh_shape, conf.cell_size)
+    rng = np.random.default_rng(41)
+    disp = (disp + rng.standard_normal(disp.shape)).astype(np.float32)
+    vel = (0.1 * rng.standard_normal(disp.shape)).astype(np.float32)
+    dc0 = rng.standard_normal(disp.shape).astype(np.float32)
+    vc0 = rng.standard_normal(disp.shape).astype(np.float32)
+    P = lambda a: torch.as_tensor(a, device=dev).clone()
+
+    state = {'armed': False, 'refused': 0}
+
+    @_lib._ALLOC_T
+    def refusing_alloc(nbytes, device, stream, user):
+        if state['armed']:
+            state['armed'] = False
+            state['refused'] += 1
+            return None
+        try:
+            return torch.cuda.caching_allocator_alloc(int(nbytes), int(device), int(stream or 0))
+        except Exception:
+            return None
+
+    def run(ctx, pipeline, arm):
+        real = L.pmx_nbody_steps_adj
+        calls = []
+
+        def armed_steps_adj(*a):
+            state['armed'] = arm
+            rc = real(*a)
+            calls.append(rc)
+            state['armed'] = False
+            return rc
+
+        monkeypatch.setattr(L, 'pmx_nbody_steps_adj', armed_steps_adj)
+        try:
+            with _lib.context_scope(lambda key, make: ctx):
+                d, v = P(disp), P(vel)
+                nbm._run_forward(P(pmid), d, v, cosmo, conf)
+                x, vv, dc, vc = d, v, P(dc0), P(vc0)
+                acc, ac, om = nbm._run_adjoint(P(pmid), x, vv, dc, vc, cosmo, conf,
+                                               pipeline=pipeline)
+                torch.cuda.synchronize(dev)
+                return [t.cpu().numpy() for t in (x, vv, dc, vc, acc, ac)], om, calls
+        finally:
+            monkeypatch.setattr(L, 'pmx_nbody_steps_adj', real)
+
+    _lib.check(L.pmx_set_allocator(refusing_alloc, _lib._torch_free, None))
+    try:
+        ctx_f = _lib.Context(conf.mesh_shape, conf.cell_size, dev)
+        ctx_p = _lib.Context(conf.mesh_shape, conf.cell_size, dev)
+    finally:
+        if _hooked_by_default():
+            _lib.check(L.pmx_set_allocator(_lib._torch_alloc, _lib._torch_free, None))
+        else:
+            _lib.check(L.pmx_set_allocator(_lib._ALLOC_T(), _lib._FREE_T(), None))
+    try:
+        got, om_got, calls = run(ctx_f, None, True)
+        assert state['refused'] == 1 and calls == [_lib.ENOMEM]
+        want, om_want, calls_p = run(ctx_p, True, False)
+        assert calls_p and all(rc == 0 for rc in calls_p)
+        for a, b in zip(got, want):
+            np.testing.assert_array_equal(a, b)
+        assert abs(om_got - om_want) <= 1e-10 * abs(om_want)
+    finally:
+        ctx_f.close()
+        ctx_p.close()
