@@ -17,6 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, 'libpmx.so')
 
 _lib = None
+_torch_pool = False  # contexts made by lib() take memory from torch's caching allocator
 _lock = threading.Lock()
 
 c_int, c_ll, c_f, c_d, c_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_float, ctypes.c_double, ctypes.c_void_p
@@ -138,12 +139,21 @@ def lib():
             # contexts take their device memory from torch's caching
             # allocator, so that what one gives back serves the other without
             # new device mappings (PMX_TORCH_ALLOCATOR=0: hipMalloc)
+            global _torch_pool
             if os.environ.get('PMX_TORCH_ALLOCATOR', '1') != '0':
                 check_rc = L.pmx_set_allocator(_torch_alloc, _torch_free, None)
                 if check_rc != 0:
                     raise PmxError(f'pmx_set_allocator failed ({check_rc})', check_rc)
+                _torch_pool = True
             _lib = L
     return _lib
+
+
+def torch_pool():
+    """Whether contexts (made after the library loaded) draw from torch's
+    caching allocator, so that torch's cached free blocks serve them too."""
+    lib()
+    return _torch_pool
 
 
 def exported_symbols():

@@ -205,14 +205,18 @@ def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=N
 
 def _adj_pipeline_fits(n, c, device, extra=0):
     """Whether the reverse pipeline's particle copies (about 132 B per record
-    slot, 1.2 slots per particle) may fit: in free device memory (after
-    torch's cached blocks are given back, when needed) plus what the context
-    already holds (the forward pipeline's copies and scratch, which the reverse
-    pipeline reuses), beside ``extra`` bytes the caller still has to
-    allocate.  The allocation itself decides; this only skips a hopeless
+    slot, 1.2 slots per particle) may fit beside ``extra`` bytes the caller
+    still has to allocate: in free device memory, plus what the context
+    already holds (the forward pipeline's copies and scratch, which the
+    reverse pipeline reuses), plus torch's cached free blocks -- counted as
+    they are when the contexts share torch's pool, else given back to the
+    device first.  The allocation itself decides; this only skips a hopeless
     attempt."""
     need = 132 * 1.2 * n + extra + (2 << 30)
     free, _ = torch.cuda.mem_get_info(device)
+    if _lib.torch_pool():
+        cached = torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
+        return need < free + cached + c.nbytes
     if need < free + c.nbytes:
         return True
     torch.cuda.empty_cache()
