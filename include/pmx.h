@@ -49,6 +49,8 @@ const char* pmx_last_error(void);
 int pmx_ctx_create(int n0, int n1, int n2, double cell_size, void* stream,
                    pmx_ctx** out);
 int pmx_ctx_destroy(pmx_ctx* ctx);
+/* Rebinds the context to `stream`; work enqueued from now on runs on it,
+ * after everything the context had queued on its previous stream.        */
 int pmx_ctx_set_stream(pmx_ctx* ctx, void* stream);
 /* Device bytes held by the context (host value). */
 long long pmx_ctx_bytes(const pmx_ctx* ctx);
@@ -68,8 +70,8 @@ int pmx_ctx_trim(pmx_ctx* ctx, int what);
  * and hipFree, the default).  alloc returns a block of `bytes` on `device`
  * for use on `stream` (the context's stream then), or NULL, which the
  * calling entry point reports as PMX_ENOMEM; release takes it back (bytes is
- * 0 when a context is destroyed).  The context's stream is synchronised
- * before a block is taken or handed back.  No counterpart in pmwd, where
+ * 0 when a context is destroyed).  The device is synchronised before a
+ * block is taken or handed back, as hipFree does.  No counterpart in pmwd, where
  * XLA owns every buffer: for hosts with a caching allocator of their own
  * (torch's), so that memory a context gives back serves the host's arrays
  * and the reverse without new device mappings.                            */

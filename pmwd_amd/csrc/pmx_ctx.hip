@@ -35,9 +35,10 @@ extern "C" int pmx_set_allocator(pmx_alloc_fn alloc, pmx_free_fn release, void* 
   return PMX_OK;
 }
 
-// With a caller's allocator the context's stream is drained before a block
-// is taken or handed back, so a block changes owner only when no queued
-// work on the stream can still touch it.
+// With a caller's allocator the device is drained before a block is taken
+// or handed back (as hipFree does), so a block changes owner only when no
+// queued work can still touch it -- also work the context queued on a
+// stream it was bound to before.
 static int dev_alloc(pmx_ctx* c, void** p, size_t bytes) {
   if (bytes == 0) bytes = 256;
   if (c->dalloc) {
@@ -46,11 +47,11 @@ static int dev_alloc(pmx_ctx* c, void** p, size_t bytes) {
       pmx_set_error("device allocation of %zu bytes failed (caller's allocator)", bytes);
       return PMX_ENOMEM;
     }
-    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    if (hipDeviceSynchronize() != hipSuccess) {
       (void)hipGetLastError();
       c->dfree(*p, bytes, c->device, (void*)c->stream, c->dalloc_user);
       *p = nullptr;
-      pmx_set_error("device allocation: stream synchronisation failed");
+      pmx_set_error("device allocation: device synchronisation failed");
       return PMX_EHIP;
     }
     c->bytes += bytes;
@@ -72,7 +73,7 @@ static int dev_alloc(pmx_ctx* c, void** p, size_t bytes) {
 static void dev_free(pmx_ctx* c, void** p, size_t bytes) {
   if (*p) {
     if (c->dfree) {
-      (void)hipStreamSynchronize(c->stream);
+      (void)hipDeviceSynchronize();
       c->dfree(*p, bytes, c->device, (void*)c->stream, c->dalloc_user);
     } else {
       hipFree(*p);
@@ -244,13 +245,29 @@ extern "C" int pmx_ctx_destroy(pmx_ctx* c) {
                 c->abuf[0], c->abuf[1], c->abuf[2], c->al_dots};
   for (void* p : qs) rel(p);
   if (c->red_host) hipHostFree(c->red_host);
+  if (c->moved) hipEventDestroy(c->moved);
   free(c);
   return PMX_OK;
 }
 
 extern "C" int pmx_ctx_set_stream(pmx_ctx* c, void* stream) {
   PMX_REQUIRE(c != nullptr, "pmx_ctx_set_stream: ctx is NULL");
-  c->stream = (hipStream_t)stream;
+  hipStream_t s = (hipStream_t)stream;
+  if (s != c->stream) {
+    // the context's scratch may still be in use by work queued on the old
+    // stream: the new stream starts after it
+    if (!c->moved && hipEventCreateWithFlags(&c->moved, hipEventDisableTiming) != hipSuccess) {
+      c->moved = nullptr;
+      pmx_set_error("pmx_ctx_set_stream: hipEventCreate failed");
+      return PMX_EHIP;
+    }
+    if (hipEventRecord(c->moved, c->stream) != hipSuccess ||
+        hipStreamWaitEvent(s, c->moved, 0) != hipSuccess) {
+      pmx_set_error("pmx_ctx_set_stream: stream ordering failed");
+      return PMX_EHIP;
+    }
+    c->stream = s;
+  }
   return PMX_OK;
 }
 

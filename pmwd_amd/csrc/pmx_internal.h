@@ -142,6 +142,9 @@ struct pmx_ctx {
   pmx_alloc_fn dalloc;
   pmx_free_fn dfree;
   void* dalloc_user;
+  // recorded on the old stream when the context moves to another one, which
+  // then waits for it (NULL until the first move)
+  hipEvent_t moved;
 
   // k tables (float32 as pmwd: fftfreq(...).astype(f32)); *_m has the Nyquist
   // bin zeroed (gravity.py:38-42)

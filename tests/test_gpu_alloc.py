@@ -163,3 +163,30 @@ def test_adjoint_falls_back_when_pipeline_allocation_fails(dev, monkeypatch):
     finally:
         ctx_f.close()
         ctx_p.close()
+
+
+def test_process_exits_cleanly_with_live_contexts(dev):
+    """A process that ends with contexts alive on torch's pool -- one in the
+    library's cache, one held by the caller -- exits with status 0: the
+    cached one is destroyed at exit while the allocator callbacks live, the
+    other is left to the process end."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = '\n'.join([
+        'import numpy as np, torch',
+        'import pmwd_amd as pa',
+        'from pmwd_amd import _lib',
+        'conf = pa.Configuration(1.0, (16, 16, 16), mesh_shape=2)',
+        'pmid = torch.zeros((16 ** 3, 3), dtype=torch.int16, device="cuda:0")',
+        'pmid[:, 0] = torch.arange(16 ** 3, device="cuda:0") % 16 * 2',
+        'disp = torch.rand((16 ** 3, 3), device="cuda:0")',
+        'dens = pa.scatter(pa.Particles(conf, pmid, disp), conf)',
+        'held = _lib.Context(conf.mesh_shape, conf.cell_size, torch.device("cuda:0"))',
+        'torch.cuda.synchronize()',
+        'print("ok", float(dens.sum()), held.nbytes > 0, flush=True)',
+    ])
+    r = subprocess.run([sys.executable, '-c', code], cwd=root, capture_output=True, text=True,
+                       timeout=150)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.startswith('ok')
