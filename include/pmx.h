@@ -70,8 +70,9 @@ int pmx_ctx_trim(pmx_ctx* ctx, int what);
  * and hipFree, the default).  alloc returns a block of `bytes` on `device`
  * for use on `stream` (the context's stream then), or NULL, which the
  * calling entry point reports as PMX_ENOMEM; release takes it back (bytes is
- * 0 when a context is destroyed).  The device is synchronised before a
- * block is taken or handed back, as hipFree does.  No counterpart in pmwd, where
+ * 0 when a context is destroyed).  The context's stream, which
+ * pmx_ctx_set_stream orders after its earlier streams, is synchronised
+ * before a block is taken or handed back.  No counterpart in pmwd, where
  * XLA owns every buffer: for hosts with a caching allocator of their own
  * (torch's), so that memory a context gives back serves the host's arrays
  * and the reverse without new device mappings.                            */

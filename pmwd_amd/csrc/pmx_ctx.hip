@@ -35,10 +35,11 @@ extern "C" int pmx_set_allocator(pmx_alloc_fn alloc, pmx_free_fn release, void* 
   return PMX_OK;
 }
 
-// With a caller's allocator the device is drained before a block is taken
-// or handed back (as hipFree does), so a block changes owner only when no
-// queued work can still touch it -- also work the context queued on a
-// stream it was bound to before.
+// With a caller's allocator the context's stream is drained before a block
+// is taken or handed back, so a block changes owner only when no queued work
+// can still touch it: pmx_ctx_set_stream orders each new stream after the
+// previous one, so the current stream's queue ends every stream's work the
+// context queued.
 static int dev_alloc(pmx_ctx* c, void** p, size_t bytes) {
   if (bytes == 0) bytes = 256;
   if (c->dalloc) {
@@ -47,11 +48,11 @@ static int dev_alloc(pmx_ctx* c, void** p, size_t bytes) {
       pmx_set_error("device allocation of %zu bytes failed (caller's allocator)", bytes);
       return PMX_ENOMEM;
     }
-    if (hipDeviceSynchronize() != hipSuccess) {
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
       (void)hipGetLastError();
       c->dfree(*p, bytes, c->device, (void*)c->stream, c->dalloc_user);
       *p = nullptr;
-      pmx_set_error("device allocation: device synchronisation failed");
+      pmx_set_error("device allocation: stream synchronisation failed");
       return PMX_EHIP;
     }
     c->bytes += bytes;
@@ -73,7 +74,7 @@ static int dev_alloc(pmx_ctx* c, void** p, size_t bytes) {
 static void dev_free(pmx_ctx* c, void** p, size_t bytes) {
   if (*p) {
     if (c->dfree) {
-      (void)hipDeviceSynchronize();
+      (void)hipStreamSynchronize(c->stream);
       c->dfree(*p, bytes, c->device, (void*)c->stream, c->dalloc_user);
     } else {
       hipFree(*p);
