@@ -50,7 +50,8 @@ int pmx_ctx_create(int n0, int n1, int n2, double cell_size, void* stream,
                    pmx_ctx** out);
 int pmx_ctx_destroy(pmx_ctx* ctx);
 /* Rebinds the context to `stream`; work enqueued from now on runs on it,
- * after everything the context had queued on its previous stream.        */
+ * after everything the context had queued on its previous stream (unless
+ * either stream is under graph capture: then the caller orders them).    */
 int pmx_ctx_set_stream(pmx_ctx* ctx, void* stream);
 /* Device bytes held by the context (host value). */
 long long pmx_ctx_bytes(const pmx_ctx* ctx);

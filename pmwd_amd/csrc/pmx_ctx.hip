@@ -254,7 +254,19 @@ extern "C" int pmx_ctx_destroy(pmx_ctx* c) {
 extern "C" int pmx_ctx_set_stream(pmx_ctx* c, void* stream) {
   PMX_REQUIRE(c != nullptr, "pmx_ctx_set_stream: ctx is NULL");
   hipStream_t s = (hipStream_t)stream;
-  if (s != c->stream) {
+  hipStreamCaptureStatus cs_old = hipStreamCaptureStatusNone, cs_new = hipStreamCaptureStatusNone;
+  if (s != c->stream && (hipStreamIsCapturing(c->stream, &cs_old) != hipSuccess ||
+                         hipStreamIsCapturing(s, &cs_new) != hipSuccess)) {
+    (void)hipGetLastError();
+    pmx_set_error("pmx_ctx_set_stream: stream capture query failed");
+    return PMX_EHIP;
+  }
+  if (s != c->stream && (cs_old != hipStreamCaptureStatusNone ||
+                         cs_new != hipStreamCaptureStatusNone)) {
+    // a stream under graph capture cannot wait on work outside its capture:
+    // the capturing caller orders the streams itself
+    c->stream = s;
+  } else if (s != c->stream) {
     // the context's scratch may still be in use by work queued on the old
     // stream: the new stream starts after it
     if (!c->moved && hipEventCreateWithFlags(&c->moved, hipEventDisableTiming) != hipSuccess) {
