@@ -105,16 +105,15 @@ def Omega_m_a(a, cosmo):
     return cosmo.Omega_m / (a**3 * E2(a, cosmo))
 
 
-def transfer_fit(k, cosmo, conf):
-    """Eisenstein & Hu 1998 fit, restated from boltzmann.py:32-123."""
-    k = np.asarray(k, dtype=np.float64)
-    k = k * cosmo.h / conf.L * conf.Mpc_SI
+def eh98_scales(cosmo, conf):
+    """Scales of the Eisenstein & Hu 1998 fit (boltzmann.py:40-66): matter-
+    radiation equality z_eq and k_eq [1/Mpc], drag epoch z_d, baryon-photon
+    ratios R_d and R_eq, sound horizon s [Mpc] (EH98 eq. 6) and Silk scale
+    k_silk [1/Mpc]."""
     T2_cmb_norm = (conf.T_cmb / 2.7)**2
     h2 = cosmo.h**2
     w_m = cosmo.Omega_m * h2
     w_b = cosmo.Omega_b * h2
-    f_b = cosmo.Omega_b / cosmo.Omega_m
-    f_c = cosmo.Omega_c / cosmo.Omega_m
     z_eq = 2.50e4 * w_m / T2_cmb_norm**2
     k_eq = 7.46e-2 * w_m / T2_cmb_norm
     b1 = 0.313 * w_m**-0.419 * (1 + 0.607 * w_m**0.674)
@@ -125,6 +124,19 @@ def transfer_fit(k, cosmo, conf):
     s = (2 / (3 * k_eq) * np.sqrt(6 / R_eq)
          * np.log((np.sqrt(1 + R_d) + np.sqrt(R_eq + R_d)) / (1 + np.sqrt(R_eq))))
     k_silk = 1.6 * w_b**0.52 * w_m**0.73 * (1 + (10.4 * w_m)**-0.95)
+    return dict(w_m=w_m, w_b=w_b, z_eq=z_eq, k_eq=k_eq, z_d=z_d, R_d=R_d, R_eq=R_eq, s=s,
+                k_silk=k_silk)
+
+
+def transfer_fit(k, cosmo, conf):
+    """Eisenstein & Hu 1998 fit, restated from boltzmann.py:32-123."""
+    k = np.asarray(k, dtype=np.float64)
+    k = k * cosmo.h / conf.L * conf.Mpc_SI
+    sc = eh98_scales(cosmo, conf)
+    w_m, w_b, z_eq, k_eq, z_d = sc['w_m'], sc['w_b'], sc['z_eq'], sc['k_eq'], sc['z_d']
+    R_d, s, k_silk = sc['R_d'], sc['s'], sc['k_silk']
+    f_b = cosmo.Omega_b / cosmo.Omega_m
+    f_c = cosmo.Omega_c / cosmo.Omega_m
 
     if conf.transfer_fit_nowiggle:
         alpha_gamma = (1 - 0.328 * np.log(431 * w_m) * f_b

@@ -347,6 +347,109 @@ def neg_grad(k, pot, spacing):
     return out
 
 
+# ------- default-argument CIC shared by one force (scatter + 3 gathers) -------
+
+def cic_default(pmid, disp, cell_size, mesh_shape, grad=False):
+    """enmesh's default branch (pm_util.py:119-155 with offset 0, a2 None,
+    periodic ``mesh_shape``) computed per axis and shared by the scatter and
+    the gathers of one force.
+
+    Same operations and dtypes as ``enmesh``: ``d = disp / cell`` in float32,
+    ``floor`` cast to int16, ``d - float(i + bit)``, the int16 sum with
+    ``pmid`` and the floor-mod wrap, ``1 - |.|`` and the products
+    ``(f0*f1)*f2`` over neighbour bits ``j >> axis`` -- so every index and
+    fraction equals ``enmesh``'s bit for bit (checked in the oracle tests);
+    it only avoids materialising the (n, 8, 3) intermediates.  Returns the
+    raveled C-order indices (n, 8) int64, the weights (n, 8) and, with
+    ``grad``, the weight gradients (n, 8, 3) of ``enmesh(grad=True)``.
+    """
+    pmid = np.asarray(pmid)
+    disp = np.asarray(disp)
+    idt, fdt = pmid.dtype, disp.dtype
+    dim = pmid.shape[1]
+    a1 = fdt.type(cell_size)
+    d1 = disp / a1
+    lo = np.floor(d1).astype(idt)
+    idx, frac, sign = [], [], []
+    for ax in range(dim):
+        s = idt.type(mesh_shape[ax])
+        i_ax, f_ax, g_ax = [], [], []
+        for bit in (0, 1):
+            i2 = (lo[:, ax] + idt.type(bit)).astype(idt)
+            d2 = d1[:, ax] - i2.astype(fdt)
+            i2 = np.mod((i2 + pmid[:, ax]).astype(idt), s).astype(idt)
+            i_ax.append(i2)
+            f_ax.append((1 - np.abs(d2)).astype(fdt))
+            g_ax.append(np.sign(-d2).astype(fdt))
+        idx.append(i_ax)
+        frac.append(f_ax)
+        sign.append(g_ax)
+    n = len(pmid)
+    nb = neighbors(dim, np.int64)
+    flat = np.empty((n, 2**dim), dtype=np.int64)
+    w = np.empty((n, 2**dim), dtype=fdt)
+    f2g = np.empty((n, 2**dim, dim), dtype=fdt) if grad else None
+    for j in range(2**dim):
+        b = nb[j]
+        r = np.zeros(n, dtype=np.int64)
+        for ax in range(dim):
+            r = r * int(mesh_shape[ax]) + idx[ax][b[ax]].astype(np.int64)
+        flat[:, j] = r
+        wj = frac[0][b[0]]
+        for ax in range(1, dim):
+            wj = wj * frac[ax][b[ax]]
+        w[:, j] = wj
+        if grad:
+            for i in range(dim):
+                not_i = tuple(range(i + 1, dim)) + tuple(range(0, i))
+                p = frac[not_i[0]][b[not_i[0]]] if not_i else np.ones(n, fdt)
+                for k in not_i[1:]:
+                    p = p * frac[k][b[k]]
+                f2g[:, j, i] = sign[i][b[i]] * p
+    return flat, w, f2g
+
+
+def _pool_map(fn, n, chunk=1 << 18):
+    """fn(s, e) over particle chunks on a thread pool (numpy releases the GIL
+    in the array work); results in chunk order."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+    spans = [(s, min(n, s + chunk)) for s in range(0, n, chunk)] or [(0, 0)]
+    workers = max(1, min(len(spans), os.cpu_count() or 1, 16))
+    if workers == 1:
+        return [fn(s, e) for s, e in spans]
+    with ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(lambda se: fn(*se), spans))
+
+
+def _cic_chunks(pmid, disp, cell_size, mesh_shape, grad=False):
+    return _pool_map(lambda s, e: (s, e) + cic_default(pmid[s:e], disp[s:e], cell_size,
+                                                        mesh_shape, grad), len(pmid))
+
+
+def _scatter_cic(cics, M, contrib_fn, dtype):
+    """Sum of contrib_fn(s, e, w) (n_chunk, 8) at the shared indices: float64
+    bincount per chunk, chunks added in order, cast to ``dtype``."""
+    parts = _pool_map(lambda i, _e: np.bincount(
+        cics[i][2].ravel(), weights=contrib_fn(*cics[i][:2], cics[i][3]).ravel().astype(np.float64),
+        minlength=M), len(cics), chunk=1)
+    acc = np.zeros(M)
+    for p in parts:
+        acc += p
+    return acc.astype(dtype)
+
+
+def _gather_cic(cics, flatmesh, n, dtype, C=1):
+    out = np.empty((n, C), dtype=dtype)
+
+    def one(i, _e):
+        s, e, flat, w = cics[i][:4]
+        m = flatmesh[flat].astype(dtype)          # (k, 8, C)
+        out[s:e] = _tree8(m * w[..., None])
+    _pool_map(one, len(cics), chunk=1)
+    return out
+
+
 def gravity_mesh(dens, omega_m, cell_size, mesh_shape, dtype=np.float32):
     """Three force meshes from a density mesh (gravity.py:51-67, before gather)."""
     kvec = fftfreq(mesh_shape, cell_size, dtype=dtype)
@@ -362,12 +465,22 @@ def gravity_mesh(dens, omega_m, cell_size, mesh_shape, dtype=np.float32):
 
 
 def gravity(pmid, disp, omega_m, cell_size, mesh_shape, ptcl_num=None,
-            dtype=np.float32, return_dens=False):
-    """Accelerations (N, 3) (gravity.py:47-72)."""
-    dens = scatter(pmid, disp, cell_size, mesh_shape, ptcl_num=ptcl_num, dtype=dtype)
+            dtype=np.float32, return_dens=False, _cics=None):
+    """Accelerations (N, 3) (gravity.py:47-72).
+
+    Equals ``scatter`` followed by three ``gather`` calls (default
+    arguments) with the CIC geometry computed once (``cic_default``)."""
+    pmid = np.asarray(pmid)
+    disp = np.asarray(disp, dtype=dtype)
+    n = len(pmid)
+    if ptcl_num is None:
+        ptcl_num = n
+    M = int(np.prod(mesh_shape))
+    val = np.asarray(M / ptcl_num, dtype=dtype)
+    cics = _cics if _cics is not None else _cic_chunks(pmid, disp, cell_size, mesh_shape)
+    dens = _scatter_cic(cics, M, lambda s, e, w: val * w, dtype).reshape(mesh_shape)
     g = gravity_mesh(dens, omega_m, cell_size, mesh_shape, dtype)
-    acc = np.stack([gather(pmid, disp, cell_size, mesh_shape, gi, dtype=dtype) for gi in g],
-                   axis=-1)
+    acc = _gather_cic(cics, np.stack([gi.ravel() for gi in g], axis=-1), n, dtype, C=3)
     if return_dens:
         return acc, dens, g
     return acc
@@ -379,21 +492,39 @@ def gravity_vjp(pmid, disp, omega_m, cell_size, mesh_shape, acc_cot, ptcl_num=No
 
     Chain: gather adjoints of the three force meshes, the transpose of
     irfftn(-ik * laplace(rfftn(.))) per axis, the scale 1.5*Omega_m, then the
-    scatter adjoint.  Returns (acc, disp_cot, omega_m_cot).
+    scatter adjoint.  Returns (acc, disp_cot, omega_m_cot).  The CIC
+    geometry is computed once and shared (``cic_default``); the arithmetic is
+    that of ``gather_adj`` and ``scatter_adj``.
     """
+    pmid = np.asarray(pmid)
+    disp = np.asarray(disp, dtype=dtype)
     n = len(pmid)
     if ptcl_num is None:
         ptcl_num = n
+    M = int(np.prod(mesh_shape))
+    cics = _cic_chunks(pmid, disp, cell_size, mesh_shape, grad=True)
     acc, dens, g = gravity(pmid, disp, omega_m, cell_size, mesh_shape, ptcl_num, dtype,
-                           return_dens=True)
+                           return_dens=True, _cics=cics)
     acc_cot = np.asarray(acc_cot, dtype=dtype)
     kvec = fftfreq(mesh_shape, cell_size, dtype=dtype)
-    M = int(np.prod(mesh_shape))
+    inv_cell = dtype(cell_size)
     disp_cot = np.zeros_like(disp, dtype=dtype)
+    gflat = [gi.ravel() for gi in g]
     dsum = None
     for d in range(3):
-        dc, mc, _ = gather_adj(pmid, disp, cell_size, mesh_shape, g[d], acc_cot[:, d], dtype=dtype)
+        # gather_adj of mesh g[d] with val_cot = acc_cot[:, d]
+        dc = np.empty_like(disp)
+
+        def one(i, _e, d=d):
+            s, e, flat, w, fg = cics[i]
+            vcot = acc_cot[s:e, d][:, None]                       # (k, 1)
+            mv = gflat[d][flat].astype(dtype)                     # (k, 8)
+            dck = vcot * mv
+            dc[s:e] = (dck[..., None] * fg).sum(axis=1) / inv_cell
+        _pool_map(one, len(cics), chunk=1)
         disp_cot = disp_cot + dc
+        mc = _scatter_cic(cics, M, lambda s, e, w, d=d: acc_cot[s:e, d][:, None] * w,
+                          dtype).reshape(mesh_shape)
         # transpose of irfftn(s=mesh): the adjoint of the c2r inverse is a
         # forward r2c scaled by 1/M with the non-self-conjugate modes doubled;
         # composed with the odd, anti-Hermitian -ik/k^2 it equals -L_d applied
@@ -402,9 +533,16 @@ def gravity_vjp(pmid, disp, omega_m, cell_size, mesh_shape, acc_cot, ptcl_num=No
         t = neg_grad(kvec[d], laplace(kvec, mk), cell_size)
         dsum = t if dsum is None else dsum + t
     coef = dtype(1.5) * dtype(omega_m)
-    dens_cot = (-sfft.irfftn(dsum, s=mesh_shape)).astype(dtype) * coef
-    dc2, _ = scatter_adj(pmid, disp, cell_size, mesh_shape, dens_cot, ptcl_num=ptcl_num,
-                         dtype=dtype)
+    dens_cot = ((-sfft.irfftn(dsum, s=mesh_shape)).astype(dtype) * coef).ravel()
+    val = np.asarray(M / ptcl_num, dtype=dtype)
+    dc2 = np.empty_like(disp)
+
+    def one2(i, _e):
+        s, e, flat, w, fg = cics[i]
+        vc = dens_cot[flat].astype(dtype)                         # (k, 8)
+        dck = vc * val
+        dc2[s:e] = (dck[..., None] * fg).sum(axis=1) / inv_cell
+    _pool_map(one2, len(cics), chunk=1)
     disp_cot = disp_cot + dc2
     om_cot = float(np.sum(acc_cot.astype(np.float64) * acc.astype(np.float64))) / omega_m
     return acc, disp_cot.astype(dtype), om_cot

@@ -292,3 +292,223 @@ def test_lpt_first_order_is_zeldovich():
     ok = ~nyq
     ok[0, 0, 0] = False
     np.testing.assert_allclose(div[ok], -D1 * delta[ok], rtol=1e-4, atol=1e-6 * np.abs(delta).max())
+
+
+# ------------------------------- pins of the cosmology functions ---
+#
+# growth_ode (boltzmann.py:197-205), the 2LPT source _L (lpt.py:40-76) and
+# transfer_fit (boltzmann.py:32-123) are pinned by what mathematics and
+# published fits fix, independently of their own code:
+#   * D1(a) of a flat LCDM (w = -1) cosmology is Heath's integral
+#     D1 = 5/2 Om E(a) int_0^a da' / (a' E(a'))^3 (Heath 1977);
+#   * D2/D1^2 follows 3/7 Om(a)^(-1/143) (Bouchet et al. 1995);
+#   * _L of a sum of axis-aligned cosines is the product of their second
+#     derivatives, and vanishes for a single plane wave;
+#   * transfer_fit: the two EH98 fits (full, eq. 16-24; no-wiggle, eq. 29-31)
+#     agree within a few per cent, and within 1.5 % without baryons; the
+#     sound horizon of eq. 6 is within 2 % of the eq. 26 fit; at high k the
+#     CDM piece falls as ln(q)/q^2.
+# A second group of tests plants plausible mistakes in copies of these
+# functions and checks that each one fails a pin.
+
+def _heath_D1(a, c):
+    E = lambda x: np.sqrt(oc.E2(x, c))
+    a = np.atleast_1d(a)
+    I = np.array([quad(lambda x: (x * E(x))**-3, 0, ai, epsabs=0, epsrel=1e-13, limit=500)[0]
+                  for ai in a])
+    dE = 0.5 * oc._dE2_da(a, c) / E(a)
+    D = 2.5 * c.Omega_m * E(a) * I
+    dlnD = 2.5 * c.Omega_m * (a * dE * I + a * E(a) / (a * E(a))**3)   # a dD/da
+    return D, dlnD
+
+
+def _growth_pins(growth_ode_fn):
+    """Max relative errors of D1, dD1/dlna against Heath's integral and of
+    D2/D1^2 against Bouchet's fit, on the growth-table nodes (no
+    interpolation), for three LCDM cosmologies."""
+    conf = OConf(1.0, (8, 8, 8), a_start=1 / 64, a_nbody_maxstep=1 / 16)
+    errs = [0.0, 0.0, 0.0]
+    orig = oc.growth_ode
+    oc.growth_ode = growth_ode_fn
+    try:
+        for Om in (0.25, 0.3, 0.4):
+            c = oc.growth_integ(oc.SimpleLCDM(conf, Omega_m=Om), conf)
+            a = conf.growth_a[1:]
+            D, dD = _heath_D1(a, c)
+            errs[0] = max(errs[0], np.abs(oc.growth(a, c, conf) / D - 1).max())
+            errs[1] = max(errs[1], np.abs(oc.growth(a, c, conf, deriv=1) / dD - 1).max())
+            r = oc.growth(a, c, conf, order=2) / oc.growth(a, c, conf)**2
+            errs[2] = max(errs[2], np.abs(r / (3 / 7 * oc.Omega_m_a(a, c)**(-1 / 143)) - 1).max())
+    finally:
+        oc.growth_ode = orig
+    return errs
+
+
+GROWTH_TOL = (1e-6, 1e-6, 1e-3)
+
+
+def test_growth_lcdm_vs_heath_and_bouchet():
+    e = _growth_pins(oc.growth_ode)
+    assert e[0] < GROWTH_TOL[0] and e[1] < GROWTH_TOL[1], e
+    assert e[2] < GROWTH_TOL[2], e
+
+
+def _mutant(fn, old, new):
+    """A copy of oracle function ``fn`` with ``old`` replaced by ``new`` in
+    its source, compiled in fn's module namespace."""
+    import inspect
+    src = inspect.getsource(fn)
+    assert old in src, old
+    ns = dict(vars(inspect.getmodule(fn)))
+    exec(compile(src.replace(old, new, 1), f'<mutant {fn.__name__}>', 'exec'), ns)
+    return ns[fn.__name__]
+
+
+@pytest.mark.parametrize('old,new', [
+    ('Omf = 1.5 * Omega_m_a(a, cosmo)', 'Omf = 1.5 * cosmo.Omega_m'),          # Om for Om(a)
+    ('G2pp = Omf * G1**2 -', 'G2pp = -'),                                      # dropped source
+    ('-(3 + dlnH - Omf) * G1', '-(3 + dlnH + Omf) * G1'),                      # sign
+    ('(4 + dlnH) * G1p', '(4 - dlnH) * G1p'),                                  # sign
+    ('(8 + 2 * dlnH - Omf) * G2', '(8 + dlnH - Omf) * G2'),                    # dropped factor
+    ('(6 + dlnH) * G2p', '(6 + dlnH) * G1p'),                                  # wrong index
+])
+def test_growth_pins_catch_planted_mistakes(old, new):
+    e = _growth_pins(_mutant(oc.growth_ode, old, new))
+    assert any(x >= t for x, t in zip(e, GROWTH_TOL)), (old, new, e)
+
+
+def _L_closed_form_err(L_fn, dtype):
+    conf = OConf(2.0, (16, 16, 16))
+    n, h = 16, 2.0
+    x = np.arange(n) * h
+    X, Y, Z = np.meshgrid(x, x, x, indexing='ij')
+    k1, k2, k3 = (2 * np.pi / (n * h) * m for m in (2, 3, 1))
+    A, B, C = 0.7, -1.3, 0.4
+    kv = opm.fftfreq(conf.ptcl_grid_shape, conf.ptcl_spacing, dtype=dtype)
+    cx, cy, cz = np.cos(k1 * X), np.cos(k2 * Y), np.cos(k3 * Z)
+    phi = (A * cx + B * cy + C * cz).astype(dtype)
+    L = L_fn(kv, sfft.rfftn(phi), conf, dtype)
+    want = A * B * k1**2 * k2**2 * cx * cy + A * C * k1**2 * k3**2 * cx * cz \
+        + B * C * k2**2 * k3**2 * cy * cz
+    e1 = np.abs(L - want).max() / np.abs(want).max()
+    # a single oblique plane wave: phi_ij = -A k_i k_j cos, every 2x2 minor vanishes
+    wave = (A * np.cos(k1 * X + k2 * Y + k3 * Z)).astype(dtype)
+    L = L_fn(kv, sfft.rfftn(wave), conf, dtype)
+    e2 = np.abs(L).max() / (A**2 * (k1 * k2)**2)
+    return e1, e2
+
+
+@pytest.mark.parametrize('dtype,tol', [(np.float64, 1e-12), (np.float32, 1e-5)])
+def test_lpt_source_closed_form(dtype, tol):
+    e1, e2 = _L_closed_form_err(ic._L, dtype)
+    assert e1 < tol and e2 < tol, (e1, e2)
+
+
+@pytest.mark.parametrize('old,new', [
+    ('L = L - s * s', 'L = L + s * s'),                                         # sign
+    ('L = L - s * s', 'L = L - s'),                                             # dropped square
+    ('for j in range(dim - 1, i, -1):', 'for j in range(dim - 1, i - 1, -1):'),  # diagonal squared
+    ('for j in range(i + 1, dim):', 'for j in range(i + 2, dim):'),             # dropped pair
+])
+def test_lpt_source_pins_catch_planted_mistakes(old, new):
+    e1, e2 = _L_closed_form_err(_mutant(ic._L, old, new), np.float64)
+    assert e1 > 1e-3 or e2 > 1e-3, (old, new, e1, e2)
+
+
+def _transfer_pins(fit, scales):
+    """Names of the transfer pins that fail for three cosmologies."""
+    conf = OConf(4.0, (8, 8, 8))
+    nw = conf.replace(transfer_fit_nowiggle=True)
+    k = np.logspace(-3, 1, 800)                      # h/Mpc
+    bad = set()
+    with np.errstate(all='ignore'):
+        for Om, Ob, h in [(0.3, 0.05, 0.7), (0.25, 0.04, 0.65), (0.35, 0.06, 0.75)]:
+            c = oc.SimpleLCDM(conf, Omega_m=Om, Omega_b=Ob, h=h)
+            if not abs(fit(conf.transfer_k[:1], c, conf)[0] - 1) < 1e-9:
+                bad.add('T(0)=1')
+            r = np.abs(fit(k, c, conf) / fit(k, c, nw) - 1)
+            for lo, hi, tol in [(1e-3, 0.02, 0.02), (0.02, 0.5, 0.05), (0.5, 10, 0.025)]:
+                m = (k >= lo) & (k < hi)
+                if not r[m].max() < tol:
+                    bad.add(f'full~nowiggle [{lo},{hi})')
+            c0 = c.replace(Omega_b=1e-8)
+            if not np.abs(fit(k, c0, conf) / fit(k, c0, nw) - 1).max() < 0.015:
+                bad.add('cdm-only full~nowiggle')
+            sc = scales(c, conf)
+            s26 = 44.5 * np.log(9.83 / sc['w_m']) / np.sqrt(1 + 10 * sc['w_b']**0.75)
+            if not abs(sc['s'] / s26 - 1) < 0.02:
+                bad.add('sound horizon')
+            kk = np.logspace(4, 5, 5)
+            slope = np.gradient(np.log(fit(kk, c, conf)), np.log(kk))
+            q = kk * c.h / (13.41 * sc['k_eq'])
+            if not np.abs(slope - (-2 + 1 / np.log(q))).max() < 0.01:
+                bad.add('high-k ln q/q^2')
+    return bad
+
+
+def test_transfer_fit_pins():
+    assert _transfer_pins(oc.transfer_fit, oc.eh98_scales) == set()
+
+
+@pytest.mark.parametrize('fn,old,new', [
+    ('fit', 'return f_c * T_c + f_b * T_b', 'return f_c * T_c'),                 # dropped term
+    ('fit', 'T0_tilde(k, alpha_c, beta_c)', 'T0_tilde(k, beta_c, alpha_c)'),     # transposed
+    ('fit', 'q = k / (13.41 * k_eq)', 'q = k / 13.41'),                          # dropped factor
+    ('fit', 'C = 14.2 / alpha_c', 'C = 14.2 * alpha_c'),                         # inverted
+    ('fit', 'L = np.log(np.e + 1.8 * beta_c * q)', 'L = np.log(np.e + 1.8 * q)'),
+    ('fit', 'L0 / (L0 + C0 * q_eff**2)', 'L0 / (L0 + C0 * q_eff)'),              # power
+    ('fit', 'alpha_c = a1**-f_b * a2**-f_b**3', 'alpha_c = a1**f_b * a2**f_b**3'),  # sign
+    ('fit', 'f = 1 / (1 + (k * s / 5.4)**4)', 'f = 1 / (1 + (k * s / 5.4)**2)'),
+    ('fit', '(1 - f) * T0_tilde', '(1 + f) * T0_tilde'),
+    ('scales', 'k_eq = 7.46e-2 * w_m', 'k_eq = 7.46e-2 * w_b'),                   # wrong operand
+    ('scales', '* np.sqrt(6 / R_eq)', '* np.sqrt(6 * R_eq)'),
+    ('scales', '(1 + np.sqrt(R_eq))', '(1 + np.sqrt(R_d))'),                      # wrong index
+    ('scales', 'R_d = 31.5 * w_b / T2_cmb_norm**2 * (1e3 / z_d)',
+     'R_d = 31.5 * w_b / T2_cmb_norm**2 * (1e3 / z_eq)'),
+])
+def test_transfer_pins_catch_planted_mistakes(fn, old, new):
+    fit, scales = oc.transfer_fit, oc.eh98_scales
+    if fn == 'fit':
+        fit = _mutant(oc.transfer_fit, old, new)
+    else:
+        scales = _mutant(oc.eh98_scales, old, new)
+        # the fit reads the scales through the module namespace
+        fit = _mutant(oc.transfer_fit, 'sc = eh98_scales(cosmo, conf)',
+                      'sc = _scales(cosmo, conf)')
+        fit.__globals__['_scales'] = scales
+    assert _transfer_pins(fit, scales), (fn, old, new)
+
+
+@pytest.mark.parametrize('shape,ratio,cell,std', [((16, 16, 16), 2, 1.7, 3.0),
+                                                  ((7, 9, 5), 1, 1.0, 9.0),
+                                                  ((8, 8, 8), 2, 0.5, 40.0)])
+@pytest.mark.parametrize('dtype', [np.float32, np.float64])
+def test_shared_cic_equals_enmesh_and_composition(shape, ratio, cell, std, dtype):
+    """gravity/gravity_vjp compute the CIC geometry once (cic_default); they
+    equal enmesh, and scatter + three gathers (and the adjoint chain of
+    gather_adj/scatter_adj), bit for bit."""
+    ms = tuple(s * ratio for s in shape)
+    pmid, disp = ptcl(shape, ms, cell, std, 7, dtype)
+    disp[::13] = 0
+    disp[1::17] = -0.0
+    ind, w, fg = opm.enmesh(pmid, disp, cell, ms, 0, None, ms, grad=True)
+    flat, w2, fg2 = opm.cic_default(pmid, disp, cell, ms, grad=True)
+    np.testing.assert_array_equal(opm.raveled(ind, ms), flat)
+    np.testing.assert_array_equal(w, w2)
+    np.testing.assert_array_equal(fg, fg2)
+    dens = opm.scatter(pmid, disp, cell, ms, dtype=dtype)
+    g = opm.gravity_mesh(dens, 0.3, cell, ms, dtype)
+    want = np.stack([opm.gather(pmid, disp, cell, ms, gi, dtype=dtype) for gi in g], -1)
+    np.testing.assert_array_equal(opm.gravity(pmid, disp, 0.3, cell, ms, dtype=dtype), want)
+    ac = np.random.default_rng(8).standard_normal(disp.shape).astype(dtype)
+    kvec = opm.fftfreq(ms, cell, dtype=dtype)
+    dcot, dsum = np.zeros_like(disp), None
+    for d in range(3):
+        dc, mc, _ = opm.gather_adj(pmid, disp, cell, ms, g[d], ac[:, d], dtype=dtype)
+        dcot = dcot + dc
+        t = opm.neg_grad(kvec[d], opm.laplace(kvec, sfft.rfftn(mc)), cell)
+        dsum = t if dsum is None else dsum + t
+    dens_cot = (-sfft.irfftn(dsum, s=ms)).astype(dtype) * (dtype(1.5) * dtype(0.3))
+    dc2, _ = opm.scatter_adj(pmid, disp, cell, ms, dens_cot, dtype=dtype)
+    _, got, _ = opm.gravity_vjp(pmid, disp, 0.3, cell, ms, ac, dtype=dtype)
+    np.testing.assert_array_equal(got, (dcot + dc2).astype(dtype))
