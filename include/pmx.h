@@ -143,6 +143,23 @@ int pmx_gather_cic_adj(pmx_ctx* ctx, const pmx_cic_geom* g,
                        const float* mesh, const float* val_cot, int nchan,
                        float* disp_cot, float* mesh_cot);
 
+/* pmwd/pm_util.py:33-156 (enmesh) as called by ptcl_enmesh
+ * (pmwd/particles.py:212-254): ind (n, 2^dim, dim) int16 multilinear
+ * indices, frac (n, 2^dim) fractions and, when frac_grad is non-NULL, their
+ * gradients (n, 2^dim, dim).  g gives the periodic shape, cell size, offset,
+ * optional output cell size and target shape; wrap = 0 drops the periodic
+ * wrap (enmesh's s1 = None), and then drop = 1 replaces negative indices by
+ * the target extent (pm_util.py:138-141).                                   */
+int pmx_enmesh(pmx_ctx* ctx, const pmx_cic_geom* g, int wrap, int drop,
+               const int16_t* pmid, const float* disp, long long n, int16_t* ind,
+               float* frac, float* frac_grad);
+/* The three readouts of pmwd/gravity.py:60-69 (gather.py:33-77 with default
+ * arguments) in one pass: acc (n,3) from the component-major force meshes
+ * mesh3 (3, n0, n1, n2) of the context's mesh, bitwise the three
+ * pmx_gather_cic calls.                                                     */
+int pmx_gather_cic3(pmx_ctx* ctx, const int16_t* pmid, const float* disp, long long n,
+                    const float* mesh3, float* acc);
+
 /* --------------------------------------------------------------- FFT ---
  * Hand-written Stockham FFTs.  pmx_rfftn: jnp.fft.rfftn, backward norm
  * (pmwd/pm_util.py:281); pmx_irfftn: jnp.fft.irfftn(s=mesh), scaled 1/M
@@ -155,6 +172,11 @@ int pmx_irfftn(pmx_ctx* ctx, const float* in_c, float* out);
  * laplace(k, rfftn((dens - 1) * coef)))), coef = 1.5*Omega_m in float32.
  * grad is 3 meshes (3, n0, n1, n2) C-order.                                */
 int pmx_poisson_grad(pmx_ctx* ctx, const float* dens, float coef, float* grad);
+/* VJP of pmx_poisson_grad with respect to dens -- the transposes jax.vjp
+ * takes through pmwd/gravity.py:51-67 (irfftn, neg_grad, laplace's own VJP
+ * gravity.py:19-34, rfftn, the coef scale): dens_cot = sum_d (d grad_d /
+ * d dens)^T grad_cot_d.  grad_cot is 3 meshes (3, n0, n1, n2) C-order.      */
+int pmx_poisson_grad_adj(pmx_ctx* ctx, const float* grad_cot, float coef, float* dens_cot);
 /* pmwd/gravity.py:47-72 with default scatter/gather arguments:
  * acc (n,3) for particles of mass val = mesh_size/ptcl_num.                */
 int pmx_gravity(pmx_ctx* ctx, const int16_t* pmid, const float* disp,
@@ -170,7 +192,7 @@ int pmx_drift(pmx_ctx* ctx, long long n, float* disp, const float* vel, float d)
 int pmx_nbody_step(pmx_ctx* ctx, const int16_t* pmid, float* disp, float* vel,
                    float* acc, long long n, float val, float coef, float k1,
                    float d, float k2);
-/* nsteps KDK steps of pmwd/nbody.py:121-140 (the loop of nbody, :186-201)
+/* nsteps KDK steps of pmwd/nbody.py:121-140 (the loop of nbody, :215-223)
  * with factors k1[s], d[s], k2[s] (host arrays): bitwise the result of
  * nsteps pmx_nbody_step calls.  Between steps the particles stay on the
  * device in tile order (two extra copies of about 32 bytes per particle);

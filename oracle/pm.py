@@ -560,3 +560,39 @@ def gen_grid(ptcl_grid_shape, mesh_shape, cell_size, dtype=np.float32, pmid_dtyp
     pmid = np.stack(np.meshgrid(*pm, indexing='ij'), axis=-1).reshape(-1, len(mesh_shape))
     disp = np.stack(np.meshgrid(*dp, indexing='ij'), axis=-1).reshape(-1, len(mesh_shape))
     return pmid, disp
+
+
+def raveled_id(pmid, mesh_shape, wrap=False):
+    """Particles.raveled_id (particles.py:156-182): sum pmid_d * stride_d with
+    C-order strides, in uint64 arithmetic (so negative ids wrap mod 2^64)."""
+    pmid = np.asarray(pmid)
+    if wrap:
+        pmid = np.mod(pmid, np.asarray(mesh_shape, dtype=pmid.dtype))
+    strides = np.cumprod((1,) + tuple(mesh_shape)[:0:-1])[::-1]
+    out = np.zeros(len(pmid), dtype=np.uint64)
+    for d, s in enumerate(strides):
+        out = out + pmid[:, d].astype(np.int64).astype(np.uint64) * np.uint64(s)
+    return out
+
+
+def pos(pmid, disp, cell_size, box_size, dtype=np.float64, wrap=True):
+    """Particles.pos (particles.py:184-209): pmid*cell + disp in ``dtype``,
+    wrapped with a floor mod into the box."""
+    p = np.asarray(pmid).astype(dtype) * dtype(cell_size) + np.asarray(disp).astype(dtype)
+    if wrap:
+        p = np.mod(p, np.asarray(box_size, dtype=dtype))
+    return p
+
+
+def from_pos(pos_, cell_size, mesh_shape, wrap=True, pmid_dtype=np.int16, float_dtype=np.float32):
+    """Particles.from_pos (particles.py:81-107): pmid = rint(pos/cell),
+    disp = pos - pmid*cell in the positions' dtype, then the casts and the
+    int16 floor-mod wrap."""
+    pos_ = np.asarray(pos_)
+    pmid = np.rint(pos_ / cell_size)
+    disp = pos_ - pmid * cell_size
+    pmid = pmid.astype(pmid_dtype)
+    disp = disp.astype(float_dtype)
+    if wrap:
+        pmid = np.mod(pmid, np.asarray(mesh_shape, dtype=pmid_dtype)).astype(pmid_dtype)
+    return pmid, disp

@@ -9,7 +9,7 @@ from .configuration import Configuration
 from .cosmology import Cosmology, SimpleLCDM, Planck18, E2, H_deriv, Omega_m_a
 from .boltzmann import (transfer_integ, transfer_fit, transfer, growth_integ, growth,
                         varlin_integ, varlin, boltzmann, linear_power)
-from .particles import Particles, ptcl_rpos, ptcl_rsd, ptcl_los
+from .particles import Particles, ptcl_enmesh, ptcl_pos, ptcl_rpos, ptcl_rsd, ptcl_los
 from .scatter import scatter
 from .gather import gather
 from .gravity import laplace, neg_grad, gravity

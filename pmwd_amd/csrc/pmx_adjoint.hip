@@ -193,6 +193,7 @@ static int force_vjp_binned(pmx_ctx* c, long long n, float coef, float val, cons
 extern "C" int pmx_gravity_vjp(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
                                float val, float coef, const float* vel_cot, float* acc,
                                float* acc_cot, double* dot) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0, "pmx_gravity_vjp: bad argument");
   PMX_REQUIRE(n == 0 || (pmid && disp && acc && acc_cot && vel_cot), "pmx_gravity_vjp: NULL pointer");
   PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_gravity_vjp: too many particles");
@@ -249,6 +250,7 @@ __global__ void __launch_bounds__(PMX_TPB)
 
 extern "C" int pmx_kick_adj(pmx_ctx* c, long long n, float* vel, const float* acc, float* disp_cot,
                             const float* acc_cot, const float* vel_cot, float k, double* dot) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0, "pmx_kick_adj: bad argument");
   PMX_REQUIRE(n == 0 || (vel && acc && disp_cot && acc_cot && vel_cot), "pmx_kick_adj: NULL pointer");
   if (n == 0) {
@@ -268,6 +270,7 @@ extern "C" int pmx_kick_adj(pmx_ctx* c, long long n, float* vel, const float* ac
 
 extern "C" int pmx_drift_adj(pmx_ctx* c, long long n, float* disp, const float* vel,
                              const float* disp_cot, float* vel_cot, float d, double* dot) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0, "pmx_drift_adj: bad argument");
   PMX_REQUIRE(n == 0 || (disp && vel && disp_cot && vel_cot), "pmx_drift_adj: NULL pointer");
   if (n == 0) {
@@ -382,6 +385,7 @@ extern "C" int pmx_nbody_step_adj(pmx_ctx* c, const int16_t* pmid, float* disp, 
                                   float* acc, float* disp_cot, float* vel_cot, float* acc_cot,
                                   long long n, float val, float coef, float kp, float ka, float d,
                                   const float* disp_snap, double* dots) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0, "pmx_nbody_step_adj: bad argument");
   PMX_REQUIRE(n == 0 || (pmid && disp && vel && acc && disp_cot && vel_cot && acc_cot),
               "pmx_nbody_step_adj: NULL pointer");
@@ -392,7 +396,14 @@ extern "C" int pmx_nbody_step_adj(pmx_ctx* c, const int16_t* pmid, float* disp, 
   }
   int rc, cap;
   const unsigned nb1 = pmx_blocks(n);
+  // every allocation (the force VJP's value channels and three-channel halo
+  // shells included) before the first kernel writes the caller's arrays, so
+  // that PMX_ENOMEM leaves them as they were
   if ((rc = pmx_ensure_red(c, (long long)nb1 + c->ntiles))) return rc;
+  if ((rc = pmx_ensure_ptcl(c, n))) return rc;
+  if ((rc = pmx_ensure_vrec(c))) return rc;
+  if ((rc = pmx_ensure_halo(c, 3))) return rc;
+  if ((rc = pmx_ensure_scan(c, c->ntiles + 1))) return rc;
   if ((rc = pmx_bin_begin(c, n, &cap))) return rc;
   const BinOut o = pmx_bin_out(c);
   if ((rc = pmx_kd_adj_bin(c, pmid, disp, vel, acc, disp_cot, vel_cot, acc_cot, disp_snap, n, kp,

@@ -59,6 +59,7 @@ __global__ void tile_keys_k(const int16_t* __restrict__ pmid, const float* __res
 
 extern "C" int pmx_cell_keys(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
                              uint32_t* key) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0, "pmx_cell_keys: bad argument");
   if (n == 0) return PMX_OK;
   PMX_REQUIRE(pmid && disp && key, "pmx_cell_keys: NULL pointer");
@@ -71,6 +72,7 @@ extern "C" int pmx_cell_keys(pmx_ctx* c, const int16_t* pmid, const float* disp,
 
 extern "C" int pmx_tile_keys(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
                              uint32_t* tile) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0, "pmx_tile_keys: bad argument");
   if (n == 0) return PMX_OK;
   PMX_REQUIRE(pmid && disp && tile, "pmx_tile_keys: NULL pointer");
@@ -649,6 +651,7 @@ int pmx_deposit_state_vals(pmx_ctx* c, const PState& st, const uint32_t* start,
 
 extern "C" int pmx_scatter_mass(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
                                 float val, float* mesh, int accumulate) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && mesh && n >= 0, "pmx_scatter_mass: bad argument");
   PMX_REQUIRE(n == 0 || (pmid && disp), "pmx_scatter_mass: NULL particle arrays");
   PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_scatter_mass: too many particles");
@@ -690,6 +693,7 @@ int pmx_gravity_fwd_meshes(pmx_ctx* c, const int16_t* pmid, const float* disp, l
 
 extern "C" int pmx_gravity(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
                            float val, float coef, float* acc) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0, "pmx_gravity: bad argument");
   PMX_REQUIRE(n == 0 || (pmid && disp && acc), "pmx_gravity: NULL pointer");
   PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_gravity: too many particles");
@@ -706,6 +710,7 @@ __global__ void axpy3_k(float* __restrict__ y, const float* __restrict__ x, long
 }
 
 extern "C" int pmx_kick(pmx_ctx* c, long long n, float* vel, const float* acc, float k) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0 && (n == 0 || (vel && acc)), "pmx_kick: bad argument");
   if (n == 0) return PMX_OK;
   axpy3_k<<<pmx_blocks(3 * n), PMX_TPB, 0, c->stream>>>(vel, acc, 3 * n, k);
@@ -714,6 +719,7 @@ extern "C" int pmx_kick(pmx_ctx* c, long long n, float* vel, const float* acc, f
 }
 
 extern "C" int pmx_drift(pmx_ctx* c, long long n, float* disp, const float* vel, float d) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0 && (n == 0 || (disp && vel)), "pmx_drift: bad argument");
   if (n == 0) return PMX_OK;
   axpy3_k<<<pmx_blocks(3 * n), PMX_TPB, 0, c->stream>>>(disp, vel, 3 * n, d);
@@ -782,10 +788,14 @@ int pmx_kick_drift_bin(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel,
 
 extern "C" int pmx_nbody_step(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, float* acc,
                               long long n, float val, float coef, float k1, float d, float k2) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0, "pmx_nbody_step: bad argument");
   PMX_REQUIRE(n == 0 || (pmid && disp && vel && acc), "pmx_nbody_step: NULL pointer");
   PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_nbody_step: too many particles");
   int rc, cap;
+  // every allocation before the first kernel writes disp and vel, so that
+  // PMX_ENOMEM leaves the caller's arrays as they were
+  if ((rc = pmx_ensure_scan(c, c->ntiles + 1))) return rc;
   if ((rc = pmx_bin_begin(c, n, &cap))) return rc;
   if (n > 0 && (rc = pmx_kick_drift_bin(c, pmid, disp, vel, acc, n, k1, d, c->tile_count,
                                          pmx_bin_out(c), cap)))

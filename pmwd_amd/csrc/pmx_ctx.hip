@@ -40,8 +40,41 @@ extern "C" int pmx_set_allocator(pmx_alloc_fn alloc, pmx_free_fn release, void* 
 // can still touch it: pmx_ctx_set_stream orders each new stream after the
 // previous one, so the current stream's queue ends every stream's work the
 // context queued.
+int pmx_capturing(pmx_ctx* c) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return cs != hipStreamCaptureStatusNone;
+}
+
+int pmx_no_capture(pmx_ctx* c, const char* what) {
+  if (!pmx_capturing(c)) return PMX_OK;
+  pmx_set_error("%s: the context's stream is under graph capture; run the call once before "
+                "capturing so that the context holds its scratch, and read host values outside "
+                "the capture", what);
+  return PMX_EINVAL;
+}
+
+void pmx_mark_queued(pmx_ctx* c) {
+  if (!c || !c->moved) return;
+  if (pmx_capturing(c)) {
+    // captured work runs when the graph is launched, which the capturing
+    // caller orders
+    c->moved_valid = 0;
+    return;
+  }
+  c->moved_valid = hipEventRecord(c->moved, c->stream) == hipSuccess;
+  if (!c->moved_valid) (void)hipGetLastError();
+}
+
 static int dev_alloc(pmx_ctx* c, void** p, size_t bytes) {
   if (bytes == 0) bytes = 256;
+  if (int rc = pmx_no_capture(c, "device allocation")) {
+    *p = nullptr;
+    return rc;
+  }
   if (c->dalloc) {
     *p = c->dalloc(bytes, c->device, (void*)c->stream, c->dalloc_user);
     if (!*p) {
@@ -160,6 +193,11 @@ extern "C" int pmx_ctx_create(int n0, int n1, int n2, double cell_size, void* st
   c->dalloc_user = g_dalloc_user;
 
   int rc = PMX_OK;
+  if (hipEventCreateWithFlags(&c->moved, hipEventDisableTiming) != hipSuccess) {
+    c->moved = nullptr;
+    pmx_set_error("pmx_ctx_create: hipEventCreate failed");
+    rc = PMX_EHIP;
+  }
   for (int a = 0; a < 3 && rc == PMX_OK; ++a) {
     std::vector<float> k, km;
     kvec_host(c->n[a], cell_size, a == 2, k, km);
@@ -254,33 +292,25 @@ extern "C" int pmx_ctx_destroy(pmx_ctx* c) {
 extern "C" int pmx_ctx_set_stream(pmx_ctx* c, void* stream) {
   PMX_REQUIRE(c != nullptr, "pmx_ctx_set_stream: ctx is NULL");
   hipStream_t s = (hipStream_t)stream;
-  hipStreamCaptureStatus cs_old = hipStreamCaptureStatusNone, cs_new = hipStreamCaptureStatusNone;
-  if (s != c->stream && (hipStreamIsCapturing(c->stream, &cs_old) != hipSuccess ||
-                         hipStreamIsCapturing(s, &cs_new) != hipSuccess)) {
+  if (s == c->stream) return PMX_OK;
+  // the context's scratch may still be in use by work queued on the old
+  // stream: the new one waits for the event recorded after the last call
+  // (the old stream's handle is not used again: its owner may have
+  // destroyed it).  A stream under capture cannot wait on work outside its
+  // capture; a capturing caller orders the streams itself.
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
     (void)hipGetLastError();
     pmx_set_error("pmx_ctx_set_stream: stream capture query failed");
     return PMX_EHIP;
   }
-  if (s != c->stream && (cs_old != hipStreamCaptureStatusNone ||
-                         cs_new != hipStreamCaptureStatusNone)) {
-    // a stream under graph capture cannot wait on work outside its capture:
-    // the capturing caller orders the streams itself
-    c->stream = s;
-  } else if (s != c->stream) {
-    // the context's scratch may still be in use by work queued on the old
-    // stream: the new stream starts after it
-    if (!c->moved && hipEventCreateWithFlags(&c->moved, hipEventDisableTiming) != hipSuccess) {
-      c->moved = nullptr;
-      pmx_set_error("pmx_ctx_set_stream: hipEventCreate failed");
-      return PMX_EHIP;
-    }
-    if (hipEventRecord(c->moved, c->stream) != hipSuccess ||
-        hipStreamWaitEvent(s, c->moved, 0) != hipSuccess) {
-      pmx_set_error("pmx_ctx_set_stream: stream ordering failed");
-      return PMX_EHIP;
-    }
-    c->stream = s;
+  if (cs == hipStreamCaptureStatusNone && c->moved_valid &&
+      hipStreamWaitEvent(s, c->moved, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    pmx_set_error("pmx_ctx_set_stream: stream ordering failed");
+    return PMX_EHIP;
   }
+  c->stream = s;
   return PMX_OK;
 }
 
@@ -289,6 +319,7 @@ extern "C" long long pmx_ctx_bytes(const pmx_ctx* c) { return c ? (long long)c->
 extern "C" int pmx_ctx_trim(pmx_ctx* c, int what) {
   PMX_REQUIRE(c != nullptr, "pmx_ctx_trim: NULL context");
   PMX_REQUIRE((what & ~(PMX_TRIM_PIPELINE | PMX_TRIM_ALL)) == 0, "pmx_ctx_trim: bad flags");
+  if (int rc = pmx_no_capture(c, "pmx_ctx_trim")) return rc;
   // work still queued on the context's stream may use the scratch
   hipError_t e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) {
@@ -332,6 +363,7 @@ extern "C" int pmx_tile_shape(const pmx_ctx* c, int t[3]) {
 
 int pmx_ensure_ptcl(pmx_ctx* c, long long n) {
   if (n <= c->ptcl_cap) return PMX_OK;
+  if (int rc = pmx_no_capture(c, "particle scratch")) return rc;
   const long long cap = n + n / 16 + 1024;
   // records also hold the per-tile slack of the capacity binning
   const long long rcap = pmx_record_cap(c, cap);
@@ -365,6 +397,7 @@ int pmx_ensure_ptcl(pmx_ctx* c, long long n) {
 
 int pmx_ensure_halo(pmx_ctx* c, int nch) {
   if (nch <= c->halo_nch) return PMX_OK;
+  if (int rc = pmx_no_capture(c, "halo shells")) return rc;
   dev_free(c, (void**)&c->halo, (size_t)c->halo_nch * c->ntiles * c->tg.shell * 4);
   c->halo_nch = 0;
   if (dev_alloc(c, (void**)&c->halo, (size_t)nch * c->ntiles * c->tg.shell * 4)) return PMX_ENOMEM;
@@ -382,6 +415,7 @@ int pmx_ensure_state(pmx_ctx* c) {
   if ((rc = pmx_ensure_vrec(c))) return rc;
   const long long cap = c->rec_cap;
   if (cap > c->pl_cap) {
+    if ((rc = pmx_no_capture(c, "pipeline particle copy"))) return rc;
     dev_free(c, (void**)&c->pl[0].s0, c->pl_cap * 16);
     dev_free(c, (void**)&c->pl[0].s1, c->pl_cap * 16);
     c->pl_cap = 0;
@@ -414,6 +448,7 @@ int pmx_ensure_adj_state(pmx_ctx* c, int nsteps) {
   if ((rc = pmx_ensure_state(c))) return rc;
   const long long cap = c->rec_cap;
   if (cap > c->al_cap) {
+    if ((rc = pmx_no_capture(c, "reverse pipeline particle copy"))) return rc;
     AState& a = c->al[0];
     const long long old = c->al_cap;
     dev_free(c, (void**)&a.p.s0, old * 16);
@@ -438,6 +473,7 @@ int pmx_ensure_adj_state(pmx_ctx* c, int nsteps) {
     if (!c->abuf[b] && dev_alloc(c, (void**)&c->abuf[b], (size_t)c->Mc * sizeof(float2)))
       return PMX_ENOMEM;
   if (2LL * nsteps > c->al_dots_cap) {
+    if ((rc = pmx_no_capture(c, "reverse pipeline dot products"))) return rc;
     dev_free(c, (void**)&c->al_dots, c->al_dots_cap * 8);
     c->al_dots_cap = 0;
     const long long k = 2LL * nsteps + 64;
@@ -467,6 +503,7 @@ static void free_adj_state(pmx_ctx* c) {
 int pmx_ensure_vrec(pmx_ctx* c) {
   const long long cap = 4 * c->rec_cap;
   if (cap <= c->vrec_cap) return PMX_OK;
+  if (int rc = pmx_no_capture(c, "value channels")) return rc;
   dev_free(c, (void**)&c->vrec, c->vrec_cap * 4);
   c->vrec_cap = 0;
   if (dev_alloc(c, (void**)&c->vrec, cap * 4)) return PMX_ENOMEM;
@@ -476,6 +513,7 @@ int pmx_ensure_vrec(pmx_ctx* c) {
 
 int pmx_ensure_red(pmx_ctx* c, long long nblk) {
   if (nblk <= c->red_cap) return PMX_OK;
+  if (int rc = pmx_no_capture(c, "reduction partials")) return rc;
   dev_free(c, (void**)&c->red, (size_t)c->red_cap * 8);
   c->red_cap = 0;
   if (dev_alloc(c, (void**)&c->red, (size_t)nblk * 8)) return PMX_ENOMEM;
@@ -571,16 +609,26 @@ __global__ void scan_apply_k(const uint32_t* __restrict__ in, long long n,
   }
 }
 
-int pmx_scan_exclusive(pmx_ctx* c, const uint32_t* in, uint32_t* out, long long n,
-                       const uint32_t* cond) {
+// Sizes the scan scratch for scans of up to n entries, so that entry points
+// can take it before their first kernel writes the caller's arrays.
+int pmx_ensure_scan(pmx_ctx* c, long long n) {
   long long nb = (n + SCAN_TILE - 1) / SCAN_TILE;
   if (nb < 1) nb = 1;
   if (nb > c->scan_tmp_cap) {
+    if (int rc = pmx_no_capture(c, "scan scratch")) return rc;
     dev_free(c, (void**)&c->scan_tmp, c->scan_tmp_cap * 4);
     c->scan_tmp_cap = 0;
     if (dev_alloc(c, (void**)&c->scan_tmp, nb * 4)) return PMX_ENOMEM;
     c->scan_tmp_cap = nb;
   }
+  return PMX_OK;
+}
+
+int pmx_scan_exclusive(pmx_ctx* c, const uint32_t* in, uint32_t* out, long long n,
+                       const uint32_t* cond) {
+  long long nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (nb < 1) nb = 1;
+  if (int rc = pmx_ensure_scan(c, n)) return rc;
   scan_reduce_k<<<(unsigned)nb, PMX_TPB, 0, c->stream>>>(in, n, c->scan_tmp, cond);
   scan_single_k<<<1, PMX_TPB, 0, c->stream>>>(c->scan_tmp, nb, cond);
   scan_apply_k<<<(unsigned)nb, PMX_TPB, 0, c->stream>>>(in, n, c->scan_tmp, out, cond);
@@ -639,6 +687,7 @@ int pmx_red_reduce_to(pmx_ctx* c, long long off, long long nblk, double* dst) {
 
 int pmx_red_fetch(pmx_ctx* c, int k, double* out) {
   PMX_REQUIRE(k > 0 && k <= PMX_RED_SLOTS, "pmx_red_fetch: bad count");
+  if (int rc = pmx_no_capture(c, "host value of a dot product")) return rc;
   PMX_CHECK_HIP(hipMemcpyAsync(c->red_host, c->red_out, (size_t)k * 8, hipMemcpyDeviceToHost,
                                c->stream));
   PMX_CHECK_HIP(hipStreamSynchronize(c->stream));
@@ -653,6 +702,7 @@ int pmx_red_finish(pmx_ctx* c, long long nblk, double* out) {
 }
 
 extern "C" int pmx_dot(pmx_ctx* c, const float* a, const float* b, long long n, double* out) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && a && b && out && n >= 0, "pmx_dot: bad argument");
   int nblk = 512;
   dot_k<<<nblk, PMX_TPB, 0, c->stream>>>(a, b, n, c->red);
@@ -734,6 +784,7 @@ extern "C" int pmx_timing_read(pmx_ctx* c, double* ms, long long* count) {
     count[i] = 0;
   }
   if (!c->timer) return PMX_OK;
+  if (int rc = pmx_no_capture(c, "pmx_timing_read")) return rc;
   PMX_CHECK_HIP(hipStreamSynchronize(c->stream));
   PmxTimer* t = c->timer;
   for (auto& r : t->recs) {

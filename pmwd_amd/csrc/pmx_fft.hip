@@ -1414,6 +1414,7 @@ int pmx_copy_rows(pmx_ctx* c, float* dst, int dst_rs, const float* src, int src_
 }
 
 extern "C" int pmx_rfftn(pmx_ctx* c, const float* in, float* out_c) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && in && out_c, "pmx_rfftn: NULL argument");
   int rc;
   float2* f[3] = {c->cbuf[0], c->cbuf[1], c->cbuf[2]};
@@ -1425,6 +1426,7 @@ extern "C" int pmx_rfftn(pmx_ctx* c, const float* in, float* out_c) {
 }
 
 extern "C" int pmx_irfftn(pmx_ctx* c, const float* in_c, float* out) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && in_c && out, "pmx_irfftn: NULL argument");
   int rc;
   float2* f[3] = {c->cbuf[0], c->cbuf[1], c->cbuf[2]};
@@ -1436,6 +1438,7 @@ extern "C" int pmx_irfftn(pmx_ctx* c, const float* in_c, float* out) {
 }
 
 extern "C" int pmx_poisson_grad(pmx_ctx* c, const float* dens, float coef, float* grad) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && dens && grad, "pmx_poisson_grad: NULL argument");
   int rc;
   if ((rc = pmx_copy_rows(c, (float*)c->cbuf[0], c->rs, dens, c->n[2]))) return rc;
@@ -1444,4 +1447,16 @@ extern "C" int pmx_poisson_grad(pmx_ctx* c, const float* dens, float coef, float
     if ((rc = pmx_copy_rows(c, grad + (long long)d * c->M, c->n[2], (const float*)c->cbuf[d], c->rs)))
       return rc;
   return PMX_OK;
+}
+
+extern "C" int pmx_poisson_grad_adj(pmx_ctx* c, const float* grad_cot, float coef, float* dens_cot) {
+  PMX_ENTRY(c);
+  PMX_REQUIRE(c && grad_cot && dens_cot, "pmx_poisson_grad_adj: NULL argument");
+  int rc;
+  for (int d = 0; d < 3; ++d)
+    if ((rc = pmx_copy_rows(c, (float*)c->cbuf[d], c->rs, grad_cot + (long long)d * c->M, c->n[2])))
+      return rc;
+  // dens_cot = -coef * sum_d L_d(grad_cot_d), L_d = irfftn (-ik_d) laplace rfftn
+  if ((rc = pmx_poisson_adj_ctx(c, (float)(-(double)coef / (double)c->M), 0))) return rc;
+  return pmx_copy_rows(c, dens_cot, c->n[2], (const float*)c->cbuf[0], c->rs);
 }

@@ -20,6 +20,7 @@ struct GenGeom {
   float doff[3];  // float32(d12)
   int s2[3];      // target shape (indices outside are dropped)
   float gcell;    // gradient divisor (a2 if given, else a1), float32
+  int wrap;       // 1: wrap around the periodic s1 (enmesh's s1 given)
 };
 
 // JAX float divmod (lax.rem + sign fix, round of the quotient)
@@ -58,7 +59,8 @@ __device__ __forceinline__ GenAxis gen_axis(int pm, float dp, const GenGeom& G, 
       float d2 = d - (float)(i + nb);
       r.f[nb] = 1.f - fabsf(d2);
       r.s[nb] = pmx_sign(-d2);
-      r.idx[nb] = pmx_wrap(i + nb + i1, G.s1[a]);
+      // int16 sum, as enmesh's index dtype
+      r.idx[nb] = G.wrap ? pmx_wrap(i + nb + i1, G.s1[a]) : (int)(short)(i + nb + i1);
     }
   } else {
     double P = (double)pm * G.a1 + (double)dp - G.b12[a];
@@ -66,11 +68,13 @@ __device__ __forceinline__ GenAxis gen_axis(int pm, float dp, const GenGeom& G, 
     for (int nb = 0; nb < 2; ++nb) {
       double x = P + (double)nb * G.a2;
       double q, m;
-      fdivmod(x, L, &q, &m);
-      x = m;
+      if (G.wrap) {
+        fdivmod(x, L, &q, &m);
+        x = m;
+      }
       fdivmod(x, G.a2, &q, &m);
       double d2 = P - q * G.a2;
-      d2 -= rint(d2 / L) * L;
+      if (G.wrap) d2 -= rint(d2 / L) * L;
       int ii = (int)(short)(long long)q;
       float d2f = (float)d2;
       d2f = d2f / G.a2f;
@@ -298,6 +302,7 @@ static int make_gen(const pmx_cic_geom* g, GenGeom* G) {
   G->a2 = G->use_a2 ? g->out_cell_size : 0.0;
   G->a2f = (float)G->a2;
   G->gcell = G->use_a2 ? G->a2f : G->a1f;
+  G->wrap = 1;
   for (int a = 0; a < 3; ++a) {
     double q, m;
     h_fdivmod(G->b12[a], (double)G->a1f, &q, &m);
@@ -310,6 +315,7 @@ static int make_gen(const pmx_cic_geom* g, GenGeom* G) {
 extern "C" int pmx_scatter_cic(pmx_ctx* c, const pmx_cic_geom* g, const int16_t* pmid,
                                const float* disp, long long n, const float* val, float val_scalar,
                                int nchan, float* mesh) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0 && nchan >= 1 && mesh, "pmx_scatter_cic: bad argument");
   GenGeom G;
   int rc = make_gen(g, &G);
@@ -325,6 +331,7 @@ extern "C" int pmx_scatter_cic(pmx_ctx* c, const pmx_cic_geom* g, const int16_t*
 extern "C" int pmx_gather_cic(pmx_ctx* c, const pmx_cic_geom* g, const int16_t* pmid,
                               const float* disp, long long n, const float* mesh, int nchan,
                               const float* val, float val_scalar, float* out) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0 && nchan >= 1, "pmx_gather_cic: bad argument");
   GenGeom G;
   int rc = make_gen(g, &G);
@@ -341,6 +348,7 @@ extern "C" int pmx_scatter_cic_adj(pmx_ctx* c, const pmx_cic_geom* g, const int1
                                    const float* disp, long long n, const float* mesh_cot,
                                    const float* val, float val_scalar, int nchan, float* disp_cot,
                                    float* val_cot) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0 && nchan >= 1, "pmx_scatter_cic_adj: bad argument");
   GenGeom G;
   int rc = make_gen(g, &G);
@@ -357,6 +365,7 @@ extern "C" int pmx_gather_cic_adj(pmx_ctx* c, const pmx_cic_geom* g, const int16
                                   const float* disp, long long n, const float* mesh,
                                   const float* val_cot, int nchan, float* disp_cot,
                                   float* mesh_cot) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0 && nchan >= 1, "pmx_gather_cic_adj: bad argument");
   GenGeom G;
   int rc = make_gen(g, &G);
@@ -365,6 +374,98 @@ extern "C" int pmx_gather_cic_adj(pmx_ctx* c, const pmx_cic_geom* g, const int16
   PMX_REQUIRE(pmid && disp && mesh && val_cot, "pmx_gather_cic_adj: NULL pointer");
   gen_gather_adj_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(G, pmid, disp, n, mesh, val_cot, nchan,
                                                              disp_cot, mesh_cot);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+// enmesh itself (pm_util.py:33-156): indices (n, 2^dim, dim) of the index
+// dtype int16, fractions (n, 2^dim) and their gradients (n, 2^dim, dim).
+// Without wrap, negative indices are replaced by the target extent when drop
+// is set (pm_util.py:138-141), so that a later .at[] drops them.
+__global__ void gen_enmesh_k(GenGeom G, int drop, const int16_t* __restrict__ pmid,
+                             const float* __restrict__ disp, long long n, int16_t* __restrict__ ind,
+                             float* __restrict__ frac, float* __restrict__ frac_grad) {
+  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  if (i >= n) return;
+  int pm[3];
+  float dp[3];
+  gen_load(pmid, disp, i, G.dim, pm, dp);
+  GenAxis ax[3];
+  for (int a = 0; a < 3; ++a) ax[a] = gen_axis(pm[a], dp[a], G, a);
+  const int nn = 1 << G.dim;
+  for (int j = 0; j < nn; ++j) {
+    for (int a = 0; a < G.dim; ++a) {
+      int v = ax[a].idx[(j >> a) & 1];
+      if (!G.wrap && drop && v < 0) v = G.s2[a];
+      ind[(i * nn + j) * G.dim + a] = (int16_t)v;
+    }
+    frac[i * nn + j] = gen_w(ax, j, G.dim);
+    if (frac_grad) {
+      float g[3];
+      gen_wgrad(ax, j, G.dim, g);
+      for (int a = 0; a < G.dim; ++a) frac_grad[(i * nn + j) * G.dim + a] = g[a];
+    }
+  }
+}
+
+extern "C" int pmx_enmesh(pmx_ctx* c, const pmx_cic_geom* g, int wrap, int drop,
+                          const int16_t* pmid, const float* disp, long long n, int16_t* ind,
+                          float* frac, float* frac_grad) {
+  PMX_ENTRY(c);
+  PMX_REQUIRE(c && n >= 0, "pmx_enmesh: bad argument");
+  GenGeom G;
+  int rc = make_gen(g, &G);
+  if (rc) return rc;
+  G.wrap = wrap != 0;
+  if (n == 0) return PMX_OK;
+  PMX_REQUIRE(pmid && disp && ind && frac, "pmx_enmesh: NULL pointer");
+  gen_enmesh_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(G, drop != 0, pmid, disp, n, ind, frac,
+                                                         frac_grad);
+  PMX_LAUNCH_CHECK();
+  return PMX_OK;
+}
+
+// Three-component readout of the default-argument CIC (gather.py:33-77 with
+// val=0, offset=0, cell_size=None) from component-major meshes
+// (3, n0, n1, n2): acc[i, d] = sum_j mesh_d[c_j] w_j, the same sum order as
+// three pmx_gather_cic calls.
+__global__ void gen_gather3_k(GenGeom G, const int16_t* __restrict__ pmid,
+                              const float* __restrict__ disp, long long n,
+                              const float* __restrict__ mesh3, long long M,
+                              float* __restrict__ acc) {
+  long long i = (long long)blockIdx.x * PMX_TPB + threadIdx.x;
+  if (i >= n) return;
+  int pm[3];
+  float dp[3];
+  gen_load(pmid, disp, i, 3, pm, dp);
+  GenAxis ax[3];
+  for (int a = 0; a < 3; ++a) ax[a] = gen_axis(pm[a], dp[a], G, a);
+  long long cs[8];
+  float ws[8];
+  for (int j = 0; j < 8; ++j) {
+    cs[j] = gen_cell(ax, j, G);
+    ws[j] = gen_w(ax, j, 3);
+  }
+  for (int d = 0; d < 3; ++d) {
+    float t[8];
+    for (int j = 0; j < 8; ++j) t[j] = mesh3[d * M + cs[j]] * ws[j];
+    acc[3 * i + d] = 0.f + gen_tree(t, 8);
+  }
+}
+
+extern "C" int pmx_gather_cic3(pmx_ctx* c, const int16_t* pmid, const float* disp, long long n,
+                               const float* mesh3, float* acc) {
+  PMX_ENTRY(c);
+  PMX_REQUIRE(c && n >= 0, "pmx_gather_cic3: bad argument");
+  if (n == 0) return PMX_OK;
+  PMX_REQUIRE(pmid && disp && mesh3 && acc, "pmx_gather_cic3: NULL pointer");
+  pmx_cic_geom g = {};
+  for (int a = 0; a < 3; ++a) g.mesh_shape[a] = g.out_shape[a] = c->n[a];
+  g.cell_size = c->cell;
+  GenGeom G;
+  int rc = make_gen(&g, &G);
+  if (rc) return rc;
+  gen_gather3_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(G, pmid, disp, n, mesh3, c->M, acc);
   PMX_LAUNCH_CHECK();
   return PMX_OK;
 }

@@ -142,9 +142,12 @@ struct pmx_ctx {
   pmx_alloc_fn dalloc;
   pmx_free_fn dfree;
   void* dalloc_user;
-  // recorded on the old stream when the context moves to another one, which
-  // then waits for it (NULL until the first move)
+  // recorded on the context's stream as each entry point returns (not under
+  // graph capture); a stream the context is rebound to waits for it, so the
+  // previous stream's handle is never touched again (moved_valid: the last
+  // record follows every call so far)
   hipEvent_t moved;
+  int moved_valid;
 
   // k tables (float32 as pmwd: fftfreq(...).astype(f32)); *_m has the Nyquist
   // bin zeroed (gravity.py:38-42)
@@ -220,6 +223,24 @@ struct pmx_ctx {
 int pmx_ensure_ptcl(pmx_ctx* c, long long n);
 int pmx_ensure_vrec(pmx_ctx* c);
 int pmx_ensure_red(pmx_ctx* c, long long nblk);
+
+// Whether the context's stream is under graph capture (1), not (0).
+int pmx_capturing(pmx_ctx* c);
+// PMX_EINVAL with a message when the context's stream is under graph
+// capture: device allocations and host waits synchronise the stream, which a
+// capture forbids.  Callers that capture run each call once beforehand, so
+// that the context has grown to its size, and read host values outside.
+int pmx_no_capture(pmx_ctx* c, const char* what);
+// Records c->moved on the context's stream unless it is capturing.
+void pmx_mark_queued(pmx_ctx* c);
+struct PmxQueued {
+  pmx_ctx* c;
+  explicit PmxQueued(pmx_ctx* c_) : c(c_) {}
+  ~PmxQueued() { pmx_mark_queued(c); }
+};
+// First statement of every entry point that enqueues work on the context.
+#define PMX_ENTRY(ctx) PmxQueued pmx_queued_guard_(ctx)
+int pmx_ensure_scan(pmx_ctx* c, long long n);
 // Device memory of a context through its allocator (pmx_set_allocator or
 // hipMalloc), counted in pmx_ctx_bytes; the free takes the allocated size.
 int pmx_dev_alloc(pmx_ctx* c, void** p, size_t bytes);

@@ -261,6 +261,7 @@ static void pl_launch(pmx_ctx* c, int src, PlOut out, PlUser user, float k2, flo
 extern "C" int pmx_nbody_steps(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, float* acc,
                                long long n, float val, float coef, int nsteps, const float* k1,
                                const float* d, const float* k2) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0 && nsteps >= 0, "pmx_nbody_steps: bad argument");
   PMX_REQUIRE(n == 0 || (pmid && disp && vel && acc), "pmx_nbody_steps: NULL pointer");
   PMX_REQUIRE(nsteps == 0 || (k1 && d && k2), "pmx_nbody_steps: NULL factor array");
@@ -275,8 +276,11 @@ extern "C" int pmx_nbody_steps(pmx_ctx* c, const int16_t* pmid, float* disp, flo
         return rc;
     return PMX_OK;
   }
+  // every allocation before the first kernel writes disp and vel, so that
+  // PMX_ENOMEM leaves the caller's arrays as they were
   if ((rc = pmx_ensure_ptcl(c, n))) return rc;
   if ((rc = pmx_ensure_state(c))) return rc;
+  if ((rc = pmx_ensure_scan(c, c->ntiles + 1))) return rc;
   PMX_REQUIRE(c->rec_cap < (1LL << 32) - 1, "pmx_nbody_steps: too many particles");
   const MeshGeom g = pmx_geom_padded(c);
   const long long nt = c->ntiles;
@@ -637,6 +641,7 @@ extern "C" int pmx_nbody_steps_adj(pmx_ctx* c, const int16_t* pmid, float* disp,
                                    long long n, float val, float coef, float kp, int nsteps,
                                    const float* ka, const float* d, const float* kb,
                                    const float* disp_snap, double* dots) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0 && nsteps >= 0, "pmx_nbody_steps_adj: bad argument");
   PMX_REQUIRE(n == 0 || (pmid && disp && vel && acc && disp_cot && vel_cot && acc_cot),
               "pmx_nbody_steps_adj: NULL pointer");
@@ -644,6 +649,8 @@ extern "C" int pmx_nbody_steps_adj(pmx_ctx* c, const int16_t* pmid, float* disp,
   PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_nbody_steps_adj: too many particles");
   if (nsteps == 0) return PMX_OK;
   int rc;
+  // the dot products are read back on the host
+  if (dots && (rc = pmx_no_capture(c, "pmx_nbody_steps_adj with dots"))) return rc;
   if (n == 0) {
     if (dots)
       for (int s = 0; s < 2 * nsteps; ++s) dots[s] = 0;
@@ -667,12 +674,17 @@ extern "C" int pmx_nbody_steps_adj(pmx_ctx* c, const int16_t* pmid, float* disp,
     }
     return PMX_OK;
   }
+  // every allocation (the three-channel halo shells of the cotangent
+  // deposits and the scan scratch included) before the first kernel writes
+  // the caller's arrays, so that PMX_ENOMEM leaves them as they were
   if ((rc = pmx_ensure_ptcl(c, n))) return rc;
   if ((rc = pmx_ensure_adj_state(c, nsteps))) return rc;
   PMX_REQUIRE(c->rec_cap < (1LL << 32) - 1, "pmx_nbody_steps_adj: too many particles");
   const long long nt = c->ntiles;
   const unsigned nb1 = pmx_blocks(n);
   if ((rc = pmx_ensure_red(c, nb1 > 2 * nt ? (long long)nb1 : 2 * nt))) return rc;
+  if ((rc = pmx_ensure_halo(c, 3))) return rc;
+  if ((rc = pmx_ensure_scan(c, nt + 1))) return rc;
   const MeshGeom g = pmx_geom_padded(c);
   const float adj_scale = (float)(-(double)coef / (double)c->M);
   uint32_t* vmax_u = reinterpret_cast<uint32_t*>(c->vmax);

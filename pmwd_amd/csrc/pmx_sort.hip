@@ -81,6 +81,7 @@ __global__ void max_u32_k(const uint32_t* __restrict__ k, long long n, uint32_t*
 
 static int ensure_sort(pmx_ctx* c, long long n, long long nhist) {
   if (n > c->sort_cap) {
+    if (int rc = pmx_no_capture(c, "sort scratch")) return rc;
     for (int b = 0; b < 2; ++b) {
       pmx_dev_free(c, (void**)&c->sort_k[b], (size_t)c->sort_cap * 4);
       pmx_dev_free(c, (void**)&c->sort_v[b], (size_t)c->sort_cap * 4);
@@ -101,6 +102,7 @@ static int ensure_sort(pmx_ctx* c, long long n, long long nhist) {
     c->sort_cap = n;
   }
   if (nhist > c->sort_hist_cap) {
+    if (int rc = pmx_no_capture(c, "sort histograms")) return rc;
     pmx_dev_free(c, (void**)&c->sort_hist, (size_t)c->sort_hist_cap * 8);
     c->sort_hist_cap = 0;
     if (pmx_dev_alloc(c, (void**)&c->sort_hist, (size_t)nhist * 8)) return PMX_ENOMEM;
@@ -110,6 +112,7 @@ static int ensure_sort(pmx_ctx* c, long long n, long long nhist) {
 }
 
 extern "C" int pmx_sort_perm(pmx_ctx* c, const uint32_t* key, long long n, uint32_t* perm) {
+  PMX_ENTRY(c);
   PMX_REQUIRE(c && n >= 0, "pmx_sort_perm: bad argument");
   if (n == 0) return PMX_OK;
   PMX_REQUIRE(key && perm, "pmx_sort_perm: NULL pointer");
@@ -117,12 +120,14 @@ extern "C" int pmx_sort_perm(pmx_ctx* c, const uint32_t* key, long long n, uint3
   long long nblocks = (n + RS_TILE - 1) / RS_TILE;
   long long nhist = 256 * nblocks;
   int rc;
+  // the digit count is read back on the host
+  if ((rc = pmx_no_capture(c, "pmx_sort_perm"))) return rc;
   if ((rc = ensure_sort(c, n, nhist))) return rc;
   uint32_t* hist = c->sort_hist;
   uint32_t* hscan = c->sort_hist + nhist;
   // number of digit passes from the largest key
   uint32_t* dmax = hscan;  // scratch word, overwritten by the first scan
-  PMX_CHECK_HIP(hipMemsetAsync(dmax, 0, 4, c->stream));
+  if ((rc = pmx_fill_u32(c, dmax, 1, 0u))) return rc;
   max_u32_k<<<256, PMX_TPB, 0, c->stream>>>(key, n, dmax);
   PMX_LAUNCH_CHECK();
   uint32_t kmax = 0;

@@ -32,6 +32,9 @@ class _GatherFn(torch.autograd.Function):
         ctx.save_for_backward(disp_c, mesh_c)
         ctx.pmid, ctx.conf, ctx.offset, ctx.cell_size = pmid, conf, offset, cell_size
         ctx.nchan, ctx.sshape, ctx.has_val = nchan, sshape, val is not None
+        # the backward uses this context too: autograd may run it on another
+        # thread, where context() would not see the caller's context provider
+        ctx.pmx = c
         return out
 
     @staticmethod
@@ -43,7 +46,7 @@ class _GatherFn(torch.autograd.Function):
         disp_cot = torch.empty_like(disp) if ctx.needs_input_grad[0] else None
         mesh_cot = torch.zeros_like(mesh) if ctx.needs_input_grad[1] else None
         if disp_cot is not None or mesh_cot is not None:
-            c = _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
+            c = ctx.pmx
             g = cic_geom(conf, ctx.sshape, ctx.offset, ctx.cell_size)
             _lib.check(_lib.lib().pmx_gather_cic_adj(
                 c.bind(), g, _lib.ptr(ctx.pmid), _lib.ptr(disp), n, _lib.ptr(mesh),

@@ -55,6 +55,9 @@ class _GravityFn(torch.autograd.Function):
                                           _lib.ptr(acc)))
         ctx.save_for_backward(disp_c)
         ctx.pmid, ctx.conf, ctx.val, ctx.coef = pmid, conf, val, coef
+        # the backward uses this context too: autograd may run it on another
+        # thread, where context() would not see the caller's context provider
+        ctx.pmx = c
         return acc
 
     @staticmethod
@@ -64,7 +67,7 @@ class _GravityFn(torch.autograd.Function):
         n = ctx.pmid.shape[0]
         acc = torch.empty_like(disp)
         disp_cot = torch.empty_like(disp)
-        c = _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
+        c = ctx.pmx
         _lib.check(_lib.lib().pmx_gravity_vjp(c.bind(), _lib.ptr(ctx.pmid), _lib.ptr(disp), n,
                                               ctx.val, ctx.coef, _lib.ptr(acc_cot.contiguous()),
                                               _lib.ptr(acc), _lib.ptr(disp_cot), None))

@@ -11,9 +11,10 @@ splittings run the ops one by one.
 The adjoint reconstructs the trajectory backward in time from the final
 state as pmwd's ``nbody_adj`` (nbody.py:226-260) does, optionally snapping
 the reconstruction to sparse forward checkpoints (every ``checkpoint_every``
-steps) to bound the float32 reversal drift.  The cosmology cotangent is for
-Omega_m: time-step factor sensitivities from the growth ODE's forward
-sensitivity, and the force term sum(vel_cot*acc)/Omega_m.
+steps) to bound the float32 reversal drift.  The cosmology cotangent holds
+Omega_m (time-step factor sensitivities from the growth ODE's forward
+sensitivity, and the force term sum(vel_cot*acc)/Omega_m) and, when set,
+Omega_k_, w_0_ and w_a_, which enter only the time-step factors.
 """
 import ctypes
 
@@ -21,7 +22,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .boltzmann import growth, growth_dOm
+from .boltzmann import boltzmann, growth, growth_dOm
 from .cosmology import E2, H_deriv, E2_dOm, H_deriv_dOm
 from .gravity import omega_coef
 
@@ -85,6 +86,59 @@ def _f32(x):
     return float(np.float32(x))
 
 
+# parameters besides Omega_m that enter the step factors (through E2 and
+# H_deriv, cosmology.py:185-242, and the growth table); the others (A_s, n_s,
+# Omega_b, h) do not enter the N-body step
+_DE_PARAMS = ('Omega_k_', 'w_0_', 'w_a_')
+_DE_STEP = 1e-6
+
+
+class _CosmoCot:
+    """Cosmology cotangent of the reverse-time adjoint (nbody.py:49-99).
+
+    Each kick and drift adjoint subtracts d(factor)/d(theta) times its dot
+    product, and each kick adds the force term: acc is proportional to
+    Omega_m, so the force VJP's Omega_m cotangent is sum(vel_cot*acc)/Omega_m
+    (gravity.py:54), and no other parameter enters the force.  The factor
+    derivatives are total derivatives, growth table included: Omega_m's from
+    the growth ODE's forward sensitivities; Omega_k_, w_0_ and w_a_ (only
+    those set, not None) by a central difference of the factors over growth
+    tables integrated at rtol = atol = 1e-12.
+    """
+
+    def __init__(self, cosmo, conf):
+        self.cosmo, self.conf = cosmo, conf
+        self.cot = {'Omega_m': 0.0}
+        self.pert = {}
+        tight = conf.replace(growth_rtol=1e-12, growth_atol=1e-12)
+        for p in _DE_PARAMS:
+            v = getattr(cosmo, p)
+            if v is None:
+                continue
+            cs = [boltzmann(cosmo.replace(**{p: v + s * _DE_STEP}, transfer=None, growth=None,
+                                          growth_dOm=None, varlin=None),
+                            tight, transfer=False, varlin=False) for s in (1, -1)]
+            self.pert[p] = (cs, tight)
+            self.cot[p] = 0.0
+
+    def _d(self, fn, args, p):
+        (cp, cm), tight = self.pert[p]
+        return (fn(*args, cp, tight) - fn(*args, cm, tight)) / (2 * _DE_STEP)
+
+    def kick(self, args, dot, cf, f):
+        """kick_adj (nbody.py:80-99): dot = sum(vel_cot*acc), cf the force
+        VJP's Omega_m cotangent, f the float32 factor."""
+        self.cot['Omega_m'] -= kick_factor_dOm(*args, self.cosmo, self.conf) * dot + cf * f
+        for p in self.pert:
+            self.cot[p] -= self._d(kick_factor, args, p) * dot
+
+    def drift(self, args, dot):
+        """drift_adj (nbody.py:49-67): dot = sum(disp_cot*vel)."""
+        self.cot['Omega_m'] -= drift_factor_dOm(*args, self.cosmo, self.conf) * dot
+        for p in self.pert:
+            self.cot[p] -= self._d(drift_factor, args, p) * dot
+
+
 def _is_kdk(conf):
     return tuple(tuple(s) for s in conf.symp_splits) == ((0, 0.5), (1, 0.5))
 
@@ -145,17 +199,18 @@ def _kdk_factors(a_prev, a_next, cosmo, conf):
 
 
 def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=None,
-                 checkpoint_device=None, pipeline=True):
+                 checkpoint_device=None, pipeline=True, pctx=None):
     """Integrates in place; returns acc and the checkpoints {step: (disp, vel)}.
 
     KDK steps between checkpoints run as one ``pmx_nbody_steps`` call (the
     particles kept in tile order on the device between steps), or one
     ``pmx_nbody_step`` call per step when ``pipeline`` is False; both give
-    the same bits.
+    the same bits.  ``pctx`` is the library context to run on (default: the
+    one ``_lib.context`` gives this thread).
     """
     L = _lib.lib()
     n = pmid.shape[0]
-    c = _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
+    c = pctx or _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
     h = c.bind()
     val, coef = _mass_val(conf), omega_coef(cosmo, conf)
     acc = torch.empty_like(disp)
@@ -225,7 +280,7 @@ def _adj_pipeline_fits(n, c, device, extra=0):
 
 
 def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False, ckpt=None,
-                 fused=True, pipeline=None):
+                 fused=True, pipeline=None, cot_out=None, pctx=None):
     """Reverse-time adjoint in place (nbody.py:143-162, 226-260).
 
     On return disp/vel hold the reconstructed initial state and
@@ -240,7 +295,7 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
     """
     L = _lib.lib()
     n = pmid.shape[0]
-    c = _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
+    c = pctx or _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
     kdk = fused and _is_kdk(conf)
     auto = pipeline is None
     if auto:
@@ -266,7 +321,7 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
     nsteps = len(a_nbody) - 1
     cf = force_adj()
     dot_force = dot.value
-    om_cot = 0.0
+    cc = _CosmoCot(cosmo, conf)
     dots = (ctypes.c_double * 2)()
     kp = 0.0  # closing kick of the previous KDK step, applied by the next call
     rev = list(zip(a_nbody[:0:-1], a_nbody[-2::-1]))
@@ -303,11 +358,11 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
                 break
             _lib.check(rc)
             for j, (ka_args, d_args, kb_args) in enumerate(args):
-                om_cot -= kick_factor_dOm(*ka_args, cosmo, conf) * dot_force + cf * float(fa[j])
-                om_cot -= drift_factor_dOm(*d_args, cosmo, conf) * dd[2 * j]
+                cc.kick(ka_args, dot_force, cf, float(fa[j]))
+                cc.drift(d_args, dd[2 * j])
                 dot_force = dd[2 * j + 1]
                 cf = dot_force / Om
-                om_cot -= kick_factor_dOm(*kb_args, cosmo, conf) * dot_force + cf * float(fb[j])
+                cc.kick(kb_args, dot_force, cf, float(fb[j]))
             kp = float(fb[-1])
             if ckpt and step_e in ckpt:
                 _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
@@ -317,7 +372,9 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
     if kdk and pipeline:
         if kp != 0.0:
             _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
-        return acc, acc_cot, om_cot
+        if cot_out is not None:
+            cot_out.update(cc.cot)
+        return acc, acc_cot, cc.cot['Omega_m']
     for s, (a_prev, a_next) in enumerate(rev):
         step = nsteps - 1 - s  # index of the forward step being undone
         if kdk:
@@ -336,11 +393,11 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
                 h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel), _lib.ptr(acc),
                 _lib.ptr(disp_cot), _lib.ptr(vel_cot), _lib.ptr(acc_cot), n, val, coef,
                 kp, fa, fd, _lib.ptr(snap), dots))
-            om_cot -= kick_factor_dOm(*ka_args, cosmo, conf) * dot_force + cf * fa
-            om_cot -= drift_factor_dOm(*d_args, cosmo, conf) * dots[0]
+            cc.kick(ka_args, dot_force, cf, fa)
+            cc.drift(d_args, dots[0])
             dot_force = dots[1]
             cf = dot_force / Om
-            om_cot -= kick_factor_dOm(*kb_args, cosmo, conf) * dot_force + cf * fb
+            cc.kick(kb_args, dot_force, cf, fb)
             kp = fb
             if ckpt and step in ckpt:
                 _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
@@ -355,22 +412,20 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
                 a_vel_next = a_prev * (1 - K) + a_next * K
                 args = (a_acc, a_vel, a_vel_next)
                 f = _f32(kick_factor(*args, cosmo, conf))
-                dfdO = kick_factor_dOm(*args, cosmo, conf)
                 _lib.check(L.pmx_kick_adj(h, n, _lib.ptr(vel), _lib.ptr(acc), _lib.ptr(disp_cot),
                                           _lib.ptr(acc_cot), _lib.ptr(vel_cot), f,
                                           ctypes.byref(dot)))
-                om_cot -= dfdO * dot.value + cf * f
+                cc.kick(args, dot.value, cf, f)
                 a_vel = a_vel_next
             if d != 0:
                 D += d
                 a_disp_next = a_prev * (1 - D) + a_next * D
                 args = (a_vel, a_disp, a_disp_next)
                 f = _f32(drift_factor(*args, cosmo, conf))
-                dfdO = drift_factor_dOm(*args, cosmo, conf)
                 _lib.check(L.pmx_drift_adj(h, n, _lib.ptr(disp), _lib.ptr(vel),
                                            _lib.ptr(disp_cot), _lib.ptr(vel_cot), f,
                                            ctypes.byref(dot)))
-                om_cot -= dfdO * dot.value
+                cc.drift(args, dot.value)
                 a_disp = a_disp_next
                 if ckpt and step in ckpt and D == 1:
                     disp.copy_(ckpt[step][0].to(disp.device))
@@ -380,7 +435,9 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
             vel.copy_(ckpt[step][1].to(vel.device))
     if kp != 0.0:
         _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
-    return acc, acc_cot, om_cot
+    if cot_out is not None:
+        cot_out.update(cc.cot)
+    return acc, acc_cot, cc.cot['Omega_m']
 
 
 def _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, k):
@@ -394,8 +451,11 @@ class _NBodyFn(torch.autograd.Function):
                 checkpoint_device):
         disp_f = disp.detach().clone().contiguous()
         vel_f = vel.detach().clone().contiguous()
+        # the backward runs on the same library context: autograd may call it
+        # on another thread, where the caller's context provider is not in scope
+        ctx.pmx = _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
         acc, ckpt = _run_forward(pmid, disp_f, vel_f, cosmo, conf, reverse, checkpoint_every,
-                                 checkpoint_device)
+                                 checkpoint_device, pctx=ctx.pmx)
         ctx.save_for_backward(disp_f, vel_f)
         ctx.pmid, ctx.cosmo, ctx.conf, ctx.reverse, ctx.ckpt = pmid, cosmo, conf, reverse, ckpt
         ctx.mark_non_differentiable(acc)
@@ -408,7 +468,7 @@ class _NBodyFn(torch.autograd.Function):
         pipeline = None
         if _is_kdk(conf):
             n = ctx.pmid.shape[0]
-            c = _lib.context(conf.mesh_shape, conf.cell_size, disp_f.device)
+            c = ctx.pmx
             # four working copies plus acc and acc_cot: 72 B per particle
             if not _adj_pipeline_fits(n, c, disp_f.device, extra=72 * n):
                 # per-step reverse calls: the forward pipeline's copy goes back
@@ -420,7 +480,7 @@ class _NBodyFn(torch.autograd.Function):
         dc = (disp_cot if disp_cot is not None else torch.zeros_like(disp)).clone().contiguous()
         vc = (vel_cot if vel_cot is not None else torch.zeros_like(vel)).clone().contiguous()
         _, _, om_cot = _run_adjoint(ctx.pmid, disp, vel, dc, vc, ctx.cosmo, conf,
-                                    ctx.reverse, ctx.ckpt, pipeline=pipeline)
+                                    ctx.reverse, ctx.ckpt, pipeline=pipeline, pctx=ctx.pmx)
         om_grad = None
         if ctx.needs_input_grad[2]:
             om_grad = torch.tensor(om_cot, dtype=torch.float64)
@@ -463,16 +523,22 @@ def nbody_adj(ptcl, ptcl_cot, cosmo, conf, reverse=False):
     ``ptcl`` is the final state, ``ptcl_cot`` its cotangent (disp, vel).
     Returns ``(ptcl0, ptcl_cot0, cosmo_cot)``: the reconstructed initial
     state, the initial cotangents (with ``acc`` holding the last force VJP,
-    as pmwd returns it) and a ``Cosmology`` cotangent whose Omega_m entry is
-    set (the other parameters do not enter the N-body step).
+    as pmwd returns it) and a ``Cosmology`` cotangent.  Its Omega_m entry,
+    and the Omega_k_, w_0_ and w_a_ entries of the parameters that are set,
+    are total derivatives (the growth table's dependence included); A_s_1e9,
+    n_s, Omega_b and h do not enter the N-body step and are 0; unset
+    parameters stay None, as in pmwd's cotangent pytree.
     """
     pmid = ptcl.pmid.contiguous()
     disp = ptcl.disp.detach().clone().contiguous()
     vel = ptcl.vel.detach().clone().contiguous()
     dc = ptcl_cot.disp.detach().clone().contiguous()
     vc = ptcl_cot.vel.detach().clone().contiguous()
-    acc, acc_cot, om_cot = _run_adjoint(pmid, disp, vel, dc, vc, cosmo, conf, reverse)
+    cot = {}
+    acc, acc_cot, om_cot = _run_adjoint(pmid, disp, vel, dc, vc, cosmo, conf, reverse,
+                                        cot_out=cot)
+    de = {p: (cot[p] if getattr(cosmo, p) is not None else None) for p in _DE_PARAMS}
     cosmo_cot = cosmo.replace(A_s_1e9=0.0, n_s=0.0, Omega_m=om_cot, Omega_b=0.0, h=0.0,
-                              transfer=None, growth=None, growth_dOm=None, varlin=None)
+                              transfer=None, growth=None, growth_dOm=None, varlin=None, **de)
     return (ptcl.replace(disp=disp, vel=vel, acc=acc),
             ptcl_cot.replace(disp=dc, vel=vc, acc=acc_cot), cosmo_cot)

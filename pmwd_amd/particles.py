@@ -3,6 +3,7 @@
 ``pmid`` is (N, dim) int16, ``disp``/``vel``/``acc`` are (N, dim) float32,
 row-major as in pmwd, so the HIP kernels read them in place.
 """
+import ctypes
 from dataclasses import dataclass, field, replace as _dc_replace
 from typing import Any, Optional
 
@@ -112,6 +113,46 @@ class Particles:
         if wrap:
             pos = torch.remainder(pos, torch.tensor(conf.box_size, dtype=dtype, device=pos.device))
         return pos
+
+
+def ptcl_enmesh(ptcl, conf, offset=0, cell_size=None, mesh_shape=None, wrap=True, drop=True,
+                grad=False):
+    """Multilinear mesh indices and fractions of particles (particles.py:212-254).
+
+    Returns ``ind`` (ptcl_num, 2**dim, dim) of ``conf.pmid_dtype``, ``frac``
+    (ptcl_num, 2**dim) and, with ``grad``, ``frac_grad`` (ptcl_num, 2**dim,
+    dim), computed on the device by ``pmx_enmesh`` with pmwd's enmesh
+    arithmetic (pm_util.py:33-156).  ``wrap`` wraps around the periodic
+    ``conf.mesh_shape``; without it, ``drop`` replaces negative indices by
+    the ``mesh_shape`` extent so that they fall outside the mesh.
+    """
+    from . import _lib
+    from .pm_util import cic_geom
+    _lib.require_device(ptcl.disp, 'ptcl.disp')
+    if mesh_shape is None:
+        mesh_shape = conf.mesh_shape
+    dim = conf.dim
+    n = len(ptcl)
+    dev = ptcl.disp.device
+    ind = torch.empty((n, 2**dim, dim), dtype=torch.int16, device=dev)
+    frac = torch.empty((n, 2**dim), dtype=torch.float32, device=dev)
+    fg = torch.empty((n, 2**dim, dim), dtype=torch.float32, device=dev) if grad else None
+    g = cic_geom(conf, mesh_shape, offset, cell_size)
+    c = _lib.context(conf.mesh_shape, conf.cell_size, dev)
+    pmid = ptcl.pmid.to(torch.int16).contiguous()
+    disp = ptcl.disp.to(torch.float32).contiguous()
+    _lib.check(_lib.lib().pmx_enmesh(c.bind(), ctypes.byref(g), int(bool(wrap)), int(bool(drop)),
+                                     _lib.ptr(pmid), _lib.ptr(disp), n, _lib.ptr(ind),
+                                     _lib.ptr(frac), _lib.ptr(fg)))
+    ind = ind.to(conf.pmid_dtype)
+    if grad:
+        return ind, frac, fg
+    return ind, frac
+
+
+def ptcl_pos(ptcl, conf, dtype=float, wrap=True):
+    """Removed from pmwd in favour of ``Particles.pos`` (particles.py:257-258)."""
+    raise RuntimeError('Deprecated and replaced by ptcl.pos')
 
 
 def ptcl_rpos(ptcl, ref, conf, wrap=True):

@@ -58,24 +58,24 @@ def _check_field(f, complex_):
         raise NotImplementedError('the HIP FFT implements float32 fields only')
 
 
-def _rfftn_raw(f):
+def _rfftn_raw(f, ctx=None):
     _check_field(f, False)
     f = f.contiguous()
     n0, n1, n2 = f.shape
     out = torch.empty((n0, n1, n2 // 2 + 1), dtype=torch.complex64, device=f.device)
-    ctx = _ctx_for(f.shape, f.device)
+    ctx = ctx or _ctx_for(f.shape, f.device)
     _lib.check(_lib.lib().pmx_rfftn(ctx.bind(), _lib.ptr(f), _lib.ptr(out)))
     return out
 
 
-def _irfftn_raw(f, shape):
+def _irfftn_raw(f, shape, ctx=None):
     _check_field(f, True)
     f = f.contiguous()
     n0, n1, n2 = shape
     if tuple(f.shape) != (n0, n1, n2 // 2 + 1):
         raise ValueError(f'complex field shape {tuple(f.shape)} does not match s={shape}')
     out = torch.empty((n0, n1, n2), dtype=torch.float32, device=f.device)
-    ctx = _ctx_for(shape, f.device)
+    ctx = ctx or _ctx_for(shape, f.device)
     _lib.check(_lib.lib().pmx_irfftn(ctx.bind(), _lib.ptr(f), _lib.ptr(out)))
     return out
 
@@ -104,26 +104,32 @@ def c2r_adjoint(y, shape, rfftn):
     return g
 
 
+# The backward of each transform runs on the context its forward used:
+# autograd may call it on another thread, where the caller's context
+# provider (serving's per-request pool) is not in scope.
+
 class _RFFTn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, f):
         ctx.shape = tuple(f.shape)
-        return _rfftn_raw(f)
+        ctx.pmx = _ctx_for(f.shape, f.device)
+        return _rfftn_raw(f, ctx.pmx)
 
     @staticmethod
     def backward(ctx, g):
-        return r2c_adjoint(g, ctx.shape, _irfftn_raw)
+        return r2c_adjoint(g, ctx.shape, lambda x, s: _irfftn_raw(x, s, ctx.pmx))
 
 
 class _IRFFTn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, f, shape):
         ctx.shape = shape
-        return _irfftn_raw(f, shape)
+        ctx.pmx = _ctx_for(shape, f.device)
+        return _irfftn_raw(f, shape, ctx.pmx)
 
     @staticmethod
     def backward(ctx, y):
-        return c2r_adjoint(y, ctx.shape, _rfftn_raw), None
+        return c2r_adjoint(y, ctx.shape, lambda x: _rfftn_raw(x, ctx.pmx)), None
 
 
 def fftfwd(f, shape=None, axes=None, norm=None):

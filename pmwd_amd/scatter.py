@@ -29,18 +29,20 @@ class _ScatterFn(torch.autograd.Function):
         out = mesh.detach().clone().contiguous()
         disp_c = disp.detach().contiguous()
         val_c = val.detach().contiguous() if val is not None else None
+        # the backward uses this context too: autograd may run it on another
+        # thread, where context() would not see the caller's context provider
+        c = _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
         if val_c is None and nchan == 1 and is_default_cic(conf, sshape, offset, cell_size):
-            c = _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
             _lib.check(L.pmx_scatter_mass(c.bind(), _lib.ptr(pmid), _lib.ptr(disp_c), n,
                                           float(val_scalar), _lib.ptr(out), 1))
         else:
-            c = _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
             g = cic_geom(conf, sshape, offset, cell_size)
             _lib.check(L.pmx_scatter_cic(c.bind(), g, _lib.ptr(pmid), _lib.ptr(disp_c), n,
                                          _lib.ptr(val_c), float(val_scalar), nchan, _lib.ptr(out)))
         ctx.save_for_backward(disp_c, val_c if val_c is not None else None)
         ctx.pmid, ctx.conf, ctx.offset, ctx.cell_size = pmid, conf, offset, cell_size
         ctx.val_scalar, ctx.nchan, ctx.sshape = val_scalar, nchan, sshape
+        ctx.pmx = c
         return out
 
     @staticmethod
@@ -54,7 +56,7 @@ class _ScatterFn(torch.autograd.Function):
         if val is not None and ctx.needs_input_grad[2]:
             val_cot = torch.empty_like(val)
         if disp_cot is not None or val_cot is not None:
-            c = _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
+            c = ctx.pmx
             g = cic_geom(conf, ctx.sshape, ctx.offset, ctx.cell_size)
             _lib.check(_lib.lib().pmx_scatter_cic_adj(
                 c.bind(), g, _lib.ptr(ctx.pmid), _lib.ptr(disp), n, _lib.ptr(mesh_cot),

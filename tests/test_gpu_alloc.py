@@ -190,3 +190,101 @@ def test_process_exits_cleanly_with_live_contexts(dev):
                        timeout=150)
     assert r.returncode == 0, r.stderr[-2000:]
     assert r.stdout.startswith('ok')
+
+
+@pytest.mark.parametrize('entry', ['pmx_nbody_steps', 'pmx_nbody_step', 'pmx_nbody_steps_adj',
+                                   'pmx_nbody_step_adj'])
+def test_enomem_leaves_arrays_unchanged(dev, entry):
+    """On a context whose particle scratch was given back (PMX_TRIM_ALL),
+    refuse the k-th device allocation of one call, for every k in turn: each
+    refused call returns PMX_ENOMEM with every caller array as it was, and
+    the first call that needs no refusal gives the same arrays as a fresh
+    context."""
+    import ctypes
+    import pmwd_amd as pa
+    from pmwd_amd import _lib
+    L = _lib.lib()
+    shape = (16, 16, 16)
+    conf = pa.Configuration(4.0, shape, mesh_shape=2)
+    pmid, disp = opm.gen_grid(shape, conf.mesh_shape, conf.cell_size)
+    rng = np.random.default_rng(43)
+    arrays0 = [(disp + rng.standard_normal(disp.shape)).astype(np.float32)]
+    arrays0 += [rng.standard_normal(disp.shape).astype(np.float32) for _ in range(5)]
+    val = float(np.float32(conf.mesh_size / conf.ptcl_num))
+    coef, n = 0.45, len(pmid)
+    ka = np.array([-0.01, -0.02], np.float32)
+    dd = np.array([-0.5, -0.4], np.float32)
+    kb = np.array([-0.015, -0.01], np.float32)
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    pm = torch.as_tensor(pmid, device=dev)
+
+    state = {'countdown': -1, 'refused': 0}
+
+    @_lib._ALLOC_T
+    def counting_alloc(nbytes, device, stream, user):
+        if state['countdown'] == 0:
+            state['countdown'] = -1
+            state['refused'] += 1
+            return None
+        if state['countdown'] > 0:
+            state['countdown'] -= 1
+        try:
+            return torch.cuda.caching_allocator_alloc(int(nbytes), int(device), int(stream or 0))
+        except Exception:
+            return None
+
+    def call(ctx, arrs):
+        x, v, a, dc, vc, ac = (_lib.ptr(t) for t in arrs)
+        h = ctx.bind()
+        if entry == 'pmx_nbody_steps':
+            return L.pmx_nbody_steps(h, _lib.ptr(pm), x, v, a, n, val, coef, 2, P(ka), P(dd), P(kb))
+        if entry == 'pmx_nbody_step':
+            return L.pmx_nbody_step(h, _lib.ptr(pm), x, v, a, n, val, coef, 0.01, 0.5, 0.02)
+        if entry == 'pmx_nbody_steps_adj':
+            dots = (ctypes.c_double * 4)()
+            return L.pmx_nbody_steps_adj(h, _lib.ptr(pm), x, v, a, dc, vc, ac, n, val, coef,
+                                         -0.01, 2, P(ka), P(dd), P(kb), None, dots)
+        dots = (ctypes.c_double * 2)()
+        return L.pmx_nbody_step_adj(h, _lib.ptr(pm), x, v, a, dc, vc, ac, n, val, coef, -0.01,
+                                    -0.02, -0.5, None, dots)
+
+    _lib.check(L.pmx_set_allocator(counting_alloc, _lib._torch_free, None))
+    try:
+        ctx = _lib.Context(conf.mesh_shape, conf.cell_size, dev)
+    finally:
+        if _hooked_by_default():
+            _lib.check(L.pmx_set_allocator(_lib._torch_alloc, _lib._torch_free, None))
+        else:
+            _lib.check(L.pmx_set_allocator(_lib._ALLOC_T(), _lib._FREE_T(), None))
+    fresh = _lib.Context(conf.mesh_shape, conf.cell_size, dev)
+    try:
+        # a first full call grows the context (the adjoint's halo shells too)
+        warm = [torch.as_tensor(a, device=dev).clone() for a in arrays0]
+        _lib.check(call(ctx, warm))
+        want = [torch.as_tensor(a, device=dev).clone() for a in arrays0]
+        _lib.check(call(fresh, want))
+        want = [t.cpu().numpy() for t in want]
+        for k in range(64):
+            ctx.trim()
+            torch.cuda.synchronize(dev)
+            arrs = [torch.as_tensor(a, device=dev).clone() for a in arrays0]
+            state['countdown'], before = k, state['refused']
+            rc = call(ctx, arrs)
+            torch.cuda.synchronize(dev)
+            got = [t.cpu().numpy() for t in arrs]
+            if state['refused'] == before:
+                state['countdown'] = -1
+                assert rc == 0, (k, _lib.lib().pmx_last_error())
+                for g, w in zip(got, want):
+                    np.testing.assert_array_equal(g, w)
+                break
+            assert rc == _lib.ENOMEM, (k, rc)
+            for g, a0 in zip(got, arrays0):
+                np.testing.assert_array_equal(g, a0, err_msg=f'{entry}: allocation {k} refused')
+        else:
+            pytest.fail('more than 64 allocations in one call')
+        assert k > 0
+    finally:
+        state['countdown'] = -1
+        ctx.close()
+        fresh.close()

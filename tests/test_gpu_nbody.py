@@ -173,6 +173,41 @@ def test_nbody_adjoint_vs_oracle(dev):
     assert rel_l2(p0.disp.cpu().numpy(), disp) < 1e-4
 
 
+@pytest.mark.parametrize('de', [dict(w_0_=-0.9), dict(Omega_k_=0.03, w_a_=0.2)])
+def test_nbody_adjoint_dark_energy_cotangents(dev, de):
+    """nbody_adj's cotangents of the set parameters among Omega_k_, w_0_,
+    w_a_ (nbody.py:267-274: value_and_grad of the factors w.r.t. the whole
+    cosmology) against central differences of the oracle's FP64 forward run,
+    growth table re-integrated; unset parameters stay None and A_s, n_s,
+    Omega_b, h are 0."""
+    pa, conf, oconf, _, _ = small_setup(shape=(8, 8, 8), a_start=1 / 4, steps_max=1 / 8)
+    cosmo = pa.boltzmann(pa.SimpleLCDM(conf, **de), conf)
+    pmid, disp = make_ptcl(conf.ptcl_grid_shape, conf.mesh_shape, conf.cell_size, 0.5, 6)
+    vel = (0.05 * np.random.default_rng(9).standard_normal(disp.shape)).astype(np.float32)
+    rng = np.random.default_rng(10)
+    dcot = rng.standard_normal(disp.shape).astype(np.float32)
+    vcot = rng.standard_normal(disp.shape).astype(np.float32)
+    P = lambda a: torch.as_tensor(a, device=dev)
+    fin, _ = pa.nbody(pa.Particles(conf, P(pmid), P(disp), P(vel)), None, cosmo, conf)
+    _, _, cc = pa.nbody_adj(fin, pa.Particles(conf, P(pmid), P(dcot), P(vcot)), cosmo, conf)
+    tight = oconf.replace(growth_rtol=1e-12, growth_atol=1e-12)
+
+    def J(**kw):
+        c = oc.boltzmann(oc.SimpleLCDM(tight, **{**de, **kw}), tight)
+        d1, v1, _ = onb.nbody(pmid, disp, vel, c, tight, dtype=np.float64)
+        return np.sum(dcot.astype(np.float64) * d1) + np.sum(vcot.astype(np.float64) * v1)
+    h = 1e-5
+    for p in ('Omega_k_', 'w_0_', 'w_a_'):
+        if p not in de:
+            assert getattr(cc, p) is None
+            continue
+        fd = (J(**{p: de[p] + h}) - J(**{p: de[p] - h})) / (2 * h)
+        assert abs(getattr(cc, p) - fd) < 2e-3 * abs(fd), (p, getattr(cc, p), fd)
+    fd = (J(Omega_m=0.3 + h) - J(Omega_m=0.3 - h)) / (2 * h)
+    assert abs(cc.Omega_m - fd) < 2e-3 * abs(fd), (cc.Omega_m, fd)
+    assert (cc.A_s_1e9, cc.n_s, cc.Omega_b, cc.h) == (0.0, 0.0, 0.0, 0.0)
+
+
 def test_nbody_autograd_matches_adjoint(dev):
     pa, conf, oconf, cosmo, ocosmo = small_setup(shape=(8, 8, 8), a_start=1 / 4, steps_max=1 / 4)
     pmid, disp = make_ptcl(conf.ptcl_grid_shape, conf.mesh_shape, conf.cell_size, 0.5, 6)

@@ -55,3 +55,31 @@ def test_server_admits_under_a_tight_budget(dev):
     res = [f.result() for f in [s.submit(r) for r in reqs]]
     s.close()
     assert len(res) == 3 and s.peak_active == 1
+
+
+def test_gradient_request_uses_only_pooled_contexts(dev, monkeypatch):
+    """A gradient request's backward (run by autograd, possibly on another
+    thread) uses the request's pooled contexts, not the process-wide cache,
+    so the server's byte accounting and eviction see every context."""
+    from pmwd_amd import _lib
+    from pmwd_amd.serving import Server, SimRequest
+    _lib.clear_contexts()
+    used = set()
+    real_bind = _lib.Context.bind
+
+    def bind(self):
+        used.add(id(self))
+        return real_bind(self)
+    monkeypatch.setattr(_lib.Context, 'bind', bind)
+    req = SimRequest(seed=21, ptcl_grid=16, mesh_ratio=2, box=32.0, Omega_m=0.3, sigma8=0.8,
+                     grad=True, a_start=1 / 8, a_nbody_maxstep=1 / 8)
+    s = Server(dev, max_slots=1)
+    res = s.submit(req).result()
+    s.close()
+    assert res.grad_modes is not None and np.isfinite(res.grad_modes).all()
+    assert _lib._ctx_cache == {}, list(_lib._ctx_cache)
+    pooled = {id(c) for cs in s._pool.values() for c in cs}
+    assert used and used <= pooled, (len(used), len(pooled))
+    for cs in s._pool.values():
+        for c in cs:
+            c.close()
