@@ -97,6 +97,18 @@ int pmx_tile_keys(pmx_ctx* ctx, const int16_t* pmid, const float* disp,
                   long long n, uint32_t* tile);
 int pmx_tile_shape(const pmx_ctx* ctx, int tile_shape[3]);
 
+/* ------------------------------------------------------- white noise ---
+ * pmwd/modes.py:12-49 (white_noise, real=True): out[i] (n float32) standard
+ * normals from Philox4x32-10 keyed by seed (two 32-bit words, low first),
+ * counter = (i/4 low, high, 0, 0), and the float64 Box-Muller transform of
+ * its word pairs; element i is normal i%4 of its block.  out must be 16-byte
+ * aligned.  pmwd draws with jax.random (threefry), whose bit stream this does
+ * not reproduce.                                                            */
+int pmx_white_noise(pmx_ctx* ctx, unsigned long long seed, long long n, float* out);
+/* The generator's raw words: out[4b+j] = word j of block b < nblocks (for
+ * checks against published known answers).  No pmwd counterpart.          */
+int pmx_random_bits(pmx_ctx* ctx, unsigned long long seed, long long nblocks, uint32_t* out);
+
 /* ----------------------------------------------------------- scatter ---
  * Default-argument CIC mass assignment (pmwd/scatter.py:33-83 with
  * mesh=None, offset=0, cell_size=None): mesh[c] (+)= sum val*w.  Particles

@@ -1,10 +1,11 @@
 """Initial conditions restated from pmwd (oracle side).
 
-``white_noise`` (modes.py:12-49) draws from ``numpy.random.default_rng(seed)``
-instead of ``jax.random`` (threefry), whose bit stream is not reproducible
-without jax: the same generator is used by ``pmwd_amd.modes.white_noise``, so
-both sides see identical modes (the mapping seed -> modes is parity unpinned
-against pmwd).  ``linear_modes`` follows modes.py:67-117 and ``lpt`` follows
+``white_noise`` (modes.py:12-49) draws from Philox4x32-10 (Salmon et al.
+2011) with a float64 Box-Muller transform instead of ``jax.random``
+(threefry), whose bit stream is not reproducible without jax; the mapping
+seed -> modes is therefore parity unpinned against pmwd.  The generator is
+pinned by the published Philox known-answer vectors and, through its 4x64
+variant with the same round function, by numpy's independent Philox.  ``linear_modes`` follows modes.py:67-117 and ``lpt`` follows
 lpt.py:13-76 and 136-212 (2LPT), in ``dtype`` arithmetic with FP64 power
 spectrum and growth factors.
 """
@@ -15,9 +16,68 @@ from . import cosmo as oc
 from . import pm
 
 
+# Philox constants (Salmon et al. 2011): round multipliers and Weyl key
+# increments of the 4x32 and 4x64 variants
+PHILOX = {32: ((0xD2511F53, 0xCD9E8D57), (0x9E3779B9, 0xBB67AE85)),
+          64: ((0xD2E7470EE14C6C93, 0xCA5A826395121157), (0x9E3779B97F4A7C15, 0xBB67AE8584CAA73B))}
+
+
+def philox(ctr, key, bits=32, rounds=10):
+    """Philox4xW-R of one counter (4 ints) and key (2 ints), Python ints:
+    round r (key bumped by the Weyl increments before every round but the
+    first) maps (c0, c1, c2, c3) to (hi(M1 c2)^c1^k0, lo(M1 c2), hi(M0 c0)^c3^k1,
+    lo(M0 c0))."""
+    (m0, m1), (w0, w1) = PHILOX[bits]
+    mask = (1 << bits) - 1
+    c, k = list(ctr), list(key)
+    for r in range(rounds):
+        if r:
+            k = [(k[0] + w0) & mask, (k[1] + w1) & mask]
+        p0, p1 = m0 * c[0], m1 * c[2]
+        c = [(p1 >> bits) ^ c[1] ^ k[0], p1 & mask, (p0 >> bits) ^ c[3] ^ k[1], p0 & mask]
+    return c
+
+
+def philox4x32_blocks(seed, blocks):
+    """Philox4x32-10 words (len(blocks), 4) uint32 of the counters
+    (b low, b high, 0, 0) under the key (seed low, seed high), vectorised in
+    uint64 arithmetic (the same rounds as ``philox``)."""
+    (m0, m1), (w0, w1) = PHILOX[32]
+    b = np.asarray(blocks, dtype=np.uint64)
+    mask = np.uint64(0xFFFFFFFF)
+    c = [b & mask, b >> np.uint64(32), np.zeros_like(b), np.zeros_like(b)]
+    seed = int(seed) % 2**64
+    k0, k1 = seed & 0xFFFFFFFF, seed >> 32
+    for r in range(10):
+        if r:
+            k0, k1 = (k0 + w0) & 0xFFFFFFFF, (k1 + w1) & 0xFFFFFFFF
+        p0 = np.uint64(m0) * c[0]
+        p1 = np.uint64(m1) * c[2]
+        c = [(p1 >> np.uint64(32)) ^ c[1] ^ np.uint64(k0), p1 & mask,
+             (p0 >> np.uint64(32)) ^ c[3] ^ np.uint64(k1), p0 & mask]
+    return np.stack(c, axis=-1).astype(np.uint32)
+
+
+def normals(seed, index):
+    """Standard normals number ``index`` (any int array) of the stream of
+    ``seed``: element i is normal i % 4 of Philox block i // 4, from the
+    float64 Box-Muller transform of the block's word pairs."""
+    index = np.asarray(index, dtype=np.int64)
+    w = philox4x32_blocks(seed, index // 4).astype(np.float64)
+    u = (w + 0.5) * 2.0**-32
+    j = index % 4
+    ua = np.where(j < 2, u[..., 0], u[..., 2])
+    ub = np.where(j < 2, u[..., 1], u[..., 3])
+    r = np.sqrt(-2.0 * np.log(ua))
+    t = 6.283185307179586 * ub
+    z = np.where(j % 2 == 0, r * np.cos(t), r * np.sin(t))
+    return z.astype(np.float32)
+
+
 def white_noise(seed, conf, real=False, unit_abs=False):
-    """modes.py:12-49 with a numpy generator (see module docstring)."""
-    modes = np.random.default_rng(seed).standard_normal(conf.ptcl_grid_shape, dtype=np.float32)
+    """modes.py:12-49 with the Philox generator (see module docstring)."""
+    n = int(np.prod(conf.ptcl_grid_shape))
+    modes = normals(seed, np.arange(n)).reshape(conf.ptcl_grid_shape)
     if real and not unit_abs:
         return modes
     modes = sfft.rfftn(modes, norm='ortho')

@@ -90,6 +90,8 @@ _SIGS = {
     'pmx_enmesh': [c_p, ctypes.POINTER(CicGeom), c_int, c_int, c_p, c_p, c_ll, c_p, c_p, c_p],
     'pmx_gather_cic3': [c_p, c_p, c_p, c_ll, c_p, c_p],
     'pmx_poisson_grad_adj': [c_p, c_p, c_f, c_p],
+    'pmx_white_noise': [c_p, ctypes.c_ulonglong, c_ll, c_p],
+    'pmx_random_bits': [c_p, ctypes.c_ulonglong, c_ll, c_p],
     'pmx_rfftn': [c_p, c_p, c_p],
     'pmx_irfftn': [c_p, c_p, c_p],
     'pmx_poisson_grad': [c_p, c_p, c_f, c_p],

@@ -71,6 +71,12 @@ void pmx_mark_queued(pmx_ctx* c) {
 
 static int dev_alloc(pmx_ctx* c, void** p, size_t bytes) {
   if (bytes == 0) bytes = 256;
+  if (c->no_alloc) {
+    *p = nullptr;
+    pmx_set_error("device allocation of %zu bytes in an entry point that allocates nothing",
+                  bytes);
+    return PMX_ENOMEM;
+  }
   if (int rc = pmx_no_capture(c, "device allocation")) {
     *p = nullptr;
     return rc;
@@ -124,6 +130,25 @@ static void free_adj_state(pmx_ctx* c);
 
 // Scatter tile extent: 16^3 cells, clipped to the mesh (the 17^3 LDS block
 // of 64-bit accumulators is 39 KB, four workgroups per CU).
+static TileGeom make_tiles(const int n[3]);
+
+void pmx_ctx_geometry(pmx_ctx* c, int n0, int n1, int n2, double cell_size, hipStream_t stream) {
+  c->n[0] = n0; c->n[1] = n1; c->n[2] = n2;
+  c->n2h = n2 / 2 + 1;
+  // 16 complex = one 128-byte cache line: column blocks of the strided passes
+  // and rows of the row passes then start on line boundaries
+  c->n2p = c->n2h <= 16 ? c->n2h : (c->n2h + 15) / 16 * 16;
+  c->rs = 2 * c->n2p;
+  c->M = (long long)n0 * n1 * n2;
+  c->Mc = (long long)n0 * n1 * c->n2p;
+  c->cell = cell_size;
+  c->cellf = (float)cell_size;
+  c->stream = stream;
+  hipGetDevice(&c->device);
+  c->tg = make_tiles(c->n);
+  c->ntiles = (long long)c->tg.ntx * c->tg.nty * c->tg.ntz;
+}
+
 static TileGeom make_tiles(const int n[3]) {
   TileGeom t;
   t.tx = n[0] < 16 ? n[0] : 16;
@@ -176,18 +201,7 @@ extern "C" int pmx_ctx_create(int n0, int n1, int n2, double cell_size, void* st
     pmx_set_error("pmx_ctx_create: out of host memory");
     return PMX_ENOMEM;
   }
-  c->n[0] = n0; c->n[1] = n1; c->n[2] = n2;
-  c->n2h = n2 / 2 + 1;
-  // 16 complex = one 128-byte cache line: column blocks of the strided passes
-  // and rows of the row passes then start on line boundaries
-  c->n2p = c->n2h <= 16 ? c->n2h : (c->n2h + 15) / 16 * 16;
-  c->rs = 2 * c->n2p;
-  c->M = (long long)n0 * n1 * n2;
-  c->Mc = (long long)n0 * n1 * c->n2p;
-  c->cell = cell_size;
-  c->cellf = (float)cell_size;
-  c->stream = (hipStream_t)stream;
-  hipGetDevice(&c->device);
+  pmx_ctx_geometry(c, n0, n1, n2, cell_size, (hipStream_t)stream);
   c->dalloc = g_dalloc;
   c->dfree = g_dfree;
   c->dalloc_user = g_dalloc_user;
@@ -607,6 +621,11 @@ __global__ void scan_apply_k(const uint32_t* __restrict__ in, long long n,
     if (j < n) out[j] = ex;
     ex += v[i];
   }
+}
+
+long long pmx_scan_blocks(long long n) {
+  long long nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  return nb < 1 ? 1 : nb;
 }
 
 // Sizes the scan scratch for scans of up to n entries, so that entry points

@@ -448,7 +448,7 @@ __global__ void gen_gather3_k(GenGeom G, const int16_t* __restrict__ pmid,
   }
   for (int d = 0; d < 3; ++d) {
     float t[8];
-    for (int j = 0; j < 8; ++j) t[j] = mesh3[d * M + cs[j]] * ws[j];
+    for (int j = 0; j < 8; ++j) t[j] = (cs[j] >= 0 ? mesh3[d * M + cs[j]] : 0.f) * ws[j];
     acc[3 * i + d] = 0.f + gen_tree(t, 8);
   }
 }

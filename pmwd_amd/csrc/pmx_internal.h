@@ -218,7 +218,12 @@ struct pmx_ctx {
   long long scan_tmp_cap;
 
   size_t bytes;        // device bytes held
+  int no_alloc;        // 1: scratch carved from static device memory; never allocate
 };
+
+// Geometry fields of a context (extents, padded strides, cell size, stream,
+// device, scatter tiles); no device memory.
+void pmx_ctx_geometry(pmx_ctx* c, int n0, int n1, int n2, double cell_size, hipStream_t stream);
 
 int pmx_ensure_ptcl(pmx_ctx* c, long long n);
 int pmx_ensure_vrec(pmx_ctx* c);
@@ -241,6 +246,8 @@ struct PmxQueued {
 // First statement of every entry point that enqueues work on the context.
 #define PMX_ENTRY(ctx) PmxQueued pmx_queued_guard_(ctx)
 int pmx_ensure_scan(pmx_ctx* c, long long n);
+// Blocks (scratch words) of a scan of n entries.
+long long pmx_scan_blocks(long long n);
 // Device memory of a context through its allocator (pmx_set_allocator or
 // hipMalloc), counted in pmx_ctx_bytes; the free takes the allocated size.
 int pmx_dev_alloc(pmx_ctx* c, void** p, size_t bytes);

@@ -1,8 +1,9 @@
 """White noise and linear modes (pmwd/modes.py:12-117).
 
-``white_noise`` draws float32 normals from ``numpy.random.default_rng(seed)``
-(pmwd's jax.random threefry stream is not reproducible without jax) and
-moves them to the GPU; Fourier modes use the HIP r2c with 'ortho' norm.
+``white_noise`` draws float32 normals on the device (``pmx_white_noise``:
+Philox4x32-10 keyed by the seed and a float64 Box-Muller transform; pmwd's
+jax.random threefry stream is not reproducible without jax); Fourier modes
+use the HIP r2c with 'ortho' norm.
 ``linear_modes`` multiplies by sqrt(V Plin(|k|)) with |k| formed in float32
 from the float32 wavevectors and Plin evaluated in float64, as pmwd.
 """
@@ -18,10 +19,15 @@ from .pm_util import fftfreq, fftfwd, fftinv
 
 
 def white_noise(seed, conf, real=False, unit_abs=False, device=None):
-    """White-noise real or Fourier modes, zero mean, unit variance."""
+    """White-noise real or Fourier modes, zero mean, unit variance
+    (modes.py:12-49); the real noise is drawn on the device."""
+    from . import _lib
     device = torch.device(device) if device is not None else default_device()
-    w = np.random.default_rng(seed).standard_normal(conf.ptcl_grid_shape, dtype=np.float32)
-    modes = torch.as_tensor(w, device=device)
+    _lib.require_device(torch.empty(0, device=device), 'white_noise device')
+    modes = torch.empty(conf.ptcl_grid_shape, dtype=torch.float32, device=device)
+    c = _lib.context_any_cell(conf.ptcl_grid_shape, device)
+    _lib.check(_lib.lib().pmx_white_noise(c.bind(), int(seed) % 2**64, modes.numel(),
+                                          _lib.ptr(modes)))
     if real and not unit_abs:
         return modes
     modes = fftfwd(modes, norm='ortho')

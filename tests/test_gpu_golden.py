@@ -58,6 +58,11 @@ def test_run_fixture(dev):
     assert len(conf.a_nbody) == 5
     cosmo = pa.boltzmann(pa.SimpleLCDM(conf), conf)
     white = torch.as_tensor(z['white'], device=dev)
+    # the device generator reproduces the fixture's noise (seed 0) within an
+    # ulp, at most a few elements differing
+    dw = pa.white_noise(0, conf, real=True, device=dev).cpu().numpy()
+    diff = np.abs(dw.astype(np.float64) - z['white'])
+    assert np.all(diff <= np.spacing(np.abs(z['white']))) and np.count_nonzero(diff) <= 2
     modes = pa.linear_modes(white, cosmo, conf)
     ptcl, _ = pa.lpt(modes, cosmo, conf)
     np.testing.assert_array_equal(ptcl.pmid.cpu().numpy(), z['pmid'])

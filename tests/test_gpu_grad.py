@@ -54,13 +54,17 @@ def test_forward_model_gradients_finite_difference(dev):
     fd_s8 = (L(s8=0.8 + eps) - L(s8=0.8 - eps)) / (2 * eps)
     v = torch.randn(white.shape, generator=torch.Generator().manual_seed(7)).to(dev)
     w0 = white.detach()
-    fd_w = (L(w=w0 + eps * v) - L(w=w0 - eps * v)) / (2 * eps)
+    # the density is piecewise linear in the positions (CIC): the difference
+    # quotient carries the kinks its perturbation crosses and moves with the
+    # step, so the modes gradient has to lie within the spread of several
+    fds = [(L(w=w0 + e * v) - L(w=w0 - e * v)) / (2 * e) for e in (1e-3, 5e-4, 2.5e-4)]
     an_w = float((g_w.double() * v.double()).sum())
     print(f'Omega_m {float(g_om):.6e} fd {fd_om:.6e}; sigma8 {float(g_s8):.6e} fd {fd_s8:.6e}; '
-          f'modes.v {an_w:.6e} fd {fd_w:.6e}')
+          f'modes.v {an_w:.6e} fd {fds}')
     assert abs(float(g_om) - fd_om) < 0.03 * abs(fd_om)
     assert abs(float(g_s8) - fd_s8) < 0.03 * abs(fd_s8)
-    assert abs(an_w - fd_w) < 0.03 * abs(fd_w)
+    pad = 0.03 * max(abs(f) for f in fds)
+    assert min(fds) - pad <= an_w <= max(fds) + pad, (an_w, fds)
 
 
 def test_library_fft_autograd_matches_rocfft_autograd(dev):

@@ -65,7 +65,14 @@ def test_rebinding_after_the_old_stream_is_destroyed(dev):
     """A caller that destroys its stream before rebinding (an external
     stream of its own) is served: the old handle is not touched."""
     _lib, conf, pm, x, val = _setup(dev)
-    hip = ctypes.CDLL('libamdhip64.so')
+    # the HIP runtime already in the process (torch's), never a second copy
+    with open('/proc/self/maps') as f:
+        paths = {ln.split()[-1] for ln in f if 'libamdhip64.so' in ln}
+    assert len(paths) == 1, paths
+    hip = ctypes.CDLL(paths.pop())
+    hip.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
+    hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+    hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
     raw = ctypes.c_void_p()
     assert hip.hipStreamCreateWithFlags(ctypes.byref(raw), 0) == 0
     ext = torch.cuda.ExternalStream(raw.value, device=dev)

@@ -512,3 +512,58 @@ def test_shared_cic_equals_enmesh_and_composition(shape, ratio, cell, std, dtype
     dc2, _ = opm.scatter_adj(pmid, disp, cell, ms, dens_cot, dtype=dtype)
     _, got, _ = opm.gravity_vjp(pmid, disp, 0.3, cell, ms, ac, dtype=dtype)
     np.testing.assert_array_equal(got, (dcot + dc2).astype(dtype))
+
+
+# ------------------------------------------------------- white noise ---
+
+# Philox4x32-10 known-answer vectors published with Random123 (Salmon et al.
+# 2011): counter, key, output
+PHILOX_KAT = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+              ((0xffffffff,) * 4, (0xffffffff,) * 2,
+               (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+              ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+               (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+
+
+@pytest.mark.parametrize('ctr,key,want', PHILOX_KAT)
+def test_philox4x32_known_answers(ctr, key, want):
+    assert tuple(ic.philox(ctr, key, 32)) == want
+
+
+def test_philox4x64_matches_numpy():
+    """The same round function with the 4x64 constants reproduces numpy's
+    independent Philox (which increments its counter before each block)."""
+    key = (123456789, 987654321)
+    raw = [int(x) for x in np.random.Philox(counter=0, key=list(key)).random_raw(12)]
+    got = sum((ic.philox((b, 0, 0, 0), key, 64) for b in (1, 2, 3)), [])
+    assert got == raw
+
+
+def test_philox_vectorised_and_normals():
+    """The vectorised generator equals the Python-int rounds (low and high
+    counter words, seeds above 2^32); the normals follow the documented
+    Box-Muller of each block's word pairs, and have the moments of N(0, 1)."""
+    seed = (7 << 32) + 99
+    blocks = np.array([0, 1, 2, 12345, 2**32 - 1, 2**32 + 5, 2**40 + 3])
+    got = ic.philox4x32_blocks(seed, blocks)
+    for b, row in zip(blocks, got):
+        want = ic.philox((int(b) & 0xffffffff, int(b) >> 32, 0, 0),
+                         (seed & 0xffffffff, seed >> 32), 32)
+        assert [int(x) for x in row] == want
+    w = ic.philox(((5), 0, 0, 0), (seed & 0xffffffff, seed >> 32), 32)
+    u = [(x + 0.5) / 2**32 for x in w]
+    import math
+    r0, r1 = math.sqrt(-2 * math.log(u[0])), math.sqrt(-2 * math.log(u[2]))
+    want = np.float32([r0 * math.cos(2 * math.pi * u[1]), r0 * math.sin(2 * math.pi * u[1]),
+                       r1 * math.cos(2 * math.pi * u[3]), r1 * math.sin(2 * math.pi * u[3])])
+    np.testing.assert_array_equal(ic.normals(seed, np.arange(20, 24)), want)
+    z = ic.normals(3, np.arange(1 << 21)).astype(np.float64)
+    n = z.size
+    assert abs(z.mean()) < 5 / np.sqrt(n)
+    assert abs(z.var() - 1) < 5 * np.sqrt(2 / n)
+    assert abs(np.mean(z**3)) < 5 * np.sqrt(15 / n)
+    assert abs(np.mean(z**4) - 3) < 5 * np.sqrt(96 / n)
+    # different seeds and neighbouring elements are uncorrelated
+    z2 = ic.normals(4, np.arange(1 << 21)).astype(np.float64)
+    assert abs(np.mean(z * z2)) < 5 / np.sqrt(n)
+    assert abs(np.mean(z[1:] * z[:-1])) < 5 / np.sqrt(n)
