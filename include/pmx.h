@@ -117,6 +117,20 @@ int pmx_random_bits(pmx_ctx* ctx, unsigned long long seed, long long nblocks, ui
 int pmx_scatter_mass(pmx_ctx* ctx, const int16_t* pmid, const float* disp,
                      long long n, float val, float* mesh, int accumulate);
 
+/* pmx_scatter_mass with no context and no device allocation.  No
+ * counterpart in pmwd: for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, hosts that forbid it).  The
+ * mesh (n0,n1,n2) of cell cell_size is given directly and the work is
+ * enqueued on `stream` of the current device.  The binning and halo scratch
+ * live in one static 64 MiB device array per device (about 1.9 million
+ * particles on a 128^3 mesh); larger requests return PMX_EINVAL with a
+ * message and enqueue nothing.  Every call on a device shares that device's
+ * array: calls that overlap on one device race, and the caller serialises
+ * them.  Bitwise the result of pmx_scatter_mass.                          */
+int pmx_scatter_mass_noalloc(const int16_t* pmid, const float* disp, long long n,
+                             int n0, int n1, int n2, double cell_size, float val,
+                             float* mesh, int accumulate, void* stream);
+
 /* Generic CIC geometry for non-default arguments: enmesh's a2 branch
  * (pmwd/pm_util.py:99-118) when out_cell_size > 0, offsets b12, a target
  * shape (indices outside are dropped, pmwd/gather.py:75), channels.        */

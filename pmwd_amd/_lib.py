@@ -82,6 +82,7 @@ _SIGS = {
     'pmx_tile_keys': [c_p, c_p, c_p, c_ll, c_p],
     'pmx_tile_shape': [c_p, ctypes.POINTER(c_int * 3)],
     'pmx_scatter_mass': [c_p, c_p, c_p, c_ll, c_f, c_p, c_int],
+    'pmx_scatter_mass_noalloc': [c_p, c_p, c_ll, c_int, c_int, c_int, c_d, c_f, c_p, c_int, c_p],
     'pmx_scatter_cic': [c_p, ctypes.POINTER(CicGeom), c_p, c_p, c_ll, c_p, c_f, c_int, c_p],
     'pmx_gather_cic': [c_p, ctypes.POINTER(CicGeom), c_p, c_p, c_ll, c_p, c_int, c_p, c_f, c_p],
     'pmx_scatter_cic_adj': [c_p, ctypes.POINTER(CicGeom), c_p, c_p, c_ll, c_p, c_p, c_f, c_int,
