@@ -2,14 +2,18 @@
 """Particle-updates/s of pmwd's KDK N-body step on one MI355X.
 
 Workload (default): 512^3 particles on a 512^3 mesh (mesh_shape=1), box
-1000 Mpc/h, FP32 state, 2LPT initial conditions at a=1/32 from seeded white
-noise (numpy generator, seed 0), sigma8=0.8, Omega_m=0.3, Omega_b=0.05,
-h=0.7, n_s=0.96, then KDK steps with a_nbody_maxstep=1/32 (31 steps to a=1).
+1000 Mpc/h, FP32 state, 2LPT initial conditions at a=1/32 from device white
+noise (Philox, seed 0), sigma8=0.8, Omega_m=0.3, Omega_b=0.05, h=0.7,
+n_s=0.96, then KDK steps with a_nbody_maxstep=1/32 (31 steps to a=1).
 A "step" is one KDK step (kick, drift, CIC scatter, FFT Poisson solve,
 3-component CIC gather, kick) over all particles; ``value`` = particles x
-steps / wall time of the timed steps, inputs resident in HBM.  After the
-forward run the reverse-time adjoint runs the same number of steps from the
-final state and ``fwd_adj_value`` = particles x steps / (forward + adjoint).
+steps / wall time of the K timed steps, inputs resident in HBM.  The timed
+steps are the last K of the 31-step schedule (ending at a=1, where the
+particles are most clustered); the steps before them run untimed, W of
+them immediately before as warm-up.  The whole 31-step ``nbody()`` call
+(first force included) is timed separately.  After the forward run the
+reverse-time adjoint runs K steps from the final state and
+``fwd_adj_value`` = particles x steps / (forward + adjoint).
 
 Prints one JSON line.  ``--gpus`` above 1 exits: this build runs on one GPU.
 """
@@ -34,6 +38,9 @@ def parse():
     p.add_argument('--no-adjoint', action='store_true')
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--no-serve', action='store_true', help='skip the serving load')
+    p.add_argument('--no-nbody-call', action='store_true',
+                   help='skip timing the whole nbody() call (profiles: the timed K-step call '
+                        'is then the last of the forward)')
     p.add_argument('--full-adjoint', type=int, default=0, metavar='G',
                    help='instead of the default run: forward plus gradient of a density MSE '
                         'for G^3 particles on a G^3 mesh (box 2000*G/1024 Mpc/h, 31 steps, '
@@ -108,8 +115,37 @@ def kernel_bytes(N, mesh_shape, tile, K, fold_on_load=False):
     }
 
 
+def step_bytes(N, M, Mh):
+    """Algorithmic bytes of one forward KDK step by kernel group, the
+    compulsory traffic of pmwd's logical operations (each input read once,
+    each output written once): scatter 18N + 4M (pmid and disp in, density
+    out); 3D r2c 4M + 8Mh and Poisson (Green, -ik, 3 c2r) 8Mh + 12M; the
+    rest of the step -- the 3-component gather, both kicks and the drift,
+    reading pmid, disp, vel, acc and the force meshes, writing disp, vel,
+    acc -- 60N + 12M.  Together 78N + 8M + 16Mh + 24M (118 B per particle
+    at mesh = particle grid)."""
+    return {'scatter': 18 * N + 4 * M,
+            'fft_poisson': 16 * M + 16 * Mh,
+            'step_gather_kick_drift': 60 * N + 12 * M}
+
+
+def adj_step_bytes(N, M, Mh):
+    """Algorithmic bytes of one reverse KDK step (kick_adj, drift_adj,
+    force_adj, kick_adj of nbody.py:49-118) by kernel group, counted the same
+    way: the density and the three cotangent deposits read pmid, disp and
+    vel_cot and write four meshes (48N + 16M); the forward Poisson solve
+    (16M + 16Mh) and the adjoint one (three r2c of the cotangent meshes, the
+    Green/-ik combination fused into the inverse: 16M + 48Mh); the reverse
+    step kernel reads pmid, disp, vel, disp_cot, vel_cot, the three force
+    meshes and the density cotangent and writes disp, vel, disp_cot and
+    vel_cot (102N + 16M)."""
+    return {'deposits': 48 * N + 16 * M,
+            'fft_poisson_fwd_adj': 32 * M + 64 * Mh,
+            'reverse_step': 102 * N + 16 * M}
+
+
 GROUPS = {
-    # first kick/drift/bin (once per call) and the scatter of every step
+    # re-binning, capacity scan, first fill of a call, and the scatter
     'scatter': ('bin', 'scan', 'fill', 'deposit', 'fold'),
     # 3D r2c plus Poisson (Green, -ik, 3 c2r)
     'fft_poisson': ('z_r2c', 'y_fwd', 'poisson_x', 'y_inv', 'z_c2r'),
@@ -117,49 +153,104 @@ GROUPS = {
     'step_gather_kick_drift': ('gather',),
 }
 
+ADJ_GROUPS = {
+    'deposits': ('bin', 'scan', 'fill', 'deposit', 'fold'),
+    'fft_poisson_fwd_adj': ('z_r2c', 'y_fwd', 'poisson_x', 'y_inv', 'z_c2r'),
+    'reverse_step': ('grad_gather', 'kd_adj', 'vjp_gather'),
+}
 
-def cpu_baseline(conf, cosmo_om, slab):
-    """Time the numpy oracle on a bounded sample of one KDK step.
 
-    Sample: the full-mesh FFT Poisson solve plus kick/drift/CIC scatter/
-    3-component gather for 1/slab of the particles; the particle part is
-    scaled by slab.  Returns (upd/s, details).
-    """
+def cpu_info():
+    model = None
+    try:
+        with open('/proc/cpuinfo') as f:
+            for ln in f:
+                if ln.startswith('model name'):
+                    model = ln.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {'cpu_count': os.cpu_count(), 'cpu_model': model}
+
+
+def cpu_baseline(conf, cosmo, ptcl, dens, slab, reps=3):
+    """The numpy oracle timed on one KDK step and one reverse step of the
+    run's own 2LPT state, on the host cores (threads: at most 16, the
+    host share of one GPU).
+
+    Sample: the full-mesh FFT Poisson solve (scipy.fft, all threads) on the
+    run's density, and the particle work -- kick, drift, CIC setup and
+    scatter, 3-component gather, kick -- for 1/slab of the particles (the
+    first 1/slab of them, numpy on a thread pool), scaled by slab.  The
+    reverse step: the oracle's force VJP (forward recompute plus the adjoint
+    chain) on the same sample, its mesh part timed with one particle.
+    Median of ``reps`` runs after one warm-up.  Returns (fwd upd/s,
+    fwd+adj upd/s, details)."""
     import numpy as np
     import scipy.fft as sfft
     from oracle import pm as opm
-    cores = min(16, os.cpu_count() or 1)
+    threads = min(16, os.cpu_count() or 1)
     shape = conf.mesh_shape
-    n_all = conf.ptcl_num
-    ns = n_all // slab
-    sub = (conf.ptcl_grid_shape[0] // slab,) + tuple(conf.ptcl_grid_shape[1:])
-    pmid, disp = opm.gen_grid(conf.ptcl_grid_shape, shape, conf.cell_size)
-    pmid, disp = pmid[:ns], disp[:ns]
-    rng = np.random.default_rng(0)
-    disp = (disp + 2.0 * rng.standard_normal(disp.shape)).astype(np.float32)
-    vel = rng.standard_normal(disp.shape).astype(np.float32)
-    acc = rng.standard_normal(disp.shape).astype(np.float32)
-    t0 = time.perf_counter()
-    vel = vel + acc * np.float32(0.01)
-    disp = disp + vel * np.float32(0.01)
-    dens = opm.scatter(pmid, disp, conf.cell_size, shape, ptcl_num=n_all)
-    t1 = time.perf_counter()
-    dens_full = (1 + 0.1 * rng.standard_normal(shape)).astype(np.float32)
-    t2 = time.perf_counter()
-    with sfft.set_workers(cores):
-        g = opm.gravity_mesh(dens_full, cosmo_om, conf.cell_size, shape)
-    t3 = time.perf_counter()
-    acc = np.stack([opm.gather(pmid, disp, conf.cell_size, shape, gi) for gi in g], axis=-1)
-    vel = vel + acc * np.float32(0.01)
-    t4 = time.perf_counter()
-    t_ptcl = (t1 - t0) + (t4 - t3)
-    t_mesh = t3 - t2
-    t_step = t_ptcl * slab + t_mesh
-    return n_all / t_step, dict(cores=cores, t_step_s=t_step, t_sample_s=t_ptcl + t_mesh,
-                                sample=f'one KDK step: full {shape[0]}^3-mesh FFT Poisson solve '
-                                       f'(scipy.fft, {cores} workers) + kick/drift/scatter/gather '
-                                       f'of 1/{slab} of the particles ({ns}, numpy, 1 thread), '
-                                       f'particle part scaled x{slab}')
+    N = conf.ptcl_num
+    ns = N // slab
+    pmid = ptcl.pmid[:ns].cpu().numpy()
+    disp0 = ptcl.disp[:ns].cpu().numpy()
+    vel0 = ptcl.vel[:ns].cpu().numpy()
+    acc0 = np.zeros_like(disp0)
+    dens = dens.cpu().numpy()
+    cell, om = conf.cell_size, cosmo.Omega_m
+    f32 = np.float32
+
+    def fwd():
+        t0 = time.perf_counter()
+        vel = vel0 + acc0 * f32(0.01)
+        disp = disp0 + vel * f32(0.01)
+        cics, _ = opm.force_particle_setup(pmid, disp, cell, shape, N)
+        t1 = time.perf_counter()
+        with sfft.set_workers(threads):
+            g = opm.gravity_mesh(dens, om, cell, shape)
+        t2 = time.perf_counter()
+        acc = opm.force_particle_readout(cics, g, ns)
+        vel = vel + acc * f32(0.01)
+        t3 = time.perf_counter()
+        return (t1 - t0) + (t3 - t2), t2 - t1
+
+    dc0 = (1e-3 * np.random.default_rng(5).standard_normal(disp0.shape)).astype(f32)
+    vc0 = (1e-3 * np.random.default_rng(6).standard_normal(disp0.shape)).astype(f32)
+
+    def adj(n):
+        """kick_adj, drift_adj, force_adj, kick_adj of n particles."""
+        k, d = f32(-0.01), f32(-0.01)
+        t0 = time.perf_counter()
+        vel = vel0[:n] + acc0[:n] * k
+        dcot = dc0[:n] - acc0[:n] * k
+        disp = disp0[:n] + vel * d
+        vcot = vc0[:n] - dcot * d
+        dots = [float(np.sum(vcot.astype(np.float64) * acc0[:n])),
+                float(np.sum(dcot.astype(np.float64) * vel))]
+        with sfft.set_workers(threads):
+            acc, acot, _ = opm.gravity_vjp(pmid[:n], disp, om, cell, shape, vcot, ptcl_num=N)
+        vel = vel + acc * k
+        dcot = dcot - acot * k
+        return time.perf_counter() - t0 + 0 * sum(dots)
+
+    fwd()
+    runs = [fwd() for _ in range(reps)]
+    t_ptcl = float(np.median([r[0] for r in runs]))
+    t_mesh = float(np.median([r[1] for r in runs]))
+    t_fwd = t_ptcl * slab + t_mesh
+    adj(1)
+    t_adj_mesh = adj(1)
+    t_adj_all = float(np.median([adj(ns) for _ in range(reps)]))
+    t_adj = max(t_adj_all - t_adj_mesh, 0.0) * slab + t_adj_mesh
+    info = dict(threads=threads, t_step_s=t_fwd, t_adj_step_s=t_adj, t_mesh_s=t_mesh,
+                t_ptcl_sample_s=t_ptcl, t_adj_mesh_s=t_adj_mesh, reps=reps,
+                sample=f'one KDK step of the run\'s 2LPT state: full {shape[0]}^3-mesh FFT '
+                       f'Poisson solve (scipy.fft, {threads} threads) on the run\'s density + '
+                       f'kick/drift/CIC scatter/3-component gather of the first 1/{slab} of the '
+                       f'particles ({ns}, numpy on {threads} threads), particle part scaled '
+                       f'x{slab}; median of {reps} after a warm-up', **cpu_info())
+    return N / t_fwd, N / (t_fwd + t_adj), info
 
 
 def serving_bench(dev, n, rate, slots):
@@ -383,17 +474,18 @@ def main():
         log('copy calibration: 3 x 1 GiB runtime copies, 3 x 1 GiB elementwise neg')
 
     log(f'initial conditions {G}^3 (2LPT)')
+    torch.cuda.synchronize()
     t = time.perf_counter()
     modes = pa.white_noise(0, conf, real=True, device=dev)
     modes = pa.linear_modes(modes, cosmo, conf)
-    ptcl, _ = pa.lpt(modes, cosmo, conf)
+    ptcl_ic, _ = pa.lpt(modes, cosmo, conf)
     del modes
     torch.cuda.synchronize()
     t_ic = time.perf_counter() - t
 
-    pmid = ptcl.pmid.contiguous()
-    disp = ptcl.disp.clone().contiguous()
-    vel = ptcl.vel.clone().contiguous()
+    pmid = ptcl_ic.pmid.contiguous()
+    disp = ptcl_ic.disp.clone().contiguous()
+    vel = ptcl_ic.vel.clone().contiguous()
     acc = torch.empty_like(disp)
     L = _lib.lib()
     ctx = _lib.context(conf.mesh_shape, conf.cell_size, dev)
@@ -402,13 +494,7 @@ def main():
 
     a = conf.a_nbody
     nsteps_sched = len(a) - 1
-    sched = []
-    for i in range(nsteps_sched):
-        a0, a1 = a[i], a[i + 1]
-        ah = a0 * (1 - 0.5) + a1 * 0.5
-        sched.append((nb._f32(nb.kick_factor(a0, a0, ah, cosmo, conf)),
-                      nb._f32(nb.drift_factor(ah, a0, a1, cosmo, conf)),
-                      nb._f32(nb.kick_factor(a1, ah, a1, cosmo, conf))))
+    sched = [nb._kdk_factors(a[i], a[i + 1], cosmo, conf) for i in range(nsteps_sched)]
 
     def steps(i0, k):
         """k KDK steps of the schedule from step i0 in one pmx_nbody_steps call"""
@@ -418,27 +504,39 @@ def main():
         _lib.check(L.pmx_nbody_steps(h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel),
                                      _lib.ptr(acc), N, val, coef, k, P(k1), P(d), P(k2)))
 
+    K, W = args.steps, args.warmup
+    # the timed steps end at a = 1; the steps before them run untimed (the W
+    # just before as warm-up).  With fewer untimed steps than W, a W-step
+    # warm-up run from the initial conditions comes first.
+    i0 = max(nsteps_sched - K, 0)
     _lib.check(L.pmx_gravity(h, _lib.ptr(pmid), _lib.ptr(disp), N, val, coef, _lib.ptr(acc)))
-    if args.warmup:
-        steps(0, args.warmup)
+    if W and i0 < W:
+        steps(0, W)
+        disp.copy_(ptcl_ic.disp)
+        vel.copy_(ptcl_ic.vel)
+        _lib.check(L.pmx_gravity(h, _lib.ptr(pmid), _lib.ptr(disp), N, val, coef, _lib.ptr(acc)))
+    if i0 > W:
+        steps(0, i0 - W)
+    if i0 and min(W, i0):
+        steps(i0 - min(W, i0), min(W, i0))
     torch.cuda.synchronize()
 
-    log(f'timed forward: {args.steps} steps')
+    log(f'timed forward: {K} steps from step {i0}')
     _lib.check(L.pmx_timing_enable(h, 1))
     _lib.timing_read(ctx)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    steps(args.warmup, args.steps)
+    steps(i0, K)
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
     timing = _lib.timing_read(ctx)
     _lib.check(L.pmx_timing_enable(h, 0))
 
-    K = args.steps
     ms_per_step = 1e3 * t_fwd / K
     value = N * K / t_fwd
+    a_to = a[(i0 + K) % nsteps_sched] if i0 + K != nsteps_sched else a[-1]
 
-    # per kernel class: time per step, launches, algorithmic bytes per launch
+    # per kernel class: time per step, launches, bytes per launch
     ts = (ctypes.c_int * 3)()
     _lib.check(L.pmx_tile_shape(h, ctypes.byref(ts)))
     # no fold launches: the row pass of the Poisson solve folds on load
@@ -457,22 +555,41 @@ def main():
             e['alg_gbs'] = alg / (ms_tot * 1e-3) / 1e9
             e['impl_gbs'] = impl / (ms_tot * 1e-3) / 1e9
         kernels[k] = e
+    sb = step_bytes(N, M, Mh)
     groups = {}
     for gname, classes in GROUPS.items():
         ms = sum(kernels[c]['ms_per_step'] for c in classes if c in kernels)
-        b = sum(kb[c][0] for c in classes) / K
-        groups[gname] = dict(ms=ms, alg_bytes=b, gbs=b / (ms * 1e-3) / 1e9 if ms > 0 else None)
+        impl = sum(kb[c][1] for c in classes) / K
+        groups[gname] = dict(ms=ms, alg_bytes=sb[gname], impl_bytes=impl,
+                             gbs=sb[gname] / (ms * 1e-3) / 1e9 if ms > 0 else None)
         groups[gname]['frac'] = groups[gname]['gbs'] / HBM_PEAK_GBS if ms > 0 else None
     dom = max((k for k in kernels if k in kb and kb[k][0]), key=lambda k: kernels[k]['ms_per_step'])
-    # whole KDK step: the algorithmic bytes of all classes per step
-    B_step = sum(v[0] for v in kb.values()) / K
+    B_step = sum(sb.values())
 
-    traffic = None
+    traffic, traffic_K = None, None
     tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
                          f'traffic_{G}.json')
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get('kernels', {}).get(dom, {}).get('hbm_bytes_per_launch')
+            tj = json.load(f)
+        traffic = tj.get('kernels', {}).get(dom, {}).get('hbm_bytes_per_launch')
+        traffic_K = tj.get('steps')
+
+    # the whole 31-step nbody() call from the initial conditions: first
+    # force, the host factors, one pmx_nbody_steps call (its first kick/drift
+    # and binning included)
+    nbody_call = None
+    if not args.no_nbody_call:
+        log('timed nbody(): the full schedule')
+        pa.nbody(ptcl_ic, None, cosmo, conf)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fin, _ = pa.nbody(ptcl_ic, None, cosmo, conf)
+        torch.cuda.synchronize()
+        t_nbody = time.perf_counter() - t0
+        del fin
+        nbody_call = {'steps': nsteps_sched, 's': t_nbody,
+                      'updates_per_s': N * nsteps_sched / t_nbody}
 
     out = {
         'metric': 'particle-updates/s fwd & fwd+adj at 512^3 (1 GPU); p50/p99 mixed-size latency',
@@ -480,27 +597,33 @@ def main():
         'unit': 'particle-updates/s',
         'n_gpus': 1,
         'steps': K,
-        'warmup': args.warmup,
+        'warmup': W,
         'ms_per_step': ms_per_step,
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
         'dtype': 'f32',
-        'data': 'synthetic: 2LPT ICs from seeded white noise (numpy seed 0)',
+        'data': 'synthetic: 2LPT ICs from device white noise (Philox4x32-10, seed 0)',
         'config': {'workload': f'pmwd KDK N-body step, {G}^3 particles on a '
                                f'{conf.mesh_shape[0]}^3 mesh, box {args.box:g} Mpc/h, '
                                f'a_start=1/32, 31-step schedule, sigma8=0.8, Omega_m=0.3',
                    'ptcl_grid': G, 'mesh': conf.mesh_shape[0], 'box_mpc_h': args.box,
                    'parallelism': 'single GPU'},
+        'timed_steps': {'first': i0, 'count': K, 'untimed_before': i0,
+                        'a_from': float(a[i0 % nsteps_sched]), 'a_to': float(a_to),
+                        'schedule_wraps': i0 + K > nsteps_sched},
         'roofline': {'bound': 'hbm', 'kernel': dom,
                      'achieved': kernels[dom]['alg_gbs'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': kernels[dom]['alg_gbs'] / HBM_PEAK_GBS, 'traffic': traffic,
+                     'traffic_steps': traffic_K,
                      'algorithmic_bytes_per_launch': kernels[dom]['alg_bytes_per_launch'],
                      'ms_per_launch': kernels[dom]['ms_per_launch']},
         'whole_step_roofline': {'bytes_per_step': B_step,
+                                'bytes_per_particle_update': B_step / N,
                                 'frac': B_step / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         'kernel_groups': groups,
         'kernels': kernels,
+        'nbody_call': nbody_call,
         't_ic_s': t_ic,
     }
 
@@ -509,7 +632,7 @@ def main():
         out['fft_yardstick'] = fft_yardstick(conf, ctx, h, L, _lib, dev)
 
     if not args.no_adjoint:
-        log(f'timed adjoint: {args.steps} steps')
+        log(f'timed adjoint: {K} steps')
         dc = torch.randn_like(disp) * 1e-3
         vc = torch.randn_like(vel) * 1e-3
         acc_cot = torch.empty_like(disp)
@@ -527,10 +650,10 @@ def main():
         _lib.check(L.pmx_timing_enable(h, 1))
         _lib.timing_read(ctx)
         t0 = time.perf_counter()
-        # K reverse KDK steps in one pmx_nbody_steps_adj call (particles and
-        # cotangents kept in tile order), the per-step float64 dots the
-        # Omega_m cotangent needs read back at the end as nbody's adjoint
-        # does, then the pending closing kick
+        # K reverse KDK steps from a = 1 in one pmx_nbody_steps_adj call
+        # (particles and cotangents kept in tile order), the per-step float64
+        # dots the Omega_m cotangent needs read back at the end as nbody's
+        # adjoint does, then the pending closing kick
         f = np.array([adj_sched[i % nsteps_sched] for i in range(K)], dtype=np.float32)
         fa, fd, fb = (np.ascontiguousarray(f[:, m]) for m in range(3))
         adots = (ctypes.c_double * (2 * K))()
@@ -549,6 +672,18 @@ def main():
         out['adj_ms_per_step'] = 1e3 * t_adj / K
         out['adj_kernels'] = {k: dict(ms_per_step=v[0] / K, launches_per_step=v[1] / K)
                               for k, v in atiming.items() if v[1]}
+        ab = adj_step_bytes(N, M, Mh)
+        agroups = {}
+        for gname, classes in ADJ_GROUPS.items():
+            ms = sum(atiming[c][0] for c in classes if c in atiming) / K
+            agroups[gname] = dict(ms=ms, alg_bytes=ab[gname],
+                                  gbs=ab[gname] / (ms * 1e-3) / 1e9 if ms > 0 else None)
+            agroups[gname]['frac'] = agroups[gname]['gbs'] / HBM_PEAK_GBS if ms > 0 else None
+        out['adj_kernel_groups'] = agroups
+        B_adj = sum(ab.values())
+        out['adj_step_roofline'] = {'bytes_per_step': B_adj, 'bytes_per_particle_update': B_adj / N,
+                                    'frac': B_adj / (1e-3 * out['adj_ms_per_step']) / 1e9
+                                    / HBM_PEAK_GBS}
 
     if not args.no_serve:
         log(f'serving: {args.serve_requests} requests at {args.serve_rate}/s')
@@ -556,11 +691,14 @@ def main():
 
     if not args.no_cpu_baseline:
         log('cpu baseline sample (numpy oracle)')
-        upd, info = cpu_baseline(conf, cosmo.Omega_m, args.cpu_slab)
-        out['cpu_baseline'] = {'value': upd, 'unit': 'particle-updates/s', 'cores': info['cores'],
-                               'kind': 'port', 'sample': info['sample'],
-                               't_step_s': info['t_step_s']}
+        dens_ic = pa.scatter(ptcl_ic, conf)
+        upd, upd_adj, info = cpu_baseline(conf, cosmo, ptcl_ic, dens_ic, args.cpu_slab)
+        out['cpu_baseline'] = {'value': upd, 'unit': 'particle-updates/s',
+                               'cores': info.pop('threads'), 'kind': 'port',
+                               'sample': info.pop('sample'), 'fwd_adj_value': upd_adj, **info}
         out['speedup_vs_cpu'] = value / upd
+        if 'fwd_adj_value' in out:
+            out['fwd_adj_speedup_vs_cpu'] = out['fwd_adj_value'] / upd_adj
 
     print(json.dumps(out))
 

@@ -193,6 +193,19 @@ int pmx_gather_cic3(pmx_ctx* ctx, const int16_t* pmid, const float* disp, long l
 int pmx_rfftn(pmx_ctx* ctx, const float* in, float* out_c);
 int pmx_irfftn(pmx_ctx* ctx, const float* in_c, float* out);
 
+/* ---------------------------------------------------- power spectrum ---
+ * The binning of pmwd/spec_util.py:47-147 (powspec) on the context's mesh:
+ * the unnormalised r2c of f (and of g for a cross spectrum), per complex
+ * cell |k| of fftfreq(shape, None) in float32, P = |F|^2 or F conj(G)
+ * (times sinc(k_d)^-deconv per axis when deconv > 0), the mode weight N (1
+ * on the kz = 0 and even-n2 Nyquist planes, else 2) and the bin among the
+ * float64 edges (host array, nedges <= 2048; right = 1: right-inclusive).
+ * out (host, 4*nedges doubles): per bin the float64 sums of k N, Re(P) N,
+ * Im(P) N and N; cells beyond the last edge are dropped.  The caller
+ * forms the averages, cuts and units as powspec does.                      */
+int pmx_powspec(pmx_ctx* ctx, const float* f, const float* g, int nedges,
+                const double* edges, int right, int deconv, double* out);
+
 /* ----------------------------------------------------------- gravity ---
  * Force meshes of pmwd/gravity.py:51-67: g_d = irfftn(neg_grad(k_d,
  * laplace(k, rfftn((dens - 1) * coef)))), coef = 1.5*Omega_m in float32.

@@ -275,13 +275,24 @@ def gather_adj(pmid, disp, cell_size, mesh_shape, mesh, val_cot, offset=0,
 
 # ---------------- FFT and Poisson (pm_util.py, gravity.py) ----------------
 
+def _jax_fftfreq(n, rfft=False):
+    """jnp.fft.fftfreq / rfftfreq: integer frequencies divided by n in
+    float64 (numpy's multiplies by 1/n instead, which can differ in the last
+    bit when n is not a power of two)."""
+    if rfft:
+        j = np.arange(n // 2 + 1, dtype=np.float64)
+    else:
+        j = np.concatenate([np.arange(0, (n - 1) // 2 + 1), np.arange(-(n // 2), 0)]).astype(np.float64)
+    return j / n
+
+
 def fftfreq(shape, spacing, dtype=np.float64):
     """Sparse angular wavevectors (pm_util.py:159-199), computed in float64."""
     period = 1.0 if spacing is None else 2 * np.pi / spacing
     kvec = []
     for s in shape[:-1]:
-        kvec.append((np.fft.fftfreq(s) * period).astype(dtype))
-    kvec.append((np.fft.rfftfreq(shape[-1]) * period).astype(dtype))
+        kvec.append((_jax_fftfreq(s) * period).astype(dtype))
+    kvec.append((_jax_fftfreq(shape[-1], rfft=True) * period).astype(dtype))
     return np.meshgrid(*kvec, sparse=True, indexing='ij')
 
 
@@ -596,3 +607,24 @@ def from_pos(pos_, cell_size, mesh_shape, wrap=True, pmid_dtype=np.int16, float_
     if wrap:
         pmid = np.mod(pmid, np.asarray(mesh_shape, dtype=pmid_dtype)).astype(pmid_dtype)
     return pmid, disp
+
+
+# ---------- pieces of one force for the timed CPU baseline (bench.py) ----------
+
+def force_particle_setup(pmid, disp, cell_size, mesh_shape, ptcl_num, dtype=np.float32):
+    """The particle half of a force before the Poisson solve: the shared CIC
+    geometry and the particles' contribution to the density (gravity's
+    scatter, values mesh_size / ptcl_num).  Returns (cics, density part)."""
+    pmid = np.asarray(pmid)
+    disp = np.asarray(disp, dtype=dtype)
+    M = int(np.prod(mesh_shape))
+    val = np.asarray(M / ptcl_num, dtype=dtype)
+    cics = _cic_chunks(pmid, disp, cell_size, mesh_shape)
+    dens = _scatter_cic(cics, M, lambda s, e, w: val * w, dtype).reshape(mesh_shape)
+    return cics, dens
+
+
+def force_particle_readout(cics, g, n, dtype=np.float32):
+    """The particle half of a force after the Poisson solve: the three
+    readouts of gravity (gather.py:33-77) from the force meshes g."""
+    return _gather_cic(cics, np.stack([gi.ravel() for gi in g], axis=-1), n, dtype, C=3)

@@ -93,6 +93,7 @@ _SIGS = {
     'pmx_poisson_grad_adj': [c_p, c_p, c_f, c_p],
     'pmx_white_noise': [c_p, ctypes.c_ulonglong, c_ll, c_p],
     'pmx_random_bits': [c_p, ctypes.c_ulonglong, c_ll, c_p],
+    'pmx_powspec': [c_p, c_p, c_p, c_int, c_p, c_int, c_int, c_p],
     'pmx_rfftn': [c_p, c_p, c_p],
     'pmx_irfftn': [c_p, c_p, c_p],
     'pmx_poisson_grad': [c_p, c_p, c_f, c_p],

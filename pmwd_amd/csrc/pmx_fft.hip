@@ -1460,3 +1460,137 @@ extern "C" int pmx_poisson_grad_adj(pmx_ctx* c, const float* grad_cot, float coe
   if ((rc = pmx_poisson_adj_ctx(c, (float)(-(double)coef / (double)c->M), 0))) return rc;
   return pmx_copy_rows(c, dens_cot, c->n[2], (const float*)c->cbuf[0], c->rs);
 }
+
+// ------------------------------------------------------- power spectrum --
+// pmwd/spec_util.py:47-147: per complex cell of the r2c layout, |k| from
+// fftfreq(shape, None) in float32 (the integer frequency divided by the
+// extent in float64, then rounded, as jnp.fft.fftfreq), P = |F|^2 or
+// F conj(G) in float32 (optionally deconvolved by sinc^-deconv per axis),
+// the weight N = 2 (1 on the kz = 0 and, for an even n2, Nyquist planes),
+// and the bin of |k| among the float64 edges (right: edge < k counts; else
+// edge <= k).  Each workgroup sums k N, Re P N, Im P N and N per bin in
+// float64 in LDS (atomic adds: their order within a workgroup is not
+// fixed) and writes its partials; a second kernel adds the partials in
+// workgroup order.
+#define PS_TPB 256
+#define PS_BLOCKS 1024
+
+__device__ __forceinline__ float ps_freq(int j, int n, bool rfft) {
+  const int jj = (!rfft && j >= (n + 1) / 2) ? j - n : j;
+  return (float)((double)jj / (double)n);
+}
+
+__device__ __forceinline__ float ps_sinc(float x) {
+  if (x == 0.f) return 1.f;
+  const float px = 3.14159265358979323846f * x;
+  return sinf(px) / px;
+}
+
+__global__ void __launch_bounds__(PS_TPB)
+    powspec_bin_k(const float2* __restrict__ F, const float2* __restrict__ G, int n0, int n1,
+                  int n2, int n2h, int n2p, const double* __restrict__ edges, int nedges, int right,
+                  int deconv, double* __restrict__ part) {
+  extern __shared__ double ps_sh[];
+  double* acc = ps_sh;                 // 4 * nedges
+  double* e = ps_sh + 4 * nedges;      // nedges
+  for (int i = threadIdx.x; i < 4 * nedges; i += PS_TPB) acc[i] = 0.0;
+  for (int i = threadIdx.x; i < nedges; i += PS_TPB) e[i] = edges[i];
+  __syncthreads();
+  const long long total = (long long)n0 * n1 * n2h;
+  for (long long idx = (long long)blockIdx.x * PS_TPB + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * PS_TPB) {
+    const int kz = (int)(idx % n2h);
+    const long long r = idx / n2h;
+    const int ky = (int)(r % n1);
+    const int kx = (int)(r / n1);
+    const float2 f = F[((long long)kx * n1 + ky) * n2p + kz];
+    float pr, pi = 0.f;
+    if (G) {
+      const float2 gg = G[((long long)kx * n1 + ky) * n2p + kz];
+      pr = f.x * gg.x + f.y * gg.y;
+      pi = f.y * gg.x - f.x * gg.y;
+    } else {
+      pr = f.x * f.x + f.y * f.y;
+    }
+    const float fx = ps_freq(kx, n0, false), fy = ps_freq(ky, n1, false),
+                fz = ps_freq(kz, n2, true);
+    const float k = sqrtf((fx * fx + fy * fy) + fz * fz);
+    if (deconv) {
+      const float sx = powf(ps_sinc(fx), (float)-deconv);
+      const float sy = powf(ps_sinc(fy), (float)-deconv);
+      const float sz = powf(ps_sinc(fz), (float)-deconv);
+      pr = ((pr * sx) * sy) * sz;
+      pi = ((pi * sx) * sy) * sz;
+    }
+    // number of edges below k (right) or at most k (left-inclusive bins)
+    const double kd = (double)k;
+    int lo = 0, hi = nedges;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      const bool below = right ? (e[mid] < kd) : (e[mid] <= kd);
+      if (below) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo >= nedges) continue;
+    const float w = (kz == 0 || ((n2 & 1) == 0 && kz == n2 / 2)) ? 1.f : 2.f;
+    atomicAdd(&acc[lo], (double)(k * w));
+    atomicAdd(&acc[nedges + lo], (double)(pr * w));
+    if (G) atomicAdd(&acc[2 * nedges + lo], (double)(pi * w));
+    atomicAdd(&acc[3 * nedges + lo], (double)w);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 4 * nedges; i += PS_TPB)
+    part[(long long)blockIdx.x * 4 * nedges + i] = acc[i];
+}
+
+__global__ void powspec_reduce_k(const double* __restrict__ part, int nblocks, int m,
+                                 double* __restrict__ out) {
+  const int i = blockIdx.x * PS_TPB + threadIdx.x;
+  if (i >= m) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += part[(long long)b * m + i];
+  out[i] = s;
+}
+
+// r2c of a caller mesh into context buffer cb (padded layout, all axes)
+static int ps_r2c(pmx_ctx* c, const float* in, int cb) {
+  int rc;
+  float2* f[3] = {c->cbuf[cb], c->cbuf[0], c->cbuf[0]};
+  if ((rc = pmx_copy_rows(c, (float*)f[0], c->rs, in, c->n[2]))) return rc;
+  if ((rc = pmx_fft_r2c_inplace(c, (float*)f[0], 0, 0.f))) return rc;
+  if ((rc = pmx_fft_strided(c, f, 1, 1, -1))) return rc;
+  return pmx_fft_strided(c, f, 1, 0, -1);
+}
+
+extern "C" int pmx_powspec(pmx_ctx* c, const float* f, const float* g, int nedges,
+                           const double* edges, int right, int deconv, double* out) {
+  PMX_ENTRY(c);
+  PMX_REQUIRE(c && f && edges && out, "pmx_powspec: NULL argument");
+  PMX_REQUIRE(nedges >= 1 && nedges <= 2048, "pmx_powspec: 1..2048 bin edges");
+  PMX_REQUIRE(deconv >= 0 && deconv <= 16, "pmx_powspec: deconv must be in 0..16");
+  int rc;
+  // the binned sums are read back on the host
+  if ((rc = pmx_no_capture(c, "pmx_powspec"))) return rc;
+  const int m = 4 * nedges;
+  // partial sums of the workgroups, then the reduced sums and the edges
+  const long long need = (long long)PS_BLOCKS * m + m + nedges;
+  if ((rc = pmx_ensure_red(c, need))) return rc;
+  double* part = c->red;
+  double* sums = c->red + (long long)PS_BLOCKS * m;
+  double* dedges = sums + m;
+  PMX_CHECK_HIP(hipMemcpyAsync(dedges, edges, (size_t)nedges * sizeof(double),
+                               hipMemcpyHostToDevice, c->stream));
+  if ((rc = ps_r2c(c, f, 1))) return rc;
+  if (g && (rc = ps_r2c(c, g, 2))) return rc;
+  const size_t lds = (size_t)(m + nedges) * sizeof(double);
+  powspec_bin_k<<<PS_BLOCKS, PS_TPB, lds, c->stream>>>(
+      c->cbuf[1], g ? c->cbuf[2] : nullptr, c->n[0], c->n[1], c->n[2], c->n2h, c->n2p, dedges,
+      nedges, right, deconv, part);
+  PMX_LAUNCH_CHECK();
+  powspec_reduce_k<<<(m + PS_TPB - 1) / PS_TPB, PS_TPB, 0, c->stream>>>(part, PS_BLOCKS, m, sums);
+  PMX_LAUNCH_CHECK();
+  PMX_CHECK_HIP(hipMemcpyAsync(out, sums, (size_t)m * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+  PMX_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return PMX_OK;
+}

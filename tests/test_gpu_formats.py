@@ -136,3 +136,25 @@ def test_poisson_grad_adj(dev, ms):
     rhs = np.sum((dens.astype(np.float64) - 1) * dc)
     assert abs(lhs - rhs) < 1e-5 * np.sqrt(np.sum(grad.cpu().numpy().astype(np.float64)**2)
                                            * np.sum(gc.astype(np.float64)**2))
+
+
+@pytest.mark.parametrize('shape,mesh', [((16, 16, 16), 2), ((16, 12, 10), 1.5),
+                                        ((9, 7, 5), 3), ((8, 10, 12), (20, 25, 30)),
+                                        ((6, 6, 6), 0.5), ((32, 32, 32), 1)])
+@pytest.mark.parametrize('spacing', [1.0, 1.953125, 0.7])
+def test_gen_grid_matches_oracle(dev, shape, mesh, spacing):
+    """Particles.gen_grid (particles.py:109-144) for integer, non-integer and
+    explicit mesh shapes: pmid and disp bit for bit against the oracle, and
+    zero velocities/accelerations on request."""
+    import pmwd_amd as pa
+    conf = pa.Configuration(spacing, shape, mesh_shape=mesh)
+    p = pa.Particles.gen_grid(conf, vel=True, acc=True, device=dev)
+    pmid, disp = opm.gen_grid(shape, conf.mesh_shape, conf.cell_size)
+    np.testing.assert_array_equal(p.pmid.cpu().numpy(), pmid)
+    np.testing.assert_array_equal(p.disp.cpu().numpy(), disp)
+    assert not p.vel.any() and not p.acc.any()
+    # the Lagrangian positions are the particle grid
+    pos = p.pos(dtype=torch.float64, wrap=False).cpu().numpy()
+    grid = np.stack(np.meshgrid(*[np.arange(s) * spacing for s in shape], indexing='ij'),
+                    -1).reshape(-1, 3)
+    np.testing.assert_allclose(pos, grid, rtol=0, atol=1e-6 * max(conf.box_size))

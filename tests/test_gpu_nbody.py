@@ -208,6 +208,39 @@ def test_nbody_adjoint_dark_energy_cotangents(dev, de):
     assert (cc.A_s_1e9, cc.n_s, cc.Omega_b, cc.h) == (0.0, 0.0, 0.0, 0.0)
 
 
+@pytest.mark.parametrize('splits', [((0.5, 1), (0.5, 0)), ((0, 0.25), (0.5, 0.5), (0.5, 0.25))])
+def test_nbody_other_symplectic_splits_vs_oracle(dev, splits):
+    """integrate with other symp_splits (nbody.py:121-140,
+    configuration.py:144, 187-191) -- DKD and a three-stage splitting -- run op
+    by op on the GPU: the forward against the oracle (disp 1e-5, vel and acc
+    1e-4), the reverse-time adjoint against the oracle's (1e-3, and the
+    Omega_m cotangent), and forward-then-reverse back to the start."""
+    import pmwd_amd as pa
+    shape = (16, 16, 16)
+    kw = dict(mesh_shape=2, a_start=1 / 8, a_nbody_maxstep=1 / 8, symp_splits=splits)
+    conf = pa.Configuration(4.0, shape, **kw)
+    oconf = OConf(4.0, shape, **kw)
+    cosmo = pa.boltzmann(pa.SimpleLCDM(conf), conf)
+    ocosmo = oc.boltzmann(oc.SimpleLCDM(oconf), oconf)
+    pmid, disp = make_ptcl(shape, conf.mesh_shape, conf.cell_size, 0.5, 21)
+    vel = (0.05 * np.random.default_rng(22).standard_normal(disp.shape)).astype(np.float32)
+    P = lambda a: torch.as_tensor(a, device=dev)
+    out, _ = pa.nbody(pa.Particles(conf, P(pmid), P(disp), P(vel)), None, cosmo, conf)
+    wd, wv, wa = onb.nbody(pmid, disp, vel, ocosmo, oconf)
+    assert rel_l2(out.disp.cpu().numpy(), wd) < 1e-5
+    assert rel_l2(out.vel.cpu().numpy(), wv) < 1e-4
+    assert rel_l2(out.acc.cpu().numpy(), wa) < 1e-4
+    rng = np.random.default_rng(23)
+    dcot = rng.standard_normal(disp.shape).astype(np.float32)
+    vcot = rng.standard_normal(disp.shape).astype(np.float32)
+    p0, pc, cc = pa.nbody_adj(out, pa.Particles(conf, P(pmid), P(dcot), P(vcot)), cosmo, conf)
+    (r0d, _, _), (wdc, wvc, _), wom = onb.nbody_adj(pmid, wd, wv, wa, dcot, vcot, ocosmo, oconf)
+    assert rel_l2(pc.disp.cpu().numpy(), wdc) < 1e-3
+    assert rel_l2(pc.vel.cpu().numpy(), wvc) < 1e-3
+    assert abs(cc.Omega_m - wom) < 1e-3 * abs(wom)
+    assert rel_l2(p0.disp.cpu().numpy(), disp) < 1e-4
+
+
 def test_nbody_autograd_matches_adjoint(dev):
     pa, conf, oconf, cosmo, ocosmo = small_setup(shape=(8, 8, 8), a_start=1 / 4, steps_max=1 / 4)
     pmid, disp = make_ptcl(conf.ptcl_grid_shape, conf.mesh_shape, conf.cell_size, 0.5, 6)

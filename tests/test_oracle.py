@@ -567,3 +567,40 @@ def test_philox_vectorised_and_normals():
     z2 = ic.normals(4, np.arange(1 << 21)).astype(np.float64)
     assert abs(np.mean(z * z2)) < 5 / np.sqrt(n)
     assert abs(np.mean(z[1:] * z[:-1])) < 5 / np.sqrt(n)
+
+
+# ---------------------------------------------------- power spectrum ---
+
+def test_powspec_white_noise_and_parseval():
+    """Unit white noise has P = spacing^3 in every bin (within five standard
+    errors, sqrt(2/N) of the mean); with no cuts the binned power summed
+    over all modes equals sum f^2 (Parseval) and the mode counts add up to
+    the full grid."""
+    from oracle import spec
+    for shape in ((32, 32, 32), (24, 20, 17)):
+        f = np.random.default_rng(1).standard_normal(shape)
+        h = 1.5
+        k, P, N, bins = spec.powspec(f, h, bins=1.0, cut_zero=False, cut_nyq=False)
+        M = np.prod(shape)
+        assert N.sum() == M
+        assert np.isclose((P * N).sum() * M / h**3, M * np.sum(f**2), rtol=1e-10)
+        k, P, N, bins = spec.powspec(f, h)
+        assert np.all(np.abs(P / h**3 - 1) < 5 * np.sqrt(2 / N) + 1e-12)
+        assert np.all(np.diff(k) > 0) and np.all((k[1:] > bins[1:len(k)]))
+
+
+def test_powspec_plane_wave():
+    """f = A cos(2 pi m x / n) puts all power in the bin holding |k| = 2 pi m
+    / (n h): that bin's P is 2 (A n^3 / 2)^2 / N_bin h^3 / n^3, every other
+    bin's is zero, and its mean k includes the wave's modes with weight 1
+    each (kz = 0 plane)."""
+    from oracle import spec
+    n, m, A, h = 32, 3, 0.7, 2.0
+    x = np.arange(n)
+    f = np.broadcast_to(A * np.cos(2 * np.pi * m * x / n)[:, None, None], (n, n, n))
+    k, P, N, bins = spec.powspec(f, h, bins=1.0)
+    kw = 2 * np.pi * m / (n * h)
+    b = int(np.argmin(np.abs(k - kw)))
+    want = 2 * (A * n**3 / 2)**2 / N[b] * h**3 / n**3
+    assert np.isclose(P[b], want, rtol=1e-10)
+    assert np.allclose(np.delete(P, b), 0, atol=1e-20 * want)
