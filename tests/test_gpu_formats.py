@@ -140,7 +140,7 @@ def test_poisson_grad_adj(dev, ms):
 
 @pytest.mark.parametrize('shape,mesh', [((16, 16, 16), 2), ((16, 12, 10), 1.5),
                                         ((9, 7, 5), 3), ((8, 10, 12), (20, 25, 30)),
-                                        ((6, 6, 6), 0.5), ((32, 32, 32), 1)])
+                                        ((6, 6, 6), 4.5), ((32, 32, 32), 1)])
 @pytest.mark.parametrize('spacing', [1.0, 1.953125, 0.7])
 def test_gen_grid_matches_oracle(dev, shape, mesh, spacing):
     """Particles.gen_grid (particles.py:109-144) for integer, non-integer and
