@@ -298,6 +298,27 @@ int pmx_nbody_steps_adj(pmx_ctx* ctx, const int16_t* pmid, float* disp,
 /* float64 dot product of two float32 arrays of length n (host value).      */
 int pmx_dot(pmx_ctx* ctx, const float* a, const float* b, long long n, double* out);
 
+/* ------------------------------------------------- host cosmology tables ---
+ * Host-only float64 functions (no context, no device).
+ *
+ * pmx_growth_ode replaces the growth integration of pmwd/boltzmann.py:163-229
+ * (odeint of pmwd/ode_util.py:60-186 on the 2LPT growth ODE): the state
+ * (G1, G1', G2, G2', S1, S1', S2, S2') in ln a from y0 at lna[0], S the
+ * Omega_m forward sensitivities, reported at each of the na non-decreasing
+ * lna; adaptive Dormand-Prince 5(4) with the error norm over the first nerr
+ * components, first step dt0 (<= 0: the Hairer-Norsett-Wanner estimate).
+ * Y and Yd (na x 8, row-major) receive the state and its ln a derivative.
+ *
+ * pmx_tophat_var replaces the variance integral of pmwd/boltzmann.py:272-291
+ * (mcfit.TophatVar there): sigma^2(R) = int k^3 P W(kR)^2 / (2 pi^2) dln k
+ * over nk log-spaced k with P(k) given, Simpson weights (trapezoid for an
+ * even nk), for each of nR radii.                                           */
+int pmx_growth_ode(double Omega_m, double Omega_k, double w_0, double w_a,
+                   const double* lna, int na, const double* y0, double rtol,
+                   double atol, double dt0, int nerr, double* Y, double* Yd);
+int pmx_tophat_var(const double* k, const double* P, int nk, const double* R,
+                   int nR, double* out);
+
 /* ------------------------------------------------------------- timing ---
  * Kernel timing with HIP events recorded on the context stream around each
  * launch of the hot path, grouped in PMX_NTIMER_CLASSES classes (bin, scan,

@@ -110,6 +110,8 @@ _SIGS = {
     'pmx_nbody_steps_adj': [c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_ll, c_f, c_f, c_f, c_int,
                             c_p, c_p, c_p, c_p, ctypes.POINTER(c_d)],
     'pmx_dot': [c_p, c_p, c_p, c_ll, ctypes.POINTER(c_d)],
+    'pmx_growth_ode': [c_d, c_d, c_d, c_d, c_p, c_int, c_p, c_d, c_d, c_d, c_int, c_p, c_p],
+    'pmx_tophat_var': [c_p, c_p, c_int, c_p, c_int, c_p],
     'pmx_timing_enable': [c_p, c_int],
     'pmx_timing_read': [c_p, ctypes.POINTER(c_d), ctypes.POINTER(c_ll)],
 }

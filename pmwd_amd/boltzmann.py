@@ -5,12 +5,13 @@ reference does (its ``cosmo_dtype``).  ``growth_integ`` additionally
 integrates the forward sensitivity of the growth ODE to Omega_m on the same
 steps, which the N-body adjoint uses for the cosmology cotangent.  The
 top-hat variance is a Simpson quadrature in ln k (pmwd uses mcfit's FFTLog).
+The growth ODE and the variance integral run in the library's host code
+(``pmx_growth_ode``, ``pmx_tophat_var``), a few milliseconds per cosmology.
 """
+import ctypes
+
 import numpy as np
 
-from .cosmology import (E2, E2_da, H_deriv, Omega_m_a, H_deriv_dOm, Omega_m_a_dOm,
-                        E2_dOm)
-from .ode_util import odeint
 
 
 def transfer_fit(k, cosmo, conf):
@@ -84,29 +85,6 @@ def transfer(k, cosmo, conf):
     return np.interp(np.asarray(k, dtype=np.float64), conf.transfer_k, cosmo.transfer)
 
 
-def _growth_rhs(cosmo):
-    """RHS of the 2LPT growth ODE in ln a plus its Omega_m sensitivity.
-
-    State (G1, G1', G2, G2', S1, S1', S2, S2') with S = dG/dOmega_m
-    (boltzmann.py:197-205 for the first four).
-    """
-    def f(y, lna):
-        a = np.exp(lna)
-        h = H_deriv(a, cosmo)
-        om = 1.5 * Omega_m_a(a, cosmo)
-        dh = H_deriv_dOm(a, cosmo)
-        dom = 1.5 * Omega_m_a_dOm(a, cosmo)
-        G1, G1p, G2, G2p, S1, S1p, S2, S2p = y
-        G1pp = -(3 + h - om) * G1 - (4 + h) * G1p
-        G2pp = om * G1**2 - (8 + 2 * h - om) * G2 - (6 + h) * G2p
-        S1pp = (-(3 + h - om) * S1 - (4 + h) * S1p
-                - (dh - dom) * G1 - dh * G1p)
-        S2pp = (2 * om * G1 * S1 - (8 + 2 * h - om) * S2 - (6 + h) * S2p
-                + dom * G1**2 - (2 * dh - dom) * G2 - dh * G2p)
-        return np.array([G1p, G1pp, G2p, G2pp, S1p, S1pp, S2p, S2pp])
-    return f
-
-
 def growth_integ(cosmo, conf):
     """Tabulate growth functions at ``conf.growth_a`` (boltzmann.py:163-229).
 
@@ -119,12 +97,16 @@ def growth_integ(cosmo, conf):
         a_ic = 0.1 * conf.a_lpt_step
     a = conf.growth_a.copy()
     a[0] = a_ic
-    lna = np.log(a)
-    f = _growth_rhs(cosmo)
+    lna = np.ascontiguousarray(np.log(a))
     y0 = np.array([1, 0, 3 / 7, 0, 0, 0, 0, 0], dtype=np.float64)
-    Y = odeint(f, y0, lna, rtol=conf.growth_rtol, atol=conf.growth_atol,
-               dt0=conf.growth_inistep, nerr=4)
-    Yd = np.stack([f(y, t) for y, t in zip(Y, lna)])
+    Y = np.empty((len(lna), 8))
+    Yd = np.empty((len(lna), 8))
+    dt0 = conf.growth_inistep
+    dt0 = dt0[0] if isinstance(dt0, tuple) else dt0
+    _host_call('pmx_growth_ode', float(cosmo.Omega_m), float(cosmo.Omega_k), float(cosmo.w_0),
+               float(cosmo.w_a), _dptr(lna), len(lna), _dptr(y0), float(conf.growth_rtol),
+               float(conf.growth_atol), float(dt0) if dt0 is not None else 0.0, 4, _dptr(Y),
+               _dptr(Yd))
     m = np.array([1.0, 2.0])[:, None]
 
     def table(G, Gp, Gpp):
@@ -169,24 +151,35 @@ def linear_power(k, a, cosmo, conf):
     return P
 
 
+def _dptr(x):
+    return x.ctypes.data_as(ctypes.c_void_p)
+
+
+def _host_call(name, *args):
+    """A host-only function of the library (float64 tables, no device)."""
+    from . import _lib
+    _lib.check(getattr(_lib.lib(), name)(*args))
+
+
 def _tophat_var(R, cosmo, conf):
-    k = conf.transfer_k[1:]
-    P = linear_power(k, None, cosmo, conf)
-    lnk = np.log(k)
-    R = np.atleast_1d(np.asarray(R, dtype=np.float64))
-    x = np.outer(R, k)
-    small = x < 1e-3
-    xs = np.where(small, 1.0, x)
-    W = np.where(small, 1 - x**2 / 10, 3 * (np.sin(xs) - xs * np.cos(xs)) / xs**3)
-    integrand = k**3 * P / (2 * np.pi**2) * W**2
-    n = integrand.shape[1]
-    h = lnk[1] - lnk[0]
-    if n % 2 == 1:
-        w = np.full(n, 2.0)
-        w[1::2] = 4.0
-        w[0] = w[-1] = 1.0
-        return integrand @ w * (h / 3)
-    return np.trapezoid(integrand, lnk, axis=1)
+    """sigma^2(R) of the linear power at a=1: Simpson in ln k over
+    ``conf.transfer_k[1:]`` (trapezoid for an even node count)."""
+    k = np.ascontiguousarray(conf.transfer_k[1:])
+    P = np.ascontiguousarray(linear_power(k, None, cosmo, conf))
+    R = np.ascontiguousarray(np.atleast_1d(np.asarray(R, dtype=np.float64)))
+    out = np.empty(len(R))
+    _host_call('pmx_tophat_var', _dptr(k), _dptr(P), len(k), _dptr(R), len(R), _dptr(out))
+    return out
+
+
+def sigma2_at(R, cosmo, conf):
+    """np.interp(R, conf.varlin_R, varlin table) computed from the two table
+    nodes around R only (the interpolation reads no other node)."""
+    grid = conf.varlin_R
+    i = int(np.searchsorted(grid, float(R), side='right'))
+    i = min(max(i, 1), len(grid) - 1)
+    v = _tophat_var(grid[i - 1:i + 1], cosmo, conf)
+    return float(np.interp(float(R), grid[i - 1:i + 1], v))
 
 
 def varlin_integ(cosmo, conf):

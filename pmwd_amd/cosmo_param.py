@@ -48,13 +48,13 @@ def check_theta(theta, cosmo):
 
 def at_fixed_sigma8(cosmo, conf, Omega_m):
     """The cosmology at another Omega_m with A_s renormalised so that sigma8
-    is unchanged (the from_sigma8 parametrisation), tables included; the
-    variance table scales with A_s exactly."""
+    is unchanged (the from_sigma8 parametrisation), with its transfer and
+    growth tables (no variance table)."""
     s8 = cosmo.sigma8
     c = boltzmann(cosmo.replace(Omega_m=float(Omega_m), transfer=None, growth=None,
-                                growth_dOm=None, varlin=None), conf)
+                                growth_dOm=None, varlin=None), conf, varlin=False)
     scale = (s8 / c.sigma8)**2
-    return c.replace(A_s_1e9=c.A_s_1e9 * scale, varlin=c.varlin * scale)
+    return c.replace(A_s_1e9=c.A_s_1e9 * scale)
 
 
 class _HostValue(torch.autograd.Function):

@@ -81,7 +81,9 @@ class Cosmology:
         """Construct with sigma8 instead of A_s (cosmology.py:95-105)."""
         from .boltzmann import boltzmann
         cosmo = cls(conf, 1, *args, **kwargs)
-        cosmo = boltzmann(cosmo, conf)
+        # sigma8 reads the transfer and growth tables and two nodes of the
+        # variance table, not the whole variance table
+        cosmo = boltzmann(cosmo, conf, varlin=False)
         A_s_1e9 = (sigma8 / cosmo.sigma8)**2
         return cls(conf, A_s_1e9, *args, **kwargs)
 
@@ -119,9 +121,15 @@ class Cosmology:
 
     @property
     def sigma8(self):
-        from .boltzmann import varlin
+        """sqrt(varlin(8 Mpc/h, a=1)): the variance table's interpolation at
+        8 Mpc/h, from its two nodes around it (the table itself need not be
+        built)."""
+        from .boltzmann import growth, sigma2_at
+        if self.transfer is None or self.growth is None:
+            raise ValueError('sigma8 needs the transfer and growth tables: call boltzmann first.')
         R = 8 * self.conf.Mpc_SI / self.conf.L
-        return float(np.sqrt(varlin(R, 1, self, self.conf)))
+        s2 = sigma2_at(R, self, self.conf) * float(growth(1.0, self, self.conf))**2
+        return float(np.sqrt(s2))
 
     @property
     def ptcl_mass(self):

@@ -54,9 +54,25 @@ def _cosmos(Om=0.3, **kw):
 def test_tables_match_oracle():
     conf, c, oconf, o = _cosmos()
     np.testing.assert_allclose(c.transfer, o.transfer, rtol=1e-13)
-    np.testing.assert_allclose(c.growth, o.growth, rtol=1e-10, atol=1e-14)
+    # rtol: the accepted ODE steps follow each implementation's rounding
+    # (see test_growth_accuracy_matches_oracle)
+    np.testing.assert_allclose(c.growth, o.growth, rtol=1e-7, atol=1e-14)
     np.testing.assert_allclose(c.varlin, o.varlin, rtol=1e-12)
     assert abs(c.sigma8 - o.sigma8) < 1e-12
+
+
+def test_growth_accuracy_matches_oracle():
+    """The library's Dormand-Prince integration and the oracle's are equally
+    close to the converged solution (both integrators at rtol = atol = 1e-13):
+    the error of each table entry at the default tolerance agrees within 5%,
+    and the two differ from each other by far less than that error."""
+    conf, c, oconf, o = _cosmos()
+    tconf = conf.replace(growth_rtol=1e-13, growth_atol=1e-13)
+    tight = pa.boltzmann(pa.SimpleLCDM(tconf), tconf).growth
+    err_lib = np.abs(c.growth - tight).reshape(6, -1).max(axis=1)
+    err_orc = np.abs(o.growth - tight).reshape(6, -1).max(axis=1)
+    np.testing.assert_allclose(err_lib, err_orc, rtol=0.05)
+    assert (np.abs(c.growth - o.growth).reshape(6, -1).max(axis=1) < 0.1 * err_orc).all()
 
 
 def test_from_sigma8():
@@ -80,13 +96,14 @@ def test_growth_sensitivity_finite_difference():
 
 
 def test_factors_match_oracle():
+    # rel 1e-7: the factors read the growth tables (test_tables_match_oracle)
     conf, c, oconf, o = _cosmos()
     for a0, a1 in zip(conf.a_nbody[:-1], conf.a_nbody[1:]):
         ah = a0 * 0.5 + a1 * 0.5
         assert pnb.kick_factor(a0, a0, ah, c, conf) == pytest.approx(
-            onb.kick_factor(a0, a0, ah, o, oconf), rel=1e-10)
+            onb.kick_factor(a0, a0, ah, o, oconf), rel=1e-7)
         assert pnb.drift_factor(ah, a0, a1, c, conf) == pytest.approx(
-            onb.drift_factor(ah, a0, a1, o, oconf), rel=1e-10)
+            onb.drift_factor(ah, a0, a1, o, oconf), rel=1e-7)
 
 
 def test_factor_derivatives_finite_difference():
