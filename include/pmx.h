@@ -240,6 +240,41 @@ int pmx_nbody_step(pmx_ctx* ctx, const int16_t* pmid, float* disp, float* vel,
 int pmx_nbody_steps(pmx_ctx* ctx, const int16_t* pmid, float* disp, float* vel,
                     float* acc, long long n, float val, float coef, int nsteps,
                     const float* k1, const float* d, const float* k2);
+/* pmx_nbody_steps with every factor read by the kernels from the device
+ * table fac (float32, 1 + 3*nsteps entries): {1.5*Omega_m, k1[0], d[0],
+ * k2[0], k1[1], d[1], k2[1], ...}; bitwise the same result.  Same loop
+ * (pmwd/nbody.py:215-223); no host copy of the factors is taken, so a
+ * graph that captures the call replays for whatever the table holds.  Needs
+ * n > 0 and mesh axes of at most 1024 cells.                               */
+int pmx_nbody_steps_dev(pmx_ctx* ctx, const int16_t* pmid, float* disp, float* vel,
+                        float* acc, long long n, float val, int nsteps, const float* fac);
+
+/* -------------------------------------------------------- graphs, packs ---
+ * The N-body part of pmwd's forward model (nbody_init's force,
+ * pmwd/nbody.py:193-201, the KDK loop :215-223, then scatter.py:8-83 of the
+ * final state into dens when dens is not NULL) captured into a HIP graph
+ * that the context keeps, factors read from the device table fac as in
+ * pmx_nbody_steps_dev.  Replays (pmx_graph_launch) run on the context's
+ * stream with the buffers given here and whatever fac then holds.  Run the
+ * same call uncaptured once before (pmx_nbody_steps_dev and pmx_scatter_mass
+ * on the same n) so that the context holds all its scratch: nothing may be
+ * allocated under capture.  Timing must be off.  The graph addresses the
+ * context's scratch, so freeing any of it (pmx_ctx_trim, or a later call
+ * that regrows it) drops the graph: pmx_graph_launch then fails until the
+ * next capture.  Packs hold their own copy and must not outlive the
+ * scratch of their contexts either.  A pack launches the graphs
+ * of k contexts (distinct buffers) as independent branches of one graph:
+ * one launch runs k simulations side by side.  pmx_pack_create copies the
+ * graphs; capturing a context again leaves existing packs unchanged.  No
+ * pmwd counterpart: jit dispatch there.                                    */
+typedef struct pmx_pack pmx_pack;
+int pmx_graph_capture(pmx_ctx* ctx, const int16_t* pmid, float* disp, float* vel,
+                      float* acc, long long n, float val, int nsteps, const float* fac,
+                      float* dens);
+int pmx_graph_launch(pmx_ctx* ctx);
+int pmx_pack_create(pmx_ctx* const* ctxs, int k, pmx_pack** out);
+int pmx_pack_launch(pmx_pack* pack, void* stream);
+int pmx_pack_destroy(pmx_pack* pack);
 
 /* ------------------------------------------------------------ adjoint ---
  * force_adj of pmwd/nbody.py:108-118: acc = gravity(disp) and

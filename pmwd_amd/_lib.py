@@ -102,6 +102,12 @@ _SIGS = {
     'pmx_drift': [c_p, c_ll, c_p, c_p, c_f],
     'pmx_nbody_step': [c_p, c_p, c_p, c_p, c_p, c_ll, c_f, c_f, c_f, c_f, c_f],
     'pmx_nbody_steps': [c_p, c_p, c_p, c_p, c_p, c_ll, c_f, c_f, c_int, c_p, c_p, c_p],
+    'pmx_nbody_steps_dev': [c_p, c_p, c_p, c_p, c_p, c_ll, c_f, c_int, c_p],
+    'pmx_graph_capture': [c_p, c_p, c_p, c_p, c_p, c_ll, c_f, c_int, c_p, c_p],
+    'pmx_graph_launch': [c_p],
+    'pmx_pack_create': [c_p, c_int, c_p],
+    'pmx_pack_launch': [c_p, c_p],
+    'pmx_pack_destroy': [c_p],
     'pmx_gravity_vjp': [c_p, c_p, c_p, c_ll, c_f, c_f, c_p, c_p, c_p, ctypes.POINTER(c_d)],
     'pmx_kick_adj': [c_p, c_ll, c_p, c_p, c_p, c_p, c_p, c_f, ctypes.POINTER(c_d)],
     'pmx_drift_adj': [c_p, c_ll, c_p, c_p, c_p, c_p, c_f, ctypes.POINTER(c_d)],

@@ -734,7 +734,12 @@ __global__ void __launch_bounds__(PMX_TPB)
     kick_drift_bin_k(const int16_t* __restrict__ pmid, float* __restrict__ disp,
                      float* __restrict__ vel, const float* __restrict__ acc, long long n, float k1,
                      float dd, MeshGeom g, TileGeom tg, uint32_t* __restrict__ tile_count,
-                     BinOut o) {
+                     BinOut o, const float* __restrict__ fdev) {
+  // factors from the device table when the call has one: {k1, d}
+  if (fdev) {
+    k1 = fdev[0];
+    dd = fdev[1];
+  }
   constexpr int T3 = 3 * PMX_TPB;
   __shared__ __align__(16) float sh[3 * T3];  // vel | acc | disp
   __shared__ __align__(16) int16_t sp[T3];
@@ -774,13 +779,15 @@ __global__ void __launch_bounds__(PMX_TPB)
 int pmx_kick_drift_bin(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, const float* acc,
                        long long n, float k1, float d, uint32_t* count, const BinOut& o, int cap) {
   const MeshGeom g = pmx_geom_padded(c);
+  // the first kick and drift of a call: entries 1 and 2 of a device table
+  const float* fdev = c->dfac ? c->dfac + 1 : nullptr;
   pmx_tic(c, PMX_T_BIN);
   if (cap)
     kick_drift_bin_k<true><<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(pmid, disp, vel, acc, n, k1, d,
-                                                                     g, c->tg, count, o);
+                                                                     g, c->tg, count, o, fdev);
   else
     kick_drift_bin_k<false><<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(pmid, disp, vel, acc, n, k1,
-                                                                      d, g, c->tg, count, o);
+                                                                      d, g, c->tg, count, o, fdev);
   pmx_toc(c, PMX_T_BIN);
   PMX_LAUNCH_CHECK();
   return PMX_OK;

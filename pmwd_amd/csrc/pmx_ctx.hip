@@ -112,6 +112,16 @@ static int dev_alloc(pmx_ctx* c, void** p, size_t bytes) {
 
 static void dev_free(pmx_ctx* c, void** p, size_t bytes) {
   if (*p) {
+    // a captured graph addresses the context's scratch: it goes with it
+    if (c->gexec) {
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipGraphExecDestroy(c->gexec);
+      c->gexec = nullptr;
+    }
+    if (c->graph) {
+      (void)hipGraphDestroy(c->graph);
+      c->graph = nullptr;
+    }
     if (c->dfree) {
       (void)hipStreamSynchronize(c->stream);
       c->dfree(*p, bytes, c->device, (void*)c->stream, c->dalloc_user);
@@ -273,6 +283,8 @@ extern "C" int pmx_ctx_destroy(pmx_ctx* c) {
   if (!c) return PMX_OK;
   hipDeviceSynchronize();
   pmx_timer_destroy(c);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
   // sizes are not needed here: the byte count ends with the context
   auto rel = [c](void* p) {
     if (!p) return;

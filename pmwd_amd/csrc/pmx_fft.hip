@@ -427,8 +427,10 @@ __device__ __forceinline__ void stage_twiddles(float2* stw, const float2* __rest
 template <int PRE, bool POW2>
 __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_k(float* __restrict__ data, long long nrows, int n2,
                                                       int rs, int P, float coef,
-                                                      const float2* __restrict__ tw) {
+                                                      const float2* __restrict__ tw,
+                                                      const float* __restrict__ fcoef) {
   extern __shared__ float2 s[];
+  if (PRE && fcoef) coef = fcoef[0];
   float2* stw = s + lds_elems(ROWS, n2, P);
   long long row0 = (long long)blockIdx.x * P;
   int np = (int)((nrows - row0) < P ? (nrows - row0) : P);
@@ -550,8 +552,10 @@ template <int PRE, int LH, bool FOLD = false>
 __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_half_k(float* __restrict__ data, long long nrows,
                                                            int n2, int rs, int P_, float coef,
                                                            const float2* __restrict__ tw,
-                                                           FoldIn fi) {
+                                                           FoldIn fi, const float* __restrict__ fcoef) {
   extern __shared__ float2 s[];
+  // 1.5 Omega_m from the device factor table when the call has one
+  if (PRE && fcoef) coef = fcoef[0];
   constexpr int PC = LH ? row_pencils(1 << LH) : 1;
   const int H = LH ? 1 << LH : n2 >> 1;
   const int lH = LH ? LH : __ffs(H) - 1;
@@ -1187,6 +1191,7 @@ int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef, const float
   const int P = r.P;
   const size_t lds = r.lds;
   const FoldIn fi = {halo, c->tg.shell, c->tg.ntx, c->tg.nty, c->tg.ntz, c->n[1]};
+  const float* fc = pre ? c->dfac : nullptr;
   PMX_REQUIRE(!halo || pmx_fold_deferrable(c), "z r2c: no fold on load for this mesh");
   pmx_tic(c, PMX_T_ZR2C);
   if (r.half) {
@@ -1194,26 +1199,26 @@ int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef, const float
 #define R2C_HALF(L)                                                                                \
   if (halo && pre)                                                                                 \
     z_r2c_half_k<1, L, (L > 0)><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef,  \
-                                                                  c->tw[2], fi);                   \
+                                                                  c->tw[2], fi, fc);                   \
   else if (halo)                                                                                   \
     z_r2c_half_k<0, L, (L > 0)><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef,  \
-                                                                  c->tw[2], fi);                   \
+                                                                  c->tw[2], fi, fc);                   \
   else if (pre)                                                                                    \
-    z_r2c_half_k<1, L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2], fi); \
+    z_r2c_half_k<1, L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2], fi, fc); \
   else                                                                                             \
-    z_r2c_half_k<0, L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2], fi)
+    z_r2c_half_k<0, L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2], fi, fc)
     PMX_CT_SWITCH(lh, R2C_HALF);
 #undef R2C_HALF
   } else {
     const bool p2 = is_pow2(n2);
     if (pre && p2)
-      z_r2c_k<1, true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
+      z_r2c_k<1, true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2], fc);
     else if (pre)
-      z_r2c_k<1, false><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
+      z_r2c_k<1, false><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2], fc);
     else if (p2)
-      z_r2c_k<0, true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
+      z_r2c_k<0, true><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2], fc);
     else
-      z_r2c_k<0, false><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2]);
+      z_r2c_k<0, false><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2], fc);
   }
   pmx_toc(c, PMX_T_ZR2C);
   PMX_LAUNCH_CHECK();

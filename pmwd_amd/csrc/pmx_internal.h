@@ -219,6 +219,15 @@ struct pmx_ctx {
 
   size_t bytes;        // device bytes held
   int no_alloc;        // 1: scratch carved from static device memory; never allocate
+
+  // Device factor table of the call in progress (pmx_nbody_steps_dev,
+  // pmx_graph_capture), NULL otherwise: {1.5 Omega_m, k1_0, d_0, k2_0, k1_1,
+  // d_1, k2_1, ...} in float32.  The kernels that take a factor read it from
+  // here at run time, so a captured graph replays for any cosmology.
+  const float* dfac;
+  // the graph of the last pmx_graph_capture and its executable
+  hipGraph_t graph;
+  hipGraphExec_t gexec;
 };
 
 // Geometry fields of a context (extents, padded strides, cell size, stream,

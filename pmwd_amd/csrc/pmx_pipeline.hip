@@ -113,8 +113,17 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
               const uint32_t* __restrict__ in_count, MeshGeom g, TileGeom tg,
               const float* __restrict__ F0, const float* __restrict__ F1,
               const float* __restrict__ F2, float k2, float k1n, float dn, PlOut out,
-              PlUser user) {
+              PlUser user, const float* __restrict__ fdev) {
   if ((MODE == PL_COUNT || MODE == PL_WRITE) && out.flag[0] == 0u) return;
+  // factors from the device table when the call has one: this step's
+  // {k1, d, k2} then the next step's {k1, d}
+  if (fdev) {
+    k2 = fdev[2];
+    if (MODE != PL_LAST) {
+      k1n = fdev[3];
+      dn = fdev[4];
+    }
+  }
   extern __shared__ float sh[];
   // PL_CAP: runs per neighbour tile and round (pl_nbr_code, pl_round_bases),
   // so a round's stores into a tile are one run instead of one per wave
@@ -247,7 +256,9 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
 }
 
 template <int MODE>
-static void pl_launch(pmx_ctx* c, int src, PlOut out, PlUser user, float k2, float k1n, float dn) {
+static void pl_launch(pmx_ctx* c, int src, PlOut out, PlUser user, float k2, float k1n, float dn,
+                      int step) {
+  const float* fdev = c->dfac ? c->dfac + 1 + 3 * (long long)step : nullptr;
   const MeshGeom g = pmx_geom_padded(c);
   const TileGeom& tg = c->tg;
   const size_t lds = (size_t)3 * (tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(float);
@@ -255,7 +266,82 @@ static void pl_launch(pmx_ctx* c, int src, PlOut out, PlUser user, float k2, flo
   const unsigned grid = rebin && c->ntiles > PL_REBIN_GRID ? PL_REBIN_GRID : (unsigned)c->ntiles;
   pl_step_k<MODE><<<grid, PL_TPB, lds, c->stream>>>(
       c->pl[src], c->pl_start[src], c->pl_count[src], g, tg, (const float*)c->cbuf[0],
-      (const float*)c->cbuf[1], (const float*)c->cbuf[2], k2, k1n, dn, out, user);
+      (const float*)c->cbuf[1], (const float*)c->cbuf[2], k2, k1n, dn, out, user, fdev);
+}
+
+// The K-step pipeline; the factors come from the host arrays k1, d, k2, or
+// (when c->dfac is set) from the device table, the host arrays then NULL.
+static int nbody_steps_run(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, float* acc,
+                           long long n, float val, float coef, int nsteps, const float* k1,
+                           const float* d, const float* k2) {
+  int rc;
+  // host factors (zero placeholders when the kernels read the device table)
+  auto K1 = [&](int s) { return k1 ? k1[s] : 0.f; };
+  auto D = [&](int s) { return d ? d[s] : 0.f; };
+  auto K2 = [&](int s) { return k2 ? k2[s] : 0.f; };
+  // every allocation before the first kernel writes disp and vel, so that
+  // PMX_ENOMEM leaves the caller's arrays as they were
+  if ((rc = pmx_ensure_ptcl(c, n))) return rc;
+  if ((rc = pmx_ensure_state(c))) return rc;
+  if ((rc = pmx_ensure_scan(c, c->ntiles + 1))) return rc;
+  PMX_REQUIRE(c->rec_cap < (1LL << 32) - 1, "pmx_nbody_steps: too many particles");
+  const MeshGeom g = pmx_geom_padded(c);
+  const long long nt = c->ntiles;
+  int A = 0;
+
+  // first kick and drift on the caller's arrays, exact binning, fill copy A
+  if ((rc = pmx_fill_u32(c, c->pl_count[A], nt + 1, 0u))) return rc;
+  const BinOut bo = {c->pkey, c->prank, nullptr, nullptr, nullptr, c->flags};
+  if ((rc = pmx_kick_drift_bin(c, pmid, disp, vel, acc, n, K1(0), D(0), c->pl_count[A], bo, 0)))
+    return rc;
+  pmx_tic(c, PMX_T_SCAN);
+  rc = pmx_scan_exclusive(c, c->pl_count[A], c->pl_start[A], nt + 1);
+  pmx_toc(c, PMX_T_SCAN);
+  if (rc) return rc;
+  pmx_tic(c, PMX_T_FILL);
+  pl_fill_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(pmid, disp, vel, n, g, c->pkey, c->prank,
+                                                      c->pl_start[A], c->pl[A]);
+  pmx_toc(c, PMX_T_FILL);
+  PMX_LAUNCH_CHECK();
+
+  const PlUser user = {disp, vel, acc};
+  for (int s = 0; s < nsteps; ++s) {
+    if ((rc = pmx_deposit_state(c, c->pl[A], c->pl_start[A], c->pl_count[A], val,
+                                (float*)c->cbuf[0], c->rs, 1)))
+      return rc;
+    if ((rc = pmx_poisson_ctx(c, coef, 1))) return rc;
+    if (s + 1 == nsteps) {
+      pmx_tic(c, PMX_T_GATHER);
+      pl_launch<PL_LAST>(c, A, PlOut{}, user, K2(s), 0.f, 0.f, s);
+      pmx_toc(c, PMX_T_GATHER);
+      PMX_LAUNCH_CHECK();
+      break;
+    }
+    const int B = 1 - A;
+    pmx_tic(c, PMX_T_SCAN);
+    pl_caps_k<<<pmx_blocks(nt + 1), PMX_TPB, 0, c->stream>>>(c->pl_count[A], c->tile_cap,
+                                                             c->pl_count[B], nt, c->flags);
+    PMX_LAUNCH_CHECK();
+    rc = pmx_scan_exclusive(c, c->tile_cap, c->pl_start[B], nt + 1);
+    pmx_toc(c, PMX_T_SCAN);
+    if (rc) return rc;
+    const PlOut out = {c->pl[B], c->pl_count[B], c->pl_start[B], c->pl_tile, c->pl_rank, c->flags};
+    pmx_tic(c, PMX_T_GATHER);
+    pl_launch<PL_CAP>(c, A, out, user, K2(s), K1(s + 1), D(s + 1), s);
+    pmx_toc(c, PMX_T_GATHER);
+    PMX_LAUNCH_CHECK();
+    // a tile overflowed: redo the step exactly (skipped on the device otherwise)
+    if ((rc = pmx_fill_u32(c, c->pl_count[B], nt + 1, 0u, c->flags))) return rc;
+    pmx_tic(c, PMX_T_BIN);
+    pl_launch<PL_COUNT>(c, A, out, user, K2(s), K1(s + 1), D(s + 1), s);
+    PMX_LAUNCH_CHECK();
+    if ((rc = pmx_scan_exclusive(c, c->pl_count[B], c->pl_start[B], nt + 1, c->flags))) return rc;
+    pl_launch<PL_WRITE>(c, A, out, user, K2(s), K1(s + 1), D(s + 1), s);
+    pmx_toc(c, PMX_T_BIN);
+    PMX_LAUNCH_CHECK();
+    A = B;
+  }
+  return PMX_OK;
 }
 
 extern "C" int pmx_nbody_steps(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, float* acc,
@@ -276,68 +362,139 @@ extern "C" int pmx_nbody_steps(pmx_ctx* c, const int16_t* pmid, float* disp, flo
         return rc;
     return PMX_OK;
   }
-  // every allocation before the first kernel writes disp and vel, so that
-  // PMX_ENOMEM leaves the caller's arrays as they were
-  if ((rc = pmx_ensure_ptcl(c, n))) return rc;
-  if ((rc = pmx_ensure_state(c))) return rc;
-  if ((rc = pmx_ensure_scan(c, c->ntiles + 1))) return rc;
-  PMX_REQUIRE(c->rec_cap < (1LL << 32) - 1, "pmx_nbody_steps: too many particles");
-  const MeshGeom g = pmx_geom_padded(c);
-  const long long nt = c->ntiles;
-  int A = 0;
+  return nbody_steps_run(c, pmid, disp, vel, acc, n, val, coef, nsteps, k1, d, k2);
+}
 
-  // first kick and drift on the caller's arrays, exact binning, fill copy A
-  if ((rc = pmx_fill_u32(c, c->pl_count[A], nt + 1, 0u))) return rc;
-  const BinOut bo = {c->pkey, c->prank, nullptr, nullptr, nullptr, c->flags};
-  if ((rc = pmx_kick_drift_bin(c, pmid, disp, vel, acc, n, k1[0], d[0], c->pl_count[A], bo, 0)))
+// Sets c->dfac for the duration of a call.
+struct DfacScope {
+  pmx_ctx* c;
+  DfacScope(pmx_ctx* c_, const float* f) : c(c_) { c->dfac = f; }
+  ~DfacScope() { c->dfac = nullptr; }
+};
+
+static int steps_dev_check(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, float* acc,
+                           long long n, int nsteps, const float* fac, const char* who) {
+  PMX_REQUIRE(c && n > 0 && nsteps > 0 && fac, "%s: bad argument", who);
+  PMX_REQUIRE(pmid && disp && vel && acc, "%s: NULL pointer", who);
+  PMX_REQUIRE(n < (1LL << 32) - 2048, "%s: too many particles", who);
+  PMX_REQUIRE(c->n[0] <= PMX_PL_MAXN && c->n[1] <= PMX_PL_MAXN && c->n[2] <= PMX_PL_MAXN,
+              "%s: mesh axes above %d cells run pmx_nbody_steps", who, PMX_PL_MAXN);
+  return PMX_OK;
+}
+
+extern "C" int pmx_nbody_steps_dev(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel,
+                                   float* acc, long long n, float val, int nsteps,
+                                   const float* fac) {
+  PMX_ENTRY(c);
+  int rc;
+  if ((rc = steps_dev_check(c, pmid, disp, vel, acc, n, nsteps, fac, "pmx_nbody_steps_dev")))
     return rc;
-  pmx_tic(c, PMX_T_SCAN);
-  rc = pmx_scan_exclusive(c, c->pl_count[A], c->pl_start[A], nt + 1);
-  pmx_toc(c, PMX_T_SCAN);
-  if (rc) return rc;
-  pmx_tic(c, PMX_T_FILL);
-  pl_fill_k<<<pmx_blocks(n), PMX_TPB, 0, c->stream>>>(pmid, disp, vel, n, g, c->pkey, c->prank,
-                                                      c->pl_start[A], c->pl[A]);
-  pmx_toc(c, PMX_T_FILL);
-  PMX_LAUNCH_CHECK();
+  DfacScope scope(c, fac);
+  return nbody_steps_run(c, pmid, disp, vel, acc, n, val, 0.f, nsteps, nullptr, nullptr, nullptr);
+}
 
-  const PlUser user = {disp, vel, acc};
-  for (int s = 0; s < nsteps; ++s) {
-    if ((rc = pmx_deposit_state(c, c->pl[A], c->pl_start[A], c->pl_count[A], val,
-                                (float*)c->cbuf[0], c->rs, 1)))
-      return rc;
-    if ((rc = pmx_poisson_ctx(c, coef, 1))) return rc;
-    if (s + 1 == nsteps) {
-      pmx_tic(c, PMX_T_GATHER);
-      pl_launch<PL_LAST>(c, A, PlOut{}, user, k2[s], 0.f, 0.f);
-      pmx_toc(c, PMX_T_GATHER);
-      PMX_LAUNCH_CHECK();
-      break;
-    }
-    const int B = 1 - A;
-    pmx_tic(c, PMX_T_SCAN);
-    pl_caps_k<<<pmx_blocks(nt + 1), PMX_TPB, 0, c->stream>>>(c->pl_count[A], c->tile_cap,
-                                                             c->pl_count[B], nt, c->flags);
-    PMX_LAUNCH_CHECK();
-    rc = pmx_scan_exclusive(c, c->tile_cap, c->pl_start[B], nt + 1);
-    pmx_toc(c, PMX_T_SCAN);
-    if (rc) return rc;
-    const PlOut out = {c->pl[B], c->pl_count[B], c->pl_start[B], c->pl_tile, c->pl_rank, c->flags};
-    pmx_tic(c, PMX_T_GATHER);
-    pl_launch<PL_CAP>(c, A, out, user, k2[s], k1[s + 1], d[s + 1]);
-    pmx_toc(c, PMX_T_GATHER);
-    PMX_LAUNCH_CHECK();
-    // a tile overflowed: redo the step exactly (skipped on the device otherwise)
-    if ((rc = pmx_fill_u32(c, c->pl_count[B], nt + 1, 0u, c->flags))) return rc;
-    pmx_tic(c, PMX_T_BIN);
-    pl_launch<PL_COUNT>(c, A, out, user, k2[s], k1[s + 1], d[s + 1]);
-    PMX_LAUNCH_CHECK();
-    if ((rc = pmx_scan_exclusive(c, c->pl_count[B], c->pl_start[B], nt + 1, c->flags))) return rc;
-    pl_launch<PL_WRITE>(c, A, out, user, k2[s], k1[s + 1], d[s + 1]);
-    pmx_toc(c, PMX_T_BIN);
-    PMX_LAUNCH_CHECK();
-    A = B;
+// The forward model's N-body part with device factors: the initial force
+// (nbody_init), nsteps KDK steps, and the CIC density of the final state.
+static int forward_dev(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, float* acc,
+                       long long n, float val, int nsteps, const float* fac, float* dens) {
+  int rc;
+  DfacScope scope(c, fac);
+  if ((rc = pmx_gravity(c, pmid, disp, n, val, 0.f, acc))) return rc;
+  if ((rc = nbody_steps_run(c, pmid, disp, vel, acc, n, val, 0.f, nsteps, nullptr, nullptr,
+                            nullptr)))
+    return rc;
+  if (dens && (rc = pmx_scatter_mass(c, pmid, disp, n, val, dens, 0))) return rc;
+  return PMX_OK;
+}
+
+extern "C" int pmx_graph_capture(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel,
+                                 float* acc, long long n, float val, int nsteps, const float* fac,
+                                 float* dens) {
+  int rc;
+  if ((rc = steps_dev_check(c, pmid, disp, vel, acc, n, nsteps, fac, "pmx_graph_capture")))
+    return rc;
+  PMX_REQUIRE(!c->timer, "pmx_graph_capture: timing is enabled on the context");
+  PMX_REQUIRE(!pmx_capturing(c), "pmx_graph_capture: the stream is already capturing");
+  PMX_REQUIRE(c->stream != nullptr, "pmx_graph_capture: the context is on the null stream");
+  PMX_CHECK_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  rc = forward_dev(c, pmid, disp, vel, acc, n, val, nsteps, fac, dens);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    if (!rc) pmx_set_error("pmx_graph_capture: %s", hipGetErrorString(e));
+    return rc ? rc : PMX_EHIP;
   }
+  hipGraphExec_t x = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    pmx_set_error("pmx_graph_capture: instantiate: %s", hipGetErrorString(ei));
+    return PMX_EHIP;
+  }
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph = g;
+  c->gexec = x;
+  return PMX_OK;
+}
+
+extern "C" int pmx_graph_launch(pmx_ctx* c) {
+  PMX_ENTRY(c);
+  PMX_REQUIRE(c && c->gexec, "pmx_graph_launch: no captured graph");
+  PMX_CHECK_HIP(hipGraphLaunch(c->gexec, c->stream));
+  return PMX_OK;
+}
+
+// A pack: one executable graph whose independent branches are the captured
+// graphs of several contexts, so one launch runs several simulations.
+struct pmx_pack {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+extern "C" int pmx_pack_create(pmx_ctx* const* cs, int k, pmx_pack** out) {
+  PMX_REQUIRE(cs && k > 0 && out, "pmx_pack_create: bad argument");
+  for (int i = 0; i < k; ++i)
+    PMX_REQUIRE(cs[i] && cs[i]->graph, "pmx_pack_create: context %d has no captured graph", i);
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < i; ++j)
+      PMX_REQUIRE(cs[i] != cs[j], "pmx_pack_create: context %d appears twice", i);
+  pmx_pack* p = (pmx_pack*)calloc(1, sizeof(pmx_pack));
+  if (!p) {
+    pmx_set_error("pmx_pack_create: out of host memory");
+    return PMX_ENOMEM;
+  }
+  hipError_t e = hipGraphCreate(&p->graph, 0);
+  for (int i = 0; i < k && e == hipSuccess; ++i) {
+    hipGraphNode_t node;
+    e = hipGraphAddChildGraphNode(&node, p->graph, nullptr, 0, cs[i]->graph);
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    free(p);
+    (void)hipGetLastError();
+    pmx_set_error("pmx_pack_create: %s", hipGetErrorString(e));
+    return PMX_EHIP;
+  }
+  *out = p;
+  return PMX_OK;
+}
+
+extern "C" int pmx_pack_launch(pmx_pack* p, void* stream) {
+  PMX_REQUIRE(p && p->exec, "pmx_pack_launch: bad pack");
+  PMX_CHECK_HIP(hipGraphLaunch(p->exec, (hipStream_t)stream));
+  return PMX_OK;
+}
+
+extern "C" int pmx_pack_destroy(pmx_pack* p) {
+  if (!p) return PMX_OK;
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  free(p);
   return PMX_OK;
 }
 

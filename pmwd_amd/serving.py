@@ -28,6 +28,7 @@ buffers, so a request's density and gradients do not depend on what runs
 beside it; only the float64 sums behind the Omega_m gradient may change in
 their last bits with the order in which tile partial sums are added.
 """
+import ctypes
 import math
 import threading
 import time
@@ -130,9 +131,11 @@ def _contexts_needed(req):
     return out
 
 
-def run_request(req, device):
+def run_request(req, device, engine=None):
     """Runs one request with the public API on the current stream; returns
-    the SimResult fields (device work finished, results on the host)."""
+    the SimResult fields (device work finished, results on the host).  With
+    ``engine`` (a PackEngine of the request's configuration) a forward-only
+    request's nbody and scatter run in the engine's packed graph launches."""
     from .boltzmann import boltzmann
     from .cosmology import Cosmology
     from .lpt import lpt
@@ -148,6 +151,9 @@ def run_request(req, device):
         with torch.no_grad():
             modes = linear_modes(white, cosmo, conf)
             ptcl, _ = lpt(modes, cosmo, conf)
+            if engine is not None:
+                out['dens'] = engine.run(ptcl.disp, ptcl.vel, nbody_factor_table(cosmo, conf))
+                return out
             ptcl, _ = nbody(ptcl, None, cosmo, conf)
             dens = scatter(ptcl, conf)
         out['dens'] = dens.cpu().numpy()
@@ -166,11 +172,220 @@ def run_request(req, device):
     return out
 
 
+def nbody_factor_table(cosmo, conf):
+    """float32 {1.5 Omega_m, k1_0, d_0, k2_0, k1_1, ...} of nbody's KDK loop
+    (nbody.py:205-223), the device table of pmx_nbody_steps_dev."""
+    from .nbody import _kdk_factors
+    from .gravity import omega_coef
+    a = conf.a_nbody
+    f = [_kdk_factors(a[i], a[i + 1], cosmo, conf) for i in range(len(a) - 1)]
+    return np.concatenate([[omega_coef(cosmo, conf)], np.asarray(f).reshape(-1)]).astype(np.float32)
+
+
+class _Slot:
+    """One simulation slot of a pack engine: a library context on its own
+    stream, fixed particle, force and density buffers, a device factor
+    table, and (after its first run) the captured graph of nbody plus the
+    final scatter on those buffers."""
+
+    def __init__(self, conf, device):
+        from .particles import Particles
+        self.stream = torch.cuda.Stream(device)
+        with torch.cuda.device(device), torch.cuda.stream(self.stream):
+            self.ctx = _lib.Context(conf.mesh_shape, conf.cell_size, device)
+            self.h = self.ctx.bind()
+            self.pmid = Particles.gen_grid(conf, device=device).pmid
+            n = conf.ptcl_num
+            self.disp = torch.empty((n, 3), dtype=torch.float32, device=device)
+            self.vel = torch.empty_like(self.disp)
+            self.acc = torch.empty_like(self.disp)
+            self.dens = torch.empty(tuple(conf.mesh_shape), dtype=torch.float32, device=device)
+            self.nsteps = len(conf.a_nbody) - 1
+            self.fac = torch.empty(1 + 3 * self.nsteps, dtype=torch.float32, device=device)
+            self.fac_host = torch.empty(1 + 3 * self.nsteps, dtype=torch.float32, pin_memory=True)
+        self.n = conf.ptcl_num
+        self.val = float(np.float32(conf.mesh_size / conf.ptcl_num))
+        self.captured = False
+        self.loaded = None       # event: state and factors in place
+        self.done = threading.Event()
+        self.finished = None     # event: the launch that ran this slot has ended
+        self.error = None        # the launch's failure, if it failed
+
+    def nbytes(self):
+        b = sum(t.numel() * t.element_size() for t in (self.pmid, self.disp, self.vel, self.acc,
+                                                       self.dens, self.fac))
+        return b + self.ctx.nbytes
+
+    def run_eager(self):
+        """The captured sequence, uncaptured (sizes the context's scratch)."""
+        L, p = _lib.lib(), _lib.ptr
+        coef = float(self.fac_host[0])
+        _lib.check(L.pmx_gravity(self.h, p(self.pmid), p(self.disp), self.n, self.val, coef,
+                                 p(self.acc)))
+        _lib.check(L.pmx_nbody_steps_dev(self.h, p(self.pmid), p(self.disp), p(self.vel),
+                                         p(self.acc), self.n, self.val, self.nsteps, p(self.fac)))
+        _lib.check(L.pmx_scatter_mass(self.h, p(self.pmid), p(self.disp), self.n, self.val,
+                                      p(self.dens), 0))
+
+    def capture(self):
+        L, p = _lib.lib(), _lib.ptr
+        _lib.check(L.pmx_graph_capture(self.h, p(self.pmid), p(self.disp), p(self.vel),
+                                       p(self.acc), self.n, self.val, self.nsteps, p(self.fac),
+                                       p(self.dens)))
+        self.captured = True
+
+
+class PackEngine:
+    """Forward-only N-body runs of one size class (one Configuration shape)
+    packed into graph launches.
+
+    Each of ``nslots`` slots keeps its context, buffers and captured graph
+    resident.  A request's initial conditions are copied into a free slot
+    with its factor table; the launcher thread runs every slot loaded by
+    then as one pack (``pmx_pack_launch``: the slots' graphs as independent
+    branches of one graph, one launch for all of them), waiting ``window``
+    seconds for slots still loading.  A slot's first run is eager and
+    captures its graph for the next ones.  Results are bitwise those of
+    ``nbody`` followed by ``scatter``.
+    """
+
+    def __init__(self, conf, device, nslots=4, window=0.002):
+        self.conf = conf
+        self.device = torch.device(device)
+        self.slots = [_Slot(conf, self.device) for _ in range(nslots)]
+        self.window = float(window)
+        self._cv = threading.Condition()
+        self._free = list(range(nslots))
+        self._ready = []
+        self._loading = 0
+        self._packs = {}
+        self._closing = False
+        self.stream = torch.cuda.Stream(self.device)
+        self.launches = 0
+        self.packed = 0
+        self.eager_runs = 0
+        self._thread = threading.Thread(target=self._loop, daemon=True)
+        self._thread.start()
+
+    def nbytes(self):
+        return sum(s.nbytes() for s in self.slots)
+
+    def run(self, disp, vel, fac, src_stream=None):
+        """Runs nbody + scatter from (disp, vel) with the factor table fac
+        (nbody_factor_table); returns the density on the host (numpy)."""
+        src_stream = src_stream or torch.cuda.current_stream(self.device)
+        with self._cv:
+            while not self._free:
+                self._cv.wait()
+            i = self._free.pop()
+            self._loading += 1
+        s = self.slots[i]
+        try:
+            s.stream.wait_stream(src_stream)
+            with torch.cuda.device(self.device), torch.cuda.stream(s.stream):
+                s.disp.copy_(disp)
+                s.vel.copy_(vel)
+                s.fac_host.copy_(torch.from_numpy(np.ascontiguousarray(fac, dtype=np.float32)))
+                s.fac.copy_(s.fac_host, non_blocking=True)
+                eager = not s.captured
+                if eager:
+                    s.run_eager()
+                    s.stream.synchronize()
+                    dens = s.dens.cpu().numpy()
+                    s.capture()
+                else:
+                    s.loaded = torch.cuda.Event()
+                    s.loaded.record(s.stream)
+            if eager:
+                with self._cv:
+                    self.eager_runs += 1
+                    self._loading -= 1
+                    self._free.append(i)
+                    self._cv.notify_all()
+                return dens
+            s.done.clear()
+            with self._cv:
+                self._loading -= 1
+                self._ready.append(i)
+                self._cv.notify_all()
+            s.done.wait()
+            if s.error is not None:
+                raise s.error
+            s.finished.synchronize()
+            with torch.cuda.device(self.device), torch.cuda.stream(s.stream):
+                dens = s.dens.cpu().numpy()
+        except BaseException:
+            with self._cv:
+                if i in self._ready:
+                    self._ready.remove(i)
+            raise
+        with self._cv:
+            self._free.append(i)
+            self._cv.notify_all()
+        return dens
+
+    def _pack(self, idx):
+        key = tuple(idx)
+        p = self._packs.get(key)
+        if p is None:
+            arr = (ctypes.c_void_p * len(idx))(*[self.slots[i].h.value for i in idx])
+            p = ctypes.c_void_p()
+            _lib.check(_lib.lib().pmx_pack_create(arr, len(idx), ctypes.byref(p)))
+            self._packs[key] = p
+        return p
+
+    def _loop(self):
+        while True:
+            with self._cv:
+                while not self._ready and not self._closing:
+                    self._cv.wait()
+                if self._closing and not self._ready:
+                    return
+                # gather the slots still loading, for one window
+                if self._loading and len(self._ready) < len(self.slots):
+                    self._cv.wait(timeout=self.window)
+                idx = sorted(self._ready)
+                self._ready.clear()
+            try:
+                for i in idx:
+                    self.stream.wait_event(self.slots[i].loaded)
+                _lib.check(_lib.lib().pmx_pack_launch(self._pack(idx),
+                                                      ctypes.c_void_p(self.stream.cuda_stream)))
+                ev = torch.cuda.Event()
+                ev.record(self.stream)
+                err = None
+            except BaseException as e:  # the waiting requests raise it
+                ev, err = None, e
+            self.launches += 1
+            self.packed += len(idx)
+            for i in idx:
+                self.slots[i].finished, self.slots[i].error = ev, err
+                self.slots[i].done.set()
+
+    def close(self):
+        with self._cv:
+            self._closing = True
+            self._cv.notify_all()
+        self._thread.join()
+        torch.cuda.synchronize(self.device)
+        L = _lib.lib()
+        for p in self._packs.values():
+            L.pmx_pack_destroy(p)
+        self._packs.clear()
+        for s in self.slots:
+            s.ctx.close()
+
+    def stats(self):
+        return {'slots': len(self.slots), 'pack_launches': self.launches,
+                'packed_runs': self.packed, 'eager_runs': self.eager_runs,
+                'mean_pack_size': self.packed / self.launches if self.launches else None}
+
+
 class Server:
     """Admits SimRequests against an HBM budget and runs up to ``max_slots``
     of them concurrently on one GPU (see the module docstring)."""
 
-    def __init__(self, device='cuda:0', hbm_budget=None, max_slots=8):
+    def __init__(self, device='cuda:0', hbm_budget=None, max_slots=8, packs=True):
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device('cuda', torch.cuda.current_device())
@@ -188,6 +403,10 @@ class Server:
         self._ctx_bytes = {}       # id(Context) -> estimated bytes
         self._closing = False
         self.peak_active = 0
+        # forward-only requests: one PackEngine per configuration
+        self.packs = bool(packs)
+        self._engines = {}
+        self._engine_lock = threading.Lock()
         self._admitter = threading.Thread(target=self._admit_loop, daemon=True)
         self._admitter.start()
 
@@ -209,13 +428,47 @@ class Server:
             while self._pending or self._active:
                 self._cv.wait()
         self._admitter.join()
+        for e in self._engines.values():
+            e.close()
+        self._engines.clear()
+
+    def pack_stats(self):
+        return {f'{k[0]}^3/{k[0] * k[1]}^3': e.stats() for k, e in self._engines.items()}
 
     # ---------------------------------------------------------- internals --
+    @staticmethod
+    def _engine_key(req):
+        return (req.ptcl_grid, req.mesh_ratio, req.box, req.a_start, req.a_nbody_maxstep)
+
+    @staticmethod
+    def _engine_slots(req):
+        # several slots for small simulations, one from 256^3 particles
+        return max(1, min(4, (1 << 22) // req.ptcl_grid**3))
+
+    @classmethod
+    def _engine_bytes(cls, req):
+        conf = req.conf()
+        n, m = conf.ptcl_num, conf.mesh_size
+        return cls._engine_slots(req) * (context_bytes(conf.mesh_shape, n, False) + 42 * n + 4 * m)
+
+    def _engine_for(self, req):
+        key = self._engine_key(req)
+        with self._engine_lock:
+            e = self._engines.get(key)
+            if e is None:
+                e = PackEngine(req.conf(), self.device, self._engine_slots(req))
+                with self._cv:
+                    self._pool_bytes += self._engine_bytes(req)
+                self._engines[key] = e
+        return e
+
     def _fits(self, req):
         need = request_bytes(req)
         for key, nb in _contexts_needed(req).items():
             if not self._pool.get(self._key(key)):
                 need += nb
+        if self.packs and not req.grad and self._engine_key(req) not in self._engines:
+            need += self._engine_bytes(req)
         return self._active_bytes + self._pool_bytes + need <= self.hbm_budget, need
 
     def _key(self, k):
@@ -277,7 +530,8 @@ class Server:
             stream = torch.cuda.Stream(self.device)
             with torch.cuda.device(self.device), torch.cuda.stream(stream), \
                     _lib.context_scope(self._provider(held)):
-                out = run_request(req, self.device)
+                engine = self._engine_for(req) if self.packs and not req.grad else None
+                out = run_request(req, self.device, engine)
                 stream.synchronize()
             res = SimResult(request=req, t_arrive=t_arr, t_admit=t_admit,
                             t_done=time.perf_counter(), **out)

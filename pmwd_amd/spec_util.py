@@ -83,7 +83,12 @@ def powspec(f, spacing, bins=1j / 3, g=None, deconv=None, cut_zero=True, cut_nyq
         gi = gg[i].contiguous() if gg is not None else None
         _lib.check(_lib.lib().pmx_powspec(h, _lib.ptr(fi), _lib.ptr(gi), bnum, P(e), int(right),
                                           int(deconv or 0), P(out)))
-        tot += out
+        # leading axes sum the power (spec_util.py:114-115); the k sums and
+        # mode counts are those of one field
+        if i == 0:
+            tot += out
+        else:
+            tot[bnum:3 * bnum] += out[bnum:3 * bnum]
     kN, Pr, Pi, N = tot.reshape(4, bnum)
     sl = slice(int(cut_zero), bcut)
     kN, Pr, Pi, N = kN[sl], Pr[sl], Pi[sl], N[sl]
