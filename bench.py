@@ -47,7 +47,8 @@ def parse():
                         'reverse-time adjoint), with device memory per phase')
     p.add_argument('--full-adjoint-checkpoints', type=int, default=0, metavar='S',
                    help='with --full-adjoint: snap the reverse run to states stored every S '
-                        'steps in host memory (0: none, pure reversal as pmwd)')
+                        'steps, on the device when they fit, else in pinned host memory '
+                        '(0: none, pure reversal as pmwd)')
     p.add_argument('--serve-requests', type=int, default=40)
     p.add_argument('--serve-rate', type=float, default=2.0, help='Poisson arrivals per second')
     p.add_argument('--serve-slots', type=int, default=8)
@@ -344,8 +345,9 @@ def full_adjoint(G, dev, ckpt_every=0):
     31 KDK steps -> density; loss = mean((rho - target)^2) against a seeded
     target density; gradients with respect to the modes, Omega_m and sigma8
     through the reverse-time adjoint, which reconstructs the trajectory from
-    the final state, snapped to the states stored every ckpt_every steps in
-    host memory (0: pure reversal as pmwd).  The modes -> 2LPT part
+    the final state, snapped to the states stored every ckpt_every steps (on
+    the device when they fit, else in pinned host memory; 0: pure reversal
+    as pmwd).  The modes -> 2LPT part
     is recomputed in the backward rather than kept.  Reports wall times and
     device memory per phase (torch peaks, library contexts, device-wide use)."""
     import time
@@ -381,7 +383,7 @@ def full_adjoint(G, dev, ckpt_every=0):
     vel0.requires_grad_(True)
     ptcl, _ = pa.nbody(pa.Particles(conf, pmid, disp0, vel0), None, cosmo, conf,
                        omega_m=theta['Omega_m'], checkpoint_every=ckpt_every or None,
-                       checkpoint_device='cpu')
+                       checkpoint_device='auto')
     dens = pa.scatter(ptcl, conf)
     del ptcl
     loss = ((dens.double() - target.double())**2).mean()
@@ -419,7 +421,8 @@ def full_adjoint(G, dev, ckpt_every=0):
                         f'{steps} KDK steps from a=1/32, 2LPT, density MSE against a seeded '
                         f'target, gradients w.r.t. modes, Omega_m, sigma8 by reverse-time '
                         f'integration from the final state'
-                        + (f', snapped to states stored every {ckpt_every} steps in host memory'
+                        + (f', snapped to states stored every {ckpt_every} steps (on the '
+                           f'device when they fit, else in pinned host memory)'
                            if ckpt_every else ''),
             't_setup_s': t_setup, 't_forward_s': t_fwd, 't_backward_s': t_bwd,
             't_nbody_backward_s': t_nbody_bwd,
@@ -476,12 +479,28 @@ def main():
     log(f'initial conditions {G}^3 (2LPT)')
     torch.cuda.synchronize()
     t = time.perf_counter()
-    modes = pa.white_noise(0, conf, real=True, device=dev)
-    modes = pa.linear_modes(modes, cosmo, conf)
+    white = pa.white_noise(0, conf, real=True, device=dev)
+    modes = pa.linear_modes(white, cosmo, conf)
     ptcl_ic, _ = pa.lpt(modes, cosmo, conf)
     del modes
     torch.cuda.synchronize()
     t_ic = time.perf_counter() - t
+    # warm lpt(linear_modes(...)), the quantity of pmwd's 2LPT timing
+    # (docs/papers/adjoint/perf.py: min over repeats); and the host tables
+    t_lpt = []
+    for _ in range(5):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        p2, _ = pa.lpt(pa.linear_modes(white, cosmo, conf), cosmo, conf)
+        torch.cuda.synchronize()
+        t_lpt.append(time.perf_counter() - t)
+        del p2
+    t_tab = []
+    for _ in range(3):
+        t = time.perf_counter()
+        pa.boltzmann(cosmo, conf, varlin=False)
+        t_tab.append(time.perf_counter() - t)
+    del white
 
     pmid = ptcl_ic.pmid.contiguous()
     disp = ptcl_ic.disp.clone().contiguous()
@@ -625,6 +644,8 @@ def main():
         'kernels': kernels,
         'nbody_call': nbody_call,
         't_ic_s': t_ic,
+        'lpt_linear_modes_warm_ms': 1e3 * min(t_lpt),
+        'growth_tables_ms': 1e3 * min(t_tab),
     }
 
     if not args.no_fft_yardstick:

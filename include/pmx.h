@@ -109,6 +109,29 @@ int pmx_white_noise(pmx_ctx* ctx, unsigned long long seed, long long n, float* o
  * checks against published known answers).  No pmwd counterpart.          */
 int pmx_random_bits(pmx_ctx* ctx, unsigned long long seed, long long nblocks, uint32_t* out);
 
+/* --------------------------------------------------------------- LPT ---
+ * pmwd/lpt.py:136-212 at lpt_order = order (0, 1 or 2) on a context of the
+ * particle grid (mesh = ptcl_grid_shape, cell = ptcl_spacing).  modes is the
+ * linear-modes spectrum (n0,n1,n2/2+1) complex64 of modes.py:67-117, divided
+ * by cell_vol here (lpt.py:164); fac (host) = {D1, a^2 H D1', D2, a^2 H D2'}
+ * in float32 (lpt.py:196-201).  disp and vel (n,3), n = n0 n1 n2 in C order
+ * (Particles.gen_grid), are added to order by order: += D_m g_m and
+ * a^2 H D_m' g_m with g_m = irfftn(neg_grad(k, pot_m)), pot_1 =
+ * laplace(modes / cell_vol), pot_2 = laplace(rfftn(L)) and L the strain
+ * source of lpt.py:40-76.  Order 2 holds one more mesh in the context.     */
+int pmx_lpt(pmx_ctx* ctx, const float* modes, float cell_vol, int order,
+            const float* fac, float* disp, float* vel);
+/* pmwd/modes.py:67-117 (linear_modes) on an r2c spectrum (n0,n1,n2/2+1)
+ * complex64 in place, on a context of the particle grid: modes *= scale
+ * (float32; the 'ortho' normalisation of the white-noise transform), then
+ * modes *= sqrt(V P(|k|)) in float32 with |k| from the float32 wavevectors
+ * and P in float64 (boltzmann.py:399-455): P = amp (k/k_pivot)^n_s
+ * (q T(k))^2 d2, T linearly interpolated in the host table (tk ascending,
+ * tT; ntab entries, clamped at the ends), par (host) = {amp = 0.32 A_s
+ * k_pivot, k_pivot, n_s, q = pi (c/H_0)^2 / Omega_m, V, d2 = D(a)^2 or 1}. */
+int pmx_linear_modes(pmx_ctx* ctx, float* modes, float scale, const double* tk,
+                     const double* tT, int ntab, const double* par);
+
 /* ----------------------------------------------------------- scatter ---
  * Default-argument CIC mass assignment (pmwd/scatter.py:33-83 with
  * mesh=None, offset=0, cell_size=None): mesh[c] (+)= sum val*w.  Particles

@@ -105,6 +105,8 @@ _SIGS = {
     'pmx_nbody_steps_dev': [c_p, c_p, c_p, c_p, c_p, c_ll, c_f, c_int, c_p],
     'pmx_graph_capture': [c_p, c_p, c_p, c_p, c_p, c_ll, c_f, c_int, c_p, c_p],
     'pmx_graph_launch': [c_p],
+    'pmx_lpt': [c_p, c_p, c_f, c_int, c_p, c_p, c_p],
+    'pmx_linear_modes': [c_p, c_p, c_f, c_p, c_p, c_int, c_p],
     'pmx_pack_create': [c_p, c_int, c_p],
     'pmx_pack_launch': [c_p, c_p],
     'pmx_pack_destroy': [c_p],

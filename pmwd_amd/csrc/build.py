@@ -15,7 +15,7 @@ PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, 'libpmx.so')
 SOURCES = ['pmx_ctx.hip', 'pmx_fft.hip', 'pmx_cic.hip', 'pmx_adjoint.hip', 'pmx_generic.hip',
-           'pmx_sort.hip', 'pmx_pipeline.hip', 'pmx_random.hip', 'pmx_noalloc.hip', 'pmx_cosmo.hip']
+           'pmx_sort.hip', 'pmx_pipeline.hip', 'pmx_random.hip', 'pmx_noalloc.hip', 'pmx_cosmo.hip', 'pmx_lpt.hip']
 
 HIPCC = os.environ.get('HIPCC', shutil.which('hipcc') or '/opt/rocm/bin/hipcc')
 

@@ -307,7 +307,7 @@ extern "C" int pmx_ctx_destroy(pmx_ctx* c) {
   // pl[1], al[1], pl_tile and pl_rank alias the scratch freed above
   void* qs[] = {c->pl[0].s0, c->pl[0].s1, c->pl_count[0], c->pl_start[0], c->pl_count[1],
                 c->pl_start[1], c->al[0].p.s0, c->al[0].p.s1, c->al[0].c0, c->al[0].c1,
-                c->abuf[0], c->abuf[1], c->abuf[2], c->al_dots};
+                c->abuf[0], c->abuf[1], c->abuf[2], c->al_dots, c->lbuf, c->tab};
   for (void* p : qs) rel(p);
   if (c->red_host) hipHostFree(c->red_host);
   if (c->moved) hipEventDestroy(c->moved);
@@ -362,6 +362,7 @@ extern "C" int pmx_ctx_trim(pmx_ctx* c, int what) {
     return PMX_OK;
   }
   free_adj_state(c);
+  dev_free(c, (void**)&c->lbuf, (size_t)c->Mc * sizeof(float2));
   dev_free(c, (void**)&c->rec, c->rec_cap * 16);
   dev_free(c, (void**)&c->pidx, c->rec_cap * 4);
   dev_free(c, (void**)&c->pkey, c->rec_cap * 4);

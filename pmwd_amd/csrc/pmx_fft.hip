@@ -773,7 +773,7 @@ struct KTab {
 // axis 0 of (n0,n1,n2h); blockIdx.y = y index, blockIdx.x = block of B kz
 // columns.  LDS: the n0 x B block, the n0-entry twiddle table, then the k_0
 // and km_0 tables.
-template <bool POW2, int LN>
+template <bool POW2, int LN, bool KIN = false>
 __global__ void __launch_bounds__(PFFT_TPB) __attribute__((amdgpu_waves_per_eu(PFFT_WPE))) poisson_x_k(Fields3 fl, int n0_, int n1, int n2h, int n2p,
                                                        int B_, KTab kt, const float2* __restrict__ tw) {
   extern __shared__ float2 s[];
@@ -810,7 +810,8 @@ __global__ void __launch_bounds__(PFFT_TPB) __attribute__((amdgpu_waves_per_eu(P
   for (int it = 0; it < PFFT_MAXV; ++it)
     if (cm.j0 + it * cm.js < n0) s[threadIdx.x + it * PFFT_TPB] = r[it];
   __syncthreads();
-  fft_run<-1, COLS, POW2, LN, ilog2c(BC), PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
+  // KIN: the input is already transformed along x (a full 3D spectrum)
+  if constexpr (!KIN) fft_run<-1, COLS, POW2, LN, ilog2c(BC), PFFT_MAXV, PFFT_TPB>(s, n0, B, stw);
   const float ky = kt.k[1][y];
   const float kz = live ? kt.k[2][c0 + cm.b] : 0.f;
   float2 h[PFFT_MAXV];
@@ -1182,7 +1183,7 @@ int pmx_fold_deferrable(const pmx_ctx* c) {
 // In-place r2c of a padded real mesh; pre applies (x - 1) * coef.  With
 // halo (one channel's shells; pmx_fold_deferrable must hold) the deposit's
 // fold is added on load.
-int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef, const float* halo = nullptr) {
+int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef, const float* halo) {
   const long long nrows = (long long)c->n[0] * c->n[1];
   const int n2 = c->n[2];
   const RowPlan r = row_plan(n2);
@@ -1304,7 +1305,7 @@ static KTab ktab(const pmx_ctx* c) {
 // f[1] into the two force components in f[1] and f[2] (y_inv_seq_k);
 // dir < 0 turns the cotangents in f[1] and f[2] into km_1 FFT_y(f[1]) +
 // km_2 FFT_y(f[2]) in f[1] (y_fwd_seq_k).
-static int pmx_fft_ypair(pmx_ctx* c, float2* const* f, int dir) {
+int pmx_fft_ypair(pmx_ctx* c, float2* const* f, int dir) {
   Fields3 fl = {{f[0], f[1], f[2]}};
   const int n = c->n[1];
   const int B = fft_pencils(n);
@@ -1334,7 +1335,7 @@ static int pmx_fft_ypair(pmx_ctx* c, float2* const* f, int dir) {
   return PMX_OK;
 }
 
-int pmx_poisson_x(pmx_ctx* c, float2* const* f, int adjoint) {
+int pmx_poisson_x(pmx_ctx* c, float2* const* f, int adjoint, int kin) {
   Fields3 fl = {{f[0], f[1], f[2]}};
   const int n0 = c->n[0];
   const int B = fft_pencils(n0, PFFT_CAP);
@@ -1351,6 +1352,10 @@ int pmx_poisson_x(pmx_ctx* c, float2* const* f, int adjoint) {
     poisson_x_adj_k<true, L><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, kt, c->tw[0]); \
   else if (adjoint)                                                                                       \
     poisson_x_adj_k<false, 0><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, kt, c->tw[0]); \
+  else if ((L || p2) && kin)                                                                              \
+    poisson_x_k<true, L, true><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, kt, c->tw[0]); \
+  else if (kin)                                                                                           \
+    poisson_x_k<false, 0, true><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, kt, c->tw[0]); \
   else if (L || p2)                                                                                       \
     poisson_x_k<true, L><<<grid, PFFT_TPB, lds, c->stream>>>(fl, n0, c->n[1], c->n2h, c->n2p, B, kt, c->tw[0]); \
   else                                                                                                    \
@@ -1372,7 +1377,7 @@ int pmx_poisson_ctx(pmx_ctx* c, float coef, int deferred) {
   const float* halo = deferred && pmx_fold_deferrable(c) ? c->halo : nullptr;
   if ((rc = pmx_fft_r2c_inplace(c, (float*)f[0], 1, coef, halo))) return rc;
   if ((rc = pmx_fft_strided(c, f, 1, 1, -1))) return rc;
-  if ((rc = pmx_poisson_x(c, f, 0))) return rc;
+  if ((rc = pmx_poisson_x(c, f, 0, 0))) return rc;
   if ((rc = pmx_fft_strided(c, f, 1, 1, 1))) return rc;
   if ((rc = pmx_fft_ypair(c, f, 1))) return rc;
   float scale = (float)(1.0 / (double)c->M);
@@ -1400,14 +1405,14 @@ int pmx_poisson_adj_bufs(pmx_ctx* c, float2* const* f, float scale_out, int defe
       return rc;
   if ((rc = pmx_fft_strided(c, f, 1, 1, -1))) return rc;
   if ((rc = pmx_fft_ypair(c, f, -1))) return rc;
-  if ((rc = pmx_poisson_x(c, f, 1))) return rc;
+  if ((rc = pmx_poisson_x(c, f, 1, 0))) return rc;
   if ((rc = pmx_fft_strided(c, f, 1, 1, 1))) return rc;
   if ((rc = pmx_fft_c2r_inplace(c, (float*)f[0], scale_out))) return rc;
   return PMX_OK;
 }
 
 // copies n0*n1 rows of ncols floats between row strides (ncols <= 0: n2)
-int pmx_copy_rows(pmx_ctx* c, float* dst, int dst_rs, const float* src, int src_rs, int ncols = 0) {
+int pmx_copy_rows(pmx_ctx* c, float* dst, int dst_rs, const float* src, int src_rs, int ncols) {
   if (ncols <= 0) ncols = c->n[2];
   long long nrows = (long long)c->n[0] * c->n[1];
   long long total = nrows * ncols;

@@ -225,6 +225,11 @@ struct pmx_ctx {
   // d_1, k2_1, ...} in float32.  The kernels that take a factor read it from
   // here at run time, so a captured graph replays for any cosmology.
   const float* dfac;
+  // the 2LPT source field of pmx_lpt (one more padded mesh, lazily allocated)
+  float2* lbuf;
+  // float64 tables of pmx_linear_modes (transfer k and T)
+  double* tab;
+  size_t tab_cap;
   // the graph of the last pmx_graph_capture and its executable
   hipGraph_t graph;
   hipGraphExec_t gexec;
@@ -944,6 +949,19 @@ int pmx_kd_adj_bin(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, con
 int pmx_deposit_binned3(pmx_ctx* c, long long n, float* const* mesh, int mrs, int defer = 0);
 int pmx_ensure_halo(pmx_ctx* c, int nch);
 int pmx_poisson_ctx(pmx_ctx* c, float coef, int deferred = 0);
+// FFT passes on meshes in the context's padded layout (pmx_fft.hip):
+// in-place z r2c (pre: (x - 1) * coef on load; halo: the deposit's fold
+// added on load), z c2r with a final scale, a strided complex FFT along
+// axis 1 (y) or 0 (x) of nf fields (dir -1 forward, +1 inverse), the paired
+// y pass of the gradient's y and z components, and the Poisson x pass (kin:
+// f[0] holds a full 3D spectrum, no forward x FFT).
+int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef, const float* halo = nullptr);
+int pmx_fft_c2r_inplace(pmx_ctx* c, float* buf, float scale);
+int pmx_fft_strided(pmx_ctx* c, float2* const* f, int nf, int axis, int dir);
+int pmx_fft_ypair(pmx_ctx* c, float2* const* f, int dir);
+int pmx_poisson_x(pmx_ctx* c, float2* const* f, int adjoint, int kin = 0);
+// copies n0*n1 rows of ncols floats between row strides (ncols <= 0: n2)
+int pmx_copy_rows(pmx_ctx* c, float* dst, int dst_rs, const float* src, int src_rs, int ncols = 0);
 int pmx_poisson_adj_ctx(pmx_ctx* c, float scale_out, int deferred = 0);
 int pmx_poisson_adj_bufs(pmx_ctx* c, float2* const* f, float scale_out, int deferred = 0);
 // bins the particles, deposits the density and solves for the three force

@@ -2,9 +2,13 @@
 
 2LPT on the particle grid with the HIP real FFTs: first-order potential,
 the second-order source from six strain fields, then ``-ik`` gradients for
-the displacements and canonical velocities.  k-space multiplies and the
-real-space combinations are device tensor ops around the FFT kernels.
+the displacements and canonical velocities.  Without gradients the whole
+of it is one library call; the differentiable path runs the k-space
+multiplies and real-space combinations as device tensor ops around the FFT
+kernels.
 """
+import ctypes
+
 import numpy as np
 import torch
 
@@ -41,11 +45,48 @@ def _L(kvec, pot, conf):
     return L
 
 
+def _growth_f32(a, cosmo, conf, order):
+    """float32 D_n(a) and a^2 H(a) D_n'(a) (lpt.py:196-201)."""
+    D = float(np.float32(growth(a, cosmo, conf, order=order)))
+    dD = growth(a, cosmo, conf, order=order, deriv=1)
+    return D, float(np.float32(a**2 * np.sqrt(E2(a, cosmo)) * dD))
+
+
+def _needs_graph(modes, theta):
+    if not torch.is_grad_enabled():
+        return False
+    if modes.requires_grad:
+        return True
+    return theta is not None and any(t.requires_grad for t in theta.values())
+
+
+def _lpt_fused(modes, cosmo, conf):
+    """The whole of lpt() in the library (pmx_lpt): first-order gradient,
+    the six strains and their source, the second-order gradient, each added
+    to the particles in one pass."""
+    from . import _lib
+    dev = modes.device
+    ptcl = Particles.gen_grid(conf, vel=True, device=dev)
+    disp, vel = ptcl.disp.clone(), ptcl.vel.clone()
+    a = conf.a_start
+    fac = np.zeros(4, dtype=np.float32)
+    for order in range(1, 1 + conf.lpt_order):
+        fac[2 * order - 2:2 * order] = _growth_f32(a, cosmo, conf, order)
+    m = modes.detach().to(torch.complex64).contiguous()
+    c = _lib.context(conf.ptcl_grid_shape, conf.ptcl_spacing, dev)
+    _lib.check(_lib.lib().pmx_lpt(c.bind(), _lib.ptr(m), float(conf.ptcl_cell_vol),
+                                  int(conf.lpt_order), fac.ctypes.data_as(ctypes.c_void_p),
+                                  _lib.ptr(disp), _lib.ptr(vel)))
+    return ptcl.replace(disp=disp, vel=vel), None
+
+
 def lpt(modes, cosmo, conf, theta=None):
     """LPT at ``conf.lpt_order`` (lpt.py:136-212); returns (ptcl, obsvbl).
 
-    ``theta`` ({'Omega_m': t}) makes the growth factors differentiable with
-    respect to Omega_m; see ``cosmo_param``.
+    Without gradients this is one library call (``pmx_lpt``); with them the
+    same steps run as device tensor ops around the library FFTs, which
+    autograd differentiates.  ``theta`` ({'Omega_m': t}) makes the growth
+    factors differentiable with respect to Omega_m; see ``cosmo_param``.
     """
     if conf.dim != 3:
         raise ValueError(f'dim={conf.dim} not supported')
@@ -53,6 +94,9 @@ def lpt(modes, cosmo, conf, theta=None):
         raise ValueError(f'lpt_order={conf.lpt_order} not supported')
     if conf.lpt_order > 2:
         raise NotImplementedError('lpt_order=3 is not implemented (as in pmwd)')
+    theta = check_theta(theta, cosmo)
+    if not _needs_graph(modes, theta):
+        return _lpt_fused(modes, cosmo, conf)
     dev = modes.device
     modes = modes / conf.ptcl_cell_vol
     kvec = fftfreq(conf.ptcl_grid_shape, conf.ptcl_spacing, dtype=conf.float_dtype, device=dev)
@@ -64,11 +108,8 @@ def lpt(modes, cosmo, conf, theta=None):
     ptcl = Particles.gen_grid(conf, vel=True, device=dev)
     disp, vel = ptcl.disp.clone(), ptcl.vel.clone()
     a = conf.a_start
-    theta = check_theta(theta, cosmo)
     for order in range(1, 1 + conf.lpt_order):
-        D = float(np.float32(growth(a, cosmo, conf, order=order)))
-        dD = growth(a, cosmo, conf, order=order, deriv=1)
-        a2HDp = float(np.float32(a**2 * np.sqrt(E2(a, cosmo)) * dD))
+        D, a2HDp = _growth_f32(a, cosmo, conf, order)
         if theta is not None and 'Omega_m' in theta:
             _, D_om, _, v_om = lpt_growth_derivs(a, cosmo, conf, order)
             f32 = lambda x: torch.tensor(x, dtype=conf.float_dtype, device=dev)

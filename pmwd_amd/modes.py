@@ -7,6 +7,7 @@ use the HIP r2c with 'ortho' norm.
 ``linear_modes`` multiplies by sqrt(V Plin(|k|)) with |k| formed in float32
 from the float32 wavevectors and Plin evaluated in float64, as pmwd.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -62,6 +63,40 @@ def linear_power_k(k, a, cosmo, conf):
     return P
 
 
+def _needs_graph(modes, theta):
+    if not torch.is_grad_enabled():
+        return False
+    if modes.requires_grad:
+        return True
+    return theta is not None and any(t.requires_grad for t in theta.values())
+
+
+def _linear_modes_fused(modes, cosmo, conf, a, real):
+    """linear_modes without gradients: the r2c of real white noise, then one
+    library pass (pmx_linear_modes) for the 'ortho' scale and sqrt(V P)."""
+    from . import _lib
+    dev = modes.device
+    c = _lib.context(conf.ptcl_grid_shape, conf.ptcl_spacing, dev)
+    if torch.is_complex(modes):
+        out = modes.detach().to(torch.complex64).clone().contiguous()
+        scale = 1.0
+    else:
+        out = fftfwd(modes.detach().to(conf.float_dtype))
+        scale = 1 / math.sqrt(math.prod(conf.ptcl_grid_shape))
+    tk = np.ascontiguousarray(conf.transfer_k, dtype=np.float64)
+    tT = np.ascontiguousarray(cosmo.transfer, dtype=np.float64)
+    par = np.array([0.32 * cosmo.A_s * cosmo.k_pivot, cosmo.k_pivot, cosmo.n_s,
+                    math.pi * (conf.c / conf.H_0)**2 / cosmo.Omega_m, conf.box_vol,
+                    float(growth(a, cosmo, conf))**2 if a is not None else 1.0],
+                   dtype=np.float64)
+    P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+    _lib.check(_lib.lib().pmx_linear_modes(c.bind(), _lib.ptr(out), float(scale), P(tk), P(tT),
+                                           len(tk), P(par)))
+    if real:
+        out = fftinv(out, shape=conf.ptcl_grid_shape, norm=conf.ptcl_spacing)
+    return out
+
+
 def linear_modes(modes, cosmo, conf, a=None, real=False, theta=None):
     """sqrt(V Plin(k)) * white-noise modes (modes.py:67-117).
 
@@ -71,6 +106,9 @@ def linear_modes(modes, cosmo, conf, a=None, real=False, theta=None):
     """
     if cosmo.transfer is None:
         raise ValueError('Transfer table is empty. Call transfer_integ or boltzmann first.')
+    theta = check_theta(theta, cosmo)
+    if not _needs_graph(modes, theta):
+        return _linear_modes_fused(modes, cosmo, conf, a, real)
     if not torch.is_complex(modes):
         modes = fftfwd(modes.to(conf.float_dtype), norm='ortho')
     kvec = fftfreq(conf.ptcl_grid_shape, conf.ptcl_spacing, dtype=conf.float_dtype,
@@ -81,7 +119,6 @@ def linear_modes(modes, cosmo, conf, a=None, real=False, theta=None):
     k = torch.sqrt(k2).to(torch.float64)
     P = linear_power_k(k, a, cosmo, conf)
     fac = torch.sqrt(P * conf.box_vol).to(conf.float_dtype)
-    theta = check_theta(theta, cosmo)
     if theta is not None:
         d_om, d_s8 = modes_factor_derivs(k, a, cosmo, conf, linear_power_k)
         fac = host_value(fac, theta, d_om, d_s8)

@@ -175,19 +175,56 @@ def _force(c, pmid, disp, acc, n, val, coef):
                                       _lib.ptr(acc)))
 
 
-_PIN_MAX = 4 << 30  # larger host checkpoints are pageable (pinned memory is limited)
+_PIN_CHUNK = 1 << 30  # host checkpoints are pinned in chunks of at most 1 GiB
+
+
+class _HostCopy:
+    """A device tensor's copy in host memory for a reversal checkpoint:
+    pinned chunks of at most 1 GiB filled by asynchronous copies on the
+    current stream (pageable where pinning fails); ``to(device)`` brings it
+    back."""
+
+    def __init__(self, t):
+        flat = t.detach().reshape(-1)
+        self.shape, self.dtype = tuple(t.shape), t.dtype
+        per = max(1, _PIN_CHUNK // t.element_size())
+        self.chunks = []
+        for i in range(0, flat.numel(), per):
+            src = flat[i:i + per]
+            try:
+                h = torch.empty(src.numel(), dtype=t.dtype, pin_memory=True)
+                h.copy_(src, non_blocking=True)
+            except RuntimeError:
+                h = src.cpu()
+            self.chunks.append(h)
+
+    def to(self, device):
+        out = torch.empty(self.shape, dtype=self.dtype, device=device)
+        flat = out.reshape(-1)
+        o = 0
+        for h in self.chunks:
+            flat[o:o + h.numel()].copy_(h, non_blocking=h.is_pinned())
+            o += h.numel()
+        return out
+
+
+def _checkpoint_where(where, t, count):
+    """'auto': the device when `count` more copies of t (disp and vel each)
+    leave a quarter of the device's free memory, else host memory."""
+    if where != 'auto':
+        return where
+    free, _ = torch.cuda.mem_get_info(t.device)
+    free += torch.cuda.memory_reserved(t.device) - torch.cuda.memory_allocated(t.device)
+    need = 2 * count * t.numel() * t.element_size()
+    return t.device if need < 0.75 * free else 'cpu'
 
 
 def _store(t, where):
     """A copy of t for a reversal checkpoint: on the device, or in host
-    memory when ``where`` is 'cpu' (pinned up to 4 GiB per array, pageable
-    beyond; copied back when the adjoint needs it)."""
+    memory when ``where`` is 'cpu'."""
     if where is None or torch.device(where) == t.device:
         return t.clone()
-    pin = t.numel() * t.element_size() <= _PIN_MAX
-    out = torch.empty(t.shape, dtype=t.dtype, pin_memory=pin)
-    out.copy_(t, non_blocking=pin)
-    return out
+    return _HostCopy(t)
 
 
 def _kdk_factors(a_prev, a_next, cosmo, conf):
@@ -218,6 +255,9 @@ def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=N
     _force(c, pmid, disp, acc, n, val, coef)
     ckpt = {}
     steps = list(zip(a_nbody[:-1], a_nbody[1:]))
+    if checkpoint_every:
+        checkpoint_device = _checkpoint_where(checkpoint_device, disp,
+                                              (len(steps) - 1) // checkpoint_every)
     if _is_kdk(conf):
         i = 0
         while i < len(steps):
@@ -367,7 +407,8 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
             if ckpt and step_e in ckpt:
                 _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
                 kp = 0.0
-                vel.copy_(ckpt[step_e][1].to(vel.device))
+                vel.copy_(ckpt.pop(step_e)[1].to(vel.device))
+            del snap
             s = e + 1
     if kdk and pipeline:
         if kp != 0.0:
@@ -402,7 +443,8 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
             if ckpt and step in ckpt:
                 _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
                 kp = 0.0
-                vel.copy_(ckpt[step][1].to(vel.device))
+                vel.copy_(ckpt.pop(step)[1].to(vel.device))
+            del snap
             continue
         D = K = 0
         a_disp = a_vel = a_acc = a_prev
@@ -496,8 +538,10 @@ def nbody(ptcl, obsvbl, cosmo, conf, reverse=False, checkpoint_every=None, omega
     and, when ``omega_m`` is a 0-d float64 tensor equal to
     ``cosmo.Omega_m`` that requires grad, w.r.t. Omega_m.  With
     ``checkpoint_every`` the adjoint restarts from states stored every that
-    many steps, kept on the device or, with ``checkpoint_device='cpu'``, in
-    host memory.
+    many steps, kept on the device (None), in host memory ('cpu': pinned
+    1 GiB chunks, copied asynchronously) or where they fit ('auto': the
+    device when they take at most three quarters of its free memory).  The
+    adjoint frees each checkpoint once it has used it.
     """
     if cosmo.growth is None:
         raise ValueError('Growth table is empty. Call growth_integ or boltzmann first.')

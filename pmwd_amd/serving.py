@@ -21,7 +21,12 @@ Scheduling:
 * library contexts (FFT twiddles, k tables, mesh buffers, binning scratch)
   stay resident in a pool keyed by mesh shape and are handed to the next
   request of that shape; idle ones are released only when the budget needs
-  their memory.
+  their memory;
+* forward-only requests hand their initial conditions to the PackEngine of
+  their configuration: slots with resident buffers and a captured HIP graph
+  of nbody + scatter whose step factors live in a device table, so one graph
+  serves every cosmology; the slots loaded at the same time run as one pack
+  (one graph launch holding each slot's graph as an independent branch).
 
 Every kernel on the path is deterministic and each request owns its
 buffers, so a request's density and gradients do not depend on what runs

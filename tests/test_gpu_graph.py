@@ -142,8 +142,8 @@ def test_graph_replays_equal_eager(dev, shape, ratio, k):
         sim.eager_dev(fac, val)      # sizes the context's scratch
         sim.reset()
         sim.capture(fac, val)
-    tb = table(0.39, fb, dev)
-    for rep, (src, want) in enumerate([(fac, want_a), (tb, want_b), (fac, want_a),
+    ta, tb = fac.clone(), table(0.39, fb, dev)
+    for rep, (src, want) in enumerate([(ta, want_a), (tb, want_b), (ta, want_a),
                                        (tb, want_b)]):
         # eager device work and host synchronisations between the replays
         junk = torch.randn(1 << 16, device=dev).sum()
