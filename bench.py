@@ -386,7 +386,9 @@ def full_adjoint(G, dev, ckpt_every=0):
                        checkpoint_device='auto')
     dens = pa.scatter(ptcl, conf)
     del ptcl
-    loss = ((dens.double() - target.double())**2).mean()
+    # the difference in float32, its square and mean in float64 (fewer
+    # full-mesh temporaries than casting both densities first)
+    loss = ((dens - target).double()**2).mean()
     del dens
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0

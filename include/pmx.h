@@ -323,7 +323,9 @@ int pmx_drift_adj(pmx_ctx* ctx, long long n, float* disp, const float* vel,
  *   [vel += acc*kp; disp_cot -= acc_cot*kp]  the previous step's closing
  *                                            kick_adj, skipped when kp == 0;
  *   vel += acc*ka; disp_cot -= acc_cot*ka; disp += vel*d; vel_cot -= disp_cot*d;
- *   [disp = disp_snap when non-NULL: a stored forward state];
+ *   [disp = disp_snap when non-NULL: a stored forward state; disp_snap may
+ *    be disp itself, holding that state on entry, so that no second array
+ *    is needed];
  *   acc = gravity(disp); acc_cot = VJP_disp(gravity)[vel_cot].
  * The caller passes this step's kb as the next call's kp, and applies the
  * last one with pmx_kick_adj.  Each kick is rounded separately, so results
@@ -340,7 +342,8 @@ int pmx_nbody_step_adj(pmx_ctx* ctx, const int16_t* pmid, float* disp,
 /* nsteps reverse KDK steps of integrate_adj (pmwd/nbody.py:143-162, the
  * loop of nbody_adj :226-260): bitwise the arrays of the chained calls
  * pmx_nbody_step_adj(kp, ka[0], d[0]), pmx_nbody_step_adj(kb[0], ka[1],
- * d[1]), ..., with disp_snap (may be NULL) given to the last; the last
+ * d[1]), ..., with disp_snap (may be NULL; disp itself only when nsteps
+ * is 1) given to the last; the last
  * closing kick kb[nsteps-1] stays pending for the caller, as there.
  * dots[2s], dots[2s+1] (host, 2*nsteps doubles) are step s's two dots,
  * the same sums in another grouping.  Between steps the particles and

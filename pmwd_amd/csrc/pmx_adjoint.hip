@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(PMX_TPB)
                  float* __restrict__ vel_cot, const float* __restrict__ acc_cot,
                  const float* __restrict__ snap, long long n, float kp, float ka, float d,
                  MeshGeom g, TileGeom tg, uint32_t* __restrict__ tile_count, BinOut o,
-                 double* __restrict__ part) {
+                 double* __restrict__ part, int keep_x) {
   constexpr int T3 = 3 * PMX_TPB;
   // vel | acc | disp | disp_cot | vel_cot | acc_cot, staged through LDS
   __shared__ __align__(16) float sh[6 * T3];
@@ -335,7 +335,8 @@ __global__ void __launch_bounds__(PMX_TPB)
       v = v + ac * ka;
       dc = dc - acot * ka;
       float x = sh[2 * T3 + e];
-      x = x + v * d;
+      // keep_x: disp already holds the stored state the drift would be replaced by
+      if (!keep_x) x = x + v * d;
       float vc = sh[4 * T3 + e];
       vc = vc - dc * d;
       s += (double)dc * v;
@@ -363,10 +364,13 @@ int pmx_kd_adj_bin(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel, con
                    int cap, double* part) {
   const MeshGeom g = pmx_geom_padded(c);
   const unsigned nb1 = pmx_blocks(n);
+  // a snapshot already copied into disp: the drift's disp update is skipped
+  const int keep_x = disp_snap != nullptr && disp_snap == disp;
+  const float* snap = keep_x ? nullptr : disp_snap;
 #define KD_ADJ_LAUNCH(KP, CAPV, KPV)                                                          \
   kd_adj_bin_k<KP, CAPV><<<nb1, PMX_TPB, 0, c->stream>>>(pmid, disp, vel, acc, disp_cot,       \
-                                                         vel_cot, acc_cot, disp_snap, n, KPV, ka, \
-                                                         d, g, c->tg, count, o, part)
+                                                         vel_cot, acc_cot, snap, n, KPV, ka, \
+                                                         d, g, c->tg, count, o, part, keep_x)
   pmx_tic(c, PMX_T_KDADJ);
   if (kp != 0.f) {
     if (cap) KD_ADJ_LAUNCH(1, true, kp);

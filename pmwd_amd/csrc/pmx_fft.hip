@@ -19,8 +19,8 @@
 #define RFFT_CAP 4096                // row passes: fewer values per thread
 #define RFFT_MAXV (RFFT_CAP / FFT_TPB)
 // Poisson x pass: 8 columns of 512 per 512-thread workgroup, three such
-// workgroups (40 KB of LDS each) per CU; 16 columns per 1024 threads and 8
-// per 256 measured slower
+// workgroups (40 KB of LDS each) per CU; 16 columns per 1024 or per 512
+// threads and 8 per 256 measured slower, two waves per SIMD no faster
 #ifndef PFFT_TPB
 #define PFFT_TPB 512
 #endif

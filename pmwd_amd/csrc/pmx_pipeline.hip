@@ -804,6 +804,8 @@ extern "C" int pmx_nbody_steps_adj(pmx_ctx* c, const int16_t* pmid, float* disp,
               "pmx_nbody_steps_adj: NULL pointer");
   PMX_REQUIRE(nsteps == 0 || (ka && d && kb), "pmx_nbody_steps_adj: NULL factor array");
   PMX_REQUIRE(n < (1LL << 32) - 2048, "pmx_nbody_steps_adj: too many particles");
+  PMX_REQUIRE(!disp_snap || disp_snap != disp || nsteps == 1,
+              "pmx_nbody_steps_adj: disp_snap may alias disp only for a single step");
   if (nsteps == 0) return PMX_OK;
   int rc;
   // the dot products are read back on the host

@@ -66,8 +66,9 @@ def _lpt_fused(modes, cosmo, conf):
     to the particles in one pass."""
     from . import _lib
     dev = modes.device
+    # gen_grid's arrays are fresh: the library adds to them in place
     ptcl = Particles.gen_grid(conf, vel=True, device=dev)
-    disp, vel = ptcl.disp.clone(), ptcl.vel.clone()
+    disp, vel = ptcl.disp, ptcl.vel
     a = conf.a_start
     fac = np.zeros(4, dtype=np.float32)
     for order in range(1, 1 + conf.lpt_order):

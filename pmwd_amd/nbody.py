@@ -198,25 +198,37 @@ class _HostCopy:
                 h = src.cpu()
             self.chunks.append(h)
 
-    def to(self, device):
-        out = torch.empty(self.shape, dtype=self.dtype, device=device)
-        flat = out.reshape(-1)
+    def copy_to(self, dst):
+        """Copies the stored values into the device tensor dst (no device
+        allocation)."""
+        flat = dst.view(-1)
         o = 0
         for h in self.chunks:
             flat[o:o + h.numel()].copy_(h, non_blocking=h.is_pinned())
             o += h.numel()
-        return out
+        return dst
+
+    def to(self, device):
+        return self.copy_to(torch.empty(self.shape, dtype=self.dtype, device=device))
+
+
+def _restore(dst, saved):
+    """dst <- a stored checkpoint array (device tensor or _HostCopy)."""
+    if isinstance(saved, _HostCopy):
+        saved.copy_to(dst)
+    else:
+        dst.copy_(saved)
 
 
 def _checkpoint_where(where, t, count):
-    """'auto': the device when `count` more copies of t (disp and vel each)
-    leave a quarter of the device's free memory, else host memory."""
+    """'auto': the device when `count` copies of t (disp and vel each) take
+    at most half the device's free memory, else host memory."""
     if where != 'auto':
         return where
     free, _ = torch.cuda.mem_get_info(t.device)
     free += torch.cuda.memory_reserved(t.device) - torch.cuda.memory_allocated(t.device)
     need = 2 * count * t.numel() * t.element_size()
-    return t.device if need < 0.75 * free else 'cpu'
+    return t.device if need <= 0.5 * free else 'cpu'
 
 
 def _store(t, where):
@@ -255,14 +267,16 @@ def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=N
     _force(c, pmid, disp, acc, n, val, coef)
     ckpt = {}
     steps = list(zip(a_nbody[:-1], a_nbody[1:]))
-    if checkpoint_every:
-        checkpoint_device = _checkpoint_where(checkpoint_device, disp,
-                                              (len(steps) - 1) // checkpoint_every)
+    # 'auto' decides at the first checkpoint, once the pipeline holds its scratch
+    ckpt_where = checkpoint_device if checkpoint_device != 'auto' else None
+    n_ckpt = (len(steps) - 1) // checkpoint_every if checkpoint_every else 0
     if _is_kdk(conf):
         i = 0
         while i < len(steps):
             if checkpoint_every and i % checkpoint_every == 0 and i > 0:
-                ckpt[i] = (_store(disp, checkpoint_device), _store(vel, checkpoint_device))
+                if checkpoint_device == 'auto' and not ckpt:
+                    ckpt_where = _checkpoint_where('auto', disp, n_ckpt)
+                ckpt[i] = (_store(disp, ckpt_where), _store(vel, ckpt_where))
             j = min(len(steps), (i // checkpoint_every + 1) * checkpoint_every
                     if checkpoint_every else len(steps))
             if not pipeline:
@@ -283,7 +297,9 @@ def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=N
         return acc, ckpt
     for i, (a_prev, a_next) in enumerate(steps):
         if checkpoint_every and i % checkpoint_every == 0 and i > 0:
-            ckpt[i] = (_store(disp, checkpoint_device), _store(vel, checkpoint_device))
+            if checkpoint_device == 'auto' and not ckpt:
+                ckpt_where = _checkpoint_where('auto', disp, n_ckpt)
+            ckpt[i] = (_store(disp, ckpt_where), _store(vel, ckpt_where))
         for op in step_ops(a_prev, a_next, conf):
             if op[0] == 'drift':
                 f = _f32(drift_factor(*op[1:], cosmo, conf))
@@ -407,7 +423,7 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
             if ckpt and step_e in ckpt:
                 _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
                 kp = 0.0
-                vel.copy_(ckpt.pop(step_e)[1].to(vel.device))
+                _restore(vel, ckpt.pop(step_e)[1])
             del snap
             s = e + 1
     if kdk and pipeline:
@@ -429,7 +445,13 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
             fa = _f32(kick_factor(*ka_args, cosmo, conf))
             fd = _f32(drift_factor(*d_args, cosmo, conf))
             fb = _f32(kick_factor(*kb_args, cosmo, conf))
-            snap = ckpt[step][0].to(disp.device) if ckpt and step in ckpt else None
+            # the step's drift result is replaced by the stored state: copied
+            # into disp first, which the step then reads as its snapshot (each
+            # particle's value is read before its new one is written)
+            snap = None
+            if ckpt and step in ckpt:
+                _restore(disp, ckpt[step][0])
+                snap = disp
             _lib.check(L.pmx_nbody_step_adj(
                 h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel), _lib.ptr(acc),
                 _lib.ptr(disp_cot), _lib.ptr(vel_cot), _lib.ptr(acc_cot), n, val, coef,
@@ -443,7 +465,7 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
             if ckpt and step in ckpt:
                 _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
                 kp = 0.0
-                vel.copy_(ckpt.pop(step)[1].to(vel.device))
+                _restore(vel, ckpt.pop(step)[1])
             del snap
             continue
         D = K = 0

@@ -6,8 +6,10 @@
   reported beside them.
 * 128^3 on a 256^3 mesh, 63 steps: gradients of the density MSE against a
   second seeded run (sigma8 = 0.85) with respect to the white modes, Omega_m
-  and sigma8, against central differences (eps = 1e-3); the N-body adjoint
-  cotangents against the oracle's reverse-time adjoint over three steps.
+  and sigma8, against the float64 oracle's central differences (committed
+  fixture); the GPU's float32 differences at eps = 1e-3 are recorded
+  beside them; the N-body adjoint cotangents against the oracle's
+  reverse-time adjoint over three steps.
 * 512^3 on a 512^3 mesh, box 1000 Mpc/h, a = 1/32, 31 steps: full-size
   properties (mass conservation, bitwise reproducibility, the multi-step
   pipeline equal to per-step calls, reversibility within pmwd's published
@@ -191,29 +193,21 @@ def test_c3_full_size_properties(dev):
     assert rev_d < 1.4e-3 and rev_v < 2.2e-3, (rev_d, rev_v)
 
 
-def fd_band(f, eps_list, tol=0.02):
-    """Central differences of f at several step sizes: (lo, hi) of the
-    values widened by tol of their largest magnitude.  The density is
-    piecewise linear in the particle positions (CIC), so the difference
-    quotient of the MSE carries the kinks the perturbation crosses, a noise
-    that changes with the step; an exact derivative lies within that
-    spread."""
-    vals = [(f(e) - f(-e)) / (2 * e) for e in eps_list]
-    pad = tol * max(abs(v) for v in vals)
-    return min(vals) - pad, max(vals) + pad, vals
+def _c2_fixture():
+    return np.load(os.path.join(os.path.dirname(__file__), 'golden', 'c2_grad.npz'))
 
 
 def test_c2_gradients(dev):
     """128^3 particles on a 256^3 mesh, box 256 Mpc/h, 63 steps; loss
     mean((rho - rho_tgt)^2) with the target from seed 1 at sigma8 = 0.85.
-    The gradients with respect to Omega_m and sigma8 match central
-    differences (eps = 1e-3) within 2%.  Along three random directions of
-    the white modes the gradient lies within the spread of the central
-    differences at eps = 1e-3, 5e-4 and 2.5e-4 (widened by 2%): the CIC
-    kinks make the difference quotient itself move by several per cent with
-    the step (the smooth part of the chain is checked exactly in
-    test_c2_initial_condition_gradients_exact)."""
+    The adjoint gradients with respect to Omega_m, sigma8 and along three
+    random directions of the white modes match the float64 oracle's central
+    differences (tests/golden/c2_grad.npz, made by make_c2_grad.py) within
+    1%.  The GPU's own float32 central differences at eps = 1e-3 are
+    recorded beside them: their rounding noise is of the same order as
+    the differences measured, so they are not the reference."""
     import pmwd_amd as pa
+    z = _c2_fixture()
     conf, _ = _confs(2.0, 128, 2, 1 / 64)
     with torch.no_grad():
         tcos = _gpu_cosmo(conf, sigma8=0.85)
@@ -241,26 +235,31 @@ def test_c2_gradients(dev):
     for _ in range(3):
         u = torch.as_tensor(rng.standard_normal(conf.ptcl_grid_shape).astype(np.float32),
                             device=dev)
-        lo, hi, vals = fd_band(lambda e: loss_at(white0 + e * u), (1e-3, 5e-4, 2.5e-4))
-        dirs.append((float((gw.double() * u.double()).sum()), lo, hi, vals))
-    _record('C2_grad', loss=float(loss), grad_Omega_m=float(gom), fd_Omega_m=fd_om,
-            grad_sigma8=float(gs8), fd_sigma8=fd_s8, modes_dirs=dirs,
-            grad_modes_std=float(gw.std()))
-    assert abs(float(gom) - fd_om) < 0.02 * abs(fd_om)
-    assert abs(float(gs8) - fd_s8) < 0.02 * abs(fd_s8)
-    for an, lo, hi, _ in dirs:
-        assert lo <= an <= hi, dirs
+        dirs.append(float((gw.double() * u.double()).sum()))
+    res = dict(loss=float(loss.detach()), loss_f64=float(z['loss']),
+               grad_Omega_m=float(gom), f64_Omega_m=float(z['loss_dOm']), fd32_Omega_m=fd_om,
+               grad_sigma8=float(gs8), f64_sigma8=float(z['loss_ds8']), fd32_sigma8=fd_s8,
+               modes_dirs=dirs, f64_modes_dirs=[float(x) for x in z['loss_dirs']],
+               grad_modes_std=float(gw.std()))
+    _record('C2_grad', **res)
+    assert abs(res['loss'] - res['loss_f64']) < 1e-3 * res['loss_f64'], res
+    assert abs(float(gom) - z['loss_dOm']) < 1e-2 * abs(z['loss_dOm']), res
+    assert abs(float(gs8) - z['loss_ds8']) < 1e-2 * abs(z['loss_ds8']), res
+    scale = np.abs(z['loss_dirs']).max()
+    for an, want in zip(dirs, z['loss_dirs']):
+        assert abs(an - want) < 1e-2 * scale, res
 
 
 def _ic_objective(conf, dev, seed_ab=11):
     """J(white, theta) = <a, disp> + <b, vel> of the 2LPT initial conditions
-    (linear_modes -> lpt) for fixed random a, b: a polynomial of degree two
-    in the white modes and in sigma8, smooth in Omega_m."""
+    (linear_modes -> lpt) for fixed standard normal a, b (numpy
+    default_rng(seed_ab), shape (N,3)): a polynomial of degree two in the
+    white modes and in sigma8, smooth in Omega_m."""
     import pmwd_amd as pa
-    g = torch.Generator(device=dev).manual_seed(seed_ab)
+    rng = np.random.default_rng(seed_ab)
     n = conf.ptcl_num
-    a = torch.randn((n, 3), generator=g, device=dev, dtype=torch.float64)
-    b = torch.randn((n, 3), generator=g, device=dev, dtype=torch.float64)
+    a = torch.as_tensor(rng.standard_normal((n, 3)), device=dev)
+    b = torch.as_tensor(rng.standard_normal((n, 3)), device=dev)
 
     def J(white, s8=0.8, Om=0.3, grad=False):
         cosmo = _gpu_cosmo(conf, s8, Om)
@@ -277,37 +276,37 @@ def _ic_objective(conf, dev, seed_ab=11):
 
 def test_c2_initial_condition_gradients_exact(dev):
     """The gradient of the 2LPT initial conditions (linear_modes -> lpt, the
-    part of the C2 chain before the N-body adjoint) with respect to the white
-    modes, Omega_m and sigma8, where central differences are exact up to
-    rounding: the initial conditions are quadratic in the modes and in
-    sigma8 and smooth in Omega_m.  Within 1e-4 along three directions of the
-    modes, 1e-4 for sigma8 and 1e-3 for Omega_m."""
+    part of the C2 chain before the N-body adjoint) with respect to Omega_m,
+    sigma8 and along three directions of the white modes, against the
+    float64 oracle's central differences (tests/golden/c2_grad.npz): within
+    1e-4 for Omega_m, 1e-5 for sigma8 and 1e-5 of the largest directional
+    derivative for the modes.  (The GPU's own float32 differences over
+    Omega_m carry about 1e-3 of rounding noise.)"""
     import pmwd_amd as pa
+    z = _c2_fixture()
     conf, _ = _confs(2.0, 128, 2, 1 / 64)
     J = _ic_objective(conf, dev)
     white0 = pa.white_noise(0, conf, real=True, device=dev)
     w = white0.clone().requires_grad_(True)
     j, theta = J(w, grad=True)
     gw, gom, gs8 = torch.autograd.grad(j, (w, theta['Omega_m'], theta['sigma8']))
-    val = lambda **kw: float(J(white0, **kw)[0].detach())
-    res = {}
-    with torch.no_grad():
-        fd_s8 = (val(s8=0.8 + 1e-2) - val(s8=0.8 - 1e-2)) / 2e-2
-        fd_om = (val(Om=0.3 + 1e-3) - val(Om=0.3 - 1e-3)) / 2e-3
-        rng = np.random.default_rng(12)
-        dirs = []
-        for _ in range(3):
-            u = torch.as_tensor(rng.standard_normal(conf.ptcl_grid_shape).astype(np.float32),
-                                device=dev)
-            fd = (float(J(white0 + 1e-2 * u)[0]) - float(J(white0 - 1e-2 * u)[0])) / 2e-2
-            dirs.append((float((gw.double() * u.double()).sum()), fd))
-    res = dict(grad_sigma8=float(gs8), fd_sigma8=fd_s8, grad_Omega_m=float(gom), fd_Omega_m=fd_om,
-               modes_dirs=dirs)
+    rng = np.random.default_rng(12)
+    dirs = []
+    for _ in range(3):
+        u = torch.as_tensor(rng.standard_normal(conf.ptcl_grid_shape).astype(np.float32),
+                            device=dev)
+        dirs.append(float((gw.double() * u.double()).sum()))
+    res = dict(J=float(j.detach()), J_f64=float(z['ic_J']), grad_sigma8=float(gs8),
+               f64_sigma8=float(z['ic_ds8']), grad_Omega_m=float(gom),
+               f64_Omega_m=float(z['ic_dOm']), modes_dirs=dirs,
+               f64_modes_dirs=[float(x) for x in z['ic_dirs']])
     _record('C2_ic_exact', **res)
-    assert abs(float(gs8) - fd_s8) < 1e-4 * abs(fd_s8), res
-    assert abs(float(gom) - fd_om) < 1e-3 * abs(fd_om), res
-    for an, fd in dirs:
-        assert abs(an - fd) < 1e-4 * abs(fd), res
+    assert abs(res['J'] - res['J_f64']) < 1e-5 * abs(res['J_f64']), res
+    assert abs(float(gs8) - z['ic_ds8']) < 1e-5 * abs(z['ic_ds8']), res
+    assert abs(float(gom) - z['ic_dOm']) < 1e-4 * abs(z['ic_dOm']), res
+    scale = np.abs(z['ic_dirs']).max()
+    for an, want in zip(dirs, z['ic_dirs']):
+        assert abs(an - want) < 1e-5 * scale, res
 
 
 def test_c2_adjoint_vs_oracle(dev):

@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 
 
 def rel_l2(a, b):
-    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    dt = np.complex128 if np.iscomplexobj(a) or np.iscomplexobj(b) else np.float64
+    a, b = np.asarray(a, dt), np.asarray(b, dt)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 

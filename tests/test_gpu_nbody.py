@@ -333,6 +333,22 @@ def test_nbody_step_adj_matches_separate_ops(dev, shape, ratio, snap):
     _lib.check(L.pmx_kick_adj(h, n, q('vel'), q('acc'), q('dc'), q('ac'), q('vc'), kp, None))
     for k in ('disp', 'vel', 'acc', 'dc', 'vc', 'ac'):
         np.testing.assert_array_equal(b[k].cpu().numpy(), a[k].cpu().numpy(), err_msg=k)
+    if snap:
+        # the snapshot copied into disp first and disp passed as its own
+        # snapshot (no second array): the same bits
+        e, _ = start()
+        r = lambda k: _lib.ptr(e[k])
+        kp = 0.0
+        for i, (ka, d, kb) in enumerate(steps):
+            if i == 0:
+                e['disp'].copy_(snap_t)
+            _lib.check(L.pmx_nbody_step_adj(h, r('pmid'), r('disp'), r('vel'), r('acc'), r('dc'),
+                                            r('vc'), r('ac'), n, val, coef, kp, ka, d,
+                                            r('disp') if i == 0 else None, None))
+            kp = kb
+        _lib.check(L.pmx_kick_adj(h, n, r('vel'), r('acc'), r('dc'), r('ac'), r('vc'), kp, None))
+        for k in ('disp', 'vel', 'acc', 'dc', 'vc', 'ac'):
+            np.testing.assert_array_equal(e[k].cpu().numpy(), a[k].cpu().numpy(), err_msg=k)
     # every kick adjoint's dot is the dot of the force it uses
     force_dots = [dot0, want_dots[0][1], want_dots[0][1], want_dots[1][1]]
     for kd, fd in zip(kick_dots, force_dots):
