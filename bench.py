@@ -271,6 +271,7 @@ def serving_bench(dev, n, rate, slots):
     t0 = time.perf_counter()
     res = sv.run_load(server, sched, t0)
     t_end = max(r.t_done for r in res)
+    packs = server.pack_stats()
     server.close()
     out = sv.latency_stats(res, t0, t_end)
     by = {}
@@ -280,6 +281,7 @@ def serving_bench(dev, n, rate, slots):
             by[f'{g}^3/{2 * g}^3'] = {'n': len(lat), 'p50_s': float(np.percentile(lat, 50)),
                                       'p99_s': float(np.percentile(lat, 99))}
     out.update(rate_per_s=rate, max_slots=slots, peak_concurrent=server.peak_active,
+               packs=packs,
                hbm_budget_gb=server.hbm_budget / 1e9, by_size=by,
                mix='40% 64^3/128^3, 40% 128^3/256^3, 20% 256^3/512^3; half forward+gradient; '
                    'Omega_m~U[0.25,0.35], sigma8~U[0.7,0.9]; 31 KDK steps; Poisson arrivals')

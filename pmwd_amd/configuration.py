@@ -7,6 +7,7 @@ reciprocal of the transfer wavenumber grid because the FFTLog provider
 (mcfit) is not used.  Floats are Python floats (FP64 on the host).
 """
 from dataclasses import dataclass, replace as _dc_replace
+from functools import cached_property
 import math
 from typing import ClassVar, Optional, Tuple, Union
 
@@ -27,6 +28,13 @@ def _as_torch_dtype(dt):
     if name not in _DT:
         raise ValueError(f'unsupported dtype {dt}')
     return _DT[name]
+
+
+def _frozen(a):
+    """A read-only array: derived tables are computed once per
+    configuration and shared by every caller."""
+    a.flags.writeable = False
+    return a
 
 
 @dataclass(frozen=True)
@@ -193,11 +201,11 @@ class Configuration:
     def transfer_lgk_step(self):
         return (self.transfer_lgk_max - self.transfer_lgk_min) / (self.transfer_k_num - 2)
 
-    @property
+    @cached_property
     def transfer_k(self):
         k = np.logspace(self.transfer_lgk_min, self.transfer_lgk_max,
                         num=self.transfer_k_num - 1, dtype=np.float64)
-        return np.concatenate((np.array([0.0]), k))
+        return _frozen(np.concatenate((np.array([0.0]), k)))
 
     @property
     def a_lpt_num(self):
@@ -215,19 +223,20 @@ class Configuration:
     def a_nbody_step(self):
         return (self.a_stop - self.a_start) / self.a_nbody_num
 
-    @property
+    @cached_property
     def a_lpt(self):
-        return np.linspace(0, self.a_start, num=self.a_lpt_num + 1, dtype=np.float64)
+        return _frozen(np.linspace(0, self.a_start, num=self.a_lpt_num + 1, dtype=np.float64))
 
-    @property
+    @cached_property
     def a_nbody(self):
-        return np.linspace(self.a_start, self.a_stop, num=1 + self.a_nbody_num, dtype=np.float64)
+        return _frozen(np.linspace(self.a_start, self.a_stop, num=1 + self.a_nbody_num,
+                                   dtype=np.float64))
 
-    @property
+    @cached_property
     def growth_a(self):
-        return np.concatenate((self.a_lpt, self.a_nbody[1:]))
+        return _frozen(np.concatenate((self.a_lpt, self.a_nbody[1:])))
 
-    @property
+    @cached_property
     def varlin_R(self):
         """Top-hat radii of the variance table: reciprocal transfer wavenumbers."""
-        return 1.0 / self.transfer_k[1:][::-1]
+        return _frozen(1.0 / self.transfer_k[1:][::-1])

@@ -198,6 +198,10 @@ class _HostCopy:
                 h = src.cpu()
             self.chunks.append(h)
 
+    @property
+    def device(self):
+        return torch.device('cpu')
+
     def copy_to(self, dst):
         """Copies the stored values into the device tensor dst (no device
         allocation)."""
@@ -336,7 +340,7 @@ def _adj_pipeline_fits(n, c, device, extra=0):
 
 
 def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False, ckpt=None,
-                 fused=True, pipeline=None, cot_out=None, pctx=None):
+                 fused=True, pipeline=None, cot_out=None, pctx=None, free_ckpt=False):
     """Reverse-time adjoint in place (nbody.py:143-162, 226-260).
 
     On return disp/vel hold the reconstructed initial state and
@@ -347,7 +351,9 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
     device has room, falling back to the per-step path when the pipeline's
     allocation fails (the library reports that before it touches any array)
     -- else as one ``pmx_nbody_step_adj`` call per step; both give the same
-    arrays.  With ``fused`` False every splitting runs op by op.
+    arrays.  With ``fused`` False every splitting runs op by op.  With
+    ``free_ckpt`` each checkpoint leaves ``ckpt`` once used, so that its
+    memory returns during the reverse run.
     """
     L = _lib.lib()
     n = pmid.shape[0]
@@ -423,7 +429,7 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
             if ckpt and step_e in ckpt:
                 _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
                 kp = 0.0
-                _restore(vel, ckpt.pop(step_e)[1])
+                _restore(vel, (ckpt.pop(step_e) if free_ckpt else ckpt[step_e])[1])
             del snap
             s = e + 1
     if kdk and pipeline:
@@ -465,7 +471,7 @@ def _run_adjoint(pmid, disp, vel, disp_cot, vel_cot, cosmo, conf, reverse=False,
             if ckpt and step in ckpt:
                 _kick_adj(L, h, n, vel, acc, disp_cot, acc_cot, vel_cot, kp)
                 kp = 0.0
-                _restore(vel, ckpt.pop(step)[1])
+                _restore(vel, (ckpt.pop(step) if free_ckpt else ckpt[step])[1])
             del snap
             continue
         D = K = 0
@@ -543,8 +549,10 @@ class _NBodyFn(torch.autograd.Function):
         vel = vel_f.clone()
         dc = (disp_cot if disp_cot is not None else torch.zeros_like(disp)).clone().contiguous()
         vc = (vel_cot if vel_cot is not None else torch.zeros_like(vel)).clone().contiguous()
+        # the checkpoints are not needed again (no double backward)
         _, _, om_cot = _run_adjoint(ctx.pmid, disp, vel, dc, vc, ctx.cosmo, conf,
-                                    ctx.reverse, ctx.ckpt, pipeline=pipeline, pctx=ctx.pmx)
+                                    ctx.reverse, ctx.ckpt, pipeline=pipeline, pctx=ctx.pmx,
+                                    free_ckpt=True)
         om_grad = None
         if ctx.needs_input_grad[2]:
             om_grad = torch.tensor(om_cot, dtype=torch.float64)

@@ -148,8 +148,10 @@ def run_request(req, device, engine=None):
     from .nbody import nbody
     from .scatter import scatter
     conf = req.conf()
+    # the variance table is not on the model's path (from_sigma8 normalises
+    # with its own two nodes)
     cosmo = boltzmann(Cosmology.from_sigma8(conf, req.sigma8, n_s=req.n_s, Omega_m=req.Omega_m,
-                                            Omega_b=req.Omega_b, h=req.h), conf)
+                                            Omega_b=req.Omega_b, h=req.h), conf, varlin=False)
     white = white_noise(req.seed, conf, real=True, device=device)
     out = {'n_steps': len(conf.a_nbody) - 1}
     if not req.grad:
@@ -390,7 +392,8 @@ class Server:
     """Admits SimRequests against an HBM budget and runs up to ``max_slots``
     of them concurrently on one GPU (see the module docstring)."""
 
-    def __init__(self, device='cuda:0', hbm_budget=None, max_slots=8, packs=True):
+    def __init__(self, device='cuda:0', hbm_budget=None, max_slots=8, packs=True,
+                 pack_window=0.002):
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device('cuda', torch.cuda.current_device())
@@ -410,6 +413,7 @@ class Server:
         self.peak_active = 0
         # forward-only requests: one PackEngine per configuration
         self.packs = bool(packs)
+        self.pack_window = float(pack_window)
         self._engines = {}
         self._engine_lock = threading.Lock()
         self._admitter = threading.Thread(target=self._admit_loop, daemon=True)
@@ -447,8 +451,8 @@ class Server:
 
     @staticmethod
     def _engine_slots(req):
-        # several slots for small simulations, one from 256^3 particles
-        return max(1, min(4, (1 << 22) // req.ptcl_grid**3))
+        # four slots for small simulations, two from 256^3 particles
+        return max(2, min(4, (1 << 22) // req.ptcl_grid**3))
 
     @classmethod
     def _engine_bytes(cls, req):
@@ -461,7 +465,8 @@ class Server:
         with self._engine_lock:
             e = self._engines.get(key)
             if e is None:
-                e = PackEngine(req.conf(), self.device, self._engine_slots(req))
+                e = PackEngine(req.conf(), self.device, self._engine_slots(req),
+                               window=self.pack_window)
                 with self._cv:
                     self._pool_bytes += self._engine_bytes(req)
                 self._engines[key] = e

@@ -63,7 +63,9 @@ def log(*a):
     print(f'[{time.strftime("%H:%M:%S")}]', *a, flush=True)
 
 
-def main():
+def ic_objective():
+    """(J, white noise): the 2LPT objective <a, disp> + <b, vel> as a
+    function of the white modes, Omega_m and sigma8."""
     n = CONF.ptcl_num
     w0 = oic.white_noise(0, CONF, real=True).astype(F64)
     rng = np.random.default_rng(11)
@@ -72,7 +74,11 @@ def main():
     def J(white=w0, Om=0.3, s8=0.8):
         _, d0, v0 = ics(white, cosmo(Om, s8))
         return float((a * d0).sum() + (b * v0).sum())
+    return J, w0
 
+
+def main():
+    J, w0 = ic_objective()
     out = {'ic_J': J()}
     out['ic_dOm'] = central(lambda h: J(Om=0.3 + h), 1e-4)
     out['ic_ds8'] = central(lambda h: J(s8=0.8 + h), 1e-2)

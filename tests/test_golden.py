@@ -58,3 +58,21 @@ def test_host_tables_match_fixture():
     # D_n / a^n, boltzmann.py:207), checked on the fixture itself
     np.testing.assert_allclose(t['eds_growth'][0][0], 1.0, rtol=1e-7)
     np.testing.assert_allclose(t['eds_growth'][1][0], 3 / 7, rtol=1e-6)
+
+
+def test_c2_gradient_fixture_reproduces():
+    """The float64 gradient fixture (make_c2_grad.py): its 2LPT objective
+    and that objective's sigma8 derivative (exact by central differences: the
+    objective is quadratic in sigma8) come out of the oracle again.  The
+    N-body entries take an hour of CPU and are not recomputed here."""
+    spec = importlib.util.spec_from_file_location('make_c2_grad',
+                                                  os.path.join(GOLDEN, 'make_c2_grad.py'))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    z = np.load(os.path.join(GOLDEN, 'c2_grad.npz'))
+    J, _ = mod.ic_objective()
+    assert abs(J() - z['ic_J']) <= 1e-10 * abs(z['ic_J'])
+    ds8 = mod.central(lambda h: J(s8=0.8 + h), 1e-2)
+    assert abs(ds8 - z['ic_ds8']) <= 1e-8 * abs(z['ic_ds8'])
+    assert set(z.files) == {'ic_J', 'ic_dOm', 'ic_ds8', 'ic_dirs', 'loss', 'loss_dOm',
+                           'loss_ds8', 'loss_dirs'}
