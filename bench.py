@@ -419,6 +419,16 @@ def full_adjoint(G, dev, ckpt_every=0):
     t_bwd = time.perf_counter() - t0
     mem['torch_peak_gb_ic_backward'] = torch.cuda.max_memory_allocated(dev) / 1e9
     mem['library_context_gb_ic_backward'] = sum(c.nbytes for c in _lib._ctx_cache.values()) / 1e9
+    mem['device_peak_gb_reported'] = max(mem['device_used_gb_after_forward'],
+                                         mem['device_used_gb_after_adjoint'])
+    # the modes gradient along three seeded directions (comparable across runs)
+    del d1, v1, g_d0, g_v0
+    gen = torch.Generator(device=dev).manual_seed(123)
+    proj = []
+    for _ in range(3):
+        u = torch.randn(g_w.shape, generator=gen, device=dev, dtype=g_w.dtype)
+        proj.append(float((g_w.double() * u.double()).sum()))
+        del u
     steps = len(conf.a_nbody) - 1
     n = conf.ptcl_num
     return {'workload': f'{G}^3 particles on a {G}^3 mesh, box {2000.0 * G / 1024:g} Mpc/h, '
@@ -433,7 +443,8 @@ def full_adjoint(G, dev, ckpt_every=0):
             'fwd_updates_per_s': n * steps / t_fwd,
             'fwd_adj_updates_per_s': n * steps / (t_fwd + t_bwd),
             'memory': mem, 'loss': loss_value,
-            'grad_modes_norm': float(g_w.double().norm()), 'grad_Omega_m': float(g_om),
+            'grad_modes_norm': float(g_w.double().norm()), 'grad_modes_proj': proj,
+            'grad_Omega_m': float(g_om),
             'grad_sigma8': float(g_s8)}
 
 

@@ -290,6 +290,13 @@ int pmx_nbody_steps_dev(pmx_ctx* ctx, const int16_t* pmid, float* disp, float* v
  * one launch runs k simulations side by side.  pmx_pack_create copies the
  * graphs; capturing a context again leaves existing packs unchanged.  No
  * pmwd counterpart: jit dispatch there.                                    */
+/* A non-blocking stream at the default priority on the given device, for
+ * a caller that captures: streams from a host's pool can be shared with
+ * other users, and work they queue during a capture invalidates it.
+ * pmx_stream_destroy waits for the stream, then destroys it.  No pmwd
+ * counterpart.                                                              */
+int pmx_stream_create(int device, void** out);
+int pmx_stream_destroy(void* stream);
 typedef struct pmx_pack pmx_pack;
 int pmx_graph_capture(pmx_ctx* ctx, const int16_t* pmid, float* disp, float* vel,
                       float* acc, long long n, float val, int nsteps, const float* fac,

@@ -110,6 +110,8 @@ _SIGS = {
     'pmx_pack_create': [c_p, c_int, c_p],
     'pmx_pack_launch': [c_p, c_p],
     'pmx_pack_destroy': [c_p],
+    'pmx_stream_create': [c_int, ctypes.POINTER(c_p)],
+    'pmx_stream_destroy': [c_p],
     'pmx_gravity_vjp': [c_p, c_p, c_p, c_ll, c_f, c_f, c_p, c_p, c_p, ctypes.POINTER(c_d)],
     'pmx_kick_adj': [c_p, c_ll, c_p, c_p, c_p, c_p, c_p, c_f, ctypes.POINTER(c_d)],
     'pmx_drift_adj': [c_p, c_ll, c_p, c_p, c_p, c_p, c_f, ctypes.POINTER(c_d)],
@@ -199,6 +201,47 @@ def require_device(t, name='tensor'):
 
 TRIM_PIPELINE, TRIM_ALL = 1, 2   # pmx.h PMX_TRIM_*
 
+# Graph captures and the calls that wait for the whole device or queue on
+# the null stream (context creation and teardown) exclude each other: such a
+# call from another thread while a stream is capturing can invalidate the
+# capture.
+capture_lock = threading.RLock()
+
+
+_free_streams = {}          # device index -> [(handle, ExternalStream)] not in use
+_streams_lock = threading.Lock()
+
+
+class PrivateStream:
+    """A stream of this process's own (pmx_stream_create), wrapped for torch.
+    torch.cuda.Stream() hands out streams from a pool of 32 per device that
+    other threads may be using at the same time; a stream that is captured
+    must not be shared.  Closing hands the stream back for reuse once its
+    queued work has ended; streams are never destroyed, since torch's
+    caching allocator keeps the blocks used on a stream tied to it for the
+    life of the process."""
+
+    def __init__(self, device):
+        self.device = torch.device(device)
+        self.index = (self.device.index if self.device.index is not None
+                      else torch.cuda.current_device())
+        with _streams_lock:
+            pool = _free_streams.setdefault(self.index, [])
+            item = pool.pop() if pool else None
+        if item is None:
+            h = c_p()
+            check(lib().pmx_stream_create(self.index, ctypes.byref(h)))
+            item = (h, torch.cuda.ExternalStream(h.value, device=self.device))
+        self._item = item
+        self.stream = item[1]
+
+    def close(self):
+        if self._item is not None:
+            self._item[1].synchronize()
+            with _streams_lock:
+                _free_streams[self.index].append(self._item)
+            self._item = None
+
 
 class Context:
     """Owner of a pmx_ctx for one (mesh shape, cell size, device)."""
@@ -208,7 +251,7 @@ class Context:
         self.cell_size = float(cell_size)
         self.device = torch.device(device)
         h = c_p()
-        with torch.cuda.device(self.device):
+        with torch.cuda.device(self.device), capture_lock:
             stream = torch.cuda.current_stream(self.device).cuda_stream
             check(lib().pmx_ctx_create(*self.mesh_shape, self.cell_size, c_p(stream),
                                        ctypes.byref(h)))
@@ -236,7 +279,8 @@ class Context:
             # left to the process end: the allocator callbacks a destroy
             # would call may already be gone
             if _exiting is False:
-                lib().pmx_ctx_destroy(self.h)
+                with capture_lock:
+                    lib().pmx_ctx_destroy(self.h)
             self.h = None
 
     def __del__(self):

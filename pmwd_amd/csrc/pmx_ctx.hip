@@ -277,6 +277,35 @@ extern "C" int pmx_ctx_create(int n0, int n1, int n2, double cell_size, void* st
   return PMX_OK;
 }
 
+// A stream of the caller's own, created here at the default priority and
+// non-blocking (no implicit waits on the null stream), so that a capture on
+// it cannot meet another thread's work: a host's stream pool may hand the
+// same stream to several users.
+extern "C" int pmx_stream_create(int device, void** out) {
+  PMX_REQUIRE(out != nullptr, "pmx_stream_create: out is NULL");
+  *out = nullptr;
+  int cur = 0;
+  PMX_CHECK_HIP(hipGetDevice(&cur));
+  PMX_CHECK_HIP(hipSetDevice(device));
+  hipStream_t st = nullptr;
+  const hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  (void)hipSetDevice(cur);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    pmx_set_error("pmx_stream_create: %s", hipGetErrorString(e));
+    return PMX_EHIP;
+  }
+  *out = (void*)st;
+  return PMX_OK;
+}
+
+extern "C" int pmx_stream_destroy(void* stream) {
+  PMX_REQUIRE(stream != nullptr, "pmx_stream_destroy: stream is NULL");
+  PMX_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  PMX_CHECK_HIP(hipStreamDestroy((hipStream_t)stream));
+  return PMX_OK;
+}
+
 void pmx_timer_destroy(pmx_ctx* c);
 
 extern "C" int pmx_ctx_destroy(pmx_ctx* c) {

@@ -53,7 +53,7 @@ def _interp(x, xp, fp):
 def linear_power_k(k, a, cosmo, conf):
     """linear_power (boltzmann.py:399-455) on a float64 device tensor of k."""
     dev = k.device
-    xp = torch.as_tensor(conf.transfer_k, dtype=torch.float64, device=dev)
+    xp = torch.tensor(conf.transfer_k, dtype=torch.float64, device=dev)
     fp = torch.as_tensor(cosmo.transfer, dtype=torch.float64, device=dev)
     T = _interp(k, xp, fp)
     P = (0.32 * cosmo.A_s * cosmo.k_pivot * (k / cosmo.k_pivot)**cosmo.n_s

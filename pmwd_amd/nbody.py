@@ -243,6 +243,19 @@ def _store(t, where):
     return _HostCopy(t)
 
 
+def kdk_factor_table(a_prev, a_next, cosmo, conf):
+    """float32 (k1, d, k2) rows of the KDK steps a_prev[i] -> a_next[i]
+    (nbody.py:205-212), evaluated on the arrays at once: the same float64
+    operations per step as ``_kdk_factors``."""
+    a_prev = np.asarray(a_prev, dtype=np.float64)
+    a_next = np.asarray(a_next, dtype=np.float64)
+    a_half = a_prev * (1 - 0.5) + a_next * 0.5
+    k1 = kick_factor(a_prev, a_prev, a_half, cosmo, conf)
+    d = drift_factor(a_half, a_prev, a_next, cosmo, conf)
+    k2 = kick_factor(a_next, a_half, a_next, cosmo, conf)
+    return np.stack([k1, d, k2], axis=-1).astype(np.float32)
+
+
 def _kdk_factors(a_prev, a_next, cosmo, conf):
     """float32 k1, d, k2 of one KDK step (nbody.py:205-212)."""
     a_half = a_prev * (1 - 0.5) + a_next * 0.5
@@ -285,8 +298,8 @@ def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=N
                     if checkpoint_every else len(steps))
             if not pipeline:
                 j = i + 1
-            f = np.array([_kdk_factors(ap, an, cosmo, conf) for ap, an in steps[i:j]],
-                         dtype=np.float32).reshape(-1, 3)
+            seg = np.array(steps[i:j], dtype=np.float64).reshape(-1, 2)
+            f = kdk_factor_table(seg[:, 0], seg[:, 1], cosmo, conf)
             k1, d, k2 = (np.ascontiguousarray(f[:, m]) for m in range(3))
             if j - i == 1:
                 _lib.check(L.pmx_nbody_step(h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel),
@@ -334,7 +347,8 @@ def _adj_pipeline_fits(n, c, device, extra=0):
         return need < free + cached + c.nbytes
     if need < free + c.nbytes:
         return True
-    torch.cuda.empty_cache()
+    with _lib.capture_lock:   # a device-wide wait
+        torch.cuda.empty_cache()
     free, _ = torch.cuda.mem_get_info(device)
     return need < free + c.nbytes
 
@@ -570,8 +584,8 @@ def nbody(ptcl, obsvbl, cosmo, conf, reverse=False, checkpoint_every=None, omega
     ``checkpoint_every`` the adjoint restarts from states stored every that
     many steps, kept on the device (None), in host memory ('cpu': pinned
     1 GiB chunks, copied asynchronously) or where they fit ('auto': the
-    device when they take at most three quarters of its free memory).  The
-    adjoint frees each checkpoint once it has used it.
+    device when, at the first checkpoint, they take at most half of its free
+    memory).  The backward frees each checkpoint once it has used it.
     """
     if cosmo.growth is None:
         raise ValueError('Growth table is empty. Call growth_integ or boltzmann first.')

@@ -182,23 +182,26 @@ def run_request(req, device, engine=None):
 def nbody_factor_table(cosmo, conf):
     """float32 {1.5 Omega_m, k1_0, d_0, k2_0, k1_1, ...} of nbody's KDK loop
     (nbody.py:205-223), the device table of pmx_nbody_steps_dev."""
-    from .nbody import _kdk_factors
+    from .nbody import kdk_factor_table
     from .gravity import omega_coef
     a = conf.a_nbody
-    f = [_kdk_factors(a[i], a[i + 1], cosmo, conf) for i in range(len(a) - 1)]
-    return np.concatenate([[omega_coef(cosmo, conf)], np.asarray(f).reshape(-1)]).astype(np.float32)
+    f = kdk_factor_table(a[:-1], a[1:], cosmo, conf)
+    return np.concatenate([[omega_coef(cosmo, conf)], f.reshape(-1)]).astype(np.float32)
 
 
 class _Slot:
-    """One simulation slot of a pack engine: a library context on its own
-    stream, fixed particle, force and density buffers, a device factor
-    table, and (after its first run) the captured graph of nbody plus the
-    final scatter on those buffers."""
+    """One simulation slot of a pack engine: a library context on a stream
+    of its own (not one of torch's pooled streams, which other threads may
+    be queueing work on while this one is captured), fixed particle, force
+    and density buffers, a device factor table, and (after its first run)
+    the captured graph of nbody plus the final scatter on those buffers."""
 
     def __init__(self, conf, device):
         from .particles import Particles
-        self.stream = torch.cuda.Stream(device)
-        with torch.cuda.device(device), torch.cuda.stream(self.stream):
+        self._ps = _lib.PrivateStream(device)
+        self.stream = self._ps.stream
+        # pinned host memory and context creation may wait on the device
+        with _lib.capture_lock, torch.cuda.device(device), torch.cuda.stream(self.stream):
             self.ctx = _lib.Context(conf.mesh_shape, conf.cell_size, device)
             self.h = self.ctx.bind()
             self.pmid = Particles.gen_grid(conf, device=device).pmid
@@ -236,10 +239,15 @@ class _Slot:
 
     def capture(self):
         L, p = _lib.lib(), _lib.ptr
-        _lib.check(L.pmx_graph_capture(self.h, p(self.pmid), p(self.disp), p(self.vel),
-                                       p(self.acc), self.n, self.val, self.nsteps, p(self.fac),
-                                       p(self.dens)))
+        with _lib.capture_lock:
+            _lib.check(L.pmx_graph_capture(self.h, p(self.pmid), p(self.disp), p(self.vel),
+                                           p(self.acc), self.n, self.val, self.nsteps,
+                                           p(self.fac), p(self.dens)))
         self.captured = True
+
+    def close(self):
+        self.ctx.close()
+        self._ps.close()
 
 
 class PackEngine:
@@ -267,7 +275,8 @@ class PackEngine:
         self._loading = 0
         self._packs = {}
         self._closing = False
-        self.stream = torch.cuda.Stream(self.device)
+        self._ps = _lib.PrivateStream(self.device)
+        self.stream = self._ps.stream
         self.launches = 0
         self.packed = 0
         self.eager_runs = 0
@@ -380,7 +389,8 @@ class PackEngine:
             L.pmx_pack_destroy(p)
         self._packs.clear()
         for s in self.slots:
-            s.ctx.close()
+            s.close()
+        self._ps.close()
 
     def stats(self):
         return {'slots': len(self.slots), 'pack_launches': self.launches,

@@ -112,3 +112,24 @@ def test_packed_graph_runs_equal_the_eager_path(dev):
     for rnd in got:
         for r, w in zip(rnd, want):
             np.testing.assert_array_equal(r.dens, w)
+
+
+def test_private_streams_are_not_pooled(dev):
+    """Slots capture on streams of their own: distinct from each other and
+    from every stream of torch's pool, handed back for reuse on close; the
+    C pair creates and destroys a stream torch never saw."""
+    import ctypes
+    from pmwd_amd import _lib
+    pooled = {torch.cuda.Stream(dev).cuda_stream for _ in range(40)}
+    ps = [_lib.PrivateStream(dev) for _ in range(3)]
+    hs = [p.stream.cuda_stream for p in ps]
+    assert len(set(hs)) == 3 and not set(hs) & pooled
+    ps[0].close()
+    again = _lib.PrivateStream(dev)
+    assert again.stream.cuda_stream == hs[0]
+    for p in ps[1:] + [again]:
+        p.close()
+    h = ctypes.c_void_p()
+    _lib.check(_lib.lib().pmx_stream_create(torch.device(dev).index or 0, ctypes.byref(h)))
+    assert h.value and h.value not in pooled
+    _lib.check(_lib.lib().pmx_stream_destroy(h))

@@ -207,3 +207,14 @@ def test_serving_load_mix():
     big = next(r for _, r in sched if r.ptcl_grid == 256)
     assert request_bytes(big) > request_bytes(small)
     assert context_bytes((512, 512, 512), 256**3, True) > context_bytes((512, 512, 512), 256**3, False)
+
+
+def test_vectorised_step_factors_equal_per_step():
+    """kdk_factor_table evaluates every step's factors on arrays at once;
+    the float32 results equal the per-step scalar evaluation bit for bit."""
+    conf, c, _, _ = _cosmos()
+    a = conf.a_nbody
+    tab = pnb.kdk_factor_table(a[:-1], a[1:], c, conf)
+    per = np.array([pnb._kdk_factors(a[i], a[i + 1], c, conf) for i in range(len(a) - 1)],
+                   dtype=np.float32)
+    np.testing.assert_array_equal(tab, per)
