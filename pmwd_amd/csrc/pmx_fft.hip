@@ -420,6 +420,15 @@ __device__ __forceinline__ void stage_twiddles(float2* stw, const float2* __rest
                        [&](int i, float2 v) { stw[base + i] = v; });
 }
 
+// Logical block of a pass: the XCD-contiguous remap of the hardware block
+// index, in reverse when rev is set.  Consecutive passes of a transform run
+// in opposite directions, so that each first reads what the pass before it
+// wrote last, while that is still in the last-level cache.
+__device__ __forceinline__ unsigned pass_block(unsigned lin, unsigned nb, int rev) {
+  const unsigned l = pmx_xcd_block(lin, nb);
+  return rev ? nb - 1 - l : l;
+}
+
 // ------------------------------------------------------------ z passes --
 // In-place r2c of padded rows as a length-n2 complex FFT of the real row
 // (any n2); PRE applies (x - 1) * coef on load (gravity.py:52-54).
@@ -552,7 +561,8 @@ template <int PRE, int LH, bool FOLD = false>
 __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_half_k(float* __restrict__ data, long long nrows,
                                                            int n2, int rs, int P_, float coef,
                                                            const float2* __restrict__ tw,
-                                                           FoldIn fi, const float* __restrict__ fcoef) {
+                                                           FoldIn fi, const float* __restrict__ fcoef,
+                                                           int rev) {
   extern __shared__ float2 s[];
   // 1.5 Omega_m from the device factor table when the call has one
   if (PRE && fcoef) coef = fcoef[0];
@@ -561,7 +571,7 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_half_k(float* __restrict__ d
   const int lH = LH ? LH : __ffs(H) - 1;
   const int P = LH ? PC : P_;
   float2* stw = s + lds_elems(ROWS, H, P);
-  const long long row0 = (long long)blockIdx.x * P;
+  const long long row0 = (long long)pass_block(blockIdx.x, gridDim.x, rev) * P;
   const int np = LH ? PC : (int)((nrows - row0) < P ? (nrows - row0) : P);
   const int rsc = rs / 2;
   float2* blk = reinterpret_cast<float2*>(data) + row0 * rsc;
@@ -621,14 +631,14 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_half_k(float* __restrict__ d
 template <int LH>
 __global__ void __launch_bounds__(FFT_TPB, 4) z_c2r_half_k(float* __restrict__ data, long long nrows,
                                                            int n2, int rs, int P_, float scale,
-                                                           const float2* __restrict__ tw) {
+                                                           const float2* __restrict__ tw, int rev) {
   extern __shared__ float2 s[];
   constexpr int PC = LH ? row_pencils(1 << LH) : 1;
   const int H = LH ? 1 << LH : n2 >> 1;
   const int lH = LH ? LH : __ffs(H) - 1;
   const int P = LH ? PC : P_;
   float2* stw = s + lds_elems(ROWS, H, P);
-  const long long row0 = (long long)blockIdx.x * P;
+  const long long row0 = (long long)pass_block(blockIdx.x, gridDim.x, rev) * P;
   const int np = LH ? PC : (int)((nrows - row0) < P ? (nrows - row0) : P);
   const int rsc = rs / 2;
   float2* blk = reinterpret_cast<float2*>(data) + row0 * rsc;
@@ -681,18 +691,20 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_c2r_half_k(float* __restrict__ d
 }
 
 // ------------------------------------------------------ strided passes --
+// rev: the pass runs its blocks in the reverse order (pass_block)
 struct Fields3 {
   float2* f[3];
+  int rev;
 };
 
 // Block coordinates of a (gx, gy, gz) grid after the XCD-contiguous remap
 // of the linear block index: neighbouring column blocks, which share the
 // 128-byte lines of a row, then run on one XCD close together and meet in
 // its L2.
-__device__ __forceinline__ void xcd_block3(int& bx, int& by, int& bz) {
+__device__ __forceinline__ void xcd_block3(int& bx, int& by, int& bz, int rev) {
   const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
   const unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  const unsigned l = pmx_xcd_block(lin, nb);
+  const unsigned l = pass_block(lin, nb, rev);
   bx = l % gridDim.x;
   by = (l / gridDim.x) % gridDim.y;
   bz = l / (gridDim.x * gridDim.y);
@@ -729,7 +741,7 @@ __global__ void __launch_bounds__(FFT_TPB, 4) strided_fft_k(Fields3 fl, int n_, 
   const int lB = LN ? ilog2c(BC) : __ffs(B_) - 1;
   float2* stw = s + n * B;
   int bx, by, bz;
-  xcd_block3(bx, by, bz);
+  xcd_block3(bx, by, bz, fl.rev);
   const int c0 = bx * B;
   const int nb = ncol - c0 < B ? ncol - c0 : B;
   const ColMap cm = col_map<FFT_TPB>(lB);
@@ -786,7 +798,7 @@ __global__ void __launch_bounds__(PFFT_TPB) __attribute__((amdgpu_waves_per_eu(P
   float* skm = sk + n0;
   const long long es = (long long)n1 * n2p;
   int bx, y, bz;
-  xcd_block3(bx, y, bz);
+  xcd_block3(bx, y, bz, fl.rev);
   const int c0 = bx * B;
   const int nb = n2h - c0 < B ? n2h - c0 : B;
   const long long off = (long long)y * n2p + c0;
@@ -885,7 +897,7 @@ __global__ void __launch_bounds__(PFFT_TPB) __attribute__((amdgpu_waves_per_eu(P
   float* skm = sk + n0;
   const long long es = (long long)n1 * n2p;
   int bx, y, bz;
-  xcd_block3(bx, y, bz);
+  xcd_block3(bx, y, bz, fl.rev);
   const int c0 = bx * B;
   const int nb = n2h - c0 < B ? n2h - c0 : B;
   const long long off = (long long)y * n2p + c0;
@@ -988,7 +1000,7 @@ __global__ void __launch_bounds__(FFT_TPB, 2) y_inv_seq_k(Fields3 fl, int n_, lo
   const int lB = LN ? ilog2c(BC) : __ffs(B_) - 1;
   float2* stw = s + n * B;
   int bx, by, bz;
-  xcd_block3(bx, by, bz);
+  xcd_block3(bx, by, bz, fl.rev);
   const int c0 = bx * B;
   const int nb = ncol - c0 < B ? ncol - c0 : B;
   const long long off = (long long)by * ostride + c0;
@@ -1047,7 +1059,7 @@ __global__ void __launch_bounds__(FFT_TPB, 2) y_fwd_seq_k(Fields3 fl, int n_, lo
   const int lB = LN ? ilog2c(BC) : __ffs(B_) - 1;
   float2* stw = s + n * B;
   int bx, by, bz;
-  xcd_block3(bx, by, bz);
+  xcd_block3(bx, by, bz, fl.rev);
   const int c0 = bx * B;
   const int nb = ncol - c0 < B ? ncol - c0 : B;
   const long long off = (long long)by * ostride + c0;
@@ -1183,7 +1195,7 @@ int pmx_fold_deferrable(const pmx_ctx* c) {
 // In-place r2c of a padded real mesh; pre applies (x - 1) * coef.  With
 // halo (one channel's shells; pmx_fold_deferrable must hold) the deposit's
 // fold is added on load.
-int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef, const float* halo) {
+int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef, const float* halo, int rev) {
   const long long nrows = (long long)c->n[0] * c->n[1];
   const int n2 = c->n[2];
   const RowPlan r = row_plan(n2);
@@ -1200,14 +1212,14 @@ int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef, const float
 #define R2C_HALF(L)                                                                                \
   if (halo && pre)                                                                                 \
     z_r2c_half_k<1, L, (L > 0)><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef,  \
-                                                                  c->tw[2], fi, fc);                   \
+                                                                  c->tw[2], fi, fc, rev);              \
   else if (halo)                                                                                   \
     z_r2c_half_k<0, L, (L > 0)><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef,  \
-                                                                  c->tw[2], fi, fc);                   \
+                                                                  c->tw[2], fi, fc, rev);              \
   else if (pre)                                                                                    \
-    z_r2c_half_k<1, L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2], fi, fc); \
+    z_r2c_half_k<1, L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2], fi, fc, rev); \
   else                                                                                             \
-    z_r2c_half_k<0, L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2], fi, fc)
+    z_r2c_half_k<0, L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, coef, c->tw[2], fi, fc, rev)
     PMX_CT_SWITCH(lh, R2C_HALF);
 #undef R2C_HALF
   } else {
@@ -1226,7 +1238,7 @@ int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef, const float
   return PMX_OK;
 }
 
-int pmx_fft_c2r_inplace(pmx_ctx* c, float* buf, float scale) {
+int pmx_fft_c2r_inplace(pmx_ctx* c, float* buf, float scale, int rev) {
   const long long nrows = (long long)c->n[0] * c->n[1];
   const int n2 = c->n[2];
   const RowPlan r = row_plan(n2);
@@ -1238,7 +1250,7 @@ int pmx_fft_c2r_inplace(pmx_ctx* c, float* buf, float scale) {
   if (r.half) {
     const int lh = nrows % P == 0 ? ct_log2(r.len) : 0;
 #define C2R_HALF(L) \
-  z_c2r_half_k<L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2])
+  z_c2r_half_k<L><<<nblk, FFT_TPB, lds, c->stream>>>(buf, nrows, n2, c->rs, P, scale, c->tw[2], rev)
     PMX_CT_SWITCH(lh, C2R_HALF);
 #undef C2R_HALF
   } else if (is_pow2(n2)) {
@@ -1252,9 +1264,10 @@ int pmx_fft_c2r_inplace(pmx_ctx* c, float* buf, float scale) {
 }
 
 // axis 1 (y) or axis 0 (x) complex FFT of nf fields in place
-int pmx_fft_strided(pmx_ctx* c, float2* const* f, int nf, int axis, int dir) {
+int pmx_fft_strided(pmx_ctx* c, float2* const* f, int nf, int axis, int dir, int rev) {
   Fields3 fl;
   for (int i = 0; i < 3; ++i) fl.f[i] = f[i < nf ? i : 0];
+  fl.rev = rev;
   int n = c->n[axis];
   int B = fft_pencils(n);
   int ncol = c->n2h;
@@ -1305,8 +1318,8 @@ static KTab ktab(const pmx_ctx* c) {
 // f[1] into the two force components in f[1] and f[2] (y_inv_seq_k);
 // dir < 0 turns the cotangents in f[1] and f[2] into km_1 FFT_y(f[1]) +
 // km_2 FFT_y(f[2]) in f[1] (y_fwd_seq_k).
-int pmx_fft_ypair(pmx_ctx* c, float2* const* f, int dir) {
-  Fields3 fl = {{f[0], f[1], f[2]}};
+int pmx_fft_ypair(pmx_ctx* c, float2* const* f, int dir, int rev) {
+  Fields3 fl = {{f[0], f[1], f[2]}, rev};
   const int n = c->n[1];
   const int B = fft_pencils(n);
   const int ncol = c->n2h;
@@ -1335,8 +1348,8 @@ int pmx_fft_ypair(pmx_ctx* c, float2* const* f, int dir) {
   return PMX_OK;
 }
 
-int pmx_poisson_x(pmx_ctx* c, float2* const* f, int adjoint, int kin) {
-  Fields3 fl = {{f[0], f[1], f[2]}};
+int pmx_poisson_x(pmx_ctx* c, float2* const* f, int adjoint, int kin, int rev) {
+  Fields3 fl = {{f[0], f[1], f[2]}, rev};
   const int n0 = c->n[0];
   const int B = fft_pencils(n0, PFFT_CAP);
   dim3 grid((c->n2h + B - 1) / B, c->n[1], 1);
@@ -1375,14 +1388,18 @@ int pmx_poisson_ctx(pmx_ctx* c, float coef, int deferred) {
   int rc;
   float2* f[3] = {c->cbuf[0], c->cbuf[1], c->cbuf[2]};
   const float* halo = deferred && pmx_fold_deferrable(c) ? c->halo : nullptr;
-  if ((rc = pmx_fft_r2c_inplace(c, (float*)f[0], 1, coef, halo))) return rc;
-  if ((rc = pmx_fft_strided(c, f, 1, 1, -1))) return rc;
-  if ((rc = pmx_poisson_x(c, f, 0, 0))) return rc;
-  if ((rc = pmx_fft_strided(c, f, 1, 1, 1))) return rc;
-  if ((rc = pmx_fft_ypair(c, f, 1))) return rc;
+  // block directions alternate from pass to pass (pass_block), the first
+  // against the deposit's and the last against the gather's; the z passes
+  // run y's components first, the x component last
+  if ((rc = pmx_fft_r2c_inplace(c, (float*)f[0], 1, coef, halo, 1))) return rc;
+  if ((rc = pmx_fft_strided(c, f, 1, 1, -1, 0))) return rc;
+  if ((rc = pmx_poisson_x(c, f, 0, 0, 1))) return rc;
+  if ((rc = pmx_fft_strided(c, f, 1, 1, 1, 0))) return rc;
+  if ((rc = pmx_fft_ypair(c, f, 1, 1))) return rc;
   float scale = (float)(1.0 / (double)c->M);
-  for (int d = 0; d < 3; ++d)
-    if ((rc = pmx_fft_c2r_inplace(c, (float*)f[d], scale))) return rc;
+  if ((rc = pmx_fft_c2r_inplace(c, (float*)f[1], scale, 0))) return rc;
+  if ((rc = pmx_fft_c2r_inplace(c, (float*)f[2], scale, 1))) return rc;
+  if ((rc = pmx_fft_c2r_inplace(c, (float*)f[0], scale, 1))) return rc;
   return PMX_OK;
 }
 
@@ -1400,14 +1417,16 @@ int pmx_poisson_adj_bufs(pmx_ctx* c, float2* const* f, float scale_out, int defe
   int rc;
   const bool fold = deferred && pmx_fold_deferrable(c);
   const long long hstride = c->ntiles * c->tg.shell;
+  // block directions alternate as in pmx_poisson_ctx
   for (int d = 0; d < 3; ++d)
-    if ((rc = pmx_fft_r2c_inplace(c, (float*)f[d], 0, 0.f, fold ? c->halo + d * hstride : nullptr)))
+    if ((rc = pmx_fft_r2c_inplace(c, (float*)f[d], 0, 0.f, fold ? c->halo + d * hstride : nullptr,
+                                  1)))
       return rc;
-  if ((rc = pmx_fft_strided(c, f, 1, 1, -1))) return rc;
-  if ((rc = pmx_fft_ypair(c, f, -1))) return rc;
-  if ((rc = pmx_poisson_x(c, f, 1, 0))) return rc;
-  if ((rc = pmx_fft_strided(c, f, 1, 1, 1))) return rc;
-  if ((rc = pmx_fft_c2r_inplace(c, (float*)f[0], scale_out))) return rc;
+  if ((rc = pmx_fft_strided(c, f, 1, 1, -1, 0))) return rc;
+  if ((rc = pmx_fft_ypair(c, f, -1, 0))) return rc;
+  if ((rc = pmx_poisson_x(c, f, 1, 0, 1))) return rc;
+  if ((rc = pmx_fft_strided(c, f, 1, 1, 1, 0))) return rc;
+  if ((rc = pmx_fft_c2r_inplace(c, (float*)f[0], scale_out, 1))) return rc;
   return PMX_OK;
 }
 

@@ -955,11 +955,13 @@ int pmx_poisson_ctx(pmx_ctx* c, float coef, int deferred = 0);
 // axis 1 (y) or 0 (x) of nf fields (dir -1 forward, +1 inverse), the paired
 // y pass of the gradient's y and z components, and the Poisson x pass (kin:
 // f[0] holds a full 3D spectrum, no forward x FFT).
-int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef, const float* halo = nullptr);
-int pmx_fft_c2r_inplace(pmx_ctx* c, float* buf, float scale);
-int pmx_fft_strided(pmx_ctx* c, float2* const* f, int nf, int axis, int dir);
-int pmx_fft_ypair(pmx_ctx* c, float2* const* f, int dir);
-int pmx_poisson_x(pmx_ctx* c, float2* const* f, int adjoint, int kin = 0);
+// rev: the pass runs its blocks in reverse order (bits unchanged)
+int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef, const float* halo = nullptr,
+                        int rev = 0);
+int pmx_fft_c2r_inplace(pmx_ctx* c, float* buf, float scale, int rev = 0);
+int pmx_fft_strided(pmx_ctx* c, float2* const* f, int nf, int axis, int dir, int rev = 0);
+int pmx_fft_ypair(pmx_ctx* c, float2* const* f, int dir, int rev = 0);
+int pmx_poisson_x(pmx_ctx* c, float2* const* f, int adjoint, int kin = 0, int rev = 0);
 // copies n0*n1 rows of ncols floats between row strides (ncols <= 0: n2)
 int pmx_copy_rows(pmx_ctx* c, float* dst, int dst_rs, const float* src, int src_rs, int ncols = 0);
 int pmx_poisson_adj_ctx(pmx_ctx* c, float scale_out, int deferred = 0);
