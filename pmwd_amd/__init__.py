@@ -15,7 +15,7 @@ from .gather import gather
 from .gravity import laplace, neg_grad, gravity
 from .modes import white_noise, linear_modes
 from .lpt import lpt
-from .nbody import nbody, nbody_adj
+from .nbody import nbody, nbody_adj, reversal_checkpoints
 from .spec_util import powspec
 
 __version__ = '0.1.0'

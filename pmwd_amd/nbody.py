@@ -11,7 +11,9 @@ splittings run the ops one by one.
 The adjoint reconstructs the trajectory backward in time from the final
 state as pmwd's ``nbody_adj`` (nbody.py:226-260) does, optionally snapping
 the reconstruction to sparse forward checkpoints (every ``checkpoint_every``
-steps) to bound the float32 reversal drift.  The cosmology cotangent holds
+steps, or at given steps) to bound the float32 reversal drift;
+``reversal_checkpoints`` finds the steps where that drift exceeds a
+tolerance, so that checkpoints go only there.  The cosmology cotangent holds
 Omega_m (time-step factor sensitivities from the growth ODE's forward
 sensitivity, and the force term sum(vel_cot*acc)/Omega_m) and, when set,
 Omega_k_, w_0_ and w_a_, which enter only the time-step factors.
@@ -243,6 +245,24 @@ def _store(t, where):
     return _HostCopy(t)
 
 
+def checkpoint_steps(checkpoint_every, nsteps):
+    """Forward step indices before which the state is stored: every k-th step
+    for an int k (k, 2k, ...; none for 0), or the steps of a sequence; each
+    in [1, nsteps - 1]."""
+    if checkpoint_every is None or checkpoint_every is False:
+        return frozenset()
+    if isinstance(checkpoint_every, (int, np.integer)):
+        k = int(checkpoint_every)
+        if k < 0:
+            raise ValueError(f'checkpoint_every={k} must not be negative')
+        return frozenset(range(k, nsteps, k)) if k else frozenset()
+    steps = frozenset(int(i) for i in checkpoint_every)
+    bad = sorted(i for i in steps if not 0 < i < nsteps)
+    if bad:
+        raise ValueError(f'checkpoint steps {bad} outside [1, {nsteps - 1}]')
+    return steps
+
+
 def kdk_factor_table(a_prev, a_next, cosmo, conf):
     """float32 (k1, d, k2) rows of the KDK steps a_prev[i] -> a_next[i]
     (nbody.py:205-212), evaluated on the arrays at once: the same float64
@@ -284,18 +304,18 @@ def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=N
     _force(c, pmid, disp, acc, n, val, coef)
     ckpt = {}
     steps = list(zip(a_nbody[:-1], a_nbody[1:]))
+    cset = checkpoint_steps(checkpoint_every, len(steps))
     # 'auto' decides at the first checkpoint, once the pipeline holds its scratch
     ckpt_where = checkpoint_device if checkpoint_device != 'auto' else None
-    n_ckpt = (len(steps) - 1) // checkpoint_every if checkpoint_every else 0
+    n_ckpt = len(cset)
     if _is_kdk(conf):
         i = 0
         while i < len(steps):
-            if checkpoint_every and i % checkpoint_every == 0 and i > 0:
+            if i in cset:
                 if checkpoint_device == 'auto' and not ckpt:
                     ckpt_where = _checkpoint_where('auto', disp, n_ckpt)
                 ckpt[i] = (_store(disp, ckpt_where), _store(vel, ckpt_where))
-            j = min(len(steps), (i // checkpoint_every + 1) * checkpoint_every
-                    if checkpoint_every else len(steps))
+            j = min([t for t in cset if t > i], default=len(steps))
             if not pipeline:
                 j = i + 1
             seg = np.array(steps[i:j], dtype=np.float64).reshape(-1, 2)
@@ -313,7 +333,7 @@ def _run_forward(pmid, disp, vel, cosmo, conf, reverse=False, checkpoint_every=N
             i = j
         return acc, ckpt
     for i, (a_prev, a_next) in enumerate(steps):
-        if checkpoint_every and i % checkpoint_every == 0 and i > 0:
+        if i in cset:
             if checkpoint_device == 'auto' and not ckpt:
                 ckpt_where = _checkpoint_where('auto', disp, n_ckpt)
             ckpt[i] = (_store(disp, ckpt_where), _store(vel, ckpt_where))
@@ -582,10 +602,12 @@ def nbody(ptcl, obsvbl, cosmo, conf, reverse=False, checkpoint_every=None, omega
     and, when ``omega_m`` is a 0-d float64 tensor equal to
     ``cosmo.Omega_m`` that requires grad, w.r.t. Omega_m.  With
     ``checkpoint_every`` the adjoint restarts from states stored every that
-    many steps, kept on the device (None), in host memory ('cpu': pinned
-    1 GiB chunks, copied asynchronously) or where they fit ('auto': the
-    device when, at the first checkpoint, they take at most half of its free
-    memory).  The backward frees each checkpoint once it has used it.
+    many steps (an int) or before the given forward steps (a sequence, as
+    ``reversal_checkpoints`` returns), kept on the device (None), in host
+    memory ('cpu': pinned 1 GiB chunks, copied asynchronously) or where they
+    fit ('auto': the device when, at the first checkpoint, they take at most
+    half of its free memory).  The backward frees each checkpoint once it
+    has used it.
     """
     if cosmo.growth is None:
         raise ValueError('Growth table is empty. Call growth_integ or boltzmann first.')
@@ -630,3 +652,62 @@ def nbody_adj(ptcl, ptcl_cot, cosmo, conf, reverse=False):
                               transfer=None, growth=None, growth_dOm=None, varlin=None, **de)
     return (ptcl.replace(disp=disp, vel=vel, acc=acc),
             ptcl_cot.replace(disp=dc, vel=vc, acc=acc_cot), cosmo_cot)
+
+
+def reversal_checkpoints(ptcl, cosmo, conf, tol=1e-4, candidates=1, checkpoint_device='auto'):
+    """The forward steps at which float32 reversal needs a checkpoint:
+    those where the reconstructed state drifts from the forward one by more
+    than ``tol``, relative: the larger of ||disp_rev - disp|| / ||disp|| and
+    the same for vel (L2 over all particles).
+
+    Runs ``nbody`` forward from ``ptcl`` storing the state before every
+    ``candidates``-th step (an int) or before the given steps (a sequence),
+    then integrates back from the final state one KDK step at a time, as the
+    adjoint reconstructs the trajectory.  Walking back, at each stored step
+    it measures the drift; where it exceeds ``tol`` the step is marked and
+    the reconstruction continues from the stored state, as the adjoint would
+    with a checkpoint there.  Returns ``(steps, drift)``: the marked steps,
+    ascending, to pass as ``nbody``'s ``checkpoint_every``, and {step:
+    relative drift} at every candidate.  No pmwd counterpart: pmwd reverses
+    without checkpoints.
+    """
+    if not _is_kdk(conf):
+        raise NotImplementedError('reversal_checkpoints needs the KDK splitting')
+    _lib.require_device(ptcl.disp, 'ptcl.disp')
+    L = _lib.lib()
+    pmid = ptcl.pmid.contiguous()
+    n = pmid.shape[0]
+    a = conf.a_nbody
+    nsteps = len(a) - 1
+    with torch.no_grad():
+        disp = ptcl.disp.detach().clone().contiguous()
+        vel = (ptcl.vel if ptcl.vel is not None else torch.zeros_like(disp)).detach().clone()
+        vel = vel.contiguous()
+        c = _lib.context(conf.mesh_shape, conf.cell_size, disp.device)
+        acc, ckpt = _run_forward(pmid, disp, vel, cosmo, conf, False, candidates,
+                                 checkpoint_device, pctx=c)
+        h = c.bind()
+        val, coef = _mass_val(conf), omega_coef(cosmo, conf)
+        f = kdk_factor_table(a[1:][::-1], a[:-1][::-1], cosmo, conf)  # reverse steps
+        ref = torch.empty_like(disp)
+        marks, drift = [], {}
+        for r, s in enumerate(range(nsteps - 1, -1, -1)):   # undoes forward step s
+            k1, d, k2 = (float(x) for x in f[r])
+            _lib.check(L.pmx_nbody_step(h, _lib.ptr(pmid), _lib.ptr(disp), _lib.ptr(vel),
+                                        _lib.ptr(acc), n, val, coef, k1, d, k2))
+            if s in ckpt:
+                sd, sv = ckpt.pop(s)
+                rel = []
+                for now, saved in ((disp, sd), (vel, sv)):
+                    _restore(ref, saved)
+                    e = float(torch.linalg.vector_norm((now - ref).double()))
+                    r = float(torch.linalg.vector_norm(ref.double()))
+                    rel.append(e / r if r > 0 else e)
+                drift[s] = max(rel)
+                if drift[s] > tol:
+                    marks.append(s)
+                    _restore(disp, sd)
+                    _restore(vel, sv)
+                    _force(c, pmid, disp, acc, n, val, coef)
+                del sd, sv
+    return sorted(marks), drift

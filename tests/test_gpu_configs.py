@@ -251,6 +251,48 @@ def test_c2_gradients(dev):
             assert abs(an - want) < 1e-2 * scale, res
 
 
+def test_c2_drift_checkpoints(dev):
+    """reversal_checkpoints on the 128^3-particle / 256^3-mesh configuration
+    (63 steps): the relative drift of pure float32 reversal grows toward the
+    start (measured: 1.4e-8 one step back from the end, 3.2e-4 at step 1),
+    so at tolerance 3e-5 one checkpoint is placed (step 3).  With that one
+    stored state the gradients match the float64 fixture nearly as well as
+    with seven every 8 steps (Omega_m 1.5e-3 and 1.3e-3 relative, against
+    1.1e-2 by pure reversal)."""
+    import pmwd_amd as pa
+    z = _c2_fixture()
+    conf, _ = _confs(2.0, 128, 2, 1 / 64)
+    cosmo = _gpu_cosmo(conf)
+    white0 = pa.white_noise(0, conf, real=True, device=dev)
+    with torch.no_grad():
+        ptcl0, _, _ = _gpu_model(white0, cosmo, conf)
+        tcos = _gpu_cosmo(conf, sigma8=0.85)
+        _, _, tgt = _gpu_model(pa.white_noise(1, conf, real=True, device=dev), tcos, conf)
+        tgt = tgt.double()
+    nsteps = len(conf.a_nbody) - 1
+    none, drift = pa.reversal_checkpoints(ptcl0, cosmo, conf, tol=float('inf'))
+    assert none == [] and sorted(drift) == list(range(1, nsteps))
+    assert all(np.isfinite(v) for v in drift.values())
+    assert drift[1] > drift[nsteps // 2] > drift[nsteps - 1] and drift[1] > 1e-4
+    every, _ = pa.reversal_checkpoints(ptcl0, cosmo, conf, tol=0.0, candidates=8)
+    assert every == list(range(8, nsteps, 8))
+    marks, drift_t = pa.reversal_checkpoints(ptcl0, cosmo, conf, tol=3e-5)
+    assert 1 <= len(marks) <= 3 and max(drift_t.values()) < 10 * 3e-5
+
+    w = white0.clone().requires_grad_(True)
+    theta = {'Omega_m': torch.tensor(cosmo.Omega_m, dtype=torch.float64, requires_grad=True),
+             'sigma8': torch.tensor(cosmo.sigma8, dtype=torch.float64, requires_grad=True)}
+    _, _, dens = _gpu_model(w, cosmo, conf, theta=theta, checkpoint_every=marks)
+    loss = ((dens.double() - tgt)**2).mean()
+    _, gom, gs8 = torch.autograd.grad(loss, (w, theta['Omega_m'], theta['sigma8']))
+    res = dict(marks=marks, drift_no_snaps={k: drift[k] for k in (1, 2, 3, 8, 31, 62)},
+               grad_Omega_m=float(gom), f64_Omega_m=float(z['loss_dOm']),
+               grad_sigma8=float(gs8), f64_sigma8=float(z['loss_ds8']))
+    _record('C2_drift_checkpoints', **res)
+    assert abs(float(gom) - z['loss_dOm']) < 2.5e-3 * abs(z['loss_dOm']), res
+    assert abs(float(gs8) - z['loss_ds8']) < 1e-3 * abs(z['loss_ds8']), res
+
+
 def _ic_objective(conf, dev, seed_ab=11):
     """J(white, theta) = <a, disp> + <b, vel> of the 2LPT initial conditions
     (linear_modes -> lpt) for fixed standard normal a, b (numpy

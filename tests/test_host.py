@@ -218,3 +218,18 @@ def test_vectorised_step_factors_equal_per_step():
     per = np.array([pnb._kdk_factors(a[i], a[i + 1], c, conf) for i in range(len(a) - 1)],
                    dtype=np.float32)
     np.testing.assert_array_equal(tab, per)
+
+
+def test_checkpoint_steps():
+    """nbody's checkpoint_every: every k-th step, or the steps given (as
+    reversal_checkpoints returns them), within [1, nsteps - 1]."""
+    assert sorted(pnb.checkpoint_steps(8, 31)) == [8, 16, 24]
+    assert sorted(pnb.checkpoint_steps(8, 32)) == [8, 16, 24]
+    assert sorted(pnb.checkpoint_steps(8, 33)) == [8, 16, 24, 32]
+    assert pnb.checkpoint_steps(None, 31) == frozenset()
+    assert pnb.checkpoint_steps(0, 31) == frozenset()
+    assert pnb.checkpoint_steps([], 31) == frozenset()
+    assert sorted(pnb.checkpoint_steps([30, 3, 3], 31)) == [3, 30]
+    for bad in ([0], [31], -2):
+        with pytest.raises(ValueError):
+            pnb.checkpoint_steps(bad, 31)
