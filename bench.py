@@ -45,6 +45,9 @@ def parse():
                    help='instead of the default run: forward plus gradient of a density MSE '
                         'for G^3 particles on a G^3 mesh (box 2000*G/1024 Mpc/h, 31 steps, '
                         'reverse-time adjoint), with device memory per phase')
+    p.add_argument('--full-adjoint-warm', action='store_true',
+                   help='with --full-adjoint: run the forward and gradient a second time in '
+                        'the same process and report its times too')
     p.add_argument('--full-adjoint-checkpoints', type=int, default=0, metavar='S',
                    help='with --full-adjoint: snap the reverse run to states stored every S '
                         'steps, on the device when they fit, else in pinned host memory '
@@ -341,7 +344,7 @@ def fft_yardstick(conf, ctx, h, L, _lib, dev, reps=5):
             'r2c_rel_l2_vs_rocfft': rel(yo, yr), 'c2r_rel_l2_vs_rocfft': rel(xo, xr)}
 
 
-def full_adjoint(G, dev, ckpt_every=0):
+def full_adjoint(G, dev, ckpt_every=0, warm=False):
     """Forward plus gradient at G^3 particles on a G^3 mesh (the 1024^3
     configuration at G=1024): white modes (seed 0) -> linear_modes -> 2LPT ->
     31 KDK steps -> density; loss = mean((rho - target)^2) against a seeded
@@ -351,7 +354,13 @@ def full_adjoint(G, dev, ckpt_every=0):
     the device when they fit, else in pinned host memory; 0: pure reversal
     as pmwd).  The modes -> 2LPT part
     is recomputed in the backward rather than kept.  Reports wall times and
-    device memory per phase (torch peaks, library contexts, device-wide use)."""
+    device memory per phase (torch peaks, library contexts, device-wide use).
+    With warm, the same forward and gradient run a second time in the
+    process (device memory already mapped, as in an optimisation loop):
+    its times, whether its loss and modes gradient equal the first run's
+    bit for bit, and how far its Omega_m and sigma8 gradients are from the
+    first run's (float64 sums whose grouping follows the particles' order
+    within a tile, which the atomics of the re-binning set)."""
     import time
     import torch
     import pmwd_amd as pa
@@ -375,54 +384,62 @@ def full_adjoint(G, dev, ckpt_every=0):
         ptcl, _ = pa.lpt(pa.linear_modes(w, cosmo, conf, theta=th), cosmo, conf, theta=th)
         return ptcl.disp, ptcl.vel, ptcl.pmid
 
-    # The initial conditions are recomputed in the backward rather than kept
-    # (as torch.utils.checkpoint would); between the N-body adjoint and that
-    # recomputation the library contexts give back their particle scratch.
-    t0 = time.perf_counter()
-    with torch.no_grad():
-        disp0, vel0, pmid = ics(white, theta['Omega_m'], theta['sigma8'])
-    disp0.requires_grad_(True)
-    vel0.requires_grad_(True)
-    ptcl, _ = pa.nbody(pa.Particles(conf, pmid, disp0, vel0), None, cosmo, conf,
-                       omega_m=theta['Omega_m'], checkpoint_every=ckpt_every or None,
-                       checkpoint_device='auto')
-    dens = pa.scatter(ptcl, conf)
-    del ptcl
-    # the difference in float32, its square and mean in float64 (fewer
-    # full-mesh temporaries than casting both densities first)
-    loss = ((dens - target).double()**2).mean()
-    del dens
-    torch.cuda.synchronize()
-    t_fwd = time.perf_counter() - t0
     def used():
         free, total = torch.cuda.mem_get_info(dev)
         return (total - free) / 1e9
 
-    mem = {'library_context_gb_forward': sum(c.nbytes for c in _lib._ctx_cache.values()) / 1e9,
-           'device_used_gb_after_forward': used()}
-    t0 = time.perf_counter()
-    g_d0, g_v0, g_om1 = torch.autograd.grad(loss, (disp0, vel0, theta['Omega_m']))
-    torch.cuda.synchronize()
-    t_nbody_bwd = time.perf_counter() - t0
-    mem['library_context_gb_adjoint'] = sum(c.nbytes for c in _lib._ctx_cache.values()) / 1e9
-    mem['device_used_gb_after_adjoint'] = used()
-    mem['torch_peak_gb_through_adjoint'] = torch.cuda.max_memory_allocated(dev) / 1e9
-    loss_value = float(loss.detach())
-    del disp0, vel0, loss
-    _lib.trim_contexts()
-    torch.cuda.reset_peak_memory_stats(dev)
-    d1, v1, _ = ics(white, theta['Omega_m'], theta['sigma8'])
-    g_w, g_om2, g_s8 = torch.autograd.grad((d1, v1), (white, theta['Omega_m'], theta['sigma8']),
-                                           grad_outputs=(g_d0, g_v0), allow_unused=True)
-    g_om = g_om1 + (g_om2 if g_om2 is not None else 0)
-    torch.cuda.synchronize()
-    t_bwd = time.perf_counter() - t0
-    mem['torch_peak_gb_ic_backward'] = torch.cuda.max_memory_allocated(dev) / 1e9
-    mem['library_context_gb_ic_backward'] = sum(c.nbytes for c in _lib._ctx_cache.values()) / 1e9
-    mem['device_peak_gb_reported'] = max(mem['device_used_gb_after_forward'],
-                                         mem['device_used_gb_after_adjoint'])
+    def once():
+        # The initial conditions are recomputed in the backward rather than
+        # kept (as torch.utils.checkpoint would); between the N-body adjoint
+        # and that recomputation the library contexts give back their
+        # particle scratch.
+        torch.cuda.reset_peak_memory_stats(dev)
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            disp0, vel0, pmid = ics(white, theta['Omega_m'], theta['sigma8'])
+        disp0.requires_grad_(True)
+        vel0.requires_grad_(True)
+        ptcl, _ = pa.nbody(pa.Particles(conf, pmid, disp0, vel0), None, cosmo, conf,
+                           omega_m=theta['Omega_m'], checkpoint_every=ckpt_every or None,
+                           checkpoint_device='auto')
+        dens = pa.scatter(ptcl, conf)
+        del ptcl
+        # the difference in float32, its square and mean in float64 (fewer
+        # full-mesh temporaries than casting both densities first)
+        loss = ((dens - target).double()**2).mean()
+        del dens
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        mem = {'library_context_gb_forward': sum(c.nbytes for c in _lib._ctx_cache.values()) / 1e9,
+               'device_used_gb_after_forward': used()}
+        t0 = time.perf_counter()
+        g_d0, g_v0, g_om1 = torch.autograd.grad(loss, (disp0, vel0, theta['Omega_m']))
+        torch.cuda.synchronize()
+        t_nbody_bwd = time.perf_counter() - t0
+        mem['library_context_gb_adjoint'] = sum(c.nbytes for c in _lib._ctx_cache.values()) / 1e9
+        mem['device_used_gb_after_adjoint'] = used()
+        mem['torch_peak_gb_through_adjoint'] = torch.cuda.max_memory_allocated(dev) / 1e9
+        loss_value = float(loss.detach())
+        del disp0, vel0, loss
+        _lib.trim_contexts()
+        torch.cuda.reset_peak_memory_stats(dev)
+        d1, v1, _ = ics(white, theta['Omega_m'], theta['sigma8'])
+        g_w, g_om2, g_s8 = torch.autograd.grad((d1, v1), (white, theta['Omega_m'], theta['sigma8']),
+                                               grad_outputs=(g_d0, g_v0), allow_unused=True)
+        g_om = g_om1 + (g_om2 if g_om2 is not None else 0)
+        torch.cuda.synchronize()
+        t_bwd = time.perf_counter() - t0
+        mem['torch_peak_gb_ic_backward'] = torch.cuda.max_memory_allocated(dev) / 1e9
+        mem['library_context_gb_ic_backward'] = sum(c.nbytes for c in _lib._ctx_cache.values()) / 1e9
+        mem['device_peak_gb_reported'] = max(mem['device_used_gb_after_forward'],
+                                             mem['device_used_gb_after_adjoint'])
+        del d1, v1, g_d0, g_v0
+        return dict(t_fwd=t_fwd, t_bwd=t_bwd, t_nbody_bwd=t_nbody_bwd, mem=mem,
+                    loss=loss_value, g_w=g_w, g_om=float(g_om), g_s8=float(g_s8))
+
+    r = once()
+    g_w = r['g_w']
     # the modes gradient along three seeded directions (comparable across runs)
-    del d1, v1, g_d0, g_v0
     gen = torch.Generator(device=dev).manual_seed(123)
     proj = []
     for _ in range(3):
@@ -431,21 +448,37 @@ def full_adjoint(G, dev, ckpt_every=0):
         del u
     steps = len(conf.a_nbody) - 1
     n = conf.ptcl_num
-    return {'workload': f'{G}^3 particles on a {G}^3 mesh, box {2000.0 * G / 1024:g} Mpc/h, '
-                        f'{steps} KDK steps from a=1/32, 2LPT, density MSE against a seeded '
-                        f'target, gradients w.r.t. modes, Omega_m, sigma8 by reverse-time '
-                        f'integration from the final state'
-                        + (f', snapped to states stored every {ckpt_every} steps (on the '
-                           f'device when they fit, else in pinned host memory)'
-                           if ckpt_every else ''),
-            't_setup_s': t_setup, 't_forward_s': t_fwd, 't_backward_s': t_bwd,
-            't_nbody_backward_s': t_nbody_bwd,
-            'fwd_updates_per_s': n * steps / t_fwd,
-            'fwd_adj_updates_per_s': n * steps / (t_fwd + t_bwd),
-            'memory': mem, 'loss': loss_value,
-            'grad_modes_norm': float(g_w.double().norm()), 'grad_modes_proj': proj,
-            'grad_Omega_m': float(g_om),
-            'grad_sigma8': float(g_s8)}
+    out = {'workload': f'{G}^3 particles on a {G}^3 mesh, box {2000.0 * G / 1024:g} Mpc/h, '
+                       f'{steps} KDK steps from a=1/32, 2LPT, density MSE against a seeded '
+                       f'target, gradients w.r.t. modes, Omega_m, sigma8 by reverse-time '
+                       f'integration from the final state'
+                       + (f', snapped to states stored every {ckpt_every} steps (on the '
+                          f'device when they fit, else in pinned host memory)'
+                          if ckpt_every else ''),
+           't_setup_s': t_setup, 't_forward_s': r['t_fwd'], 't_backward_s': r['t_bwd'],
+           't_nbody_backward_s': r['t_nbody_bwd'],
+           'fwd_updates_per_s': n * steps / r['t_fwd'],
+           'fwd_adj_updates_per_s': n * steps / (r['t_fwd'] + r['t_bwd']),
+           'memory': r['mem'], 'loss': r['loss'],
+           'grad_modes_norm': float(g_w.double().norm()), 'grad_modes_proj': proj,
+           'grad_Omega_m': r['g_om'],
+           'grad_sigma8': r['g_s8']}
+    if warm:
+        # the first run's modes gradient waits in host memory (the device is
+        # nearly full at the adjoint's peak)
+        first = (r['loss'], r['g_om'], r['g_s8'], g_w.detach().cpu())
+        del r, g_w
+        w = once()
+        out['warm'] = {'t_forward_s': w['t_fwd'], 't_backward_s': w['t_bwd'],
+                       't_nbody_backward_s': w['t_nbody_bwd'],
+                       'fwd_updates_per_s': n * steps / w['t_fwd'],
+                       'fwd_adj_updates_per_s': n * steps / (w['t_fwd'] + w['t_bwd']),
+                       'memory': w['mem'],
+                       'same_loss': w['loss'] == first[0],
+                       'same_modes_gradient': bool(torch.equal(w['g_w'].cpu(), first[3])),
+                       'rel_diff_Omega_m': abs(w['g_om'] - first[1]) / abs(first[1]),
+                       'rel_diff_sigma8': abs(w['g_s8'] - first[2]) / abs(first[2])}
+    return out
 
 
 def main():
@@ -469,7 +502,8 @@ def main():
     if args.full_adjoint:
         log(f'full forward + gradient at {args.full_adjoint}^3')
         print(json.dumps({'full_adjoint': full_adjoint(args.full_adjoint, dev,
-                                                       args.full_adjoint_checkpoints)}))
+                                                       args.full_adjoint_checkpoints,
+                                                       args.full_adjoint_warm)}))
         return
     G = args.grid
     spacing = args.box / G
