@@ -291,6 +291,43 @@ def serving_bench(dev, n, rate, slots):
     return out
 
 
+def serving_burst(dev, n=48, grid=64, slots=8):
+    """Throughput of a burst of n forward-only 64^3/128^3 requests (31 steps,
+    seeds and cosmologies differing) submitted at once, with the N-body runs
+    replaying captured graphs (the server's default: each on its slot's
+    stream) and without (each request eager on its own worker stream).
+    Every slot has run and captured its graph before the timed burst."""
+    import time
+    import numpy as np
+    from pmwd_amd import serving as sv
+    rng = np.random.default_rng(5)
+    out = {}
+    for packs in (True, False):
+        server = sv.Server(dev, max_slots=slots, packs=packs)
+        warm = [sv.SimRequest(seed=100 + i, ptcl_grid=grid, mesh_ratio=2, box=2.0 * grid,
+                              Omega_m=0.3, sigma8=0.8) for i in range(2 * slots)]
+        for f in [server.submit(r) for r in warm]:
+            f.result()
+        reqs = [sv.SimRequest(seed=1000 + i, ptcl_grid=grid, mesh_ratio=2, box=2.0 * grid,
+                              Omega_m=float(rng.uniform(0.25, 0.35)),
+                              sigma8=float(rng.uniform(0.7, 0.9))) for i in range(n)]
+        t0 = time.perf_counter()
+        res = [f.result() for f in [server.submit(r) for r in reqs]]
+        wall = time.perf_counter() - t0
+        stats = server.pack_stats()
+        server.close()
+        lat = [r.latency for r in res]
+        key = 'graphs' if packs else 'eager'
+        steps = 31
+        out[key] = {'wall_s': wall, 'req_per_s': n / wall,
+                    'particle_updates_per_s': n * grid**3 * steps / wall,
+                    'p50_s': float(np.percentile(lat, 50)), 'p99_s': float(np.percentile(lat, 99)),
+                    'packs': stats.get(f'{grid}^3/{2 * grid}^3') if packs else None}
+    out['workload'] = (f'{n} forward-only requests of {grid}^3 particles on a {2 * grid}^3 mesh '
+                       f'(31 KDK steps) submitted at once, {slots} slots')
+    return out
+
+
 def fft_yardstick(conf, ctx, h, L, _lib, dev, reps=5):
     """rocFFT (torch.fft.rfftn/irfftn) beside the library's pmx_rfftn/irfftn on
     one mesh: mean ms per transform (HIP events on the library stream), the
@@ -758,6 +795,8 @@ def main():
     if not args.no_serve:
         log(f'serving: {args.serve_requests} requests at {args.serve_rate}/s')
         out['serving'] = serving_bench(dev, args.serve_requests, args.serve_rate, args.serve_slots)
+        log('serving: a burst of small forward requests, packed and eager')
+        out['serving_burst'] = serving_burst(dev)
 
     if not args.no_cpu_baseline:
         log('cpu baseline sample (numpy oracle)')

@@ -25,8 +25,10 @@ Scheduling:
 * forward-only requests hand their initial conditions to the PackEngine of
   their configuration: slots with resident buffers and a captured HIP graph
   of nbody + scatter whose step factors live in a device table, so one graph
-  serves every cosmology; the slots loaded at the same time run as one pack
-  (one graph launch holding each slot's graph as an independent branch).
+  serves every cosmology; each request replays its slot's graph on the
+  slot's own stream, so the slots' runs proceed side by side (or, with
+  pack_mode='pack', the slots loaded at the same time run as one pack: one
+  graph launch holding each slot's graph as an independent branch).
 
 Every kernel on the path is deterministic and each request owns its
 buffers, so a request's density and gradients do not depend on what runs
@@ -252,21 +254,29 @@ class _Slot:
 
 class PackEngine:
     """Forward-only N-body runs of one size class (one Configuration shape)
-    packed into graph launches.
+    as replays of captured graphs.
 
     Each of ``nslots`` slots keeps its context, buffers and captured graph
     resident.  A request's initial conditions are copied into a free slot
-    with its factor table; the launcher thread runs every slot loaded by
-    then as one pack (``pmx_pack_launch``: the slots' graphs as independent
-    branches of one graph, one launch for all of them), waiting ``window``
-    seconds for slots still loading.  A slot's first run is eager and
-    captures its graph for the next ones.  Results are bitwise those of
-    ``nbody`` followed by ``scatter``.
+    with its factor table.  A slot's first run is eager and captures its
+    graph for the next ones.  Results are bitwise those of ``nbody``
+    followed by ``scatter``.  Then, by ``mode``:
+
+    * ``'replay'``: the request replays its slot's graph on the slot's own
+      stream at once (``pmx_graph_launch``); the slots' streams run side by
+      side.
+    * ``'pack'``: the launcher thread runs every slot loaded by then as one
+      pack (``pmx_pack_launch``: the slots' graphs as independent branches of
+      one graph, one launch for all of them), waiting ``window`` seconds for
+      slots still loading.
     """
 
-    def __init__(self, conf, device, nslots=4, window=0.002):
+    def __init__(self, conf, device, nslots=4, window=0.002, mode='replay'):
+        if mode not in ('replay', 'pack'):
+            raise ValueError(f"mode must be 'replay' or 'pack', not {mode!r}")
         self.conf = conf
         self.device = torch.device(device)
+        self.mode = mode
         self.slots = [_Slot(conf, self.device) for _ in range(nslots)]
         self.window = float(window)
         self._cv = threading.Condition()
@@ -309,12 +319,19 @@ class PackEngine:
                     s.stream.synchronize()
                     dens = s.dens.cpu().numpy()
                     s.capture()
+                elif self.mode == 'replay':
+                    _lib.check(_lib.lib().pmx_graph_launch(s.h))
+                    dens = s.dens.cpu().numpy()
                 else:
                     s.loaded = torch.cuda.Event()
                     s.loaded.record(s.stream)
-            if eager:
+            if eager or self.mode == 'replay':
                 with self._cv:
-                    self.eager_runs += 1
+                    if eager:
+                        self.eager_runs += 1
+                    else:
+                        self.launches += 1
+                        self.packed += 1
                     self._loading -= 1
                     self._free.append(i)
                     self._cv.notify_all()
@@ -403,7 +420,7 @@ class Server:
     of them concurrently on one GPU (see the module docstring)."""
 
     def __init__(self, device='cuda:0', hbm_budget=None, max_slots=8, packs=True,
-                 pack_window=0.002):
+                 pack_window=0.002, pack_slots=None, pack_mode='replay'):
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device('cuda', torch.cuda.current_device())
@@ -424,6 +441,8 @@ class Server:
         # forward-only requests: one PackEngine per configuration
         self.packs = bool(packs)
         self.pack_window = float(pack_window)
+        self.pack_slots = pack_slots
+        self.pack_mode = pack_mode
         self._engines = {}
         self._engine_lock = threading.Lock()
         self._admitter = threading.Thread(target=self._admit_loop, daemon=True)
@@ -459,24 +478,27 @@ class Server:
     def _engine_key(req):
         return (req.ptcl_grid, req.mesh_ratio, req.box, req.a_start, req.a_nbody_maxstep)
 
+    def _slots_for(self, req):
+        return int(self.pack_slots) if self.pack_slots else self._engine_slots(req)
+
     @staticmethod
     def _engine_slots(req):
-        # four slots for small simulations, two from 256^3 particles
-        return max(2, min(4, (1 << 22) // req.ptcl_grid**3))
+        # as many slots as requests in flight for small simulations (their
+        # replays run side by side on the slots' streams), two from 256^3
+        return 8 if req.ptcl_grid**3 <= 1 << 21 else 2
 
-    @classmethod
-    def _engine_bytes(cls, req):
+    def _engine_bytes(self, req):
         conf = req.conf()
         n, m = conf.ptcl_num, conf.mesh_size
-        return cls._engine_slots(req) * (context_bytes(conf.mesh_shape, n, False) + 42 * n + 4 * m)
+        return self._slots_for(req) * (context_bytes(conf.mesh_shape, n, False) + 42 * n + 4 * m)
 
     def _engine_for(self, req):
         key = self._engine_key(req)
         with self._engine_lock:
             e = self._engines.get(key)
             if e is None:
-                e = PackEngine(req.conf(), self.device, self._engine_slots(req),
-                               window=self.pack_window)
+                e = PackEngine(req.conf(), self.device, self._slots_for(req),
+                               window=self.pack_window, mode=self.pack_mode)
                 with self._cv:
                     self._pool_bytes += self._engine_bytes(req)
                 self._engines[key] = e

@@ -85,11 +85,13 @@ def test_gradient_request_uses_only_pooled_contexts(dev, monkeypatch):
             c.close()
 
 
-def test_packed_graph_runs_equal_the_eager_path(dev):
-    """Forward-only requests of one configuration submitted together run in
-    packs (several slots' captured graphs in one launch) once each slot has
-    captured its graph; every density equals, bit for bit, the uncaptured
-    public-API path (nbody then scatter) of run_request."""
+@pytest.mark.parametrize('mode', ['pack', 'replay'])
+def test_packed_graph_runs_equal_the_eager_path(dev, mode):
+    """Forward-only requests of one configuration submitted together replay
+    their slots' captured graphs once each slot has captured its graph: in
+    packs (several slots' graphs in one launch) or each on its slot's
+    stream; every density equals, bit for bit, the uncaptured public-API
+    path (nbody then scatter) of run_request."""
     from pmwd_amd import _lib
     from pmwd_amd.serving import Server, SimRequest, run_request
     kw = dict(a_start=1 / 8, a_nbody_maxstep=1 / 8)
@@ -97,7 +99,7 @@ def test_packed_graph_runs_equal_the_eager_path(dev):
                        Omega_m=0.26 + 0.02 * i, sigma8=0.72 + 0.03 * i, **kw) for i in range(8)]
     with torch.cuda.stream(torch.cuda.Stream(dev)):
         want = [run_request(r, dev)['dens'] for r in reqs]
-    s = Server(dev, max_slots=8, pack_window=0.05)
+    s = Server(dev, max_slots=8, pack_window=0.05, pack_mode=mode)
     # a slot's first run is eager and captures its graph; later runs replay
     # it, several slots per launch when they load within the window
     got = []
@@ -108,7 +110,10 @@ def test_packed_graph_runs_equal_the_eager_path(dev):
     st = next(iter(stats.values()))
     assert 1 <= st['eager_runs'] <= st['slots'], stats
     assert st['packed_runs'] + st['eager_runs'] == 3 * len(reqs), stats
-    assert st['mean_pack_size'] > 1, stats
+    if mode == 'pack':
+        assert st['mean_pack_size'] > 1, stats
+    else:
+        assert st['mean_pack_size'] == 1, stats
     for rnd in got:
         for r, w in zip(rnd, want):
             np.testing.assert_array_equal(r.dens, w)
