@@ -795,7 +795,7 @@ def main():
     if not args.no_serve:
         log(f'serving: {args.serve_requests} requests at {args.serve_rate}/s')
         out['serving'] = serving_bench(dev, args.serve_requests, args.serve_rate, args.serve_slots)
-        log('serving: a burst of small forward requests, packed and eager')
+        log('serving: a burst of small forward requests, graph replays and eager')
         out['serving_burst'] = serving_burst(dev)
 
     if not args.no_cpu_baseline:
