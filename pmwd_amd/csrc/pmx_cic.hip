@@ -213,10 +213,37 @@ __device__ __forceinline__ void dep_one(unsigned long long* acc, const CicFrac& 
   }
 }
 
+// The same for non-negative contributions (a non-negative constant value,
+// the density): the tile's 64-bit sums as two 32-bit words, lo[] taking
+// every contribution (< 2^30) and hi[] one per carry out of lo, so each add
+// touches one LDS bank instead of two.  hi * 2^32 + lo is the 64-bit sum.
+__device__ __forceinline__ void dep_one_u32(unsigned* lo, unsigned* hi, const CicFrac& fr, int b,
+                                            int LY, int LZ, float vv, float sf) {
+  const float a0 = fr.w[0][0], a1 = fr.w[0][1], b0 = fr.w[1][0], b1 = fr.w[1][1];
+  const float c0 = fr.w[2][0], c1 = fr.w[2][1];
+  const int sxy = __umul24(LY, LZ);
+  unsigned q[8], old[8];
+  int li[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    float f0 = (j & 1) ? a1 : a0;
+    float f1 = (j & 2) ? b1 : b0;
+    float f2 = (j & 4) ? c1 : c0;
+    li[j] = b + ((j & 1) ? sxy : 0) + ((j & 2) ? LZ : 0) + ((j & 4) ? 1 : 0);
+    float x = vv * ((f0 * f1) * f2);
+    q[j] = (unsigned)(int)rintf(x * sf);
+    old[j] = atomicAdd(&lo[li[j]], q[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    if (old[j] + q[j] < old[j]) atomicAdd(&hi[li[j]], 1u);
+}
+
 // Converts the accumulated tile back to float: the interior to the mesh
 // (row stride mrs), the +1 shell to the tile's halo slot hs.
-template <int TPB>
-__device__ __forceinline__ void dep_store(const unsigned long long* acc, float* mesh, float* hs,
+// sum(i): the 64-bit fixed-point sum of LDS cell i
+template <int TPB, class Sum>
+__device__ __forceinline__ void dep_store(Sum sum, float* mesh, float* hs,
                                           const int o[3], const int s[3], int LY, int LZ,
                                           const MeshGeom& g, int mrs, int accumulate,
                                           double inv) {
@@ -227,7 +254,7 @@ __device__ __forceinline__ void dep_store(const unsigned long long* acc, float* 
     for (int i = threadIdx.x; i < 4096; i += TPB) {
       const int lz = i & 15, ly = (i >> 4) & 15, lx = i >> 8;
       float* p = base + ((long long)lx * g.n1 + ly) * mrs + lz;
-      float a = (float)((double)(long long)acc[(lx * 17 + ly) * 17 + lz] * inv);
+      float a = (float)((double)sum((lx * 17 + ly) * 17 + lz) * inv);
       *p = accumulate ? *p + a : a;
     }
   } else {
@@ -236,7 +263,7 @@ __device__ __forceinline__ void dep_store(const unsigned long long* acc, float* 
       int ly = (i / s[2]) % s[1];
       int lx = i / (s[2] * s[1]);
       long long m = ((long long)(o[0] + lx) * g.n1 + (o[1] + ly)) * mrs + (o[2] + lz);
-      float a = (float)((double)(long long)acc[(lx * LY + ly) * LZ + lz] * inv);
+      float a = (float)((double)sum((lx * LY + ly) * LZ + lz) * inv);
       mesh[m] = accumulate ? mesh[m] + a : a;
     }
   }
@@ -254,7 +281,7 @@ __device__ __forceinline__ void dep_store(const unsigned long long* acc, float* 
       int k = i - offA - offB;
       qx = k / s[1]; qy = k % s[1]; qz = s[2];
     }
-    hs[i] = (float)((double)(long long)acc[(qx * LY + qy) * LZ + qz] * inv);
+    hs[i] = (float)((double)sum((qx * LY + qy) * LZ + qz) * inv);
   }
 }
 
@@ -284,8 +311,9 @@ __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DE
     // the particle rounds, with the float or the double fixed-point conversion
     // (the next round's records are loaded before this round's arithmetic,
     // so they are in flight meanwhile)
-    auto rounds = [&](auto fqc) {
+    auto rounds = [&](auto fqc, auto u32c) {
     constexpr bool FQ = decltype(fqc)::value;
+    constexpr bool U32 = decltype(u32c)::value;
     float4 rn[DEP_U];
     float vn[DEP_U];
     auto fetch = [&](uint32_t q0, float4* r, float* v) {
@@ -327,18 +355,35 @@ __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DE
           const float d[3] = {r[u].x, r[u].y, r[u].z};
           fr = pmx_frac_of_d(d);
         }
-        dep_one<FQ>(acc, fr, __umul24(__umul24(lx, LY) + ly, LZ) + lz, LY, LZ, v[u], scale, sf);
+        const int b = __umul24(__umul24(lx, LY) + ly, LZ) + lz;
+        if constexpr (U32)
+          dep_one_u32(reinterpret_cast<unsigned*>(acc), reinterpret_cast<unsigned*>(acc) + ltot,
+                      fr, b, LY, LZ, v[u], sf);
+        else
+          dep_one<FQ>(acc, fr, b, LY, LZ, v[u], scale, sf);
       }
     }
     };
-    if (fq)
-      rounds(std::true_type{});
+    // split words for a non-negative constant value (the density)
+    const bool u32 = !PERVAL && fq && val >= 0.f;
+    if (u32)
+      rounds(std::true_type{}, std::true_type{});
+    else if (fq)
+      rounds(std::true_type{}, std::false_type{});
     else
-      rounds(std::false_type{});
+      rounds(std::false_type{}, std::false_type{});
     __syncthreads();
     float* mesh = ch == 0 ? out.mesh[0] : (ch == 1 ? out.mesh[1] : out.mesh[2]);
-    dep_store<DEP_TPB>(acc, mesh, halo + ch * hstride + (long long)t * tg.shell, o, s, LY, LZ, g,
-                       mrs, accumulate, inv);
+    float* hs = halo + ch * hstride + (long long)t * tg.shell;
+    if (u32) {
+      const unsigned* lo = reinterpret_cast<const unsigned*>(acc);
+      const unsigned* hi = lo + ltot;
+      dep_store<DEP_TPB>([&](int i) { return (long long)(((unsigned long long)hi[i] << 32) | lo[i]); },
+                         mesh, hs, o, s, LY, LZ, g, mrs, accumulate, inv);
+    } else {
+      dep_store<DEP_TPB>([&](int i) { return (long long)acc[i]; }, mesh, hs, o, s, LY, LZ, g,
+                         mrs, accumulate, inv);
+    }
     __syncthreads();  // acc is cleared for the next channel
   }
 }

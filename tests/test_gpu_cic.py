@@ -133,6 +133,29 @@ def test_scatter_fixed_point_extremes(dev, val):
     np.testing.assert_allclose(got, want, rtol=2e-6, atol=2e-6 * val)
 
 
+@pytest.mark.parametrize('clustered', [False, True])
+def test_scatter_sign_symmetry(dev, clustered):
+    """A non-negative value accumulates in split 32-bit words (a carry word
+    beside the low word), a negative one in 64-bit words; both hold the same
+    integer sums, so scatter with -v is -(scatter with v) bit for bit.  The
+    clustered case puts 32^3 particles within a few cells, so each cell takes
+    thousands of contributions and the low words carry many times."""
+    import pmwd_amd as pa
+    conf = conf_of((32, 32, 32), 1)
+    pmid, disp = make_ptcl((32, 32, 32), conf.mesh_shape, conf.cell_size, 1.3, seed=4)
+    if clustered:
+        rng = np.random.default_rng(8)
+        target = np.array([5.3, 17.8, 30.1]) * conf.cell_size
+        disp = (target - pmid.astype(np.float64) * conf.cell_size
+                + 1.5 * conf.cell_size * rng.standard_normal(disp.shape)).astype(np.float32)
+    ptcl = to_ptcl(conf, pmid, disp, dev)
+    pos = pa.scatter(ptcl, conf, val=1.7).cpu().numpy()
+    neg = pa.scatter(ptcl, conf, val=-1.7).cpu().numpy()
+    np.testing.assert_array_equal(neg, -pos)
+    want = opm.scatter(pmid, disp, conf.cell_size, conf.mesh_shape, val=1.7, dtype=np.float64)
+    assert rel_l2(pos, want) < 1e-6
+
+
 def test_scatter_accumulates_into_mesh(dev):
     import pmwd_amd as pa
     conf = conf_of((16, 16, 16), 1)
