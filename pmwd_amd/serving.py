@@ -142,7 +142,8 @@ def run_request(req, device, engine=None):
     """Runs one request with the public API on the current stream; returns
     the SimResult fields (device work finished, results on the host).  With
     ``engine`` (a PackEngine of the request's configuration) a forward-only
-    request's nbody and scatter run in the engine's packed graph launches."""
+    request's nbody and scatter run as a replay of the engine's captured
+    graphs."""
     from .boltzmann import boltzmann
     from .cosmology import Cosmology
     from .lpt import lpt
@@ -306,6 +307,7 @@ class PackEngine:
             i = self._free.pop()
             self._loading += 1
         s = self.slots[i]
+        loading = True
         try:
             s.stream.wait_stream(src_stream)
             with torch.cuda.device(self.device), torch.cuda.stream(s.stream):
@@ -332,13 +334,11 @@ class PackEngine:
                     else:
                         self.launches += 1
                         self.packed += 1
-                    self._loading -= 1
-                    self._free.append(i)
-                    self._cv.notify_all()
                 return dens
             s.done.clear()
             with self._cv:
                 self._loading -= 1
+                loading = False
                 self._ready.append(i)
                 self._cv.notify_all()
             s.done.wait()
@@ -346,16 +346,17 @@ class PackEngine:
                 raise s.error
             s.finished.synchronize()
             with torch.cuda.device(self.device), torch.cuda.stream(s.stream):
-                dens = s.dens.cpu().numpy()
-        except BaseException:
+                return s.dens.cpu().numpy()
+        finally:
+            # the slot goes back in every case, a failed run's included (work
+            # queued on its stream orders the next user's copies after it)
             with self._cv:
+                if loading:
+                    self._loading -= 1
                 if i in self._ready:
                     self._ready.remove(i)
-            raise
-        with self._cv:
-            self._free.append(i)
-            self._cv.notify_all()
-        return dens
+                self._free.append(i)
+                self._cv.notify_all()
 
     def _pack(self, idx):
         key = tuple(idx)

@@ -138,3 +138,31 @@ def test_private_streams_are_not_pooled(dev):
     _lib.check(_lib.lib().pmx_stream_create(torch.device(dev).index or 0, ctypes.byref(h)))
     assert h.value and h.value not in pooled
     _lib.check(_lib.lib().pmx_stream_destroy(h))
+
+
+@pytest.mark.parametrize('mode', ['pack', 'replay'])
+def test_failed_run_returns_its_slot(dev, mode):
+    """A run that fails (initial conditions of the wrong shape) raises, and
+    its slot goes back: afterwards every slot serves requests, with results
+    bitwise those of the uncaptured path."""
+    from pmwd_amd.serving import PackEngine, SimRequest, run_request, nbody_factor_table
+    import pmwd_amd as pa
+    kw = dict(a_start=1 / 8, a_nbody_maxstep=1 / 8)
+    reqs = [SimRequest(seed=60 + i, ptcl_grid=16, mesh_ratio=2, box=32.0,
+                       Omega_m=0.27 + 0.01 * i, sigma8=0.8, **kw) for i in range(3)]
+    conf = reqs[0].conf()
+    eng = PackEngine(conf, dev, nslots=2, window=0.0, mode=mode)
+    try:
+        cosmo = pa.boltzmann(pa.SimpleLCDM(conf), conf)
+        bad = torch.zeros((5, 3), device=dev)
+        for _ in range(3):
+            with pytest.raises(RuntimeError):
+                eng.run(bad, bad, nbody_factor_table(cosmo, conf))
+        assert sorted(eng._free) == [0, 1] and eng._loading == 0
+        for _ in range(2):   # eager runs that capture, then replays
+            for r in reqs:
+                got = run_request(r, dev, eng)['dens']
+                np.testing.assert_array_equal(got, run_request(r, dev)['dens'])
+        assert sorted(eng._free) == [0, 1] and eng._loading == 0
+    finally:
+        eng.close()
