@@ -217,6 +217,10 @@ __device__ __forceinline__ void dep_one(unsigned long long* acc, const CicFrac& 
 // the density): the tile's 64-bit sums as two 32-bit words, lo[] taking
 // every contribution (< 2^30) and hi[] one per carry out of lo, so each add
 // touches one LDS bank instead of two.  hi * 2^32 + lo is the 64-bit sum.
+// PRE: vv is the value times sf already (a constant value of at least 2^-50,
+// so that vv * w stays a normal float and the product with the power of
+// two sf is the same float either way).
+template <bool PRE = false>
 __device__ __forceinline__ void dep_one_u32(unsigned* lo, unsigned* hi, const CicFrac& fr, int b,
                                             int LY, int LZ, float vv, float sf) {
   const float a0 = fr.w[0][0], a1 = fr.w[0][1], b0 = fr.w[1][0], b1 = fr.w[1][1];
@@ -231,7 +235,7 @@ __device__ __forceinline__ void dep_one_u32(unsigned* lo, unsigned* hi, const Ci
     float f2 = (j & 4) ? c1 : c0;
     li[j] = b + ((j & 1) ? sxy : 0) + ((j & 2) ? LZ : 0) + ((j & 4) ? 1 : 0);
     float x = vv * ((f0 * f1) * f2);
-    q[j] = (unsigned)(int)rintf(x * sf);
+    q[j] = (unsigned)(int)rintf(PRE ? x : x * sf);
     old[j] = atomicAdd(&lo[li[j]], q[j]);
   }
 #pragma unroll
@@ -285,7 +289,9 @@ __device__ __forceinline__ void dep_store(Sum sum, float* mesh, float* hs,
   }
 }
 
-template <int PERVAL, int STATE = 0>
+// FULL: every tile is a full 16^3 tile, so the LDS strides are constants
+// (corner offsets become immediates).
+template <int PERVAL, int STATE = 0, int FULL = 0>
 __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DEP_WPE)))
     deposit_k(const float4* __restrict__ rec, ValSrc vals, const float* __restrict__ vmaxp, const uint32_t* __restrict__ tile_start,
               const uint32_t* __restrict__ tile_count, MeshGeom g, TileGeom tg, float val,
@@ -295,6 +301,7 @@ __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DE
   int o[3], s[3], ti[3];
   const int t = (int)pmx_xcd_block(blockIdx.x, gridDim.x);
   tile_origin(t, tg, g, o, s, ti);
+  if (FULL) s[0] = s[1] = s[2] = 16;
   const int LY = s[1] + 1, LZ = s[2] + 1;
   const int ltot = (s[0] + 1) * LY * LZ;
   const uint32_t start = tile_start[t];
@@ -311,9 +318,10 @@ __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DE
     // the particle rounds, with the float or the double fixed-point conversion
     // (the next round's records are loaded before this round's arithmetic,
     // so they are in flight meanwhile)
-    auto rounds = [&](auto fqc, auto u32c) {
+    auto rounds = [&](auto fqc, auto u32c, auto prec) {
     constexpr bool FQ = decltype(fqc)::value;
     constexpr bool U32 = decltype(u32c)::value;
+    constexpr bool PRE = decltype(prec)::value;
     float4 rn[DEP_U];
     float vn[DEP_U];
     auto fetch = [&](uint32_t q0, float4* r, float* v) {
@@ -357,8 +365,8 @@ __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DE
         }
         const int b = __umul24(__umul24(lx, LY) + ly, LZ) + lz;
         if constexpr (U32)
-          dep_one_u32(reinterpret_cast<unsigned*>(acc), reinterpret_cast<unsigned*>(acc) + ltot,
-                      fr, b, LY, LZ, v[u], sf);
+          dep_one_u32<PRE>(reinterpret_cast<unsigned*>(acc), reinterpret_cast<unsigned*>(acc) + ltot,
+                           fr, b, LY, LZ, PRE ? v[u] * sf : v[u], sf);
         else
           dep_one<FQ>(acc, fr, b, LY, LZ, v[u], scale, sf);
       }
@@ -366,12 +374,14 @@ __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DE
     };
     // split words for a non-negative constant value (the density)
     const bool u32 = !PERVAL && fq && val >= 0.f;
-    if (u32)
-      rounds(std::true_type{}, std::true_type{});
+    if (u32 && val >= 0x1p-50f)
+      rounds(std::true_type{}, std::true_type{}, std::true_type{});
+    else if (u32)
+      rounds(std::true_type{}, std::true_type{}, std::false_type{});
     else if (fq)
-      rounds(std::true_type{}, std::false_type{});
+      rounds(std::true_type{}, std::false_type{}, std::false_type{});
     else
-      rounds(std::false_type{}, std::false_type{});
+      rounds(std::false_type{}, std::false_type{}, std::false_type{});
     __syncthreads();
     float* mesh = ch == 0 ? out.mesh[0] : (ch == 1 ? out.mesh[1] : out.mesh[2]);
     float* hs = halo + ch * hstride + (long long)t * tg.shell;
@@ -623,14 +633,20 @@ static int deposit_run(pmx_ctx* c, float val, int perval, int nch, long long n, 
     vs.step[ch] = 1;
   }
   pmx_tic(c, PMX_T_DEPOSIT);
-  if (perval)
-    deposit_k<1><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, vs, c->vmax, c->tile_start,
-                                                  c->tile_count, g, tg, val, out, nch, mrs,
-                                                  accumulate, c->halo, hstride);
+  const bool full = pmx_full16(c);
+#define DEPREC(PV, F)                                                                          \
+  deposit_k<PV, 0, F><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, vs, PV ? c->vmax : nullptr,      \
+                                                       c->tile_start, c->tile_count, g, tg, val, \
+                                                       out, nch, mrs, accumulate, c->halo, hstride)
+  if (perval && full)
+    DEPREC(1, 1);
+  else if (perval)
+    DEPREC(1, 0);
+  else if (full)
+    DEPREC(0, 1);
   else
-    deposit_k<0><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, vs, nullptr, c->tile_start,
-                                                  c->tile_count, g, tg, val, out, nch, mrs,
-                                                  accumulate, c->halo, hstride);
+    DEPREC(0, 0);
+#undef DEPREC
   pmx_toc(c, PMX_T_DEPOSIT);
   PMX_LAUNCH_CHECK();
   if (defer && !accumulate && pmx_fold_deferrable(c)) return PMX_OK;
@@ -678,12 +694,20 @@ int pmx_deposit_state_vals(pmx_ctx* c, const PState& st, const uint32_t* start,
   int rc;
   if ((rc = pmx_ensure_halo(c, nch))) return rc;
   pmx_tic(c, PMX_T_DEPOSIT);
-  if (vals)
-    deposit_k<1, 1><<<nt, DEP_TPB, lds, c->stream>>>(st.s0, *vals, c->vmax, start, count, g, tg,
-                                                     0.f, out, nch, mrs, 0, c->halo, hstride);
+  const bool full = pmx_full16(c);
+#define DEPST(PV, F)                                                                            \
+  deposit_k<PV, 1, F><<<nt, DEP_TPB, lds, c->stream>>>(                                          \
+      st.s0, PV ? *vals : ValSrc{}, PV ? c->vmax : nullptr, start, count, g, tg, PV ? 0.f : val, \
+      out, nch, mrs, 0, c->halo, hstride)
+  if (vals && full)
+    DEPST(1, 1);
+  else if (vals)
+    DEPST(1, 0);
+  else if (full)
+    DEPST(0, 1);
   else
-    deposit_k<0, 1><<<nt, DEP_TPB, lds, c->stream>>>(st.s0, ValSrc{}, nullptr, start, count, g,
-                                                     tg, val, out, nch, mrs, 0, c->halo, hstride);
+    DEPST(0, 0);
+#undef DEPST
   pmx_toc(c, PMX_T_DEPOSIT);
   PMX_LAUNCH_CHECK();
   if (defer && pmx_fold_deferrable(c)) return PMX_OK;

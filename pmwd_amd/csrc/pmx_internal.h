@@ -955,6 +955,14 @@ int pmx_poisson_ctx(pmx_ctx* c, float coef, int deferred = 0);
 // axis 1 (y) or 0 (x) of nf fields (dir -1 forward, +1 inverse), the paired
 // y pass of the gradient's y and z components, and the Poisson x pass (kin:
 // f[0] holds a full 3D spectrum, no forward x FFT).
+// Every tile a full 16^3 tile: kernels built for that case (their FULL)
+// take the tile's LDS strides as constants.
+inline bool pmx_full16(const pmx_ctx* c) {
+  const TileGeom& t = c->tg;
+  return t.tx == 16 && t.ty == 16 && t.tz == 16 && c->n[0] % 16 == 0 && c->n[1] % 16 == 0 &&
+         c->n[2] % 16 == 0;
+}
+
 // rev: the pass runs its blocks in reverse order (bits unchanged)
 int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef, const float* halo = nullptr,
                         int rev = 0);
