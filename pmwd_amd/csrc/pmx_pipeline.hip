@@ -107,7 +107,7 @@ __global__ void pl_caps_k(const uint32_t* __restrict__ count_in, uint32_t* __res
 //   PL_WRITE  exact re-bin, pass 2: the particle to its exact slot;
 //   PL_LAST   acc, vel and disp to the caller's arrays at the particle index.
 // PL_COUNT and PL_WRITE return at once unless the overflow flag is set.
-template <int MODE>
+template <int MODE, int FULL = 0>
 __global__ void __launch_bounds__(PL_TPB, PL_MINB)
     pl_step_k(PState in, const uint32_t* __restrict__ in_start,
               const uint32_t* __restrict__ in_count, MeshGeom g, TileGeom tg,
@@ -133,6 +133,7 @@ __global__ void __launch_bounds__(PL_TPB, PL_MINB)
   auto tile = [&](const int t) {
     int o[3], s[3], ti[3];
     tile_origin(t, tg, g, o, s, ti);
+    if (FULL) s[0] = s[1] = s[2] = 16;
     const int LY = s[1] + 1, LZ = s[2] + 1;
     const int ltot = (s[0] + 1) * LY * LZ;
     const float* F[3] = {F0, F1, F2};
@@ -264,9 +265,15 @@ static void pl_launch(pmx_ctx* c, int src, PlOut out, PlUser user, float k2, flo
   const size_t lds = (size_t)3 * (tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(float);
   const bool rebin = MODE == PL_COUNT || MODE == PL_WRITE;
   const unsigned grid = rebin && c->ntiles > PL_REBIN_GRID ? PL_REBIN_GRID : (unsigned)c->ntiles;
-  pl_step_k<MODE><<<grid, PL_TPB, lds, c->stream>>>(
-      c->pl[src], c->pl_start[src], c->pl_count[src], g, tg, (const float*)c->cbuf[0],
-      (const float*)c->cbuf[1], (const float*)c->cbuf[2], k2, k1n, dn, out, user, fdev);
+  // tile strides as constants when every tile is a full 16^3 tile
+  if (pmx_full16(c))
+    pl_step_k<MODE, 1><<<grid, PL_TPB, lds, c->stream>>>(
+        c->pl[src], c->pl_start[src], c->pl_count[src], g, tg, (const float*)c->cbuf[0],
+        (const float*)c->cbuf[1], (const float*)c->cbuf[2], k2, k1n, dn, out, user, fdev);
+  else
+    pl_step_k<MODE, 0><<<grid, PL_TPB, lds, c->stream>>>(
+        c->pl[src], c->pl_start[src], c->pl_count[src], g, tg, (const float*)c->cbuf[0],
+        (const float*)c->cbuf[1], (const float*)c->cbuf[2], k2, k1n, dn, out, user, fdev);
 }
 
 // The K-step pipeline; the factors come from the host arrays k1, d, k2, or
@@ -595,7 +602,7 @@ struct AlUser {
 // to the other copy as pl_step_k's modes; LAST writes the caller's arrays.
 // Partials and maxima come from PL_CAP and PL_LAST only.
 // (two workgroups per CU: at most 128 VGPRs, as the 79 KB of LDS allows)
-template <int MODE>
+template <int MODE, int FULL = 0>
 __global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4)))
     al_step_k(AState in, const uint32_t* __restrict__ in_start,
               const uint32_t* __restrict__ in_count, MeshGeom g, TileGeom tg,
@@ -611,6 +618,7 @@ __global__ void __launch_bounds__(AL_TPB) __attribute__((amdgpu_waves_per_eu(4))
   auto tile = [&](const int t) {
     int o[3], s[3], ti[3];
     tile_origin(t, tg, g, o, s, ti);
+    if (FULL) s[0] = s[1] = s[2] = 16;
     const int LY = s[1] + 1, LZ = s[2] + 1;
     const int ltot = (s[0] + 1) * LY * LZ;
     const float* F[4] = {F0, F1, F2, R};
@@ -787,10 +795,16 @@ static void al_launch(pmx_ctx* c, int src, float val, float kp, float ka, float 
   const size_t lds = (size_t)4 * (tg.tx + 1) * (tg.ty + 1) * (tg.tz + 1) * sizeof(float);
   const bool rebin = MODE == PL_COUNT || MODE == PL_WRITE;
   const unsigned grid = rebin && c->ntiles > PL_REBIN_GRID ? PL_REBIN_GRID : (unsigned)c->ntiles;
-  al_step_k<MODE><<<grid, AL_TPB, lds, c->stream>>>(
-      c->al[src], c->pl_start[src], c->pl_count[src], g, tg, (const float*)c->cbuf[0],
-      (const float*)c->cbuf[1], (const float*)c->cbuf[2], (const float*)c->abuf[0], val, kp, ka,
-      dd, snap, out, user, c->vmax, c->red, c->red + c->ntiles);
+#define AL_LAUNCH(F)                                                                            \
+  al_step_k<MODE, F><<<grid, AL_TPB, lds, c->stream>>>(                                          \
+      c->al[src], c->pl_start[src], c->pl_count[src], g, tg, (const float*)c->cbuf[0],           \
+      (const float*)c->cbuf[1], (const float*)c->cbuf[2], (const float*)c->abuf[0], val, kp, ka, \
+      dd, snap, out, user, c->vmax, c->red, c->red + c->ntiles)
+  if (pmx_full16(c))
+    AL_LAUNCH(1);
+  else
+    AL_LAUNCH(0);
+#undef AL_LAUNCH
 }
 
 extern "C" int pmx_nbody_steps_adj(pmx_ctx* c, const int16_t* pmid, float* disp, float* vel,
