@@ -555,6 +555,48 @@ __device__ __forceinline__ float2 fold_pair(float2 v, const FoldIn& fi, int x, i
   return v;
 }
 
+// fold_pair over a block's NV values per thread (row p = idx >> lH of the x
+// plane x, y = y0 + p; pair m = idx & (H - 1)), all loaded beforehand.  A
+// wave's values of one iteration lie in one row.  Off a tile's low x and y
+// faces a cell takes only the z-face term (pmx_fold_sum16's S = 4, on the
+// even cell of pairs with z % 16 == 0); those terms are loaded for every
+// iteration before any is added, so that one wait covers them instead of
+// one per iteration.  Rows on a low x or y face (about an eighth) take
+// fold_pair as it is: loading their terms the same way needs registers
+// that cost more occupancy than the waits they save.
+template <int NV, int TPB>
+__device__ __forceinline__ void fold_rows(float2 (&r)[NV], const FoldIn& fi, int x, int y0, int lH,
+                                          int H, int total) {
+  const int lx = x & 15, tx = x >> 4;
+  float t4[NV];
+#pragma unroll
+  for (int it = 0; it < NV; ++it) {
+    const int idx = threadIdx.x + it * TPB;
+    const int y = y0 + (idx >> lH), m = idx & (H - 1);
+    const int ly = y & 15, ty = y >> 4, lz = (2 * m) & 15, tz = (2 * m) >> 4;
+    t4[it] = 0.f;
+    if (idx < total) {
+      if (lx == 0 || ly == 0) {
+        r[it] = fold_pair(r[it], fi, x, y, m);
+      } else if (lz == 0) {
+        const int tzm = tz == 0 ? fi.ntz - 1 : tz - 1;
+        t4[it] =
+            fi.halo[(long long)((tx * fi.nty + ty) * fi.ntz + tzm) * fi.shell + 561 + lx * 16 + ly];
+      }
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < NV; ++it) {
+    const int idx = threadIdx.x + it * TPB;
+    const int ly = (y0 + (idx >> lH)) & 15, lz = (2 * (idx & (H - 1))) & 15;
+    if (idx < total && lx != 0 && ly != 0 && lz == 0) {
+      float sx = 0.f;
+      sx += t4[it];
+      r[it].x = r[it].x + sx;
+    }
+  }
+}
+
 // FOLD (with LH > 0, 16^3 tiles, P | n1): the density is the deposit's tile
 // interiors plus the fold of fi.halo, added here on load.
 template <int PRE, int LH, bool FOLD = false>
@@ -585,17 +627,16 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_half_k(float* __restrict__ d
 #pragma unroll
     for (int it = 0; it < RFFT_MAXV; ++it) {
       const int idx = threadIdx.x + it * FFT_TPB;
-      if (idx < total) {
-        const int p = idx >> lH, m = idx & (H - 1);
-        float2 v = blk[__umul24(p, rsc) + m];
-        if (FOLD) v = fold_pair(v, fi, fx, fy0 + p, m);
-        if (PRE) {
-          v.x = v.x - 1.f;
-          v.x = v.x * coef;
-          v.y = v.y - 1.f;
-          v.y = v.y * coef;
-        }
-        r[it] = v;
+      if (idx < total) r[it] = blk[__umul24(idx >> lH, rsc) + (idx & (H - 1))];
+    }
+    if (FOLD) fold_rows<RFFT_MAXV, FFT_TPB>(r, fi, fx, fy0, lH, H, total);
+    if (PRE) {
+#pragma unroll
+      for (int it = 0; it < RFFT_MAXV; ++it) {
+        r[it].x = r[it].x - 1.f;
+        r[it].x = r[it].x * coef;
+        r[it].y = r[it].y - 1.f;
+        r[it].y = r[it].y * coef;
       }
     }
 #pragma unroll
