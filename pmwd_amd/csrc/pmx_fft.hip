@@ -555,43 +555,43 @@ __device__ __forceinline__ float2 fold_pair(float2 v, const FoldIn& fi, int x, i
   return v;
 }
 
+// The z-face fold terms of a block of P rows (x plane x, rows y0 + p): for
+// row p and z tile tz, pmx_fold_sum16's S = 4 term of the cell (x, y, 16 tz),
+// the sole term of a cell off the tile's low x and y faces.  Thread j takes
+// p = j % P, tz = j / P (P * H / 8 <= TPB of them; 0 where no such cell).
+template <int P>
+__device__ __forceinline__ float fold_zface(const FoldIn& fi, int x, int y0, int H) {
+  constexpr int lP = ilog2c(P);
+  const int j = (int)threadIdx.x, lx = x & 15, tx = x >> 4;
+  if (lx == 0 || j >= P * (H >> 3)) return 0.f;
+  const int y = y0 + (j & (P - 1)), tz = j >> lP, ly = y & 15, ty = y >> 4;
+  if (ly == 0) return 0.f;
+  const int tzm = tz == 0 ? fi.ntz - 1 : tz - 1;
+  return fi.halo[(long long)((tx * fi.nty + ty) * fi.ntz + tzm) * fi.shell + 561 + lx * 16 + ly];
+}
+
 // fold_pair over a block's NV values per thread (row p = idx >> lH of the x
 // plane x, y = y0 + p; pair m = idx & (H - 1)), all loaded beforehand.  A
-// wave's values of one iteration lie in one row.  Off a tile's low x and y
-// faces a cell takes only the z-face term (pmx_fold_sum16's S = 4, on the
-// even cell of pairs with z % 16 == 0); those terms are loaded for every
-// iteration before any is added, so that one wait covers them instead of
-// one per iteration.  Rows on a low x or y face (about an eighth) take
-// fold_pair as it is: loading their terms the same way needs registers
-// that cost more occupancy than the waits they save.
-template <int NV, int TPB>
+// cell off a tile's low x and y faces takes only its z-face term, from the
+// block's table sz (fold_zface, p + P tz); rows on a low x or y face (about
+// an eighth) take fold_pair.  The terms come from one load per thread,
+// issued before the row loads, so no wait falls between two row loads and
+// no registers are held per value.
+template <int NV, int TPB, int P>
 __device__ __forceinline__ void fold_rows(float2 (&r)[NV], const FoldIn& fi, int x, int y0, int lH,
-                                          int H, int total) {
-  const int lx = x & 15, tx = x >> 4;
-  float t4[NV];
+                                          int H, int total, const float* sz) {
+  const int lx = x & 15;
 #pragma unroll
   for (int it = 0; it < NV; ++it) {
     const int idx = threadIdx.x + it * TPB;
-    const int y = y0 + (idx >> lH), m = idx & (H - 1);
-    const int ly = y & 15, ty = y >> 4, lz = (2 * m) & 15, tz = (2 * m) >> 4;
-    t4[it] = 0.f;
-    if (idx < total) {
-      if (lx == 0 || ly == 0) {
-        r[it] = fold_pair(r[it], fi, x, y, m);
-      } else if (lz == 0) {
-        const int tzm = tz == 0 ? fi.ntz - 1 : tz - 1;
-        t4[it] =
-            fi.halo[(long long)((tx * fi.nty + ty) * fi.ntz + tzm) * fi.shell + 561 + lx * 16 + ly];
-      }
-    }
-  }
-#pragma unroll
-  for (int it = 0; it < NV; ++it) {
-    const int idx = threadIdx.x + it * TPB;
-    const int ly = (y0 + (idx >> lH)) & 15, lz = (2 * (idx & (H - 1))) & 15;
-    if (idx < total && lx != 0 && ly != 0 && lz == 0) {
+    if (idx >= total) continue;
+    const int p = idx >> lH, y = y0 + p, m = idx & (H - 1);
+    const int ly = y & 15, lz = (2 * m) & 15;
+    if (lx == 0 || ly == 0) {
+      r[it] = fold_pair(r[it], fi, x, y, m);
+    } else if (lz == 0) {
       float sx = 0.f;
-      sx += t4[it];
+      sx += sz[(m >> 3) * P + p];
       r[it].x = r[it].x + sx;
     }
   }
@@ -623,13 +623,20 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_half_k(float* __restrict__ d
   const int fx = FOLD ? (int)row0 / fi.n1 : 0;
   const int fy0 = FOLD ? (int)row0 - fx * fi.n1 : 0;
   {
+    // the block's z-face fold terms (after the twiddles), loaded first
+    float* sz = reinterpret_cast<float*>(stw + 2 * H);
+    const float zf = FOLD ? fold_zface<PC>(fi, fx, fy0, H) : 0.f;
     float2 r[RFFT_MAXV];
 #pragma unroll
     for (int it = 0; it < RFFT_MAXV; ++it) {
       const int idx = threadIdx.x + it * FFT_TPB;
       if (idx < total) r[it] = blk[__umul24(idx >> lH, rsc) + (idx & (H - 1))];
     }
-    if (FOLD) fold_rows<RFFT_MAXV, FFT_TPB>(r, fi, fx, fy0, lH, H, total);
+    if (FOLD) {
+      if ((int)threadIdx.x < PC * (H >> 3)) sz[threadIdx.x] = zf;
+      __syncthreads();
+      fold_rows<RFFT_MAXV, FFT_TPB, PC>(r, fi, fx, fy0, lH, H, total, sz);
+    }
     if (PRE) {
 #pragma unroll
       for (int it = 0; it < RFFT_MAXV; ++it) {
@@ -1243,7 +1250,8 @@ int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef, const float
   if (int rc = fft_fits("z r2c", n2, r.len * r.P, RFFT_CAP, r.lds)) return rc;
   const unsigned nblk = (unsigned)((nrows + r.P - 1) / r.P);
   const int P = r.P;
-  const size_t lds = r.lds;
+  // the fold's z-face table of a block: P rows x len / 8 z tiles
+  const size_t lds = r.lds + (halo ? (size_t)P * (r.len / 8) * sizeof(float) : 0);
   const FoldIn fi = {halo, c->tg.shell, c->tg.ntx, c->tg.nty, c->tg.ntz, c->n[1]};
   const float* fc = pre ? c->dfac : nullptr;
   PMX_REQUIRE(!halo || pmx_fold_deferrable(c), "z r2c: no fold on load for this mesh");
