@@ -555,43 +555,87 @@ __device__ __forceinline__ float2 fold_pair(float2 v, const FoldIn& fi, int x, i
   return v;
 }
 
-// The z-face fold terms of a block of P rows (x plane x, rows y0 + p): for
-// row p and z tile tz, pmx_fold_sum16's S = 4 term of the cell (x, y, 16 tz),
-// the sole term of a cell off the tile's low x and y faces.  Thread j takes
-// p = j % P, tz = j / P (P * H / 8 <= TPB of them; 0 where no such cell).
-template <int P>
-__device__ __forceinline__ float fold_zface(const FoldIn& fi, int x, int y0, int H) {
+// The fold terms of a block of P rows (x plane x, rows y = y0 + p, pairs
+// z = 2m, 2m + 1; 2H cells a row), staged in LDS.  A cell takes
+// pmx_fold_sum16's terms; by the faces its row lies on (lx = x % 16,
+// ly = y % 16, lz = z % 16):
+//   lx != 0, ly != 0: S = 4 on cells with lz = 0;
+//   lx == 0, ly != 0 (a face row): S = 1 on every cell, then S = 4, 5 when lz = 0;
+//   lx != 0, ly == 0 (a face row): S = 2 on every cell, then S = 4, 6 when lz = 0;
+//   lx == 0, ly == 0 (a corner row, 1 in 256): fold_pair.
+// T[p * 2H + z] holds the face rows' S = 1 or S = 2 term, loaded straight into
+// LDS (the rows' FFT buffer, before the rows go there), and Z0[p + P tz] the
+// S = 4 term and Z1[p + P tz] the S = 5 or S = 6 term of cell (p, 16 tz),
+// one each per thread (P * H / 8 <= TPB of them).  All of it is issued before
+// the row loads, so no wait falls between two of them.
+template <int P, int TPB>
+__device__ __forceinline__ void fold_tables(const FoldIn& fi, int x, int y0, int lH, int H, float* T,
+                                            float& z0, float& z1) {
+  typedef __attribute__((address_space(1))) const void gptr;
+  typedef __attribute__((address_space(3))) void lptr;
   constexpr int lP = ilog2c(P);
-  const int j = (int)threadIdx.x, lx = x & 15, tx = x >> 4;
-  if (lx == 0 || j >= P * (H >> 3)) return 0.f;
-  const int y = y0 + (j & (P - 1)), tz = j >> lP, ly = y & 15, ty = y >> 4;
-  if (ly == 0) return 0.f;
-  const int tzm = tz == 0 ? fi.ntz - 1 : tz - 1;
-  return fi.halo[(long long)((tx * fi.nty + ty) * fi.ntz + tzm) * fi.shell + 561 + lx * 16 + ly];
+  const int lx = x & 15, tx = x >> 4, txm = tx == 0 ? fi.ntx - 1 : tx - 1;
+  auto tile = [&](int nx, int ny, int nz) {
+    return fi.halo + (long long)((nx * fi.nty + ny) * fi.ntz + nz) * fi.shell;
+  };
+  const int j = (int)threadIdx.x;
+  z0 = z1 = 0.f;
+  if (j < P * (H >> 3)) {
+    const int y = y0 + (j & (P - 1)), tz = j >> lP, ly = y & 15, ty = y >> 4;
+    const int tym = ty == 0 ? fi.nty - 1 : ty - 1, tzm = tz == 0 ? fi.ntz - 1 : tz - 1;
+    if (lx != 0 || ly != 0) {
+      z0 = tile(tx, ty, tzm)[561 + lx * 16 + ly];
+      if (lx == 0)
+        z1 = tile(txm, ty, tzm)[ly * 17 + 16];
+      else if (ly == 0)
+        z1 = tile(tx, tym, tzm)[289 + lx * 17 + 16];
+    }
+  }
+  // a wave's 64 entries lie in one row (2H >= 128), so the face-row test is
+  // wave-uniform; lane l of a wave writes the LDS word at its base plus l
+  const int n2 = 2 * H, wave0 = j & ~63;
+  for (int e0 = 0; e0 < P * n2; e0 += TPB) {
+    const int e = e0 + j;
+    if (e >= P * n2) break;
+    const int p = e >> (lH + 1), z = e & (n2 - 1);
+    const int y = y0 + p, ly = y & 15, ty = y >> 4;
+    if ((lx == 0) == (ly == 0)) continue;
+    const float* src = lx == 0 ? tile(txm, ty, z >> 4) + ly * 17 + (z & 15)
+                               : tile(tx, ty == 0 ? fi.nty - 1 : ty - 1, z >> 4) + 289 + lx * 17 + (z & 15);
+    __builtin_amdgcn_global_load_lds((gptr*)src, (lptr*)(T + e0 + wave0), 4, 0, 0);
+  }
 }
 
-// fold_pair over a block's NV values per thread (row p = idx >> lH of the x
-// plane x, y = y0 + p; pair m = idx & (H - 1)), all loaded beforehand.  A
-// cell off a tile's low x and y faces takes only its z-face term, from the
-// block's table sz (fold_zface, p + P tz); rows on a low x or y face (about
-// an eighth) take fold_pair.  The terms come from one load per thread,
-// issued before the row loads, so no wait falls between two row loads and
-// no registers are held per value.
+// Adds the fold (fold_tables' terms, in pmx_fold_sum16's order) to a block's
+// NV values per thread (row p = idx >> lH, pair m = idx & (H - 1)).
 template <int NV, int TPB, int P>
 __device__ __forceinline__ void fold_rows(float2 (&r)[NV], const FoldIn& fi, int x, int y0, int lH,
-                                          int H, int total, const float* sz) {
+                                          int H, int total, const float* T, const float* Z0,
+                                          const float* Z1) {
   const int lx = x & 15;
 #pragma unroll
   for (int it = 0; it < NV; ++it) {
     const int idx = threadIdx.x + it * TPB;
     if (idx >= total) continue;
     const int p = idx >> lH, y = y0 + p, m = idx & (H - 1);
-    const int ly = y & 15, lz = (2 * m) & 15;
-    if (lx == 0 || ly == 0) {
+    const int ly = y & 15, z = 2 * m, lz = z & 15, k = (m >> 3) * P + p;
+    if (lx == 0 && ly == 0) {
       r[it] = fold_pair(r[it], fi, x, y, m);
+    } else if (lx == 0 || ly == 0) {
+      const float* t = T + (p << (lH + 1)) + z;
+      float sx = 0.f;
+      sx += t[0];
+      if (lz == 0) {
+        sx += Z0[k];
+        sx += Z1[k];
+      }
+      r[it].x = r[it].x + sx;
+      float sy = 0.f;
+      sy += t[1];
+      r[it].y = r[it].y + sy;
     } else if (lz == 0) {
       float sx = 0.f;
-      sx += sz[(m >> 3) * P + p];
+      sx += Z0[k];
       r[it].x = r[it].x + sx;
     }
   }
@@ -623,9 +667,14 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_half_k(float* __restrict__ d
   const int fx = FOLD ? (int)row0 / fi.n1 : 0;
   const int fy0 = FOLD ? (int)row0 - fx * fi.n1 : 0;
   {
-    // the block's z-face fold terms (after the twiddles), loaded first
-    float* sz = reinterpret_cast<float*>(stw + 2 * H);
-    const float zf = FOLD ? fold_zface<PC>(fi, fx, fy0, H) : 0.f;
+    // the block's fold terms (fold_tables), issued before the row loads, in
+    // the rows' buffer: T (P * 2H floats), then Z0 and Z1 (P * H / 8 each) in
+    // what the rows' padding leaves (P * (H + H / 8) float2 in all)
+    float* T = reinterpret_cast<float*>(s);
+    float* Z0 = T + PC * 2 * H;
+    float* Z1 = Z0 + PC * (H >> 3);
+    float z0 = 0.f, z1 = 0.f;
+    if (FOLD) fold_tables<PC, FFT_TPB>(fi, fx, fy0, lH, H, T, z0, z1);
     float2 r[RFFT_MAXV];
 #pragma unroll
     for (int it = 0; it < RFFT_MAXV; ++it) {
@@ -633,9 +682,14 @@ __global__ void __launch_bounds__(FFT_TPB, 4) z_r2c_half_k(float* __restrict__ d
       if (idx < total) r[it] = blk[__umul24(idx >> lH, rsc) + (idx & (H - 1))];
     }
     if (FOLD) {
-      if ((int)threadIdx.x < PC * (H >> 3)) sz[threadIdx.x] = zf;
+      if ((int)threadIdx.x < PC * (H >> 3)) {
+        Z0[threadIdx.x] = z0;
+        Z1[threadIdx.x] = z1;
+      }
+      __builtin_amdgcn_s_waitcnt(0);  // T's loads into LDS have landed
       __syncthreads();
-      fold_rows<RFFT_MAXV, FFT_TPB, PC>(r, fi, fx, fy0, lH, H, total, sz);
+      fold_rows<RFFT_MAXV, FFT_TPB, PC>(r, fi, fx, fy0, lH, H, total, T, Z0, Z1);
+      __syncthreads();  // T is read before the rows take its place
     }
     if (PRE) {
 #pragma unroll
@@ -1250,8 +1304,7 @@ int pmx_fft_r2c_inplace(pmx_ctx* c, float* buf, int pre, float coef, const float
   if (int rc = fft_fits("z r2c", n2, r.len * r.P, RFFT_CAP, r.lds)) return rc;
   const unsigned nblk = (unsigned)((nrows + r.P - 1) / r.P);
   const int P = r.P;
-  // the fold's z-face table of a block: P rows x len / 8 z tiles
-  const size_t lds = r.lds + (halo ? (size_t)P * (r.len / 8) * sizeof(float) : 0);
+  const size_t lds = r.lds;
   const FoldIn fi = {halo, c->tg.shell, c->tg.ntx, c->tg.nty, c->tg.ntz, c->n[1]};
   const float* fc = pre ? c->dfac : nullptr;
   PMX_REQUIRE(!halo || pmx_fold_deferrable(c), "z r2c: no fold on load for this mesh");
