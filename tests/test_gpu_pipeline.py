@@ -279,7 +279,10 @@ def test_adjoint_driver_pipeline_matches_per_step(dev):
         assert abs(outs[0][1] - outs[1][1]) <= 1e-10 * abs(outs[1][1])
 
 
-@pytest.mark.parametrize('shape,ratio', [((64, 64, 64), 2), ((32, 48, 64), 4)])
+# row blocks of 64, 32, 16 and 8 rows (rows of 64, 128, 256 and 512 complex
+# values): a block spans four, two or one tile rows in y, or half of one
+@pytest.mark.parametrize('shape,ratio', [((64, 64, 64), 2), ((32, 48, 64), 4),
+                                         ((16, 16, 128), 4), ((8, 16, 256), 4)])
 def test_fold_on_load_matches_fold_kernel(dev, monkeypatch, shape, ratio):
     """On meshes whose Poisson row pass adds the deposit's halo shells as it
     loads the density (16^3 tiles, half-length rows of 64..1024 complex
