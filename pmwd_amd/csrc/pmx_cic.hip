@@ -290,8 +290,12 @@ __device__ __forceinline__ void dep_store(Sum sum, float* mesh, float* hs,
 }
 
 // FULL: every tile is a full 16^3 tile, so the LDS strides are constants
-// (corner offsets become immediates).
-template <int PERVAL, int STATE = 0, int FULL = 0>
+// (corner offsets become immediates).  SPLIT: one channel per workgroup
+// (nch * ntiles workgroups, ntiles a multiple of 8): the nch workgroups of a
+// tile take consecutive slots of one XCD, so they run together and read the
+// tile's records through the same L2, instead of one workgroup reading them
+// nch times.
+template <int PERVAL, int STATE = 0, int FULL = 0, int SPLIT = 0>
 __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DEP_WPE)))
     deposit_k(const float4* __restrict__ rec, ValSrc vals, const float* __restrict__ vmaxp, const uint32_t* __restrict__ tile_start,
               const uint32_t* __restrict__ tile_count, MeshGeom g, TileGeom tg, float val,
@@ -299,14 +303,23 @@ __global__ void __launch_bounds__(DEP_TPB) __attribute__((amdgpu_waves_per_eu(DE
               long long hstride) {
   extern __shared__ unsigned long long acc[];
   int o[3], s[3], ti[3];
-  const int t = (int)pmx_xcd_block(blockIdx.x, gridDim.x);
+  int t, ch0 = 0, ch1 = nch;
+  if (SPLIT) {
+    const unsigned x = blockIdx.x % PMX_NXCD, sl = blockIdx.x / PMX_NXCD;
+    const unsigned j = sl / (unsigned)nch;
+    ch0 = (int)(sl - j * (unsigned)nch);
+    ch1 = ch0 + 1;
+    t = (int)pmx_xcd_block(j * PMX_NXCD + x, gridDim.x / (unsigned)nch);
+  } else {
+    t = (int)pmx_xcd_block(blockIdx.x, gridDim.x);
+  }
   tile_origin(t, tg, g, o, s, ti);
   if (FULL) s[0] = s[1] = s[2] = 16;
   const int LY = s[1] + 1, LZ = s[2] + 1;
   const int ltot = (s[0] + 1) * LY * LZ;
   const uint32_t start = tile_start[t];
   const uint32_t cnt = tile_count[t];
-  for (int ch = 0; ch < nch; ++ch) {
+  for (int ch = ch0; ch < ch1; ++ch) {
     for (int i = threadIdx.x; i < ltot; i += DEP_TPB) acc[i] = 0ull;
     const double scale = fixed_scale(PERVAL ? vmaxp[ch] : fabsf(val));
     const double inv = 1.0 / scale;
@@ -695,18 +708,24 @@ int pmx_deposit_state_vals(pmx_ctx* c, const PState& st, const uint32_t* start,
   if ((rc = pmx_ensure_halo(c, nch))) return rc;
   pmx_tic(c, PMX_T_DEPOSIT);
   const bool full = pmx_full16(c);
-#define DEPST(PV, F)                                                                            \
-  deposit_k<PV, 1, F><<<nt, DEP_TPB, lds, c->stream>>>(                                          \
+  // several channels: one workgroup per tile and channel
+  const bool split = vals && nch > 1 && nt % PMX_NXCD == 0;
+#define DEPST(PV, F, SP)                                                                           \
+  deposit_k<PV, 1, F, SP><<<SP ? nt * (unsigned)nch : nt, DEP_TPB, lds, c->stream>>>(              \
       st.s0, PV ? *vals : ValSrc{}, PV ? c->vmax : nullptr, start, count, g, tg, PV ? 0.f : val, \
       out, nch, mrs, 0, c->halo, hstride)
-  if (vals && full)
-    DEPST(1, 1);
+  if (split && full)
+    DEPST(1, 1, 1);
+  else if (split)
+    DEPST(1, 0, 1);
+  else if (vals && full)
+    DEPST(1, 1, 0);
   else if (vals)
-    DEPST(1, 0);
+    DEPST(1, 0, 0);
   else if (full)
-    DEPST(0, 1);
+    DEPST(0, 1, 0);
   else
-    DEPST(0, 0);
+    DEPST(0, 0, 0);
 #undef DEPST
   pmx_toc(c, PMX_T_DEPOSIT);
   PMX_LAUNCH_CHECK();
