@@ -647,18 +647,24 @@ static int deposit_run(pmx_ctx* c, float val, int perval, int nch, long long n, 
   }
   pmx_tic(c, PMX_T_DEPOSIT);
   const bool full = pmx_full16(c);
-#define DEPREC(PV, F)                                                                          \
-  deposit_k<PV, 0, F><<<nt, DEP_TPB, lds, c->stream>>>(c->rec, vs, PV ? c->vmax : nullptr,      \
-                                                       c->tile_start, c->tile_count, g, tg, val, \
-                                                       out, nch, mrs, accumulate, c->halo, hstride)
-  if (perval && full)
-    DEPREC(1, 1);
+  // several channels: one workgroup per tile and channel
+  const bool split = perval && nch > 1 && nt % PMX_NXCD == 0;
+#define DEPREC(PV, F, SP)                                                                        \
+  deposit_k<PV, 0, F, SP><<<SP ? nt * (unsigned)nch : nt, DEP_TPB, lds, c->stream>>>(            \
+      c->rec, vs, PV ? c->vmax : nullptr, c->tile_start, c->tile_count, g, tg, val, out, nch, mrs, \
+      accumulate, c->halo, hstride)
+  if (split && full)
+    DEPREC(1, 1, 1);
+  else if (split)
+    DEPREC(1, 0, 1);
+  else if (perval && full)
+    DEPREC(1, 1, 0);
   else if (perval)
-    DEPREC(1, 0);
+    DEPREC(1, 0, 0);
   else if (full)
-    DEPREC(0, 1);
+    DEPREC(0, 1, 0);
   else
-    DEPREC(0, 0);
+    DEPREC(0, 0, 0);
 #undef DEPREC
   pmx_toc(c, PMX_T_DEPOSIT);
   PMX_LAUNCH_CHECK();
