@@ -648,7 +648,7 @@ static int deposit_run(pmx_ctx* c, float val, int perval, int nch, long long n, 
   pmx_tic(c, PMX_T_DEPOSIT);
   const bool full = pmx_full16(c);
   // several channels: one workgroup per tile and channel
-  const bool split = perval && nch > 1 && nt % PMX_NXCD == 0;
+  const bool split = c->deposit_split && perval && nch > 1 && nt % PMX_NXCD == 0;
 #define DEPREC(PV, F, SP)                                                                        \
   deposit_k<PV, 0, F, SP><<<SP ? nt * (unsigned)nch : nt, DEP_TPB, lds, c->stream>>>(            \
       c->rec, vs, PV ? c->vmax : nullptr, c->tile_start, c->tile_count, g, tg, val, out, nch, mrs, \
@@ -715,7 +715,7 @@ int pmx_deposit_state_vals(pmx_ctx* c, const PState& st, const uint32_t* start,
   pmx_tic(c, PMX_T_DEPOSIT);
   const bool full = pmx_full16(c);
   // several channels: one workgroup per tile and channel
-  const bool split = vals && nch > 1 && nt % PMX_NXCD == 0;
+  const bool split = c->deposit_split && vals && nch > 1 && nt % PMX_NXCD == 0;
 #define DEPST(PV, F, SP)                                                                           \
   deposit_k<PV, 1, F, SP><<<SP ? nt * (unsigned)nch : nt, DEP_TPB, lds, c->stream>>>(              \
       st.s0, PV ? *vals : ValSrc{}, PV ? c->vmax : nullptr, start, count, g, tg, PV ? 0.f : val, \

@@ -246,6 +246,8 @@ extern "C" int pmx_ctx_create(int n0, int n1, int n2, double cell_size, void* st
   c->tg = make_tiles(c->n);
   const char* fol = getenv("PMX_FOLD_ON_LOAD");
   c->fold_on_load = !(fol && strcmp(fol, "0") == 0);
+  const char* dsp = getenv("PMX_DEPOSIT_SPLIT");
+  c->deposit_split = !(dsp && strcmp(dsp, "0") == 0);
   c->ntiles = (long long)c->tg.ntx * c->tg.nty * c->tg.ntz;
   if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->tile_count, (size_t)(c->ntiles + 1) * 4);
   if (rc == PMX_OK) rc |= dev_alloc(c, (void**)&c->tile_start, (size_t)(c->ntiles + 1) * 4);

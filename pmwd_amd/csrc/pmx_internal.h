@@ -193,6 +193,9 @@ struct pmx_ctx {
   int halo_nch;        // channels the halo buffer holds
   int fold_on_load;    // the Poisson row pass may take the deposit's fold
                        // (environment PMX_FOLD_ON_LOAD=0 at creation: never)
+  int deposit_split;   // several-channel deposits: a workgroup per tile and
+                       // channel (environment PMX_DEPOSIT_SPLIT=0 at
+                       // creation: the channels in turn in one workgroup)
   long long ptcl_cap;
   float4* rec;         // binned particle records (d per axis, packed local cell)
   uint32_t* pkey;      // tile of each particle

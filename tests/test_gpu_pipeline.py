@@ -309,3 +309,24 @@ def test_fold_on_load_matches_fold_kernel(dev, monkeypatch, shape, ratio):
     for g, gd in got_adj:
         assert_same(g, want_adj)
         np.testing.assert_allclose(gd, want_dots, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize('shape,ratio', [((64, 64, 64), 2), ((32, 48, 64), 4)])
+def test_deposit_split_matches_channels_in_turn(dev, monkeypatch, shape, ratio):
+    """The reverse steps' three-channel deposits run one workgroup per tile
+    and channel; chained single steps and the multi-step entry point equal
+    bit for bit the runs whose deposits take the channels in turn in one
+    workgroup (PMX_DEPOSIT_SPLIT=0 when the context is made)."""
+    import pmwd_amd as pa
+    conf = pa.Configuration(1.0, shape, mesh_shape=ratio)
+    pmid, disp = particles(shape, ratio, conf.cell_size, 35, std=0.8)
+    rng = np.random.default_rng(36)
+    arrs = [disp] + [(0.5 * rng.standard_normal(disp.shape)).astype(np.float32)
+                     for _ in range(3)]
+    ka, da, kb = factors(2, 37)
+    got = [run_adj(conf, dev, pmid, arrs, 0.2, ka, -da, kb, m) for m in (True, False)]
+    monkeypatch.setenv('PMX_DEPOSIT_SPLIT', '0')
+    want = [run_adj(conf, dev, pmid, arrs, 0.2, ka, -da, kb, m) for m in (True, False)]
+    for (g, gd), (w, wd) in zip(got, want):
+        assert_same(g, w)
+        np.testing.assert_allclose(gd, wd, rtol=1e-12, atol=1e-300)
